@@ -1,0 +1,124 @@
+/* polyq_amd: MI355X (gfx950) branch evaluator for Clifford+T path sums.
+ *
+ * C ABI of the drop-in GPU strategy.  Every entry point takes plain pointers
+ * and sizes, returns 0 on success and a negative status on error (message in
+ * pq_last_error(), thread local), and never aborts or throws across the ABI.
+ * An amplitude that is exactly zero because the boundary is contradictory,
+ * the path sum vanishes identically, or K s = kappa is inconsistent is a
+ * successful result 0 (src/pathsum.rs:430-450, src/engine.rs:570-572,
+ * :595-597), not an error.
+ *
+ * Conventions: basis-state index bit q = qubit q (src/sim.rs:74-76,
+ * src/circuit.rs:18-19); gate codes follow the Gate enum of
+ * src/circuit.rs:25-42; a phase is an angle 2*pi*k/2^32 (src/phase.rs:22-24).
+ *
+ * Handles are immutable after creation apart from internal caches guarded by
+ * locks, so concurrent calls from several host threads are safe; device work
+ * is serialised on the library's stream.
+ */
+#ifndef POLYQ_AMD_H
+#define POLYQ_AMD_H
+#include <stddef.h>
+#include <stdint.h>
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Gate codes (src/circuit.rs:25-42). */
+enum pq_gate {
+    PQ_H = 0, PQ_X = 1, PQ_Y = 2, PQ_Z = 3, PQ_S = 4, PQ_SDG = 5, PQ_T = 6, PQ_TDG = 7,
+    PQ_P = 8,       /* diag(1, e^{i phase}) */
+    PQ_CZ = 9, PQ_CX = 10 /* q0 control, q1 target */, PQ_SWAP = 11,
+    PQ_GPHASE = 12  /* global phase e^{i phase} */
+};
+
+/* Non-Clifford strategy (src/sim.rs:40-48): Auto picks the number of
+ * decomposed T-pairs with the cost model of src/rank.rs:205-222, Cached is the
+ * plain Z-branch engine (2^d branches per amplitude), Pairs decomposes exactly
+ * `pairs` pairs (2^p stabiliser branch engines, src/rank.rs:88-103). */
+enum pq_strategy { PQ_AUTO = 0, PQ_CACHED = 1, PQ_PAIRS = 2 };
+
+/* Device evaluation form.  Z8: integer Z8 phases, FP32 two-sum compensated
+ * lane accumulation, FP64 cross-workgroup reduction (default).  Z8_ACCURATE:
+ * the same walk folding every 32 branches instead of 256.  PLAIN_FP64: a
+ * straightforward FP64 port of Engine::gray, one query per launch. */
+enum pq_form { PQ_FORM_Z8 = 0, PQ_FORM_Z8_ACCURATE = 1, PQ_FORM_PLAIN_FP64 = 2 };
+
+typedef struct pq_sim pq_sim;
+
+/* Simulator::with_options (src/sim.rs:144-148): compile the circuit
+ * (src/pathsum.rs:274-297), build the branch engines on the host
+ * (src/engine.rs:289-390, src/rank.rs:88-123) and upload them to HBM, where
+ * they stay resident until pq_sim_destroy.  phase[] is read for PQ_P and
+ * PQ_GPHASE only; q1[] for two-qubit gates only. */
+int pq_sim_create(uint32_t num_qubits, size_t num_gates, const uint8_t* ops, const uint32_t* q0, const uint32_t* q1,
+                  const uint32_t* phase, int strategy, uint32_t pairs, pq_sim** out);
+void pq_sim_destroy(pq_sim* sim);
+
+/* Simulator::stats + pairs (src/sim.rs:161-168; order of the Python tuple,
+ * src/pybind.rs:177-180): free_vars, planes, lines, radical, terms,
+ * branch_dim, pairs. */
+int pq_sim_stats(const pq_sim* sim, uint64_t out7[7]);
+uint32_t pq_sim_num_qubits(const pq_sim* sim);
+
+/* Simulator::amplitude (src/sim.rs:171-176): <output|C|input>, one byte per
+ * qubit (0/1). */
+int pq_sim_amplitude(pq_sim* sim, const uint8_t* input, const uint8_t* output, int form, double out[2]);
+/* Simulator::amplitude_index (src/sim.rs:179-182). */
+int pq_sim_amplitude_index(pq_sim* sim, uint64_t input, uint64_t output, int form, double out[2]);
+/* Simulator::probability (src/sim.rs:196-198) = |amplitude|^2. */
+int pq_sim_probability(pq_sim* sim, uint64_t input, uint64_t output, int form, double* out);
+/* Simulator::marginal_probability (src/sim.rs:225-231): probability that the
+ * k listed qubits measure the listed outcomes; sandwich engine
+ * (src/pathsum.rs:304-314), cached per measured-qubit set (the reference
+ * rebuilds it on every call). */
+int pq_sim_marginal_probability(pq_sim* sim, uint64_t input, const uint32_t* qubits, const uint8_t* outcomes, size_t k,
+                                int form, double* out);
+/* Many amplitudes <outputs[i]|C|inputs[i]> of one simulator in one launch
+ * (the query loop of Simulator::statevector, src/sim.rs:201-213). */
+int pq_sim_amplitudes(pq_sim* sim, size_t n, const uint64_t* inputs, const uint64_t* outputs, int form, double* out);
+
+/* Heterogeneous batch: query i is <outputs[i]|sims[i]|inputs[i]>; all
+ * branch-engine sums of all queries run in one persistent launch (Z8 forms
+ * only).  out has 2n doubles. */
+int pq_batch_amplitudes(size_t n, pq_sim* const* sims, const uint64_t* inputs, const uint64_t* outputs, int form, double* out);
+
+/* Timing of the last device evaluation on this thread: kernel_ms, total_ms,
+ * branches, items, grid (doubles). */
+int pq_last_timing(double out5[5]);
+
+/* Replayable batch captured in a hipGraph (descriptor upload, counter clear,
+ * persistent walk, FP64 reduction, result download). */
+typedef struct pq_graph pq_graph;
+int pq_graph_create(size_t n, pq_sim* const* sims, const uint64_t* inputs, const uint64_t* outputs, int form, pq_graph** out);
+int pq_graph_run(pq_graph* g, double* out, double* total_ms);
+void pq_graph_destroy(pq_graph* g);
+
+/* ---- parity and diagnostics (no counterpart in the reference) ---- */
+/* Per-branch Gauss-sum outputs on the first branch engine of the cached
+ * strategy for <y|C|x>, for explicit t-bit branch vectors s (bit k = term k):
+ * zero flag (normal_form.rs:472-474), k8 = phase8 (normal_form.rs:479-493)
+ * and the total branch phase k8 + 6|s| + 4|neg \ s| mod 8.  r = 2 log2|G| is
+ * constant per engine and returned in *r.  Returns 1 if the boundary is
+ * contradictory. */
+int pq_sim_branch_dump(pq_sim* sim, uint64_t x, uint64_t y, size_t n, const uint64_t* s, uint8_t* zero, uint8_t* k8,
+                       uint8_t* ktot, uint32_t* r);
+/* Every branch the Z8 walk visits for <y|C|x> with the given mid bits: k(c)
+ * and prod_k r_k[s_k], plus the t-bit branch vector s of each index (host
+ * reconstruction from the walk's slot layout).  n = 2^(8 + slots) entries. */
+int pq_sim_walk_dump(pq_sim* sim, uint64_t x, uint64_t y, uint32_t mid_bits, size_t cap, uint8_t* k, float* mag, uint64_t* s,
+                     size_t* n);
+/* Host-only check of the Z8 quadratic form against the direct phase for
+ * `trials` random coset points; returns the number of mismatches in *bad. */
+int pq_sim_selfcheck(pq_sim* sim, uint64_t x, uint64_t y, uint32_t trials, uint64_t seed, uint64_t* bad);
+/* Z8 image summary: d, inner coordinates, slots, compact inner pivots, rank K. */
+int pq_sim_z8_info(pq_sim* sim, uint32_t out5[5]);
+
+const char* pq_last_error(void);
+/* Device name, architecture and CU count; returns -1 without a device. */
+int pq_device_info(char* buf, size_t len);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,1338 @@
+// CPU oracle: a plain C++ restatement of the polyq reference's path-sum pipeline.
+//
+// TEST INFRASTRUCTURE ONLY.  This file is the checker for the HIP branch
+// evaluator and the host-core CPU baseline.  Nothing in the product library
+// (polyq-v1.0_amd/) links, loads or calls it; only tests/, the smoke check and
+// the cpu_baseline leg of bench.py do.
+//
+// Every function follows the reference file:line named in its comment.  The
+// restatement keeps the reference's algorithms, orderings and tie-breaking
+// (pivot rule, RREF column order, Gray enumeration, 2^14 chunking, weight
+// recompute every 2048 steps) so that per-branch outputs keyed by the branch
+// vector s, EngineStats and amplitudes can be compared.
+//
+// Parity pins (each exercised by tests/test_oracle.py):
+//   phase ring         known answers of src/phase.rs:154-182
+//   SplitMix64         published first outputs (seed 0 -> 0xE220A8397B1DCDAF)
+//   dense statevector  numpy Kronecker-product statevector (independent)
+//   compiler           brute_force vs dense (src/pathsum.rs:504-530)
+//   normal form        block-diagonal Gram + invertibility (normal_form.rs:554-606),
+//                      closed-form Gauss sum vs 2^m brute force (normal_form.rs:608-625)
+//   engine / rank /    engine vs brute force, GHZ, HTZH (engine.rs:745-802),
+//   simulator          rank == cached (rank.rs:243-276), statevector and
+//                      marginals vs dense (sim.rs:329-379)
+//   per-branch phase8  2^m brute-force Gauss sum at the same shift
+// Parity unpinned (implementation-defined in the reference and not observable
+// through any mathematical check): the exact basis vectors chosen by the
+// MinWeight BTreeSet tie-break, hence the planes/lines split of EngineStats
+// (only 2*planes+lines is invariant), and bitwise f64 sums across the
+// reference's nondeterministic Rayon reduction tree.
+
+#include <cstdint>
+#include <cstring>
+#include <cmath>
+#include <complex>
+#include <vector>
+#include <set>
+#include <map>
+#include <string>
+#include <algorithm>
+#include <stdexcept>
+#include <chrono>
+#include <omp.h>
+
+namespace orc {
+
+// ---------------------------------------------------------------- bits.rs ---
+// BitVec: u64 words, LSB-first, tail bits zero (src/bits.rs:10-19).
+struct BitVec {
+    std::vector<uint64_t> w;
+    size_t len = 0;
+    BitVec() {}
+    explicit BitVec(size_t n) : w((n + 63) / 64, 0), len(n) {}
+    bool get(size_t i) const { return (w[i >> 6] >> (i & 63)) & 1; }
+    void set(size_t i, bool v) {
+        if (v) w[i >> 6] |= 1ull << (i & 63); else w[i >> 6] &= ~(1ull << (i & 63));
+    }
+    void flip(size_t i) { w[i >> 6] ^= 1ull << (i & 63); }
+    void xor_assign(const BitVec& o) { for (size_t k = 0; k < w.size(); ++k) w[k] ^= o.w[k]; }
+    void and_assign(const BitVec& o) { for (size_t k = 0; k < w.size(); ++k) w[k] &= o.w[k]; }
+    uint32_t count_ones() const { uint32_t c = 0; for (auto x : w) c += __builtin_popcountll(x); return c; }
+    // bits.rs:175-178 (and_count), scalar form of and_count_words (bits.rs:23-38)
+    uint32_t and_count(const BitVec& o) const {
+        uint32_t c = 0; for (size_t k = 0; k < w.size(); ++k) c += __builtin_popcountll(w[k] & o.w[k]); return c;
+    }
+    // bits.rs:182-192
+    bool dot(const BitVec& o) const {
+        uint64_t a = 0; for (size_t k = 0; k < w.size(); ++k) a ^= w[k] & o.w[k];
+        return __builtin_popcountll(a) & 1;
+    }
+    bool is_zero() const { for (auto x : w) if (x) return false; return true; }
+    long first_one() const {
+        for (size_t k = 0; k < w.size(); ++k) if (w[k]) return (long)(k * 64 + __builtin_ctzll(w[k]));
+        return -1;
+    }
+    std::vector<size_t> ones() const {
+        std::vector<size_t> r;
+        for (size_t k = 0; k < w.size(); ++k) { uint64_t x = w[k]; while (x) { r.push_back(k * 64 + __builtin_ctzll(x)); x &= x - 1; } }
+        return r;
+    }
+    void clear() { std::fill(w.begin(), w.end(), 0); }
+    bool operator==(const BitVec& o) const { return len == o.len && w == o.w; }
+};
+
+// BitMat row-major (src/bits.rs:246-336).
+struct BitMat {
+    std::vector<BitVec> rows;
+    size_t ncols = 0;
+    BitMat() {}
+    BitMat(size_t r, size_t c) : rows(r, BitVec(c)), ncols(c) {}
+    bool get(size_t i, size_t j) const { return rows[i].get(j); }
+    void set(size_t i, size_t j, bool v) { rows[i].set(j, v); }
+    void flip_sym(size_t i, size_t j) { rows[i].flip(j); rows[j].flip(i); }
+};
+
+// --------------------------------------------------------------- phase.rs ---
+// Complex arithmetic with num-complex's formulas (plain products, Smith-free
+// division as in num-complex 0.4 Div).
+struct C64 {
+    double re = 0, im = 0;
+    C64() {}
+    C64(double r, double i) : re(r), im(i) {}
+};
+static inline C64 operator+(C64 a, C64 b) { return C64(a.re + b.re, a.im + b.im); }
+static inline C64 operator-(C64 a, C64 b) { return C64(a.re - b.re, a.im - b.im); }
+static inline C64 operator*(C64 a, C64 b) { return C64(a.re * b.re - a.im * b.im, a.re * b.im + a.im * b.re); }
+static inline C64 operator*(C64 a, double s) { return C64(a.re * s, a.im * s); }
+static inline C64 cdiv(C64 a, C64 b) {
+    double n = b.re * b.re + b.im * b.im;
+    return C64((a.re * b.re + a.im * b.im) / n, (a.im * b.re - a.re * b.im) / n);
+}
+
+typedef uint32_t Phase;  // angle 2*pi*k/2^32 (src/phase.rs:22-24)
+static const Phase PH_PI = 1u << 31, PH_HALF_PI = 1u << 30, PH_QUARTER_PI = 1u << 29;
+
+// phase.rs:96-108
+static const double HS = 0.70710678118654752440;
+static const C64 ROOT8[8] = {C64(1, 0), C64(HS, HS), C64(0, 1), C64(-HS, HS),
+                             C64(-1, 0), C64(-HS, -HS), C64(0, -1), C64(HS, -HS)};
+
+// phase.rs:39-44
+static Phase from_radians(double rad) {
+    double turns = rad / (2.0 * M_PI);
+    double frac = turns - std::floor(turns);
+    double k = std::round(frac * 4294967296.0);
+    return (Phase)((uint64_t)k & 0xFFFFFFFFull);
+}
+// phase.rs:55-57
+static double to_radians(Phase p) { return 2.0 * M_PI * (double)p / 4294967296.0; }
+// phase.rs:60-66
+static C64 to_complex(Phase p) {
+    if ((p & ((1u << 29) - 1)) == 0) return ROOT8[p >> 29];
+    double t = to_radians(p);
+    return C64(std::cos(t), std::sin(t));
+}
+// phase.rs:70-76
+static int clifford_level(Phase p) { return (p & ((1u << 30) - 1)) == 0 ? (int)(p >> 30) : -1; }
+// phase.rs:112-114
+static Phase z4_phase(uint32_t k) { return (k & 3u) << 30; }
+
+// ----------------------------------------------------------------- rng.rs ---
+// SplitMix64 (src/rng.rs:9-32)
+struct Rng {
+    uint64_t s;
+    explicit Rng(uint64_t seed) : s(seed + 0x9E3779B97F4A7C15ull) {}
+    uint64_t next_u64() {
+        s += 0x9E3779B97F4A7C15ull;
+        uint64_t z = s;
+        z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+        z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+        return z ^ (z >> 31);
+    }
+    double next_f64() { return (double)(next_u64() >> 11) / 9007199254740992.0; }
+    size_t below(size_t n) { return (size_t)(next_u64() % (uint64_t)n); }
+};
+
+// ------------------------------------------------------------- circuit.rs ---
+enum Op : uint8_t { H = 0, X, Y, Z, S, Sdg, T, Tdg, P, CZ, CX, Swap, GPhase };
+struct Gate { uint8_t op; uint32_t q0, q1; Phase ph; };
+struct Circuit {
+    size_t n = 0;
+    std::vector<Gate> gates;
+    void push(uint8_t op, uint32_t a, uint32_t b = 0, Phase ph = 0) {
+        // circuit.rs:104-110: grow num_qubits to cover touched wires
+        if (op != GPhase) { n = std::max<size_t>(n, a + 1); }
+        if (op == CZ || op == CX || op == Swap) n = std::max<size_t>(n, b + 1);
+        gates.push_back(Gate{op, a, b, ph});
+    }
+    // circuit.rs:206-215
+    size_t h_count() const {
+        size_t c = 0;
+        for (auto& g : gates) c += g.op == H ? 1 : (g.op == CX ? 2 : 0);
+        return c;
+    }
+    // circuit.rs:46-56 + 189-191
+    Circuit inverse() const {
+        Circuit r; r.n = n;
+        for (size_t i = gates.size(); i-- > 0;) {
+            Gate g = gates[i];
+            switch (g.op) {
+                case S: g.op = Sdg; break;
+                case Sdg: g.op = S; break;
+                case T: g.op = Tdg; break;
+                case Tdg: g.op = T; break;
+                case P: case GPhase: g.ph = (Phase)(0u - g.ph); break;
+                default: break;
+            }
+            r.gates.push_back(g);
+        }
+        return r;
+    }
+};
+
+// rng.rs:36-74
+static Circuit random_clifford_t(size_t n, size_t depth, size_t t_count, bool generic, Rng& rng) {
+    Circuit c; c.n = n;
+    size_t t_left = t_count;
+    size_t total = std::max(depth, t_count);
+    for (size_t step = 0; step < total; ++step) {
+        if (t_left > 0 && (total - step <= t_left || rng.next_f64() < 0.3)) {
+            uint32_t q = (uint32_t)rng.below(n);
+            switch (rng.below(generic ? 4 : 2)) {
+                case 0: c.push(T, q); break;
+                case 1: c.push(Tdg, q); break;
+                case 2: { int64_t num = 1 + 2 * (int64_t)rng.below(7); c.push(P, q, 0, (Phase)((uint64_t)(num << (31 - 4)) & 0xFFFFFFFFull)); break; }
+                default: { int64_t num = 1 + 2 * (int64_t)rng.below(3); c.push(P, q, 0, (Phase)((uint64_t)(num << (31 - 3)) & 0xFFFFFFFFull)); break; }
+            }
+            t_left -= 1;
+            continue;
+        }
+        uint32_t q = (uint32_t)rng.below(n);
+        size_t r = rng.below(n > 1 ? 9 : 5);
+        switch (r) {
+            case 0: case 1: c.push(H, q); break;
+            case 2: c.push(S, q); break;
+            case 3: c.push(Z, q); break;
+            case 4: c.push(X, q); break;
+            default: {
+                uint32_t q2 = (uint32_t)rng.below(n);
+                while (q2 == q) q2 = (uint32_t)rng.below(n);
+                if (r == 5 || r == 6) c.push(CZ, q, q2);
+                else if (r == 7) c.push(CX, q, q2);
+                else c.push(Swap, q, q2);
+            }
+        }
+    }
+    return c;
+}
+
+// rng.rs:79-102
+static Circuit random_layered(size_t n, size_t rounds, Rng& rng) {
+    Circuit c; c.n = n;
+    for (size_t r = 0; r < rounds; ++r) {
+        for (size_t q = 0; q < n; ++q) c.push(H, (uint32_t)q);
+        for (size_t q = 0; q < n; ++q) {
+            switch (rng.below(3)) { case 0: c.push(T, (uint32_t)q); break; case 1: c.push(Tdg, (uint32_t)q); break; default: c.push(S, (uint32_t)q); }
+        }
+        if (n > 1) for (size_t k = 0; k < n; ++k) {
+            size_t a = rng.below(n), b = rng.below(n);
+            if (a != b) c.push(CZ, (uint32_t)a, (uint32_t)b);
+        }
+    }
+    return c;
+}
+
+// --------------------------------------------------------------- dense.rs ---
+// dense.rs:12-107
+static C64 polar1(double t) { return C64(std::cos(t), std::sin(t)); }
+static void diag1(std::vector<C64>& sv, size_t q, C64 ph) {
+    size_t m = (size_t)1 << q;
+    for (size_t i = 0; i < sv.size(); ++i) if (i & m) sv[i] = sv[i] * ph;
+}
+static std::vector<C64> dense_statevector(const Circuit& c, const std::vector<bool>& input) {
+    size_t n = c.n, dim = (size_t)1 << n;
+    std::vector<C64> sv(dim);
+    size_t idx = 0;
+    for (size_t q = 0; q < n; ++q) if (input[q]) idx |= (size_t)1 << q;
+    sv[idx] = C64(1, 0);
+    const C64 I(0, 1);
+    for (auto& g : c.gates) {
+        switch (g.op) {
+            case H: { size_t m = (size_t)1 << g.q0; double r = HS;
+                for (size_t i = 0; i < dim; ++i) if (!(i & m)) { C64 a = sv[i], b = sv[i | m]; sv[i] = (a + b) * r; sv[i | m] = (a - b) * r; } break; }
+            case X: { size_t m = (size_t)1 << g.q0; for (size_t i = 0; i < dim; ++i) if (!(i & m)) std::swap(sv[i], sv[i | m]); break; }
+            case Y: { size_t m = (size_t)1 << g.q0;
+                for (size_t i = 0; i < dim; ++i) if (!(i & m)) { C64 a = sv[i], b = sv[i | m]; sv[i] = C64(0, -1) * b; sv[i | m] = I * a; } break; }
+            case Z: diag1(sv, g.q0, C64(-1, 0)); break;
+            case S: diag1(sv, g.q0, I); break;
+            case Sdg: diag1(sv, g.q0, C64(0, -1)); break;
+            case T: diag1(sv, g.q0, polar1(M_PI / 4)); break;
+            case Tdg: diag1(sv, g.q0, polar1(-M_PI / 4)); break;
+            case P: diag1(sv, g.q0, polar1(to_radians(g.ph))); break;
+            case CZ: { size_t ma = (size_t)1 << g.q0, mb = (size_t)1 << g.q1;
+                for (size_t i = 0; i < dim; ++i) if ((i & ma) && (i & mb)) sv[i] = C64(-sv[i].re, -sv[i].im); break; }
+            case CX: { size_t mc = (size_t)1 << g.q0, mt = (size_t)1 << g.q1;
+                for (size_t i = 0; i < dim; ++i) if ((i & mc) && !(i & mt)) std::swap(sv[i], sv[i | mt]); break; }
+            case Swap: { size_t ma = (size_t)1 << g.q0, mb = (size_t)1 << g.q1;
+                for (size_t i = 0; i < dim; ++i) if ((i & ma) && !(i & mb)) std::swap(sv[i], sv[(i ^ ma) | mb]); break; }
+            case GPhase: { C64 f = polar1(to_radians(g.ph)); for (auto& a : sv) a = a * f; break; }
+        }
+    }
+    return sv;
+}
+
+// ------------------------------------------------------------- pathsum.rs ---
+struct PathSum {  // pathsum.rs:58-81
+    size_t num_qubits = 0, num_vars = 0, num_h = 0;
+    BitMat b;
+    std::vector<uint8_t> v;
+    std::vector<Phase> theta;
+    Phase global = 0;
+    std::vector<bool> dead;
+    std::vector<size_t> outputs;
+    std::vector<bool> out_neg;
+    bool zero = false;
+
+    // pathsum.rs:317-324
+    void normalize() {
+        for (size_t k = 0; k < num_vars; ++k) {
+            int l = clifford_level(theta[k]);
+            if (l >= 0) { v[k] = (uint8_t)((v[k] + l) & 3); theta[k] = 0; }
+        }
+    }
+    // pathsum.rs:328-336
+    std::vector<size_t> io_vars() const {
+        std::vector<size_t> f;
+        for (size_t q = 0; q < num_qubits; ++q) f.push_back(q);
+        for (size_t o : outputs) if (std::find(f.begin(), f.end(), o) == f.end()) f.push_back(o);
+        return f;
+    }
+    std::vector<size_t> live_vars() const {
+        std::vector<size_t> r; for (size_t i = 0; i < num_vars; ++i) if (!dead[i]) r.push_back(i); return r;
+    }
+    // pathsum.rs:360-394
+    void substitute(size_t a, size_t bb, bool c) {
+        if (a == bb || dead[a] || dead[bb]) throw std::runtime_error("substitute: bad variables");
+        bool bab = b.get(a, bb);
+        Phase ta = theta[a], tb = theta[bb];
+        uint8_t vb = v[bb];
+        BitVec nb = b.rows[bb];
+        nb.set(a, false);
+        auto nbl = nb.ones();
+        if (!c) {
+            v[a] = (uint8_t)((v[a] + vb + 2 * (bab ? 1 : 0)) & 3);
+            theta[a] = ta + tb;
+        } else {
+            v[a] = (uint8_t)((v[a] + 4 - vb) & 3);
+            theta[a] = ta - tb;
+            global += z4_phase(vb) + tb;
+            for (size_t k : nbl) v[k] = (uint8_t)((v[k] + 2) & 3);
+        }
+        for (size_t k : nbl) b.flip_sym(a, k);
+        for (size_t k : b.rows[bb].ones()) b.rows[k].set(bb, false);
+        b.rows[bb].clear();
+        v[bb] = 0; theta[bb] = 0; dead[bb] = true;
+        int l = clifford_level(theta[a]);
+        if (l >= 0) { v[a] = (uint8_t)((v[a] + l) & 3); theta[a] = 0; }
+    }
+    // pathsum.rs:430-450
+    bool boundary(const std::vector<bool>& in, const std::vector<bool>& out, std::vector<std::pair<size_t, bool>>& map) const {
+        map.clear();
+        auto put = [&](size_t var, bool val) {
+            for (auto& e : map) if (e.first == var) return e.second == val;
+            map.push_back({var, val});
+            return true;
+        };
+        for (size_t q = 0; q < num_qubits; ++q) if (!put(q, in[q])) return false;
+        for (size_t q = 0; q < num_qubits; ++q) if (!put(outputs[q], out[q] ^ out_neg[q])) return false;
+        return true;
+    }
+    // pathsum.rs:454-489
+    C64 brute_force(const std::vector<std::pair<size_t, bool>>& fixed) const {
+        if (zero) return C64(0, 0);
+        std::vector<bool> val(num_vars, false), is_fixed(num_vars, false);
+        for (auto& f : fixed) { val[f.first] = f.second; is_fixed[f.first] = true; }
+        std::vector<size_t> fr;
+        for (size_t i = 0; i < num_vars; ++i) if (!dead[i] && !is_fixed[i]) fr.push_back(i);
+        if (fr.size() > 26) throw std::runtime_error("brute force over too many variables");
+        auto live = live_vars();
+        C64 total(0, 0);
+        for (uint64_t mask = 0; mask < (1ull << fr.size()); ++mask) {
+            for (size_t bi = 0; bi < fr.size(); ++bi) val[fr[bi]] = (mask >> bi) & 1;
+            uint32_t z4 = 0; Phase th = 0;
+            for (size_t j : live) if (val[j]) {
+                z4 += v[j]; th += theta[j];
+                for (size_t k : b.rows[j].ones()) if (k > j && val[k]) z4 += 2;
+            }
+            total = total + to_complex(z4_phase(z4 & 3) + th);
+        }
+        return total * to_complex(global) * std::pow(2.0, -(double)num_h / 2.0);
+    }
+};
+
+// Compiler: pathsum.rs:85-267
+struct Compiler {
+    size_t n;
+    std::vector<size_t> wire;
+    std::vector<bool> neg;
+    std::vector<BitVec> adj;   // grown geometrically like BitMat::grow_square
+    size_t cap;
+    std::vector<uint8_t> v;
+    std::vector<Phase> theta;
+    size_t nvars, num_h = 0;
+    Phase global = 0;
+    Compiler(size_t n_, size_t extra) : n(n_), neg(n_, false), cap(n_ + extra), v(n_, 0), theta(n_, 0), nvars(n_) {
+        for (size_t q = 0; q < n; ++q) wire.push_back(q);
+        adj.assign(cap, BitVec(cap));
+    }
+    size_t new_var() {  // pathsum.rs:120-129
+        if (nvars == cap) {
+            size_t nc = std::max<size_t>(cap * 2, 8);
+            std::vector<BitVec> fresh(nc, BitVec(nc));
+            for (size_t i = 0; i < cap; ++i) for (size_t j : adj[i].ones()) fresh[i].set(j, true);
+            adj.swap(fresh); cap = nc;
+        }
+        v.push_back(0); theta.push_back(0);
+        return nvars++;
+    }
+    void add_v(size_t z, uint8_t k) { v[z] = (uint8_t)((v[z] + k) & 3); }
+    void flip_sym(size_t a, size_t b) { adj[a].flip(b); adj[b].flip(a); }
+    void rotate(size_t q, Phase th) {  // pathsum.rs:215-224
+        size_t z = wire[q];
+        if (neg[q]) { theta[z] -= th; global += th; } else theta[z] += th;
+    }
+    void apply(const Gate& g) {  // pathsum.rs:136-213
+        size_t q = g.q0;
+        switch (g.op) {
+            case H: { size_t z = wire[q], zp = new_var(); flip_sym(z, zp); num_h++;
+                if (neg[q]) { add_v(zp, 2); neg[q] = false; } wire[q] = zp; break; }
+            case X: neg[q] = !neg[q]; break;
+            case Z: { add_v(wire[q], 2); if (neg[q]) global += PH_PI; break; }
+            case S: { if (neg[q]) { add_v(wire[q], 3); global += PH_HALF_PI; } else add_v(wire[q], 1); break; }
+            case Sdg: { if (neg[q]) { add_v(wire[q], 1); global -= PH_HALF_PI; } else add_v(wire[q], 3); break; }
+            case T: rotate(q, PH_QUARTER_PI); break;
+            case Tdg: rotate(q, (Phase)(0u - PH_QUARTER_PI)); break;
+            case P: rotate(q, g.ph); break;
+            case Y: { apply(Gate{Z, g.q0, 0, 0}); apply(Gate{X, g.q0, 0, 0}); global += PH_HALF_PI; break; }
+            case CZ: { size_t a = g.q0, b = g.q1, za = wire[a], zb = wire[b];
+                flip_sym(za, zb);
+                if (neg[b]) add_v(za, 2);
+                if (neg[a]) add_v(zb, 2);
+                if (neg[a] && neg[b]) global += PH_PI;
+                break; }
+            case CX: { apply(Gate{H, g.q1, 0, 0}); apply(Gate{CZ, g.q0, g.q1, 0}); apply(Gate{H, g.q1, 0, 0}); break; }
+            case Swap: { std::swap(wire[g.q0], wire[g.q1]); bool t = neg[g.q0]; neg[g.q0] = neg[g.q1]; neg[g.q1] = t; break; }
+            case GPhase: global += g.ph; break;
+        }
+    }
+    PathSum snapshot() const {  // pathsum.rs:236-261
+        PathSum ps;
+        size_t nv = nvars;
+        ps.num_qubits = n; ps.num_vars = nv; ps.num_h = num_h;
+        ps.b = BitMat(nv, nv);
+        for (size_t i = 0; i < nv; ++i) for (size_t j : adj[i].ones()) if (j < nv) ps.b.set(i, j, true);
+        ps.v = v; ps.theta = theta; ps.global = global;
+        ps.dead.assign(nv, false);
+        ps.outputs = wire; ps.out_neg = neg; ps.zero = false;
+        ps.normalize();
+        return ps;
+    }
+};
+
+// pathsum.rs:274-297 (no faults)
+static PathSum from_circuit(const Circuit& c) {
+    Compiler comp(c.n, c.h_count());
+    for (auto& g : c.gates) comp.apply(g);
+    return comp.snapshot();
+}
+// pathsum.rs:304-314
+struct SandwichPS { PathSum ps; std::vector<size_t> junction; std::vector<bool> jneg; };
+static SandwichPS sandwich(const Circuit& c) {
+    Compiler comp(c.n, 2 * c.h_count());
+    for (auto& g : c.gates) comp.apply(g);
+    SandwichPS s; s.junction = comp.wire; s.jneg = comp.neg;
+    Circuit inv = c.inverse();
+    for (auto& g : inv.gates) comp.apply(g);
+    s.ps = comp.snapshot();
+    return s;
+}
+
+// --------------------------------------------------------- normal_form.rs ---
+static const size_t PAR_WORK = 1 << 15;  // normal_form.rs:56 (only a threading threshold)
+
+// normal_form.rs:61-82
+static void single_step(std::vector<BitVec>& gram, std::vector<BitVec>& basis, BitVec& alive, size_t a, BitVec& s_out, BitVec& pa_out) {
+    BitVec s = gram[a];
+    s.and_assign(alive);
+    s.set(a, false);
+    BitVec pa = basis[a];
+    for (size_t bi : s.ones()) { gram[bi].xor_assign(s); basis[bi].xor_assign(pa); }
+    alive.set(a, false);
+    s_out = s; pa_out = pa;
+}
+// normal_form.rs:87-129
+static size_t hyper_step(std::vector<BitVec>& gram, std::vector<BitVec>& basis, BitVec& alive, size_t a, BitVec& touched, BitVec& pa, BitVec& pb) {
+    BitVec ra = gram[a]; ra.and_assign(alive);
+    long bl = ra.first_one();
+    if (bl < 0) throw std::runtime_error("row has no neighbour");
+    size_t b = (size_t)bl;
+    BitVec sa = ra; sa.set(b, false);
+    BitVec sb = gram[b]; sb.and_assign(alive); sb.set(a, false);
+    pa = basis[a]; pb = basis[b];
+    touched = sa;
+    BitVec both = sa; both.and_assign(sb);
+    touched.xor_assign(sb); touched.xor_assign(both);
+    for (size_t k : touched.ones()) {
+        if (sa.get(k)) { gram[k].xor_assign(sb); basis[k].xor_assign(pb); }
+        if (sb.get(k)) { gram[k].xor_assign(sa); basis[k].xor_assign(pa); }
+    }
+    alive.set(a, false); alive.set(b, false);
+    return b;
+}
+
+struct UVec {  // normal_form.rs:161-201
+    BitVec a, b, o, e;
+    void xor_assign(const UVec& x) { a.xor_assign(x.a); b.xor_assign(x.b); o.xor_assign(x.o); e.xor_assign(x.e); }
+    void clear() { a.clear(); b.clear(); o.clear(); e.clear(); }
+};
+struct Diag { BitVec wa, wb, o1, o3, we; int32_t sigma = 0; };  // normal_form.rs:205-218
+
+enum PivotRule { First = 0, MinWeight = 1 };
+
+struct Structure {  // normal_form.rs:146-157
+    size_t m = 0;
+    std::vector<BitVec> pairs_a, pairs_b, odd, even;
+    size_t g() const { return pairs_a.size(); }
+    UVec zero_u() const { return UVec{BitVec(pairs_a.size()), BitVec(pairs_b.size()), BitVec(odd.size()), BitVec(even.size())}; }
+
+    // normal_form.rs:240-336
+    static Structure build(const BitMat& b, const BitVec& parity, PivotRule rule) {
+        size_t m = b.rows.size();
+        std::vector<BitVec> gram(m), basis(m, BitVec(m));
+        for (size_t i = 0; i < m; ++i) { gram[i] = b.rows[i]; gram[i].set(i, parity.get(i)); basis[i].set(i, true); }
+        BitVec alive(m);
+        for (size_t i = 0; i < m; ++i) alive.set(i, true);
+        Structure st; st.m = m;
+        if (rule == First) {
+            for (;;) {
+                long a = -1;
+                for (size_t i : alive.ones()) if (gram[i].get(i)) { a = (long)i; break; }
+                if (a >= 0) { BitVec s, pa; single_step(gram, basis, alive, (size_t)a, s, pa); st.odd.push_back(pa); continue; }
+                for (size_t i : alive.ones()) if (gram[i].and_count(alive) > 0) { a = (long)i; break; }
+                if (a < 0) break;
+                BitVec t, pa, pb; hyper_step(gram, basis, alive, (size_t)a, t, pa, pb);
+                st.pairs_a.push_back(pa); st.pairs_b.push_back(pb);
+            }
+        } else {
+            std::vector<std::pair<uint8_t, uint32_t>> key(m, {0, 0});
+            std::set<std::pair<uint32_t, size_t>> diag_set, off_set;
+            auto reclass = [&](size_t a) {
+                if (key[a].first == 1) diag_set.erase({key[a].second, a});
+                else if (key[a].first == 2) off_set.erase({key[a].second, a});
+                if (!alive.get(a)) { key[a] = {0, 0}; return; }
+                uint32_t w = gram[a].and_count(alive);
+                if (gram[a].get(a)) { key[a] = {1, w}; diag_set.insert({w, a}); }
+                else if (w > 0) { key[a] = {2, w}; off_set.insert({w, a}); }
+                else key[a] = {0, 0};
+            };
+            for (size_t a = 0; a < m; ++a) reclass(a);
+            for (;;) {
+                if (!diag_set.empty()) {
+                    size_t a = diag_set.begin()->second;
+                    BitVec s, pa; single_step(gram, basis, alive, a, s, pa);
+                    st.odd.push_back(pa);
+                    reclass(a);
+                    for (size_t bi : s.ones()) reclass(bi);
+                    continue;
+                }
+                if (!off_set.empty()) {
+                    size_t a = off_set.begin()->second;
+                    BitVec t, pa, pb;
+                    size_t partner = hyper_step(gram, basis, alive, a, t, pa, pb);
+                    st.pairs_a.push_back(pa); st.pairs_b.push_back(pb);
+                    reclass(a); reclass(partner);
+                    for (size_t k : t.ones()) reclass(k);
+                    continue;
+                }
+                break;
+            }
+        }
+        for (size_t r : alive.ones()) st.even.push_back(basis[r]);
+        return st;
+    }
+
+    // normal_form.rs:339-348
+    static uint8_t z4_value(const BitMat& b, const BitVec& v0, const BitVec& v1, const BitVec& p) {
+        uint32_t lin = p.and_count(v0) + 2 * p.and_count(v1);
+        uint32_t t = 0;
+        for (size_t j : p.ones()) t += b.rows[j].and_count(p);
+        uint32_t ep = (t / 2) & 1;
+        return (uint8_t)((lin + 2 * ep) & 3);
+    }
+    // normal_form.rs:354-391
+    Diag diagonal(const BitMat& b, const std::vector<uint8_t>& v) const {
+        BitVec v0(m), v1(m);
+        for (size_t i = 0; i < m; ++i) { v0.set(i, v[i] & 1); v1.set(i, v[i] & 2); }
+        Diag d;
+        size_t gg = g();
+        d.wa = BitVec(gg); d.wb = BitVec(gg);
+        for (size_t j = 0; j < gg; ++j) {
+            d.wa.set(j, z4_value(b, v0, v1, pairs_a[j]) == 2);
+            d.wb.set(j, z4_value(b, v0, v1, pairs_b[j]) == 2);
+        }
+        d.o1 = BitVec(odd.size()); d.o3 = BitVec(odd.size());
+        for (size_t l = 0; l < odd.size(); ++l) {
+            if (z4_value(b, v0, v1, odd[l]) == 1) { d.o1.set(l, true); d.sigma += 1; }
+            else { d.o3.set(l, true); d.sigma -= 1; }
+        }
+        d.we = BitVec(even.size());
+        for (size_t l = 0; l < even.size(); ++l) d.we.set(l, z4_value(b, v0, v1, even[l]) == 2);
+        return d;
+    }
+    // normal_form.rs:395-406
+    UVec transform(const BitVec& x) const {
+        UVec u = zero_u();
+        for (size_t i = 0; i < pairs_a.size(); ++i) u.a.set(i, pairs_a[i].dot(x));
+        for (size_t i = 0; i < pairs_b.size(); ++i) u.b.set(i, pairs_b[i].dot(x));
+        for (size_t i = 0; i < odd.size(); ++i) u.o.set(i, odd[i].dot(x));
+        for (size_t i = 0; i < even.size(); ++i) u.e.set(i, even[i].dot(x));
+        return u;
+    }
+    // normal_form.rs:437-452
+    UVec column_of(size_t j) const {
+        UVec u = zero_u();
+        for (size_t l = 0; l < pairs_a.size(); ++l) u.a.set(l, pairs_a[l].get(j));
+        for (size_t l = 0; l < pairs_b.size(); ++l) u.b.set(l, pairs_b[l].get(j));
+        for (size_t l = 0; l < odd.size(); ++l) u.o.set(l, odd[l].get(j));
+        for (size_t l = 0; l < even.size(); ++l) u.e.set(l, even[l].get(j));
+        return u;
+    }
+    // normal_form.rs:465-467
+    double log2_magnitude() const { return (double)g() + (double)even.size() + (double)odd.size() / 2.0; }
+    // normal_form.rs:472-474
+    static bool survives(const Diag& d, const UVec& u) { return u.e == d.we; }
+    // normal_form.rs:479-493
+    static uint8_t phase8(const Diag& d, const UVec& u) {
+        int32_t pc1 = (int32_t)u.o.and_count(d.o1), pc3 = (int32_t)u.o.and_count(d.o3);
+        int32_t n = d.sigma - 2 * (pc1 - pc3);
+        uint32_t cnt = 0;
+        for (size_t i = 0; i < d.wa.w.size(); ++i) cnt += __builtin_popcountll((d.wa.w[i] ^ u.a.w[i]) & (d.wb.w[i] ^ u.b.w[i]));
+        if (cnt & 1) n += 4;
+        return (uint8_t)(((n % 8) + 8) % 8);
+    }
+    // normal_form.rs:496-501
+    C64 gauss_sum(const Diag& d, const UVec& u) const {
+        if (!survives(d, u)) return C64(0, 0);
+        return ROOT8[phase8(d, u)] * std::pow(2.0, log2_magnitude());
+    }
+};
+
+// -------------------------------------------------------------- engine.rs ---
+static const size_t NONE = (size_t)-1;
+static const size_t LOW_BITS = 14;         // engine.rs:102
+static const size_t RECOMPUTE_EVERY = 2048; // engine.rs:103
+
+struct EngineStats { size_t free_vars, planes, lines, radical, terms, branch_dim; };  // engine.rs:64-79
+
+struct Elim {  // engine.rs:106-116
+    size_t t = 0;
+    std::vector<std::pair<size_t, BitVec>> pivots;
+    std::vector<BitVec> consistency, kernel;
+    std::vector<std::vector<size_t>> kernel_supp;
+    std::vector<UVec> kernel_u;
+
+    // engine.rs:119-184
+    static Elim build(const Structure& st, const std::vector<UVec>& mcols) {
+        Elim el; el.t = mcols.size();
+        size_t t = el.t, ne = st.even.size();
+        std::vector<std::pair<BitVec, BitVec>> rows;
+        for (size_t e = 0; e < ne; ++e) {
+            BitVec k(t);
+            for (size_t c = 0; c < t; ++c) if (mcols[c].e.get(e)) k.set(c, true);
+            BitVec r(ne); r.set(e, true);
+            rows.push_back({k, r});
+        }
+        std::vector<size_t> pivot_cols;
+        size_t rank = 0;
+        for (size_t col = 0; col < t; ++col) {
+            size_t r = NONE;
+            for (size_t i = rank; i < rows.size(); ++i) if (rows[i].first.get(col)) { r = i; break; }
+            if (r == NONE) continue;
+            std::swap(rows[rank], rows[r]);
+            BitVec pk = rows[rank].first, pr = rows[rank].second;
+            for (size_t r2 = 0; r2 < rows.size(); ++r2)
+                if (r2 != rank && rows[r2].first.get(col)) { rows[r2].first.xor_assign(pk); rows[r2].second.xor_assign(pr); }
+            pivot_cols.push_back(col);
+            rank++;
+        }
+        for (size_t i = 0; i < rank; ++i) el.pivots.push_back({pivot_cols[i], rows[i].second});
+        for (size_t i = rank; i < rows.size(); ++i) el.consistency.push_back(rows[i].second);
+        std::vector<bool> is_pivot(t, false);
+        for (size_t c : pivot_cols) is_pivot[c] = true;
+        for (size_t j = 0; j < t; ++j) {
+            if (is_pivot[j]) continue;
+            BitVec nv(t); nv.set(j, true);
+            for (size_t i = 0; i < rank; ++i) if (rows[i].first.get(j)) nv.set(pivot_cols[i], true);
+            el.kernel.push_back(nv);
+        }
+        for (auto& nv : el.kernel) el.kernel_supp.push_back(nv.ones());
+        for (auto& supp : el.kernel_supp) {
+            UVec u = st.zero_u();
+            for (size_t k : supp) u.xor_assign(mcols[k]);
+            el.kernel_u.push_back(u);
+        }
+        return el;
+    }
+    // engine.rs:188-201
+    bool solve_into(const BitVec& kappa, BitVec& s) const {
+        for (auto& r : consistency) if (r.dot(kappa)) return false;
+        s.clear();
+        for (auto& p : pivots) if (p.second.dot(kappa)) s.set(p.first, true);
+        return true;
+    }
+};
+
+struct Term { C64 a[2], up, down; };  // engine.rs:221-238
+static Term make_term(Phase th) {
+    C64 e = to_complex(th), one(1, 0);
+    Term t;
+    t.a[0] = (one + e) * 0.5;
+    t.a[1] = (one - e) * 0.5;
+    t.up = cdiv(t.a[1], t.a[0]);
+    t.down = cdiv(t.a[0], t.a[1]);
+    return t;
+}
+
+struct Engine {  // engine.rs:240-284 (Core + Branching + Engine flattened)
+    std::vector<size_t> fixed, fixed_pos, free_;
+    BitMat b_fixed, b_ff;
+    Structure st;
+    std::vector<UVec> ucols, mcols;
+    std::vector<size_t> term_vars;
+    Elim elim;
+    Diag diag;
+    std::vector<uint8_t> fixed_v;
+    std::vector<Phase> fixed_theta;
+    std::vector<Term> terms;
+    std::vector<Phase> term_theta;
+    size_t num_h = 0;
+    Phase global = 0;
+    bool zero = false;
+    size_t parallel_min_dim = 16;   // engine.rs:57-61
+    int threads = 1;
+
+    // engine.rs:289-370 (+ build_branching :372-378, bind_values :381-390)
+    static Engine build(const PathSum& ps, const std::vector<size_t>& fixed, PivotRule rule) {
+        Engine E;
+        size_t nv = ps.num_vars;
+        E.fixed_pos.assign(nv, NONE);
+        for (size_t i = 0; i < fixed.size(); ++i) {
+            size_t f = fixed[i];
+            if (f >= nv || ps.dead[f]) throw std::runtime_error("fixed variable is not live");
+            if (E.fixed_pos[f] != NONE) throw std::runtime_error("duplicate fixed variable");
+            E.fixed_pos[f] = i;
+        }
+        for (size_t i = 0; i < nv; ++i) if (!ps.dead[i] && E.fixed_pos[i] == NONE) E.free_.push_back(i);
+        size_t m = E.free_.size();
+        std::vector<size_t> fidx(nv, NONE);
+        for (size_t i = 0; i < m; ++i) fidx[E.free_[i]] = i;
+        E.b_ff = BitMat(m, m);
+        for (size_t i = 0; i < m; ++i) for (size_t j : ps.b.rows[E.free_[i]].ones()) if (fidx[j] != NONE) E.b_ff.set(i, fidx[j], true);
+        BitVec parity(m);
+        for (size_t i = 0; i < m; ++i) parity.set(i, ps.v[E.free_[i]] & 1);
+        size_t nf = fixed.size();
+        E.b_fixed = BitMat(nf, nf);
+        for (size_t i = 0; i < nf; ++i) for (size_t j : ps.b.rows[fixed[i]].ones()) if (E.fixed_pos[j] != NONE) E.b_fixed.set(i, E.fixed_pos[j], true);
+        E.st = Structure::build(E.b_ff, parity, rule);
+        // ucols: P^T (column of B between free vars and fixed f); transform_sparse over
+        // column vectors is the same map as transform (normal_form.rs:413-461).
+        for (size_t i = 0; i < nf; ++i) {
+            BitVec col(m);
+            for (size_t j : ps.b.rows[fixed[i]].ones()) if (fidx[j] != NONE) col.set(fidx[j], true);
+            E.ucols.push_back(E.st.transform(col));
+        }
+        for (size_t i = 0; i < m; ++i) if (ps.theta[E.free_[i]] != 0) E.term_vars.push_back(i);
+        for (size_t i : E.term_vars) E.mcols.push_back(E.st.column_of(i));
+        E.elim = Elim::build(E.st, E.mcols);
+        std::vector<uint8_t> v_free(m);
+        for (size_t i = 0; i < m; ++i) v_free[i] = ps.v[E.free_[i]];
+        E.diag = E.st.diagonal(E.b_ff, v_free);
+        for (size_t i : E.term_vars) { E.terms.push_back(make_term(ps.theta[E.free_[i]])); E.term_theta.push_back(ps.theta[E.free_[i]]); }
+        E.fixed = fixed;
+        for (size_t f : fixed) { E.fixed_v.push_back(ps.v[f]); E.fixed_theta.push_back(ps.theta[f]); }
+        E.num_h = ps.num_h; E.global = ps.global; E.zero = ps.zero;
+        return E;
+    }
+    // engine.rs:415-425
+    EngineStats stats() const { return EngineStats{free_.size(), st.g(), st.odd.size(), st.even.size(), term_vars.size(), elim.kernel.size()}; }
+    // engine.rs:563-565
+    double magnitude() const { return std::pow(2.0, st.log2_magnitude() - (double)num_h / 2.0); }
+
+    // engine.rs:569-610
+    bool prepare(const std::vector<bool>& values, Phase& phase, C64& c_start, UVec& u, BitVec& s0, UVec* u0_out = nullptr) const {
+        if (zero) return false;
+        std::vector<size_t> ones;
+        for (size_t i = 0; i < values.size(); ++i) if (values[i]) ones.push_back(i);
+        uint32_t z4 = 0; Phase th = global;
+        for (size_t ii = 0; ii < ones.size(); ++ii) {
+            size_t f = ones[ii];
+            z4 += fixed_v[f]; th += fixed_theta[f];
+            for (size_t jj = ii + 1; jj < ones.size(); ++jj) if (b_fixed.get(f, ones[jj])) z4 += 2;
+        }
+        phase = th + z4_phase(z4 & 3);
+        UVec u0 = st.zero_u();
+        for (size_t f : ones) u0.xor_assign(ucols[f]);
+        if (u0_out) *u0_out = u0;
+        BitVec kappa = diag.we; kappa.xor_assign(u0.e);
+        s0 = BitVec(terms.size());
+        if (!elim.solve_into(kappa, s0)) return false;
+        u = u0;
+        c_start = C64(1, 0);
+        for (size_t k = 0; k < terms.size(); ++k) {
+            if (s0.get(k)) { u.xor_assign(mcols[k]); c_start = c_start * terms[k].a[1]; }
+            else c_start = c_start * terms[k].a[0];
+        }
+        return true;
+    }
+    // engine.rs:661-667
+    C64 weight(const BitVec& sv) const {
+        C64 c(1, 0);
+        for (size_t k = 0; k < terms.size(); ++k) c = c * terms[k].a[sv.get(k) ? 1 : 0];
+        return c;
+    }
+    // engine.rs:696-722
+    C64 gray(UVec u, BitVec sv, C64 c, size_t dl) const {
+        C64 acc(0, 0);
+        size_t steps = (size_t)1 << dl;
+        for (size_t i = 0; i < steps; ++i) {
+            uint8_t k8 = Structure::phase8(diag, u);
+            acc = acc + c * ROOT8[k8];
+            if (i + 1 < steps) {
+                size_t j = (size_t)__builtin_ctzll(i + 1);
+                u.xor_assign(elim.kernel_u[j]);
+                if ((i + 1) % RECOMPUTE_EVERY == 0) { sv.xor_assign(elim.kernel[j]); c = weight(sv); }
+                else for (size_t k : elim.kernel_supp[j]) {
+                    c = c * (sv.get(k) ? terms[k].down : terms[k].up);
+                    sv.flip(k);
+                }
+            }
+        }
+        return acc;
+    }
+    // engine.rs:669-693 (Rayon par_iter over chunks -> OpenMP dynamic schedule)
+    C64 sum_subspace(const UVec& u_start, const BitVec& s_start, C64 c_start) const {
+        size_t d = elim.kernel.size();
+        size_t dl = std::min(d, LOW_BITS);
+        size_t nhi = (size_t)1 << (d - dl);
+        auto chunk = [&](size_t hi) {
+            UVec u = u_start; BitVec sv = s_start; C64 c = c_start;
+            if (hi != 0) {
+                for (size_t j = 0; j < d - dl; ++j) if ((hi >> j) & 1) { u.xor_assign(elim.kernel_u[dl + j]); sv.xor_assign(elim.kernel[dl + j]); }
+                c = weight(sv);
+            }
+            return gray(u, sv, c, dl);
+        };
+        if (d >= parallel_min_dim && nhi > 1 && threads > 1) {
+            double sr = 0, si = 0;
+            #pragma omp parallel for schedule(dynamic, 1) reduction(+:sr, si) num_threads(threads)
+            for (long hi = 0; hi < (long)nhi; ++hi) { C64 r = chunk((size_t)hi); sr += r.re; si += r.im; }
+            return C64(sr, si);
+        }
+        C64 tot(0, 0);
+        for (size_t hi = 0; hi < nhi; ++hi) tot = tot + chunk(hi);
+        return tot;
+    }
+    // engine.rs:557-561
+    C64 eval(const std::vector<bool>& values) const {
+        Phase ph; C64 cs; UVec u; BitVec s0;
+        if (!prepare(values, ph, cs, u, s0)) return C64(0, 0);
+        C64 total = sum_subspace(u, s0, cs);
+        return to_complex(ph) * total * magnitude();
+    }
+    // engine.rs:515-533
+    C64 eval_assignment(const std::vector<std::pair<size_t, bool>>& asg) const {
+        std::vector<bool> vals(fixed.size(), false);
+        size_t seen = 0;
+        for (auto& a : asg) {
+            size_t p = a.first < fixed_pos.size() ? fixed_pos[a.first] : NONE;
+            if (p == NONE) throw std::runtime_error("variable is not fixed in this engine");
+            vals[p] = a.second; seen++;
+        }
+        if (seen < fixed.size()) throw std::runtime_error("assignment does not cover all fixed variables");
+        return eval(vals);
+    }
+    void values_of(const std::vector<std::pair<size_t, bool>>& asg, std::vector<bool>& vals) const {
+        vals.assign(fixed.size(), false);
+        for (auto& a : asg) vals[fixed_pos[a.first]] = a.second;
+    }
+    // Per-branch Gauss-sum outputs for an arbitrary t-bit branch vector s (the
+    // shift u(s) = u0 xor M s of engine.rs:28-31): zero flag (normal_form.rs:472),
+    // r = 2*log2|G| (normal_form.rs:465) and k8 = phase8 (normal_form.rs:479).
+    void branch(const UVec& u0, const BitVec& s, uint8_t& zero_flag, uint16_t& r, uint8_t& k8) const {
+        UVec u = u0;
+        for (size_t k = 0; k < terms.size(); ++k) if (s.get(k)) u.xor_assign(mcols[k]);
+        zero_flag = Structure::survives(diag, u) ? 0 : 1;
+        r = (uint16_t)(2 * st.g() + 2 * st.even.size() + st.odd.size());
+        k8 = zero_flag ? 0 : Structure::phase8(diag, u);
+    }
+};
+
+// ---------------------------------------------------------------- rank.rs ---
+static const size_t MAX_PAIRS = 14;  // rank.rs:44
+static const uint32_t MASK30 = (1u << 30) - 1;
+
+// rank.rs:62-84
+static std::vector<std::pair<size_t, size_t>> pair_candidates(const PathSum& ps, const std::vector<size_t>& fixed, size_t& nt) {
+    std::map<uint32_t, std::vector<size_t>> groups;
+    for (size_t k = 0; k < ps.num_vars; ++k) {
+        if (ps.dead[k] || ps.theta[k] == 0 || std::find(fixed.begin(), fixed.end(), k) != fixed.end()) continue;
+        uint32_t r = ps.theta[k] & MASK30;
+        uint32_t other = (uint32_t)(((uint64_t)MASK30 + 1 - r) & MASK30);
+        groups[std::min(r, other)].push_back(k);
+    }
+    std::vector<std::pair<size_t, size_t>> tp, gp;
+    for (auto& kv : groups) {
+        auto& target = kv.first == (1u << 29) ? tp : gp;
+        for (size_t i = 0; i + 1 < kv.second.size(); i += 2) target.push_back({kv.second[i], kv.second[i + 1]});
+    }
+    nt = tp.size();
+    tp.insert(tp.end(), gp.begin(), gp.end());
+    return tp;
+}
+// rank.rs:205-222
+static size_t best_pairs(const EngineStats& st, size_t avail, size_t tcp) {
+    double m = (double)st.free_vars;
+    double words = (double)(st.free_vars / 64 + 1);
+    double ldl = 2.0 * m * m * m / 64.0 + 50.0 * m + 2000.0;
+    double step = 20.0 + 4.0 * words + (double)st.terms / 2.0;
+    size_t bp = 0; double bc = INFINITY;
+    for (size_t p = 0; p <= std::min(avail, MAX_PAIRS); ++p) {
+        size_t removed = 2 * std::min(p, tcp) + (p > tcp ? p - tcp : 0);
+        int d = (int)(st.branch_dim > removed ? st.branch_dim - removed : 0);
+        double cost = std::pow(2.0, (double)p) * ((p == 0 ? 0.0 : ldl) + std::pow(2.0, (double)d) * step);
+        if (cost < bc) { bp = p; bc = cost; }
+    }
+    return bp;
+}
+
+struct RankEngine {  // rank.rs:47-51
+    std::vector<Engine> branches;
+    size_t pairs = 0;
+    // rank.rs:88-103
+    static RankEngine build(const PathSum& ps, const std::vector<size_t>& fixed, size_t pairs, PivotRule rule) {
+        size_t nt;
+        auto cands = pair_candidates(ps, fixed, nt);
+        size_t p = std::min(std::min(pairs, cands.size()), MAX_PAIRS);
+        RankEngine R; R.pairs = p;
+        for (size_t pattern = 0; pattern < ((size_t)1 << p); ++pattern) {
+            PathSum b = ps;
+            for (size_t i = 0; i < p; ++i) b.substitute(cands[i].first, cands[i].second, (pattern >> i) & 1);
+            R.branches.push_back(Engine::build(b, fixed, rule));
+        }
+        return R;
+    }
+    // rank.rs:115-123
+    static RankEngine automatic(const PathSum& ps, const std::vector<size_t>& fixed, PivotRule rule) {
+        Engine base = Engine::build(ps, fixed, rule);
+        size_t nt;
+        auto cands = pair_candidates(ps, fixed, nt);
+        size_t p = best_pairs(base.stats(), cands.size(), nt);
+        if (p == 0) { RankEngine R; R.branches.push_back(std::move(base)); R.pairs = 0; return R; }
+        return build(ps, fixed, p, rule);
+    }
+    // rank.rs:184-190
+    C64 eval_assignment(const std::vector<std::pair<size_t, bool>>& asg) const {
+        C64 tot(0, 0);
+        for (auto& e : branches) tot = tot + e.eval_assignment(asg);
+        return tot;
+    }
+    void set_threads(int t) { for (auto& e : branches) e.threads = t; }
+};
+
+// ----------------------------------------------------------------- sim.rs ---
+enum Strategy { Auto = 0, Cached = 1, Pairs = 2 };  // sim.rs:40-48
+
+static RankEngine build_engine(const PathSum& ps, const std::vector<size_t>& fixed, int strategy, size_t pairs) {  // sim.rs:65-71
+    if (strategy == Auto) return RankEngine::automatic(ps, fixed, MinWeight);
+    if (strategy == Cached) return RankEngine::build(ps, fixed, 0, MinWeight);
+    return RankEngine::build(ps, fixed, pairs, MinWeight);
+}
+
+struct Simulator {  // sim.rs:128-135
+    Circuit circuit;
+    PathSum ps;
+    RankEngine engine;
+    int strategy = Auto;
+    size_t pairs_opt = 0;
+    bool have_sw = false;
+    SandwichPS sw;
+    int threads = 1;
+
+    // sim.rs:171-176
+    C64 amplitude(const std::vector<bool>& in, const std::vector<bool>& out) const {
+        std::vector<std::pair<size_t, bool>> bd;
+        if (!ps.boundary(in, out, bd)) return C64(0, 0);
+        return engine.eval_assignment(bd);
+    }
+    // sim.rs:225-231 with Sandwich::fixed_for / assignment (sim.rs:95-124)
+    double marginal(const std::vector<bool>& in, const std::vector<std::pair<size_t, bool>>& measured) {
+        if (!have_sw) { sw = sandwich(circuit); have_sw = true; }
+        std::vector<std::pair<size_t, bool>> a;
+        auto put = [&](size_t var, bool val) {
+            for (auto& e : a) if (e.first == var) return e.second == val;
+            a.push_back({var, val}); return true;
+        };
+        for (size_t q = 0; q < sw.ps.num_qubits; ++q) if (!put(q, in[q])) return 0.0;
+        for (size_t q = 0; q < sw.ps.num_qubits; ++q) if (!put(sw.ps.outputs[q], in[q] ^ sw.ps.out_neg[q])) return 0.0;
+        for (auto& m : measured) if (!put(sw.junction[m.first], m.second ^ sw.jneg[m.first])) return 0.0;
+        std::vector<size_t> f = sw.ps.io_vars();
+        for (auto& m : measured) if (std::find(f.begin(), f.end(), sw.junction[m.first]) == f.end()) f.push_back(sw.junction[m.first]);
+        RankEngine eng = build_engine(sw.ps, f, strategy, pairs_opt);
+        eng.set_threads(threads);
+        return std::max(eng.eval_assignment(a).re, 0.0);
+    }
+};
+
+static Circuit circuit_from(uint32_t n, size_t ng, const uint8_t* ops, const uint32_t* q0, const uint32_t* q1, const uint32_t* ph) {
+    Circuit c; c.n = n;
+    for (size_t i = 0; i < ng; ++i) c.push(ops[i], q0[i], q1[i], ph[i]);
+    c.n = std::max<size_t>(c.n, n);
+    return c;
+}
+static std::vector<bool> bits_of(size_t n, const uint8_t* b) { std::vector<bool> r(n); for (size_t i = 0; i < n; ++i) r[i] = b[i] != 0; return r; }
+
+static thread_local std::string g_err;
+
+}  // namespace orc
+
+using namespace orc;
+
+// ----------------------------------------------------------- C interface ----
+// Status codes: 0 ok, -1 error (message in orc_last_error()).
+extern "C" {
+
+const char* orc_last_error() { return g_err.c_str(); }
+
+uint64_t orc_splitmix_next(uint64_t* state) {
+    Rng r(0); r.s = *state; uint64_t x = r.next_u64(); *state = r.s; return x;
+}
+uint64_t orc_rng_u64(uint64_t seed) { Rng r(seed); return r.next_u64(); }  // pybind.rs:221-224
+
+// Phase ring helpers (phase.rs:39-66)
+uint32_t orc_phase_from_radians(double r) { return from_radians(r); }
+void orc_phase_to_complex(uint32_t p, double* out) { C64 c = to_complex(p); out[0] = c.re; out[1] = c.im; }
+
+// Generators (rng.rs:36-102).  Writes at most cap gates; returns gate count or -1.
+long orc_random_clifford_t(uint64_t* state, uint32_t n, uint32_t depth, uint32_t tcount, int generic,
+                           uint8_t* ops, uint32_t* q0, uint32_t* q1, uint32_t* ph, long cap) {
+    Rng r(0); r.s = *state;
+    Circuit c = random_clifford_t(n, depth, tcount, generic != 0, r);
+    *state = r.s;
+    if ((long)c.gates.size() > cap) return -1;
+    for (size_t i = 0; i < c.gates.size(); ++i) { ops[i] = c.gates[i].op; q0[i] = c.gates[i].q0; q1[i] = c.gates[i].q1; ph[i] = c.gates[i].ph; }
+    return (long)c.gates.size();
+}
+long orc_random_layered(uint64_t* state, uint32_t n, uint32_t rounds, uint8_t* ops, uint32_t* q0, uint32_t* q1, uint32_t* ph, long cap) {
+    Rng r(0); r.s = *state;
+    Circuit c = random_layered(n, rounds, r);
+    *state = r.s;
+    if ((long)c.gates.size() > cap) return -1;
+    for (size_t i = 0; i < c.gates.size(); ++i) { ops[i] = c.gates[i].op; q0[i] = c.gates[i].q0; q1[i] = c.gates[i].q1; ph[i] = c.gates[i].ph; }
+    return (long)c.gates.size();
+}
+
+// Dense statevector (dense.rs:12-27); out has 2*2^n doubles.
+int orc_dense_statevector(uint32_t n, size_t ng, const uint8_t* ops, const uint32_t* q0, const uint32_t* q1, const uint32_t* ph,
+                          const uint8_t* input, double* out) {
+    try {
+        Circuit c = circuit_from(n, ng, ops, q0, q1, ph);
+        if (c.n > 26) { g_err = "dense: too many qubits"; return -1; }
+        auto sv = dense_statevector(c, bits_of(c.n, input));
+        for (size_t i = 0; i < sv.size(); ++i) { out[2 * i] = sv[i].re; out[2 * i + 1] = sv[i].im; }
+        return 0;
+    } catch (std::exception& e) { g_err = e.what(); return -1; }
+}
+
+// Path-sum brute force amplitude <y|C|x> (pathsum.rs:454-489); returns 1 if the
+// boundary is contradictory (amplitude 0).
+int orc_pathsum_brute(uint32_t n, size_t ng, const uint8_t* ops, const uint32_t* q0, const uint32_t* q1, const uint32_t* ph,
+                      const uint8_t* x, const uint8_t* y, double* out) {
+    try {
+        Circuit c = circuit_from(n, ng, ops, q0, q1, ph);
+        PathSum ps = from_circuit(c);
+        std::vector<std::pair<size_t, bool>> bd;
+        if (!ps.boundary(bits_of(c.n, x), bits_of(c.n, y), bd)) { out[0] = out[1] = 0; return 1; }
+        C64 a = ps.brute_force(bd);
+        out[0] = a.re; out[1] = a.im;
+        return 0;
+    } catch (std::exception& e) { g_err = e.what(); return -1; }
+}
+
+// PathSum summary: num_vars, live vars, num_h, rotation count.
+int orc_pathsum_info(uint32_t n, size_t ng, const uint8_t* ops, const uint32_t* q0, const uint32_t* q1, const uint32_t* ph, uint64_t* out4) {
+    try {
+        PathSum ps = from_circuit(circuit_from(n, ng, ops, q0, q1, ph));
+        size_t rot = 0; for (size_t k = 0; k < ps.num_vars; ++k) if (!ps.dead[k] && ps.theta[k] != 0) rot++;
+        out4[0] = ps.num_vars; out4[1] = ps.live_vars().size(); out4[2] = ps.num_h; out4[3] = rot;
+        return 0;
+    } catch (std::exception& e) { g_err = e.what(); return -1; }
+}
+
+// Normal form check on an explicit form: B (m x m, row-major bytes), v (Z4).
+// Writes rank info (g, odd, even) and the closed-form Gauss sum at the shift
+// and the 2^m brute-force sum (normal_form.rs:522-543) for m <= 24; also
+// returns whether the basis block-diagonalises the Gram matrix and is invertible
+// (normal_form.rs:554-606).
+int orc_normal_form_check(uint32_t m, const uint8_t* B, const uint8_t* v, const uint8_t* shift, int rule, int do_brute,
+                          uint64_t* counts3, double* gs2, double* brute2, int* block_ok) {
+    try {
+        BitMat b(m, m);
+        for (uint32_t i = 0; i < m; ++i) for (uint32_t j = 0; j < m; ++j) if (B[i * m + j]) b.set(i, j, true);
+        BitVec parity(m);
+        std::vector<uint8_t> vv(v, v + m);
+        for (uint32_t i = 0; i < m; ++i) parity.set(i, v[i] & 1);
+        Structure st = Structure::build(b, parity, rule ? MinWeight : First);
+        counts3[0] = st.g(); counts3[1] = st.odd.size(); counts3[2] = st.even.size();
+        Diag d = st.diagonal(b, vv);
+        BitVec sh(m); for (uint32_t i = 0; i < m; ++i) sh.set(i, shift[i]);
+        C64 g = st.gauss_sum(d, st.transform(sh));
+        gs2[0] = g.re; gs2[1] = g.im;
+        if (do_brute && m <= 24) {
+            C64 tot(0, 0);
+            for (uint64_t x = 0; x < (1ull << m); ++x) {
+                uint32_t q = 0;
+                for (uint32_t j = 0; j < m; ++j) if ((x >> j) & 1) {
+                    q += v[j]; if (shift[j]) q += 2;
+                    for (uint32_t k = j + 1; k < m; ++k) if (((x >> k) & 1) && b.get(j, k)) q += 2;
+                }
+                tot = tot + ROOT8[2 * (q & 3)];
+            }
+            brute2[0] = tot.re; brute2[1] = tot.im;
+        }
+        // Gram check in the basis (gram_entry of normal_form.rs:545-552)
+        struct Tagged { const BitVec* p; int tp; size_t idx; };
+        std::vector<Tagged> all;
+        for (size_t j = 0; j < st.pairs_a.size(); ++j) all.push_back({&st.pairs_a[j], 0, j});
+        for (size_t j = 0; j < st.pairs_b.size(); ++j) all.push_back({&st.pairs_b[j], 1, j});
+        for (size_t j = 0; j < st.odd.size(); ++j) all.push_back({&st.odd[j], 2, j});
+        for (size_t j = 0; j < st.even.size(); ++j) all.push_back({&st.even[j], 3, j});
+        bool ok = all.size() == m;
+        for (auto& P : all) for (auto& Q : all) {
+            bool acc = false;
+            for (size_t j : P.p->ones()) { acc ^= b.rows[j].dot(*Q.p); acc ^= parity.get(j) && Q.p->get(j); }
+            bool expect = ((P.tp == 0 && Q.tp == 1) || (P.tp == 1 && Q.tp == 0)) ? P.idx == Q.idx : (P.tp == 2 && Q.tp == 2 ? P.idx == Q.idx : false);
+            if (acc != expect) ok = false;
+        }
+        std::vector<BitVec> rows; for (auto& P : all) rows.push_back(*P.p);
+        size_t rank = 0;
+        for (uint32_t c = 0; c < m; ++c) {
+            size_t r = NONE;
+            for (size_t i = rank; i < rows.size(); ++i) if (rows[i].get(c)) { r = i; break; }
+            if (r == NONE) continue;
+            std::swap(rows[rank], rows[r]);
+            for (size_t r2 = 0; r2 < rows.size(); ++r2) if (r2 != rank && rows[r2].get(c)) rows[r2].xor_assign(rows[rank]);
+            rank++;
+        }
+        if (rank != m) ok = false;
+        *block_ok = ok ? 1 : 0;
+        return 0;
+    } catch (std::exception& e) { g_err = e.what(); return -1; }
+}
+
+// ------------------------------------------------------------ simulator ---
+struct orc_sim { Simulator s; };
+
+int orc_sim_create(uint32_t n, size_t ng, const uint8_t* ops, const uint32_t* q0, const uint32_t* q1, const uint32_t* ph,
+                   int strategy, uint32_t pairs, orc_sim** out) {
+    try {
+        auto* h = new orc_sim;
+        h->s.circuit = circuit_from(n, ng, ops, q0, q1, ph);
+        h->s.strategy = strategy; h->s.pairs_opt = pairs;
+        h->s.ps = from_circuit(h->s.circuit);
+        h->s.engine = build_engine(h->s.ps, h->s.ps.io_vars(), strategy, pairs);
+        *out = h;
+        return 0;
+    } catch (std::exception& e) { g_err = e.what(); return -1; }
+}
+void orc_sim_destroy(orc_sim* h) { delete h; }
+void orc_sim_set_threads(orc_sim* h, int t) { h->s.threads = t; h->s.engine.set_threads(t); }
+
+// sim.rs:161-168 + pybind.rs:177-180 order: free, planes, lines, radical, terms, branch_dim, pairs
+int orc_sim_stats(const orc_sim* h, uint64_t* out7) {
+    EngineStats s = h->s.engine.branches[0].stats();
+    out7[0] = s.free_vars; out7[1] = s.planes; out7[2] = s.lines; out7[3] = s.radical; out7[4] = s.terms; out7[5] = s.branch_dim;
+    out7[6] = h->s.engine.pairs;
+    return 0;
+}
+int orc_sim_amplitude(const orc_sim* h, const uint8_t* x, const uint8_t* y, double* out) {
+    try {
+        size_t n = h->s.ps.num_qubits;
+        C64 a = h->s.amplitude(bits_of(n, x), bits_of(n, y));
+        out[0] = a.re; out[1] = a.im; return 0;
+    } catch (std::exception& e) { g_err = e.what(); return -1; }
+}
+int orc_sim_marginal(orc_sim* h, const uint8_t* x, const uint32_t* qubits, const uint8_t* outcomes, size_t k, double* out) {
+    try {
+        size_t n = h->s.ps.num_qubits;
+        std::vector<std::pair<size_t, bool>> meas;
+        for (size_t i = 0; i < k; ++i) meas.push_back({qubits[i], outcomes[i] != 0});
+        *out = h->s.marginal(bits_of(n, x), meas);
+        return 0;
+    } catch (std::exception& e) { g_err = e.what(); return -1; }
+}
+
+// Per-branch dump on the first branch engine for <y|C|x>: for each t-bit branch
+// vector s (t <= 64, bit k = term k) writes zero flag, r = 2 log2|G|, k8 =
+// phase8 and the total per-branch phase k8 + 6|s| + 4|neg \ s| mod 8, where
+// the branch weight prod_k a_{s_k}(theta_k) (engine.rs:661-667) is written as
+// e^{i sum theta/2} (-i)^{|s|} (-1)^{|neg \ s|} prod |cos| prod sin with neg =
+// {k: theta_k > pi}.  Returns 1 if the boundary is contradictory.
+int orc_branch_dump(const orc_sim* h, const uint8_t* x, const uint8_t* y, size_t count, const uint64_t* s_in,
+                    uint8_t* zero, uint16_t* r, uint8_t* k8, uint8_t* ktot) {
+    try {
+        const PathSum& ps = h->s.ps;
+        const Engine& E = h->s.engine.branches[0];
+        std::vector<std::pair<size_t, bool>> bd;
+        size_t n = ps.num_qubits;
+        if (!ps.boundary(bits_of(n, x), bits_of(n, y), bd)) return 1;
+        std::vector<bool> vals; E.values_of(bd, vals);
+        Phase ph; C64 cs; UVec u, u0; BitVec s0;
+        E.prepare(vals, ph, cs, u, s0, &u0);
+        if (E.zero) { g_err = "engine is identically zero"; return -1; }
+        if (E.terms.size() > 64) { g_err = "t > 64"; return -1; }
+        for (size_t i = 0; i < count; ++i) {
+            BitVec s(E.terms.size());
+            for (size_t k = 0; k < E.terms.size(); ++k) s.set(k, (s_in[i] >> k) & 1);
+            E.branch(u0, s, zero[i], r[i], k8[i]);
+            // total phase of the branch term: weight(s) = e^{i sum th/2} prod_{k in s}(-i sin) prod_{k not in s} cos
+            uint32_t ns = 0, nn = 0;
+            for (size_t k = 0; k < E.terms.size(); ++k) {
+                bool bit = (s_in[i] >> k) & 1;
+                if (bit) ns++;
+                else if (E.term_theta[k] > PH_PI) nn++;
+            }
+            ktot[i] = zero[i] ? 0 : (uint8_t)((k8[i] + 6 * ns + 4 * nn) & 7);
+        }
+        return 0;
+    } catch (std::exception& e) { g_err = e.what(); return -1; }
+}
+
+// Brute-force Gauss sum of one branch, independent of the normal form: the
+// sum over all 2^m free assignments z of i^{q_free(z)} (-1)^{shift(s).z} with
+// shift(s) = B_{free,F} x_F + sum_{k in s} e_{term var k} (engine.rs:8-31,
+// normal_form.rs:522-543 pattern).  m <= 22.
+int orc_branch_brute(const orc_sim* h, const uint8_t* x, const uint8_t* y, size_t count, const uint64_t* s_in, double* out) {
+    try {
+        const PathSum& ps = h->s.ps;
+        const Engine& E = h->s.engine.branches[0];
+        std::vector<std::pair<size_t, bool>> bd;
+        size_t n = ps.num_qubits;
+        if (!ps.boundary(bits_of(n, x), bits_of(n, y), bd)) return 1;
+        std::vector<bool> vals; E.values_of(bd, vals);
+        size_t m = E.free_.size();
+        if (m > 22) { g_err = "brute branch: too many free variables"; return -1; }
+        std::vector<size_t> fidx(ps.num_vars, NONE);
+        for (size_t i = 0; i < m; ++i) fidx[E.free_[i]] = i;
+        uint64_t base = 0;
+        for (size_t i = 0; i < E.fixed.size(); ++i) if (vals[i])
+            for (size_t j : ps.b.rows[E.fixed[i]].ones()) if (fidx[j] != NONE) base ^= 1ull << fidx[j];
+        std::vector<uint64_t> adj(m, 0);
+        std::vector<uint8_t> v(m);
+        for (size_t i = 0; i < m; ++i) { v[i] = ps.v[E.free_[i]]; for (size_t j = 0; j < m; ++j) if (E.b_ff.get(i, j)) adj[i] |= 1ull << j; }
+        for (size_t q = 0; q < count; ++q) {
+            uint64_t shift = base;
+            for (size_t k = 0; k < E.term_vars.size(); ++k) if ((s_in[q] >> k) & 1) shift ^= 1ull << E.term_vars[k];
+            C64 tot(0, 0);
+            for (uint64_t z = 0; z < (1ull << m); ++z) {
+                uint32_t qv = 2u * (uint32_t)(__builtin_popcountll(shift & z) & 1);
+                for (size_t j = 0; j < m; ++j) if ((z >> j) & 1) qv += v[j] + 2u * (uint32_t)(__builtin_popcountll(adj[j] & z & ((1ull << j) - 1)) & 1);
+                tot = tot + ROOT8[2 * (qv & 3)];
+            }
+            out[2 * q] = tot.re; out[2 * q + 1] = tot.im;
+        }
+        return 0;
+    } catch (std::exception& e) { g_err = e.what(); return -1; }
+}
+
+// Enumerates the surviving affine branch subspace of <y|C|x> in the reference's
+// Gray order of the first 2^min(d,14) chunk: writes up to cap t-bit vectors s.
+long orc_surviving_branches(const orc_sim* h, const uint8_t* x, const uint8_t* y, uint64_t* s_out, long cap) {
+    try {
+        const PathSum& ps = h->s.ps;
+        const Engine& E = h->s.engine.branches[0];
+        std::vector<std::pair<size_t, bool>> bd;
+        size_t n = ps.num_qubits;
+        if (!ps.boundary(bits_of(n, x), bits_of(n, y), bd)) return 0;
+        std::vector<bool> vals; E.values_of(bd, vals);
+        Phase ph; C64 cs; UVec u; BitVec s0;
+        if (!E.prepare(vals, ph, cs, u, s0)) return 0;
+        size_t d = E.elim.kernel.size();
+        long cnt = 0;
+        uint64_t total = d >= 63 ? ~0ull : (1ull << d);
+        for (uint64_t i = 0; i < total && cnt < cap; ++i) {
+            uint64_t gcode = i ^ (i >> 1);
+            BitVec s = s0;
+            for (size_t j = 0; j < d; ++j) if ((gcode >> j) & 1) s.xor_assign(E.elim.kernel[j]);
+            uint64_t w = 0; for (size_t k = 0; k < s.len; ++k) if (s.get(k)) w |= 1ull << k;
+            s_out[cnt++] = w;
+        }
+        return cnt;
+    } catch (std::exception& e) { g_err = e.what(); return -1; }
+}
+
+// Timed batch of amplitudes with the engine's eval only (prepare + sum_subspace),
+// OpenMP over Gray chunks with `threads` workers.  xs/ys are n-byte rows.
+int orc_sim_amplitudes_timed(orc_sim* h, size_t nq, const uint8_t* xs, const uint8_t* ys, int threads, double* out, double* seconds) {
+    try {
+        orc_sim_set_threads(h, threads);
+        size_t n = h->s.ps.num_qubits;
+        auto t0 = std::chrono::steady_clock::now();
+        for (size_t i = 0; i < nq; ++i) {
+            C64 a = h->s.amplitude(bits_of(n, xs + i * n), bits_of(n, ys + i * n));
+            out[2 * i] = a.re; out[2 * i + 1] = a.im;
+        }
+        *seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+        return 0;
+    } catch (std::exception& e) { g_err = e.what(); return -1; }
+}
+
+// Sum over one explicit range of Gray chunks [hi0, hi1) of the first engine:
+// the bounded CPU-baseline sample (same per-chunk work as sum_subspace).
+int orc_sim_chunk_range_timed(orc_sim* h, const uint8_t* x, const uint8_t* y, uint64_t hi0, uint64_t hi1, int threads,
+                              double* out, double* seconds, uint64_t* branches) {
+    try {
+        const PathSum& ps = h->s.ps;
+        const Engine& E = h->s.engine.branches[0];
+        std::vector<std::pair<size_t, bool>> bd;
+        size_t n = ps.num_qubits;
+        out[0] = out[1] = 0; *branches = 0; *seconds = 0;
+        if (!ps.boundary(bits_of(n, x), bits_of(n, y), bd)) return 1;
+        std::vector<bool> vals; E.values_of(bd, vals);
+        Phase ph; C64 cs; UVec u; BitVec s0;
+        if (!E.prepare(vals, ph, cs, u, s0)) return 1;
+        size_t d = E.elim.kernel.size();
+        size_t dl = std::min(d, LOW_BITS);
+        uint64_t nhi = 1ull << (d - dl);
+        if (hi1 > nhi) hi1 = nhi;
+        auto t0 = std::chrono::steady_clock::now();
+        double sr = 0, si = 0;
+        #pragma omp parallel for schedule(dynamic, 1) reduction(+:sr, si) num_threads(threads)
+        for (long hi = (long)hi0; hi < (long)hi1; ++hi) {
+            UVec uu = u; BitVec sv = s0; C64 c = cs;
+            if (hi != 0) {
+                for (size_t j = 0; j < d - dl; ++j) if (((uint64_t)hi >> j) & 1) { uu.xor_assign(E.elim.kernel_u[dl + j]); sv.xor_assign(E.elim.kernel[dl + j]); }
+                c = E.weight(sv);
+            }
+            C64 r = E.gray(uu, sv, c, dl);
+            sr += r.re; si += r.im;
+        }
+        *seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+        out[0] = sr; out[1] = si;
+        *branches = (hi1 > hi0 ? hi1 - hi0 : 0) << dl;
+        return 0;
+    } catch (std::exception& e) { g_err = e.what(); return -1; }
+}
+
+int orc_max_threads() { return omp_get_max_threads(); }
+
+}  // extern "C"

@@ -1,0 +1,60 @@
+"""Build libpolyq_amd.so in-tree: host C++ with g++, HIP kernels with hipcc for gfx950.
+
+Usage: python build_lib.py [--jobs N]
+The library lands in polyq_amd/libpolyq_amd.so next to the Python bindings.
+"""
+import os
+import subprocess
+import sys
+from concurrent.futures import ThreadPoolExecutor
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+CSRC = os.path.join(HERE, "csrc")
+OBJ = os.path.join(HERE, "_build")
+OUT = os.path.join(HERE, "polyq_amd", "libpolyq_amd.so")
+
+HOST_SOURCES = ["pq_compile.cpp", "pq_engine.cpp", "pq_z8.cpp", "pq_api.cpp"]
+HIP_SOURCES = ["pq_kernels.hip"]
+
+# x86-64-v3: the library built here also runs on the GPU host's CPU.
+HOST_FLAGS = ["-O3", "-std=c++17", "-fPIC", "-march=x86-64-v3", "-Wall", "-Wno-unused-function", "-pthread"]
+# Packed FP32 instructions stay disabled; no FMA contraction so the two-sum
+# accumulators and the reference-style FP64 products round as written.
+HIP_FLAGS = ["--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-ffp-contract=off",
+             "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops", "-Wno-unused-command-line-argument"]
+
+
+def _run(cmd):
+    r = subprocess.run(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True)
+    if r.returncode != 0:
+        raise RuntimeError("build step failed: %s\n%s" % (" ".join(cmd), r.stdout))
+    return r.stdout
+
+
+def build(jobs=8, verbose=False):
+    os.makedirs(OBJ, exist_ok=True)
+    cmds = []
+    objs = []
+    for s in HOST_SOURCES:
+        o = os.path.join(OBJ, s + ".o")
+        objs.append(o)
+        cmds.append(["g++"] + HOST_FLAGS + ["-c", os.path.join(CSRC, s), "-o", o])
+    for s in HIP_SOURCES:
+        o = os.path.join(OBJ, s + ".o")
+        objs.append(o)
+        cmds.append(["hipcc"] + HIP_FLAGS + ["-c", os.path.join(CSRC, s), "-o", o])
+    with ThreadPoolExecutor(max_workers=max(1, jobs)) as ex:
+        for out in ex.map(_run, cmds):
+            if verbose and out.strip():
+                print(out)
+    tmp = OUT + ".tmp"
+    _run(["hipcc", "--offload-arch=gfx950", "-shared", "-fPIC", "-pthread", "-o", tmp] + objs)
+    os.replace(tmp, OUT)
+    return OUT
+
+
+if __name__ == "__main__":
+    j = 8
+    if "--jobs" in sys.argv:
+        j = int(sys.argv[sys.argv.index("--jobs") + 1])
+    print(build(jobs=j, verbose=True))

@@ -1,0 +1,401 @@
+// Simulator / RankEngine mirror (src/sim.rs, src/rank.rs) with the GPU branch
+// evaluator as the strategy backend, and the C ABI declared in
+// include/polyq_amd.h.
+#include "pq_host.hpp"
+#include "pq_device.hpp"
+#include "pq_api_internal.hpp"
+#include "../../include/polyq_amd.h"
+#include <mutex>
+#include <map>
+#include <cstring>
+#include <algorithm>
+
+namespace pq {
+
+thread_local std::string g_err;
+thread_local double g_timing[5] = {0, 0, 0, 0, 0};
+
+std::shared_ptr<RankEng> build_rank(const PathSum& ps, const std::vector<uint32_t>& fixed, int strategy, uint32_t pairs) {
+    auto R = std::make_shared<RankEng>();
+    auto add = [&](std::shared_ptr<HostEngine> he) {
+        auto z = std::make_shared<Z8Image>(build_z8_image(*he));
+        R->host.push_back(he);
+        R->z8.push_back(z);
+    };
+    uint32_t nt = 0;
+    auto cands = pair_candidates(ps, fixed, nt);
+    uint32_t p = 0;
+    if (strategy == PQ_AUTO) {
+        // rank.rs:115-123: base engine, cost model, rebuild with p pairs if p > 0
+        auto base = build_engine(ps, fixed);
+        p = best_pairs(base->stats(), (uint32_t)cands.size(), nt);
+        if (p == 0) { add(base); R->pairs = 0; }
+    } else if (strategy == PQ_CACHED) {
+        p = 0;
+        add(build_engine(ps, fixed));
+    } else if (strategy == PQ_PAIRS) {
+        p = std::min<uint32_t>(std::min<uint32_t>(pairs, (uint32_t)cands.size()), kMaxPairs);
+        if (p == 0) add(build_engine(ps, fixed));
+    } else {
+        throw std::invalid_argument("strategy must be PQ_AUTO, PQ_CACHED or PQ_PAIRS");
+    }
+    if (R->host.empty()) {
+        // rank.rs:88-103: one engine per substitution pattern of the chosen pairs
+        for (uint32_t pattern = 0; pattern < (1u << p); ++pattern) {
+            PathSum b = ps;
+            for (uint32_t i = 0; i < p; ++i) b.substitute(cands[i].first, cands[i].second, (pattern >> i) & 1);
+            add(build_engine(b, fixed));
+        }
+        R->pairs = p;
+    }
+    return R;
+}
+
+// Append the device jobs of one (engine set, assignment) query; returns false
+// if every branch engine vanishes for it.
+void add_jobs(const RankEng& R, const std::vector<std::pair<uint32_t, uint8_t>>& asg, std::vector<Z8Job>& jobs, std::vector<size_t>& owner,
+              size_t qid, std::vector<Prepared>* keep) {
+    R.ensure_device();
+    for (size_t b = 0; b < R.host.size(); ++b) {
+        Prepared P;
+        if (!prepare_assignment(*R.host[b], asg, P)) continue;
+        Z8Job J;
+        J.eng = R.dev[b].get();
+        J.q = z8_query(*R.host[b], *R.z8[b], P);
+        J.scale = P.scale;
+        J.mid_bits = 0;
+        jobs.push_back(J);
+        owner.push_back(qid);
+        if (keep) keep->push_back(std::move(P));
+    }
+}
+
+void assign_mid_bits(std::vector<Z8Job>& jobs) {
+    uint64_t total = 0;
+    for (auto& J : jobs) total += 1ull << dev_z8(*J.eng).d;
+    for (auto& J : jobs) J.mid_bits = z8_auto_mid_bits(dev_z8(*J.eng).d, total);
+}
+
+void run_jobs(std::vector<Z8Job>& jobs, const std::vector<size_t>& owner, size_t nq, bool accurate, std::vector<std::complex<double>>& out) {
+    out.assign(nq, {0, 0});
+    if (jobs.empty()) { for (double& x : g_timing) x = 0; return; }
+    assign_mid_bits(jobs);
+    std::vector<std::complex<double>> r;
+    Z8Timing T;
+    dev_z8_batch(jobs, r, &T, accurate);
+    for (size_t i = 0; i < jobs.size(); ++i) out[owner[i]] += r[i];
+    g_timing[0] = T.kernel_ms; g_timing[1] = T.total_ms; g_timing[2] = (double)T.branches; g_timing[3] = (double)T.items; g_timing[4] = T.grid;
+}
+
+std::complex<double> plain_eval(const RankEng& R, const std::vector<std::pair<uint32_t, uint8_t>>& asg) {
+    std::complex<double> tot(0, 0);
+    double kms = 0, kall = 0;
+    uint64_t br = 0;
+    R.ensure_device();
+    for (size_t b = 0; b < R.host.size(); ++b) {
+        Prepared P;
+        if (!prepare_assignment(*R.host[b], asg, P)) continue;
+        std::complex<double> s = dev_plain_sum(*R.dev[b], P, &kms);
+        const HostEngine& E = *R.host[b];
+        // engine.rs:560: phase * total * 2^{log2mag - h/2}
+        tot += phase_complex(P.phase) * s * std::pow(2.0, E.log2mag - (double)E.nh / 2.0);
+        kall += kms;
+        br += 1ull << E.d;
+    }
+    g_timing[0] = kall; g_timing[1] = kall; g_timing[2] = (double)br; g_timing[3] = 0; g_timing[4] = 0;
+    return tot;
+}
+
+bool Sim::assignment(uint64_t x, uint64_t y, std::vector<std::pair<uint32_t, uint8_t>>& asg) const {
+    std::vector<uint8_t> xb(ps.nq), yb(ps.nq);
+    for (uint32_t q = 0; q < ps.nq; ++q) { xb[q] = (x >> q) & 1; yb[q] = (y >> q) & 1; }
+    return ps.boundary(xb.data(), yb.data(), asg);
+}
+
+const RankEng& Sim::marginal_engine(const std::vector<uint32_t>& qubits) {
+    std::lock_guard<std::mutex> lk(mu);
+    if (!sw) sw = std::make_unique<Sandwich>(compile_sandwich(circ));
+    std::vector<uint32_t> key(qubits);
+    std::sort(key.begin(), key.end());
+    key.erase(std::unique(key.begin(), key.end()), key.end());
+    auto it = marg.find(key);
+    if (it != marg.end()) return *it->second;
+    // Sandwich::fixed_for (sim.rs:95-103)
+    std::vector<uint32_t> f = sw->ps.io_vars();
+    for (uint32_t q : qubits) if (std::find(f.begin(), f.end(), sw->junction[q]) == f.end()) f.push_back(sw->junction[q]);
+    auto R = build_rank(sw->ps, f, strategy, pairs_opt);
+    marg[key] = R;
+    return *R;
+}
+
+bool Sim::marginal_assignment(uint64_t x, const uint32_t* qubits, const uint8_t* outcomes, size_t k, std::vector<std::pair<uint32_t, uint8_t>>& a) {
+    // Sandwich::assignment (sim.rs:106-124)
+    a.clear();
+    auto put = [&](uint32_t var, uint8_t val) {
+        for (auto& e : a) if (e.first == var) return e.second == val;
+        a.push_back({var, val});
+        return true;
+    };
+    const PathSum& s = sw->ps;
+    for (uint32_t q = 0; q < s.nq; ++q) if (!put(q, (x >> q) & 1)) return false;
+    for (uint32_t q = 0; q < s.nq; ++q) if (!put(s.out_var[q], (uint8_t)(((x >> q) & 1) ^ s.out_neg[q]))) return false;
+    for (size_t i = 0; i < k; ++i) if (!put(sw->junction[qubits[i]], (uint8_t)((outcomes[i] ? 1 : 0) ^ sw->jneg[qubits[i]]))) return false;
+    return true;
+}
+
+}  // namespace pq
+
+using namespace pq;
+
+struct pq_sim { Sim s; };
+struct pq_graph { std::shared_ptr<Z8Graph> g; std::vector<size_t> owner; size_t nq = 0; std::vector<std::shared_ptr<RankEng>> keep; };
+
+#define PQ_TRY(...)                                                                                   \
+    try { __VA_ARGS__ } catch (std::exception & e) { g_err = e.what(); return -1; } catch (...) { g_err = "unknown error"; return -1; }
+
+static bool qubits_ok(const pq_sim* h, uint64_t x) { return h->s.ps.nq >= 64 || (x >> h->s.ps.nq) == 0; }
+
+extern "C" {
+
+const char* pq_last_error(void) { return g_err.c_str(); }
+
+int pq_device_info(char* buf, size_t len) {
+    PQ_TRY({
+        if (dev_count() == 0) { g_err = "no HIP device"; return -1; }
+        std::string s = dev_name();
+        if (buf && len) { std::strncpy(buf, s.c_str(), len - 1); buf[len - 1] = 0; }
+        return 0;
+    })
+}
+
+int pq_sim_create(uint32_t nq, size_t ng, const uint8_t* ops, const uint32_t* q0, const uint32_t* q1, const uint32_t* ph, int strategy,
+                  uint32_t pairs, pq_sim** out) {
+    PQ_TRY({
+        if (!out) throw std::invalid_argument("null output handle");
+        if (nq > 64) throw std::invalid_argument("at most 64 qubits (basis indices are u64)");
+        auto h = std::make_unique<pq_sim>();
+        h->s.circ.n = nq;
+        for (size_t i = 0; i < ng; ++i) h->s.circ.add(ops[i], q0[i], q1 ? q1[i] : 0, ph ? ph[i] : 0);
+        if (h->s.circ.n > 64) throw std::invalid_argument("gate on a qubit beyond 63");
+        h->s.strategy = strategy; h->s.pairs_opt = pairs;
+        h->s.ps = compile(h->s.circ);
+        h->s.eng = build_rank(h->s.ps, h->s.ps.io_vars(), strategy, pairs);
+        *out = h.release();
+        return 0;
+    })
+}
+
+void pq_sim_destroy(pq_sim* h) { delete h; }
+
+uint32_t pq_sim_num_qubits(const pq_sim* h) { return h ? h->s.ps.nq : 0; }
+
+int pq_sim_stats(const pq_sim* h, uint64_t* o) {
+    PQ_TRY({
+        EngineStats s = h->s.eng->host[0]->stats();
+        o[0] = s.free_vars; o[1] = s.planes; o[2] = s.lines; o[3] = s.radical; o[4] = s.terms; o[5] = s.branch_dim; o[6] = h->s.eng->pairs;
+        return 0;
+    })
+}
+
+int pq_sim_amplitudes(pq_sim* h, size_t n, const uint64_t* xs, const uint64_t* ys, int form, double* out) {
+    PQ_TRY({
+        std::vector<std::complex<double>> res(n, {0, 0});
+        std::vector<std::pair<uint32_t, uint8_t>> asg;
+        if (form == PQ_FORM_PLAIN_FP64) {
+            for (size_t i = 0; i < n; ++i) {
+                if (!qubits_ok(h, xs[i]) || !qubits_ok(h, ys[i])) throw std::invalid_argument("basis index has bits beyond the qubit count");
+                if (!h->s.assignment(xs[i], ys[i], asg)) continue;
+                res[i] = plain_eval(*h->s.eng, asg);
+            }
+        } else if (form == PQ_FORM_Z8 || form == PQ_FORM_Z8_ACCURATE) {
+            std::vector<Z8Job> jobs;
+            std::vector<size_t> owner;
+            for (size_t i = 0; i < n; ++i) {
+                if (!qubits_ok(h, xs[i]) || !qubits_ok(h, ys[i])) throw std::invalid_argument("basis index has bits beyond the qubit count");
+                if (!h->s.assignment(xs[i], ys[i], asg)) continue;   // contradictory boundary: exactly 0
+                add_jobs(*h->s.eng, asg, jobs, owner, i, nullptr);
+            }
+            run_jobs(jobs, owner, n, form == PQ_FORM_Z8_ACCURATE, res);
+        } else throw std::invalid_argument("unknown form");
+        for (size_t i = 0; i < n; ++i) { out[2 * i] = res[i].real(); out[2 * i + 1] = res[i].imag(); }
+        return 0;
+    })
+}
+
+int pq_sim_amplitude_index(pq_sim* h, uint64_t x, uint64_t y, int form, double* out) { return pq_sim_amplitudes(h, 1, &x, &y, form, out); }
+
+int pq_sim_amplitude(pq_sim* h, const uint8_t* input, const uint8_t* output, int form, double* out) {
+    uint64_t x = 0, y = 0;
+    for (uint32_t q = 0; q < h->s.ps.nq; ++q) { x |= (uint64_t)(input[q] ? 1 : 0) << q; y |= (uint64_t)(output[q] ? 1 : 0) << q; }
+    return pq_sim_amplitudes(h, 1, &x, &y, form, out);
+}
+
+int pq_sim_probability(pq_sim* h, uint64_t x, uint64_t y, int form, double* out) {
+    double a[2];
+    int r = pq_sim_amplitudes(h, 1, &x, &y, form, a);
+    if (r == 0) *out = a[0] * a[0] + a[1] * a[1];
+    return r;
+}
+
+int pq_sim_marginal_probability(pq_sim* h, uint64_t x, const uint32_t* qubits, const uint8_t* outcomes, size_t k, int form, double* out) {
+    PQ_TRY({
+        for (size_t i = 0; i < k; ++i) if (qubits[i] >= h->s.ps.nq) throw std::invalid_argument("measured qubit out of range");
+        std::vector<uint32_t> qs(qubits, qubits + k);
+        const RankEng& R = h->s.marginal_engine(qs);
+        std::vector<std::pair<uint32_t, uint8_t>> a;
+        if (!h->s.marginal_assignment(x, qubits, outcomes, k, a)) { *out = 0; return 0; }
+        std::complex<double> v(0, 0);
+        if (form == PQ_FORM_PLAIN_FP64) v = plain_eval(R, a);
+        else {
+            std::vector<Z8Job> jobs; std::vector<size_t> owner;
+            add_jobs(R, a, jobs, owner, 0, nullptr);
+            std::vector<std::complex<double>> res;
+            run_jobs(jobs, owner, 1, form == PQ_FORM_Z8_ACCURATE, res);
+            v = res[0];
+        }
+        *out = std::max(v.real(), 0.0);   // sim.rs:230
+        return 0;
+    })
+}
+
+int pq_batch_amplitudes(size_t n, pq_sim* const* sims, const uint64_t* xs, const uint64_t* ys, int form, double* out) {
+    PQ_TRY({
+        if (form != PQ_FORM_Z8 && form != PQ_FORM_Z8_ACCURATE) throw std::invalid_argument("batches use the Z8 forms");
+        std::vector<Z8Job> jobs; std::vector<size_t> owner;
+        std::vector<std::pair<uint32_t, uint8_t>> asg;
+        for (size_t i = 0; i < n; ++i) {
+            if (!qubits_ok(sims[i], xs[i]) || !qubits_ok(sims[i], ys[i])) throw std::invalid_argument("basis index has bits beyond the qubit count");
+            if (!sims[i]->s.assignment(xs[i], ys[i], asg)) continue;
+            add_jobs(*sims[i]->s.eng, asg, jobs, owner, i, nullptr);
+        }
+        std::vector<std::complex<double>> res;
+        run_jobs(jobs, owner, n, form == PQ_FORM_Z8_ACCURATE, res);
+        for (size_t i = 0; i < n; ++i) { out[2 * i] = res[i].real(); out[2 * i + 1] = res[i].imag(); }
+        return 0;
+    })
+}
+
+int pq_last_timing(double* o) { for (int i = 0; i < 5; ++i) o[i] = g_timing[i]; return 0; }
+
+int pq_graph_create(size_t n, pq_sim* const* sims, const uint64_t* xs, const uint64_t* ys, int form, pq_graph** out) {
+    PQ_TRY({
+        if (form != PQ_FORM_Z8 && form != PQ_FORM_Z8_ACCURATE) throw std::invalid_argument("graphs use the Z8 forms");
+        auto g = std::make_unique<pq_graph>();
+        std::vector<Z8Job> jobs;
+        std::vector<std::pair<uint32_t, uint8_t>> asg;
+        for (size_t i = 0; i < n; ++i) {
+            if (!sims[i]->s.assignment(xs[i], ys[i], asg)) continue;
+            add_jobs(*sims[i]->s.eng, asg, jobs, g->owner, i, nullptr);
+            g->keep.push_back(sims[i]->s.eng);
+        }
+        if (jobs.empty()) throw std::invalid_argument("every query of the graph is identically zero");
+        assign_mid_bits(jobs);
+        g->nq = n;
+        g->g = dev_z8_graph(jobs, form == PQ_FORM_Z8_ACCURATE);
+        *out = g.release();
+        return 0;
+    })
+}
+
+int pq_graph_run(pq_graph* g, double* out, double* total_ms) {
+    PQ_TRY({
+        std::vector<std::complex<double>> r;
+        Z8Timing T;
+        dev_z8_graph_run(*g->g, r, &T);
+        std::vector<std::complex<double>> acc(g->nq, {0, 0});
+        for (size_t i = 0; i < r.size(); ++i) acc[g->owner[i]] += r[i];
+        for (size_t i = 0; i < g->nq; ++i) { out[2 * i] = acc[i].real(); out[2 * i + 1] = acc[i].imag(); }
+        if (total_ms) *total_ms = T.total_ms;
+        return 0;
+    })
+}
+
+void pq_graph_destroy(pq_graph* g) { delete g; }
+
+int pq_sim_branch_dump(pq_sim* h, uint64_t x, uint64_t y, size_t n, const uint64_t* s, uint8_t* zero, uint8_t* k8, uint8_t* ktot, uint32_t* r) {
+    PQ_TRY({
+        std::vector<std::pair<uint32_t, uint8_t>> asg;
+        if (!h->s.assignment(x, y, asg)) return 1;
+        const RankEng& R = *h->s.eng;
+        const HostEngine& E = *R.host[0];
+        if (E.zero) throw std::invalid_argument("path sum vanishes identically");
+        Prepared P;
+        prepare_assignment(E, asg, P);   // u0 is filled even when K s = kappa is inconsistent
+        if (P.u0.empty()) throw std::logic_error("prepare did not produce the boundary shift");
+        if (r) *r = 2 * E.L.g + 2 * E.L.neven + E.L.nodd;
+        R.ensure_device();
+        dev_dump_s(*R.dev[0], P, s, n, zero, k8, ktot);
+        return 0;
+    })
+}
+
+int pq_sim_walk_dump(pq_sim* h, uint64_t x, uint64_t y, uint32_t mid_bits, size_t cap, uint8_t* k, float* mag, uint64_t* s, size_t* n) {
+    PQ_TRY({
+        std::vector<std::pair<uint32_t, uint8_t>> asg;
+        *n = 0;
+        if (!h->s.assignment(x, y, asg)) return 1;
+        const RankEng& R = *h->s.eng;
+        const HostEngine& E = *R.host[0];
+        const Z8Image& Z = *R.z8[0];
+        Prepared P;
+        if (!prepare_assignment(E, asg, P)) return 1;
+        Z8QueryHost q = z8_query(E, Z, P);
+        const uint32_t nout = Z.nslots - 8, M = std::min(mid_bits, nout), H = nout - M;
+        const size_t total = (size_t)1 << (8 + M + H);
+        if (total > cap) throw std::invalid_argument("walk dump buffer too small");
+        R.ensure_device();
+        dev_dump_walk(*R.dev[0], q, M, k, mag);
+        for (size_t idx = 0; idx < total; ++idx) {
+            const uint64_t i = idx & 255, o = (idx >> 8) & ((1ull << M) - 1), hi = idx >> (8 + M);
+            const uint64_t gi = i ^ (i >> 1), go = o ^ (o >> 1);
+            uint64_t sv = P.s0;
+            for (uint32_t p = 0; p < Z.nslots; ++p) {
+                bool bit = p < 8 ? ((gi >> p) & 1) : (p < 8 + M ? ((go >> (p - 8)) & 1) : ((hi >> (p - 8 - M)) & 1));
+                if (bit && Z.slot_coord[p] >= 0) sv ^= Z.slot_n[p];
+            }
+            s[idx] = sv;
+        }
+        *n = total;
+        return 0;
+    })
+}
+
+int pq_sim_selfcheck(pq_sim* h, uint64_t x, uint64_t y, uint32_t trials, uint64_t seed, uint64_t* bad) {
+    PQ_TRY({
+        std::vector<std::pair<uint32_t, uint8_t>> asg;
+        *bad = 0;
+        if (!h->s.assignment(x, y, asg)) return 1;
+        for (size_t b = 0; b < h->s.eng->host.size(); ++b) {
+            const HostEngine& E = *h->s.eng->host[b];
+            Prepared P;
+            if (!prepare_assignment(E, asg, P)) continue;
+            const uint32_t W = E.L.words();
+            uint64_t st = seed + 0x9E3779B97F4A7C15ull * (b + 1);
+            for (uint32_t tr = 0; tr < trials; ++tr) {
+                st += 0x9E3779B97F4A7C15ull;
+                uint64_t z = st; z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull; z = (z ^ (z >> 27)) * 0x94D049BB133111EBull; z ^= z >> 31;
+                const uint64_t c = E.d >= 64 ? z : (z & ((1ull << E.d) - 1));
+                std::vector<uint64_t> u(P.u_start);
+                uint64_t sv = P.s0;
+                uint32_t kq = P.K0;
+                for (uint32_t j = 0; j < E.d; ++j) if ((c >> j) & 1) {
+                    for (uint32_t w = 0; w < W; ++w) u[w] ^= E.kernel_u[(size_t)j * W + w];
+                    sv ^= E.kernel[j];
+                    kq += P.ell[j] + 4 * ((uint32_t)__builtin_popcountll(E.Q[j] & c & ((1ull << j) - 1)) & 1);
+                }
+                if (!survives_row(E, u.data()) || total_phase(E, u.data(), sv) != (kq & 7)) (*bad)++;
+            }
+        }
+        return 0;
+    })
+}
+
+int pq_sim_z8_info(pq_sim* h, uint32_t* o) {
+    PQ_TRY({
+        const Z8Image& Z = *h->s.eng->z8[0];
+        o[0] = Z.d; o[1] = Z.nin; o[2] = Z.nslots; o[3] = Z.npin; o[4] = h->s.eng->host[0]->rk;
+        return 0;
+    })
+}
+
+}  // extern "C"

@@ -1,0 +1,53 @@
+// Internal host structures shared by the C ABI and the serving loop.
+#pragma once
+#include "pq_host.hpp"
+#include "pq_device.hpp"
+#include <map>
+#include <mutex>
+
+namespace pq {
+
+// RankEngine (src/rank.rs:47-51): 2^p branch engines, each with its device image.
+struct RankEng {
+    std::vector<std::shared_ptr<HostEngine>> host;
+    std::vector<std::shared_ptr<Z8Image>> z8;
+    // Device images are uploaded on first device use, so host-only callers
+    // (engine statistics, self checks) never need a GPU.
+    mutable std::vector<std::shared_ptr<DevEngine>> dev;
+    mutable std::mutex dev_mu;
+    uint32_t pairs = 0;
+    void ensure_device() const {
+        std::lock_guard<std::mutex> lk(dev_mu);
+        for (size_t i = dev.size(); i < host.size(); ++i) dev.push_back(dev_engine_upload(host[i], z8[i]));
+    }
+    size_t device_bytes() const { size_t b = 0; for (auto& d : dev) b += dev_engine_bytes(*d); return b; }
+};
+
+std::shared_ptr<RankEng> build_rank(const PathSum& ps, const std::vector<uint32_t>& fixed, int strategy, uint32_t pairs);
+
+// Simulator (src/sim.rs:128-135)
+struct Sim {
+    Circuit circ;
+    PathSum ps;
+    std::shared_ptr<RankEng> eng;
+    int strategy = 0;
+    uint32_t pairs_opt = 0;
+    std::mutex mu;
+    std::unique_ptr<Sandwich> sw;
+    std::map<std::vector<uint32_t>, std::shared_ptr<RankEng>> marg;
+
+    bool assignment(uint64_t x, uint64_t y, std::vector<std::pair<uint32_t, uint8_t>>& asg) const;
+    const RankEng& marginal_engine(const std::vector<uint32_t>& qubits);
+    bool marginal_assignment(uint64_t x, const uint32_t* qubits, const uint8_t* outcomes, size_t k, std::vector<std::pair<uint32_t, uint8_t>>& a);
+};
+
+void add_jobs(const RankEng& R, const std::vector<std::pair<uint32_t, uint8_t>>& asg, std::vector<Z8Job>& jobs, std::vector<size_t>& owner,
+              size_t qid, std::vector<Prepared>* keep);
+void assign_mid_bits(std::vector<Z8Job>& jobs);
+void run_jobs(std::vector<Z8Job>& jobs, const std::vector<size_t>& owner, size_t nq, bool accurate, std::vector<std::complex<double>>& out);
+std::complex<double> plain_eval(const RankEng& R, const std::vector<std::pair<uint32_t, uint8_t>>& asg);
+
+extern thread_local std::string g_err;
+extern thread_local double g_timing[5];
+
+}  // namespace pq

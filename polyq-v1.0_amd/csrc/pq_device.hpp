@@ -1,0 +1,66 @@
+// Device-side engine images and launches (implemented in pq_kernels.hip).
+// No HIP types appear here, so host C++ compiled by g++ can include it.
+#pragma once
+#include "pq_host.hpp"
+#include <complex>
+#include <memory>
+#include <vector>
+
+namespace pq {
+
+struct DevEngine;   // engine resident in HBM: plain FP64 image + Z8 walk image
+
+// Upload both device images of one engine to the current device.
+std::shared_ptr<DevEngine> dev_engine_upload(std::shared_ptr<const HostEngine> E, std::shared_ptr<const Z8Image> Z);
+size_t dev_engine_bytes(const DevEngine& D);
+const HostEngine& dev_host(const DevEngine& D);
+const Z8Image& dev_z8(const DevEngine& D);
+
+// Plain form: a straightforward FP64 port of engine.rs:669-722 (sum_subspace +
+// gray), one query per launch, Gray chunks of 2^L branches per thread.
+// Returns the raw FP64 sum over the affine subspace (before the boundary phase
+// and the 2^{log2mag - h/2} scale).  kernel_ms receives the walk kernel time.
+std::complex<double> dev_plain_sum(const DevEngine& D, const Prepared& P, double* kernel_ms);
+
+// Batched Z8 walk: one persistent launch over all (query, lane-block) items.
+struct Z8Job {
+    const DevEngine* eng;
+    Z8QueryHost q;
+    std::complex<double> scale;   // applied in FP64 after the cross-item reduction
+    uint32_t mid_bits;            // outer Gray bits per lane (autotuned)
+};
+struct Z8Timing {
+    double kernel_ms = 0;         // walk kernel, HIP events on the launch stream
+    double total_ms = 0;          // copies + launches + reduction, events
+    uint64_t branches = 0;        // sum over jobs of 2^d (real coordinates)
+    uint64_t items = 0;
+    uint32_t grid = 0;
+};
+// Results are scale * (EV + omega * OD) per job, FP64.
+void dev_z8_batch(const std::vector<Z8Job>& jobs, std::vector<std::complex<double>>& out, Z8Timing* timing, bool accurate_fold);
+// Mid-bit heuristic: enough items to fill every CU several times, long
+// enough walks to amortise the lane seed.
+uint32_t z8_auto_mid_bits(uint32_t d, uint64_t batch_branches);
+
+// Replayable batch: the whole batch (descriptor upload, counter clear, walk,
+// reduction, result download) captured once into a hipGraph and relaunched.
+struct Z8Graph;
+std::shared_ptr<Z8Graph> dev_z8_graph(const std::vector<Z8Job>& jobs, bool accurate_fold);
+void dev_z8_graph_run(Z8Graph& G, std::vector<std::complex<double>>& out, Z8Timing* timing);
+
+// Per-branch dumps for parity tests.
+// (a) Gauss-sum outputs for explicit t-bit branch vectors s via the packed
+//     popcount form (normal_form.rs:472-493): zero flag, k8, and the total
+//     phase k8 + 6|s| + 4|neg \ s| mod 8.
+void dev_dump_s(const DevEngine& D, const Prepared& P, const uint64_t* s, size_t n, uint8_t* zero, uint8_t* k8, uint8_t* ktot);
+// (b) Every branch visited by the Z8 walk for one query with the given mid
+//     bits: the integer phase k(c) and the magnitude prod r, indexed by
+//     ((hi << M) + o) * 256 + i.  Phantom entries get k = 255.
+void dev_dump_walk(const DevEngine& D, const Z8QueryHost& q, uint32_t mid_bits, uint8_t* k, float* mag);
+
+// Device / runtime helpers
+int dev_count();
+std::string dev_name();
+void dev_sync();
+
+}  // namespace pq

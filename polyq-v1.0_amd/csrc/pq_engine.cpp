@@ -1,0 +1,413 @@
+// Engine construction on the host (once per circuit and fixed set) and the
+// per-query prologue.
+//
+// Reference: src/engine.rs:289-390 (Engine::new, build_branching, bind_values),
+// src/normal_form.rs:56-391 (Structure::new with the MinWeight rule, diagonal),
+// src/engine.rs:106-201 (Elim), src/engine.rs:569-610 (prepare).
+//
+// Beyond the reference, the builder derives the per-branch phase as a Z8
+// quadratic form over the kernel coordinates c of the surviving affine
+// subspace:
+//     k(c) = K0 + sum_j c_j l_j + 4 sum_{j<k} c_j c_k Q_jk   (mod 8)
+// where k(c) = phase8(u(c)) + 6|s(c)| + 4|neg \ s(c)| is the Gauss-sum phase
+// (normal_form.rs:479-493) plus the integer phase of the factorised branch
+// weight (alpha = e^{i th/2} cos(th/2), beta = -i e^{i th/2} sin(th/2)).
+// Q is query independent; l_j = lam_j + 4 delta_j with one query-dependent bit
+// delta_j per coordinate.  The device walks c and never touches the packed
+// shift rows.
+#include "pq_host.hpp"
+#include <algorithm>
+#include <cmath>
+#include <queue>
+#include <map>
+
+namespace pq {
+
+// --------------------------------------------------------- normal form ----
+// Symmetric LDL^T over F2 with 1x1 and 2x2 pivots.  Pivot choice follows the
+// reference's MinWeight rule (normal_form.rs:269-330): first the lightest
+// alive row with a diagonal bit, else the lightest alive row with a neighbour,
+// ties by row index.  A lazy min-heap with per-row stamps replaces the two
+// ordered sets: popping the smallest (weight, index) entry whose stamp is
+// current gives the same pivot sequence.
+NormalForm normal_form(const std::vector<Bits>& b, const Bits& parity) {
+    const uint32_t m = (uint32_t)b.size();
+    std::vector<Bits> gram(b), basis(m, Bits(m));
+    for (uint32_t i = 0; i < m; ++i) { gram[i].put(i, parity.get(i)); basis[i].set(i); }
+    Bits alive(m);
+    for (uint32_t i = 0; i < m; ++i) alive.set(i);
+
+    typedef std::pair<uint32_t, uint32_t> Key;  // (weight, row)
+    std::priority_queue<Key, std::vector<Key>, std::greater<Key>> hd, ho;  // diagonal / off-diagonal classes
+    std::vector<uint8_t> cls(m, 0);
+    std::vector<uint32_t> wt(m, 0);
+    auto reclass = [&](uint32_t a) {
+        if (!alive.get(a)) { cls[a] = 0; return; }
+        uint32_t w = gram[a].and_popc(alive);
+        if (gram[a].get(a)) { cls[a] = 1; wt[a] = w; hd.push({w, a}); }
+        else if (w > 0) { cls[a] = 2; wt[a] = w; ho.push({w, a}); }
+        else cls[a] = 0;
+    };
+    auto pop_current = [&](std::priority_queue<Key, std::vector<Key>, std::greater<Key>>& h, uint8_t want) -> long {
+        while (!h.empty()) {
+            Key k = h.top();
+            if (cls[k.second] == want && wt[k.second] == k.first && alive.get(k.second)) return (long)k.second;
+            h.pop();
+        }
+        return -1;
+    };
+    for (uint32_t a = 0; a < m; ++a) reclass(a);
+
+    NormalForm nf; nf.m = m;
+    for (;;) {
+        long a = pop_current(hd, 1);
+        if (a >= 0) {
+            // 1x1 pivot: A'_bc = A_bc + A_ba A_ac, p_b += p_a for neighbours b
+            Bits s = gram[a]; s &= alive; s.put((uint32_t)a, false);
+            const Bits pa = basis[a];
+            s.each([&](uint32_t k) { gram[k] ^= s; basis[k] ^= pa; });
+            alive.put((uint32_t)a, false);
+            nf.odd.push_back(pa);
+            cls[a] = 0;
+            s.each([&](uint32_t k) { reclass(k); });
+            continue;
+        }
+        a = pop_current(ho, 2);
+        if (a < 0) break;
+        // 2x2 (hyperbolic) pivot on a and its first alive neighbour
+        Bits ra = gram[a]; ra &= alive;
+        uint32_t bb = (uint32_t)ra.lowest();
+        Bits sa = ra; sa.put(bb, false);
+        Bits sb = gram[bb]; sb &= alive; sb.put((uint32_t)a, false);
+        const Bits pa = basis[a], pb = basis[bb];
+        Bits touched = sa;
+        for (size_t k = 0; k < touched.w.size(); ++k) touched.w[k] |= sb.w[k];
+        touched.each([&](uint32_t k) {
+            if (sa.get(k)) { gram[k] ^= sb; basis[k] ^= pb; }
+            if (sb.get(k)) { gram[k] ^= sa; basis[k] ^= pa; }
+        });
+        alive.put((uint32_t)a, false); alive.put(bb, false);
+        nf.pa.push_back(pa); nf.pb.push_back(pb);
+        cls[a] = 0; cls[bb] = 0;
+        touched.each([&](uint32_t k) { reclass(k); });
+    }
+    alive.each([&](uint32_t r) { nf.even.push_back(basis[r]); });
+    return nf;
+}
+
+// d(p) = q(p) mod 4 for a basis vector (normal_form.rs:339-348)
+static uint32_t z4_of(const std::vector<Bits>& b, const Bits& v0, const Bits& v1, const Bits& p) {
+    uint32_t lin = p.and_popc(v0) + 2 * p.and_popc(v1);
+    uint32_t t = 0;
+    p.each([&](uint32_t j) { t += b[j].and_popc(p); });
+    return (lin + 2 * ((t / 2) & 1)) & 3;
+}
+
+static inline bool row_parity(const uint64_t* x, const uint64_t* y, uint32_t nw) {
+    uint64_t a = 0; for (uint32_t k = 0; k < nw; ++k) a ^= x[k] & y[k]; return __builtin_popcountll(a) & 1;
+}
+static inline uint32_t row_popc(const uint64_t* x, uint32_t nw) {
+    uint32_t c = 0; for (uint32_t k = 0; k < nw; ++k) c += (uint32_t)__builtin_popcountll(x[k]); return c;
+}
+static inline uint32_t row_and_popc(const uint64_t* x, const uint64_t* y, uint32_t nw) {
+    uint32_t c = 0; for (uint32_t k = 0; k < nw; ++k) c += (uint32_t)__builtin_popcountll(x[k] & y[k]); return c;
+}
+
+// Gauss-sum phase of a shift row (normal_form.rs:479-493), written as
+// sigma + 2|u_o| + 4(|u_o & o1| + |(wa^ua)&(wb^ub)|) mod 8, which is the same
+// integer because o3 is the complement of o1 among the lines.
+uint32_t phase8_row(const HostEngine& E, const uint64_t* u) {
+    const ULayout& L = E.L;
+    const uint64_t* D = E.diag.data();
+    uint32_t uo = row_popc(u + L.off_o(), L.Wo);
+    uint32_t uo1 = row_and_popc(u + L.off_o(), D + L.off_o(), L.Wo);
+    uint32_t pl = 0;
+    for (uint32_t k = 0; k < L.Wg; ++k)
+        pl += (uint32_t)__builtin_popcountll((D[L.off_a() + k] ^ u[L.off_a() + k]) & (D[L.off_b() + k] ^ u[L.off_b() + k]));
+    int v = E.sigma + 2 * (int)uo + 4 * (int)((uo1 + pl) & 1);
+    return (uint32_t)(((v % 8) + 8) % 8);
+}
+bool survives_row(const HostEngine& E, const uint64_t* u) {
+    const ULayout& L = E.L;
+    for (uint32_t k = 0; k < L.We; ++k) if (u[L.off_e() + k] != E.diag[L.off_e() + k]) return false;
+    return true;
+}
+uint32_t total_phase(const HostEngine& E, const uint64_t* u, uint64_t s) {
+    uint32_t ns = (uint32_t)__builtin_popcountll(s);
+    uint32_t nn = (uint32_t)__builtin_popcountll(E.neg_mask & ~s);
+    return (phase8_row(E, u) + 6 * ns + 4 * nn) & 7;
+}
+
+// ---------------------------------------------------------------- engine --
+std::shared_ptr<HostEngine> build_engine(const PathSum& ps, const std::vector<uint32_t>& fixed) {
+    auto Ep = std::make_shared<HostEngine>();
+    HostEngine& E = *Ep;
+    const uint32_t nv = ps.nv;
+    E.fixed = fixed;
+    E.nf = (uint32_t)fixed.size();
+    E.fixed_pos.assign(nv, -1);
+    for (uint32_t i = 0; i < E.nf; ++i) {
+        uint32_t f = fixed[i];
+        if (f >= nv || ps.dead[f]) throw std::invalid_argument("fixed variable is not live");
+        if (E.fixed_pos[f] >= 0) throw std::invalid_argument("duplicate fixed variable");
+        E.fixed_pos[f] = (int32_t)i;
+    }
+    std::vector<uint32_t> freev;
+    std::vector<int32_t> fidx(nv, -1);
+    for (uint32_t i = 0; i < nv; ++i) if (!ps.dead[i] && E.fixed_pos[i] < 0) { fidx[i] = (int32_t)freev.size(); freev.push_back(i); }
+    const uint32_t m = E.m = (uint32_t)freev.size();
+    std::vector<Bits> bff(m, Bits(m));
+    Bits parity(m);
+    for (uint32_t i = 0; i < m; ++i) {
+        ps.adj[freev[i]].each([&](uint32_t j) { if (fidx[j] >= 0) bff[i].set((uint32_t)fidx[j]); });
+        parity.put(i, ps.v[freev[i]] & 1);
+    }
+    E.b_fixed.assign(E.nf, Bits(E.nf));
+    for (uint32_t i = 0; i < E.nf; ++i) ps.adj[fixed[i]].each([&](uint32_t j) { if (E.fixed_pos[j] >= 0) E.b_fixed[i].set((uint32_t)E.fixed_pos[j]); });
+
+    NormalForm nf = normal_form(bff, parity);
+    ULayout& L = E.L;
+    L.g = (uint32_t)nf.pa.size(); L.nodd = (uint32_t)nf.odd.size(); L.neven = (uint32_t)nf.even.size();
+    L.Wg = (L.g + 63) / 64; L.Wo = (L.nodd + 63) / 64; L.We = (L.neven + 63) / 64;
+    const uint32_t W = L.words();
+    E.log2mag = (double)L.g + (double)L.neven + (double)L.nodd / 2.0;   // normal_form.rs:465-467
+
+    // Column rows: for each needed free variable j, the packed coordinates
+    // (p_l[j])_l (normal_form.rs:413-433).  Needed = neighbours of fixed
+    // variables and rotated variables.
+    std::vector<uint8_t> needed(m, 0);
+    for (uint32_t f : fixed) ps.adj[f].each([&](uint32_t j) { if (fidx[j] >= 0) needed[fidx[j]] = 1; });
+    std::vector<uint32_t> term_vars;
+    for (uint32_t i = 0; i < m; ++i) if (ps.theta[freev[i]] != 0) { needed[i] = 1; term_vars.push_back(i); }
+    std::vector<int32_t> crow(m, -1);
+    uint32_t ncr = 0;
+    for (uint32_t i = 0; i < m; ++i) if (needed[i]) crow[i] = (int32_t)ncr++;
+    std::vector<uint64_t> cols((size_t)ncr * W, 0);
+    auto scatter = [&](const std::vector<Bits>& vecs, uint32_t off) {
+        for (uint32_t l = 0; l < vecs.size(); ++l)
+            vecs[l].each([&](uint32_t j) { if (crow[j] >= 0) cols[(size_t)crow[j] * W + off + (l >> 6)] |= 1ull << (l & 63); });
+    };
+    scatter(nf.pa, L.off_a()); scatter(nf.pb, L.off_b()); scatter(nf.odd, L.off_o()); scatter(nf.even, L.off_e());
+    E.ucols.assign((size_t)E.nf * W, 0);
+    for (uint32_t i = 0; i < E.nf; ++i)
+        ps.adj[fixed[i]].each([&](uint32_t j) {
+            if (fidx[j] >= 0) { const uint64_t* c = &cols[(size_t)crow[fidx[j]] * W]; for (uint32_t k = 0; k < W; ++k) E.ucols[(size_t)i * W + k] ^= c[k]; }
+        });
+    E.t = (uint32_t)term_vars.size();
+    if (E.t > 64) throw std::invalid_argument("more than 64 non-Clifford terms in one engine");
+    E.mcols.assign((size_t)E.t * W, 0);
+    for (uint32_t k = 0; k < E.t; ++k) std::copy_n(&cols[(size_t)crow[term_vars[k]] * W], W, &E.mcols[(size_t)k * W]);
+
+    // Diag (normal_form.rs:354-391)
+    Bits v0(m), v1(m);
+    for (uint32_t i = 0; i < m; ++i) { v0.put(i, ps.v[freev[i]] & 1); v1.put(i, ps.v[freev[i]] & 2); }
+    E.diag.assign(W, 0);
+    E.o3.assign(L.Wo, 0);
+    for (uint32_t j = 0; j < L.g; ++j) {
+        if (z4_of(bff, v0, v1, nf.pa[j]) == 2) E.diag[L.off_a() + (j >> 6)] |= 1ull << (j & 63);
+        if (z4_of(bff, v0, v1, nf.pb[j]) == 2) E.diag[L.off_b() + (j >> 6)] |= 1ull << (j & 63);
+    }
+    E.sigma = 0;
+    for (uint32_t l = 0; l < L.nodd; ++l) {
+        if (z4_of(bff, v0, v1, nf.odd[l]) == 1) { E.diag[L.off_o() + (l >> 6)] |= 1ull << (l & 63); E.sigma += 1; }
+        else { E.o3[l >> 6] |= 1ull << (l & 63); E.sigma -= 1; }
+    }
+    for (uint32_t l = 0; l < L.neven; ++l)
+        if (z4_of(bff, v0, v1, nf.even[l]) == 2) E.diag[L.off_e() + (l >> 6)] |= 1ull << (l & 63);
+
+    // RREF of K = M restricted to the radical coordinates (engine.rs:119-184)
+    const uint32_t t = E.t, ne = L.neven;
+    std::vector<uint64_t> krow(ne, 0);
+    std::vector<Bits> rrow(ne, Bits(ne));
+    for (uint32_t e = 0; e < ne; ++e) {
+        for (uint32_t c = 0; c < t; ++c) if ((E.mcols[(size_t)c * W + L.off_e() + (e >> 6)] >> (e & 63)) & 1) krow[e] |= 1ull << c;
+        rrow[e].set(e);
+    }
+    uint32_t rank = 0;
+    std::vector<uint8_t> is_piv(t, 0);
+    for (uint32_t col = 0; col < t; ++col) {
+        uint32_t r = rank;
+        while (r < ne && !((krow[r] >> col) & 1)) ++r;
+        if (r == ne) continue;
+        std::swap(krow[rank], krow[r]); std::swap(rrow[rank], rrow[r]);
+        for (uint32_t r2 = 0; r2 < ne; ++r2)
+            if (r2 != rank && ((krow[r2] >> col) & 1)) { krow[r2] ^= krow[rank]; rrow[r2] ^= rrow[rank]; }
+        E.pivot_col.push_back(col);
+        is_piv[col] = 1;
+        rank++;
+    }
+    E.rk = rank;
+    // the R rows are read after the elimination finished (engine.rs:149)
+    for (uint32_t i = 0; i < rank; ++i) E.pivot_row.push_back(rrow[i]);
+    for (uint32_t r = rank; r < ne; ++r) E.consistency.push_back(rrow[r]);
+    for (uint32_t j = 0; j < t; ++j) {
+        if (is_piv[j]) continue;
+        uint64_t nvec = 1ull << j;
+        for (uint32_t i = 0; i < rank; ++i) if ((krow[i] >> j) & 1) nvec |= 1ull << E.pivot_col[i];
+        E.kernel.push_back(nvec);
+        E.free_col.push_back(j);
+    }
+    E.d = (uint32_t)E.kernel.size();
+    E.kernel_u.assign((size_t)E.d * W, 0);
+    for (uint32_t j = 0; j < E.d; ++j)
+        for (uint32_t k = 0; k < t; ++k) if ((E.kernel[j] >> k) & 1)
+            for (uint32_t w = 0; w < W; ++w) E.kernel_u[(size_t)j * W + w] ^= E.mcols[(size_t)k * W + w];
+
+    // Rotation terms.  Reference-style FP64 multipliers (engine.rs:221-238) for
+    // the plain kernel, and the factorised form for the Z8 kernel.
+    double half_sum = 0;   // sum theta_k / 2 in units of 2pi/2^32
+    uint64_t tsum = 0;
+    for (uint32_t k = 0; k < t; ++k) {
+        Phase th = ps.theta[freev[term_vars[k]]];
+        E.theta.push_back(th);
+        std::complex<double> e = phase_complex(th), one(1, 0);
+        std::complex<double> a0 = (one + e) * 0.5, a1 = (one - e) * 0.5;
+        E.a0.push_back(a0); E.a1.push_back(a1);
+        // num-complex division formula (conj multiply over the squared norm)
+        auto cdiv = [](std::complex<double> x, std::complex<double> y) {
+            double n = y.real() * y.real() + y.imag() * y.imag();
+            return std::complex<double>((x.real() * y.real() + x.imag() * y.imag()) / n, (x.imag() * y.real() - x.real() * y.imag()) / n);
+        };
+        E.up.push_back(cdiv(a1, a0)); E.down.push_back(cdiv(a0, a1));
+        double half = M_PI * (double)th / 4294967296.0;   // theta/2 in (0, pi)
+        double cs = std::fabs(std::cos(half)), sn = std::sin(half);
+        double mx = std::max(cs, sn);
+        E.r0.push_back(cs / mx); E.r1.push_back(sn / mx);
+        E.Bmag *= mx;
+        if (th > kPi) E.neg_mask |= 1ull << k;
+        tsum += th;
+    }
+    (void)half_sum;
+    {
+        // E = e^{i sum theta_k / 2}: the half-angle sum is periodic mod 2^33 units
+        uint64_t hs = tsum & ((1ull << 33) - 1);
+        double ang = M_PI * (double)hs / 4294967296.0;
+        E.Ephase = std::complex<double>(std::cos(ang), std::sin(ang));
+    }
+
+    // Z8 quadratic form of the per-branch phase over the kernel coordinates.
+    const uint32_t d = E.d;
+    E.Q.assign(d, 0);
+    E.lam.assign(d, 0);
+    E.dual.assign((size_t)d * W, 0);
+    const uint64_t* D = E.diag.data();
+    for (uint32_t j = 0; j < d; ++j) {
+        const uint64_t* U = &E.kernel_u[(size_t)j * W];
+        const uint64_t* A = U + L.off_a(); const uint64_t* Bq = U + L.off_b(); const uint64_t* Y = U + L.off_o();
+        uint64_t Nj = E.kernel[j];
+        uint32_t l = 2 * row_popc(Y, L.Wo) + 4 * (row_and_popc(Y, D + L.off_o(), L.Wo) & 1)
+                   + 4 * row_parity(A, Bq, L.Wg) + 4 * row_parity(A, D + L.off_b(), L.Wg) + 4 * row_parity(D + L.off_a(), Bq, L.Wg)
+                   + 6 * (uint32_t)__builtin_popcountll(Nj) + 4 * ((uint32_t)__builtin_popcountll(E.neg_mask & Nj) & 1);
+        E.lam[j] = (uint8_t)(l & 7);
+        // dual row: pairs with the query shift u_start: <ua, b_j> + <ub, a_j> + <uo, y_j>
+        uint64_t* Dd = &E.dual[(size_t)j * W];
+        std::copy_n(Bq, L.Wg, Dd + L.off_a());
+        std::copy_n(A, L.Wg, Dd + L.off_b());
+        std::copy_n(Y, L.Wo, Dd + L.off_o());
+        for (uint32_t k = j + 1; k < d; ++k) {
+            const uint64_t* U2 = &E.kernel_u[(size_t)k * W];
+            bool q = row_parity(Y, U2 + L.off_o(), L.Wo) ^ row_parity(A, U2 + L.off_b(), L.Wg) ^ row_parity(U2 + L.off_a(), Bq, L.Wg)
+                   ^ ((__builtin_popcountll(Nj & E.kernel[k]) & 1) != 0);
+            if (q) { E.Q[j] |= 1ull << k; E.Q[k] |= 1ull << j; }
+        }
+    }
+
+    for (uint32_t f : fixed) { E.fixed_v.push_back(ps.v[f]); E.fixed_theta.push_back(ps.theta[f]); }
+    E.nh = ps.nh; E.global = ps.global; E.zero = ps.zero;
+    return Ep;
+}
+
+// -------------------------------------------------------------- prologue --
+bool prepare(const HostEngine& E, const uint8_t* values, Prepared& P) {
+    P.nonzero = false;
+    if (E.zero) return false;                                   // engine.rs:570-572
+    const ULayout& L = E.L;
+    const uint32_t W = L.words();
+    uint32_t z = 0;
+    Phase th = E.global;
+    P.u0.assign(W, 0);
+    std::vector<uint32_t> ones;
+    for (uint32_t i = 0; i < E.nf; ++i) if (values[i]) ones.push_back(i);
+    for (size_t a = 0; a < ones.size(); ++a) {
+        uint32_t f = ones[a];
+        z += E.fixed_v[f]; th += E.fixed_theta[f];
+        for (size_t b2 = a + 1; b2 < ones.size(); ++b2) if (E.b_fixed[f].get(ones[b2])) z += 2;
+        const uint64_t* c = &E.ucols[(size_t)f * W];
+        for (uint32_t k = 0; k < W; ++k) P.u0[k] ^= c[k];
+    }
+    P.phase = th + z4(z);
+    // kappa = w_e + u0_e; particular solution of K s = kappa (engine.rs:188-201)
+    Bits kappa(L.neven);
+    for (uint32_t k = 0; k < L.We; ++k) kappa.w[k] = E.diag[L.off_e() + k] ^ P.u0[L.off_e() + k];
+    for (auto& r : E.consistency) if (r.dot(kappa)) return false;
+    uint64_t s0 = 0;
+    for (uint32_t i = 0; i < E.rk; ++i) if (E.pivot_row[i].dot(kappa)) s0 |= 1ull << E.pivot_col[i];
+    P.s0 = s0;
+    P.u_start = P.u0;
+    P.c_start = {1, 0};
+    for (uint32_t k = 0; k < E.t; ++k) {
+        if ((s0 >> k) & 1) {
+            const uint64_t* c = &E.mcols[(size_t)k * W];
+            for (uint32_t w = 0; w < W; ++w) P.u_start[w] ^= c[w];
+            P.c_start *= E.a1[k];
+        } else P.c_start *= E.a0[k];
+    }
+    P.K0 = total_phase(E, P.u_start.data(), s0);
+    P.ell.resize(E.d);
+    for (uint32_t j = 0; j < E.d; ++j) {
+        bool dq = row_parity(P.u_start.data(), &E.dual[(size_t)j * W], W) ^ ((__builtin_popcountll(s0 & E.kernel[j]) & 1) != 0);
+        P.ell[j] = (uint8_t)((E.lam[j] + (dq ? 4 : 0)) & 7);
+    }
+    double mag = std::pow(2.0, E.log2mag - (double)E.nh / 2.0) * E.Bmag;   // engine.rs:563-565
+    P.scale = phase_complex(P.phase) * E.Ephase * mag;
+    P.nonzero = true;
+    return true;
+}
+
+bool prepare_assignment(const HostEngine& E, const std::vector<std::pair<uint32_t, uint8_t>>& asg, Prepared& P) {
+    std::vector<uint8_t> vals(E.nf, 0);
+    size_t seen = 0;
+    for (auto& a : asg) {
+        int32_t p = a.first < E.fixed_pos.size() ? E.fixed_pos[a.first] : -1;
+        if (p < 0) throw std::invalid_argument("variable is not fixed in this engine");   // engine.rs:527
+        vals[p] = a.second; seen++;
+    }
+    if (seen < E.nf) throw std::invalid_argument("assignment does not cover all fixed variables");  // engine.rs:531
+    return prepare(E, vals.data(), P);
+}
+
+// ---------------------------------------------------------------- rank.rs --
+std::vector<std::pair<uint32_t, uint32_t>> pair_candidates(const PathSum& ps, const std::vector<uint32_t>& fixed, uint32_t& n_tclass) {
+    const uint32_t M30 = (1u << 30) - 1;
+    std::map<uint32_t, std::vector<uint32_t>> groups;   // ascending canonical residue (rank.rs:62-84)
+    for (uint32_t k = 0; k < ps.nv; ++k) {
+        if (ps.dead[k] || ps.theta[k] == 0 || std::find(fixed.begin(), fixed.end(), k) != fixed.end()) continue;
+        uint32_t r = ps.theta[k] & M30;
+        uint32_t o = (uint32_t)(((uint64_t)M30 + 1 - r) & M30);
+        groups[std::min(r, o)].push_back(k);
+    }
+    std::vector<std::pair<uint32_t, uint32_t>> tp, gp;
+    for (auto& g : groups) {
+        auto& dst = g.first == (1u << 29) ? tp : gp;
+        for (size_t i = 0; i + 1 < g.second.size(); i += 2) dst.push_back({g.second[i], g.second[i + 1]});
+    }
+    n_tclass = (uint32_t)tp.size();
+    tp.insert(tp.end(), gp.begin(), gp.end());
+    return tp;
+}
+
+uint32_t best_pairs(const EngineStats& st, uint32_t avail, uint32_t tclass) {   // rank.rs:205-222
+    double m = (double)st.free_vars, words = (double)(st.free_vars / 64 + 1);
+    double ldl = 2.0 * m * m * m / 64.0 + 50.0 * m + 2000.0;
+    double step = 20.0 + 4.0 * words + (double)st.terms / 2.0;
+    uint32_t best = 0; double bc = INFINITY;
+    for (uint32_t p = 0; p <= std::min(avail, kMaxPairs); ++p) {
+        uint32_t removed = 2 * std::min(p, tclass) + (p > tclass ? p - tclass : 0);
+        double dd = st.branch_dim > removed ? (double)(st.branch_dim - removed) : 0.0;
+        double cost = std::pow(2.0, (double)p) * ((p ? ldl : 0.0) + std::pow(2.0, dd) * step);
+        if (cost < bc) { bc = cost; best = p; }
+    }
+    return best;
+}
+
+}  // namespace pq

@@ -1,0 +1,219 @@
+// Host side of the MI355X branch evaluator: circuit model, path-sum compiler,
+// symmetric F2 normal form, branch-subspace elimination and the packed engine
+// image that the HIP kernels read.  One engine is built per (path sum, fixed
+// variable set), once, on the host; every amplitude query afterwards is a
+// prologue of word XORs plus a device launch.
+#pragma once
+#include <cstdint>
+#include <cstddef>
+#include <vector>
+#include <string>
+#include <memory>
+#include <complex>
+#include <stdexcept>
+
+namespace pq {
+
+// ------------------------------------------------------------------ F2 rows
+// A packed F2 vector: bit i in word i/64 at position i%64, tail bits zero
+// (same byte layout as the reference BitVec, src/bits.rs:10-19, so a u32 view
+// (i/32, i%32) is the same bytes).
+struct Bits {
+    std::vector<uint64_t> w;
+    uint32_t n = 0;
+    Bits() {}
+    explicit Bits(uint32_t len) : w((len + 63) / 64, 0), n(len) {}
+    inline bool get(uint32_t i) const { return (w[i >> 6] >> (i & 63)) & 1; }
+    inline void set(uint32_t i) { w[i >> 6] |= 1ull << (i & 63); }
+    inline void put(uint32_t i, bool v) { if (v) set(i); else w[i >> 6] &= ~(1ull << (i & 63)); }
+    inline void flip(uint32_t i) { w[i >> 6] ^= 1ull << (i & 63); }
+    inline void operator^=(const Bits& o) { for (size_t k = 0; k < w.size(); ++k) w[k] ^= o.w[k]; }
+    inline void operator&=(const Bits& o) { for (size_t k = 0; k < w.size(); ++k) w[k] &= o.w[k]; }
+    inline uint32_t popc() const { uint32_t c = 0; for (uint64_t x : w) c += (uint32_t)__builtin_popcountll(x); return c; }
+    inline uint32_t and_popc(const Bits& o) const {
+        uint32_t c = 0; for (size_t k = 0; k < w.size(); ++k) c += (uint32_t)__builtin_popcountll(w[k] & o.w[k]); return c;
+    }
+    inline bool dot(const Bits& o) const {
+        uint64_t a = 0; for (size_t k = 0; k < w.size(); ++k) a ^= w[k] & o.w[k]; return __builtin_popcountll(a) & 1;
+    }
+    inline bool any() const { for (uint64_t x : w) if (x) return true; return false; }
+    inline void clear() { for (auto& x : w) x = 0; }
+    template <class F> inline void each(F f) const {
+        for (size_t k = 0; k < w.size(); ++k) { uint64_t x = w[k]; while (x) { f((uint32_t)(k * 64 + __builtin_ctzll(x))); x &= x - 1; } }
+    }
+    inline long lowest() const {
+        for (size_t k = 0; k < w.size(); ++k) if (w[k]) return (long)(k * 64 + __builtin_ctzll(w[k]));
+        return -1;
+    }
+    bool operator==(const Bits& o) const { return n == o.n && w == o.w; }
+};
+
+// ------------------------------------------------------------ phase ring
+// Angle 2*pi*k/2^32 with wrapping addition (src/phase.rs:22-36).
+typedef uint32_t Phase;
+constexpr Phase kPi = 1u << 31, kHalfPi = 1u << 30, kQuarterPi = 1u << 29;
+inline Phase z4(uint32_t k) { return (k & 3u) << 30; }                     // phase.rs:112-114
+inline int clifford_level(Phase p) { return (p & ((1u << 30) - 1)) ? -1 : (int)(p >> 30); }  // phase.rs:70-76
+std::complex<double> phase_complex(Phase p);                               // phase.rs:60-66
+Phase phase_from_radians(double r);                                        // phase.rs:39-44
+
+// ------------------------------------------------------------- circuits
+// Gate codes shared by the C ABI (the Gate enum of src/circuit.rs:25-42).
+enum GateOp : uint8_t { G_H = 0, G_X, G_Y, G_Z, G_S, G_SDG, G_T, G_TDG, G_P, G_CZ, G_CX, G_SWAP, G_GPHASE, G_COUNT };
+struct Gate { uint8_t op; uint32_t a, b; Phase ph; };
+struct Circuit {
+    uint32_t n = 0;
+    std::vector<Gate> gates;
+    void add(uint8_t op, uint32_t a, uint32_t b = 0, Phase ph = 0);
+    uint32_t hadamards() const;     // circuit.rs:206-215 (CX counts two)
+    Circuit inverse() const;        // circuit.rs:46-56, 189-191
+};
+
+// ------------------------------------------------------------- path sum
+// <y|C|x> = 2^{-h/2} e^{i phi} sum_z i^{Q(z)} e^{i sum theta_k z_k}  (src/pathsum.rs:1-30)
+struct PathSum {
+    uint32_t nq = 0, nv = 0, nh = 0;
+    std::vector<Bits> adj;          // symmetric, zero diagonal
+    std::vector<uint8_t> v;         // Z4 linear part
+    std::vector<Phase> theta;       // non-Clifford rotations
+    Phase global = 0;
+    std::vector<uint8_t> dead;
+    std::vector<uint32_t> out_var;
+    std::vector<uint8_t> out_neg;
+    bool zero = false;
+
+    std::vector<uint32_t> io_vars() const;                                  // pathsum.rs:328-336
+    void substitute(uint32_t a, uint32_t b, bool c);                        // pathsum.rs:360-394
+    // Boundary (var, value) pairs for <y|C|x>; false if contradictory
+    // (pathsum.rs:430-450).
+    bool boundary(const uint8_t* x, const uint8_t* y, std::vector<std::pair<uint32_t, uint8_t>>& out) const;
+};
+PathSum compile(const Circuit& c);                                          // pathsum.rs:274-297
+struct Sandwich { PathSum ps; std::vector<uint32_t> junction; std::vector<uint8_t> jneg; };
+Sandwich compile_sandwich(const Circuit& c);                                // pathsum.rs:304-314
+
+// ----------------------------------------------------------- normal form
+// Block-diagonalising basis of the F2 Gram matrix (normal_form.rs:146-157):
+// hyperbolic planes (a_j, b_j), non-isotropic lines, radical vectors.
+struct NormalForm {
+    uint32_t m = 0;
+    std::vector<Bits> pa, pb, odd, even;
+};
+NormalForm normal_form(const std::vector<Bits>& b, const Bits& parity);    // normal_form.rs:240-336 (MinWeight)
+
+// Shift coordinates packed in one row: [a: Wg words | b: Wg | o: Wo | e: We]
+// (u64 words).  Along kernel directions the e part is zero.
+struct ULayout {
+    uint32_t g = 0, nodd = 0, neven = 0;
+    uint32_t Wg = 0, Wo = 0, We = 0;
+    uint32_t words() const { return 2 * Wg + Wo + We; }
+    uint32_t off_a() const { return 0; }
+    uint32_t off_b() const { return Wg; }
+    uint32_t off_o() const { return 2 * Wg; }
+    uint32_t off_e() const { return 2 * Wg + Wo; }
+};
+
+struct EngineStats { uint64_t free_vars, planes, lines, radical, terms, branch_dim; };  // engine.rs:64-79
+
+// Everything the device needs about one engine, host copy.  Built once per
+// (path sum, fixed set): engine.rs:289-390 restated for a packed layout.
+struct HostEngine {
+    // sizes
+    uint32_t m = 0, t = 0, d = 0, nf = 0, rk = 0;   // rk = rank K
+    uint32_t nh = 0;
+    Phase global = 0;
+    bool zero = false;
+    double log2mag = 0;
+    ULayout L;
+    // fixed variables
+    std::vector<uint32_t> fixed;
+    std::vector<int32_t> fixed_pos;
+    std::vector<uint8_t> fixed_v;
+    std::vector<Phase> fixed_theta;
+    std::vector<Bits> b_fixed;
+    // packed shift rows (L.words() u64 each)
+    std::vector<uint64_t> ucols;    // nf rows
+    std::vector<uint64_t> mcols;    // t rows
+    // Diag (normal_form.rs:205-218), packed like a shift row: wa in the a slot,
+    // wb in the b slot, o1 in the o slot, we in the e slot.
+    std::vector<uint64_t> diag;
+    std::vector<uint64_t> o3;       // complement of o1 among the lines (Wo words)
+    int32_t sigma = 0;
+    // branch subspace: RREF of K (engine.rs:106-201)
+    std::vector<uint32_t> pivot_col;         // rk
+    std::vector<Bits> pivot_row;             // rk rows over neven
+    std::vector<Bits> consistency;           // rows over neven
+    std::vector<uint64_t> kernel;            // d kernel vectors over t (t <= 64)
+    std::vector<uint32_t> free_col;          // d: the non-pivot column of kernel j
+    std::vector<uint64_t> kernel_u;          // d packed shift rows
+    // rotation terms
+    std::vector<Phase> theta;                // t
+    uint64_t neg_mask = 0;                   // theta_k > pi
+    // factorised weights: weight(s) = E * B * omega^{6|s| + 4|neg\s|} * prod_k r_k[s_k]
+    std::vector<double> r0, r1;              // per term, max(r0, r1) = 1
+    double Bmag = 1.0;                       // prod_k max(|cos|, sin)
+    std::complex<double> Ephase{1, 0};       // e^{i sum theta_k / 2}
+    // quadratic Z8 form of the per-branch phase over kernel coordinates
+    std::vector<uint64_t> Q;                 // d rows (bit k of row j = Q_jk), symmetric, zero diagonal
+    std::vector<uint8_t> lam;                // query-independent part of the linear coefficients
+    std::vector<uint64_t> dual;              // d packed rows: (b_j | a_j | y_j | 0) for the query-dependent parity
+    // reference-style FP64 term multipliers for the plain kernel (engine.rs:221-238)
+    std::vector<std::complex<double>> a0, a1, up, down;
+    EngineStats stats() const { return EngineStats{m, L.g, L.nodd, L.neven, t, d}; }
+};
+
+// Build an engine for `ps` with the given fixed variables (engine.rs:289-370).
+std::shared_ptr<HostEngine> build_engine(const PathSum& ps, const std::vector<uint32_t>& fixed);
+
+// Per-query prologue (engine.rs:569-610) extended with the Z8 form data.
+struct Prepared {
+    bool nonzero = false;            // false: the amplitude is exactly 0
+    Phase phase = 0;                 // boundary phase (integer exact)
+    uint64_t s0 = 0;                 // particular solution over the t terms
+    std::vector<uint64_t> u_start;   // packed shift row at c = 0
+    std::vector<uint64_t> u0;        // shift row of the boundary alone (s = 0)
+    std::complex<double> c_start;    // reference-style weight at s0 (plain kernel)
+    uint32_t K0 = 0;                 // total Z8 phase at c = 0
+    std::vector<uint8_t> ell;        // d linear Z8 coefficients
+    std::complex<double> scale;      // e^{i phase} 2^{log2mag - h/2} E B
+};
+bool prepare(const HostEngine& E, const uint8_t* values, Prepared& P);
+// Same, with a (var, value) assignment covering every fixed variable.
+bool prepare_assignment(const HostEngine& E, const std::vector<std::pair<uint32_t, uint8_t>>& asg, Prepared& P);
+
+// Host evaluation of the Gauss-sum phase (normal_form.rs:479-493) for a packed
+// shift row, and of the total per-branch phase of the factorised weight.
+uint32_t phase8_row(const HostEngine& E, const uint64_t* u);
+bool survives_row(const HostEngine& E, const uint64_t* u);
+uint32_t total_phase(const HostEngine& E, const uint64_t* u, uint64_t s);
+
+// ------------------------------------------------------- Z8 walk image
+// Coordinates of the branch subspace are laid out in slots: 8 inner slots
+// (walked from a uniform 256-entry table; slots without a real coordinate are
+// phantoms whose table entries are zero), then the outer slots, whose first M
+// are walked in Gray order per lane and the rest index the lane's coset.
+constexpr uint32_t kInner = 8;
+struct Z8Image {
+    uint32_t d = 0, nslots = 0, nin = 0, t = 0, npin = 0, nbytes = 0;
+    std::vector<int32_t> slot_coord;   // kernel-direction index per slot, -1 for a phantom
+    std::vector<uint64_t> slot_q;      // Q rows over slots
+    std::vector<uint64_t> slot_n;      // kernel vector N over the t terms
+    std::vector<uint32_t> slot_pin;    // inner-touched pivot bits of N, compacted
+    std::vector<uint8_t> slot_lam;     // query-independent linear coefficient
+    std::vector<float> inner;          // 256 x {w_re, w_im, pin*4 (bits), kin (bits)}
+    std::vector<float> lt;             // nbytes x 256 lane-factor byte tables
+    std::vector<float> gin;            // 2^npin inner pivot factor table
+    std::vector<uint32_t> pin_term;    // term index of each compact inner pivot bit
+    uint64_t lane_terms = 0;           // terms whose ratio is carried by the lane factor
+};
+Z8Image build_z8_image(const HostEngine& E);
+// Query part of the Z8 walk: K0, delta bits over slots, s0, compact pivots.
+struct Z8QueryHost { uint32_t K0; uint64_t delta; uint64_t s0; uint32_t pio0; };
+Z8QueryHost z8_query(const HostEngine& E, const Z8Image& Z, const Prepared& P);
+
+// Rank-2 decomposition of rotation pairs (src/rank.rs:44-222).
+constexpr uint32_t kMaxPairs = 14;
+std::vector<std::pair<uint32_t, uint32_t>> pair_candidates(const PathSum& ps, const std::vector<uint32_t>& fixed, uint32_t& n_tclass);
+uint32_t best_pairs(const EngineStats& st, uint32_t avail, uint32_t tclass);
+
+}  // namespace pq

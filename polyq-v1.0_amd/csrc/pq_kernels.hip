@@ -1,0 +1,865 @@
+// HIP kernels for gfx950 (MI355X): the branch evaluator of the polyq path-sum
+// engine.
+//
+//  k_plain<NW>    straightforward FP64 port of Engine::gray / sum_subspace
+//                 (src/engine.rs:669-722): one thread per Gray chunk, packed
+//                 shift state (a, b, o words) in VGPRs, phase8 by popcounts
+//                 (normal_form.rs:479-493), FP64 complex weight updated with
+//                 the up/down multipliers and recomputed every 2048 steps.
+//  k_z8_walk      persistent, work-queue kernel over (query, 256-lane block)
+//                 items of many queries and engines.  A lane owns one coset of
+//                 the outer coordinates, walks the mid coordinates in Gray
+//                 order by single-slot updates of its integer state, and sums
+//                 the 256 inner branches of each outer point from a uniform
+//                 table (scalar loads): per branch one sign flip and two FP32
+//                 FMAs (plus one LDS lookup when inner directions touch
+//                 pivots).  Partials fold into per-lane two-sum compensated
+//                 FP32 accumulators split by the parity of the lane phase.
+//  k_z8_reduce    FP64 cross-workgroup reduction and global scaling.
+//  k_dump_s, k_dump_walk   per-branch outputs for parity tests.
+//
+// Uniform data (structure rows, tables indexed by the loop counter) is read
+// through the scalar cache; lane-indexed tables (lane-factor byte tables, the
+// inner pivot table) are staged in LDS.
+#include <hip/hip_runtime.h>
+#include "pq_device.hpp"
+#include <mutex>
+#include <cstring>
+#include <algorithm>
+#include <numeric>
+
+namespace pq {
+
+#define HIPCHECK(x)                                                                                   \
+    do {                                                                                              \
+        hipError_t e_ = (x);                                                                          \
+        if (e_ != hipSuccess) throw std::runtime_error(std::string("HIP: ") + hipGetErrorString(e_) + " (" #x ")"); \
+    } while (0)
+
+// ----------------------------------------------------------------- runtime --
+namespace {
+struct DBuf {
+    void* p = nullptr;
+    size_t cap = 0;
+    void* get(size_t n) {
+        if (n > cap) {
+            if (p) HIPCHECK(hipFree(p));
+            size_t c = std::max<size_t>(n, cap * 2);
+            c = std::max<size_t>(c, 4096);
+            HIPCHECK(hipMalloc(&p, c));
+            cap = c;
+        }
+        return p;
+    }
+};
+struct HBuf {
+    void* p = nullptr;
+    size_t cap = 0;
+    void* get(size_t n) {
+        if (n > cap) {
+            if (p) HIPCHECK(hipHostFree(p));
+            size_t c = std::max<size_t>(n, cap * 2);
+            c = std::max<size_t>(c, 4096);
+            HIPCHECK(hipHostMalloc(&p, c, hipHostMallocDefault));
+            cap = c;
+        }
+        return p;
+    }
+};
+struct Ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    hipEvent_t e0, e1, e2, e3;
+    int cus = 0;
+    std::mutex mu;
+    DBuf d_engs, d_qs, d_counter, d_partial, d_out, d_misc, d_misc2, d_misc3;
+    HBuf h_in, h_out;
+};
+Ctx& ctx() {
+    static Ctx* c = nullptr;
+    static std::once_flag once;
+    std::call_once(once, [] {
+        int n = 0;
+        if (hipGetDeviceCount(&n) != hipSuccess || n == 0) throw std::runtime_error("no HIP device available");
+        c = new Ctx();
+        HIPCHECK(hipGetDevice(&c->device));
+        HIPCHECK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+        HIPCHECK(hipEventCreate(&c->e0)); HIPCHECK(hipEventCreate(&c->e1));
+        HIPCHECK(hipEventCreate(&c->e2)); HIPCHECK(hipEventCreate(&c->e3));
+        hipDeviceProp_t prop;
+        HIPCHECK(hipGetDeviceProperties(&prop, c->device));
+        c->cus = prop.multiProcessorCount;
+    });
+    return *c;
+}
+}  // namespace
+
+int dev_count() { int n = 0; if (hipGetDeviceCount(&n) != hipSuccess) return 0; return n; }
+std::string dev_name() {
+    Ctx& C = ctx();
+    hipDeviceProp_t p; HIPCHECK(hipGetDeviceProperties(&p, C.device));
+    return std::string(p.name) + " / " + p.gcnArchName + " / " + std::to_string(p.multiProcessorCount) + " CUs";
+}
+void dev_sync() { HIPCHECK(hipStreamSynchronize(ctx().stream)); }
+
+// ------------------------------------------------------------ device images --
+struct PlainDev {
+    const uint32_t* ku;      // d rows x 3NW: [a | b | o]
+    const uint64_t* kn;      // d kernel vectors over t terms
+    const uint32_t* soff;    // d+1 support offsets
+    const uint8_t* supp;     // kernel supports
+    const double2* a0; const double2* a1; const double2* up; const double2* down;
+    const uint32_t* diag;    // 4NW: wa | wb | o1 | o3
+    const uint32_t* mrow;    // t rows x (3NW + NWe): [a | b | o | e]
+    const uint32_t* we;      // NWe
+    uint32_t d, t, NW, NWe;
+    int32_t sigma;
+    uint32_t rbits;          // r = 2g + 2#even + #odd
+    uint64_t neg;
+};
+struct Z8Dev {
+    const uint64_t* slot_q;
+    const uint64_t* slot_n;
+    const uint32_t* slot_pin;
+    const uint32_t* slot_lam;
+    const float4* inner;     // 256 x {w_re, w_im, pin*4, kin}
+    const float* lt;         // nbytes x 256
+    const float* gin;        // 2^npin
+    uint32_t nslots, nbytes, npin, pad;
+};
+struct Z8Q {                 // one query of a batch (64 B)
+    uint64_t delta, s0, item0, nitems;
+    uint32_t eng, K0, pio0, M, H, pad0, pad1, pad2;
+};
+
+struct DevEngine {
+    std::shared_ptr<const HostEngine> host;
+    std::shared_ptr<const Z8Image> z8;
+    void* blob = nullptr;
+    size_t bytes = 0;
+    PlainDev plain{};
+    Z8Dev z{};
+    ~DevEngine() { if (blob) (void)hipFree(blob); }
+};
+
+size_t dev_engine_bytes(const DevEngine& D) { return D.bytes; }
+const HostEngine& dev_host(const DevEngine& D) { return *D.host; }
+const Z8Image& dev_z8(const DevEngine& D) { return *D.z8; }
+
+namespace {
+struct BlobBuilder {
+    std::vector<uint8_t> bytes;
+    size_t put(const void* p, size_t n) {
+        size_t off = (bytes.size() + 255) & ~size_t(255);
+        bytes.resize(off + std::max<size_t>(n, 16), 0);
+        if (n) std::memcpy(bytes.data() + off, p, n);
+        return off;
+    }
+};
+// u64 packed words -> u32 dwords padded to nw dwords
+void put_u32(std::vector<uint32_t>& dst, const uint64_t* src, uint32_t nwords64, uint32_t nbits, uint32_t nw) {
+    for (uint32_t k = 0; k < nw; ++k) {
+        uint32_t v = 0;
+        if (k / 2 < nwords64 && 32 * k < std::max(nbits, 1u)) v = (uint32_t)(src[k / 2] >> (32 * (k & 1)));
+        dst.push_back(v);
+    }
+}
+uint32_t pick_nw(uint32_t bits) {
+    uint32_t need = (bits + 31) / 32;
+    for (uint32_t nw : {2u, 4u, 8u, 16u, 32u}) if (need <= nw) return nw;
+    throw std::invalid_argument("normal form too large for the plain kernel (more than 1024 planes or lines)");
+}
+}  // namespace
+
+std::shared_ptr<DevEngine> dev_engine_upload(std::shared_ptr<const HostEngine> Ep, std::shared_ptr<const Z8Image> Zp) {
+    Ctx& C = ctx();
+    const HostEngine& E = *Ep;
+    const Z8Image& Z = *Zp;
+    const ULayout& L = E.L;
+    auto D = std::make_shared<DevEngine>();
+    D->host = Ep; D->z8 = Zp;
+    BlobBuilder B;
+    // plain image
+    const uint32_t NW = pick_nw(std::max(L.g, L.nodd)), NWe = std::max<uint32_t>(2, (L.neven + 31) / 32 + ((L.neven + 31) / 32) % 2);
+    const uint32_t W = L.words();
+    std::vector<uint32_t> ku, diag, mrow, we;
+    for (uint32_t j = 0; j < E.d; ++j) {
+        const uint64_t* r = &E.kernel_u[(size_t)j * W];
+        put_u32(ku, r + L.off_a(), L.Wg, L.g, NW); put_u32(ku, r + L.off_b(), L.Wg, L.g, NW); put_u32(ku, r + L.off_o(), L.Wo, L.nodd, NW);
+    }
+    put_u32(diag, E.diag.data() + L.off_a(), L.Wg, L.g, NW); put_u32(diag, E.diag.data() + L.off_b(), L.Wg, L.g, NW);
+    put_u32(diag, E.diag.data() + L.off_o(), L.Wo, L.nodd, NW); put_u32(diag, E.o3.data(), L.Wo, L.nodd, NW);
+    for (uint32_t k = 0; k < E.t; ++k) {
+        const uint64_t* r = &E.mcols[(size_t)k * W];
+        put_u32(mrow, r + L.off_a(), L.Wg, L.g, NW); put_u32(mrow, r + L.off_b(), L.Wg, L.g, NW);
+        put_u32(mrow, r + L.off_o(), L.Wo, L.nodd, NW); put_u32(mrow, r + L.off_e(), L.We, L.neven, NWe);
+    }
+    put_u32(we, E.diag.data() + L.off_e(), L.We, L.neven, NWe);
+    std::vector<uint32_t> soff{0};
+    std::vector<uint8_t> supp;
+    for (uint32_t j = 0; j < E.d; ++j) {
+        for (uint32_t k = 0; k < E.t; ++k) if ((E.kernel[j] >> k) & 1) supp.push_back((uint8_t)k);
+        soff.push_back((uint32_t)supp.size());
+    }
+    std::vector<double> a0, a1, up, dn;
+    for (uint32_t k = 0; k < E.t; ++k) {
+        a0.push_back(E.a0[k].real()); a0.push_back(E.a0[k].imag());
+        a1.push_back(E.a1[k].real()); a1.push_back(E.a1[k].imag());
+        up.push_back(E.up[k].real()); up.push_back(E.up[k].imag());
+        dn.push_back(E.down[k].real()); dn.push_back(E.down[k].imag());
+    }
+    size_t o_ku = B.put(ku.data(), ku.size() * 4), o_kn = B.put(E.kernel.data(), E.kernel.size() * 8);
+    size_t o_soff = B.put(soff.data(), soff.size() * 4), o_supp = B.put(supp.data(), supp.size());
+    size_t o_a0 = B.put(a0.data(), a0.size() * 8), o_a1 = B.put(a1.data(), a1.size() * 8);
+    size_t o_up = B.put(up.data(), up.size() * 8), o_dn = B.put(dn.data(), dn.size() * 8);
+    size_t o_diag = B.put(diag.data(), diag.size() * 4), o_mrow = B.put(mrow.data(), mrow.size() * 4), o_we = B.put(we.data(), we.size() * 4);
+    // Z8 image
+    std::vector<uint32_t> lam32(Z.slot_lam.begin(), Z.slot_lam.end());
+    size_t o_sq = B.put(Z.slot_q.data(), Z.slot_q.size() * 8), o_sn = B.put(Z.slot_n.data(), Z.slot_n.size() * 8);
+    size_t o_sp = B.put(Z.slot_pin.data(), Z.slot_pin.size() * 4), o_sl = B.put(lam32.data(), lam32.size() * 4);
+    size_t o_in = B.put(Z.inner.data(), Z.inner.size() * 4), o_lt = B.put(Z.lt.data(), Z.lt.size() * 4);
+    size_t o_gin = B.put(Z.gin.data(), Z.gin.size() * 4);
+
+    std::lock_guard<std::mutex> lk(C.mu);
+    HIPCHECK(hipMalloc(&D->blob, B.bytes.size()));
+    D->bytes = B.bytes.size();
+    HIPCHECK(hipMemcpyAsync(D->blob, B.bytes.data(), B.bytes.size(), hipMemcpyHostToDevice, C.stream));
+    HIPCHECK(hipStreamSynchronize(C.stream));
+    auto at = [&](size_t off) { return (const uint8_t*)D->blob + off; };
+    PlainDev& P = D->plain;
+    P.ku = (const uint32_t*)at(o_ku); P.kn = (const uint64_t*)at(o_kn); P.soff = (const uint32_t*)at(o_soff); P.supp = at(o_supp);
+    P.a0 = (const double2*)at(o_a0); P.a1 = (const double2*)at(o_a1); P.up = (const double2*)at(o_up); P.down = (const double2*)at(o_dn);
+    P.diag = (const uint32_t*)at(o_diag); P.mrow = (const uint32_t*)at(o_mrow); P.we = (const uint32_t*)at(o_we);
+    P.d = E.d; P.t = E.t; P.NW = NW; P.NWe = NWe; P.sigma = E.sigma;
+    P.rbits = 2 * L.g + 2 * L.neven + L.nodd; P.neg = E.neg_mask;
+    Z8Dev& z = D->z;
+    z.slot_q = (const uint64_t*)at(o_sq); z.slot_n = (const uint64_t*)at(o_sn); z.slot_pin = (const uint32_t*)at(o_sp); z.slot_lam = (const uint32_t*)at(o_sl);
+    z.inner = (const float4*)at(o_in); z.lt = (const float*)at(o_lt); z.gin = (const float*)at(o_gin);
+    z.nslots = Z.nslots; z.nbytes = Z.nbytes; z.npin = Z.npin; z.pad = 0;
+    return D;
+}
+
+// ============================================================= plain FP64 ===
+struct PlainQ {
+    uint32_t ustart[96];     // [a | b | o] padded to NW dwords each
+    uint64_t s0;
+    double cre, cim;
+    uint32_t L;
+};
+
+__device__ __forceinline__ void cmul(double& r, double& i, double mr, double mi) {
+    const double nr = r * mr - i * mi;
+    const double ni = r * mi + i * mr;
+    r = nr; i = ni;
+}
+__device__ __forceinline__ void plain_weight(const PlainDev& E, uint64_t sv, double& cr, double& ci) {
+    cr = 1.0; ci = 0.0;
+    for (uint32_t k = 0; k < E.t; ++k) {
+        const double2 a = ((sv >> k) & 1) ? E.a1[k] : E.a0[k];
+        cmul(cr, ci, a.x, a.y);
+    }
+}
+
+template <int NW>
+__global__ __launch_bounds__(256) void k_plain(PlainDev E, PlainQ Q, double* __restrict__ partial) {
+    __shared__ double r8[16];
+    __shared__ double red[8];
+    const uint32_t tid = threadIdx.x;
+    if (tid < 8) {
+        const double h = 0.70710678118654752440;
+        const double re[8] = {1, h, 0, -h, -1, -h, 0, h}, im[8] = {0, h, 1, h, 0, -h, -1, -h};
+        r8[2 * tid] = re[tid]; r8[2 * tid + 1] = im[tid];
+    }
+    __syncthreads();
+    const uint64_t chunk = (uint64_t)blockIdx.x * 256 + tid;
+    const uint32_t L = Q.L, dhi = E.d - L;
+    double ar = 0.0, ai = 0.0;
+    if ((chunk >> dhi) == 0) {
+        uint32_t ua[NW], ub[NW], uo[NW];
+#pragma unroll
+        for (int w = 0; w < NW; ++w) { ua[w] = Q.ustart[w]; ub[w] = Q.ustart[NW + w]; uo[w] = Q.ustart[2 * NW + w]; }
+        uint64_t sv = Q.s0;
+        double cr = Q.cre, ci = Q.cim;
+        if (chunk) {
+            for (uint32_t j = 0; j < dhi; ++j) {
+                if ((chunk >> j) & 1) {
+                    const uint32_t* row = E.ku + (size_t)(L + j) * 3 * NW;
+#pragma unroll
+                    for (int w = 0; w < NW; ++w) { ua[w] ^= row[w]; ub[w] ^= row[NW + w]; uo[w] ^= row[2 * NW + w]; }
+                    sv ^= E.kn[L + j];
+                }
+            }
+            plain_weight(E, sv, cr, ci);
+        }
+        const uint32_t steps = 1u << L;
+        for (uint32_t i = 0; i < steps; ++i) {
+            int pc1 = 0, pc3 = 0, cnt = 0;
+#pragma unroll
+            for (int w = 0; w < NW; ++w) {
+                pc1 += __popc(uo[w] & E.diag[2 * NW + w]);
+                pc3 += __popc(uo[w] & E.diag[3 * NW + w]);
+                cnt += __popc((E.diag[w] ^ ua[w]) & (E.diag[NW + w] ^ ub[w]));
+            }
+            const int n = E.sigma - 2 * (pc1 - pc3) + ((cnt & 1) << 2);
+            const int k8 = n & 7;
+            const double rr = r8[2 * k8], ri = r8[2 * k8 + 1];
+            const double tr = cr * rr - ci * ri, ti = cr * ri + ci * rr;
+            ar += tr; ai += ti;
+            if (i + 1 < steps) {
+                const uint32_t j = __builtin_ctz(i + 1);
+                const uint32_t* row = E.ku + (size_t)j * 3 * NW;
+#pragma unroll
+                for (int w = 0; w < NW; ++w) { ua[w] ^= row[w]; ub[w] ^= row[NW + w]; uo[w] ^= row[2 * NW + w]; }
+                if (((i + 1) & 2047u) == 0) {
+                    sv ^= E.kn[j];
+                    plain_weight(E, sv, cr, ci);
+                } else {
+                    for (uint32_t e = E.soff[j]; e < E.soff[j + 1]; ++e) {
+                        const uint32_t k = E.supp[e];
+                        const double2 m = ((sv >> k) & 1) ? E.down[k] : E.up[k];
+                        cmul(cr, ci, m.x, m.y);
+                        sv ^= 1ull << k;
+                    }
+                }
+            }
+        }
+    }
+    // block reduction (FP64)
+    for (int off = 32; off > 0; off >>= 1) { ar += __shfl_xor(ar, off); ai += __shfl_xor(ai, off); }
+    if ((tid & 63) == 0) { red[2 * (tid >> 6)] = ar; red[2 * (tid >> 6) + 1] = ai; }
+    __syncthreads();
+    if (tid == 0) {
+        partial[2 * blockIdx.x] = (red[0] + red[2]) + (red[4] + red[6]);
+        partial[2 * blockIdx.x + 1] = (red[1] + red[3]) + (red[5] + red[7]);
+    }
+}
+
+__global__ __launch_bounds__(256) void k_sum_pairs(const double* __restrict__ in, uint64_t n, double* __restrict__ out) {
+    __shared__ double red[8];
+    const uint32_t tid = threadIdx.x;
+    double r = 0, i = 0;
+    for (uint64_t k = tid; k < n; k += 256) { r += in[2 * k]; i += in[2 * k + 1]; }
+    for (int off = 32; off > 0; off >>= 1) { r += __shfl_xor(r, off); i += __shfl_xor(i, off); }
+    if ((tid & 63) == 0) { red[2 * (tid >> 6)] = r; red[2 * (tid >> 6) + 1] = i; }
+    __syncthreads();
+    if (tid == 0) { out[0] = (red[0] + red[2]) + (red[4] + red[6]); out[1] = (red[1] + red[3]) + (red[5] + red[7]); }
+}
+
+std::complex<double> dev_plain_sum(const DevEngine& D, const Prepared& P, double* kernel_ms) {
+    Ctx& C = ctx();
+    const PlainDev& E = D.plain;
+    const HostEngine& H = *D.host;
+    if (!P.nonzero) return {0, 0};
+    PlainQ Q{};
+    const ULayout& L = H.L;
+    std::vector<uint32_t> us;
+    put_u32(us, P.u_start.data() + L.off_a(), L.Wg, L.g, E.NW);
+    put_u32(us, P.u_start.data() + L.off_b(), L.Wg, L.g, E.NW);
+    put_u32(us, P.u_start.data() + L.off_o(), L.Wo, L.nodd, E.NW);
+    if (us.size() > 96) throw std::invalid_argument("plain kernel state above 96 dwords");
+    std::copy(us.begin(), us.end(), Q.ustart);
+    Q.s0 = P.s0; Q.cre = P.c_start.real(); Q.cim = P.c_start.imag();
+    // Gray chunk length: the reference's 2^14, shortened so that at least 2^16
+    // threads exist when d allows (one query has to fill the device alone).
+    const uint32_t d = E.d;
+    uint32_t Lb = d > 16 ? std::min<uint32_t>(14, d - 16) : 0;
+    Lb = std::max<uint32_t>(Lb, std::min<uint32_t>(d, 4));
+    Q.L = Lb;
+    const uint64_t chunks = 1ull << (d - Lb);
+    const uint64_t blocks = (chunks + 255) / 256;
+    if (blocks > (1ull << 31)) throw std::invalid_argument("branch dimension too large for the plain kernel");
+    std::lock_guard<std::mutex> lk(C.mu);
+    double* part = (double*)C.d_misc.get(blocks * 16);
+    double* res = (double*)C.d_out.get(16);
+    HIPCHECK(hipEventRecord(C.e0, C.stream));
+    switch (E.NW) {
+        case 2: hipLaunchKernelGGL(k_plain<2>, dim3((uint32_t)blocks), dim3(256), 0, C.stream, E, Q, part); break;
+        case 4: hipLaunchKernelGGL(k_plain<4>, dim3((uint32_t)blocks), dim3(256), 0, C.stream, E, Q, part); break;
+        case 8: hipLaunchKernelGGL(k_plain<8>, dim3((uint32_t)blocks), dim3(256), 0, C.stream, E, Q, part); break;
+        case 16: hipLaunchKernelGGL(k_plain<16>, dim3((uint32_t)blocks), dim3(256), 0, C.stream, E, Q, part); break;
+        default: hipLaunchKernelGGL(k_plain<32>, dim3((uint32_t)blocks), dim3(256), 0, C.stream, E, Q, part); break;
+    }
+    HIPCHECK(hipGetLastError());
+    HIPCHECK(hipEventRecord(C.e1, C.stream));
+    hipLaunchKernelGGL(k_sum_pairs, dim3(1), dim3(256), 0, C.stream, part, blocks, res);
+    HIPCHECK(hipGetLastError());
+    double h[2];
+    HIPCHECK(hipMemcpyAsync(h, res, 16, hipMemcpyDeviceToHost, C.stream));
+    HIPCHECK(hipStreamSynchronize(C.stream));
+    if (kernel_ms) { float ms; HIPCHECK(hipEventElapsedTime(&ms, C.e0, C.e1)); *kernel_ms = ms; }
+    return {h[0], h[1]};
+}
+
+// ============================================================== Z8 walk ====
+__device__ __forceinline__ void two_sum(float& s, float& c, float x) {
+    const float t = s + x;
+    const float bp = t - s;
+    const float e = (s - (t - bp)) + (x - bp);
+    s = t;
+    c += e;
+}
+__device__ __forceinline__ void merge2(float& s, float& c, float s2, float c2) {
+    const float t = s + s2;
+    const float bp = t - s;
+    const float e = (s - (t - bp)) + (s2 - bp);
+    s = t;
+    c = (c + c2) + e;
+}
+
+// Rotate (xr, xi) by i^((K>>1)&3) and add into the even or odd accumulator.
+__device__ __forceinline__ void fold(float xr, float xi, uint32_t K, float (&acc)[8]) {
+    const uint32_t r2 = (K >> 1) & 3u;
+    const float a = (r2 & 1u) ? xi : xr;
+    const float b = (r2 & 1u) ? xr : xi;
+    const float yr = (r2 == 1u || r2 == 2u) ? -a : a;
+    const float yi = (r2 >= 2u) ? -b : b;
+    if (K & 1u) { two_sum(acc[4], acc[5], yr); two_sum(acc[6], acc[7], yi); }
+    else { two_sum(acc[0], acc[1], yr); two_sum(acc[2], acc[3], yi); }
+}
+
+// Lane state over the outer coordinates: integer phase K at c_in = 0, the
+// inner sign mask hs (cross terms with the outer point, xor the query bits of
+// the inner slots), inner-touched pivots pio, branch vector s at c_in = 0.
+struct LaneState {
+    uint64_t cout, sout;
+    uint32_t K, hs, pio;
+};
+__device__ __forceinline__ void seed_lane(const Z8Dev& E, const Z8Q& Q, uint64_t lane, LaneState& S) {
+    S.cout = 0; S.sout = Q.s0; S.K = Q.K0; S.hs = (uint32_t)Q.delta & 0xFFu; S.pio = Q.pio0;
+    const uint32_t base = 8u + Q.M;
+    for (uint32_t b = 0; b < Q.H; ++b) {
+        const uint32_t p = base + b;
+        const uint64_t qp = E.slot_q[p];
+        const uint32_t ell = E.slot_lam[p] + (((uint32_t)(Q.delta >> p) & 1u) << 2);
+        const uint32_t pin = E.slot_pin[p];
+        const uint64_t np = E.slot_n[p];
+        if ((lane >> b) & 1) {
+            S.K += ell + ((__popcll(qp & S.cout) & 1u) << 2);
+            S.cout |= 1ull << p;
+            S.hs ^= (uint32_t)qp & 0xFFu;
+            S.pio ^= pin;
+            S.sout ^= np;
+        }
+    }
+}
+__device__ __forceinline__ void flip_mid(const Z8Dev& E, const Z8Q& Q, uint32_t p, LaneState& S) {
+    const uint64_t qp = E.slot_q[p];
+    const uint32_t ell = E.slot_lam[p] + (((uint32_t)(Q.delta >> p) & 1u) << 2);
+    const uint32_t par = (__popcll(qp & S.cout) & 1u) << 2;
+    const bool was = (S.cout >> p) & 1;
+    S.K += (was ? (8u - ell) : ell) + par;
+    S.cout ^= 1ull << p;
+    S.hs ^= (uint32_t)qp & 0xFFu;
+    S.pio ^= E.slot_pin[p];
+    S.sout ^= E.slot_n[p];
+}
+__device__ __forceinline__ float lane_factor(const float* s_lt, uint32_t nbytes, uint64_t sout) {
+    float F = 1.f;
+    for (uint32_t b = 0; b < nbytes; ++b) F *= s_lt[(b << 8) | (uint32_t)((sout >> (8 * b)) & 0xFFu)];
+    return F;
+}
+
+// The 256 inner branches of one outer point, folded every FOLD steps.
+template <bool HASG, int FOLD>
+__device__ __forceinline__ void inner_walk(const float4* __restrict__ W, const float* s_gin, const LaneState& S, float F, float (&acc)[8]) {
+    uint32_t sm[8];
+#pragma unroll
+    for (int j = 0; j < 8; ++j) sm[j] = (S.hs << (31 - j)) & 0x80000000u;
+    uint32_t sgn = 0;
+    const uint32_t pio4 = S.pio << 2;
+#pragma unroll
+    for (int blk = 0; blk < 256 / FOLD; ++blk) {
+        float tr = 0.f, ti = 0.f;
+#pragma unroll
+        for (int ii = 0; ii < FOLD; ++ii) {
+            const int i = blk * FOLD + ii;
+            if (i) sgn ^= sm[__builtin_ctz(i)];
+            const float4 w = W[i];
+            float m;
+            if constexpr (HASG) {
+                const float g = *(const float*)((const char*)s_gin + (pio4 ^ __float_as_uint(w.z)));
+                m = __uint_as_float(__float_as_uint(g) ^ sgn);
+            } else {
+                m = __uint_as_float(0x3f800000u ^ sgn);
+            }
+            tr = fmaf(m, w.x, tr);
+            ti = fmaf(m, w.y, ti);
+        }
+        fold(tr * F, ti * F, S.K, acc);
+    }
+}
+
+template <int FOLD>
+__global__ __launch_bounds__(256) void k_z8_walk(const Z8Dev* __restrict__ engs, const Z8Q* __restrict__ qs, uint32_t nq, uint64_t total,
+                                                 unsigned long long* __restrict__ counter, double* __restrict__ partial) {
+    __shared__ float s_lt[8 * 256];
+    __shared__ float s_gin[4096];
+    __shared__ unsigned long long s_item;
+    __shared__ int s_eng;
+    __shared__ float s_red[4][8];
+    const uint32_t tid = threadIdx.x;
+    if (tid == 0) s_eng = -1;
+    for (;;) {
+        __syncthreads();
+        if (tid == 0) s_item = atomicAdd(counter, 1ull);
+        __syncthreads();
+        const uint64_t item = s_item;
+        if (item >= total) break;
+        uint32_t lo = 0, hi = nq;
+        while (hi - lo > 1) { const uint32_t mid = (lo + hi) >> 1; if (qs[mid].item0 <= item) lo = mid; else hi = mid; }
+        const Z8Q Q = qs[lo];
+        const Z8Dev E = engs[Q.eng];
+        if ((int)Q.eng != s_eng) {
+            for (uint32_t k = tid; k < E.nbytes * 256u; k += 256) s_lt[k] = E.lt[k];
+            if (E.npin) for (uint32_t k = tid; k < (1u << E.npin); k += 256) s_gin[k] = E.gin[k];
+            __syncthreads();
+            if (tid == 0) s_eng = (int)Q.eng;
+        }
+        float acc[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+        const uint64_t lane = (item - Q.item0) * 256 + tid;
+        if ((lane >> Q.H) == 0) {
+            LaneState S;
+            seed_lane(E, Q, lane, S);
+            const uint32_t nout = 1u << Q.M;
+            for (uint32_t o = 0; o < nout; ++o) {
+                if (o) flip_mid(E, Q, 8u + (uint32_t)__builtin_ctz(o), S);
+                const float F = lane_factor(s_lt, E.nbytes, S.sout);
+                if (E.npin) inner_walk<true, FOLD>(E.inner, s_gin, S, F, acc);
+                else inner_walk<false, FOLD>(E.inner, s_gin, S, F, acc);
+            }
+        }
+        // workgroup reduction of the four compensated pairs (FP32 two-sum)
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) {
+#pragma unroll
+            for (int k = 0; k < 4; ++k) {
+                const float s2 = __shfl_xor(acc[2 * k], off), c2 = __shfl_xor(acc[2 * k + 1], off);
+                merge2(acc[2 * k], acc[2 * k + 1], s2, c2);
+            }
+        }
+        if ((tid & 63) == 0) {
+#pragma unroll
+            for (int k = 0; k < 8; ++k) s_red[tid >> 6][k] = acc[k];
+        }
+        __syncthreads();
+        if (tid == 0) {
+#pragma unroll
+            for (int k = 0; k < 4; ++k) {
+                float s = s_red[0][2 * k], c = s_red[0][2 * k + 1];
+                for (int w = 1; w < 4; ++w) merge2(s, c, s_red[w][2 * k], s_red[w][2 * k + 1]);
+                partial[4 * item + k] = (double)s + (double)c;
+            }
+        }
+    }
+}
+
+// FP64 cross-workgroup reduction (fixed order) and global scaling per query.
+__global__ __launch_bounds__(256) void k_z8_reduce(const Z8Q* __restrict__ qs, const double* __restrict__ partial, const double2* __restrict__ scale,
+                                                   const uint32_t* __restrict__ order, double* __restrict__ out) {
+    __shared__ double red[4][4];
+    const uint32_t qi = blockIdx.x, tid = threadIdx.x;
+    const Z8Q Q = qs[qi];
+    double v[4] = {0, 0, 0, 0};
+    for (uint64_t k = tid; k < Q.nitems; k += 256) {
+#pragma unroll
+        for (int c = 0; c < 4; ++c) v[c] += partial[4 * (Q.item0 + k) + c];
+    }
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+#pragma unroll
+        for (int c = 0; c < 4; ++c) v[c] += __shfl_xor(v[c], off);
+    }
+    if ((tid & 63) == 0) {
+#pragma unroll
+        for (int c = 0; c < 4; ++c) red[tid >> 6][c] = v[c];
+    }
+    __syncthreads();
+    if (tid == 0) {
+        double s[4];
+        for (int c = 0; c < 4; ++c) s[c] = (red[0][c] + red[1][c]) + (red[2][c] + red[3][c]);
+        const double h = 0.70710678118654752440;
+        // EV + omega * OD
+        const double re = s[0] + h * (s[2] - s[3]);
+        const double im = s[1] + h * (s[2] + s[3]);
+        const double2 sc = scale[qi];
+        const uint32_t dst = order[qi];
+        out[2 * dst] = sc.x * re - sc.y * im;
+        out[2 * dst + 1] = sc.x * im + sc.y * re;
+    }
+}
+
+__global__ void k_zero_u64(unsigned long long* p) { *p = 0ull; }
+
+uint32_t z8_auto_mid_bits(uint32_t d, uint64_t batch_branches) {
+    // Lanes walk 2^(8+M) branches; aim for at least ~8 items per CU over the
+    // whole batch while keeping M <= 6 so a lane's walk stays a few
+    // microseconds long.
+    if (d <= 8) return 0;
+    const double target_items = 256.0 * 8.0;
+    uint32_t best = 0;
+    for (uint32_t M = 0; M <= 6 && 8 + M < d; ++M) {
+        const double items = (double)batch_branches / std::pow(2.0, 16.0 + M);
+        if (items >= target_items) best = M;
+    }
+    return std::min(best, d - 8);
+}
+
+namespace {
+struct BatchPlan {
+    std::vector<Z8Dev> engs;
+    std::vector<Z8Q> qs;
+    std::vector<double2> scale;
+    std::vector<uint32_t> order;
+    uint64_t total_items = 0, branches = 0;
+};
+BatchPlan plan_batch(const std::vector<Z8Job>& jobs) {
+    BatchPlan B;
+    std::vector<const DevEngine*> seen;
+    std::vector<uint32_t> idx(jobs.size());
+    std::iota(idx.begin(), idx.end(), 0u);
+    // heavy items first; jobs of one engine stay together
+    std::stable_sort(idx.begin(), idx.end(), [&](uint32_t a, uint32_t b) {
+        if (jobs[a].mid_bits != jobs[b].mid_bits) return jobs[a].mid_bits > jobs[b].mid_bits;
+        return jobs[a].eng < jobs[b].eng;
+    });
+    for (uint32_t ji : idx) {
+        const Z8Job& J = jobs[ji];
+        const Z8Image& Z = *J.eng->z8;
+        auto it = std::find(seen.begin(), seen.end(), J.eng);
+        uint32_t e;
+        if (it == seen.end()) { e = (uint32_t)seen.size(); seen.push_back(J.eng); B.engs.push_back(J.eng->z); }
+        else e = (uint32_t)(it - seen.begin());
+        Z8Q q{};
+        q.delta = J.q.delta; q.s0 = J.q.s0; q.eng = e; q.K0 = J.q.K0; q.pio0 = J.q.pio0;
+        const uint32_t nout = Z.nslots - 8;
+        q.M = std::min(J.mid_bits, nout);
+        q.H = nout - q.M;
+        if (q.H > 40) throw std::invalid_argument("branch dimension too large for one batch");
+        q.item0 = B.total_items;
+        q.nitems = ((1ull << q.H) + 255) / 256;
+        B.total_items += q.nitems;
+        B.branches += 1ull << Z.d;
+        B.qs.push_back(q);
+        B.scale.push_back(make_double2(J.scale.real(), J.scale.imag()));
+        B.order.push_back(ji);
+    }
+    return B;
+}
+template <int FOLD>
+uint32_t walk_grid(Ctx& C, uint64_t items) {
+    int per_cu = 0;
+    HIPCHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_z8_walk<FOLD>, 256, 0));
+    uint64_t g = (uint64_t)std::max(per_cu, 1) * (uint64_t)C.cus;
+    return (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(g, items));
+}
+void launch_walk(Ctx& C, bool accurate, uint32_t grid, const Z8Dev* de, const Z8Q* dq, uint32_t nq, uint64_t items, unsigned long long* ctr, double* part) {
+    if (accurate) hipLaunchKernelGGL(k_z8_walk<32>, dim3(grid), dim3(256), 0, C.stream, de, dq, nq, items, ctr, part);
+    else hipLaunchKernelGGL(k_z8_walk<256>, dim3(grid), dim3(256), 0, C.stream, de, dq, nq, items, ctr, part);
+    HIPCHECK(hipGetLastError());
+}
+}  // namespace
+
+void dev_z8_batch(const std::vector<Z8Job>& jobs, std::vector<std::complex<double>>& out, Z8Timing* timing, bool accurate) {
+    out.assign(jobs.size(), {0, 0});
+    if (jobs.empty()) return;
+    Ctx& C = ctx();
+    BatchPlan B = plan_batch(jobs);
+    const uint32_t nq = (uint32_t)B.qs.size();
+    std::lock_guard<std::mutex> lk(C.mu);
+    // one pinned staging buffer: engines | queries | scales | order
+    const size_t be = B.engs.size() * sizeof(Z8Dev), bq = nq * sizeof(Z8Q), bs = nq * sizeof(double2), bo = nq * 4;
+    const size_t oq = (be + 255) & ~size_t(255), os = oq + ((bq + 255) & ~size_t(255)), oo = os + ((bs + 255) & ~size_t(255));
+    const size_t tot = oo + bo;
+    uint8_t* h = (uint8_t*)C.h_in.get(tot);
+    std::memcpy(h, B.engs.data(), be); std::memcpy(h + oq, B.qs.data(), bq); std::memcpy(h + os, B.scale.data(), bs); std::memcpy(h + oo, B.order.data(), bo);
+    uint8_t* d = (uint8_t*)C.d_qs.get(tot);
+    unsigned long long* ctr = (unsigned long long*)C.d_counter.get(64);
+    double* part = (double*)C.d_partial.get(B.total_items * 32);
+    double* res = (double*)C.d_out.get(nq * 16);
+    double* hres = (double*)C.h_out.get(nq * 16);
+    const uint32_t grid = accurate ? walk_grid<32>(C, B.total_items) : walk_grid<256>(C, B.total_items);
+    HIPCHECK(hipEventRecord(C.e0, C.stream));
+    HIPCHECK(hipMemcpyAsync(d, h, tot, hipMemcpyHostToDevice, C.stream));
+    hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(1), 0, C.stream, ctr);
+    HIPCHECK(hipEventRecord(C.e1, C.stream));
+    launch_walk(C, accurate, grid, (const Z8Dev*)d, (const Z8Q*)(d + oq), nq, B.total_items, ctr, part);
+    HIPCHECK(hipEventRecord(C.e2, C.stream));
+    hipLaunchKernelGGL(k_z8_reduce, dim3(nq), dim3(256), 0, C.stream, (const Z8Q*)(d + oq), part, (const double2*)(d + os), (const uint32_t*)(d + oo), res);
+    HIPCHECK(hipGetLastError());
+    HIPCHECK(hipMemcpyAsync(hres, res, nq * 16, hipMemcpyDeviceToHost, C.stream));
+    HIPCHECK(hipEventRecord(C.e3, C.stream));
+    HIPCHECK(hipStreamSynchronize(C.stream));
+    for (uint32_t i = 0; i < nq; ++i) out[i] = {hres[2 * i], hres[2 * i + 1]};
+    if (timing) {
+        float k, t;
+        HIPCHECK(hipEventElapsedTime(&k, C.e1, C.e2)); HIPCHECK(hipEventElapsedTime(&t, C.e0, C.e3));
+        timing->kernel_ms = k; timing->total_ms = t; timing->branches = B.branches; timing->items = B.total_items; timing->grid = grid;
+    }
+}
+
+// ------------------------------------------------------------- hipGraph ----
+struct Z8Graph {
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    void *h_in = nullptr, *h_out = nullptr, *d_in = nullptr, *d_part = nullptr, *d_res = nullptr, *d_ctr = nullptr;
+    uint32_t nq = 0, grid = 0;
+    uint64_t items = 0, branches = 0;
+    hipEvent_t k0 = nullptr, k1 = nullptr;
+    ~Z8Graph() {
+        if (exec) (void)hipGraphExecDestroy(exec);
+        if (graph) (void)hipGraphDestroy(graph);
+        if (h_in) (void)hipHostFree(h_in);
+        if (h_out) (void)hipHostFree(h_out);
+        for (void* p : {d_in, d_part, d_res, d_ctr}) if (p) (void)hipFree(p);
+        if (k0) (void)hipEventDestroy(k0);
+        if (k1) (void)hipEventDestroy(k1);
+    }
+};
+
+std::shared_ptr<Z8Graph> dev_z8_graph(const std::vector<Z8Job>& jobs, bool accurate) {
+    Ctx& C = ctx();
+    auto G = std::make_shared<Z8Graph>();
+    BatchPlan B = plan_batch(jobs);
+    const uint32_t nq = (uint32_t)B.qs.size();
+    G->nq = nq; G->items = B.total_items; G->branches = B.branches;
+    const size_t be = B.engs.size() * sizeof(Z8Dev), bq = nq * sizeof(Z8Q), bs = nq * sizeof(double2), bo = nq * 4;
+    const size_t oq = (be + 255) & ~size_t(255), os = oq + ((bq + 255) & ~size_t(255)), oo = os + ((bs + 255) & ~size_t(255));
+    const size_t tot = oo + bo;
+    std::lock_guard<std::mutex> lk(C.mu);
+    HIPCHECK(hipHostMalloc(&G->h_in, tot, hipHostMallocDefault));
+    HIPCHECK(hipHostMalloc(&G->h_out, nq * 16, hipHostMallocDefault));
+    HIPCHECK(hipMalloc(&G->d_in, tot));
+    HIPCHECK(hipMalloc(&G->d_part, std::max<uint64_t>(B.total_items * 32, 32)));
+    HIPCHECK(hipMalloc(&G->d_res, nq * 16));
+    HIPCHECK(hipMalloc(&G->d_ctr, 64));
+    HIPCHECK(hipEventCreate(&G->k0)); HIPCHECK(hipEventCreate(&G->k1));
+    uint8_t* h = (uint8_t*)G->h_in;
+    std::memcpy(h, B.engs.data(), be); std::memcpy(h + oq, B.qs.data(), bq); std::memcpy(h + os, B.scale.data(), bs); std::memcpy(h + oo, B.order.data(), bo);
+    G->grid = accurate ? walk_grid<32>(C, B.total_items) : walk_grid<256>(C, B.total_items);
+    uint8_t* d = (uint8_t*)G->d_in;
+    HIPCHECK(hipStreamBeginCapture(C.stream, hipStreamCaptureModeThreadLocal));
+    HIPCHECK(hipMemcpyAsync(d, h, tot, hipMemcpyHostToDevice, C.stream));
+    hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(1), 0, C.stream, (unsigned long long*)G->d_ctr);   // a kernel, not a memset node
+    launch_walk(C, accurate, G->grid, (const Z8Dev*)d, (const Z8Q*)(d + oq), nq, B.total_items, (unsigned long long*)G->d_ctr, (double*)G->d_part);
+    hipLaunchKernelGGL(k_z8_reduce, dim3(nq), dim3(256), 0, C.stream, (const Z8Q*)(d + oq), (const double*)G->d_part, (const double2*)(d + os), (const uint32_t*)(d + oo), (double*)G->d_res);
+    HIPCHECK(hipMemcpyAsync(G->h_out, G->d_res, nq * 16, hipMemcpyDeviceToHost, C.stream));
+    HIPCHECK(hipStreamEndCapture(C.stream, &G->graph));
+    HIPCHECK(hipGraphInstantiate(&G->exec, G->graph, nullptr, nullptr, 0));
+    return G;
+}
+
+void dev_z8_graph_run(Z8Graph& G, std::vector<std::complex<double>>& out, Z8Timing* timing) {
+    Ctx& C = ctx();
+    std::lock_guard<std::mutex> lk(C.mu);
+    HIPCHECK(hipEventRecord(G.k0, C.stream));
+    HIPCHECK(hipGraphLaunch(G.exec, C.stream));
+    HIPCHECK(hipEventRecord(G.k1, C.stream));
+    HIPCHECK(hipStreamSynchronize(C.stream));
+    const double* r = (const double*)G.h_out;
+    out.resize(G.nq);
+    for (uint32_t i = 0; i < G.nq; ++i) out[i] = {r[2 * i], r[2 * i + 1]};
+    if (timing) {
+        float ms; HIPCHECK(hipEventElapsedTime(&ms, G.k0, G.k1));
+        timing->total_ms = ms; timing->kernel_ms = 0; timing->branches = G.branches; timing->items = G.items; timing->grid = G.grid;
+    }
+}
+
+// ================================================================= dumps ====
+__global__ __launch_bounds__(256) void k_dump_s(PlainDev E, const uint32_t* __restrict__ u0, const uint64_t* __restrict__ s, uint64_t n,
+                                                uint8_t* __restrict__ zero, uint8_t* __restrict__ k8, uint8_t* __restrict__ ktot) {
+    const uint64_t i = (uint64_t)blockIdx.x * 256 + threadIdx.x;
+    if (i >= n) return;
+    const uint32_t NW = E.NW, NWe = E.NWe, RW = 3 * NW + NWe;
+    const uint64_t sv = s[i];
+    int pc1 = 0, pc3 = 0, cnt = 0;
+    bool z = false;
+    for (uint32_t w = 0; w < NW; ++w) {
+        uint32_t a = u0[w], b = u0[NW + w], o = u0[2 * NW + w];
+        for (uint32_t k = 0; k < E.t; ++k) if ((sv >> k) & 1) { const uint32_t* r = E.mrow + (size_t)k * RW; a ^= r[w]; b ^= r[NW + w]; o ^= r[2 * NW + w]; }
+        pc1 += __popc(o & E.diag[2 * NW + w]); pc3 += __popc(o & E.diag[3 * NW + w]);
+        cnt += __popc((E.diag[w] ^ a) & (E.diag[NW + w] ^ b));
+    }
+    for (uint32_t w = 0; w < NWe; ++w) {
+        uint32_t e = u0[3 * NW + w];
+        for (uint32_t k = 0; k < E.t; ++k) if ((sv >> k) & 1) e ^= E.mrow[(size_t)k * RW + 3 * NW + w];
+        if (e != E.we[w]) z = true;
+    }
+    const int nn = E.sigma - 2 * (pc1 - pc3) + ((cnt & 1) << 2);
+    const uint32_t g = (uint32_t)(nn & 7);
+    zero[i] = z ? 1 : 0;
+    k8[i] = z ? 0 : (uint8_t)g;
+    ktot[i] = z ? 0 : (uint8_t)((g + 6u * (uint32_t)__popcll(sv) + 4u * (uint32_t)__popcll(E.neg & ~sv)) & 7u);
+}
+
+void dev_dump_s(const DevEngine& D, const Prepared& P, const uint64_t* s, size_t n, uint8_t* zero, uint8_t* k8, uint8_t* ktot) {
+    Ctx& C = ctx();
+    const HostEngine& H = *D.host;
+    const ULayout& L = H.L;
+    std::vector<uint32_t> u0;
+    put_u32(u0, P.u0.data() + L.off_a(), L.Wg, L.g, D.plain.NW); put_u32(u0, P.u0.data() + L.off_b(), L.Wg, L.g, D.plain.NW);
+    put_u32(u0, P.u0.data() + L.off_o(), L.Wo, L.nodd, D.plain.NW); put_u32(u0, P.u0.data() + L.off_e(), L.We, L.neven, D.plain.NWe);
+    std::lock_guard<std::mutex> lk(C.mu);
+    uint32_t* du = (uint32_t*)C.d_misc.get(u0.size() * 4 + 256);
+    uint64_t* ds = (uint64_t*)C.d_misc2.get(n * 8 + 256);
+    uint8_t* dout = (uint8_t*)C.d_misc3.get(3 * n + 256);
+    HIPCHECK(hipMemcpyAsync(du, u0.data(), u0.size() * 4, hipMemcpyHostToDevice, C.stream));
+    HIPCHECK(hipMemcpyAsync(ds, s, n * 8, hipMemcpyHostToDevice, C.stream));
+    if (n) hipLaunchKernelGGL(k_dump_s, dim3((uint32_t)((n + 255) / 256)), dim3(256), 0, C.stream, D.plain, du, ds, (uint64_t)n, dout, dout + n, dout + 2 * n);
+    HIPCHECK(hipGetLastError());
+    HIPCHECK(hipMemcpyAsync(zero, dout, n, hipMemcpyDeviceToHost, C.stream));
+    HIPCHECK(hipMemcpyAsync(k8, dout + n, n, hipMemcpyDeviceToHost, C.stream));
+    HIPCHECK(hipMemcpyAsync(ktot, dout + 2 * n, n, hipMemcpyDeviceToHost, C.stream));
+    HIPCHECK(hipStreamSynchronize(C.stream));
+}
+
+// Every branch of the walk for one query, with the walk's own lane state.
+__global__ __launch_bounds__(256) void k_dump_walk(Z8Dev E, Z8Q Q, uint8_t* __restrict__ kout, float* __restrict__ mout) {
+    __shared__ float s_lt[8 * 256];
+    __shared__ float s_gin[4096];
+    const uint32_t tid = threadIdx.x;
+    for (uint32_t k = tid; k < E.nbytes * 256u; k += 256) s_lt[k] = E.lt[k];
+    if (E.npin) for (uint32_t k = tid; k < (1u << E.npin); k += 256) s_gin[k] = E.gin[k];
+    __syncthreads();
+    const uint64_t lane = (uint64_t)blockIdx.x * 256 + tid;
+    if ((lane >> Q.H) != 0) return;
+    LaneState S;
+    seed_lane(E, Q, lane, S);
+    const uint32_t nout = 1u << Q.M;
+    for (uint32_t o = 0; o < nout; ++o) {
+        if (o) flip_mid(E, Q, 8u + (uint32_t)__builtin_ctz(o), S);
+        const float F = lane_factor(s_lt, E.nbytes, S.sout);
+        for (uint32_t i = 0; i < 256; ++i) {
+            const uint32_t g = i ^ (i >> 1);
+            const float4 w = E.inner[i];
+            const uint64_t idx = (((lane << Q.M) + o) << 8) + i;
+            if (w.x == 0.f && w.y == 0.f) { kout[idx] = 255; mout[idx] = 0.f; continue; }
+            const uint32_t par = (uint32_t)__popc(g & S.hs) & 1u;
+            const uint32_t k = (S.K + __float_as_uint(w.w) + 4u * par) & 7u;
+            const float gi = E.npin ? s_gin[(S.pio ^ (__float_as_uint(w.z) >> 2))] : 1.f;
+            kout[idx] = (uint8_t)k;
+            mout[idx] = F * gi * sqrtf(w.x * w.x + w.y * w.y);
+        }
+    }
+}
+
+void dev_dump_walk(const DevEngine& D, const Z8QueryHost& qh, uint32_t mid_bits, uint8_t* k, float* mag) {
+    Ctx& C = ctx();
+    const Z8Image& Z = *D.z8;
+    Z8Q q{};
+    q.delta = qh.delta; q.s0 = qh.s0; q.K0 = qh.K0; q.pio0 = qh.pio0;
+    const uint32_t nout = Z.nslots - 8;
+    q.M = std::min(mid_bits, nout); q.H = nout - q.M;
+    if (8 + q.M + q.H > 26) throw std::invalid_argument("dump_walk is limited to 2^26 branches");
+    const uint64_t n = 1ull << (8 + q.M + q.H);
+    std::lock_guard<std::mutex> lk(C.mu);
+    uint8_t* dk = (uint8_t*)C.d_misc.get(n + 256);
+    float* dm = (float*)C.d_misc2.get(n * 4 + 256);
+    const uint64_t lanes = 1ull << q.H;
+    hipLaunchKernelGGL(k_dump_walk, dim3((uint32_t)((lanes + 255) / 256)), dim3(256), 0, C.stream, D.z, q, dk, dm);
+    HIPCHECK(hipGetLastError());
+    HIPCHECK(hipMemcpyAsync(k, dk, n, hipMemcpyDeviceToHost, C.stream));
+    HIPCHECK(hipMemcpyAsync(mag, dm, n * 4, hipMemcpyDeviceToHost, C.stream));
+    HIPCHECK(hipStreamSynchronize(C.stream));
+}
+
+}  // namespace pq

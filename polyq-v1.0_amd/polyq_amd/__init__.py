@@ -1,0 +1,309 @@
+"""polyq_amd: MI355X branch evaluator for Clifford+T path sums.
+
+Python mirror of the reference's Python surface (src/pybind.rs:15-234):
+``Circuit`` builders, ``Simulator(circuit, strategy="auto"|"cached", pairs=None)``
+with ``amplitude(input, output) -> (re, im)``, ``probability``, ``marginal`` and
+``stats()``.  Every evaluation runs on the GPU through the C ABI of
+``libpolyq_amd.so`` (include/polyq_amd.h); there is no CPU fallback, and a
+missing library or device raises.
+"""
+import ctypes
+import math
+import os
+
+import numpy as np
+
+__all__ = ["Circuit", "Simulator", "Graph", "batch_amplitudes", "rng_u64", "Phase", "lib", "last_timing", "device_info"]
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB_PATH = os.path.join(_HERE, "libpolyq_amd.so")
+
+if not os.path.exists(_LIB_PATH):
+    raise ImportError("libpolyq_amd.so is not built: run polyq-v1.0_amd/build_lib.py (or __graft_entry__.build())")
+lib = ctypes.CDLL(_LIB_PATH)
+
+_u8p = ctypes.POINTER(ctypes.c_uint8)
+_u32p = ctypes.POINTER(ctypes.c_uint32)
+_u64p = ctypes.POINTER(ctypes.c_uint64)
+_f64p = ctypes.POINTER(ctypes.c_double)
+_f32p = ctypes.POINTER(ctypes.c_float)
+_sim_p = ctypes.c_void_p
+
+lib.pq_last_error.restype = ctypes.c_char_p
+lib.pq_sim_create.argtypes = [ctypes.c_uint32, ctypes.c_size_t, _u8p, _u32p, _u32p, _u32p, ctypes.c_int, ctypes.c_uint32, ctypes.POINTER(_sim_p)]
+lib.pq_sim_destroy.argtypes = [_sim_p]
+lib.pq_sim_destroy.restype = None
+lib.pq_sim_stats.argtypes = [_sim_p, _u64p]
+lib.pq_sim_num_qubits.argtypes = [_sim_p]
+lib.pq_sim_num_qubits.restype = ctypes.c_uint32
+lib.pq_sim_amplitude_index.argtypes = [_sim_p, ctypes.c_uint64, ctypes.c_uint64, ctypes.c_int, _f64p]
+lib.pq_sim_probability.argtypes = [_sim_p, ctypes.c_uint64, ctypes.c_uint64, ctypes.c_int, _f64p]
+lib.pq_sim_marginal_probability.argtypes = [_sim_p, ctypes.c_uint64, _u32p, _u8p, ctypes.c_size_t, ctypes.c_int, _f64p]
+lib.pq_sim_amplitudes.argtypes = [_sim_p, ctypes.c_size_t, _u64p, _u64p, ctypes.c_int, _f64p]
+lib.pq_batch_amplitudes.argtypes = [ctypes.c_size_t, ctypes.POINTER(_sim_p), _u64p, _u64p, ctypes.c_int, _f64p]
+lib.pq_last_timing.argtypes = [_f64p]
+lib.pq_graph_create.argtypes = [ctypes.c_size_t, ctypes.POINTER(_sim_p), _u64p, _u64p, ctypes.c_int, ctypes.POINTER(ctypes.c_void_p)]
+lib.pq_graph_run.argtypes = [ctypes.c_void_p, _f64p, _f64p]
+lib.pq_graph_destroy.argtypes = [ctypes.c_void_p]
+lib.pq_graph_destroy.restype = None
+lib.pq_sim_branch_dump.argtypes = [_sim_p, ctypes.c_uint64, ctypes.c_uint64, ctypes.c_size_t, _u64p, _u8p, _u8p, _u8p, _u32p]
+lib.pq_sim_walk_dump.argtypes = [_sim_p, ctypes.c_uint64, ctypes.c_uint64, ctypes.c_uint32, ctypes.c_size_t, _u8p, _f32p, _u64p, ctypes.POINTER(ctypes.c_size_t)]
+lib.pq_sim_selfcheck.argtypes = [_sim_p, ctypes.c_uint64, ctypes.c_uint64, ctypes.c_uint32, ctypes.c_uint64, _u64p]
+lib.pq_sim_z8_info.argtypes = [_sim_p, _u32p]
+lib.pq_device_info.argtypes = [ctypes.c_char_p, ctypes.c_size_t]
+
+FORMS = {"z8": 0, "z8_accurate": 1, "plain_fp64": 2}
+STRATEGIES = {"auto": 0, "cached": 1}
+
+# Gate codes (src/circuit.rs:25-42)
+H, X, Y, Z, S, SDG, T, TDG, P, CZ, CX, SWAP, GPHASE = range(13)
+
+
+class PolyqError(RuntimeError):
+    pass
+
+
+def _check(rc):
+    if rc < 0:
+        raise PolyqError(lib.pq_last_error().decode())
+    return rc
+
+
+def _ptr(a, t):
+    return a.ctypes.data_as(t)
+
+
+class Phase:
+    """Angle 2*pi*k/2^32 (src/phase.rs:22-52)."""
+
+    ZERO, PI, HALF_PI, QUARTER_PI = 0, 1 << 31, 1 << 30, 1 << 29
+
+    @staticmethod
+    def from_radians(r):
+        turns = r / (2.0 * math.pi)
+        frac = turns - math.floor(turns)
+        return int(round(frac * 4294967296.0)) & 0xFFFFFFFF
+
+    @staticmethod
+    def from_pi_fraction(num, log2_den):
+        assert log2_den <= 31
+        return (num << (31 - log2_den)) % (1 << 32)
+
+
+class Circuit:
+    """Gate list with the reference's builders (src/circuit.rs:97-216)."""
+
+    def __init__(self, num_qubits=0):
+        self.num_qubits = num_qubits
+        self.ops, self.q0, self.q1, self.ph = [], [], [], []
+
+    def _push(self, op, a, b=0, ph=0):
+        if op != GPHASE:
+            self.num_qubits = max(self.num_qubits, a + 1)
+        if op in (CZ, CX, SWAP):
+            self.num_qubits = max(self.num_qubits, b + 1)
+        self.ops.append(op); self.q0.append(a); self.q1.append(b); self.ph.append(ph & 0xFFFFFFFF)
+        return self
+
+    def h(self, q): return self._push(H, q)
+    def x(self, q): return self._push(X, q)
+    def y(self, q): return self._push(Y, q)
+    def z(self, q): return self._push(Z, q)
+    def s(self, q): return self._push(S, q)
+    def sdg(self, q): return self._push(SDG, q)
+    def t(self, q): return self._push(T, q)
+    def tdg(self, q): return self._push(TDG, q)
+    def cz(self, a, b): return self._push(CZ, a, b)
+    def cx(self, c, t): return self._push(CX, c, t)
+    def swap(self, a, b): return self._push(SWAP, a, b)
+    def gphase(self, phase): return self._push(GPHASE, 0, 0, phase)
+
+    def p(self, q, theta):
+        """diag(1, e^{i theta}), theta in radians rounded to the 2^32 grid (pybind.rs:83-86)."""
+        return self._push(P, q, 0, Phase.from_radians(theta))
+
+    def p_phase(self, q, phase):
+        """diag(1, e^{i phase}) with an exact ring phase."""
+        return self._push(P, q, 0, phase)
+
+    def rz(self, q, theta):
+        """RZ = P(theta) plus global phase e^{-i theta/2} (src/circuit.rs:136-139)."""
+        self._push(P, q, 0, Phase.from_radians(theta))
+        return self._push(GPHASE, 0, 0, Phase.from_radians(-theta / 2.0))
+
+    def ccx(self, a, b, c):
+        """Toffoli with 7 T gates (src/circuit.rs:163-179)."""
+        self.h(c).cx(b, c).tdg(c).cx(a, c).t(c).cx(b, c).tdg(c).cx(a, c).t(b).t(c).h(c).cx(a, b).t(a).tdg(b)
+        return self.cx(a, b)
+
+    def ccz(self, a, b, c):
+        return self.h(c).ccx(a, b, c).h(c)
+
+    def t_count(self):
+        n = 0
+        for op, ph in zip(self.ops, self.ph):
+            if op in (T, TDG) or (op == P and (ph & ((1 << 30) - 1)) != 0):
+                n += 1
+        return n
+
+    def __len__(self):
+        return len(self.ops)
+
+    def arrays(self):
+        return (np.asarray(self.ops, dtype=np.uint8), np.asarray(self.q0, dtype=np.uint32),
+                np.asarray(self.q1, dtype=np.uint32), np.asarray(self.ph, dtype=np.uint32))
+
+
+class Simulator:
+    """Simulator::with_options (src/sim.rs:144-148) with the GPU branch evaluator.
+
+    strategy: "auto" | "cached"; pairs=p selects Strategy::Pairs(p)
+    (pybind.rs:124-131).  form: "z8" (default), "z8_accurate", "plain_fp64".
+    """
+
+    def __init__(self, circuit, strategy="auto", pairs=None, form="z8"):
+        if pairs is not None:
+            st, pr = 2, int(pairs)
+        elif strategy in STRATEGIES:
+            st, pr = STRATEGIES[strategy], 0
+        else:
+            raise ValueError("strategy must be 'auto' or 'cached'")
+        if form not in FORMS:
+            raise ValueError("form must be one of %s" % sorted(FORMS))
+        self.form = form
+        self.circuit = circuit
+        ops, q0, q1, ph = circuit.arrays()
+        h = _sim_p()
+        _check(lib.pq_sim_create(circuit.num_qubits, len(ops), _ptr(ops, _u8p), _ptr(q0, _u32p), _ptr(q1, _u32p), _ptr(ph, _u32p), st, pr, ctypes.byref(h)))
+        self._h = h
+        self.num_qubits = lib.pq_sim_num_qubits(h)
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h:
+            lib.pq_sim_destroy(h)
+            self._h = None
+
+    @property
+    def handle(self):
+        return self._h
+
+    def _form(self, form):
+        return FORMS[form or self.form]
+
+    def amplitude(self, input, output, form=None):
+        out = (ctypes.c_double * 2)()
+        _check(lib.pq_sim_amplitude_index(self._h, int(input), int(output), self._form(form), out))
+        return (out[0], out[1])
+
+    def probability(self, input, output, form=None):
+        out = ctypes.c_double()
+        _check(lib.pq_sim_probability(self._h, int(input), int(output), self._form(form), ctypes.byref(out)))
+        return out.value
+
+    def marginal(self, input, measured, form=None):
+        """Marginal probability of [(qubit, outcome), ...] (src/sim.rs:225-231)."""
+        qs = np.asarray([m[0] for m in measured], dtype=np.uint32)
+        os_ = np.asarray([1 if m[1] else 0 for m in measured], dtype=np.uint8)
+        out = ctypes.c_double()
+        _check(lib.pq_sim_marginal_probability(self._h, int(input), _ptr(qs, _u32p), _ptr(os_, _u8p), len(qs), self._form(form), ctypes.byref(out)))
+        return out.value
+
+    def amplitudes(self, inputs, outputs, form=None):
+        xs = np.ascontiguousarray(inputs, dtype=np.uint64)
+        ys = np.ascontiguousarray(outputs, dtype=np.uint64)
+        out = np.zeros(2 * len(xs), dtype=np.float64)
+        _check(lib.pq_sim_amplitudes(self._h, len(xs), _ptr(xs, _u64p), _ptr(ys, _u64p), self._form(form), _ptr(out, _f64p)))
+        return out[0::2] + 1j * out[1::2]
+
+    def stats(self):
+        """(free_vars, planes, lines, radical, terms, branch_dim, pairs) (pybind.rs:177-180)."""
+        o = np.zeros(7, dtype=np.uint64)
+        _check(lib.pq_sim_stats(self._h, _ptr(o, _u64p)))
+        return tuple(int(v) for v in o)
+
+    def z8_info(self):
+        o = np.zeros(5, dtype=np.uint32)
+        _check(lib.pq_sim_z8_info(self._h, _ptr(o, _u32p)))
+        return dict(zip(("d", "inner", "slots", "inner_pivots", "rank_k"), (int(v) for v in o)))
+
+    def selfcheck(self, x, y, trials=256, seed=1):
+        bad = ctypes.c_uint64()
+        rc = _check(lib.pq_sim_selfcheck(self._h, int(x), int(y), trials, seed, ctypes.byref(bad)))
+        return None if rc == 1 else int(bad.value)
+
+    def branch_dump(self, x, y, s_list):
+        s = np.ascontiguousarray(s_list, dtype=np.uint64)
+        n = len(s)
+        zero = np.zeros(n, np.uint8); k8 = np.zeros(n, np.uint8); kt = np.zeros(n, np.uint8)
+        r = ctypes.c_uint32()
+        rc = _check(lib.pq_sim_branch_dump(self._h, int(x), int(y), n, _ptr(s, _u64p), _ptr(zero, _u8p), _ptr(k8, _u8p), _ptr(kt, _u8p), ctypes.byref(r)))
+        if rc == 1:
+            return None
+        return zero, k8, kt, int(r.value)
+
+    def walk_dump(self, x, y, mid_bits=0):
+        info = self.z8_info()
+        total = 1 << info["slots"]
+        k = np.zeros(total, np.uint8); mag = np.zeros(total, np.float32); s = np.zeros(total, np.uint64)
+        n = ctypes.c_size_t()
+        rc = _check(lib.pq_sim_walk_dump(self._h, int(x), int(y), mid_bits, total, _ptr(k, _u8p), _ptr(mag, _f32p), _ptr(s, _u64p), ctypes.byref(n)))
+        if rc == 1:
+            return None
+        return k[: n.value], mag[: n.value], s[: n.value]
+
+
+def batch_amplitudes(sims, inputs, outputs, form="z8"):
+    """One persistent launch over queries of many simulators."""
+    n = len(sims)
+    hs = (_sim_p * n)(*[s.handle for s in sims])
+    xs = np.ascontiguousarray(inputs, dtype=np.uint64)
+    ys = np.ascontiguousarray(outputs, dtype=np.uint64)
+    out = np.zeros(2 * n, dtype=np.float64)
+    _check(lib.pq_batch_amplitudes(n, hs, _ptr(xs, _u64p), _ptr(ys, _u64p), FORMS[form], _ptr(out, _f64p)))
+    return out[0::2] + 1j * out[1::2]
+
+
+class Graph:
+    """A batch captured into a hipGraph and replayed."""
+
+    def __init__(self, sims, inputs, outputs, form="z8"):
+        n = len(sims)
+        self._sims = list(sims)
+        hs = (_sim_p * n)(*[s.handle for s in sims])
+        self._xs = np.ascontiguousarray(inputs, dtype=np.uint64)
+        self._ys = np.ascontiguousarray(outputs, dtype=np.uint64)
+        g = ctypes.c_void_p()
+        _check(lib.pq_graph_create(n, hs, _ptr(self._xs, _u64p), _ptr(self._ys, _u64p), FORMS[form], ctypes.byref(g)))
+        self._g = g
+        self.n = n
+
+    def run(self):
+        out = np.zeros(2 * self.n, dtype=np.float64)
+        ms = ctypes.c_double()
+        _check(lib.pq_graph_run(self._g, _ptr(out, _f64p), ctypes.byref(ms)))
+        return out[0::2] + 1j * out[1::2], ms.value
+
+    def __del__(self):
+        g = getattr(self, "_g", None)
+        if g:
+            lib.pq_graph_destroy(g)
+            self._g = None
+
+
+def last_timing():
+    o = np.zeros(5, dtype=np.float64)
+    lib.pq_last_timing(_ptr(o, _f64p))
+    return dict(zip(("kernel_ms", "total_ms", "branches", "items", "grid"), o.tolist()))
+
+
+def device_info():
+    buf = ctypes.create_string_buffer(256)
+    _check(lib.pq_device_info(buf, 256))
+    return buf.value.decode()
+
+
+def rng_u64(seed):
+    """First SplitMix64 output of Rng::new(seed) (src/rng.rs:13-23, pybind.rs:221-224)."""
+    from .workloads import Rng
+    return Rng(seed).next_u64()
