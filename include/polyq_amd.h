@@ -94,6 +94,29 @@ int pq_graph_create(size_t n, pq_sim* const* sims, const uint64_t* inputs, const
 int pq_graph_run(pq_graph* g, double* out, double* total_ms);
 void pq_graph_destroy(pq_graph* g);
 
+/* ---- request loop (no counterpart in the reference; the serving layer of
+ * the GPU strategy).  Circuits are keyed by content, compiled and built on
+ * the host once, and stay resident in HBM within hbm_budget_bytes (LRU
+ * eviction of idle entries, back-pressure when everything is in flight).
+ * Requests that arrive while a batch runs are launched as the next batch on a
+ * second stream.  kind: 0 amplitude <y|C|x>, 1 probability |<y|C|x>|^2,
+ * 2 marginal probability of the k listed (qubit, outcome) pairs for input x
+ * (src/sim.rs:171-231). */
+typedef struct pq_server pq_server;
+int pq_server_create(uint64_t hbm_budget_bytes, uint32_t host_workers, int form, pq_server** out);
+void pq_server_destroy(pq_server* srv);
+int pq_server_submit(pq_server* srv, uint32_t num_qubits, size_t num_gates, const uint8_t* ops, const uint32_t* q0,
+                     const uint32_t* q1, const uint32_t* phase, int strategy, int kind, uint64_t x, uint64_t y,
+                     const uint32_t* measured_qubits, const uint8_t* outcomes, size_t k, uint64_t* ticket);
+/* Waits for a ticket (timeout_s < 0: forever).  Returns 0 with the result in
+ * out[0..1] and the submit-to-completion latency, 1 on timeout, -1 if the
+ * request failed (message in pq_last_error()). */
+int pq_server_wait(pq_server* srv, uint64_t ticket, double timeout_s, double out[2], double* latency_ms);
+/* completed, failed, batches, evictions, resident bytes, builds, largest
+ * batch, batches launched while another was in flight, walk-kernel ms,
+ * resident circuits. */
+int pq_server_stats(pq_server* srv, double out10[10]);
+
 /* ---- parity and diagnostics (no counterpart in the reference) ---- */
 /* Per-branch Gauss-sum outputs on the first branch engine of the cached
  * strategy for <y|C|x>, for explicit t-bit branch vectors s (bit k = term k):

@@ -20,7 +20,12 @@ struct RankEng {
         std::lock_guard<std::mutex> lk(dev_mu);
         for (size_t i = dev.size(); i < host.size(); ++i) dev.push_back(dev_engine_upload(host[i], z8[i]));
     }
-    size_t device_bytes() const { size_t b = 0; for (auto& d : dev) b += dev_engine_bytes(*d); return b; }
+    size_t device_bytes() const {
+        std::lock_guard<std::mutex> lk(dev_mu);
+        size_t b = 0;
+        for (auto& d : dev) b += dev_engine_bytes(*d);
+        return b;
+    }
 };
 
 std::shared_ptr<RankEng> build_rank(const PathSum& ps, const std::vector<uint32_t>& fixed, int strategy, uint32_t pairs);
@@ -37,6 +42,12 @@ struct Sim {
     std::map<std::vector<uint32_t>, std::shared_ptr<RankEng>> marg;
 
     bool assignment(uint64_t x, uint64_t y, std::vector<std::pair<uint32_t, uint8_t>>& asg) const;
+    size_t device_bytes() {
+        std::lock_guard<std::mutex> lk(mu);
+        size_t b = eng ? eng->device_bytes() : 0;
+        for (auto& kv : marg) b += kv.second->device_bytes();
+        return b;
+    }
     const RankEng& marginal_engine(const std::vector<uint32_t>& qubits);
     bool marginal_assignment(uint64_t x, const uint32_t* qubits, const uint8_t* outcomes, size_t k, std::vector<std::pair<uint32_t, uint8_t>>& a);
 };

@@ -38,6 +38,12 @@ struct Z8Timing {
 };
 // Results are scale * (EV + omega * OD) per job, FP64.
 void dev_z8_batch(const std::vector<Z8Job>& jobs, std::vector<std::complex<double>>& out, Z8Timing* timing, bool accurate_fold);
+// Asynchronous form: a slot holds one batch in flight on its own stream.
+struct Z8Slot;
+std::shared_ptr<Z8Slot> dev_slot_create();
+void dev_z8_launch(Z8Slot& S, const std::vector<Z8Job>& jobs, bool accurate_fold);
+bool dev_z8_ready(Z8Slot& S);
+void dev_z8_wait(Z8Slot& S, std::vector<std::complex<double>>& out, Z8Timing* timing);
 // Mid-bit heuristic: enough items to fill every CU several times, long
 // enough walks to amortise the lane seed.
 uint32_t z8_auto_mid_bits(uint32_t d, uint64_t batch_branches);

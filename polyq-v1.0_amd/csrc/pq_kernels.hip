@@ -590,18 +590,14 @@ __global__ __launch_bounds__(256) void k_z8_reduce(const Z8Q* __restrict__ qs, c
 
 __global__ void k_zero_u64(unsigned long long* p) { *p = 0ull; }
 
-uint32_t z8_auto_mid_bits(uint32_t d, uint64_t batch_branches) {
-    // Lanes walk 2^(8+M) branches; aim for at least ~8 items per CU over the
-    // whole batch while keeping M <= 6 so a lane's walk stays a few
-    // microseconds long.
-    if (d <= 8) return 0;
-    const double target_items = 256.0 * 8.0;
-    uint32_t best = 0;
-    for (uint32_t M = 0; M <= 6 && 8 + M < d; ++M) {
-        const double items = (double)batch_branches / std::pow(2.0, 16.0 + M);
-        if (items >= target_items) best = M;
-    }
-    return std::min(best, d - 8);
+uint32_t z8_auto_mid_bits(uint32_t d, uint64_t /*batch_branches*/) {
+    // A lane walks 2^(8+M) branches and 2^H lanes cover the query, H = d-8-M.
+    // M depends on d alone, so a query is decomposed the same way in every
+    // batch and its FP32 partial sums (hence its result) are bitwise the same
+    // alone and in any mix.  M = d - 27 keeps H = 19 (2048 items of 256 lanes
+    // per query, enough to fill 256 CUs from one query) until M reaches 6.
+    if (d <= 27) return 0;
+    return std::min<uint32_t>(d - 27, 6);
 }
 
 namespace {
@@ -659,42 +655,100 @@ void launch_walk(Ctx& C, bool accurate, uint32_t grid, const Z8Dev* de, const Z8
 }
 }  // namespace
 
-void dev_z8_batch(const std::vector<Z8Job>& jobs, std::vector<std::complex<double>>& out, Z8Timing* timing, bool accurate) {
-    out.assign(jobs.size(), {0, 0});
-    if (jobs.empty()) return;
+// One in-flight batch: its own stream, events and buffers, so a second batch
+// can be queued while this one still runs (continuous batching).
+struct Z8Slot {
+    hipStream_t stream = nullptr;
+    hipEvent_t e0 = nullptr, e1 = nullptr, e2 = nullptr, e3 = nullptr;
+    DBuf d_in, d_ctr, d_part, d_res;
+    HBuf h_in, h_res;
+    uint32_t nq = 0, grid = 0;
+    uint64_t items = 0, branches = 0;
+    bool pending = false;
+    Z8Slot() {
+        HIPCHECK(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));   // default priority
+        HIPCHECK(hipEventCreate(&e0)); HIPCHECK(hipEventCreate(&e1)); HIPCHECK(hipEventCreate(&e2)); HIPCHECK(hipEventCreate(&e3));
+    }
+    ~Z8Slot() {
+        if (pending) (void)hipStreamSynchronize(stream);
+        for (hipEvent_t e : {e0, e1, e2, e3}) if (e) (void)hipEventDestroy(e);
+        if (stream) (void)hipStreamDestroy(stream);
+        if (d_in.p) (void)hipFree(d_in.p);
+        if (d_ctr.p) (void)hipFree(d_ctr.p);
+        if (d_part.p) (void)hipFree(d_part.p);
+        if (d_res.p) (void)hipFree(d_res.p);
+        if (h_in.p) (void)hipHostFree(h_in.p);
+        if (h_res.p) (void)hipHostFree(h_res.p);
+    }
+};
+
+std::shared_ptr<Z8Slot> dev_slot_create() { (void)ctx(); return std::make_shared<Z8Slot>(); }
+
+void dev_z8_launch(Z8Slot& S, const std::vector<Z8Job>& jobs, bool accurate) {
+    if (S.pending) throw std::logic_error("batch slot already has a batch in flight");
     Ctx& C = ctx();
     BatchPlan B = plan_batch(jobs);
     const uint32_t nq = (uint32_t)B.qs.size();
-    std::lock_guard<std::mutex> lk(C.mu);
+    S.nq = nq; S.items = B.total_items; S.branches = B.branches;
+    if (nq == 0) return;
     // one pinned staging buffer: engines | queries | scales | order
     const size_t be = B.engs.size() * sizeof(Z8Dev), bq = nq * sizeof(Z8Q), bs = nq * sizeof(double2), bo = nq * 4;
     const size_t oq = (be + 255) & ~size_t(255), os = oq + ((bq + 255) & ~size_t(255)), oo = os + ((bs + 255) & ~size_t(255));
     const size_t tot = oo + bo;
-    uint8_t* h = (uint8_t*)C.h_in.get(tot);
+    uint8_t* h = (uint8_t*)S.h_in.get(tot);
     std::memcpy(h, B.engs.data(), be); std::memcpy(h + oq, B.qs.data(), bq); std::memcpy(h + os, B.scale.data(), bs); std::memcpy(h + oo, B.order.data(), bo);
-    uint8_t* d = (uint8_t*)C.d_qs.get(tot);
-    unsigned long long* ctr = (unsigned long long*)C.d_counter.get(64);
-    double* part = (double*)C.d_partial.get(B.total_items * 32);
-    double* res = (double*)C.d_out.get(nq * 16);
-    double* hres = (double*)C.h_out.get(nq * 16);
-    const uint32_t grid = accurate ? walk_grid<32>(C, B.total_items) : walk_grid<256>(C, B.total_items);
-    HIPCHECK(hipEventRecord(C.e0, C.stream));
-    HIPCHECK(hipMemcpyAsync(d, h, tot, hipMemcpyHostToDevice, C.stream));
-    hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(1), 0, C.stream, ctr);
-    HIPCHECK(hipEventRecord(C.e1, C.stream));
-    launch_walk(C, accurate, grid, (const Z8Dev*)d, (const Z8Q*)(d + oq), nq, B.total_items, ctr, part);
-    HIPCHECK(hipEventRecord(C.e2, C.stream));
-    hipLaunchKernelGGL(k_z8_reduce, dim3(nq), dim3(256), 0, C.stream, (const Z8Q*)(d + oq), part, (const double2*)(d + os), (const uint32_t*)(d + oo), res);
+    uint8_t* d = (uint8_t*)S.d_in.get(tot);
+    unsigned long long* ctr = (unsigned long long*)S.d_ctr.get(64);
+    double* part = (double*)S.d_part.get(B.total_items * 32);
+    double* res = (double*)S.d_res.get(nq * 16);
+    double* hres = (double*)S.h_res.get(nq * 16);
+    S.grid = accurate ? walk_grid<32>(C, B.total_items) : walk_grid<256>(C, B.total_items);
+    HIPCHECK(hipEventRecord(S.e0, S.stream));
+    HIPCHECK(hipMemcpyAsync(d, h, tot, hipMemcpyHostToDevice, S.stream));
+    hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(1), 0, S.stream, ctr);
+    HIPCHECK(hipEventRecord(S.e1, S.stream));
+    if (accurate) hipLaunchKernelGGL(k_z8_walk<32>, dim3(S.grid), dim3(256), 0, S.stream, (const Z8Dev*)d, (const Z8Q*)(d + oq), nq, B.total_items, ctr, part);
+    else hipLaunchKernelGGL(k_z8_walk<256>, dim3(S.grid), dim3(256), 0, S.stream, (const Z8Dev*)d, (const Z8Q*)(d + oq), nq, B.total_items, ctr, part);
     HIPCHECK(hipGetLastError());
-    HIPCHECK(hipMemcpyAsync(hres, res, nq * 16, hipMemcpyDeviceToHost, C.stream));
-    HIPCHECK(hipEventRecord(C.e3, C.stream));
-    HIPCHECK(hipStreamSynchronize(C.stream));
-    for (uint32_t i = 0; i < nq; ++i) out[i] = {hres[2 * i], hres[2 * i + 1]};
+    HIPCHECK(hipEventRecord(S.e2, S.stream));
+    hipLaunchKernelGGL(k_z8_reduce, dim3(nq), dim3(256), 0, S.stream, (const Z8Q*)(d + oq), part, (const double2*)(d + os), (const uint32_t*)(d + oo), res);
+    HIPCHECK(hipGetLastError());
+    HIPCHECK(hipMemcpyAsync(hres, res, nq * 16, hipMemcpyDeviceToHost, S.stream));
+    HIPCHECK(hipEventRecord(S.e3, S.stream));
+    S.pending = true;
+}
+
+bool dev_z8_ready(Z8Slot& S) {
+    if (!S.pending) return true;
+    hipError_t e = hipEventQuery(S.e3);
+    if (e == hipErrorNotReady) return false;
+    HIPCHECK(e);
+    return true;
+}
+
+void dev_z8_wait(Z8Slot& S, std::vector<std::complex<double>>& out, Z8Timing* timing) {
+    out.assign(S.nq, {0, 0});
+    if (!S.pending) return;
+    HIPCHECK(hipEventSynchronize(S.e3));
+    S.pending = false;
+    const double* hres = (const double*)S.h_res.p;
+    for (uint32_t i = 0; i < S.nq; ++i) out[i] = {hres[2 * i], hres[2 * i + 1]};
     if (timing) {
         float k, t;
-        HIPCHECK(hipEventElapsedTime(&k, C.e1, C.e2)); HIPCHECK(hipEventElapsedTime(&t, C.e0, C.e3));
-        timing->kernel_ms = k; timing->total_ms = t; timing->branches = B.branches; timing->items = B.total_items; timing->grid = grid;
+        HIPCHECK(hipEventElapsedTime(&k, S.e1, S.e2)); HIPCHECK(hipEventElapsedTime(&t, S.e0, S.e3));
+        timing->kernel_ms = k; timing->total_ms = t; timing->branches = S.branches; timing->items = S.items; timing->grid = S.grid;
     }
+}
+
+void dev_z8_batch(const std::vector<Z8Job>& jobs, std::vector<std::complex<double>>& out, Z8Timing* timing, bool accurate) {
+    out.assign(jobs.size(), {0, 0});
+    if (jobs.empty()) return;
+    Ctx& C = ctx();
+    static std::shared_ptr<Z8Slot> slot;
+    std::lock_guard<std::mutex> lk(C.mu);
+    if (!slot) slot = std::make_shared<Z8Slot>();
+    dev_z8_launch(*slot, jobs, accurate);
+    dev_z8_wait(*slot, out, timing);
 }
 
 // ------------------------------------------------------------- hipGraph ----

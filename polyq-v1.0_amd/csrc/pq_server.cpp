@@ -1,0 +1,370 @@
+// Request loop of the branch evaluator: resident engines in HBM, admission by
+// device memory, continuous batching.
+//
+//  * Circuits are keyed by a content hash; an engine set is compiled and built
+//    on the host once (src/sim.rs:144-148 + src/rank.rs:88-123), uploaded and
+//    kept resident until evicted (LRU, only when the HBM budget is exceeded
+//    and the entry has no request in flight).
+//  * Worker threads run every request's host prologue (boundary, prepare,
+//    Z8 query bits) and any build it needs, off the dispatch path.
+//  * The dispatcher keeps up to two batches in flight on separate streams:
+//    requests that become ready while one batch runs are launched as the next
+//    batch immediately and start on the device as the first one drains.
+//  * A request's result does not depend on the batch it lands in: its items
+//    and reduction order depend only on its own branch dimension.
+#include "pq_api_internal.hpp"
+#include "../../include/polyq_amd.h"
+#include <atomic>
+#include <chrono>
+#include <condition_variable>
+#include <deque>
+#include <thread>
+#include <unordered_map>
+#include <algorithm>
+#include <cmath>
+
+namespace pq {
+namespace {
+
+double now_s() { return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count(); }
+
+struct Entry {
+    uint64_t key = 0;
+    std::mutex build_mu;
+    bool built = false;
+    std::string err;
+    std::unique_ptr<Sim> sim;
+    size_t bytes = 0;
+    uint64_t last_use = 0;
+    int inflight = 0;
+};
+
+struct Req {
+    uint64_t ticket = 0;
+    int kind = 0;                  // 0 amplitude, 1 probability, 2 marginal
+    uint64_t x = 0, y = 0;
+    std::vector<uint32_t> mq;
+    std::vector<uint8_t> mo;
+    std::shared_ptr<Entry> entry;
+    Circuit circ;
+    int strategy = PQ_CACHED;
+    double t_submit = 0, t_done = 0;
+    std::vector<Z8Job> jobs;       // prepared device work
+    std::complex<double> val{0, 0};
+    int status = 0;                // 0 pending, 1 done, -1 failed
+    std::string err;
+};
+
+uint64_t circuit_key(const Circuit& c, int strategy) {
+    uint64_t h = 1469598103934665603ull;
+    auto mix = [&](uint64_t v) { for (int i = 0; i < 8; ++i) { h ^= (v >> (8 * i)) & 0xFF; h *= 1099511628211ull; } };
+    mix(c.n); mix((uint64_t)strategy);
+    for (auto& g : c.gates) { mix(g.op); mix(g.a); mix(g.b); mix(g.ph); }
+    return h;
+}
+
+}  // namespace
+
+struct Server {
+    uint64_t budget;
+    int form;
+    std::atomic<uint64_t> resident{0};
+    std::mutex mu;
+    std::condition_variable cv_work, cv_ready, cv_done, cv_mem;
+    std::unordered_map<uint64_t, std::shared_ptr<Entry>> entries;
+    std::deque<std::shared_ptr<Req>> work_q, ready_q;
+    std::unordered_map<uint64_t, std::shared_ptr<Req>> reqs;
+    std::vector<std::thread> workers;
+    std::thread dispatcher;
+    bool stop = false;
+    uint64_t next_ticket = 1, use_clock = 0;
+    // statistics
+    uint64_t completed = 0, failed = 0, batches = 0, evictions = 0, builds = 0, max_batch = 0, overlapped = 0;
+    double busy_kernel_ms = 0;
+
+    Server(uint64_t b, unsigned nworkers, int f) : budget(b), form(f) {
+        for (unsigned i = 0; i < std::max(1u, nworkers); ++i) workers.emplace_back([this] { worker(); });
+        dispatcher = std::thread([this] { dispatch(); });
+    }
+    ~Server() {
+        {
+            std::lock_guard<std::mutex> lk(mu);
+            stop = true;
+        }
+        cv_work.notify_all(); cv_ready.notify_all(); cv_mem.notify_all(); cv_done.notify_all();
+        for (auto& t : workers) t.join();
+        dispatcher.join();
+    }
+
+    // ---- admission: make room for `bytes` of engine images in the HBM budget
+    bool admit(size_t bytes, const std::shared_ptr<Entry>& self) {
+        std::unique_lock<std::mutex> lk(mu);
+        for (;;) {
+            if (resident.load() + bytes <= budget) { resident += bytes; return true; }
+            // evict the least recently used entry with nothing in flight
+            std::shared_ptr<Entry> victim;
+            for (auto& kv : entries) {
+                auto& e = kv.second;
+                if (e == self || e->inflight > 0 || !e->built || e->bytes == 0) continue;
+                if (!victim || e->last_use < victim->last_use) victim = e;
+            }
+            if (victim) {
+                entries.erase(victim->key);
+                resident -= victim->bytes;
+                victim->bytes = 0;
+                evictions++;
+                continue;
+            }
+            if (bytes > budget) return false;              // can never fit
+            if (stop) return false;
+            cv_mem.wait_for(lk, std::chrono::milliseconds(5));   // back-pressure until memory frees
+        }
+    }
+
+    void build_entry(const std::shared_ptr<Entry>& E, const Req& r) {
+        std::lock_guard<std::mutex> bl(E->build_mu);
+        if (E->built || !E->err.empty()) return;
+        try {
+            auto s = std::make_unique<Sim>();
+            s->circ = r.circ;
+            s->strategy = r.strategy;
+            s->ps = compile(s->circ);
+            s->eng = build_rank(s->ps, s->ps.io_vars(), r.strategy, 0);
+            size_t est = 0;
+            for (auto& z : s->eng->z8) est += 4096 + z->inner.size() * 4 + z->lt.size() * 4 + z->gin.size() * 4 + z->nslots * 32;
+            for (auto& h : s->eng->host) est += (h->kernel_u.size() + h->mcols.size()) * 8 + h->t * 64 + 8192;
+            if (!admit(est, E)) throw std::runtime_error("engine does not fit the HBM budget");
+            s->eng->ensure_device();
+            size_t real = s->eng->device_bytes();
+            resident += real; resident -= est;
+            E->bytes = real;
+            E->sim = std::move(s);
+            E->built = true;
+            std::lock_guard<std::mutex> lk(mu);
+            builds++;
+        } catch (std::exception& e) {
+            E->err = e.what();
+        }
+    }
+
+    void worker() {
+        for (;;) {
+            std::shared_ptr<Req> r;
+            {
+                std::unique_lock<std::mutex> lk(mu);
+                cv_work.wait(lk, [&] { return stop || !work_q.empty(); });
+                if (stop) return;
+                r = work_q.front(); work_q.pop_front();
+            }
+            try {
+                build_entry(r->entry, *r);
+                if (!r->entry->err.empty()) throw std::runtime_error(r->entry->err);
+                Sim& S = *r->entry->sim;
+                std::vector<std::pair<uint32_t, uint8_t>> asg;
+                if (r->kind == 2) {
+                    const size_t before = S.device_bytes();
+                    const RankEng& R = S.marginal_engine(r->mq);
+                    R.ensure_device();
+                    const size_t grown = S.device_bytes() - before;
+                    if (grown) {
+                        std::lock_guard<std::mutex> lk(mu);
+                        resident += grown;
+                        r->entry->bytes += grown;
+                    }
+                    if (S.marginal_assignment(r->x, r->mq.data(), r->mo.data(), r->mq.size(), asg)) {
+                        std::vector<size_t> owner;
+                        add_jobs(R, asg, r->jobs, owner, 0, nullptr);
+                    }
+                } else if (S.assignment(r->x, r->y, asg)) {
+                    std::vector<size_t> owner;
+                    add_jobs(*S.eng, asg, r->jobs, owner, 0, nullptr);
+                }
+                for (auto& J : r->jobs) {
+                    const uint32_t d = dev_z8(*J.eng).d;
+                    if (d > 40) throw std::runtime_error("branch dimension above 40: request rejected");
+                    J.mid_bits = z8_auto_mid_bits(d, 0);
+                }
+                std::lock_guard<std::mutex> lk(mu);
+                ready_q.push_back(r);
+            } catch (std::exception& e) {
+                std::lock_guard<std::mutex> lk(mu);
+                r->status = -1; r->err = e.what(); r->t_done = now_s();
+                r->entry->inflight--;
+                r->entry.reset();
+                failed++;
+                cv_done.notify_all();
+                continue;
+            }
+            cv_ready.notify_one();
+        }
+    }
+
+    void complete(std::vector<std::shared_ptr<Req>>& batch, const std::vector<std::complex<double>>& res) {
+        const double t = now_s();
+        std::lock_guard<std::mutex> lk(mu);
+        size_t k = 0;
+        for (auto& r : batch) {
+            std::complex<double> a(0, 0);
+            for (size_t j = 0; j < r->jobs.size(); ++j) a += res[k++];
+            if (r->kind == 0) r->val = a;
+            else if (r->kind == 1) r->val = {std::norm(a), 0};
+            else r->val = {std::max(a.real(), 0.0), 0};        // sim.rs:230
+            r->status = 1; r->t_done = t;
+            r->entry->inflight--;
+            r->entry.reset();
+            r->jobs.clear();
+            completed++;
+        }
+        cv_done.notify_all();
+        cv_mem.notify_all();
+    }
+
+    void dispatch() {
+        std::shared_ptr<Z8Slot> slots[2];
+        std::vector<std::shared_ptr<Req>> held[2];
+        int order[2] = {-1, -1};     // launch order of the in-flight slots (oldest first)
+        try {
+            slots[0] = dev_slot_create(); slots[1] = dev_slot_create();
+        } catch (std::exception&) {
+            // no device: requests fail when they reach the dispatcher
+        }
+        const bool accurate = form == PQ_FORM_Z8_ACCURATE;
+        for (;;) {
+            std::vector<std::shared_ptr<Req>> batch;
+            int inflight = (order[0] >= 0) + (order[1] >= 0);
+            {
+                std::unique_lock<std::mutex> lk(mu);
+                if (inflight == 0) cv_ready.wait(lk, [&] { return stop || !ready_q.empty(); });
+                if (stop && ready_q.empty() && inflight == 0) return;
+                if (inflight < 2) while (!ready_q.empty()) { batch.push_back(ready_q.front()); ready_q.pop_front(); }
+            }
+            if (!batch.empty()) {
+                if (!slots[0]) {
+                    std::vector<std::complex<double>> none;
+                    std::lock_guard<std::mutex> lk(mu);
+                    for (auto& r : batch) { r->status = -1; r->err = "no HIP device available"; r->t_done = now_s(); r->entry->inflight--; r->entry.reset(); failed++; }
+                    cv_done.notify_all();
+                    continue;
+                }
+                const int s = order[0] < 0 ? 0 : (order[0] == 0 ? 1 : 0);   // a free slot (at most one is in flight here)
+                std::vector<Z8Job> jobs;
+                for (auto& r : batch) for (auto& J : r->jobs) jobs.push_back(J);
+                if (jobs.empty()) { complete(batch, {}); continue; }
+                dev_z8_launch(*slots[s], jobs, accurate);
+                held[s] = std::move(batch);
+                if (order[0] < 0) order[0] = s; else { order[1] = s; overlapped++; }
+                std::lock_guard<std::mutex> lk(mu);
+                batches++;
+                max_batch = std::max<uint64_t>(max_batch, held[s].size());
+            }
+            // retire the oldest batch: block when both slots are busy, else
+            // poll it while watching for new requests to launch beside it
+            if (order[0] >= 0) {
+                const int s = order[0];
+                bool done = order[1] >= 0 || dev_z8_ready(*slots[s]);
+                if (!done) {
+                    std::unique_lock<std::mutex> lk(mu);
+                    if (ready_q.empty() && !stop) cv_ready.wait_for(lk, std::chrono::microseconds(200));
+                    continue;
+                }
+                std::vector<std::complex<double>> res;
+                Z8Timing T;
+                dev_z8_wait(*slots[s], res, &T);
+                busy_kernel_ms += T.kernel_ms;
+                complete(held[s], res);
+                held[s].clear();
+                order[0] = order[1]; order[1] = -1;
+            }
+        }
+    }
+
+    uint64_t submit(const Circuit& c, int strategy, int kind, uint64_t x, uint64_t y, const uint32_t* mq, const uint8_t* mo, size_t k) {
+        auto r = std::make_shared<Req>();
+        r->kind = kind; r->x = x; r->y = y; r->circ = c; r->strategy = strategy;
+        r->mq.assign(mq, mq + k); r->mo.assign(mo, mo + k);
+        r->t_submit = now_s();
+        const uint64_t key = circuit_key(c, strategy);
+        {
+            std::lock_guard<std::mutex> lk(mu);
+            auto it = entries.find(key);
+            if (it == entries.end()) {
+                auto e = std::make_shared<Entry>();
+                e->key = key;
+                it = entries.emplace(key, e).first;
+            }
+            r->entry = it->second;
+            r->entry->inflight++;
+            r->entry->last_use = ++use_clock;
+            r->ticket = next_ticket++;
+            reqs[r->ticket] = r;
+            work_q.push_back(r);
+        }
+        cv_work.notify_one();
+        return r->ticket;
+    }
+
+    int wait(uint64_t ticket, double timeout_s, double* out, double* latency_ms) {
+        std::unique_lock<std::mutex> lk(mu);
+        auto it = reqs.find(ticket);
+        if (it == reqs.end()) throw std::invalid_argument("unknown ticket");
+        auto r = it->second;
+        auto pred = [&] { return r->status != 0; };
+        if (timeout_s < 0) cv_done.wait(lk, pred);
+        else if (!cv_done.wait_for(lk, std::chrono::duration<double>(timeout_s), pred)) return 1;
+        reqs.erase(it);
+        if (latency_ms) *latency_ms = 1e3 * (r->t_done - r->t_submit);
+        if (r->status < 0) throw std::runtime_error(r->err);
+        out[0] = r->val.real(); out[1] = r->val.imag();
+        return 0;
+    }
+};
+
+}  // namespace pq
+
+using namespace pq;
+struct pq_server { std::unique_ptr<Server> s; };
+
+extern "C" {
+
+int pq_server_create(uint64_t hbm_budget_bytes, uint32_t workers, int form, pq_server** out) {
+    try {
+        if (form != PQ_FORM_Z8 && form != PQ_FORM_Z8_ACCURATE) throw std::invalid_argument("the server runs the Z8 forms");
+        auto h = std::make_unique<pq_server>();
+        h->s = std::make_unique<Server>(hbm_budget_bytes, workers, form);
+        *out = h.release();
+        return 0;
+    } catch (std::exception& e) { g_err = e.what(); return -1; }
+}
+
+void pq_server_destroy(pq_server* h) { delete h; }
+
+int pq_server_submit(pq_server* h, uint32_t nq, size_t ng, const uint8_t* ops, const uint32_t* q0, const uint32_t* q1, const uint32_t* ph,
+                     int strategy, int kind, uint64_t x, uint64_t y, const uint32_t* mq, const uint8_t* mo, size_t k, uint64_t* ticket) {
+    try {
+        if (kind < 0 || kind > 2) throw std::invalid_argument("kind must be 0 (amplitude), 1 (probability) or 2 (marginal)");
+        if (nq > 64) throw std::invalid_argument("at most 64 qubits");
+        Circuit c; c.n = nq;
+        for (size_t i = 0; i < ng; ++i) c.add(ops[i], q0[i], q1 ? q1[i] : 0, ph ? ph[i] : 0);
+        if ((nq < 64 && ((x >> nq) || (y >> nq)))) throw std::invalid_argument("basis index has bits beyond the qubit count");
+        for (size_t i = 0; i < k; ++i) if (mq[i] >= c.n) throw std::invalid_argument("measured qubit out of range");
+        *ticket = h->s->submit(c, strategy, kind, x, y, mq, mo, k);
+        return 0;
+    } catch (std::exception& e) { g_err = e.what(); return -1; }
+}
+
+int pq_server_wait(pq_server* h, uint64_t ticket, double timeout_s, double* out, double* latency_ms) {
+    try {
+        return h->s->wait(ticket, timeout_s, out, latency_ms);
+    } catch (std::exception& e) { g_err = e.what(); return -1; }
+}
+
+int pq_server_stats(pq_server* h, double* o) {
+    Server& S = *h->s;
+    std::lock_guard<std::mutex> lk(S.mu);
+    o[0] = (double)S.completed; o[1] = (double)S.failed; o[2] = (double)S.batches; o[3] = (double)S.evictions;
+    o[4] = (double)S.resident.load(); o[5] = (double)S.builds; o[6] = (double)S.max_batch; o[7] = (double)S.overlapped;
+    o[8] = S.busy_kernel_ms; o[9] = (double)S.entries.size();
+    return 0;
+}
+
+}  // extern "C"

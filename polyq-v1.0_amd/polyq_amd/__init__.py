@@ -307,3 +307,58 @@ def rng_u64(seed):
     """First SplitMix64 output of Rng::new(seed) (src/rng.rs:13-23, pybind.rs:221-224)."""
     from .workloads import Rng
     return Rng(seed).next_u64()
+
+
+lib.pq_server_create.argtypes = [ctypes.c_uint64, ctypes.c_uint32, ctypes.c_int, ctypes.POINTER(ctypes.c_void_p)]
+lib.pq_server_destroy.argtypes = [ctypes.c_void_p]
+lib.pq_server_destroy.restype = None
+lib.pq_server_submit.argtypes = [ctypes.c_void_p, ctypes.c_uint32, ctypes.c_size_t, _u8p, _u32p, _u32p, _u32p, ctypes.c_int, ctypes.c_int,
+                                 ctypes.c_uint64, ctypes.c_uint64, _u32p, _u8p, ctypes.c_size_t, _u64p]
+lib.pq_server_wait.argtypes = [ctypes.c_void_p, ctypes.c_uint64, ctypes.c_double, _f64p, _f64p]
+lib.pq_server_stats.argtypes = [ctypes.c_void_p, _f64p]
+
+KINDS = {"amplitude": 0, "probability": 1, "marginal": 2}
+
+
+class Server:
+    """Request loop with resident engines, HBM admission and continuous batching."""
+
+    def __init__(self, hbm_budget_bytes=64 << 30, workers=8, form="z8", strategy="cached"):
+        h = ctypes.c_void_p()
+        _check(lib.pq_server_create(int(hbm_budget_bytes), workers, FORMS[form], ctypes.byref(h)))
+        self._h = h
+        self.strategy = STRATEGIES[strategy]
+
+    def submit(self, circuit, kind, x=0, y=0, measured=()):
+        ops, q0, q1, ph = circuit.arrays()
+        mq = np.asarray([m[0] for m in measured], dtype=np.uint32)
+        mo = np.asarray([1 if m[1] else 0 for m in measured], dtype=np.uint8)
+        t = ctypes.c_uint64()
+        _check(lib.pq_server_submit(self._h, circuit.num_qubits, len(ops), _ptr(ops, _u8p), _ptr(q0, _u32p), _ptr(q1, _u32p), _ptr(ph, _u32p),
+                                    self.strategy, KINDS[kind], int(x), int(y), _ptr(mq, _u32p), _ptr(mo, _u8p), len(mq), ctypes.byref(t)))
+        return t.value
+
+    def wait(self, ticket, timeout=-1.0):
+        """(value, latency_ms); value is complex for amplitudes, float otherwise."""
+        out = (ctypes.c_double * 2)()
+        lat = ctypes.c_double()
+        rc = lib.pq_server_wait(self._h, ticket, timeout, out, ctypes.byref(lat))
+        if rc < 0:
+            raise PolyqError(lib.pq_last_error().decode())
+        if rc == 1:
+            raise TimeoutError("request %d not finished" % ticket)
+        return complex(out[0], out[1]), lat.value
+
+    def stats(self):
+        o = np.zeros(10, dtype=np.float64)
+        _check(lib.pq_server_stats(self._h, _ptr(o, _f64p)))
+        keys = ("completed", "failed", "batches", "evictions", "resident_bytes", "builds", "max_batch", "overlapped_batches", "kernel_ms", "resident_circuits")
+        return dict(zip(keys, o.tolist()))
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib.pq_server_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        self.close()

@@ -1,0 +1,94 @@
+"""Writes the golden fixtures of this directory from the CPU oracle.
+
+Every vector is checked here against an independent computation before it is
+written: amplitudes against the dense statevector (or, for 40 qubits, against
+a second strategy of the oracle), per-branch Gauss-sum outputs against the
+2^m brute-force Gauss sum.  Run: python tests/golden/make_golden.py
+"""
+import json
+import math
+import os
+import sys
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(os.path.dirname(HERE))
+sys.path.insert(0, os.path.join(ROOT, "polyq-v1.0_amd"))
+sys.path.insert(0, os.path.join(ROOT, "tests"))
+
+import numpy as np  # noqa: E402
+
+import oracle_lib as O  # noqa: E402
+from polyq_amd import workloads as W  # noqa: E402
+
+
+def circ_json(c):
+    ops, q0, q1, ph = c.arrays()
+    return {"n": c.num_qubits, "ops": ops.tolist(), "q0": q0.tolist(), "q1": q1.tolist(), "ph": ph.tolist()}
+
+
+def amplitudes_config1():
+    c, xs, ys = W.config1(1)
+    oc = O.OCircuit.from_circuit(c)
+    sim = O.OSim(oc, "cached")
+    sv = O.dense(oc, 0)
+    amps = [sim.amplitude(x, y) for x, y in zip(xs, ys)]
+    for a, y in zip(amps, ys):
+        assert abs(a - sv[y]) < 1e-10
+    return {"circuit": circ_json(c), "x": xs, "y": ys, "re": [a.real for a in amps], "im": [a.imag for a in amps],
+            "tolerance": 1e-10, "check": "dense statevector"}
+
+
+def amplitudes_config2_seed1():
+    c, xs, ys = W.config2(1)
+    oc = O.OCircuit.from_circuit(c)
+    sim = O.OSim(oc, "cached")
+    sim.set_threads(8)
+    amps = [sim.amplitude(x, y) for x, y in zip(xs, ys)]
+    alt = O.OSim(oc, "pairs", 4)
+    alt.set_threads(8)
+    for a, x, y in zip(amps, xs, ys):
+        assert abs(a - alt.amplitude(x, y)) < 1e-12
+    return {"circuit": circ_json(c), "x": [int(v) for v in xs], "y": [int(v) for v in ys], "re": [a.real for a in amps],
+            "im": [a.imag for a in amps], "stats": list(sim.stats()), "check": "Pairs(4) decomposition of the oracle"}
+
+
+def branch_vectors():
+    rng = W.Rng(2024)
+    cases = []
+    while len(cases) < 8:
+        n = 2 + rng.below(3)
+        c = W.random_clifford_t(n, 14 + rng.below(12), 2 + rng.below(4), len(cases) % 2 == 0, rng)
+        oc = O.OCircuit.from_circuit(c)
+        sim = O.OSim(oc, "cached")
+        st = sim.stats()
+        if st[0] > 18 or st[4] == 0:
+            continue
+        x, y = rng.below(1 << n), rng.below(1 << n)
+        S = np.arange(1 << st[4], dtype=np.uint64)
+        d = sim.branch_dump(x, y, S)
+        if d is None:
+            continue
+        zero, r, k8, kt = d
+        G = O.branch_brute(sim, x, y, S)
+        for i in range(len(S)):
+            want = 0 if zero[i] else 2 ** (r[i] / 2) * complex(math.cos(math.pi * k8[i] / 4), math.sin(math.pi * k8[i] / 4))
+            assert abs(G[i] - want) < 1e-9 * max(1, abs(want))
+        cases.append({"circuit": circ_json(c), "x": int(x), "y": int(y), "s": S.tolist(), "zero": zero.tolist(),
+                      "r": r.tolist(), "k8": k8.tolist(), "ktot": kt.tolist(), "stats": list(st)})
+    return {"cases": cases, "check": "2^m brute-force Gauss sum at the branch shift"}
+
+
+def main():
+    out = {
+        "amplitudes_config1_seed1.json": amplitudes_config1(),
+        "amplitudes_config2_seed1.json": amplitudes_config2_seed1(),
+        "branches_small.json": branch_vectors(),
+    }
+    for name, data in out.items():
+        with open(os.path.join(HERE, name), "w") as f:
+            json.dump(data, f, separators=(",", ":"))
+        print("wrote", name)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,110 @@
+"""Batched, graph-replayed and served evaluation on the GPU: every result is
+the same alone and in the mix."""
+import random
+
+import numpy as np
+import pytest
+
+import oracle_lib as O
+import polyq_amd as pq
+from polyq_amd import workloads as W
+
+pytestmark = pytest.mark.gpu
+
+
+def _mixed_circuits(seed, count, nmax=24, tmax=16):
+    rng = W.Rng(seed)
+    out = []
+    for _ in range(count):
+        n = 6 + rng.below(nmax - 5)
+        t = 2 + rng.below(tmax)
+        c = W.sandwich_layer(n, t, (15 * n) // 2, (5 * n) // 2, rng)
+        pairs = [(rng.below(1 << n), rng.below(1 << n)) for _ in range(4)]
+        out.append((c, pairs))
+    return out
+
+
+def test_batch_results_equal_alone_results_bitwise():
+    work = _mixed_circuits(11, 24)
+    sims = [pq.Simulator(c, strategy="cached") for c, _ in work]
+    S, X, Y, alone = [], [], [], []
+    for s, (c, pairs) in zip(sims, work):
+        for x, y in pairs:
+            S.append(s); X.append(x); Y.append(y)
+            alone.append(s.amplitudes([x], [y])[0])
+    mixed = pq.batch_amplitudes(S, X, Y)
+    np.testing.assert_array_equal(mixed, np.array(alone))
+    # and against the FP64 oracle on the small ones
+    for s, x, y, a in list(zip(S, X, Y, mixed))[:24]:
+        o = O.OSim(O.OCircuit.from_circuit(s.circuit), "cached")
+        if s.stats()[5] <= 18:
+            assert abs(o.amplitude(x, y) - a) < 1e-6
+
+
+def test_graph_replays_match_eager():
+    work = _mixed_circuits(12, 12)
+    sims = [pq.Simulator(c, strategy="cached") for c, _ in work]
+    S, X, Y = [], [], []
+    for s, (c, pairs) in zip(sims, work):
+        for x, y in pairs:
+            S.append(s); X.append(x); Y.append(y)
+    eager = pq.batch_amplitudes(S, X, Y)
+    g = pq.Graph(S, X, Y)
+    for rep in range(4):
+        got, ms = g.run()
+        np.testing.assert_array_equal(got, eager)
+        # eager work and a synchronisation between replays
+        sims[rep].amplitudes(X[:2], Y[:2])
+
+
+def test_server_results_equal_alone_results():
+    work = _mixed_circuits(13, 10, nmax=16)
+    srv = pq.Server(hbm_budget_bytes=64 << 30, workers=4)
+    rnd = random.Random(5)
+    tickets = []
+    for rep in range(3):
+        for c, pairs in work:
+            kind = rnd.choice(["amplitude", "amplitude", "probability", "marginal"])
+            x, y = pairs[rnd.randrange(4)]
+            meas = [(q, rnd.random() < 0.5) for q in rnd.sample(range(c.num_qubits), 2)] if kind == "marginal" else ()
+            tickets.append((srv.submit(c, kind, x, y, meas), c, kind, x, y, meas))
+    for t, c, kind, x, y, meas in tickets:
+        val, lat = srv.wait(t, timeout=120.0)
+        s = pq.Simulator(c, strategy="cached")
+        if kind == "amplitude":
+            want = complex(*s.amplitude(x, y))
+            assert val == want
+        elif kind == "probability":
+            assert val.real == s.probability(x, y)
+        else:
+            assert abs(val.real - s.marginal(x, meas)) < 1e-12
+        assert lat > 0
+    st = srv.stats()
+    assert st["completed"] == len(tickets) and st["failed"] == 0
+    assert st["builds"] == len(work)          # repeats hit resident engines
+    srv.close()
+
+
+def test_server_tiny_budget_evicts_and_stays_correct():
+    work = _mixed_circuits(14, 8, nmax=14)
+    srv = pq.Server(hbm_budget_bytes=200_000, workers=2)
+    tickets = []
+    for rep in range(2):
+        for c, pairs in work:
+            x, y = pairs[0]
+            tickets.append((srv.submit(c, "amplitude", x, y), c, x, y))
+            srv.wait(tickets[-1][0], timeout=120.0) if rep == 0 else None
+    results = {}
+    for t, c, x, y in tickets:
+        try:
+            val, _ = srv.wait(t, timeout=120.0)
+        except KeyError:
+            continue
+        except pq.PolyqError:
+            continue
+        results[t] = val
+    st = srv.stats()
+    assert st["evictions"] > 0
+    for t, c, x, y in tickets[len(work):]:
+        assert results[t] == complex(*pq.Simulator(c, strategy="cached").amplitude(x, y))
+    srv.close()
