@@ -200,7 +200,7 @@ struct Z8Image {
     std::vector<uint64_t> slot_n;      // kernel vector N over the t terms
     std::vector<uint32_t> slot_pin;    // inner-touched pivot bits of N, compacted
     std::vector<uint8_t> slot_lam;     // query-independent linear coefficient
-    std::vector<float> inner;          // 256 x {w_re, w_im, pin*4 (bits), kin (bits)}
+    std::vector<float> inner;          // 256 x (w_re, w_im), 256 x pin*4 (bits), 256 x kin (bits)
     std::vector<float> lt;             // nbytes x 256 lane-factor byte tables
     std::vector<float> gin;            // 2^npin inner pivot factor table
     std::vector<uint32_t> pin_term;    // term index of each compact inner pivot bit

@@ -122,7 +122,7 @@ struct Z8Dev {
     const uint64_t* slot_n;
     const uint32_t* slot_pin;
     const uint32_t* slot_lam;
-    const float4* inner;     // 256 x {w_re, w_im, pin*4, kin}
+    const float* inner;      // 256 x (w_re, w_im), then 256 x pin*4, then 256 x kin
     const float* lt;         // nbytes x 256
     const float* gin;        // 2^npin
     uint32_t nslots, nbytes, npin, pad;
@@ -234,7 +234,7 @@ std::shared_ptr<DevEngine> dev_engine_upload(std::shared_ptr<const HostEngine> E
     P.rbits = 2 * L.g + 2 * L.neven + L.nodd; P.neg = E.neg_mask;
     Z8Dev& z = D->z;
     z.slot_q = (const uint64_t*)at(o_sq); z.slot_n = (const uint64_t*)at(o_sn); z.slot_pin = (const uint32_t*)at(o_sp); z.slot_lam = (const uint32_t*)at(o_sl);
-    z.inner = (const float4*)at(o_in); z.lt = (const float*)at(o_lt); z.gin = (const float*)at(o_gin);
+    z.inner = (const float*)at(o_in); z.lt = (const float*)at(o_lt); z.gin = (const float*)at(o_gin);
     z.nslots = Z.nslots; z.nbytes = Z.nbytes; z.npin = Z.npin; z.pad = 0;
     return D;
 }
@@ -391,6 +391,23 @@ std::complex<double> dev_plain_sum(const DevEngine& D, const Prepared& P, double
 }
 
 // ============================================================== Z8 walk ====
+// Engine tables are read through the constant address space: they are never
+// written while a kernel runs, and uniform loads through such pointers become
+// scalar (SGPR) loads even though the persistent loop also performs atomics.
+#define PQ_CONST __attribute__((address_space(4)))
+template <class T> __device__ __forceinline__ const PQ_CONST T* cview(const T* p) { return (const PQ_CONST T*)p; }
+// Whole-struct scalar load of a descriptor (dword by dword through the
+// constant address space).
+template <class T> __device__ __forceinline__ T cload(const T* p) {
+    static_assert(sizeof(T) % 4 == 0, "descriptor size");
+    T out;
+    const PQ_CONST uint32_t* src = cview((const uint32_t*)p);
+    uint32_t* dst = (uint32_t*)&out;
+#pragma unroll
+    for (int k = 0; k < (int)(sizeof(T) / 4); ++k) dst[k] = src[k];
+    return out;
+}
+
 __device__ __forceinline__ void two_sum(float& s, float& c, float x) {
     const float t = s + x;
     const float bp = t - s;
@@ -406,15 +423,20 @@ __device__ __forceinline__ void merge2(float& s, float& c, float s2, float c2) {
     c = (c + c2) + e;
 }
 
-// Rotate (xr, xi) by i^((K>>1)&3) and add into the even or odd accumulator.
+// Rotate (xr, xi) by i^((K>>1)&3) and add into the even or odd accumulator
+// (both pairs are updated, the other with zero, so the accumulator index stays
+// static and the array stays in registers).
 __device__ __forceinline__ void fold(float xr, float xi, uint32_t K, float (&acc)[8]) {
     const uint32_t r2 = (K >> 1) & 3u;
     const float a = (r2 & 1u) ? xi : xr;
     const float b = (r2 & 1u) ? xr : xi;
     const float yr = (r2 == 1u || r2 == 2u) ? -a : a;
     const float yi = (r2 >= 2u) ? -b : b;
-    if (K & 1u) { two_sum(acc[4], acc[5], yr); two_sum(acc[6], acc[7], yi); }
-    else { two_sum(acc[0], acc[1], yr); two_sum(acc[2], acc[3], yi); }
+    const bool odd = K & 1u;
+    two_sum(acc[0], acc[1], odd ? 0.f : yr);
+    two_sum(acc[2], acc[3], odd ? 0.f : yi);
+    two_sum(acc[4], acc[5], odd ? yr : 0.f);
+    two_sum(acc[6], acc[7], odd ? yi : 0.f);
 }
 
 // Lane state over the outer coordinates: integer phase K at c_in = 0, the
@@ -429,10 +451,10 @@ __device__ __forceinline__ void seed_lane(const Z8Dev& E, const Z8Q& Q, uint64_t
     const uint32_t base = 8u + Q.M;
     for (uint32_t b = 0; b < Q.H; ++b) {
         const uint32_t p = base + b;
-        const uint64_t qp = E.slot_q[p];
-        const uint32_t ell = E.slot_lam[p] + (((uint32_t)(Q.delta >> p) & 1u) << 2);
-        const uint32_t pin = E.slot_pin[p];
-        const uint64_t np = E.slot_n[p];
+        const uint64_t qp = cview(E.slot_q)[p];
+        const uint32_t ell = cview(E.slot_lam)[p] + (((uint32_t)(Q.delta >> p) & 1u) << 2);
+        const uint32_t pin = cview(E.slot_pin)[p];
+        const uint64_t np = cview(E.slot_n)[p];
         if ((lane >> b) & 1) {
             S.K += ell + ((__popcll(qp & S.cout) & 1u) << 2);
             S.cout |= 1ull << p;
@@ -443,15 +465,15 @@ __device__ __forceinline__ void seed_lane(const Z8Dev& E, const Z8Q& Q, uint64_t
     }
 }
 __device__ __forceinline__ void flip_mid(const Z8Dev& E, const Z8Q& Q, uint32_t p, LaneState& S) {
-    const uint64_t qp = E.slot_q[p];
-    const uint32_t ell = E.slot_lam[p] + (((uint32_t)(Q.delta >> p) & 1u) << 2);
+    const uint64_t qp = cview(E.slot_q)[p];
+    const uint32_t ell = cview(E.slot_lam)[p] + (((uint32_t)(Q.delta >> p) & 1u) << 2);
     const uint32_t par = (__popcll(qp & S.cout) & 1u) << 2;
     const bool was = (S.cout >> p) & 1;
     S.K += (was ? (8u - ell) : ell) + par;
     S.cout ^= 1ull << p;
     S.hs ^= (uint32_t)qp & 0xFFu;
-    S.pio ^= E.slot_pin[p];
-    S.sout ^= E.slot_n[p];
+    S.pio ^= cview(E.slot_pin)[p];
+    S.sout ^= cview(E.slot_n)[p];
 }
 __device__ __forceinline__ float lane_factor(const float* s_lt, uint32_t nbytes, uint64_t sout) {
     float F = 1.f;
@@ -459,33 +481,47 @@ __device__ __forceinline__ float lane_factor(const float* s_lt, uint32_t nbytes,
     return F;
 }
 
-// The 256 inner branches of one outer point, folded every FOLD steps.
+// The 256 inner branches of one outer point, folded every FOLD steps.  The
+// uniform table is read 16 steps at a time (contiguous re/im pairs, then the
+// inner pivot offsets), which the compiler issues as wide scalar loads.
 template <bool HASG, int FOLD>
-__device__ __forceinline__ void inner_walk(const float4* __restrict__ W, const float* s_gin, const LaneState& S, float F, float (&acc)[8]) {
+__device__ __forceinline__ void inner_walk(const float* Wg, const float* s_gin, const LaneState& S, float F, float (&acc)[8]) {
+    const PQ_CONST float* Wri = cview(Wg);                                   // 256 x (re, im)
+    const PQ_CONST uint32_t* Wp = cview((const uint32_t*)(Wg + 512));        // 256 x pin*4
     uint32_t sm[8];
 #pragma unroll
     for (int j = 0; j < 8; ++j) sm[j] = (S.hs << (31 - j)) & 0x80000000u;
     uint32_t sgn = 0;
     const uint32_t pio4 = S.pio << 2;
+    float tr = 0.f, ti = 0.f;
 #pragma unroll
-    for (int blk = 0; blk < 256 / FOLD; ++blk) {
-        float tr = 0.f, ti = 0.f;
+    for (int blk = 0; blk < 16; ++blk) {
+        float w[32];
+        uint32_t pn[16];
 #pragma unroll
-        for (int ii = 0; ii < FOLD; ++ii) {
-            const int i = blk * FOLD + ii;
+        for (int k = 0; k < 32; ++k) w[k] = Wri[32 * blk + k];
+        if constexpr (HASG) {
+#pragma unroll
+            for (int k = 0; k < 16; ++k) pn[k] = Wp[16 * blk + k];
+        }
+#pragma unroll
+        for (int ii = 0; ii < 16; ++ii) {
+            const int i = 16 * blk + ii;
             if (i) sgn ^= sm[__builtin_ctz(i)];
-            const float4 w = W[i];
             float m;
             if constexpr (HASG) {
-                const float g = *(const float*)((const char*)s_gin + (pio4 ^ __float_as_uint(w.z)));
+                const float g = *(const float*)((const char*)s_gin + (pio4 ^ pn[ii]));
                 m = __uint_as_float(__float_as_uint(g) ^ sgn);
             } else {
                 m = __uint_as_float(0x3f800000u ^ sgn);
             }
-            tr = fmaf(m, w.x, tr);
-            ti = fmaf(m, w.y, ti);
+            tr = fmaf(m, w[2 * ii], tr);
+            ti = fmaf(m, w[2 * ii + 1], ti);
+            if (((i + 1) % FOLD) == 0) {
+                fold(tr * F, ti * F, S.K, acc);
+                tr = 0.f; ti = 0.f;
+            }
         }
-        fold(tr * F, ti * F, S.K, acc);
     }
 }
 
@@ -506,9 +542,9 @@ __global__ __launch_bounds__(256) void k_z8_walk(const Z8Dev* __restrict__ engs,
         const uint64_t item = s_item;
         if (item >= total) break;
         uint32_t lo = 0, hi = nq;
-        while (hi - lo > 1) { const uint32_t mid = (lo + hi) >> 1; if (qs[mid].item0 <= item) lo = mid; else hi = mid; }
-        const Z8Q Q = qs[lo];
-        const Z8Dev E = engs[Q.eng];
+        while (hi - lo > 1) { const uint32_t mid = (lo + hi) >> 1; if (cview(qs)[mid].item0 <= item) lo = mid; else hi = mid; }
+        const Z8Q Q = cload(qs + lo);
+        const Z8Dev E = cload(engs + Q.eng);
         if ((int)Q.eng != s_eng) {
             for (uint32_t k = tid; k < E.nbytes * 256u; k += 256) s_lt[k] = E.lt[k];
             if (E.npin) for (uint32_t k = tid; k < (1u << E.npin); k += 256) s_gin[k] = E.gin[k];
@@ -884,14 +920,15 @@ __global__ __launch_bounds__(256) void k_dump_walk(Z8Dev E, Z8Q Q, uint8_t* __re
         const float F = lane_factor(s_lt, E.nbytes, S.sout);
         for (uint32_t i = 0; i < 256; ++i) {
             const uint32_t g = i ^ (i >> 1);
-            const float4 w = E.inner[i];
+            const float wre = E.inner[2 * i], wim = E.inner[2 * i + 1];
+            const uint32_t pin4 = __float_as_uint(E.inner[512 + i]), kin = __float_as_uint(E.inner[768 + i]);
             const uint64_t idx = (((lane << Q.M) + o) << 8) + i;
-            if (w.x == 0.f && w.y == 0.f) { kout[idx] = 255; mout[idx] = 0.f; continue; }
+            if (wre == 0.f && wim == 0.f) { kout[idx] = 255; mout[idx] = 0.f; continue; }
             const uint32_t par = (uint32_t)__popc(g & S.hs) & 1u;
-            const uint32_t k = (S.K + __float_as_uint(w.w) + 4u * par) & 7u;
-            const float gi = E.npin ? s_gin[(S.pio ^ (__float_as_uint(w.z) >> 2))] : 1.f;
+            const uint32_t k = (S.K + kin + 4u * par) & 7u;
+            const float gi = E.npin ? s_gin[(S.pio ^ (pin4 >> 2))] : 1.f;
             kout[idx] = (uint8_t)k;
-            mout[idx] = F * gi * sqrtf(w.x * w.x + w.y * w.y);
+            mout[idx] = F * gi * sqrtf(wre * wre + wim * wim);
         }
     }
 }

@@ -98,11 +98,10 @@ Z8Image build_z8_image(const HostEngine& E) {
             }
         }
         kin &= 7;
-        float* e = &Z.inner[4 * i];
-        if (!phantom) { e[0] = (float)(fin * C8[kin]); e[1] = (float)(fin * S8[kin]); }
+        if (!phantom) { Z.inner[2 * i] = (float)(fin * C8[kin]); Z.inner[2 * i + 1] = (float)(fin * S8[kin]); }
         uint32_t pin4 = pin << 2;
-        std::memcpy(&e[2], &pin4, 4);
-        std::memcpy(&e[3], &kin, 4);
+        std::memcpy(&Z.inner[512 + i], &pin4, 4);
+        std::memcpy(&Z.inner[768 + i], &kin, 4);
     }
     // Lane-factor byte tables
     Z.nbytes = (t + 7) / 8;
