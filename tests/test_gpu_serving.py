@@ -54,7 +54,7 @@ def test_graph_replays_match_eager():
         got, ms = g.run()
         np.testing.assert_array_equal(got, eager)
         # eager work and a synchronisation between replays
-        sims[rep].amplitudes(X[:2], Y[:2])
+        S[4 * rep].amplitudes(X[4 * rep:4 * rep + 2], Y[4 * rep:4 * rep + 2])
 
 
 def test_server_results_equal_alone_results():
