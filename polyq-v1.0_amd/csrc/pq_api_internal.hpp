@@ -20,6 +20,10 @@ struct RankEng {
         std::lock_guard<std::mutex> lk(dev_mu);
         for (size_t i = dev.size(); i < host.size(); ++i) dev.push_back(dev_engine_upload(host[i], z8[i]));
     }
+    void drop_device() const {
+        std::lock_guard<std::mutex> lk(dev_mu);
+        dev.clear();
+    }
     size_t device_bytes() const {
         std::lock_guard<std::mutex> lk(dev_mu);
         size_t b = 0;
@@ -47,6 +51,12 @@ struct Sim {
         size_t b = eng ? eng->device_bytes() : 0;
         for (auto& kv : marg) b += kv.second->device_bytes();
         return b;
+    }
+    // Free every device image (host engines stay; the next use re-uploads).
+    void drop_device() {
+        std::lock_guard<std::mutex> lk(mu);
+        if (eng) eng->drop_device();
+        for (auto& kv : marg) kv.second->drop_device();
     }
     const RankEng& marginal_engine(const std::vector<uint32_t>& qubits);
     bool marginal_assignment(uint64_t x, const uint32_t* qubits, const uint8_t* outcomes, size_t k, std::vector<std::pair<uint32_t, uint8_t>>& a);

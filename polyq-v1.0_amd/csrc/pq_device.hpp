@@ -13,6 +13,8 @@ struct DevEngine;   // engine resident in HBM: plain FP64 image + Z8 walk image
 // Upload both device images of one engine to the current device.
 std::shared_ptr<DevEngine> dev_engine_upload(std::shared_ptr<const HostEngine> E, std::shared_ptr<const Z8Image> Z);
 size_t dev_engine_bytes(const DevEngine& D);
+// Exact device bytes the upload of (E, Z) will allocate (no device needed).
+size_t dev_engine_image_bytes(const HostEngine& E, const Z8Image& Z);
 const HostEngine& dev_host(const DevEngine& D);
 const Z8Image& dev_z8(const DevEngine& D);
 

@@ -171,14 +171,15 @@ uint32_t pick_nw(uint32_t bits) {
 }
 }  // namespace
 
-std::shared_ptr<DevEngine> dev_engine_upload(std::shared_ptr<const HostEngine> Ep, std::shared_ptr<const Z8Image> Zp) {
-    Ctx& C = ctx();
-    const HostEngine& E = *Ep;
-    const Z8Image& Z = *Zp;
+namespace {
+struct BlobLayout {
+    size_t o_ku, o_kn, o_soff, o_supp, o_a0, o_a1, o_up, o_dn, o_diag, o_mrow, o_we, o_sq, o_sn, o_sp, o_sl, o_in, o_lt, o_gin;
+    uint32_t NW, NWe;
+};
+// Host image of one engine: the plain FP64 part and the Z8 walk part in one blob.
+BlobLayout build_blob(const HostEngine& E, const Z8Image& Z, BlobBuilder& B) {
     const ULayout& L = E.L;
-    auto D = std::make_shared<DevEngine>();
-    D->host = Ep; D->z8 = Zp;
-    BlobBuilder B;
+    BlobLayout O;
     // plain image
     const uint32_t NW = pick_nw(std::max(L.g, L.nodd)), NWe = std::max<uint32_t>(2, (L.neven + 31) / 32 + ((L.neven + 31) / 32) % 2);
     const uint32_t W = L.words();
@@ -208,17 +209,38 @@ std::shared_ptr<DevEngine> dev_engine_upload(std::shared_ptr<const HostEngine> E
         up.push_back(E.up[k].real()); up.push_back(E.up[k].imag());
         dn.push_back(E.down[k].real()); dn.push_back(E.down[k].imag());
     }
-    size_t o_ku = B.put(ku.data(), ku.size() * 4), o_kn = B.put(E.kernel.data(), E.kernel.size() * 8);
-    size_t o_soff = B.put(soff.data(), soff.size() * 4), o_supp = B.put(supp.data(), supp.size());
-    size_t o_a0 = B.put(a0.data(), a0.size() * 8), o_a1 = B.put(a1.data(), a1.size() * 8);
-    size_t o_up = B.put(up.data(), up.size() * 8), o_dn = B.put(dn.data(), dn.size() * 8);
-    size_t o_diag = B.put(diag.data(), diag.size() * 4), o_mrow = B.put(mrow.data(), mrow.size() * 4), o_we = B.put(we.data(), we.size() * 4);
+    O.NW = NW; O.NWe = NWe;
+    O.o_ku = B.put(ku.data(), ku.size() * 4); O.o_kn = B.put(E.kernel.data(), E.kernel.size() * 8);
+    O.o_soff = B.put(soff.data(), soff.size() * 4); O.o_supp = B.put(supp.data(), supp.size());
+    O.o_a0 = B.put(a0.data(), a0.size() * 8); O.o_a1 = B.put(a1.data(), a1.size() * 8);
+    O.o_up = B.put(up.data(), up.size() * 8); O.o_dn = B.put(dn.data(), dn.size() * 8);
+    O.o_diag = B.put(diag.data(), diag.size() * 4); O.o_mrow = B.put(mrow.data(), mrow.size() * 4); O.o_we = B.put(we.data(), we.size() * 4);
     // Z8 image
     std::vector<uint32_t> lam32(Z.slot_lam.begin(), Z.slot_lam.end());
-    size_t o_sq = B.put(Z.slot_q.data(), Z.slot_q.size() * 8), o_sn = B.put(Z.slot_n.data(), Z.slot_n.size() * 8);
-    size_t o_sp = B.put(Z.slot_pin.data(), Z.slot_pin.size() * 4), o_sl = B.put(lam32.data(), lam32.size() * 4);
-    size_t o_in = B.put(Z.inner.data(), Z.inner.size() * 4), o_lt = B.put(Z.lt.data(), Z.lt.size() * 4);
-    size_t o_gin = B.put(Z.gin.data(), Z.gin.size() * 4);
+    O.o_sq = B.put(Z.slot_q.data(), Z.slot_q.size() * 8); O.o_sn = B.put(Z.slot_n.data(), Z.slot_n.size() * 8);
+    O.o_sp = B.put(Z.slot_pin.data(), Z.slot_pin.size() * 4); O.o_sl = B.put(lam32.data(), lam32.size() * 4);
+    O.o_in = B.put(Z.inner.data(), Z.inner.size() * 4); O.o_lt = B.put(Z.lt.data(), Z.lt.size() * 4);
+    O.o_gin = B.put(Z.gin.data(), Z.gin.size() * 4);
+    return O;
+}
+}  // namespace
+
+size_t dev_engine_image_bytes(const HostEngine& E, const Z8Image& Z) {
+    BlobBuilder B;
+    build_blob(E, Z, B);
+    return B.bytes.size();
+}
+
+std::shared_ptr<DevEngine> dev_engine_upload(std::shared_ptr<const HostEngine> Ep, std::shared_ptr<const Z8Image> Zp) {
+    Ctx& C = ctx();
+    const HostEngine& E = *Ep;
+    const Z8Image& Z = *Zp;
+    const ULayout& L = E.L;
+    auto D = std::make_shared<DevEngine>();
+    D->host = Ep; D->z8 = Zp;
+    BlobBuilder B;
+    const BlobLayout O = build_blob(E, Z, B);
+    const uint32_t NW = O.NW, NWe = O.NWe;
 
     std::lock_guard<std::mutex> lk(C.mu);
     HIPCHECK(hipMalloc(&D->blob, B.bytes.size()));
@@ -227,14 +249,14 @@ std::shared_ptr<DevEngine> dev_engine_upload(std::shared_ptr<const HostEngine> E
     HIPCHECK(hipStreamSynchronize(C.stream));
     auto at = [&](size_t off) { return (const uint8_t*)D->blob + off; };
     PlainDev& P = D->plain;
-    P.ku = (const uint32_t*)at(o_ku); P.kn = (const uint64_t*)at(o_kn); P.soff = (const uint32_t*)at(o_soff); P.supp = at(o_supp);
-    P.a0 = (const double2*)at(o_a0); P.a1 = (const double2*)at(o_a1); P.up = (const double2*)at(o_up); P.down = (const double2*)at(o_dn);
-    P.diag = (const uint32_t*)at(o_diag); P.mrow = (const uint32_t*)at(o_mrow); P.we = (const uint32_t*)at(o_we);
+    P.ku = (const uint32_t*)at(O.o_ku); P.kn = (const uint64_t*)at(O.o_kn); P.soff = (const uint32_t*)at(O.o_soff); P.supp = at(O.o_supp);
+    P.a0 = (const double2*)at(O.o_a0); P.a1 = (const double2*)at(O.o_a1); P.up = (const double2*)at(O.o_up); P.down = (const double2*)at(O.o_dn);
+    P.diag = (const uint32_t*)at(O.o_diag); P.mrow = (const uint32_t*)at(O.o_mrow); P.we = (const uint32_t*)at(O.o_we);
     P.d = E.d; P.t = E.t; P.NW = NW; P.NWe = NWe; P.sigma = E.sigma;
     P.rbits = 2 * L.g + 2 * L.neven + L.nodd; P.neg = E.neg_mask;
     Z8Dev& z = D->z;
-    z.slot_q = (const uint64_t*)at(o_sq); z.slot_n = (const uint64_t*)at(o_sn); z.slot_pin = (const uint32_t*)at(o_sp); z.slot_lam = (const uint32_t*)at(o_sl);
-    z.inner = (const float*)at(o_in); z.lt = (const float*)at(o_lt); z.gin = (const float*)at(o_gin);
+    z.slot_q = (const uint64_t*)at(O.o_sq); z.slot_n = (const uint64_t*)at(O.o_sn); z.slot_pin = (const uint32_t*)at(O.o_sp); z.slot_lam = (const uint32_t*)at(O.o_sl);
+    z.inner = (const float*)at(O.o_in); z.lt = (const float*)at(O.o_lt); z.gin = (const float*)at(O.o_gin);
     z.nslots = Z.nslots; z.nbytes = Z.nbytes; z.npin = Z.npin; z.pad = 0;
     return D;
 }

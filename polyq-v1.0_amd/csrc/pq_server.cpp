@@ -28,15 +28,20 @@ namespace {
 
 double now_s() { return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count(); }
 
+// One circuit: the host build is kept for the server's lifetime, the device
+// images only while resident.  `pinned` counts requests whose device jobs
+// exist (prepared, queued or running); only unpinned entries are evictable,
+// and requests still waiting for a worker pin nothing.
 struct Entry {
     uint64_t key = 0;
     std::mutex build_mu;
     bool built = false;
     std::string err;
     std::unique_ptr<Sim> sim;
-    size_t bytes = 0;
+    size_t bytes = 0;          // device bytes accounted while resident
+    bool resident = false;
     uint64_t last_use = 0;
-    int inflight = 0;
+    int pinned = 0;
 };
 
 struct Req {
@@ -96,31 +101,33 @@ struct Server {
         dispatcher.join();
     }
 
-    // ---- admission: make room for `bytes` of engine images in the HBM budget
-    bool admit(size_t bytes, const std::shared_ptr<Entry>& self) {
+    // ---- admission: make room for `bytes` of engine images in the HBM budget.
+    // Called with the requesting entry pinned, so it is never its own victim.
+    bool admit(size_t bytes) {
         std::unique_lock<std::mutex> lk(mu);
         for (;;) {
             if (resident.load() + bytes <= budget) { resident += bytes; return true; }
-            // evict the least recently used entry with nothing in flight
+            // evict the least recently used resident entry that nothing pins
             std::shared_ptr<Entry> victim;
             for (auto& kv : entries) {
                 auto& e = kv.second;
-                if (e == self || e->inflight > 0 || !e->built || e->bytes == 0) continue;
+                if (!e->resident || e->pinned > 0) continue;
                 if (!victim || e->last_use < victim->last_use) victim = e;
             }
             if (victim) {
-                entries.erase(victim->key);
+                victim->sim->drop_device();
                 resident -= victim->bytes;
                 victim->bytes = 0;
+                victim->resident = false;
                 evictions++;
                 continue;
             }
-            if (bytes > budget) return false;              // can never fit
-            if (stop) return false;
-            cv_mem.wait_for(lk, std::chrono::milliseconds(5));   // back-pressure until memory frees
+            if (bytes > budget || stop) return false;      // can never fit / shutting down
+            cv_mem.wait_for(lk, std::chrono::milliseconds(5));   // back-pressure until a batch retires
         }
     }
 
+    // Host build once per circuit (src/sim.rs:144-148, src/rank.rs:88-123).
     void build_entry(const std::shared_ptr<Entry>& E, const Req& r) {
         std::lock_guard<std::mutex> bl(E->build_mu);
         if (E->built || !E->err.empty()) return;
@@ -130,14 +137,6 @@ struct Server {
             s->strategy = r.strategy;
             s->ps = compile(s->circ);
             s->eng = build_rank(s->ps, s->ps.io_vars(), r.strategy, 0);
-            size_t est = 0;
-            for (auto& z : s->eng->z8) est += 4096 + z->inner.size() * 4 + z->lt.size() * 4 + z->gin.size() * 4 + z->nslots * 32;
-            for (auto& h : s->eng->host) est += (h->kernel_u.size() + h->mcols.size()) * 8 + h->t * 64 + 8192;
-            if (!admit(est, E)) throw std::runtime_error("engine does not fit the HBM budget");
-            s->eng->ensure_device();
-            size_t real = s->eng->device_bytes();
-            resident += real; resident -= est;
-            E->bytes = real;
             E->sim = std::move(s);
             E->built = true;
             std::lock_guard<std::mutex> lk(mu);
@@ -145,6 +144,37 @@ struct Server {
         } catch (std::exception& e) {
             E->err = e.what();
         }
+    }
+
+    static size_t estimate_bytes(const RankEng& R) {
+        size_t b = 0;
+        for (size_t i = 0; i < R.host.size(); ++i) b += dev_engine_image_bytes(*R.host[i], *R.z8[i]);
+        return b;
+    }
+
+    // Upload the engine set `R` of entry E if it is not on the device,
+    // within the budget.  E is pinned by the caller.
+    void make_resident(const std::shared_ptr<Entry>& E, const RankEng& R) {
+        std::lock_guard<std::mutex> bl(E->build_mu);
+        if (R.dev.size() == R.host.size()) return;
+        const size_t est = estimate_bytes(R);
+        if (!admit(est)) throw std::runtime_error("engine does not fit the HBM budget");
+        try {
+            R.ensure_device();
+        } catch (...) {
+            resident -= est;
+            throw;
+        }
+        const size_t real = R.device_bytes();
+        std::lock_guard<std::mutex> lk(mu);
+        resident += real;
+        resident -= est;
+        E->bytes += real;
+        E->resident = true;
+    }
+
+    void unpin(Req& r) {   // with mu held
+        if (r.entry) { r.entry->pinned--; r.entry.reset(); }
     }
 
     void worker() {
@@ -155,29 +185,22 @@ struct Server {
                 cv_work.wait(lk, [&] { return stop || !work_q.empty(); });
                 if (stop) return;
                 r = work_q.front(); work_q.pop_front();
+                r->entry->pinned++;
+                r->entry->last_use = ++use_clock;
             }
             try {
                 build_entry(r->entry, *r);
                 if (!r->entry->err.empty()) throw std::runtime_error(r->entry->err);
                 Sim& S = *r->entry->sim;
                 std::vector<std::pair<uint32_t, uint8_t>> asg;
+                std::vector<size_t> owner;
                 if (r->kind == 2) {
-                    const size_t before = S.device_bytes();
                     const RankEng& R = S.marginal_engine(r->mq);
-                    R.ensure_device();
-                    const size_t grown = S.device_bytes() - before;
-                    if (grown) {
-                        std::lock_guard<std::mutex> lk(mu);
-                        resident += grown;
-                        r->entry->bytes += grown;
-                    }
-                    if (S.marginal_assignment(r->x, r->mq.data(), r->mo.data(), r->mq.size(), asg)) {
-                        std::vector<size_t> owner;
-                        add_jobs(R, asg, r->jobs, owner, 0, nullptr);
-                    }
-                } else if (S.assignment(r->x, r->y, asg)) {
-                    std::vector<size_t> owner;
-                    add_jobs(*S.eng, asg, r->jobs, owner, 0, nullptr);
+                    make_resident(r->entry, R);
+                    if (S.marginal_assignment(r->x, r->mq.data(), r->mo.data(), r->mq.size(), asg)) add_jobs(R, asg, r->jobs, owner, 0, nullptr);
+                } else {
+                    make_resident(r->entry, *S.eng);
+                    if (S.assignment(r->x, r->y, asg)) add_jobs(*S.eng, asg, r->jobs, owner, 0, nullptr);
                 }
                 for (auto& J : r->jobs) {
                     const uint32_t d = dev_z8(*J.eng).d;
@@ -189,10 +212,11 @@ struct Server {
             } catch (std::exception& e) {
                 std::lock_guard<std::mutex> lk(mu);
                 r->status = -1; r->err = e.what(); r->t_done = now_s();
-                r->entry->inflight--;
-                r->entry.reset();
+                r->jobs.clear();
+                unpin(*r);
                 failed++;
                 cv_done.notify_all();
+                cv_mem.notify_all();
                 continue;
             }
             cv_ready.notify_one();
@@ -210,9 +234,8 @@ struct Server {
             else if (r->kind == 1) r->val = {std::norm(a), 0};
             else r->val = {std::max(a.real(), 0.0), 0};        // sim.rs:230
             r->status = 1; r->t_done = t;
-            r->entry->inflight--;
-            r->entry.reset();
             r->jobs.clear();
+            unpin(*r);
             completed++;
         }
         cv_done.notify_all();
@@ -242,7 +265,7 @@ struct Server {
                 if (!slots[0]) {
                     std::vector<std::complex<double>> none;
                     std::lock_guard<std::mutex> lk(mu);
-                    for (auto& r : batch) { r->status = -1; r->err = "no HIP device available"; r->t_done = now_s(); r->entry->inflight--; r->entry.reset(); failed++; }
+                    for (auto& r : batch) { r->status = -1; r->err = "no HIP device available"; r->t_done = now_s(); r->jobs.clear(); unpin(*r); failed++; }
                     cv_done.notify_all();
                     continue;
                 }
@@ -293,8 +316,6 @@ struct Server {
                 it = entries.emplace(key, e).first;
             }
             r->entry = it->second;
-            r->entry->inflight++;
-            r->entry->last_use = ++use_clock;
             r->ticket = next_ticket++;
             reqs[r->ticket] = r;
             work_q.push_back(r);
