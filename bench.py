@@ -35,7 +35,19 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="only run three 256 MiB device copies (reference traffic for FETCH_SIZE/WRITE_SIZE counters)")
     args = ap.parse_args()
+    if args.copy_calibration:
+        import torch
+        n = 64 << 20
+        a = torch.ones(n, device="cuda", dtype=torch.float32)
+        b = torch.empty_like(a)
+        for _ in range(3):
+            b.copy_(a)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_calibration_bytes_per_kernel": n * 4, "kernels": 3}))
+        return
     if args.gpus != 1:
         print("bench.py: this build runs on one GPU only; --gpus %d is not supported (the 2^d branch range would shard by "
               "coset index across GPUs with one 64-byte reduction per query)" % args.gpus, file=sys.stderr)
@@ -86,13 +98,15 @@ def main():
         }
 
     # ---- roofline of the walk kernel: VALU issue rate
-    # ops per branch of k_z8_walk: sign update + 2 FMA (+1 address xor and
-    # +1 sign xor when inner directions touch pivots), plus the per-outer-point
-    # work (slot flip, lane factor, fold: about 70 lane-ops) over 256 branches.
-    ops_per_branch = (3.0 if info["inner_pivots"] == 0 else 5.0) + 70.0 / 256.0
+    # Algorithmic work per branch of k_z8_walk: the inner-table step (one sign
+    # update and the FMAs into the accumulators; one more lane-op for the
+    # LDS address when inner directions touch pivots).  Per-outer-point,
+    # seed and reduction work is overhead, not counted.
+    ops_per_branch = float(info["ops_per_branch"])
     cus = int(dev.split("/")[-1].split()[0]) if "CUs" in dev else 256
-    clock_ghz = 2.4
-    peak = cus * 64 * clock_ghz * 1e9
+    # 128 lanes/clk/CU for FP32/int VALU ops (157.3 TFLOP/s FP32 / 2 per FMA);
+    # rocprof counters on this kernel confirm issue above 64 lanes/clk/CU.
+    peak = cus * 128 * 2.4e9
     achieved = branches_per_step * ops_per_branch / (kernel_avg_ms * 1e-3)
     roofline = {
         "bound": "valu",

@@ -134,8 +134,10 @@ int pq_sim_walk_dump(pq_sim* sim, uint64_t x, uint64_t y, uint32_t mid_bits, siz
 /* Host-only check of the Z8 quadratic form against the direct phase for
  * `trials` random coset points; returns the number of mismatches in *bad. */
 int pq_sim_selfcheck(pq_sim* sim, uint64_t x, uint64_t y, uint32_t trials, uint64_t seed, uint64_t* bad);
-/* Z8 image summary: d, inner coordinates, slots, compact inner pivots, rank K. */
-int pq_sim_z8_info(pq_sim* sim, uint32_t out5[5]);
+/* Z8 image summary: d, inner directions, slots, compact inner pivots,
+ * rank K, whether inner entries alternate real/imaginary, VALU lane-ops per
+ * branch of the inner step. */
+int pq_sim_z8_info(pq_sim* sim, uint32_t out7[7]);
 
 const char* pq_last_error(void);
 /* Device name, architecture and CU count; returns -1 without a device. */

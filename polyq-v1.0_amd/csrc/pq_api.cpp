@@ -351,7 +351,7 @@ int pq_sim_walk_dump(pq_sim* h, uint64_t x, uint64_t y, uint32_t mid_bits, size_
             uint64_t sv = P.s0;
             for (uint32_t p = 0; p < Z.nslots; ++p) {
                 bool bit = p < 8 ? ((gi >> p) & 1) : (p < 8 + M ? ((go >> (p - 8)) & 1) : ((hi >> (p - 8 - M)) & 1));
-                if (bit && Z.slot_coord[p] >= 0) sv ^= Z.slot_n[p];
+                if (bit && Z.slot_real[p]) sv ^= Z.slot_n[p];
             }
             s[idx] = sv;
         }
@@ -393,7 +393,7 @@ int pq_sim_selfcheck(pq_sim* h, uint64_t x, uint64_t y, uint32_t trials, uint64_
 int pq_sim_z8_info(pq_sim* h, uint32_t* o) {
     PQ_TRY({
         const Z8Image& Z = *h->s.eng->z8[0];
-        o[0] = Z.d; o[1] = Z.nin; o[2] = Z.nslots; o[3] = Z.npin; o[4] = h->s.eng->host[0]->rk;
+        o[0] = Z.d; o[1] = Z.nin; o[2] = Z.nslots; o[3] = Z.npin; o[4] = h->s.eng->host[0]->rk; o[5] = Z.imag0 ? 1 : 0; o[6] = Z.ops_per_branch();
         return 0;
     })
 }

@@ -138,6 +138,34 @@ uint32_t total_phase(const HostEngine& E, const uint64_t* u, uint64_t s) {
     return (phase8_row(E, u) + 6 * ns + 4 * nn) & 7;
 }
 
+// Z8 form data of a kernel direction (N over the terms, U its packed shift
+// row, e part zero).  Derivation: expand phase8(u0 + sum c_j U_j) + 6|s| +
+// 4|neg \ s| with |x + y| = |x| + |y| - 2|x & y| mod 4; every cross term is a
+// multiple of 4 and every linear coefficient is even.
+uint32_t z8_lambda(const HostEngine& E, const uint64_t* U, uint64_t N) {
+    const ULayout& L = E.L;
+    const uint64_t* D = E.diag.data();
+    const uint64_t* A = U + L.off_a(); const uint64_t* B = U + L.off_b(); const uint64_t* Y = U + L.off_o();
+    uint32_t l = 2 * row_popc(Y, L.Wo) + 4 * (row_and_popc(Y, D + L.off_o(), L.Wo) & 1)
+               + 4 * row_parity(A, B, L.Wg) + 4 * row_parity(A, D + L.off_b(), L.Wg) + 4 * row_parity(D + L.off_a(), B, L.Wg)
+               + 6 * (uint32_t)__builtin_popcountll(N) + 4 * ((uint32_t)__builtin_popcountll(E.neg_mask & N) & 1);
+    return l & 7;
+}
+bool z8_cross(const HostEngine& E, const uint64_t* U1, uint64_t N1, const uint64_t* U2, uint64_t N2) {
+    const ULayout& L = E.L;
+    return row_parity(U1 + L.off_o(), U2 + L.off_o(), L.Wo) ^ row_parity(U1 + L.off_a(), U2 + L.off_b(), L.Wg)
+         ^ row_parity(U2 + L.off_a(), U1 + L.off_b(), L.Wg) ^ ((__builtin_popcountll(N1 & N2) & 1) != 0);
+}
+// Row whose parity with the query's start shift gives the query bit of a
+// direction: <ua, b> + <ub, a> + <uo, y>.
+void z8_dual(const HostEngine& E, const uint64_t* U, uint64_t* out) {
+    const ULayout& L = E.L;
+    std::fill(out, out + L.words(), 0ull);
+    std::copy_n(U + L.off_b(), L.Wg, out + L.off_a());
+    std::copy_n(U + L.off_a(), L.Wg, out + L.off_b());
+    std::copy_n(U + L.off_o(), L.Wo, out + L.off_o());
+}
+
 // ---------------------------------------------------------------- engine --
 std::shared_ptr<HostEngine> build_engine(const PathSum& ps, const std::vector<uint32_t>& fixed) {
     auto Ep = std::make_shared<HostEngine>();
@@ -290,26 +318,12 @@ std::shared_ptr<HostEngine> build_engine(const PathSum& ps, const std::vector<ui
     E.Q.assign(d, 0);
     E.lam.assign(d, 0);
     E.dual.assign((size_t)d * W, 0);
-    const uint64_t* D = E.diag.data();
     for (uint32_t j = 0; j < d; ++j) {
         const uint64_t* U = &E.kernel_u[(size_t)j * W];
-        const uint64_t* A = U + L.off_a(); const uint64_t* Bq = U + L.off_b(); const uint64_t* Y = U + L.off_o();
-        uint64_t Nj = E.kernel[j];
-        uint32_t l = 2 * row_popc(Y, L.Wo) + 4 * (row_and_popc(Y, D + L.off_o(), L.Wo) & 1)
-                   + 4 * row_parity(A, Bq, L.Wg) + 4 * row_parity(A, D + L.off_b(), L.Wg) + 4 * row_parity(D + L.off_a(), Bq, L.Wg)
-                   + 6 * (uint32_t)__builtin_popcountll(Nj) + 4 * ((uint32_t)__builtin_popcountll(E.neg_mask & Nj) & 1);
-        E.lam[j] = (uint8_t)(l & 7);
-        // dual row: pairs with the query shift u_start: <ua, b_j> + <ub, a_j> + <uo, y_j>
-        uint64_t* Dd = &E.dual[(size_t)j * W];
-        std::copy_n(Bq, L.Wg, Dd + L.off_a());
-        std::copy_n(A, L.Wg, Dd + L.off_b());
-        std::copy_n(Y, L.Wo, Dd + L.off_o());
-        for (uint32_t k = j + 1; k < d; ++k) {
-            const uint64_t* U2 = &E.kernel_u[(size_t)k * W];
-            bool q = row_parity(Y, U2 + L.off_o(), L.Wo) ^ row_parity(A, U2 + L.off_b(), L.Wg) ^ row_parity(U2 + L.off_a(), Bq, L.Wg)
-                   ^ ((__builtin_popcountll(Nj & E.kernel[k]) & 1) != 0);
-            if (q) { E.Q[j] |= 1ull << k; E.Q[k] |= 1ull << j; }
-        }
+        E.lam[j] = z8_lambda(E, U, E.kernel[j]);
+        z8_dual(E, U, &E.dual[(size_t)j * W]);
+        for (uint32_t k = j + 1; k < d; ++k)
+            if (z8_cross(E, U, E.kernel[j], &E.kernel_u[(size_t)k * W], E.kernel[k])) { E.Q[j] |= 1ull << k; E.Q[k] |= 1ull << j; }
     }
 
     for (uint32_t f : fixed) { E.fixed_v.push_back(ps.v[f]); E.fixed_theta.push_back(ps.theta[f]); }

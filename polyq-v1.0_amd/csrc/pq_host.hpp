@@ -184,6 +184,10 @@ bool prepare_assignment(const HostEngine& E, const std::vector<std::pair<uint32_
 // Host evaluation of the Gauss-sum phase (normal_form.rs:479-493) for a packed
 // shift row, and of the total per-branch phase of the factorised weight.
 uint32_t phase8_row(const HostEngine& E, const uint64_t* u);
+// Z8 form data of an arbitrary kernel direction (N over terms, U packed row).
+uint32_t z8_lambda(const HostEngine& E, const uint64_t* U, uint64_t N);
+bool z8_cross(const HostEngine& E, const uint64_t* U1, uint64_t N1, const uint64_t* U2, uint64_t N2);
+void z8_dual(const HostEngine& E, const uint64_t* U, uint64_t* out);
 bool survives_row(const HostEngine& E, const uint64_t* u);
 uint32_t total_phase(const HostEngine& E, const uint64_t* u, uint64_t s);
 
@@ -195,16 +199,20 @@ uint32_t total_phase(const HostEngine& E, const uint64_t* u, uint64_t s);
 constexpr uint32_t kInner = 8;
 struct Z8Image {
     uint32_t d = 0, nslots = 0, nin = 0, t = 0, npin = 0, nbytes = 0;
-    std::vector<int32_t> slot_coord;   // kernel-direction index per slot, -1 for a phantom
+    bool imag0 = false;                // inner entries alternate real / imaginary with slot 0
+    std::vector<uint64_t> slot_u;      // packed shift row of each slot's direction (host only)
+    std::vector<uint64_t> slot_dual;   // query dual row of each slot (host only)
     std::vector<uint64_t> slot_q;      // Q rows over slots
-    std::vector<uint64_t> slot_n;      // kernel vector N over the t terms
+    std::vector<uint64_t> slot_n;      // kernel vector N over the t terms (0 for a phantom)
     std::vector<uint32_t> slot_pin;    // inner-touched pivot bits of N, compacted
     std::vector<uint8_t> slot_lam;     // query-independent linear coefficient
-    std::vector<float> inner;          // 256 x (w_re, w_im), 256 x pin*4 (bits), 256 x kin (bits)
+    std::vector<uint8_t> slot_real;    // slot holds a direction (not a phantom)
+    std::vector<float> inner;          // 256 signed values, 256 x pin*4 (bits), 256 x kin (bits)
     std::vector<float> lt;             // nbytes x 256 lane-factor byte tables
     std::vector<float> gin;            // 2^npin inner pivot factor table
     std::vector<uint32_t> pin_term;    // term index of each compact inner pivot bit
     uint64_t lane_terms = 0;           // terms whose ratio is carried by the lane factor
+    uint32_t ops_per_branch() const { return npin ? 4u : 2u; }
 };
 Z8Image build_z8_image(const HostEngine& E);
 // Query part of the Z8 walk: K0, delta bits over slots, s0, compact pivots.

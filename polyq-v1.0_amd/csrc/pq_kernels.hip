@@ -122,10 +122,10 @@ struct Z8Dev {
     const uint64_t* slot_n;
     const uint32_t* slot_pin;
     const uint32_t* slot_lam;
-    const float* inner;      // 256 x (w_re, w_im), then 256 x pin*4, then 256 x kin
+    const float* inner;      // 256 signed values, then 256 x pin*4, then 256 x kin
     const float* lt;         // nbytes x 256
     const float* gin;        // 2^npin
-    uint32_t nslots, nbytes, npin, pad;
+    uint32_t nslots, nbytes, npin, imag0;
 };
 struct Z8Q {                 // one query of a batch (64 B)
     uint64_t delta, s0, item0, nitems;
@@ -257,7 +257,7 @@ std::shared_ptr<DevEngine> dev_engine_upload(std::shared_ptr<const HostEngine> E
     Z8Dev& z = D->z;
     z.slot_q = (const uint64_t*)at(O.o_sq); z.slot_n = (const uint64_t*)at(O.o_sn); z.slot_pin = (const uint32_t*)at(O.o_sp); z.slot_lam = (const uint32_t*)at(O.o_sl);
     z.inner = (const float*)at(O.o_in); z.lt = (const float*)at(O.o_lt); z.gin = (const float*)at(O.o_gin);
-    z.nslots = Z.nslots; z.nbytes = Z.nbytes; z.npin = Z.npin; z.pad = 0;
+    z.nslots = Z.nslots; z.nbytes = Z.nbytes; z.npin = Z.npin; z.imag0 = Z.imag0 ? 1u : 0u;
     return D;
 }
 
@@ -503,42 +503,47 @@ __device__ __forceinline__ float lane_factor(const float* s_lt, uint32_t nbytes,
     return F;
 }
 
-// The 256 inner branches of one outer point, folded every FOLD steps.  The
-// uniform table is read 16 steps at a time (contiguous re/im pairs, then the
-// inner pivot offsets), which the compiler issues as wide scalar loads.
-template <bool HASG, int FOLD>
-__device__ __forceinline__ void inner_walk(const float* Wg, const float* s_gin, const LaneState& S, float F, float (&acc)[8]) {
-    const PQ_CONST float* Wri = cview(Wg);                                   // 256 x (re, im)
-    const PQ_CONST uint32_t* Wp = cview((const uint32_t*)(Wg + 512));        // 256 x pin*4
+// The 256 inner branches of one outer point, folded every FOLD steps.  Each
+// table entry is real or imaginary; with IMAG the class is bit 0 of gray(i)
+// (the builder arranges the inner basis so), without it every entry is real.
+// Per branch: one XOR for the sign (and, with HASG, an LDS lookup of the
+// inner pivot factor) and one FMA into the accumulator of the entry's class.
+// The uniform table is read 16 steps at a time as wide scalar loads.
+template <bool HASG, bool IMAG, int FOLD>
+__device__ __forceinline__ void inner_walk(const float* Wg, const uint32_t (&ps)[8], const float* s_gin, const LaneState& S, float F, float (&acc)[8]) {
+    const PQ_CONST float* Wv = cview(Wg);                                    // 256 signed values
     uint32_t sm[8];
 #pragma unroll
     for (int j = 0; j < 8; ++j) sm[j] = (S.hs << (31 - j)) & 0x80000000u;
     uint32_t sgn = 0;
-    const uint32_t pio4 = S.pio << 2;
+    uint32_t addr = S.pio << 2;      // LDS byte offset of gin[pio ^ pin(gray(i))], same Gray recurrence as the sign
     float tr = 0.f, ti = 0.f;
 #pragma unroll
     for (int blk = 0; blk < 16; ++blk) {
-        float w[32];
-        uint32_t pn[16];
+        float w[16];
+        // An empty asm on the block pointer keeps each block's scalar loads
+        // next to their use: the table lives in the constant address space,
+        // so otherwise all 256 entries are hoisted and the SGPRs spill.
+        const PQ_CONST float* bw = Wv + 16 * blk;
+        asm volatile("" : "+s"(bw) : : "memory");
 #pragma unroll
-        for (int k = 0; k < 32; ++k) w[k] = Wri[32 * blk + k];
-        if constexpr (HASG) {
-#pragma unroll
-            for (int k = 0; k < 16; ++k) pn[k] = Wp[16 * blk + k];
-        }
+        for (int k = 0; k < 16; ++k) w[k] = bw[k];
 #pragma unroll
         for (int ii = 0; ii < 16; ++ii) {
             const int i = 16 * blk + ii;
-            if (i) sgn ^= sm[__builtin_ctz(i)];
+            if (i) {
+                sgn ^= sm[__builtin_ctz(i)];
+                if constexpr (HASG) addr ^= ps[__builtin_ctz(i)];
+            }
             float m;
             if constexpr (HASG) {
-                const float g = *(const float*)((const char*)s_gin + (pio4 ^ pn[ii]));
+                const float g = *(const float*)((const char*)s_gin + addr);
                 m = __uint_as_float(__float_as_uint(g) ^ sgn);
             } else {
                 m = __uint_as_float(0x3f800000u ^ sgn);
             }
-            tr = fmaf(m, w[2 * ii], tr);
-            ti = fmaf(m, w[2 * ii + 1], ti);
+            if (IMAG && ((i ^ (i >> 1)) & 1)) ti = fmaf(m, w[ii], ti);
+            else tr = fmaf(m, w[ii], tr);
             if (((i + 1) % FOLD) == 0) {
                 fold(tr * F, ti * F, S.K, acc);
                 tr = 0.f; ti = 0.f;
@@ -547,9 +552,13 @@ __device__ __forceinline__ void inner_walk(const float* Wg, const float* s_gin, 
     }
 }
 
-template <int FOLD>
-__global__ __launch_bounds__(256) void k_z8_walk(const Z8Dev* __restrict__ engs, const Z8Q* __restrict__ qs, uint32_t nq, uint64_t total,
-                                                 unsigned long long* __restrict__ counter, double* __restrict__ partial) {
+// One instantiation per inner-step variant (pivot lookups or not, imaginary
+// class pattern or not); a batch launches one persistent grid per variant
+// present, each over its own contiguous range of queries and items.
+template <int FOLD, bool HASG, bool IMAG>
+__global__ __launch_bounds__(256) void k_z8_walk(const Z8Dev* __restrict__ engs, const Z8Q* __restrict__ qs, uint32_t q_lo, uint32_t q_hi,
+                                                 uint64_t item_begin, uint64_t item_end, unsigned long long* __restrict__ counter,
+                                                 double* __restrict__ partial) {
     __shared__ float s_lt[8 * 256];
     __shared__ float s_gin[4096];
     __shared__ unsigned long long s_item;
@@ -559,17 +568,17 @@ __global__ __launch_bounds__(256) void k_z8_walk(const Z8Dev* __restrict__ engs,
     if (tid == 0) s_eng = -1;
     for (;;) {
         __syncthreads();
-        if (tid == 0) s_item = atomicAdd(counter, 1ull);
+        if (tid == 0) s_item = item_begin + atomicAdd(counter, 1ull);
         __syncthreads();
         const uint64_t item = s_item;
-        if (item >= total) break;
-        uint32_t lo = 0, hi = nq;
+        if (item >= item_end) break;
+        uint32_t lo = q_lo, hi = q_hi;
         while (hi - lo > 1) { const uint32_t mid = (lo + hi) >> 1; if (cview(qs)[mid].item0 <= item) lo = mid; else hi = mid; }
         const Z8Q Q = cload(qs + lo);
         const Z8Dev E = cload(engs + Q.eng);
         if ((int)Q.eng != s_eng) {
             for (uint32_t k = tid; k < E.nbytes * 256u; k += 256) s_lt[k] = E.lt[k];
-            if (E.npin) for (uint32_t k = tid; k < (1u << E.npin); k += 256) s_gin[k] = E.gin[k];
+            if constexpr (HASG) for (uint32_t k = tid; k < (1u << E.npin); k += 256) s_gin[k] = E.gin[k];
             __syncthreads();
             if (tid == 0) s_eng = (int)Q.eng;
         }
@@ -578,12 +587,14 @@ __global__ __launch_bounds__(256) void k_z8_walk(const Z8Dev* __restrict__ engs,
         if ((lane >> Q.H) == 0) {
             LaneState S;
             seed_lane(E, Q, lane, S);
+            uint32_t ps[8];
+#pragma unroll
+            for (int j = 0; j < 8; ++j) ps[j] = HASG ? (cview(E.slot_pin)[j] << 2) : 0u;
             const uint32_t nout = 1u << Q.M;
             for (uint32_t o = 0; o < nout; ++o) {
                 if (o) flip_mid(E, Q, 8u + (uint32_t)__builtin_ctz(o), S);
                 const float F = lane_factor(s_lt, E.nbytes, S.sout);
-                if (E.npin) inner_walk<true, FOLD>(E.inner, s_gin, S, F, acc);
-                else inner_walk<false, FOLD>(E.inner, s_gin, S, F, acc);
+                inner_walk<HASG, IMAG, FOLD>(E.inner, ps, s_gin, S, F, acc);
             }
         }
         // workgroup reduction of the four compensated pairs (FP32 two-sum)
@@ -646,7 +657,7 @@ __global__ __launch_bounds__(256) void k_z8_reduce(const Z8Q* __restrict__ qs, c
     }
 }
 
-__global__ void k_zero_u64(unsigned long long* p) { *p = 0ull; }
+__global__ void k_zero_u64(unsigned long long* p) { p[threadIdx.x] = 0ull; }   // one thread per counter
 
 uint32_t z8_auto_mid_bits(uint32_t d, uint64_t /*batch_branches*/) {
     // A lane walks 2^(8+M) branches and 2^H lanes cover the query, H = d-8-M.
@@ -659,26 +670,38 @@ uint32_t z8_auto_mid_bits(uint32_t d, uint64_t /*batch_branches*/) {
 }
 
 namespace {
+constexpr int kVariants = 4;   // bit 1: inner pivot lookups, bit 0: imaginary class pattern
+int variant_of(const Z8Image& Z) { return (Z.npin ? 2 : 0) | (Z.imag0 ? 1 : 0); }
 struct BatchPlan {
     std::vector<Z8Dev> engs;
     std::vector<Z8Q> qs;
     std::vector<double2> scale;
     std::vector<uint32_t> order;
     uint64_t total_items = 0, branches = 0;
+    uint32_t q_lo[kVariants] = {0, 0, 0, 0}, q_hi[kVariants] = {0, 0, 0, 0};
+    uint64_t it_lo[kVariants] = {0, 0, 0, 0}, it_hi[kVariants] = {0, 0, 0, 0};
 };
 BatchPlan plan_batch(const std::vector<Z8Job>& jobs) {
     BatchPlan B;
     std::vector<const DevEngine*> seen;
     std::vector<uint32_t> idx(jobs.size());
     std::iota(idx.begin(), idx.end(), 0u);
-    // heavy items first; jobs of one engine stay together
+    // grouped by kernel variant; heavy items first; jobs of one engine together
     std::stable_sort(idx.begin(), idx.end(), [&](uint32_t a, uint32_t b) {
+        const int va = variant_of(*jobs[a].eng->z8), vb = variant_of(*jobs[b].eng->z8);
+        if (va != vb) return va < vb;
         if (jobs[a].mid_bits != jobs[b].mid_bits) return jobs[a].mid_bits > jobs[b].mid_bits;
         return jobs[a].eng < jobs[b].eng;
     });
+    int cur = -1;
     for (uint32_t ji : idx) {
         const Z8Job& J = jobs[ji];
         const Z8Image& Z = *J.eng->z8;
+        const int v = variant_of(Z);
+        if (v != cur) {
+            B.q_lo[v] = (uint32_t)B.qs.size(); B.it_lo[v] = B.total_items;
+            cur = v;
+        }
         auto it = std::find(seen.begin(), seen.end(), J.eng);
         uint32_t e;
         if (it == seen.end()) { e = (uint32_t)seen.size(); seen.push_back(J.eng); B.engs.push_back(J.eng->z); }
@@ -696,20 +719,40 @@ BatchPlan plan_batch(const std::vector<Z8Job>& jobs) {
         B.qs.push_back(q);
         B.scale.push_back(make_double2(J.scale.real(), J.scale.imag()));
         B.order.push_back(ji);
+        B.q_hi[v] = (uint32_t)B.qs.size(); B.it_hi[v] = B.total_items;
     }
     return B;
 }
-template <int FOLD>
-uint32_t walk_grid(Ctx& C, uint64_t items) {
+template <int FOLD, bool HASG, bool IMAG>
+uint32_t walk_grid_v(Ctx& C, uint64_t items) {
     int per_cu = 0;
-    HIPCHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_z8_walk<FOLD>, 256, 0));
-    uint64_t g = (uint64_t)std::max(per_cu, 1) * (uint64_t)C.cus;
+    HIPCHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_z8_walk<FOLD, HASG, IMAG>, 256, 0));
+    const uint64_t g = (uint64_t)std::max(per_cu, 1) * (uint64_t)C.cus;
     return (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(g, items));
 }
-void launch_walk(Ctx& C, bool accurate, uint32_t grid, const Z8Dev* de, const Z8Q* dq, uint32_t nq, uint64_t items, unsigned long long* ctr, double* part) {
-    if (accurate) hipLaunchKernelGGL(k_z8_walk<32>, dim3(grid), dim3(256), 0, C.stream, de, dq, nq, items, ctr, part);
-    else hipLaunchKernelGGL(k_z8_walk<256>, dim3(grid), dim3(256), 0, C.stream, de, dq, nq, items, ctr, part);
+template <int FOLD, bool HASG, bool IMAG>
+uint32_t launch_v(Ctx& C, hipStream_t st, const BatchPlan& B, int v, const Z8Dev* de, const Z8Q* dq, unsigned long long* ctr, double* part) {
+    const uint64_t items = B.it_hi[v] - B.it_lo[v];
+    const uint32_t grid = walk_grid_v<FOLD, HASG, IMAG>(C, items);
+    hipLaunchKernelGGL((k_z8_walk<FOLD, HASG, IMAG>), dim3(grid), dim3(256), 0, st, de, dq, B.q_lo[v], B.q_hi[v], B.it_lo[v], B.it_hi[v], ctr + v, part);
     HIPCHECK(hipGetLastError());
+    return grid;
+}
+// Launch one persistent grid per variant present; returns the summed grid.
+uint32_t launch_walks(Ctx& C, hipStream_t st, bool accurate, const BatchPlan& B, const Z8Dev* de, const Z8Q* dq, unsigned long long* ctr, double* part) {
+    uint32_t g = 0;
+    for (int v = 0; v < kVariants; ++v) {
+        if (B.it_hi[v] == B.it_lo[v]) continue;
+        const bool hasg = v & 2, imag = v & 1;
+        if (accurate) {
+            if (hasg) g += imag ? launch_v<32, true, true>(C, st, B, v, de, dq, ctr, part) : launch_v<32, true, false>(C, st, B, v, de, dq, ctr, part);
+            else g += imag ? launch_v<32, false, true>(C, st, B, v, de, dq, ctr, part) : launch_v<32, false, false>(C, st, B, v, de, dq, ctr, part);
+        } else {
+            if (hasg) g += imag ? launch_v<256, true, true>(C, st, B, v, de, dq, ctr, part) : launch_v<256, true, false>(C, st, B, v, de, dq, ctr, part);
+            else g += imag ? launch_v<256, false, true>(C, st, B, v, de, dq, ctr, part) : launch_v<256, false, false>(C, st, B, v, de, dq, ctr, part);
+        }
+    }
+    return g;
 }
 }  // namespace
 
@@ -760,14 +803,11 @@ void dev_z8_launch(Z8Slot& S, const std::vector<Z8Job>& jobs, bool accurate) {
     double* part = (double*)S.d_part.get(B.total_items * 32);
     double* res = (double*)S.d_res.get(nq * 16);
     double* hres = (double*)S.h_res.get(nq * 16);
-    S.grid = accurate ? walk_grid<32>(C, B.total_items) : walk_grid<256>(C, B.total_items);
     HIPCHECK(hipEventRecord(S.e0, S.stream));
     HIPCHECK(hipMemcpyAsync(d, h, tot, hipMemcpyHostToDevice, S.stream));
-    hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(1), 0, S.stream, ctr);
+    hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(kVariants), 0, S.stream, ctr);
     HIPCHECK(hipEventRecord(S.e1, S.stream));
-    if (accurate) hipLaunchKernelGGL(k_z8_walk<32>, dim3(S.grid), dim3(256), 0, S.stream, (const Z8Dev*)d, (const Z8Q*)(d + oq), nq, B.total_items, ctr, part);
-    else hipLaunchKernelGGL(k_z8_walk<256>, dim3(S.grid), dim3(256), 0, S.stream, (const Z8Dev*)d, (const Z8Q*)(d + oq), nq, B.total_items, ctr, part);
-    HIPCHECK(hipGetLastError());
+    S.grid = launch_walks(C, S.stream, accurate, B, (const Z8Dev*)d, (const Z8Q*)(d + oq), ctr, part);
     HIPCHECK(hipEventRecord(S.e2, S.stream));
     hipLaunchKernelGGL(k_z8_reduce, dim3(nq), dim3(256), 0, S.stream, (const Z8Q*)(d + oq), part, (const double2*)(d + os), (const uint32_t*)(d + oo), res);
     HIPCHECK(hipGetLastError());
@@ -847,12 +887,11 @@ std::shared_ptr<Z8Graph> dev_z8_graph(const std::vector<Z8Job>& jobs, bool accur
     HIPCHECK(hipEventCreate(&G->k0)); HIPCHECK(hipEventCreate(&G->k1));
     uint8_t* h = (uint8_t*)G->h_in;
     std::memcpy(h, B.engs.data(), be); std::memcpy(h + oq, B.qs.data(), bq); std::memcpy(h + os, B.scale.data(), bs); std::memcpy(h + oo, B.order.data(), bo);
-    G->grid = accurate ? walk_grid<32>(C, B.total_items) : walk_grid<256>(C, B.total_items);
     uint8_t* d = (uint8_t*)G->d_in;
     HIPCHECK(hipStreamBeginCapture(C.stream, hipStreamCaptureModeThreadLocal));
     HIPCHECK(hipMemcpyAsync(d, h, tot, hipMemcpyHostToDevice, C.stream));
-    hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(1), 0, C.stream, (unsigned long long*)G->d_ctr);   // a kernel, not a memset node
-    launch_walk(C, accurate, G->grid, (const Z8Dev*)d, (const Z8Q*)(d + oq), nq, B.total_items, (unsigned long long*)G->d_ctr, (double*)G->d_part);
+    hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(kVariants), 0, C.stream, (unsigned long long*)G->d_ctr);   // a kernel, not a memset node
+    G->grid = launch_walks(C, C.stream, accurate, B, (const Z8Dev*)d, (const Z8Q*)(d + oq), (unsigned long long*)G->d_ctr, (double*)G->d_part);
     hipLaunchKernelGGL(k_z8_reduce, dim3(nq), dim3(256), 0, C.stream, (const Z8Q*)(d + oq), (const double*)G->d_part, (const double2*)(d + os), (const uint32_t*)(d + oo), (double*)G->d_res);
     HIPCHECK(hipMemcpyAsync(G->h_out, G->d_res, nq * 16, hipMemcpyDeviceToHost, C.stream));
     HIPCHECK(hipStreamEndCapture(C.stream, &G->graph));
@@ -942,15 +981,15 @@ __global__ __launch_bounds__(256) void k_dump_walk(Z8Dev E, Z8Q Q, uint8_t* __re
         const float F = lane_factor(s_lt, E.nbytes, S.sout);
         for (uint32_t i = 0; i < 256; ++i) {
             const uint32_t g = i ^ (i >> 1);
-            const float wre = E.inner[2 * i], wim = E.inner[2 * i + 1];
-            const uint32_t pin4 = __float_as_uint(E.inner[512 + i]), kin = __float_as_uint(E.inner[768 + i]);
+            const float wv = E.inner[i];
+            const uint32_t pin4 = __float_as_uint(E.inner[256 + i]), kin = __float_as_uint(E.inner[512 + i]);
             const uint64_t idx = (((lane << Q.M) + o) << 8) + i;
-            if (wre == 0.f && wim == 0.f) { kout[idx] = 255; mout[idx] = 0.f; continue; }
+            if (wv == 0.f) { kout[idx] = 255; mout[idx] = 0.f; continue; }
             const uint32_t par = (uint32_t)__popc(g & S.hs) & 1u;
             const uint32_t k = (S.K + kin + 4u * par) & 7u;
             const float gi = E.npin ? s_gin[(S.pio ^ (pin4 >> 2))] : 1.f;
             kout[idx] = (uint8_t)k;
-            mout[idx] = F * gi * sqrtf(wre * wre + wim * wim);
+            mout[idx] = F * gi * fabsf(wv);
         }
     }
 }

@@ -18,90 +18,116 @@ namespace pq {
 
 Z8Image build_z8_image(const HostEngine& E) {
     Z8Image Z;
-    const uint32_t d = E.d, t = E.t;
+    const uint32_t d = E.d, t = E.t, W = E.L.words();
     if (d > 56) throw std::invalid_argument("branch dimension above 56 is not supported by the Z8 walk");
     Z.d = d; Z.t = t;
-    // Pivot index of each pivot term.
     std::vector<int32_t> piv_index(t, -1);
     for (uint32_t i = 0; i < E.rk; ++i) piv_index[E.pivot_col[i]] = (int32_t)i;
-    auto pivots_of = [&](uint32_t j) { uint64_t pm = 0; for (uint32_t k = 0; k < t; ++k) if (((E.kernel[j] >> k) & 1) && piv_index[k] >= 0) pm |= 1ull << piv_index[k]; return pm; };
+    auto pivots_of = [&](uint64_t N) { uint64_t pm = 0; for (uint32_t k = 0; k < t; ++k) if (((N >> k) & 1) && piv_index[k] >= 0) pm |= 1ull << piv_index[k]; return pm; };
 
-    // Choose up to 8 inner coordinates: pivot-free directions first, then the
-    // directions that add the fewest new pivots while at most 12 pivots are
+    // Choose up to 8 inner directions: pivot-free kernel vectors first, then
+    // the ones adding the fewest new pivots while at most 12 pivots are
     // touched (the gin table stays within 16 KB of LDS).
     std::vector<uint8_t> used(d, 0);
     std::vector<uint32_t> inner;
     uint64_t pin_set = 0;
-    for (uint32_t j = 0; j < d && inner.size() < kInner; ++j) if (pivots_of(j) == 0) { inner.push_back(j); used[j] = 1; }
+    for (uint32_t j = 0; j < d && inner.size() < kInner; ++j) if (pivots_of(E.kernel[j]) == 0) { inner.push_back(j); used[j] = 1; }
     while (inner.size() < kInner) {
         int best = -1; uint32_t bestc = 99;
         for (uint32_t j = 0; j < d; ++j) {
             if (used[j]) continue;
-            uint32_t c = (uint32_t)__builtin_popcountll(pivots_of(j) & ~pin_set);
-            if ((uint32_t)__builtin_popcountll(pin_set | pivots_of(j)) <= 12 && c < bestc) { bestc = c; best = (int)j; }
+            const uint64_t pj = pivots_of(E.kernel[j]);
+            const uint32_t c = (uint32_t)__builtin_popcountll(pj & ~pin_set);
+            if ((uint32_t)__builtin_popcountll(pin_set | pj) <= 12 && c < bestc) { bestc = c; best = (int)j; }
         }
         if (best < 0) break;
-        inner.push_back((uint32_t)best); used[best] = 1; pin_set |= pivots_of((uint32_t)best);
+        inner.push_back((uint32_t)best); used[best] = 1; pin_set |= pivots_of(E.kernel[best]);
     }
     Z.nin = (uint32_t)inner.size();
-    Z.slot_coord.assign(kInner, -1);
-    for (uint32_t p = 0; p < Z.nin; ++p) Z.slot_coord[p] = (int32_t)inner[p];
-    for (uint32_t j = 0; j < d; ++j) if (!used[j]) Z.slot_coord.push_back((int32_t)j);
-    Z.nslots = (uint32_t)Z.slot_coord.size();
-    if (Z.nslots > 64) throw std::invalid_argument("too many slots");
 
-    // compact pivot numbering for the inner-touched pivots
+    // Slot directions.  Inner ones may be recombined (any basis of the inner
+    // subspace gives the same sum): every linear coefficient of the Z8 form is
+    // even, so an inner entry is real or imaginary according to sum_p c_p
+    // (lam_p / 2) mod 2; folding every "imaginary" direction but one into the
+    // others leaves that one in slot 0, and the class of entry i becomes
+    // bit 0 of gray(i), known at compile time in the kernel.
+    std::vector<uint64_t> N;
+    std::vector<std::vector<uint64_t>> U;
+    for (uint32_t j : inner) { N.push_back(E.kernel[j]); U.emplace_back(&E.kernel_u[(size_t)j * W], &E.kernel_u[(size_t)j * W] + W); }
+    int p1 = -1;
+    for (uint32_t p = 0; p < Z.nin; ++p) if ((z8_lambda(E, U[p].data(), N[p]) >> 1) & 1) { p1 = (int)p; break; }
+    if (p1 >= 0) {
+        for (uint32_t p = 0; p < Z.nin; ++p) {
+            if ((int)p == p1 || !((z8_lambda(E, U[p].data(), N[p]) >> 1) & 1)) continue;
+            N[p] ^= N[p1];
+            for (uint32_t w = 0; w < W; ++w) U[p][w] ^= U[p1][w];
+        }
+        std::swap(N[0], N[p1]); std::swap(U[0], U[p1]);
+        Z.imag0 = true;
+    }
+    while (N.size() < kInner) { N.push_back(0); U.emplace_back(W, 0ull); }
+    for (uint32_t j = 0; j < d; ++j) if (!used[j]) { N.push_back(E.kernel[j]); U.emplace_back(&E.kernel_u[(size_t)j * W], &E.kernel_u[(size_t)j * W] + W); }
+    Z.nslots = (uint32_t)N.size();
+    if (Z.nslots > 64) throw std::invalid_argument("too many slots");
+    const uint32_t S = Z.nslots;
+    Z.slot_real.assign(S, 0);
+    for (uint32_t p = 0; p < S; ++p) Z.slot_real[p] = (p < Z.nin || p >= kInner) ? 1 : 0;
+
+    // compact numbering of the pivots touched by inner directions
+    uint64_t touched = 0;
+    for (uint32_t p = 0; p < Z.nin; ++p) touched |= pivots_of(N[p]);
     std::vector<int32_t> cpos(E.rk, -1);
-    for (uint32_t i = 0; i < E.rk; ++i) if ((pin_set >> i) & 1) { cpos[i] = (int32_t)Z.pin_term.size(); Z.pin_term.push_back(E.pivot_col[i]); }
+    for (uint32_t i = 0; i < E.rk; ++i) if ((touched >> i) & 1) { cpos[i] = (int32_t)Z.pin_term.size(); Z.pin_term.push_back(E.pivot_col[i]); }
     Z.npin = (uint32_t)Z.pin_term.size();
 
-    const uint32_t S = Z.nslots;
     Z.slot_q.assign(S, 0); Z.slot_n.assign(S, 0); Z.slot_pin.assign(S, 0); Z.slot_lam.assign(S, 0);
+    Z.slot_u.assign((size_t)S * W, 0); Z.slot_dual.assign((size_t)S * W, 0);
     for (uint32_t p = 0; p < S; ++p) {
-        int32_t j = Z.slot_coord[p];
-        if (j < 0) continue;
-        Z.slot_n[p] = E.kernel[j];
-        Z.slot_lam[p] = E.lam[j];
-        uint64_t pm = pivots_of((uint32_t)j);
+        if (!Z.slot_real[p]) continue;
+        Z.slot_n[p] = N[p];
+        std::copy(U[p].begin(), U[p].end(), &Z.slot_u[(size_t)p * W]);
+        Z.slot_lam[p] = (uint8_t)z8_lambda(E, U[p].data(), N[p]);
+        z8_dual(E, U[p].data(), &Z.slot_dual[(size_t)p * W]);
+        const uint64_t pm = pivots_of(N[p]);
         for (uint32_t i = 0; i < E.rk; ++i) if (((pm >> i) & 1) && cpos[i] >= 0) Z.slot_pin[p] |= 1u << cpos[i];
-        for (uint32_t p2 = 0; p2 < S; ++p2) {
-            int32_t j2 = Z.slot_coord[p2];
-            if (j2 >= 0 && ((E.Q[j] >> j2) & 1)) Z.slot_q[p] |= 1ull << p2;
-        }
+        for (uint32_t q = 0; q < S; ++q)
+            if (q != p && Z.slot_real[q] && z8_cross(E, U[p].data(), N[p], U[q].data(), N[q])) Z.slot_q[p] |= 1ull << q;
     }
 
-    // Terms carried by the lane factor: all but the inner free terms and the
-    // inner-touched pivots.
-    uint64_t all = t == 64 ? ~0ull : ((1ull << t) - 1);
-    uint64_t excl = 0;
-    for (uint32_t p = 0; p < Z.nin; ++p) excl |= 1ull << E.free_col[Z.slot_coord[p]];
+    // Terms carried by the lane factor: all but the free terms of the inner
+    // directions and the inner-touched pivots.
+    const uint64_t all = t == 64 ? ~0ull : ((1ull << t) - 1);
+    uint64_t inner_free = 0;
+    for (uint32_t j : inner) inner_free |= 1ull << E.free_col[j];
+    uint64_t excl = inner_free;
     for (uint32_t term : Z.pin_term) excl |= 1ull << term;
     Z.lane_terms = all & ~excl;
 
-    // Inner table.  Entry i covers c_in = gray(i); phantom bits give zero.
-    static const double S2 = 0.70710678118654752440;
-    static const double C8[8] = {1, S2, 0, -S2, -1, -S2, 0, S2}, S8[8] = {0, S2, 1, S2, 0, -S2, -1, -S2};
-    Z.inner.assign(256 * 4, 0.f);
+    // Inner table: entry i covers c_in = gray(i); phantom bits give zero.
+    Z.inner.assign(256 * 3, 0.f);
     for (uint32_t i = 0; i < 256; ++i) {
-        uint32_t c = i ^ (i >> 1);
+        const uint32_t c = i ^ (i >> 1);
         uint32_t pin = 0, kin = 0;
-        bool phantom = (c >> Z.nin) != 0;
-        double fin = 1.0;
+        uint64_t sin = 0;
+        const bool phantom = (c >> Z.nin) != 0;
         for (uint32_t p = 0; p < Z.nin; ++p) {
-            uint32_t term = E.free_col[Z.slot_coord[p]];
-            bool b = (c >> p) & 1;
-            fin *= b ? E.r1[term] : E.r0[term];
-            if (b) {
-                kin += Z.slot_lam[p];
-                kin += 4 * ((uint32_t)__builtin_popcountll(Z.slot_q[p] & (uint64_t)(c & ((1u << p) - 1))) & 1);
-                pin ^= Z.slot_pin[p];
-            }
+            if (!((c >> p) & 1)) continue;
+            kin += Z.slot_lam[p] + 4 * ((uint32_t)__builtin_popcountll(Z.slot_q[p] & (uint64_t)(c & ((1u << p) - 1))) & 1);
+            pin ^= Z.slot_pin[p];
+            sin ^= N[p];
         }
         kin &= 7;
-        if (!phantom) { Z.inner[2 * i] = (float)(fin * C8[kin]); Z.inner[2 * i + 1] = (float)(fin * S8[kin]); }
-        uint32_t pin4 = pin << 2;
-        std::memcpy(&Z.inner[512 + i], &pin4, 4);
-        std::memcpy(&Z.inner[768 + i], &kin, 4);
+        if (!phantom) {
+            if (kin & 1) throw std::logic_error("odd inner phase: the Z8 form has odd linear coefficients");
+            const uint32_t cls = (kin >> 1) & 1;
+            if (cls != (Z.imag0 ? (c & 1u) : 0u)) throw std::logic_error("inner real/imaginary classes do not follow slot 0");
+            double fin = 1.0;
+            for (uint32_t k = 0; k < t; ++k) if ((inner_free >> k) & 1) fin *= ((sin >> k) & 1) ? E.r1[k] : E.r0[k];
+            Z.inner[i] = (float)((kin & 4) ? -fin : fin);
+        }
+        const uint32_t pin4 = pin << 2;
+        std::memcpy(&Z.inner[256 + i], &pin4, 4);
+        std::memcpy(&Z.inner[512 + i], &kin, 4);
     }
     // Lane-factor byte tables
     Z.nbytes = (t + 7) / 8;
@@ -110,7 +136,7 @@ Z8Image build_z8_image(const HostEngine& E) {
         for (uint32_t v = 0; v < 256; ++v) {
             double f = 1.0;
             for (uint32_t q = 0; q < 8; ++q) {
-                uint32_t k = 8 * b + q;
+                const uint32_t k = 8 * b + q;
                 if (k >= t || !((Z.lane_terms >> k) & 1)) continue;
                 f *= ((v >> q) & 1) ? E.r1[k] : E.r0[k];
             }
@@ -127,13 +153,18 @@ Z8Image build_z8_image(const HostEngine& E) {
 }
 
 Z8QueryHost z8_query(const HostEngine& E, const Z8Image& Z, const Prepared& P) {
+    const uint32_t W = E.L.words();
     Z8QueryHost q;
     q.K0 = P.K0;
     q.s0 = P.s0;
     q.delta = 0;
     for (uint32_t p = 0; p < Z.nslots; ++p) {
-        int32_t j = Z.slot_coord[p];
-        if (j >= 0 && ((P.ell[j] ^ E.lam[j]) & 4)) q.delta |= 1ull << p;
+        if (!Z.slot_real[p]) continue;
+        uint64_t a = 0;
+        const uint64_t* dr = &Z.slot_dual[(size_t)p * W];
+        for (uint32_t w = 0; w < W; ++w) a ^= P.u_start[w] & dr[w];
+        const bool bit = ((__builtin_popcountll(a) + __builtin_popcountll(P.s0 & Z.slot_n[p])) & 1) != 0;
+        if (bit) q.delta |= 1ull << p;
     }
     q.pio0 = 0;
     for (uint32_t b = 0; b < Z.npin; ++b) if ((P.s0 >> Z.pin_term[b]) & 1) q.pio0 |= 1u << b;

@@ -223,9 +223,9 @@ class Simulator:
         return tuple(int(v) for v in o)
 
     def z8_info(self):
-        o = np.zeros(5, dtype=np.uint32)
+        o = np.zeros(7, dtype=np.uint32)
         _check(lib.pq_sim_z8_info(self._h, _ptr(o, _u32p)))
-        return dict(zip(("d", "inner", "slots", "inner_pivots", "rank_k"), (int(v) for v in o)))
+        return dict(zip(("d", "inner", "slots", "inner_pivots", "rank_k", "imag_slot0", "ops_per_branch"), (int(v) for v in o)))
 
     def selfcheck(self, x, y, trials=256, seed=1):
         bad = ctypes.c_uint64()
