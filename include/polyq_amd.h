@@ -36,7 +36,11 @@ enum pq_gate {
  * decomposed T-pairs with the cost model of src/rank.rs:205-222, Cached is the
  * plain Z-branch engine (2^d branches per amplitude), Pairs decomposes exactly
  * `pairs` pairs (2^p stabiliser branch engines, src/rank.rs:88-103). */
-enum pq_strategy { PQ_AUTO = 0, PQ_CACHED = 1, PQ_PAIRS = 2 };
+enum pq_strategy { PQ_AUTO = 0, PQ_CACHED = 1, PQ_PAIRS = 2,
+    /* Auto with the cost model refit to the GPU (no counterpart in the
+     * reference): 2^p host engine builds against 2^(d - removed) branches at
+     * the device's walk rate; picks p = 0 unless d is far beyond 40. */
+    PQ_GPU_AUTO = 3 };
 
 /* Device evaluation form.  Z8: integer Z8 phases, FP32 two-sum compensated
  * lane accumulation, FP64 cross-workgroup reduction (default).  Z8_ACCURATE:
@@ -82,6 +86,13 @@ int pq_sim_amplitudes(pq_sim* sim, size_t n, const uint64_t* inputs, const uint6
  * branch-engine sums of all queries run in one persistent launch (Z8 forms
  * only).  out has 2n doubles. */
 int pq_batch_amplitudes(size_t n, pq_sim* const* sims, const uint64_t* inputs, const uint64_t* outputs, int form, double* out);
+
+/* Segment split of the Z8 walk (no counterpart in the reference): a query of
+ * branch dimension d is walked by lanes of 2^(8+M) branches with
+ * M = clamp(d - offset, 0, max_mid).  Set by the autotuner; a fixed setting
+ * makes every result independent of the batch it runs in. */
+int pq_set_walk_tuning(uint32_t offset, uint32_t max_mid);
+int pq_get_walk_tuning(uint32_t out2[2]);
 
 /* Timing of the last device evaluation on this thread: kernel_ms, total_ms,
  * branches, items, grid (doubles). */
@@ -134,6 +145,9 @@ int pq_sim_walk_dump(pq_sim* sim, uint64_t x, uint64_t y, uint32_t mid_bits, siz
 /* Host-only check of the Z8 quadratic form against the direct phase for
  * `trials` random coset points; returns the number of mismatches in *bad. */
 int pq_sim_selfcheck(pq_sim* sim, uint64_t x, uint64_t y, uint32_t trials, uint64_t seed, uint64_t* bad);
+/* Host-only: stats of the marginal (sandwich) engine for a measured-qubit set,
+ * in pq_sim_stats order; builds and caches it (src/sim.rs:225-231). */
+int pq_sim_marginal_stats(pq_sim* sim, const uint32_t* qubits, size_t k, uint64_t out7[7]);
 /* Z8 image summary: d, inner directions, slots, compact inner pivots,
  * rank K, whether inner entries alternate real/imaginary, VALU lane-ops per
  * branch of the inner step. */

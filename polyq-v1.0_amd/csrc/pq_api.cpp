@@ -25,10 +25,11 @@ std::shared_ptr<RankEng> build_rank(const PathSum& ps, const std::vector<uint32_
     uint32_t nt = 0;
     auto cands = pair_candidates(ps, fixed, nt);
     uint32_t p = 0;
-    if (strategy == PQ_AUTO) {
+    if (strategy == PQ_AUTO || strategy == PQ_GPU_AUTO) {
         // rank.rs:115-123: base engine, cost model, rebuild with p pairs if p > 0
         auto base = build_engine(ps, fixed);
-        p = best_pairs(base->stats(), (uint32_t)cands.size(), nt);
+        p = strategy == PQ_AUTO ? best_pairs(base->stats(), (uint32_t)cands.size(), nt)
+                                : best_pairs_gpu(base->stats(), (uint32_t)cands.size(), nt, kGpuBranchSeconds);
         if (p == 0) { add(base); R->pairs = 0; }
     } else if (strategy == PQ_CACHED) {
         p = 0;
@@ -37,7 +38,7 @@ std::shared_ptr<RankEng> build_rank(const PathSum& ps, const std::vector<uint32_
         p = std::min<uint32_t>(std::min<uint32_t>(pairs, (uint32_t)cands.size()), kMaxPairs);
         if (p == 0) add(build_engine(ps, fixed));
     } else {
-        throw std::invalid_argument("strategy must be PQ_AUTO, PQ_CACHED or PQ_PAIRS");
+        throw std::invalid_argument("strategy must be PQ_AUTO, PQ_CACHED, PQ_PAIRS or PQ_GPU_AUTO");
     }
     if (R->host.empty()) {
         // rank.rs:88-103: one engine per substitution pattern of the chosen pairs
@@ -275,6 +276,9 @@ int pq_batch_amplitudes(size_t n, pq_sim* const* sims, const uint64_t* xs, const
     })
 }
 
+int pq_set_walk_tuning(uint32_t offset, uint32_t max_mid) { PQ_TRY({ z8_set_tuning(offset, max_mid); return 0; }) }
+int pq_get_walk_tuning(uint32_t* o) { z8_get_tuning(&o[0], &o[1]); return 0; }
+
 int pq_last_timing(double* o) { for (int i = 0; i < 5; ++i) o[i] = g_timing[i]; return 0; }
 
 int pq_graph_create(size_t n, pq_sim* const* sims, const uint64_t* xs, const uint64_t* ys, int form, pq_graph** out) {
@@ -386,6 +390,17 @@ int pq_sim_selfcheck(pq_sim* h, uint64_t x, uint64_t y, uint32_t trials, uint64_
                 if (!survives_row(E, u.data()) || total_phase(E, u.data(), sv) != (kq & 7)) (*bad)++;
             }
         }
+        return 0;
+    })
+}
+
+int pq_sim_marginal_stats(pq_sim* h, const uint32_t* qubits, size_t k, uint64_t* o) {
+    PQ_TRY({
+        for (size_t i = 0; i < k; ++i) if (qubits[i] >= h->s.ps.nq) throw std::invalid_argument("measured qubit out of range");
+        std::vector<uint32_t> qs(qubits, qubits + k);
+        const RankEng& R = h->s.marginal_engine(qs);
+        EngineStats s = R.host[0]->stats();
+        o[0] = s.free_vars; o[1] = s.planes; o[2] = s.lines; o[3] = s.radical; o[4] = s.terms; o[5] = s.branch_dim; o[6] = R.pairs;
         return 0;
     })
 }

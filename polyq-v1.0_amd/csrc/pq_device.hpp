@@ -49,6 +49,8 @@ void dev_z8_wait(Z8Slot& S, std::vector<std::complex<double>>& out, Z8Timing* ti
 // Mid-bit heuristic: enough items to fill every CU several times, long
 // enough walks to amortise the lane seed.
 uint32_t z8_auto_mid_bits(uint32_t d, uint64_t batch_branches);
+void z8_set_tuning(uint32_t offset, uint32_t max_mid);
+void z8_get_tuning(uint32_t* offset, uint32_t* max_mid);
 
 // Replayable batch: the whole batch (descriptor upload, counter clear, walk,
 // reduction, result download) captured once into a hipGraph and relaunched.

@@ -410,6 +410,23 @@ std::vector<std::pair<uint32_t, uint32_t>> pair_candidates(const PathSum& ps, co
     return tp;
 }
 
+// The same decision with GPU costs (the cost model of rank.rs:205-222 refit):
+// each of the 2^p engines costs a host build (normal form, O(m^3/64) word ops
+// at ~1 ns each plus a fixed upload), and every branch costs the measured
+// walk rate of the Z8 kernel instead of the CPU's per-branch word operations.
+uint32_t best_pairs_gpu(const EngineStats& st, uint32_t avail, uint32_t tclass, double branch_s) {
+    const double m = (double)st.free_vars;
+    const double build_s = 2e-4 + 1e-9 * (2.0 * m * m * m / 64.0 + 50.0 * m);
+    uint32_t best = 0; double bc = INFINITY;
+    for (uint32_t p = 0; p <= std::min(avail, kMaxPairs); ++p) {
+        const uint32_t removed = 2 * std::min(p, tclass) + (p > tclass ? p - tclass : 0);
+        const double dd = st.branch_dim > removed ? (double)(st.branch_dim - removed) : 0.0;
+        const double cost = std::pow(2.0, (double)p) * ((p ? build_s : 0.0) + std::pow(2.0, dd) * branch_s);
+        if (cost < bc) { bc = cost; best = p; }
+    }
+    return best;
+}
+
 uint32_t best_pairs(const EngineStats& st, uint32_t avail, uint32_t tclass) {   // rank.rs:205-222
     double m = (double)st.free_vars, words = (double)(st.free_vars / 64 + 1);
     double ldl = 2.0 * m * m * m / 64.0 + 50.0 * m + 2000.0;

@@ -223,5 +223,8 @@ Z8QueryHost z8_query(const HostEngine& E, const Z8Image& Z, const Prepared& P);
 constexpr uint32_t kMaxPairs = 14;
 std::vector<std::pair<uint32_t, uint32_t>> pair_candidates(const PathSum& ps, const std::vector<uint32_t>& fixed, uint32_t& n_tclass);
 uint32_t best_pairs(const EngineStats& st, uint32_t avail, uint32_t tclass);
+// GPU refit of the cost model: host build per branch engine vs. device walk.
+uint32_t best_pairs_gpu(const EngineStats& st, uint32_t avail, uint32_t tclass, double branch_s);
+constexpr double kGpuBranchSeconds = 1.0 / 1.5e13;   // measured Z8 walk rate on MI355X (config 2)
 
 }  // namespace pq

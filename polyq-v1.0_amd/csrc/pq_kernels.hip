@@ -24,6 +24,7 @@
 #include <hip/hip_runtime.h>
 #include "pq_device.hpp"
 #include <mutex>
+#include <atomic>
 #include <cstring>
 #include <algorithm>
 #include <numeric>
@@ -659,14 +660,22 @@ __global__ __launch_bounds__(256) void k_z8_reduce(const Z8Q* __restrict__ qs, c
 
 __global__ void k_zero_u64(unsigned long long* p) { p[threadIdx.x] = 0ull; }   // one thread per counter
 
+// Segment split: a lane walks 2^(8+M) branches and 2^H lanes cover a query
+// (H = d-8-M).  M depends on d alone (M = clamp(d - offset, 0, max)), so a
+// query is decomposed the same way in every batch and its FP32 partial sums,
+// hence its result, are bitwise the same alone and in any mix.  The offset
+// and cap are set by the autotuner (polyq_amd.autotune); the default keeps
+// H = 19 (2048 items of 256 lanes per query) until M reaches 6.
+static std::atomic<uint32_t> g_mid_offset{27}, g_mid_max{6};
+void z8_set_tuning(uint32_t offset, uint32_t max_mid) {
+    if (offset < 8 || max_mid > 16) throw std::invalid_argument("walk tuning out of range");
+    g_mid_offset = offset; g_mid_max = max_mid;
+}
+void z8_get_tuning(uint32_t* offset, uint32_t* max_mid) { *offset = g_mid_offset; *max_mid = g_mid_max; }
 uint32_t z8_auto_mid_bits(uint32_t d, uint64_t /*batch_branches*/) {
-    // A lane walks 2^(8+M) branches and 2^H lanes cover the query, H = d-8-M.
-    // M depends on d alone, so a query is decomposed the same way in every
-    // batch and its FP32 partial sums (hence its result) are bitwise the same
-    // alone and in any mix.  M = d - 27 keeps H = 19 (2048 items of 256 lanes
-    // per query, enough to fill 256 CUs from one query) until M reaches 6.
-    if (d <= 27) return 0;
-    return std::min<uint32_t>(d - 27, 6);
+    const uint32_t off = g_mid_offset, mx = g_mid_max;
+    if (d <= off) return 0;
+    return std::min<uint32_t>(std::min<uint32_t>(d - off, mx), d > 8 ? d - 8 : 0);
 }
 
 namespace {

@@ -42,6 +42,8 @@ lib.pq_sim_marginal_probability.argtypes = [_sim_p, ctypes.c_uint64, _u32p, _u8p
 lib.pq_sim_amplitudes.argtypes = [_sim_p, ctypes.c_size_t, _u64p, _u64p, ctypes.c_int, _f64p]
 lib.pq_batch_amplitudes.argtypes = [ctypes.c_size_t, ctypes.POINTER(_sim_p), _u64p, _u64p, ctypes.c_int, _f64p]
 lib.pq_last_timing.argtypes = [_f64p]
+lib.pq_set_walk_tuning.argtypes = [ctypes.c_uint32, ctypes.c_uint32]
+lib.pq_get_walk_tuning.argtypes = [_u32p]
 lib.pq_graph_create.argtypes = [ctypes.c_size_t, ctypes.POINTER(_sim_p), _u64p, _u64p, ctypes.c_int, ctypes.POINTER(ctypes.c_void_p)]
 lib.pq_graph_run.argtypes = [ctypes.c_void_p, _f64p, _f64p]
 lib.pq_graph_destroy.argtypes = [ctypes.c_void_p]
@@ -50,10 +52,11 @@ lib.pq_sim_branch_dump.argtypes = [_sim_p, ctypes.c_uint64, ctypes.c_uint64, cty
 lib.pq_sim_walk_dump.argtypes = [_sim_p, ctypes.c_uint64, ctypes.c_uint64, ctypes.c_uint32, ctypes.c_size_t, _u8p, _f32p, _u64p, ctypes.POINTER(ctypes.c_size_t)]
 lib.pq_sim_selfcheck.argtypes = [_sim_p, ctypes.c_uint64, ctypes.c_uint64, ctypes.c_uint32, ctypes.c_uint64, _u64p]
 lib.pq_sim_z8_info.argtypes = [_sim_p, _u32p]
+lib.pq_sim_marginal_stats.argtypes = [_sim_p, _u32p, ctypes.c_size_t, _u64p]
 lib.pq_device_info.argtypes = [ctypes.c_char_p, ctypes.c_size_t]
 
 FORMS = {"z8": 0, "z8_accurate": 1, "plain_fp64": 2}
-STRATEGIES = {"auto": 0, "cached": 1}
+STRATEGIES = {"auto": 0, "cached": 1, "gpu_auto": 3}
 
 # Gate codes (src/circuit.rs:25-42)
 H, X, Y, Z, S, SDG, T, TDG, P, CZ, CX, SWAP, GPHASE = range(13)
@@ -167,7 +170,7 @@ class Simulator:
         elif strategy in STRATEGIES:
             st, pr = STRATEGIES[strategy], 0
         else:
-            raise ValueError("strategy must be 'auto' or 'cached'")
+            raise ValueError("strategy must be 'auto', 'cached' or 'gpu_auto'")
         if form not in FORMS:
             raise ValueError("form must be one of %s" % sorted(FORMS))
         self.form = form
@@ -220,6 +223,12 @@ class Simulator:
         """(free_vars, planes, lines, radical, terms, branch_dim, pairs) (pybind.rs:177-180)."""
         o = np.zeros(7, dtype=np.uint64)
         _check(lib.pq_sim_stats(self._h, _ptr(o, _u64p)))
+        return tuple(int(v) for v in o)
+
+    def marginal_stats(self, qubits):
+        qs = np.asarray(list(qubits), dtype=np.uint32)
+        o = np.zeros(7, dtype=np.uint64)
+        _check(lib.pq_sim_marginal_stats(self._h, _ptr(qs, _u32p), len(qs), _ptr(o, _u64p)))
         return tuple(int(v) for v in o)
 
     def z8_info(self):
@@ -289,6 +298,66 @@ class Graph:
         if g:
             lib.pq_graph_destroy(g)
             self._g = None
+
+
+def set_walk_tuning(offset, max_mid):
+    _check(lib.pq_set_walk_tuning(int(offset), int(max_mid)))
+
+
+def get_walk_tuning():
+    o = np.zeros(2, dtype=np.uint32)
+    lib.pq_get_walk_tuning(_ptr(o, _u32p))
+    return int(o[0]), int(o[1])
+
+
+def autotune(sims, inputs, outputs, offsets=(23, 24, 25, 26, 27, 28, 29), max_mids=(4, 6, 8), repeats=3, cache=True):
+    """Pick the segment split (offset, max_mid) that minimises the walk-kernel
+    time of this batch; apply it and, if cache, store it per device in
+    ~/.cache/polyq_amd/walk_tuning.json.  Returns (offset, max_mid, table)."""
+    import json
+    table = {}
+    best = None
+    for off in offsets:
+        for mx in max_mids:
+            set_walk_tuning(off, mx)
+            ms = []
+            for _ in range(repeats):
+                batch_amplitudes(sims, inputs, outputs)
+                ms.append(last_timing()["kernel_ms"])
+            t = sorted(ms)[len(ms) // 2]
+            table["%d/%d" % (off, mx)] = t
+            if best is None or t < best[0]:
+                best = (t, off, mx)
+    set_walk_tuning(best[1], best[2])
+    if cache:
+        try:
+            path = os.path.join(os.path.expanduser("~"), ".cache", "polyq_amd", "walk_tuning.json")
+            os.makedirs(os.path.dirname(path), exist_ok=True)
+            data = {}
+            if os.path.exists(path):
+                with open(path) as f:
+                    data = json.load(f)
+            data[device_info()] = {"offset": best[1], "max_mid": best[2], "table_ms": table}
+            with open(path, "w") as f:
+                json.dump(data, f)
+        except OSError:
+            pass
+    return best[1], best[2], table
+
+
+def load_tuning():
+    """Apply a cached autotuner result for this device, if any."""
+    import json
+    path = os.path.join(os.path.expanduser("~"), ".cache", "polyq_amd", "walk_tuning.json")
+    try:
+        with open(path) as f:
+            rec = json.load(f).get(device_info())
+    except (OSError, ValueError):
+        return None
+    if rec:
+        set_walk_tuning(rec["offset"], rec["max_mid"])
+        return rec["offset"], rec["max_mid"]
+    return None
 
 
 def last_timing():
