@@ -1,22 +1,37 @@
-"""Benchmark of the MI355X branch evaluator on the plain-GPU-at-scale workload.
+"""Benchmark of the MI355X branch evaluator.
 
-Workload (one step): the 16 amplitudes <y|C|0> of the n=40 circuit
-"H on all qubits; 36 T + 300 CX/CZ + 100 S interleaved (SplitMix64 seed 5);
-H on all qubits" (t = 30 terms after T merging, branch dimension d = 30), all
-16 evaluated in one persistent Z8-walk launch.  Inputs (the engine images) are
-resident in HBM before timing starts.  Reported beside it:
+Headline (the JSON line's `value`): the plain-GPU-at-scale workload, one step
+= the 16 amplitudes <0|C|y> of the n=40 circuit "H on all qubits; 36 T + 300
+CX/CZ + 100 S interleaved (SplitMix64 seed 5); H on all qubits" (t = 30 terms
+after T merging, branch dimension d = 30: 16 x 2^30 branches), all 16
+evaluated in one persistent Z8-walk launch.  Engine images are resident in HBM
+before timing starts.  Reported beside it:
   * the plain FP64 form (one query per launch, FP64 port of the reference's
-    Gray walk) on the same 16 amplitudes,
+    Gray walk) on the same 16 amplitudes, and per-amplitude error against it,
   * the CPU oracle (a C++ restatement of the reference's Rayon engine, OpenMP
     over 2^14-branch Gray chunks) on the host cores, on a bounded sample,
   * the roofline fraction of the walk kernel against the VALU issue rate.
+Further sections (each bounded; --sections picks them):
+  precision  n=48, 40 Rz(theta) (seed 4, d = 28): 1024 amplitudes in the Z8
+             and Z8-accurate forms against the FP64 GPU form, the FP64 form
+             against the CPU oracle on 16 of them; per-amplitude errors, with
+             |a| < 2^-20 reported separately (--precision-out writes them all)
+  batch      256 circuits (n ~ U[20,56], T ~ U[8,32]) x 16 (x, y) pairs: host
+             builds, segment-split autotuner, one persistent launch, the same
+             batch captured in a hipGraph and replayed
+  serve      Poisson arrivals (200 req/s, 60 s): 50% new circuits of the
+             batch generator, 50% repeats; 70% amplitude, 20% probability,
+             10% 4-qubit marginal; 64 GB HBM budget; p50/p99 latency,
+             throughput, and results re-evaluated alone for comparison
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu] [--no-plain]
+                       [--sections headline,precision,batch,serve] [--serve-seconds S]
 Prints one JSON line.
 """
 import argparse
 import json
 import os
+import random
 import sys
 import time
 
@@ -25,6 +40,273 @@ sys.path.insert(0, os.path.join(ROOT, "polyq-v1.0_amd"))
 
 METRIC = "Gray-code branches/sec at T=36; amplitudes/sec + p50/p99 latency under load"
 SEED = 5
+SMALL = 2.0 ** -20
+
+
+def _oracle():
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import oracle_lib as O
+    return O
+
+
+def _err_summary(got, ref):
+    import numpy as np
+    got = np.asarray(got)
+    ref = np.asarray(ref)
+    ab = np.abs(got - ref)
+    mag = np.abs(ref)
+    small = mag < SMALL
+    big = ~small
+    rel = np.where(mag > 0, ab / np.where(mag > 0, mag, 1), ab)
+    out = {"n": int(len(ref)), "max_abs": float(ab.max()) if len(ab) else 0.0}
+    if big.any():
+        out["large"] = {"n": int(big.sum()), "max_rel": float(rel[big].max()), "median_rel": float(np.median(rel[big]))}
+    if small.any():
+        out["below_2^-20"] = {"n": int(small.sum()), "max_abs": float(ab[small].max()),
+                              "max_rel": float(rel[small].max()), "median_rel": float(np.median(rel[small])),
+                              "median_abs_amp": float(np.median(mag[small]))}
+    return out
+
+
+def headline(args, pq, W, np):
+    dev = pq.device_info()
+    c, xs, ys = W.config2(SEED)
+    sim = pq.Simulator(c, strategy="cached", form="z8")
+    st = sim.stats()
+    info = sim.z8_info()
+    d = st[5]
+    nq = len(ys)
+    branches_per_step = nq * (1 << d)
+
+    for _ in range(args.warmup):
+        sim.amplitudes(xs, ys)
+    kern_ms = []
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        got = sim.amplitudes(xs, ys)     # synchronous: returns after the device results are on the host
+        kern_ms.append(pq.last_timing()["kernel_ms"])
+    t1 = time.perf_counter()
+    elapsed = t1 - t0
+    kernel_avg_ms = float(np.mean(kern_ms))
+    timing = pq.last_timing()
+    res = {
+        "sim": sim, "c": c, "xs": xs, "ys": ys, "st": st, "info": info, "d": d, "nq": nq, "got": got,
+        "branches_per_step": branches_per_step, "elapsed": elapsed, "kernel_avg_ms": kernel_avg_ms,
+        "timing": timing, "dev": dev, "tuning": list(pq.get_walk_tuning()),
+    }
+
+    if not args.no_plain:
+        psim = pq.Simulator(c, strategy="cached", form="plain_fp64")
+        psim.amplitudes(xs[:1], ys[:1])
+        tp0 = time.perf_counter()
+        pa = psim.amplitudes(xs, ys)
+        tp1 = time.perf_counter()
+        scale = float(np.max(np.abs(pa)))
+        res["plain"] = {
+            "amplitudes_per_s": nq / (tp1 - tp0),
+            "branches_per_s": branches_per_step / (tp1 - tp0),
+            "max_abs_err_z8_over_max_amp": float(np.max(np.abs(got - pa)) / scale),
+            "z8_vs_fp64": _err_summary(got, pa),
+        }
+
+    # Roofline of the walk kernel: VALU issue rate.  Algorithmic work per
+    # branch of k_z8_walk: the inner-table step (one sign update and the FMA
+    # into the accumulator; one more lane-op for the LDS address when inner
+    # directions touch pivots).  Per-outer-point, seed and reduction work is
+    # overhead, not counted.
+    ops_per_branch = float(info["ops_per_branch"])
+    cus = int(dev.split("/")[-1].split()[0]) if "CUs" in dev else 256
+    # 128 lanes/clk/CU for FP32/int VALU ops (157.3 TFLOP/s FP32 / 2 per FMA);
+    # rocprof counters on this kernel confirm issue above 64 lanes/clk/CU.
+    peak = cus * 128 * 2.4e9
+    achieved = branches_per_step * ops_per_branch / (kernel_avg_ms * 1e-3)
+    res["roofline"] = {
+        "bound": "valu", "achieved": achieved / 1e12, "peak": peak / 1e12, "unit": "Tlane-op/s",
+        "frac": achieved / peak, "traffic": None, "kernel": "k_z8_walk", "kernel_avg_ms": kernel_avg_ms,
+        "ops_per_branch": ops_per_branch, "branches_per_launch": branches_per_step,
+    }
+
+    if not args.no_cpu:
+        O = _oracle()
+        threads = max(1, min(16, os.cpu_count() or 1))
+        osim = O.OSim(O.OCircuit.from_circuit(c), "cached")
+        nhi = 1 << (d - min(d, 14))
+        done_br, done_sec, amps = 0, 0.0, 0
+        cpu_err = 0.0
+        for i in range(nq):
+            a, sec, br = osim.chunk_range_timed(xs[i], ys[i], 0, nhi, threads)
+            done_br += br; done_sec += sec; amps += 1
+            cpu_err = max(cpu_err, abs(a - got[i]) / max(abs(a), 1e-300))
+            if done_sec > args.cpu_seconds:
+                break
+        res["cpu"] = {
+            "value": done_br / done_sec, "unit": "branches/s", "cores": threads, "kind": "port",
+            "sample": "%d full amplitudes (%d x 2^%d branches, 2^14-branch Gray chunks, OpenMP dynamic) of the same workload"
+                      % (amps, amps, d),
+            "amplitudes_per_s": amps / done_sec, "max_rel_err_z8_vs_cpu": cpu_err,
+        }
+    return res
+
+
+def precision(args, pq, W, np):
+    """n=48 with 40 Rz(theta): every amplitude's error against FP64."""
+    c, xs, ys = W.config3(4, n_amps=1024)
+    z = pq.Simulator(c, strategy="cached", form="z8")
+    d = z.stats()[5]
+    t0 = time.perf_counter(); az = z.amplitudes(xs, ys); t1 = time.perf_counter()
+    aa = z.amplitudes(xs, ys, form="z8_accurate"); t2 = time.perf_counter()
+    p = pq.Simulator(c, strategy="cached", form="plain_fp64")
+    t3 = time.perf_counter(); ap = p.amplitudes(xs, ys); t4 = time.perf_counter()
+    out = {"workload": "n=48: H^48; 40 Rz(theta ~ U[0,2pi)) + 360 CX/CZ + 120 S (seed 4); H^48; 1024 amplitudes <0|C|y>",
+           "branch_dim": d, "z8_vs_fp64": _err_summary(az, ap), "z8_accurate_vs_fp64": _err_summary(aa, ap),
+           "z8_s": t1 - t0, "z8_accurate_s": t2 - t1, "plain_fp64_s": t4 - t3}
+    if not args.no_cpu:
+        O = _oracle()
+        osim = O.OSim(O.OCircuit.from_circuit(c), "cached")
+        osim.set_threads(max(1, min(16, os.cpu_count() or 1)))
+        idx = list(range(0, 1024, 64))
+        ao = np.array([osim.amplitude(xs[i], ys[i]) for i in idx])
+        out["fp64_gpu_vs_cpu_oracle"] = _err_summary(ap[idx], ao)
+        out["z8_vs_cpu_oracle"] = _err_summary(az[idx], ao)
+    if args.precision_out:
+        with open(args.precision_out, "w") as f:
+            json.dump({"y": [int(v) for v in ys], "fp64": [[float(a.real), float(a.imag)] for a in ap],
+                       "z8": [[float(a.real), float(a.imag)] for a in az],
+                       "z8_accurate": [[float(a.real), float(a.imag)] for a in aa]}, f)
+    return out
+
+
+def batch(args, pq, W, np):
+    """256 circuits x 16 queries: one persistent launch, autotuned, graph-replayed."""
+    tg = time.perf_counter()
+    work = W.config4(1, n_circuits=args.batch_circuits)
+    tb = time.perf_counter()
+    sims = [pq.Simulator(c, strategy="cached") for c, _ in work]
+    te = time.perf_counter()
+    S, X, Y = [], [], []
+    for s, (c, pairs) in zip(sims, work):
+        for x, y in pairs:
+            S.append(s); X.append(x); Y.append(y)
+    ds = [s.stats()[5] for s in sims]
+    branches = sum(len(p) * (1 << dd) for dd, (_, p) in zip(ds, work))
+    first = pq.batch_amplitudes(S, X, Y)          # uploads the engine images
+    default = pq.get_walk_tuning()
+    off, mx, table = pq.autotune(S, X, Y, repeats=3, cache=False)
+    eager_ms = []
+    for _ in range(3):
+        t0 = time.perf_counter(); eager = pq.batch_amplitudes(S, X, Y); eager_ms.append(1e3 * (time.perf_counter() - t0))
+    tm = pq.last_timing()
+    g = pq.Graph(S, X, Y)
+    reps, gms, wall = 10, [], []
+    same = True
+    for _ in range(reps):
+        t0 = time.perf_counter()
+        got, ms = g.run()
+        wall.append(1e3 * (time.perf_counter() - t0))
+        gms.append(ms)
+        same = same and bool(np.array_equal(got, eager))
+    out = {
+        "workload": "%d circuits (n ~ U[20,56], T ~ U[8,32], 7.5n CX/CZ, 2.5n S, seed 1) x 16 (x, y)" % len(work),
+        "queries": len(S), "branch_dim_min": min(ds), "branch_dim_max": max(ds), "branches": branches,
+        "generate_s": tb - tg, "host_build_s": te - tb, "host_build_ms_per_circuit": 1e3 * (te - tb) / len(work),
+        "tuning_default": list(default), "tuning_chosen": [off, mx], "autotune_kernel_ms": table,
+        "eager_ms_median": float(np.median(eager_ms)), "eager_kernel_ms": tm["kernel_ms"], "items": tm["items"],
+        "graph_ms_median": float(np.median(gms)), "graph_wall_ms_median": float(np.median(wall)),
+        "graph_replays_equal_eager": same,
+        "amplitudes_per_s": len(S) / (1e-3 * float(np.median(wall))),
+        "branches_per_s": branches / (1e-3 * float(np.median(wall))),
+    }
+    if not args.no_plain:
+        P = {}
+        for s, (c, _) in zip(sims, work):
+            P[id(s)] = pq.Simulator(c, strategy="cached", form="plain_fp64")
+        t0 = time.perf_counter()
+        ref = np.array([P[id(s)].amplitudes([x], [y])[0] for s, x, y in zip(S, X, Y)])
+        t1 = time.perf_counter()
+        out["plain_fp64_amplitudes_per_s"] = len(S) / (t1 - t0)
+        out["z8_vs_fp64"] = _err_summary(eager, ref)
+        out["speedup_vs_plain_amps"] = out["amplitudes_per_s"] / out["plain_fp64_amplitudes_per_s"]
+    del first
+    return out
+
+
+def serve(args, pq, W, np):
+    """Open-loop Poisson load against the request loop."""
+    rnd = random.Random(2024)
+    rng = W.Rng(77)
+    rate, secs = args.serve_rate, args.serve_seconds
+    # Pre-generate the schedule so the submitting thread only sleeps and submits.
+    t, sched, circuits = 0.0, [], []
+    while True:
+        t += rnd.expovariate(rate)
+        if t >= secs:
+            break
+        if not circuits or rnd.random() < 0.5:
+            c = W.config4_circuit(rng)
+            circuits.append(c)
+        else:
+            c = circuits[rnd.randrange(len(circuits))]
+        u = rnd.random()
+        n = c.num_qubits
+        x, y = rnd.getrandbits(n), rnd.getrandbits(n)
+        if u < 0.7:
+            sched.append((t, c, "amplitude", x, y, ()))
+        elif u < 0.9:
+            sched.append((t, c, "probability", x, y, ()))
+        else:
+            sched.append((t, c, "marginal", x, 0, tuple((q, rnd.random() < 0.5) for q in rnd.sample(range(n), 4))))
+    srv = pq.Server(hbm_budget_bytes=64 << 30, workers=8)
+    tickets = []
+    t0 = time.perf_counter()
+    for (ta, c, kind, x, y, meas) in sched:
+        now = time.perf_counter() - t0
+        if ta > now:
+            time.sleep(ta - now)
+        tickets.append(srv.submit(c, kind, x, y, meas))
+    t_sub = time.perf_counter() - t0
+    lat, vals, failed = [], [], 0
+    for tk in tickets:
+        try:
+            v, l = srv.wait(tk, timeout=120.0)
+            lat.append(l); vals.append(v)
+        except pq.PolyqError:
+            failed += 1; vals.append(None)
+    t_end = time.perf_counter() - t0
+    st = srv.stats()
+    srv.close()
+    lat = np.array(lat)
+    # Alone vs in the mix: re-evaluate a sample of served requests one by one.
+    sample = rnd.sample(range(len(sched)), min(96, len(sched)))
+    equal, maxdiff, cache = 0, 0.0, {}
+    for i in sample:
+        _, c, kind, x, y, meas = sched[i]
+        if vals[i] is None:
+            continue
+        s = cache.get(id(c))
+        if s is None:
+            s = cache[id(c)] = pq.Simulator(c, strategy="cached")
+        if kind == "amplitude":
+            want = complex(*s.amplitude(x, y))
+        elif kind == "probability":
+            want = complex(s.probability(x, y), 0.0)
+        else:
+            want = complex(s.marginal(x, meas), 0.0)
+        equal += int(want == vals[i])
+        maxdiff = max(maxdiff, abs(want - vals[i]))
+    kinds = {k: sum(1 for r in sched if r[2] == k) for k in ("amplitude", "probability", "marginal")}
+    return {
+        "workload": "Poisson %.0f req/s for %.0f s; 50%% new circuits (n ~ U[20,56], T ~ U[8,32]), 50%% repeats; "
+                    "70/20/10 amplitude/probability/4-qubit marginal; 64 GB HBM budget" % (rate, secs),
+        "requests": len(sched), "new_circuits": len(circuits), "kinds": kinds, "failed": failed,
+        "p50_ms": float(np.percentile(lat, 50)) if len(lat) else None,
+        "p99_ms": float(np.percentile(lat, 99)) if len(lat) else None,
+        "max_ms": float(lat.max()) if len(lat) else None,
+        "throughput_req_per_s": len(lat) / t_end, "submit_span_s": t_sub, "drain_s": t_end - t_sub,
+        "batches": st["batches"], "overlapped_batches": st["overlapped_batches"], "max_batch": st["max_batch"],
+        "builds": st["builds"], "evictions": st["evictions"], "resident_bytes": st["resident_bytes"],
+        "walk_kernel_ms_total": st["kernel_ms"],
+        "alone_vs_mix": {"checked": len(sample), "bitwise_equal": equal, "max_abs_diff": maxdiff},
+    }
 
 
 def main():
@@ -35,6 +317,11 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
+    ap.add_argument("--sections", default="headline,precision,batch,serve")
+    ap.add_argument("--serve-seconds", type=float, default=60.0)
+    ap.add_argument("--serve-rate", type=float, default=200.0)
+    ap.add_argument("--batch-circuits", type=int, default=256)
+    ap.add_argument("--precision-out", default=None)
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run three 256 MiB device copies (reference traffic for FETCH_SIZE/WRITE_SIZE counters)")
     args = ap.parse_args()
@@ -57,93 +344,17 @@ def main():
     import polyq_amd as pq
     from polyq_amd import workloads as W
 
-    dev = pq.device_info()
-    c, xs, ys = W.config2(SEED)
-    sim = pq.Simulator(c, strategy="cached", form="z8")
-    st = sim.stats()
-    info = sim.z8_info()
-    d = st[5]
-    nq = len(ys)
-    branches_per_step = nq * (1 << d)
-
-    # ---- Z8 batched form (the measured path)
-    for _ in range(args.warmup):
-        ref = sim.amplitudes(xs, ys)
-    kern_ms = []
-    t0 = time.perf_counter()
-    for _ in range(args.steps):
-        got = sim.amplitudes(xs, ys)     # synchronous: returns after the device results are on the host
-        kern_ms.append(pq.last_timing()["kernel_ms"])
-    t1 = time.perf_counter()
-    elapsed = t1 - t0
-    ms_per_step = 1e3 * elapsed / args.steps
-    value = branches_per_step * args.steps / elapsed
-    kernel_avg_ms = float(np.mean(kern_ms))
-    timing = pq.last_timing()
-
-    # ---- plain FP64 form, one query per launch
-    plain = None
-    if not args.no_plain:
-        psim = pq.Simulator(c, strategy="cached", form="plain_fp64")
-        psim.amplitudes(xs[:1], ys[:1])
-        tp0 = time.perf_counter()
-        pa = psim.amplitudes(xs, ys)
-        tp1 = time.perf_counter()
-        pk = pq.last_timing()
-        scale = float(np.max(np.abs(pa)))
-        plain = {
-            "amplitudes_per_s": nq / (tp1 - tp0),
-            "branches_per_s": branches_per_step / (tp1 - tp0),
-            "max_rel_err_z8_vs_fp64": float(np.max(np.abs(got - pa)) / scale),
-        }
-
-    # ---- roofline of the walk kernel: VALU issue rate
-    # Algorithmic work per branch of k_z8_walk: the inner-table step (one sign
-    # update and the FMAs into the accumulators; one more lane-op for the
-    # LDS address when inner directions touch pivots).  Per-outer-point,
-    # seed and reduction work is overhead, not counted.
-    ops_per_branch = float(info["ops_per_branch"])
-    cus = int(dev.split("/")[-1].split()[0]) if "CUs" in dev else 256
-    # 128 lanes/clk/CU for FP32/int VALU ops (157.3 TFLOP/s FP32 / 2 per FMA);
-    # rocprof counters on this kernel confirm issue above 64 lanes/clk/CU.
-    peak = cus * 128 * 2.4e9
-    achieved = branches_per_step * ops_per_branch / (kernel_avg_ms * 1e-3)
-    roofline = {
-        "bound": "valu",
-        "achieved": achieved / 1e12,
-        "peak": peak / 1e12,
-        "unit": "Tlane-op/s",
-        "frac": achieved / peak,
-        "traffic": None,
-        "kernel": "k_z8_walk",
-        "kernel_avg_ms": kernel_avg_ms,
-        "ops_per_branch": ops_per_branch,
-    }
-
-    # ---- CPU baseline: the oracle's Rayon-engine restatement on host cores
-    cpu = None
-    if not args.no_cpu:
-        sys.path.insert(0, os.path.join(ROOT, "tests"))
-        import oracle_lib as O
-        threads = max(1, min(16, os.cpu_count() or 1))
-        osim = O.OSim(O.OCircuit.from_circuit(c), "cached")
-        nhi = 1 << (d - min(d, 14))
-        done_br, done_sec, amps = 0, 0.0, 0
-        max_err = 0.0
-        for i in range(nq):
-            a, sec, br = osim.chunk_range_timed(xs[i], ys[i], 0, nhi, threads)
-            done_br += br; done_sec += sec; amps += 1
-            if done_sec > args.cpu_seconds:
-                break
-        cpu = {
-            "value": done_br / done_sec,
-            "unit": "branches/s",
-            "cores": threads,
-            "kind": "port",
-            "sample": "%d full amplitudes (%d x 2^%d branches, 2^14-branch Gray chunks, OpenMP dynamic) of the same workload" % (amps, amps, d),
-            "amplitudes_per_s": amps / done_sec,
-        }
-
+    sections = set(args.sections.split(","))
+    h = headline(args, pq, W, np)
+    st, d, nq, elapsed = h["st"], h["d"], h["nq"], h["elapsed"]
+    value = h["branches_per_step"] * args.steps / elapsed
+    amps = nq * args.steps / elapsed
+    extra = {}
+    for name, fn in (("precision", precision), ("batch", batch), ("serve", serve)):
+        if name in sections:
+            print("bench: section %s" % name, file=sys.stderr, flush=True)
+            extra[name] = fn(args, pq, W, np)
+    plain, cpu = h.get("plain"), h.get("cpu")
     line = {
         "metric": METRIC,
         "value": value,
@@ -151,25 +362,30 @@ def main():
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": ms_per_step,
+        "ms_per_step": 1e3 * elapsed / args.steps,
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "int32 phases + fp32 two-sum (fp64 cross-workgroup)",
-        "data": "synthetic (seeded SplitMix64 circuit)",
+        "data": "synthetic (seeded SplitMix64 circuits)",
         "config": {
-            "workload": "n=40: H^40; 36 T + 300 CX/CZ + 100 S interleaved (seed %d); H^40; 16 amplitudes <y|C|0>" % SEED,
+            "workload": "n=40: H^40; 36 T + 300 CX/CZ + 100 S interleaved (seed %d); H^40; 16 amplitudes <0|C|y>" % SEED,
             "terms": st[4], "branch_dim": d, "free_vars": st[0], "planes": st[1], "lines": st[2], "radical": st[3],
-            "branches_per_step": branches_per_step, "inner_pivots": info["inner_pivots"], "slots": info["slots"],
-            "items": timing["items"], "grid": timing["grid"], "device": dev,
+            "branches_per_step": h["branches_per_step"], "inner_pivots": h["info"]["inner_pivots"],
+            "slots": h["info"]["slots"], "items": h["timing"]["items"], "grid": h["timing"]["grid"], "device": h["dev"],
+            "walk_tuning": h["tuning"],
         },
-        "amplitudes_per_s": nq * args.steps / elapsed,
+        "amplitudes_per_s": amps,
         "plain_fp64": plain,
-        "speedup_vs_plain_amps": (nq * args.steps / elapsed) / plain["amplitudes_per_s"] if plain else None,
-        "roofline": roofline,
+        "speedup_vs_plain_amps": amps / plain["amplitudes_per_s"] if plain else None,
+        "roofline": h["roofline"],
         "cpu_baseline": cpu,
         "speedup_vs_cpu": value / cpu["value"] if cpu else None,
     }
+    line.update(extra)
+    if "serve" in extra:
+        line["p50_ms"] = extra["serve"]["p50_ms"]
+        line["p99_ms"] = extra["serve"]["p99_ms"]
     print(json.dumps(line))
 
 

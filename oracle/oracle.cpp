@@ -1327,7 +1327,10 @@ int orc_sim_chunk_range_timed(orc_sim* h, const uint8_t* x, const uint8_t* y, ui
             sr += r.re; si += r.im;
         }
         *seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
-        out[0] = sr; out[1] = si;
+        // the partial sum carries the engine's phase and magnitude (engine.rs:557-561),
+        // so a full range [0, nhi) is the amplitude of a single-engine strategy
+        const C64 a = to_complex(ph) * C64(sr, si) * E.magnitude();
+        out[0] = a.re; out[1] = a.im;
         *branches = (hi1 > hi0 ? hi1 - hi0 : 0) << dl;
         return 0;
     } catch (std::exception& e) { g_err = e.what(); return -1; }

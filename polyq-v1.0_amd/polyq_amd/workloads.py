@@ -162,27 +162,30 @@ def sandwich_layer(n, n_nonclifford, n_twoq, n_s, rng, angle=None):
 
 
 def config1(seed=1):
-    """n=12, 150 uniform {H,S,CX,CZ,T} gates; x = 0 and 32 random y."""
+    """n=12, 150 uniform {H,S,CX,CZ,T} gates; amplitudes <0|C|y> for 32 random
+    inputs y.  Returns (circuit, inputs, outputs)."""
     rng = Rng(seed)
     c = uniform_gates(12, 150, rng)
     ys = [rng.below(1 << 12) for _ in range(32)]
-    return c, [0] * 32, ys
+    return c, ys, [0] * 32
 
 
 def config2(seed=1):
-    """n=40: H^40; 36 T + 300 CX/CZ + 100 S interleaved; H^40; x = 0 and 16 random y."""
+    """n=40: H^40; 36 T + 300 CX/CZ + 100 S interleaved; H^40; amplitudes
+    <0|C|y> for 16 random inputs y.  Returns (circuit, inputs, outputs)."""
     rng = Rng(seed)
     c = sandwich_layer(40, 36, 300, 100, rng)
     ys = [rng.below(1 << 40) for _ in range(16)]
-    return c, [0] * 16, ys
+    return c, ys, [0] * 16
 
 
 def config3(seed=1, n_amps=1024):
-    """n=48, the config-2 shape with 40 Rz(theta), theta ~ U[0, 2pi); x = 0 and random y."""
+    """n=48, the config-2 shape with 40 Rz(theta), theta ~ U[0, 2pi);
+    amplitudes <0|C|y> for random inputs y.  Returns (circuit, inputs, outputs)."""
     rng = Rng(seed)
     c = sandwich_layer(48, 40, 360, 120, rng, angle="uniform")
     ys = [rng.below(1 << 48) for _ in range(n_amps)]
-    return c, [0] * n_amps, ys
+    return c, ys, [0] * n_amps
 
 
 def config4_circuit(rng):

@@ -284,3 +284,16 @@ def test_toffoli_and_circuit_known_answers():
     oc = O.OCircuit.from_circuit(c)
     s = O.OSim(oc, "auto")
     np.testing.assert_allclose([s.amplitude(0, y) for y in range(16)], O.dense(oc, 0), atol=1e-9)
+
+
+def test_timed_full_range_is_the_amplitude():
+    # the bench's CPU leg times whole coset ranges; over [0, nhi) it must be the amplitude
+    rng = W.Rng(31)
+    for trial in range(6):
+        c = W.sandwich_layer(10, 8 + trial, 40, 12, rng)
+        o = O.OSim(O.OCircuit.from_circuit(c), "cached")
+        d = o.stats()[5]
+        x, y = rng.below(1 << 10), rng.below(1 << 10)
+        a, sec, br = o.chunk_range_timed(x, y, 0, 1 << (d - min(d, 14)), 2)
+        assert abs(a - o.amplitude(x, y)) < 1e-12
+        assert br == (1 << d) or br == 0
