@@ -4,8 +4,12 @@ Headline (the JSON line's `value`): the plain-GPU-at-scale workload, one step
 = the 16 amplitudes <0|C|y> of the n=40 circuit "H on all qubits; 36 T + 300
 CX/CZ + 100 S interleaved (SplitMix64 seed 5); H on all qubits" (t = 30 terms
 after T merging, branch dimension d = 30: 16 x 2^30 branches), all 16
-evaluated in one persistent Z8-walk launch.  Engine images are resident in HBM
-before timing starts.  Reported beside it:
+evaluated in one persistent segment-walk launch (2^14-branch Walsh-Hadamard
+segments built once per circuit, the other 16 coordinates walked in Gray
+order).  `value` counts the 2^d branches each amplitude sums.  Engine images
+are resident in HBM before timing starts.  Reported beside it:
+  * the segment-walk kernel on a saturating launch (4096 amplitudes),
+  * the per-branch Z8 walk (every branch visited) on the same 16 amplitudes,
   * the plain FP64 form (one query per launch, FP64 port of the reference's
     Gray walk) on the same 16 amplitudes, and per-amplitude error against it,
   * the CPU oracle (a C++ restatement of the reference's Rayon engine, OpenMP
@@ -68,15 +72,43 @@ def _err_summary(got, ref):
     return out
 
 
+SEG_OPS_PER_POINT = 30.0
+# Algorithmic lane-ops per outer point of k_seg_walk: the integer walk step
+# (phase update with the cross-term parity, sign-mask flip, table index: 6)
+# and the double-float arithmetic (walk magnitude times the complex table
+# entry: 2 x 4; compensated complex accumulation: 2 x 8).  The lane seed, the
+# lane factor and the reductions are overhead, not counted.
+
+
+def _peak(dev):
+    cus = int(dev.split("/")[-1].split()[0]) if "CUs" in dev else 256
+    # 128 lanes/clk/CU for FP32/int VALU ops (157.3 TFLOP/s FP32 / 2 per FMA);
+    # rocprof counters on the walk kernels confirm issue above 64 lanes/clk/CU.
+    return cus * 128 * 2.4e9
+
+
+def _seg_roofline(dev, points, kernel_ms, extra=None):
+    peak = _peak(dev)
+    achieved = points * SEG_OPS_PER_POINT / (kernel_ms * 1e-3)
+    r = {"bound": "valu", "achieved": achieved / 1e12, "peak": peak / 1e12, "unit": "Tlane-op/s", "frac": achieved / peak,
+         "traffic": None, "kernel": "k_seg_walk", "kernel_avg_ms": kernel_ms, "ops_per_outer_point": SEG_OPS_PER_POINT,
+         "outer_points_per_launch": points}
+    if extra:
+        r.update(extra)
+    return r
+
+
 def headline(args, pq, W, np):
     dev = pq.device_info()
     c, xs, ys = W.config2(SEED)
     sim = pq.Simulator(c, strategy="cached", form="z8")
     st = sim.stats()
+    seg = sim.seg_info()
     info = sim.z8_info()
     d = st[5]
     nq = len(ys)
     branches_per_step = nq * (1 << d)
+    points_per_step = nq * (1 << seg["outer"])
 
     for _ in range(args.warmup):
         sim.amplitudes(xs, ys)
@@ -90,9 +122,43 @@ def headline(args, pq, W, np):
     kernel_avg_ms = float(np.mean(kern_ms))
     timing = pq.last_timing()
     res = {
-        "sim": sim, "c": c, "xs": xs, "ys": ys, "st": st, "info": info, "d": d, "nq": nq, "got": got,
+        "sim": sim, "c": c, "xs": xs, "ys": ys, "st": st, "info": info, "seg": seg, "d": d, "nq": nq, "got": got,
         "branches_per_step": branches_per_step, "elapsed": elapsed, "kernel_avg_ms": kernel_avg_ms,
         "timing": timing, "dev": dev, "tuning": list(pq.get_walk_tuning()),
+    }
+    # Roofline of k_seg_walk on this launch (16 queries: 2^20 outer points,
+    # too few lanes to fill 256 CUs) and on a saturating launch of the same
+    # engine (4096 queries <0|C|y>, random y).
+    res["roofline"] = _seg_roofline(dev, points_per_step, kernel_avg_ms)
+    rng = W.Rng(99)
+    big_x = [rng.below(1 << 40) for _ in range(4096)]
+    big_y = [0] * 4096
+    sim.amplitudes(big_x, big_y)
+    bms, bwall = [], []
+    for _ in range(5):
+        tb = time.perf_counter()
+        sim.amplitudes(big_x, big_y)
+        bwall.append(time.perf_counter() - tb)
+        bms.append(pq.last_timing()["kernel_ms"])
+    res["roofline_saturated"] = _seg_roofline(dev, 4096 * (1 << seg["outer"]), float(np.median(bms)),
+                                              {"queries": 4096, "amplitudes_per_s": 4096 / float(np.median(bwall)),
+                                               "branches_per_s": 4096 * (1 << d) / float(np.median(bwall))})
+
+    # The per-branch Z8 walk on the same 16 amplitudes (every branch visited:
+    # one sign XOR and one FMA per branch, one more LDS lookup with inner pivots).
+    bw = []
+    for _ in range(3):
+        sim.amplitudes(xs, ys, form="branch_walk")
+        bw.append(pq.last_timing()["kernel_ms"])
+    bw_ms = float(np.median(bw))
+    ga = sim.amplitudes(xs, ys, form="branch_walk")
+    peak = _peak(dev)
+    res["branch_walk"] = {
+        "kernel": "k_z8_walk", "kernel_ms": bw_ms, "branches_per_s": branches_per_step / (bw_ms * 1e-3),
+        "roofline": {"bound": "valu", "achieved": branches_per_step * info["ops_per_branch"] / (bw_ms * 1e-3) / 1e12,
+                     "peak": peak / 1e12, "unit": "Tlane-op/s",
+                     "frac": branches_per_step * info["ops_per_branch"] / (bw_ms * 1e-3) / peak,
+                     "ops_per_branch": info["ops_per_branch"]},
     }
 
     if not args.no_plain:
@@ -107,24 +173,8 @@ def headline(args, pq, W, np):
             "branches_per_s": branches_per_step / (tp1 - tp0),
             "max_abs_err_z8_over_max_amp": float(np.max(np.abs(got - pa)) / scale),
             "z8_vs_fp64": _err_summary(got, pa),
+            "branch_walk_vs_fp64": _err_summary(ga, pa),
         }
-
-    # Roofline of the walk kernel: VALU issue rate.  Algorithmic work per
-    # branch of k_z8_walk: the inner-table step (one sign update and the FMA
-    # into the accumulator; one more lane-op for the LDS address when inner
-    # directions touch pivots).  Per-outer-point, seed and reduction work is
-    # overhead, not counted.
-    ops_per_branch = float(info["ops_per_branch"])
-    cus = int(dev.split("/")[-1].split()[0]) if "CUs" in dev else 256
-    # 128 lanes/clk/CU for FP32/int VALU ops (157.3 TFLOP/s FP32 / 2 per FMA);
-    # rocprof counters on this kernel confirm issue above 64 lanes/clk/CU.
-    peak = cus * 128 * 2.4e9
-    achieved = branches_per_step * ops_per_branch / (kernel_avg_ms * 1e-3)
-    res["roofline"] = {
-        "bound": "valu", "achieved": achieved / 1e12, "peak": peak / 1e12, "unit": "Tlane-op/s",
-        "frac": achieved / peak, "traffic": None, "kernel": "k_z8_walk", "kernel_avg_ms": kernel_avg_ms,
-        "ops_per_branch": ops_per_branch, "branches_per_launch": branches_per_step,
-    }
 
     if not args.no_cpu:
         O = _oracle()
@@ -154,12 +204,12 @@ def precision(args, pq, W, np):
     z = pq.Simulator(c, strategy="cached", form="z8")
     d = z.stats()[5]
     t0 = time.perf_counter(); az = z.amplitudes(xs, ys); t1 = time.perf_counter()
-    aa = z.amplitudes(xs, ys, form="z8_accurate"); t2 = time.perf_counter()
+    aa = z.amplitudes(xs, ys, form="branch_walk"); t2 = time.perf_counter()
     p = pq.Simulator(c, strategy="cached", form="plain_fp64")
     t3 = time.perf_counter(); ap = p.amplitudes(xs, ys); t4 = time.perf_counter()
     out = {"workload": "n=48: H^48; 40 Rz(theta ~ U[0,2pi)) + 360 CX/CZ + 120 S (seed 4); H^48; 1024 amplitudes <0|C|y>",
-           "branch_dim": d, "z8_vs_fp64": _err_summary(az, ap), "z8_accurate_vs_fp64": _err_summary(aa, ap),
-           "z8_s": t1 - t0, "z8_accurate_s": t2 - t1, "plain_fp64_s": t4 - t3}
+           "branch_dim": d, "z8_vs_fp64": _err_summary(az, ap), "branch_walk_vs_fp64": _err_summary(aa, ap),
+           "z8_s": t1 - t0, "branch_walk_s": t2 - t1, "plain_fp64_s": t4 - t3}
     if not args.no_cpu:
         O = _oracle()
         osim = O.OSim(O.OCircuit.from_circuit(c), "cached")
@@ -172,7 +222,7 @@ def precision(args, pq, W, np):
         with open(args.precision_out, "w") as f:
             json.dump({"y": [int(v) for v in ys], "fp64": [[float(a.real), float(a.imag)] for a in ap],
                        "z8": [[float(a.real), float(a.imag)] for a in az],
-                       "z8_accurate": [[float(a.real), float(a.imag)] for a in aa]}, f)
+                       "branch_walk": [[float(a.real), float(a.imag)] for a in aa]}, f)
     return out
 
 
@@ -312,8 +362,8 @@ def serve(args, pq, W, np):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=10)
-    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
@@ -366,19 +416,21 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "int32 phases + fp32 two-sum (fp64 cross-workgroup)",
+        "dtype": "int32 Z8 phases + fp32 hi/lo pairs (fp64 tables built once per circuit; fp64 cross-workgroup)",
         "data": "synthetic (seeded SplitMix64 circuits)",
         "config": {
             "workload": "n=40: H^40; 36 T + 300 CX/CZ + 100 S interleaved (seed %d); H^40; 16 amplitudes <0|C|y>" % SEED,
             "terms": st[4], "branch_dim": d, "free_vars": st[0], "planes": st[1], "lines": st[2], "radical": st[3],
-            "branches_per_step": h["branches_per_step"], "inner_pivots": h["info"]["inner_pivots"],
-            "slots": h["info"]["slots"], "items": h["timing"]["items"], "grid": h["timing"]["grid"], "device": h["dev"],
+            "branches_per_step": h["branches_per_step"], "segment": h["seg"],
+            "items": h["timing"]["items"], "grid": h["timing"]["grid"], "device": h["dev"],
             "walk_tuning": h["tuning"],
         },
         "amplitudes_per_s": amps,
         "plain_fp64": plain,
         "speedup_vs_plain_amps": amps / plain["amplitudes_per_s"] if plain else None,
         "roofline": h["roofline"],
+        "roofline_saturated": h["roofline_saturated"],
+        "branch_walk": h["branch_walk"],
         "cpu_baseline": cpu,
         "speedup_vs_cpu": value / cpu["value"] if cpu else None,
     }

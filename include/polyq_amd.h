@@ -42,11 +42,17 @@ enum pq_strategy { PQ_AUTO = 0, PQ_CACHED = 1, PQ_PAIRS = 2,
      * the device's walk rate; picks p = 0 unless d is far beyond 40. */
     PQ_GPU_AUTO = 3 };
 
-/* Device evaluation form.  Z8: integer Z8 phases, FP32 two-sum compensated
- * lane accumulation, FP64 cross-workgroup reduction (default).  Z8_ACCURATE:
- * the same walk folding every 32 branches instead of 256.  PLAIN_FP64: a
- * straightforward FP64 port of Engine::gray, one query per launch. */
-enum pq_form { PQ_FORM_Z8 = 0, PQ_FORM_Z8_ACCURATE = 1, PQ_FORM_PLAIN_FP64 = 2 };
+/* Device evaluation form.
+ *  Z8 (default): segment walk.  The sum over 2^I inner kernel coordinates is
+ *    a per-engine Walsh-Hadamard table (built once in FP64, stored as FP32
+ *    hi/lo pairs); lanes walk the outer coordinates in Gray order with integer
+ *    Z8 phases and add one table entry per outer point in double-float (FP32
+ *    pair) arithmetic; FP64 only in the cross-workgroup reduction and scaling.
+ *  BRANCH_WALK: the per-branch Z8 walk (one FMA per branch over a 256-branch
+ *    inner table, FP32 two-sum compensated lane accumulators).
+ *  PLAIN_FP64: a straightforward FP64 port of Engine::gray, one query per
+ *    launch (src/engine.rs:669-722). */
+enum pq_form { PQ_FORM_Z8 = 0, PQ_FORM_BRANCH_WALK = 1, PQ_FORM_PLAIN_FP64 = 2 };
 
 typedef struct pq_sim pq_sim;
 
@@ -87,12 +93,17 @@ int pq_sim_amplitudes(pq_sim* sim, size_t n, const uint64_t* inputs, const uint6
  * only).  out has 2n doubles. */
 int pq_batch_amplitudes(size_t n, pq_sim* const* sims, const uint64_t* inputs, const uint64_t* outputs, int form, double* out);
 
-/* Segment split of the Z8 walk (no counterpart in the reference): a query of
- * branch dimension d is walked by lanes of 2^(8+M) branches with
- * M = clamp(d - offset, 0, max_mid).  Set by the autotuner; a fixed setting
- * makes every result independent of the batch it runs in. */
-int pq_set_walk_tuning(uint32_t offset, uint32_t max_mid);
-int pq_get_walk_tuning(uint32_t out2[2]);
+/* Lane walk length of a batched form (no counterpart in the reference).  A
+ * lane walks 2^M outer Gray points, M = clamp(n - offset, 0, max_mid), with n
+ * the outer coordinates of the query (PQ_FORM_Z8: d - I; PQ_FORM_BRANCH_WALK:
+ * d - 8).  Set by the autotuner; a fixed setting makes every result
+ * independent of the batch it runs in. */
+int pq_set_walk_tuning(int form, uint32_t offset, uint32_t max_mid);
+int pq_get_walk_tuning(int form, uint32_t out2[2]);
+/* Table bits I + npin of the segment tables of engines built afterwards
+ * (default 14: 2^14 entries x 8 rotations x 16 B = 2 MiB per engine). */
+int pq_set_segment_bits(uint32_t bits);
+uint32_t pq_get_segment_bits(void);
 
 /* Timing of the last device evaluation on this thread: kernel_ms, total_ms,
  * branches, items, grid (doubles). */
@@ -148,7 +159,24 @@ int pq_sim_selfcheck(pq_sim* sim, uint64_t x, uint64_t y, uint32_t trials, uint6
 /* Host-only: stats of the marginal (sandwich) engine for a measured-qubit set,
  * in pq_sim_stats order; builds and caches it (src/sim.rs:225-231). */
 int pq_sim_marginal_stats(pq_sim* sim, const uint32_t* qubits, size_t k, uint64_t out7[7]);
-/* Z8 image summary: d, inner directions, slots, compact inner pivots,
+/* Segment image summary: d, inner directions I, inner pivots, outer
+ * coordinates, lane-low slots, walkable slots, table bytes, lane walk length
+ * M at the current tuning. */
+int pq_sim_seg_info(pq_sim* sim, uint64_t out8[8]);
+/* Integer state (K mod 8, table-basis sign mask, inner pivots) at every outer
+ * point of the segment walk for <y|C|x>, indexed by (lane << M) + o: from the
+ * device walk (seg_dump) and from the host restatement (seg_lane_ref).
+ * n = 2^(d - I) entries. */
+int pq_sim_seg_dump(pq_sim* sim, uint64_t x, uint64_t y, uint32_t mid_bits, size_t cap, uint8_t* k, uint32_t* hs, uint32_t* pio,
+                    size_t* n);
+int pq_sim_seg_lane_ref(pq_sim* sim, uint64_t x, uint64_t y, uint32_t mid_bits, size_t cap, uint8_t* k, uint32_t* hs,
+                        uint32_t* pio, size_t* n);
+/* Host-only check of the segment tables: for `trials` random outer points,
+ * the table entry times the lane magnitude against the direct sum of the
+ * 2^I inner branches with the packed-row phase (normal_form.rs:479-493);
+ * *max_err = max |difference| / sum of branch magnitudes. */
+int pq_sim_seg_selfcheck(pq_sim* sim, uint64_t x, uint64_t y, uint32_t trials, uint64_t seed, double* max_err);
+/* Branch-walk image summary: d, inner directions, slots, compact inner pivots,
  * rank K, whether inner entries alternate real/imaginary, VALU lane-ops per
  * branch of the inner step. */
 int pq_sim_z8_info(pq_sim* sim, uint32_t out7[7]);

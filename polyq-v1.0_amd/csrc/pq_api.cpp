@@ -19,8 +19,10 @@ std::shared_ptr<RankEng> build_rank(const PathSum& ps, const std::vector<uint32_
     auto R = std::make_shared<RankEng>();
     auto add = [&](std::shared_ptr<HostEngine> he) {
         auto z = std::make_shared<Z8Image>(build_z8_image(*he));
+        auto g = std::make_shared<SegImage>(build_seg_image(*he));
         R->host.push_back(he);
         R->z8.push_back(z);
+        R->seg.push_back(g);
     };
     uint32_t nt = 0;
     auto cands = pair_candidates(ps, fixed, nt);
@@ -63,6 +65,7 @@ void add_jobs(const RankEng& R, const std::vector<std::pair<uint32_t, uint8_t>>&
         Z8Job J;
         J.eng = R.dev[b].get();
         J.q = z8_query(*R.host[b], *R.z8[b], P);
+        J.sq = seg_query(*R.host[b], *R.seg[b], P);
         J.scale = P.scale;
         J.mid_bits = 0;
         jobs.push_back(J);
@@ -71,19 +74,23 @@ void add_jobs(const RankEng& R, const std::vector<std::pair<uint32_t, uint8_t>>&
     }
 }
 
-void assign_mid_bits(std::vector<Z8Job>& jobs) {
-    uint64_t total = 0;
-    for (auto& J : jobs) total += 1ull << dev_z8(*J.eng).d;
-    for (auto& J : jobs) J.mid_bits = z8_auto_mid_bits(dev_z8(*J.eng).d, total);
+int walk_of_form(int form) {
+    if (form == PQ_FORM_Z8) return kWalkSeg;
+    if (form == PQ_FORM_BRANCH_WALK) return kWalkBranch;
+    throw std::invalid_argument("batched evaluation uses PQ_FORM_Z8 or PQ_FORM_BRANCH_WALK");
 }
 
-void run_jobs(std::vector<Z8Job>& jobs, const std::vector<size_t>& owner, size_t nq, bool accurate, std::vector<std::complex<double>>& out) {
+void assign_mid_bits(std::vector<Z8Job>& jobs, int walk) {
+    for (auto& J : jobs) J.mid_bits = dev_mid_bits(*J.eng, walk);
+}
+
+void run_jobs(std::vector<Z8Job>& jobs, const std::vector<size_t>& owner, size_t nq, int walk, std::vector<std::complex<double>>& out) {
     out.assign(nq, {0, 0});
     if (jobs.empty()) { for (double& x : g_timing) x = 0; return; }
-    assign_mid_bits(jobs);
+    assign_mid_bits(jobs, walk);
     std::vector<std::complex<double>> r;
     Z8Timing T;
-    dev_z8_batch(jobs, r, &T, accurate);
+    dev_z8_batch(jobs, r, &T, walk);
     for (size_t i = 0; i < jobs.size(); ++i) out[owner[i]] += r[i];
     g_timing[0] = T.kernel_ms; g_timing[1] = T.total_ms; g_timing[2] = (double)T.branches; g_timing[3] = (double)T.items; g_timing[4] = T.grid;
 }
@@ -208,7 +215,8 @@ int pq_sim_amplitudes(pq_sim* h, size_t n, const uint64_t* xs, const uint64_t* y
                 if (!h->s.assignment(xs[i], ys[i], asg)) continue;
                 res[i] = plain_eval(*h->s.eng, asg);
             }
-        } else if (form == PQ_FORM_Z8 || form == PQ_FORM_Z8_ACCURATE) {
+        } else {
+            const int walk = walk_of_form(form);
             std::vector<Z8Job> jobs;
             std::vector<size_t> owner;
             for (size_t i = 0; i < n; ++i) {
@@ -216,8 +224,8 @@ int pq_sim_amplitudes(pq_sim* h, size_t n, const uint64_t* xs, const uint64_t* y
                 if (!h->s.assignment(xs[i], ys[i], asg)) continue;   // contradictory boundary: exactly 0
                 add_jobs(*h->s.eng, asg, jobs, owner, i, nullptr);
             }
-            run_jobs(jobs, owner, n, form == PQ_FORM_Z8_ACCURATE, res);
-        } else throw std::invalid_argument("unknown form");
+            run_jobs(jobs, owner, n, walk, res);
+        }
         for (size_t i = 0; i < n; ++i) { out[2 * i] = res[i].real(); out[2 * i + 1] = res[i].imag(); }
         return 0;
     })
@@ -251,7 +259,7 @@ int pq_sim_marginal_probability(pq_sim* h, uint64_t x, const uint32_t* qubits, c
             std::vector<Z8Job> jobs; std::vector<size_t> owner;
             add_jobs(R, a, jobs, owner, 0, nullptr);
             std::vector<std::complex<double>> res;
-            run_jobs(jobs, owner, 1, form == PQ_FORM_Z8_ACCURATE, res);
+            run_jobs(jobs, owner, 1, walk_of_form(form), res);
             v = res[0];
         }
         *out = std::max(v.real(), 0.0);   // sim.rs:230
@@ -261,7 +269,7 @@ int pq_sim_marginal_probability(pq_sim* h, uint64_t x, const uint32_t* qubits, c
 
 int pq_batch_amplitudes(size_t n, pq_sim* const* sims, const uint64_t* xs, const uint64_t* ys, int form, double* out) {
     PQ_TRY({
-        if (form != PQ_FORM_Z8 && form != PQ_FORM_Z8_ACCURATE) throw std::invalid_argument("batches use the Z8 forms");
+        const int walk = walk_of_form(form);
         std::vector<Z8Job> jobs; std::vector<size_t> owner;
         std::vector<std::pair<uint32_t, uint8_t>> asg;
         for (size_t i = 0; i < n; ++i) {
@@ -270,20 +278,34 @@ int pq_batch_amplitudes(size_t n, pq_sim* const* sims, const uint64_t* xs, const
             add_jobs(*sims[i]->s.eng, asg, jobs, owner, i, nullptr);
         }
         std::vector<std::complex<double>> res;
-        run_jobs(jobs, owner, n, form == PQ_FORM_Z8_ACCURATE, res);
+        run_jobs(jobs, owner, n, walk, res);
         for (size_t i = 0; i < n; ++i) { out[2 * i] = res[i].real(); out[2 * i + 1] = res[i].imag(); }
         return 0;
     })
 }
 
-int pq_set_walk_tuning(uint32_t offset, uint32_t max_mid) { PQ_TRY({ z8_set_tuning(offset, max_mid); return 0; }) }
-int pq_get_walk_tuning(uint32_t* o) { z8_get_tuning(&o[0], &o[1]); return 0; }
+int pq_set_walk_tuning(int form, uint32_t offset, uint32_t max_mid) {
+    PQ_TRY({
+        if (walk_of_form(form) == kWalkSeg) seg_set_tuning(offset, max_mid);
+        else z8_set_tuning(offset, max_mid);
+        return 0;
+    })
+}
+int pq_get_walk_tuning(int form, uint32_t* o) {
+    PQ_TRY({
+        if (walk_of_form(form) == kWalkSeg) seg_get_tuning(&o[0], &o[1]);
+        else z8_get_tuning(&o[0], &o[1]);
+        return 0;
+    })
+}
+int pq_set_segment_bits(uint32_t bits) { PQ_TRY({ seg_set_bits(bits); return 0; }) }
+uint32_t pq_get_segment_bits(void) { return seg_get_bits(); }
 
 int pq_last_timing(double* o) { for (int i = 0; i < 5; ++i) o[i] = g_timing[i]; return 0; }
 
 int pq_graph_create(size_t n, pq_sim* const* sims, const uint64_t* xs, const uint64_t* ys, int form, pq_graph** out) {
     PQ_TRY({
-        if (form != PQ_FORM_Z8 && form != PQ_FORM_Z8_ACCURATE) throw std::invalid_argument("graphs use the Z8 forms");
+        const int walk = walk_of_form(form);
         auto g = std::make_unique<pq_graph>();
         std::vector<Z8Job> jobs;
         std::vector<std::pair<uint32_t, uint8_t>> asg;
@@ -293,9 +315,9 @@ int pq_graph_create(size_t n, pq_sim* const* sims, const uint64_t* xs, const uin
             g->keep.push_back(sims[i]->s.eng);
         }
         if (jobs.empty()) throw std::invalid_argument("every query of the graph is identically zero");
-        assign_mid_bits(jobs);
+        assign_mid_bits(jobs, walk);
         g->nq = n;
-        g->g = dev_z8_graph(jobs, form == PQ_FORM_Z8_ACCURATE);
+        g->g = dev_z8_graph(jobs, walk);
         *out = g.release();
         return 0;
     })
@@ -401,6 +423,126 @@ int pq_sim_marginal_stats(pq_sim* h, const uint32_t* qubits, size_t k, uint64_t*
         const RankEng& R = h->s.marginal_engine(qs);
         EngineStats s = R.host[0]->stats();
         o[0] = s.free_vars; o[1] = s.planes; o[2] = s.lines; o[3] = s.radical; o[4] = s.terms; o[5] = s.branch_dim; o[6] = R.pairs;
+        return 0;
+    })
+}
+
+int pq_sim_seg_info(pq_sim* h, uint64_t* o) {
+    PQ_TRY({
+        const SegImage& G = *h->s.eng->seg[0];
+        uint32_t off, mx;
+        seg_get_tuning(&off, &mx);
+        o[0] = G.d; o[1] = G.I; o[2] = G.npin; o[3] = G.nout; o[4] = G.L6; o[5] = G.Mcap; o[6] = G.table.size() * 4;
+        o[7] = seg_mid_bits(G, off, mx);
+        return 0;
+    })
+}
+
+// Shared set-up of the segment dumps: query data and sizes for <y|C|x>.
+static bool seg_dump_setup(pq_sim* h, uint64_t x, uint64_t y, uint32_t mid_bits, size_t cap, SegQueryHost& q, uint32_t& M, size_t& total) {
+    std::vector<std::pair<uint32_t, uint8_t>> asg;
+    if (!h->s.assignment(x, y, asg)) return false;
+    const RankEng& R = *h->s.eng;
+    const HostEngine& E = *R.host[0];
+    const SegImage& G = *R.seg[0];
+    Prepared P;
+    if (!prepare_assignment(E, asg, P)) return false;
+    q = seg_query(E, G, P);
+    M = std::min(mid_bits, std::min(G.Mcap, G.nout - G.L6));
+    if (G.nout > 24) throw std::invalid_argument("segment dumps are limited to 2^24 outer points");
+    total = (size_t)1 << G.nout;
+    if (total > cap) throw std::invalid_argument("segment dump buffer too small");
+    return true;
+}
+
+int pq_sim_seg_dump(pq_sim* h, uint64_t x, uint64_t y, uint32_t mid_bits, size_t cap, uint8_t* k, uint32_t* hs, uint32_t* pio, size_t* n) {
+    PQ_TRY({
+        SegQueryHost q; uint32_t M; size_t total;
+        *n = 0;
+        if (!seg_dump_setup(h, x, y, mid_bits, cap, q, M, total)) return 1;
+        const RankEng& R = *h->s.eng;
+        R.ensure_device();
+        dev_dump_seg(*R.dev[0], q, M, k, hs, pio);
+        *n = total;
+        return 0;
+    })
+}
+
+int pq_sim_seg_lane_ref(pq_sim* h, uint64_t x, uint64_t y, uint32_t mid_bits, size_t cap, uint8_t* k, uint32_t* hs, uint32_t* pio, size_t* n) {
+    PQ_TRY({
+        SegQueryHost q; uint32_t M; size_t total;
+        *n = 0;
+        if (!seg_dump_setup(h, x, y, mid_bits, cap, q, M, total)) return 1;
+        const SegImage& G = *h->s.eng->seg[0];
+        for (size_t idx = 0; idx < total; ++idx) {
+            const SegLane L = seg_lane_state(G, q, M, idx >> M, (uint32_t)(idx & ((1ull << M) - 1)));
+            k[idx] = (uint8_t)L.K; hs[idx] = L.hs; pio[idx] = L.pio;
+        }
+        *n = total;
+        return 0;
+    })
+}
+
+int pq_sim_seg_selfcheck(pq_sim* h, uint64_t x, uint64_t y, uint32_t trials, uint64_t seed, double* max_err) {
+    PQ_TRY({
+        std::vector<std::pair<uint32_t, uint8_t>> asg;
+        *max_err = 0;
+        if (!h->s.assignment(x, y, asg)) return 1;
+        static const double Hh = 0.70710678118654752440;
+        const double w8r[8] = {1, Hh, 0, -Hh, -1, -Hh, 0, Hh}, w8i[8] = {0, Hh, 1, Hh, 0, -Hh, -1, -Hh};
+        for (size_t b = 0; b < h->s.eng->host.size(); ++b) {
+            const HostEngine& E = *h->s.eng->host[b];
+            const SegImage& G = *h->s.eng->seg[b];
+            Prepared P;
+            if (!prepare_assignment(E, asg, P)) continue;
+            const SegQueryHost q = seg_query(E, G, P);
+            const uint32_t W = E.L.words(), M = 0;
+            uint64_t st = seed + 0x9E3779B97F4A7C15ull * (b + 1);
+            for (uint32_t tr = 0; tr < trials; ++tr) {
+                st += 0x9E3779B97F4A7C15ull;
+                uint64_t z = st; z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull; z = (z ^ (z >> 27)) * 0x94D049BB133111EBull; z ^= z >> 31;
+                const uint64_t lane = G.nout >= 64 ? z : (z & ((1ull << G.nout) - 1));
+                const SegLane L = seg_lane_state(G, q, M, lane, 0);
+                // segment value from the image: omega^K T[pio][hs] * lane magnitude (FP64 from hi + lo)
+                const float* e = &G.table[(((size_t)L.pio << G.I) | L.hs) * 4];
+                double fl = 1.0;
+                for (uint32_t k = 0; k < E.t; ++k) {
+                    const bool inner_or_pin = !((G.lane_terms >> k) & 1);
+                    bool walk_free = false;
+                    for (uint32_t m = 0; m < G.Mcap; ++m) if (E.free_col[G.outer_j[G.L6 + m]] == k) walk_free = true;
+                    if (inner_or_pin && !walk_free) continue;
+                    fl *= ((L.s >> k) & 1) ? E.r1[k] : E.r0[k];
+                }
+                const std::complex<double> seg = std::complex<double>(((double)e[0] + (double)e[1]) * fl, ((double)e[2] + (double)e[3]) * fl) *
+                                                 std::complex<double>(w8r[L.K], w8i[L.K]);
+                // direct sum over the 2^I inner branches with the per-branch phase of the packed row
+                std::complex<double> dir(0, 0);
+                double mass = 0;
+                std::vector<uint64_t> u0(P.u_start);
+                uint64_t s0 = P.s0;
+                for (uint32_t k = 0; k < G.nout; ++k) if ((L.c >> k) & 1) {
+                    const uint32_t j = G.outer_j[k];
+                    for (uint32_t w = 0; w < W; ++w) u0[w] ^= E.kernel_u[(size_t)j * W + w];
+                    s0 ^= E.kernel[j];
+                }
+                for (uint32_t ci = 0; ci < (1u << G.I); ++ci) {
+                    std::vector<uint64_t> u(u0);
+                    uint64_t sv = s0;
+                    for (uint32_t p = 0; p < G.I; ++p) if ((ci >> p) & 1) {
+                        const uint32_t j = G.inner_j[p];
+                        for (uint32_t w = 0; w < W; ++w) u[w] ^= E.kernel_u[(size_t)j * W + w];
+                        sv ^= E.kernel[j];
+                    }
+                    if (!survives_row(E, u.data())) throw std::logic_error("branch outside the surviving subspace");
+                    const uint32_t kk = total_phase(E, u.data(), sv);
+                    double mg = 1.0;
+                    for (uint32_t k = 0; k < E.t; ++k) mg *= ((sv >> k) & 1) ? E.r1[k] : E.r0[k];
+                    dir += std::complex<double>(mg * w8r[kk], mg * w8i[kk]);
+                    mass += mg;
+                }
+                *max_err = std::max(*max_err, std::abs(dir - seg) / std::max(mass, 1e-300));
+            }
+        }
         return 0;
     })
 }

@@ -11,6 +11,7 @@ namespace pq {
 struct RankEng {
     std::vector<std::shared_ptr<HostEngine>> host;
     std::vector<std::shared_ptr<Z8Image>> z8;
+    std::vector<std::shared_ptr<SegImage>> seg;
     // Device images are uploaded on first device use, so host-only callers
     // (engine statistics, self checks) never need a GPU.
     mutable std::vector<std::shared_ptr<DevEngine>> dev;
@@ -18,7 +19,7 @@ struct RankEng {
     uint32_t pairs = 0;
     void ensure_device() const {
         std::lock_guard<std::mutex> lk(dev_mu);
-        for (size_t i = dev.size(); i < host.size(); ++i) dev.push_back(dev_engine_upload(host[i], z8[i]));
+        for (size_t i = dev.size(); i < host.size(); ++i) dev.push_back(dev_engine_upload(host[i], z8[i], seg[i]));
     }
     void drop_device() const {
         std::lock_guard<std::mutex> lk(dev_mu);
@@ -64,8 +65,11 @@ struct Sim {
 
 void add_jobs(const RankEng& R, const std::vector<std::pair<uint32_t, uint8_t>>& asg, std::vector<Z8Job>& jobs, std::vector<size_t>& owner,
               size_t qid, std::vector<Prepared>* keep);
-void assign_mid_bits(std::vector<Z8Job>& jobs);
-void run_jobs(std::vector<Z8Job>& jobs, const std::vector<size_t>& owner, size_t nq, bool accurate, std::vector<std::complex<double>>& out);
+void assign_mid_bits(std::vector<Z8Job>& jobs, int walk);
+void run_jobs(std::vector<Z8Job>& jobs, const std::vector<size_t>& owner, size_t nq, int walk, std::vector<std::complex<double>>& out);
+// Walk of a batched form (PQ_FORM_Z8 -> segment walk, PQ_FORM_BRANCH_WALK ->
+// per-branch walk); throws for the plain form.
+int walk_of_form(int form);
 std::complex<double> plain_eval(const RankEng& R, const std::vector<std::pair<uint32_t, uint8_t>>& asg);
 
 extern thread_local std::string g_err;

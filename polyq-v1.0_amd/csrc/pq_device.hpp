@@ -8,15 +8,20 @@
 
 namespace pq {
 
-struct DevEngine;   // engine resident in HBM: plain FP64 image + Z8 walk image
+struct DevEngine;   // engine resident in HBM: plain FP64, branch-walk and segment-table images
 
-// Upload both device images of one engine to the current device.
-std::shared_ptr<DevEngine> dev_engine_upload(std::shared_ptr<const HostEngine> E, std::shared_ptr<const Z8Image> Z);
+// Walk forms of a batch.
+enum { kWalkSeg = 0, kWalkBranch = 1 };
+
+// Upload the device images of one engine to the current device.
+std::shared_ptr<DevEngine> dev_engine_upload(std::shared_ptr<const HostEngine> E, std::shared_ptr<const Z8Image> Z,
+                                             std::shared_ptr<const SegImage> G);
 size_t dev_engine_bytes(const DevEngine& D);
-// Exact device bytes the upload of (E, Z) will allocate (no device needed).
-size_t dev_engine_image_bytes(const HostEngine& E, const Z8Image& Z);
+// Exact device bytes the upload of (E, Z, G) will allocate (no device needed).
+size_t dev_engine_image_bytes(const HostEngine& E, const Z8Image& Z, const SegImage& G);
 const HostEngine& dev_host(const DevEngine& D);
 const Z8Image& dev_z8(const DevEngine& D);
+const SegImage& dev_seg(const DevEngine& D);
 
 // Plain form: a straightforward FP64 port of engine.rs:669-722 (sum_subspace +
 // gray), one query per launch, Gray chunks of 2^L branches per thread.
@@ -27,7 +32,8 @@ std::complex<double> dev_plain_sum(const DevEngine& D, const Prepared& P, double
 // Batched Z8 walk: one persistent launch over all (query, lane-block) items.
 struct Z8Job {
     const DevEngine* eng;
-    Z8QueryHost q;
+    Z8QueryHost q;                // branch walk query data
+    SegQueryHost sq;              // segment walk query data
     std::complex<double> scale;   // applied in FP64 after the cross-item reduction
     uint32_t mid_bits;            // outer Gray bits per lane (autotuned)
 };
@@ -39,11 +45,11 @@ struct Z8Timing {
     uint32_t grid = 0;
 };
 // Results are scale * (EV + omega * OD) per job, FP64.
-void dev_z8_batch(const std::vector<Z8Job>& jobs, std::vector<std::complex<double>>& out, Z8Timing* timing, bool accurate_fold);
+void dev_z8_batch(const std::vector<Z8Job>& jobs, std::vector<std::complex<double>>& out, Z8Timing* timing, int walk);
 // Asynchronous form: a slot holds one batch in flight on its own stream.
 struct Z8Slot;
 std::shared_ptr<Z8Slot> dev_slot_create();
-void dev_z8_launch(Z8Slot& S, const std::vector<Z8Job>& jobs, bool accurate_fold);
+void dev_z8_launch(Z8Slot& S, const std::vector<Z8Job>& jobs, int walk);
 bool dev_z8_ready(Z8Slot& S);
 void dev_z8_wait(Z8Slot& S, std::vector<std::complex<double>>& out, Z8Timing* timing);
 // Mid-bit heuristic: enough items to fill every CU several times, long
@@ -51,11 +57,15 @@ void dev_z8_wait(Z8Slot& S, std::vector<std::complex<double>>& out, Z8Timing* ti
 uint32_t z8_auto_mid_bits(uint32_t d, uint64_t batch_branches);
 void z8_set_tuning(uint32_t offset, uint32_t max_mid);
 void z8_get_tuning(uint32_t* offset, uint32_t* max_mid);
+void seg_set_tuning(uint32_t offset, uint32_t max_mid);
+void seg_get_tuning(uint32_t* offset, uint32_t* max_mid);
+// Lane walk length of an engine's queries for the given form.
+uint32_t dev_mid_bits(const DevEngine& D, int walk);
 
 // Replayable batch: the whole batch (descriptor upload, counter clear, walk,
 // reduction, result download) captured once into a hipGraph and relaunched.
 struct Z8Graph;
-std::shared_ptr<Z8Graph> dev_z8_graph(const std::vector<Z8Job>& jobs, bool accurate_fold);
+std::shared_ptr<Z8Graph> dev_z8_graph(const std::vector<Z8Job>& jobs, int walk);
 void dev_z8_graph_run(Z8Graph& G, std::vector<std::complex<double>>& out, Z8Timing* timing);
 
 // Per-branch dumps for parity tests.
@@ -67,6 +77,9 @@ void dev_dump_s(const DevEngine& D, const Prepared& P, const uint64_t* s, size_t
 //     bits: the integer phase k(c) and the magnitude prod r, indexed by
 //     ((hi << M) + o) * 256 + i.  Phantom entries get k = 255.
 void dev_dump_walk(const DevEngine& D, const Z8QueryHost& q, uint32_t mid_bits, uint8_t* k, float* mag);
+// (c) Integer state (K mod 8, table-basis sign mask, inner pivots) at every
+//     outer point of the segment walk, indexed by (lane << M) + o.
+void dev_dump_seg(const DevEngine& D, const SegQueryHost& q, uint32_t mid_bits, uint8_t* k, uint32_t* hs, uint32_t* pio);
 
 // Device / runtime helpers
 int dev_count();

@@ -219,6 +219,38 @@ Z8Image build_z8_image(const HostEngine& E);
 struct Z8QueryHost { uint32_t K0; uint64_t delta; uint64_t s0; uint32_t pio0; };
 Z8QueryHost z8_query(const HostEngine& E, const Z8Image& Z, const Prepared& P);
 
+// ------------------------------------------------- segment-table image
+// (pq_seg.cpp) Inner sums over 2^I kernel coordinates precomputed per engine
+// as a Walsh-Hadamard table; the device walks only the outer coordinates.
+constexpr uint32_t kSegMaxMid = 10;
+struct SegImage {
+    uint32_t d = 0, t = 0, I = 0, npin = 0, nout = 0, L6 = 0, Mcap = 0, nbytes = 0;
+    std::vector<uint32_t> inner_j, outer_j;   // kernel index of each inner / outer slot
+    std::vector<uint64_t> oq;                 // per outer slot: Q row over outer slots
+    std::vector<uint32_t> ox;                 // per outer slot: cross mask over inner slots, table basis
+    std::vector<uint32_t> opin;               // per outer slot: compact inner pivots touched
+    std::vector<uint32_t> olam;               // per outer slot: query-independent linear coefficient
+    std::vector<uint64_t> on;                 // per outer slot: kernel vector over the t terms
+    std::vector<uint32_t> hsmap;              // inner sign bit p -> table-basis mask
+    std::vector<uint32_t> pin_term;           // term index of each compact inner pivot bit
+    uint64_t lane_terms = 0;
+    std::vector<float> table;                 // 2^(I+npin) x {re hi, re lo, im hi, im lo}, index (pio << I) | h
+    std::vector<float> lt;                    // nbytes x 256 x {hi, lo}: lane-factor byte tables
+    std::vector<float> fmid;                  // M = 0..Mcap: 2^M x {hi, lo} walk magnitudes at 2(2^M - 1)
+    std::vector<float> rmid;                  // Mcap x {r0 hi, r0 lo, r1 hi, r1 lo}
+};
+SegImage build_seg_image(const HostEngine& E);
+void seg_set_bits(uint32_t bits);             // table bits I + npin of engines built afterwards
+uint32_t seg_get_bits();
+struct SegQueryHost { uint32_t K0, hs0, pio0; uint64_t delta, s0; };
+SegQueryHost seg_query(const HostEngine& E, const SegImage& S, const Prepared& P);
+uint32_t seg_mid_bits(const SegImage& S, uint32_t offset, uint32_t max_mid);
+// Integer lane state at walk point o of a lane (host restatement of the
+// device walk, for parity checks): K mod 8, table-basis sign mask, inner
+// pivots, branch vector and the outer coordinates.
+struct SegLane { uint32_t K, hs, pio; uint64_t s, c; };
+SegLane seg_lane_state(const SegImage& S, const SegQueryHost& q, uint32_t M, uint64_t lane, uint32_t o);
+
 // Rank-2 decomposition of rotation pairs (src/rank.rs:44-222).
 constexpr uint32_t kMaxPairs = 14;
 std::vector<std::pair<uint32_t, uint32_t>> pair_candidates(const PathSum& ps, const std::vector<uint32_t>& fixed, uint32_t& n_tclass);

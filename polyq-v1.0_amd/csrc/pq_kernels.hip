@@ -132,20 +132,42 @@ struct Z8Q {                 // one query of a batch (64 B)
     uint64_t delta, s0, item0, nitems;
     uint32_t eng, K0, pio0, M, H, pad0, pad1, pad2;
 };
+// Segment-table image (see pq_seg.cpp): outer slot data, the FP32 hi/lo
+// Walsh-Hadamard table and the magnitude tables.
+struct SegDev {
+    const uint64_t* oq;      // nout: Q rows over outer slots
+    const uint32_t* ox;      // nout: cross masks over inner slots (table basis)
+    const uint32_t* opin;    // nout: compact inner pivots
+    const uint32_t* olam;    // nout: lambda
+    const uint64_t* on;      // nout: kernel vectors over terms
+    const float4* table;     // 2^(I+npin) entries, index (pio << I) | hs
+    const float2* lt;        // nbytes x 256
+    const float* fmid;       // walk magnitudes {hi, lo}, M = 0..Mcap at 2(2^M - 1)
+    const float* rmid;       // Mcap x {r0 hi, r0 lo, r1 hi, r1 lo}
+    uint32_t I, npin, nout, L6, Mcap, nbytes, pad0, pad1;
+};
+struct SegQ {                // one query of a segment batch (64 B; item0 / nitems where Z8Q has them)
+    uint64_t delta, s0, item0, nitems;
+    uint32_t eng, K0, hs0, pio0, M, H, pad0, pad1;
+};
+static_assert(sizeof(Z8Q) == 64 && sizeof(SegQ) == 64, "query records are 64 bytes");
 
 struct DevEngine {
     std::shared_ptr<const HostEngine> host;
     std::shared_ptr<const Z8Image> z8;
+    std::shared_ptr<const SegImage> seg;
     void* blob = nullptr;
     size_t bytes = 0;
     PlainDev plain{};
     Z8Dev z{};
+    SegDev sg{};
     ~DevEngine() { if (blob) (void)hipFree(blob); }
 };
 
 size_t dev_engine_bytes(const DevEngine& D) { return D.bytes; }
 const HostEngine& dev_host(const DevEngine& D) { return *D.host; }
 const Z8Image& dev_z8(const DevEngine& D) { return *D.z8; }
+const SegImage& dev_seg(const DevEngine& D) { return *D.seg; }
 
 namespace {
 struct BlobBuilder {
@@ -175,10 +197,11 @@ uint32_t pick_nw(uint32_t bits) {
 namespace {
 struct BlobLayout {
     size_t o_ku, o_kn, o_soff, o_supp, o_a0, o_a1, o_up, o_dn, o_diag, o_mrow, o_we, o_sq, o_sn, o_sp, o_sl, o_in, o_lt, o_gin;
+    size_t g_oq, g_ox, g_opin, g_olam, g_on, g_tab, g_lt, g_fmid, g_rmid;
     uint32_t NW, NWe;
 };
 // Host image of one engine: the plain FP64 part and the Z8 walk part in one blob.
-BlobLayout build_blob(const HostEngine& E, const Z8Image& Z, BlobBuilder& B) {
+BlobLayout build_blob(const HostEngine& E, const Z8Image& Z, const SegImage& G, BlobBuilder& B) {
     const ULayout& L = E.L;
     BlobLayout O;
     // plain image
@@ -222,25 +245,33 @@ BlobLayout build_blob(const HostEngine& E, const Z8Image& Z, BlobBuilder& B) {
     O.o_sp = B.put(Z.slot_pin.data(), Z.slot_pin.size() * 4); O.o_sl = B.put(lam32.data(), lam32.size() * 4);
     O.o_in = B.put(Z.inner.data(), Z.inner.size() * 4); O.o_lt = B.put(Z.lt.data(), Z.lt.size() * 4);
     O.o_gin = B.put(Z.gin.data(), Z.gin.size() * 4);
+    // segment-table image
+    O.g_oq = B.put(G.oq.data(), G.oq.size() * 8); O.g_ox = B.put(G.ox.data(), G.ox.size() * 4);
+    O.g_opin = B.put(G.opin.data(), G.opin.size() * 4); O.g_olam = B.put(G.olam.data(), G.olam.size() * 4);
+    O.g_on = B.put(G.on.data(), G.on.size() * 8); O.g_tab = B.put(G.table.data(), G.table.size() * 4);
+    O.g_lt = B.put(G.lt.data(), G.lt.size() * 4); O.g_fmid = B.put(G.fmid.data(), G.fmid.size() * 4);
+    O.g_rmid = B.put(G.rmid.data(), G.rmid.size() * 4);
     return O;
 }
 }  // namespace
 
-size_t dev_engine_image_bytes(const HostEngine& E, const Z8Image& Z) {
+size_t dev_engine_image_bytes(const HostEngine& E, const Z8Image& Z, const SegImage& G) {
     BlobBuilder B;
-    build_blob(E, Z, B);
+    build_blob(E, Z, G, B);
     return B.bytes.size();
 }
 
-std::shared_ptr<DevEngine> dev_engine_upload(std::shared_ptr<const HostEngine> Ep, std::shared_ptr<const Z8Image> Zp) {
+std::shared_ptr<DevEngine> dev_engine_upload(std::shared_ptr<const HostEngine> Ep, std::shared_ptr<const Z8Image> Zp,
+                                             std::shared_ptr<const SegImage> Gp) {
     Ctx& C = ctx();
     const HostEngine& E = *Ep;
     const Z8Image& Z = *Zp;
+    const SegImage& G = *Gp;
     const ULayout& L = E.L;
     auto D = std::make_shared<DevEngine>();
-    D->host = Ep; D->z8 = Zp;
+    D->host = Ep; D->z8 = Zp; D->seg = Gp;
     BlobBuilder B;
-    const BlobLayout O = build_blob(E, Z, B);
+    const BlobLayout O = build_blob(E, Z, G, B);
     const uint32_t NW = O.NW, NWe = O.NWe;
 
     std::lock_guard<std::mutex> lk(C.mu);
@@ -259,6 +290,12 @@ std::shared_ptr<DevEngine> dev_engine_upload(std::shared_ptr<const HostEngine> E
     z.slot_q = (const uint64_t*)at(O.o_sq); z.slot_n = (const uint64_t*)at(O.o_sn); z.slot_pin = (const uint32_t*)at(O.o_sp); z.slot_lam = (const uint32_t*)at(O.o_sl);
     z.inner = (const float*)at(O.o_in); z.lt = (const float*)at(O.o_lt); z.gin = (const float*)at(O.o_gin);
     z.nslots = Z.nslots; z.nbytes = Z.nbytes; z.npin = Z.npin; z.imag0 = Z.imag0 ? 1u : 0u;
+    SegDev& g = D->sg;
+    g.oq = (const uint64_t*)at(O.g_oq); g.ox = (const uint32_t*)at(O.g_ox); g.opin = (const uint32_t*)at(O.g_opin);
+    g.olam = (const uint32_t*)at(O.g_olam); g.on = (const uint64_t*)at(O.g_on); g.table = (const float4*)at(O.g_tab);
+    g.lt = (const float2*)at(O.g_lt); g.fmid = (const float*)at(O.g_fmid); g.rmid = (const float*)at(O.g_rmid);
+    g.I = G.I; g.npin = G.npin; g.nout = G.nout; g.L6 = G.L6; g.Mcap = G.Mcap; g.nbytes = G.nbytes;
+    if (G.nbytes > 8) throw std::invalid_argument("more than 64 terms");
     return D;
 }
 
@@ -623,12 +660,216 @@ __global__ __launch_bounds__(256) void k_z8_walk(const Z8Dev* __restrict__ engs,
     }
 }
 
+// ======================================================= segment walk ====
+// Double-float (FP32 hi/lo pair) products and sums.  ph + pl is the product
+// of (ah + al)(bh + bl) up to the dropped al*bl term; the sum keeps the
+// rounding error of the hi part in the lo part.
+__device__ __forceinline__ void df_mul(float ah, float al, float bh, float bl, float& ph, float& pl) {
+    const float p = ah * bh;
+    float e = fmaf(ah, bh, -p);
+    e = fmaf(ah, bl, e);
+    e = fmaf(al, bh, e);
+    ph = p; pl = e;
+}
+__device__ __forceinline__ void df_add(float& sh, float& sl, float xh, float xl) {
+    const float t = sh + xh;
+    const float bp = t - sh;
+    const float e = (sh - (t - bp)) + (xh - bp);
+    sh = t;
+    sl = sl + (xl + e);
+}
+
+// Lane state of the segment walk over the outer coordinates.
+struct SegLaneDev {
+    uint64_t cout, s;
+    uint32_t K, hs, pio;
+    float fh, fl;            // magnitude of the walkable slots taken as lane bits
+};
+// Lane bit b -> outer slot: the lane-low slots first, then past the M walked slots.
+__device__ __forceinline__ uint32_t seg_slot_of_bit(const SegDev& E, uint32_t M, uint32_t b) { return b < E.L6 ? b : E.L6 + M + (b - E.L6); }
+__device__ __forceinline__ void seg_seed(const SegDev& E, const SegQ& Q, uint64_t lane, SegLaneDev& L) {
+    L.cout = 0; L.s = Q.s0; L.K = Q.K0; L.hs = Q.hs0; L.pio = Q.pio0; L.fh = 1.f; L.fl = 0.f;
+    const uint32_t M = Q.M;
+    for (uint32_t b = 0; b < Q.H; ++b) {
+        const uint32_t k = seg_slot_of_bit(E, M, b);
+        const bool on = (lane >> b) & 1;
+        if (k < E.L6 + E.Mcap && k >= E.L6 + M) {
+            const PQ_CONST float* r = cview(E.rmid) + 4 * (k - E.L6);
+            const float r0h = r[0], r0l = r[1], r1h = r[2], r1l = r[3];
+            const float rh = on ? r1h : r0h, rl = on ? r1l : r0l;
+            float ph, pl;
+            df_mul(L.fh, L.fl, rh, rl, ph, pl);
+            L.fh = ph; L.fl = pl;
+        }
+        if (on) {
+            const uint32_t ell = cview(E.olam)[k] + (((uint32_t)(Q.delta >> k) & 1u) << 2);
+            L.K += ell + ((__popcll(cview(E.oq)[k] & L.cout) & 1u) << 2);
+            L.cout |= 1ull << k;
+            L.hs ^= cview(E.ox)[k];
+            L.pio ^= cview(E.opin)[k];
+            L.s ^= cview(E.on)[k];
+        }
+    }
+}
+// Walk step onto a walkable (pivot-free) slot k: only K, hs and cout change.
+__device__ __forceinline__ void seg_flip(const SegDev& E, const SegQ& Q, uint32_t k, SegLaneDev& L) {
+    const uint32_t ell = cview(E.olam)[k] + (((uint32_t)(Q.delta >> k) & 1u) << 2);
+    const uint32_t par = (__popcll(cview(E.oq)[k] & L.cout) & 1u) << 2;
+    const bool was = (L.cout >> k) & 1;
+    L.K += (was ? (8u - ell) : ell) + par;
+    L.cout ^= 1ull << k;
+    L.hs ^= cview(E.ox)[k];
+}
+__device__ __forceinline__ uint32_t seg_index(const SegDev& E, const SegLaneDev& L) {
+    return (L.pio << E.I) | L.hs;
+}
+// Lane factor of the terms outside the table and the walk: byte tables in LDS.
+__device__ __forceinline__ void seg_lane_factor(const float2* s_lt, uint32_t nbytes, uint64_t s, float& fh, float& fl) {
+    for (uint32_t b = 0; b < nbytes; ++b) {
+        const float2 v = s_lt[(b << 8) | (uint32_t)((s >> (8 * b)) & 0xFFu)];
+        float ph, pl;
+        df_mul(fh, fl, v.x, v.y, ph, pl);
+        fh = ph; fl = pl;
+    }
+}
+
+// Persistent work-queue kernel over (query, 256-lane block) items.  A lane
+// seeds its state from its coset index, walks the 2^M outer points of its
+// segment in Gray order (one slot flip each) and adds, per point, the table
+// entry T[pio][hs] rotated by i^(K>>1) and scaled by the walk magnitude, in
+// double-float.  The parity of K is the lane's parity at the first point xor
+// the parity of the walked linear coefficients, which is uniform: points go
+// to one of two accumulators by a uniform branch, and the odd one is turned
+// by omega in the FP64 reduction.  The 64 lanes of a wavefront differ in the
+// lane-low coordinates, whose sign masks the table basis maps to the low six
+// index bits: one wavefront reads one contiguous 1 KB block per point.
+__device__ __forceinline__ void seg_add(float (&acc)[4], float fh, float fl, uint32_t K, const float4& T) {
+    const uint32_t r = (K >> 1) & 3u;
+    const bool sw = r & 1u;
+    const uint32_t sre = ((r ^ (r >> 1)) & 1u) << 31, sim = (r >> 1) << 31;
+    const float ah = __uint_as_float(__float_as_uint(sw ? T.z : T.x) ^ sre);
+    const float al = __uint_as_float(__float_as_uint(sw ? T.w : T.y) ^ sre);
+    const float bh = __uint_as_float(__float_as_uint(sw ? T.x : T.z) ^ sim);
+    const float bl = __uint_as_float(__float_as_uint(sw ? T.y : T.w) ^ sim);
+    float ph, pl, qh, ql;
+    df_mul(fh, fl, ah, al, ph, pl);
+    df_mul(fh, fl, bh, bl, qh, ql);
+    df_add(acc[0], acc[1], ph, pl);
+    df_add(acc[2], acc[3], qh, ql);
+}
+__global__ __launch_bounds__(256) void k_seg_walk(const SegDev* __restrict__ engs, const SegQ* __restrict__ qs, uint32_t nq,
+                                                  uint64_t item_end, unsigned long long* __restrict__ counter, double* __restrict__ partial) {
+    __shared__ float2 s_lt[8 * 256];
+    __shared__ unsigned long long s_item;
+    __shared__ int s_eng;
+    __shared__ float s_red[4][8];
+    const uint32_t tid = threadIdx.x;
+    if (tid == 0) s_eng = -1;
+    for (;;) {
+        __syncthreads();
+        if (tid == 0) s_item = atomicAdd(counter, 1ull);
+        __syncthreads();
+        const uint64_t item = s_item;
+        if (item >= item_end) break;
+        uint32_t lo = 0, hi = nq;
+        while (hi - lo > 1) { const uint32_t mid = (lo + hi) >> 1; if (cview(qs)[mid].item0 <= item) lo = mid; else hi = mid; }
+        const SegQ Q = cload(qs + lo);
+        const SegDev E = cload(engs + Q.eng);
+        if ((int)Q.eng != s_eng) {
+            for (uint32_t k = tid; k < E.nbytes * 256u; k += 256) s_lt[k] = E.lt[k];
+            __syncthreads();
+            if (tid == 0) s_eng = (int)Q.eng;
+        }
+        float a0[4] = {0.f, 0.f, 0.f, 0.f}, a1[4] = {0.f, 0.f, 0.f, 0.f};   // uniform walk parity 0 / 1
+        float ev[4] = {0.f, 0.f, 0.f, 0.f}, od[4] = {0.f, 0.f, 0.f, 0.f};
+        const uint64_t lane = (item - Q.item0) * 256 + tid;
+        if ((lane >> Q.H) == 0) {
+            SegLaneDev L;
+            seg_seed(E, Q, lane, L);
+            float fh = L.fh, fl = L.fl;
+            seg_lane_factor(s_lt, E.nbytes, L.s, fh, fl);
+            const uint32_t pb = L.K & 1u;
+            const PQ_CONST float* fm = cview(E.fmid) + 2 * ((1u << Q.M) - 1u);
+            const PQ_CONST uint32_t* lam = cview(E.olam) + E.L6;
+            const float4* tab = E.table;
+            const uint32_t npts = 1u << Q.M;
+            uint32_t u = 0;
+            if (npts >= 4) {
+                // four points per step: the slot flips inside a block of four
+                // are always walk slots 0, 1, 0; all four table loads issue first
+                const uint32_t p0 = lam[0] & 1u, p1 = lam[1] & 1u;
+                for (uint32_t o0 = 0; o0 < npts; o0 += 4) {
+                    if (o0) { const uint32_t j = (uint32_t)__builtin_ctz(o0); seg_flip(E, Q, E.L6 + j, L); u ^= lam[j] & 1u; }
+                    const uint32_t K0 = L.K, i0 = seg_index(E, L);
+                    seg_flip(E, Q, E.L6, L);
+                    const uint32_t K1 = L.K, i1 = seg_index(E, L);
+                    seg_flip(E, Q, E.L6 + 1, L);
+                    const uint32_t K2 = L.K, i2 = seg_index(E, L);
+                    seg_flip(E, Q, E.L6, L);
+                    const uint32_t K3 = L.K, i3 = seg_index(E, L);
+                    const float4 T0 = tab[i0], T1 = tab[i1], T2 = tab[i2], T3 = tab[i3];
+                    const uint32_t u1 = u ^ p0, u2 = u1 ^ p1, u3 = u2 ^ p0;
+                    if (u) seg_add(a1, fm[2 * o0], fm[2 * o0 + 1], K0, T0); else seg_add(a0, fm[2 * o0], fm[2 * o0 + 1], K0, T0);
+                    if (u1) seg_add(a1, fm[2 * o0 + 2], fm[2 * o0 + 3], K1, T1); else seg_add(a0, fm[2 * o0 + 2], fm[2 * o0 + 3], K1, T1);
+                    if (u2) seg_add(a1, fm[2 * o0 + 4], fm[2 * o0 + 5], K2, T2); else seg_add(a0, fm[2 * o0 + 4], fm[2 * o0 + 5], K2, T2);
+                    if (u3) seg_add(a1, fm[2 * o0 + 6], fm[2 * o0 + 7], K3, T3); else seg_add(a0, fm[2 * o0 + 6], fm[2 * o0 + 7], K3, T3);
+                    u = u3;
+                }
+            } else {
+                for (uint32_t o = 0; o < npts; ++o) {
+                    if (o) { const uint32_t j = (uint32_t)__builtin_ctz(o); seg_flip(E, Q, E.L6 + j, L); u ^= lam[j] & 1u; }
+                    const float4 T = tab[seg_index(E, L)];
+                    if (u) seg_add(a1, fm[2 * o], fm[2 * o + 1], L.K, T); else seg_add(a0, fm[2 * o], fm[2 * o + 1], L.K, T);
+                }
+            }
+            // even / odd K by the lane's base parity; times the lane factor
+#pragma unroll
+            for (int c = 0; c < 4; c += 2) {
+                const float eh = pb ? a1[c] : a0[c], el = pb ? a1[c + 1] : a0[c + 1];
+                const float oh = pb ? a0[c] : a1[c], ol = pb ? a0[c + 1] : a1[c + 1];
+                df_mul(eh, el, fh, fl, ev[c], ev[c + 1]);
+                df_mul(oh, ol, fh, fl, od[c], od[c + 1]);
+            }
+        }
+        // workgroup reduction of the four double-float sums
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) {
+#pragma unroll
+            for (int c = 0; c < 4; c += 2) {
+                const float a = __shfl_xor(ev[c], off), b = __shfl_xor(ev[c + 1], off);
+                const float e = __shfl_xor(od[c], off), f = __shfl_xor(od[c + 1], off);
+                merge2(ev[c], ev[c + 1], a, b);
+                merge2(od[c], od[c + 1], e, f);
+            }
+        }
+        if ((tid & 63) == 0) {
+#pragma unroll
+            for (int c = 0; c < 4; ++c) { s_red[tid >> 6][c] = ev[c]; s_red[tid >> 6][4 + c] = od[c]; }
+        }
+        __syncthreads();
+        if (tid == 0) {
+            double v[4] = {0.0, 0.0, 0.0, 0.0};
+            for (int w = 0; w < 4; ++w) {
+                v[0] += (double)s_red[w][0] + (double)s_red[w][1];
+                v[1] += (double)s_red[w][2] + (double)s_red[w][3];
+                v[2] += (double)s_red[w][4] + (double)s_red[w][5];
+                v[3] += (double)s_red[w][6] + (double)s_red[w][7];
+            }
+            partial[4 * item + 0] = v[0];
+            partial[4 * item + 1] = v[1];
+            partial[4 * item + 2] = v[2];
+            partial[4 * item + 3] = v[3];
+        }
+    }
+}
+
 // FP64 cross-workgroup reduction (fixed order) and global scaling per query.
-__global__ __launch_bounds__(256) void k_z8_reduce(const Z8Q* __restrict__ qs, const double* __restrict__ partial, const double2* __restrict__ scale,
+// Both query records (Z8Q, SegQ) carry item0 and nitems in words 2 and 3.
+__global__ __launch_bounds__(256) void k_z8_reduce(const uint64_t* __restrict__ qwords, const double* __restrict__ partial, const double2* __restrict__ scale,
                                                    const uint32_t* __restrict__ order, double* __restrict__ out) {
     __shared__ double red[4][4];
     const uint32_t qi = blockIdx.x, tid = threadIdx.x;
-    const Z8Q Q = qs[qi];
+    struct { uint64_t item0, nitems; } Q = {qwords[8 * qi + 2], qwords[8 * qi + 3]};
     double v[4] = {0, 0, 0, 0};
     for (uint64_t k = tid; k < Q.nitems; k += 256) {
 #pragma unroll
@@ -677,21 +918,43 @@ uint32_t z8_auto_mid_bits(uint32_t d, uint64_t /*batch_branches*/) {
     if (d <= off) return 0;
     return std::min<uint32_t>(std::min<uint32_t>(d - off, mx), d > 8 ? d - 8 : 0);
 }
+// Segment walk: a lane walks 2^M outer points (each a 2^I-branch table
+// entry) with M = clamp(nout - offset, 0, max), capped by the walkable slots
+// of the engine; again a function of the engine alone.
+static std::atomic<uint32_t> g_seg_offset{12}, g_seg_max{6};
+void seg_set_tuning(uint32_t offset, uint32_t max_mid) {
+    if (max_mid > kSegMaxMid) throw std::invalid_argument("segment walk tuning out of range");
+    g_seg_offset = offset; g_seg_max = max_mid;
+}
+void seg_get_tuning(uint32_t* offset, uint32_t* max_mid) { *offset = g_seg_offset; *max_mid = g_seg_max; }
+uint32_t dev_mid_bits(const DevEngine& D, int walk) {
+    if (walk == kWalkSeg) return seg_mid_bits(*D.seg, g_seg_offset, g_seg_max);
+    return z8_auto_mid_bits(D.z8->d, 0);
+}
 
 namespace {
-constexpr int kVariants = 4;   // bit 1: inner pivot lookups, bit 0: imaginary class pattern
+constexpr int kVariants = 4;   // branch walk: bit 1 inner pivot lookups, bit 0 imaginary class pattern
 int variant_of(const Z8Image& Z) { return (Z.npin ? 2 : 0) | (Z.imag0 ? 1 : 0); }
+// What one batch uploads: engine descriptors, 64-byte query records, scales
+// and the output order, plus the item ranges of the launches.
 struct BatchPlan {
-    std::vector<Z8Dev> engs;
-    std::vector<Z8Q> qs;
+    int walk = kWalkSeg;
+    std::vector<uint8_t> engs, qs;
     std::vector<double2> scale;
     std::vector<uint32_t> order;
     uint64_t total_items = 0, branches = 0;
     uint32_t q_lo[kVariants] = {0, 0, 0, 0}, q_hi[kVariants] = {0, 0, 0, 0};
     uint64_t it_lo[kVariants] = {0, 0, 0, 0}, it_hi[kVariants] = {0, 0, 0, 0};
+    uint32_t nq() const { return (uint32_t)(qs.size() / 64); }
 };
-BatchPlan plan_batch(const std::vector<Z8Job>& jobs) {
+template <class T> void push_rec(std::vector<uint8_t>& v, const T& r) {
+    const size_t o = v.size();
+    v.resize(o + sizeof(T));
+    std::memcpy(v.data() + o, &r, sizeof(T));
+}
+BatchPlan plan_branch(const std::vector<Z8Job>& jobs) {
     BatchPlan B;
+    B.walk = kWalkBranch;
     std::vector<const DevEngine*> seen;
     std::vector<uint32_t> idx(jobs.size());
     std::iota(idx.begin(), idx.end(), 0u);
@@ -708,12 +971,12 @@ BatchPlan plan_batch(const std::vector<Z8Job>& jobs) {
         const Z8Image& Z = *J.eng->z8;
         const int v = variant_of(Z);
         if (v != cur) {
-            B.q_lo[v] = (uint32_t)B.qs.size(); B.it_lo[v] = B.total_items;
+            B.q_lo[v] = B.nq(); B.it_lo[v] = B.total_items;
             cur = v;
         }
         auto it = std::find(seen.begin(), seen.end(), J.eng);
         uint32_t e;
-        if (it == seen.end()) { e = (uint32_t)seen.size(); seen.push_back(J.eng); B.engs.push_back(J.eng->z); }
+        if (it == seen.end()) { e = (uint32_t)seen.size(); seen.push_back(J.eng); push_rec(B.engs, J.eng->z); }
         else e = (uint32_t)(it - seen.begin());
         Z8Q q{};
         q.delta = J.q.delta; q.s0 = J.q.s0; q.eng = e; q.K0 = J.q.K0; q.pio0 = J.q.pio0;
@@ -725,43 +988,107 @@ BatchPlan plan_batch(const std::vector<Z8Job>& jobs) {
         q.nitems = ((1ull << q.H) + 255) / 256;
         B.total_items += q.nitems;
         B.branches += 1ull << Z.d;
-        B.qs.push_back(q);
+        push_rec(B.qs, q);
         B.scale.push_back(make_double2(J.scale.real(), J.scale.imag()));
         B.order.push_back(ji);
-        B.q_hi[v] = (uint32_t)B.qs.size(); B.it_hi[v] = B.total_items;
+        B.q_hi[v] = B.nq(); B.it_hi[v] = B.total_items;
     }
     return B;
 }
-template <int FOLD, bool HASG, bool IMAG>
-uint32_t walk_grid_v(Ctx& C, uint64_t items) {
+BatchPlan plan_seg(const std::vector<Z8Job>& jobs) {
+    BatchPlan B;
+    B.walk = kWalkSeg;
+    std::vector<const DevEngine*> seen;
+    std::vector<uint32_t> idx(jobs.size());
+    std::iota(idx.begin(), idx.end(), 0u);
+    // heavy items first, jobs of one engine together (the LDS byte tables
+    // are reloaded only when a workgroup moves to another engine)
+    std::stable_sort(idx.begin(), idx.end(), [&](uint32_t a, uint32_t b) {
+        const uint32_t ha = jobs[a].eng->seg->nout - jobs[a].mid_bits, hb = jobs[b].eng->seg->nout - jobs[b].mid_bits;
+        if (ha != hb) return ha > hb;
+        return jobs[a].eng < jobs[b].eng;
+    });
+    for (uint32_t ji : idx) {
+        const Z8Job& J = jobs[ji];
+        const SegImage& G = *J.eng->seg;
+        auto it = std::find(seen.begin(), seen.end(), J.eng);
+        uint32_t e;
+        if (it == seen.end()) { e = (uint32_t)seen.size(); seen.push_back(J.eng); push_rec(B.engs, J.eng->sg); }
+        else e = (uint32_t)(it - seen.begin());
+        SegQ q{};
+        q.delta = J.sq.delta; q.s0 = J.sq.s0; q.eng = e; q.K0 = J.sq.K0; q.hs0 = J.sq.hs0; q.pio0 = J.sq.pio0;
+        q.M = std::min(J.mid_bits, std::min(G.Mcap, G.nout - G.L6));
+        q.H = G.nout - q.M;
+        if (q.H > 40) throw std::invalid_argument("branch dimension too large for one batch");
+        q.item0 = B.total_items;
+        q.nitems = ((1ull << q.H) + 255) / 256;
+        B.total_items += q.nitems;
+        B.branches += 1ull << G.d;
+        push_rec(B.qs, q);
+        B.scale.push_back(make_double2(J.scale.real(), J.scale.imag()));
+        B.order.push_back(ji);
+    }
+    return B;
+}
+BatchPlan plan_batch(const std::vector<Z8Job>& jobs, int walk) {
+    if (walk == kWalkSeg) return plan_seg(jobs);
+    if (walk == kWalkBranch) return plan_branch(jobs);
+    throw std::invalid_argument("unknown walk form");
+}
+template <class K>
+uint32_t grid_for(Ctx& C, K kernel, uint64_t items) {
     int per_cu = 0;
-    HIPCHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_z8_walk<FOLD, HASG, IMAG>, 256, 0));
+    HIPCHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, 256, 0));
     const uint64_t g = (uint64_t)std::max(per_cu, 1) * (uint64_t)C.cus;
     return (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(g, items));
 }
-template <int FOLD, bool HASG, bool IMAG>
+template <bool HASG, bool IMAG>
 uint32_t launch_v(Ctx& C, hipStream_t st, const BatchPlan& B, int v, const Z8Dev* de, const Z8Q* dq, unsigned long long* ctr, double* part) {
     const uint64_t items = B.it_hi[v] - B.it_lo[v];
-    const uint32_t grid = walk_grid_v<FOLD, HASG, IMAG>(C, items);
-    hipLaunchKernelGGL((k_z8_walk<FOLD, HASG, IMAG>), dim3(grid), dim3(256), 0, st, de, dq, B.q_lo[v], B.q_hi[v], B.it_lo[v], B.it_hi[v], ctr + v, part);
+    const uint32_t grid = grid_for(C, k_z8_walk<256, HASG, IMAG>, items);
+    hipLaunchKernelGGL((k_z8_walk<256, HASG, IMAG>), dim3(grid), dim3(256), 0, st, de, dq, B.q_lo[v], B.q_hi[v], B.it_lo[v], B.it_hi[v], ctr + v, part);
     HIPCHECK(hipGetLastError());
     return grid;
 }
-// Launch one persistent grid per variant present; returns the summed grid.
-uint32_t launch_walks(Ctx& C, hipStream_t st, bool accurate, const BatchPlan& B, const Z8Dev* de, const Z8Q* dq, unsigned long long* ctr, double* part) {
+// Launch the walk of a staged batch (one persistent grid per branch-walk
+// variant present, or one segment-walk grid); returns the summed grid.
+uint32_t launch_walks(Ctx& C, hipStream_t st, const BatchPlan& B, const void* de, const void* dq, unsigned long long* ctr, double* part) {
+    if (B.walk == kWalkSeg) {
+        if (B.total_items == 0) return 0;
+        const uint32_t grid = grid_for(C, k_seg_walk, B.total_items);
+        hipLaunchKernelGGL(k_seg_walk, dim3(grid), dim3(256), 0, st, (const SegDev*)de, (const SegQ*)dq, B.nq(), B.total_items, ctr, part);
+        HIPCHECK(hipGetLastError());
+        return grid;
+    }
     uint32_t g = 0;
+    const Z8Dev* ze = (const Z8Dev*)de;
+    const Z8Q* zq = (const Z8Q*)dq;
     for (int v = 0; v < kVariants; ++v) {
         if (B.it_hi[v] == B.it_lo[v]) continue;
         const bool hasg = v & 2, imag = v & 1;
-        if (accurate) {
-            if (hasg) g += imag ? launch_v<32, true, true>(C, st, B, v, de, dq, ctr, part) : launch_v<32, true, false>(C, st, B, v, de, dq, ctr, part);
-            else g += imag ? launch_v<32, false, true>(C, st, B, v, de, dq, ctr, part) : launch_v<32, false, false>(C, st, B, v, de, dq, ctr, part);
-        } else {
-            if (hasg) g += imag ? launch_v<256, true, true>(C, st, B, v, de, dq, ctr, part) : launch_v<256, true, false>(C, st, B, v, de, dq, ctr, part);
-            else g += imag ? launch_v<256, false, true>(C, st, B, v, de, dq, ctr, part) : launch_v<256, false, false>(C, st, B, v, de, dq, ctr, part);
-        }
+        if (hasg) g += imag ? launch_v<true, true>(C, st, B, v, ze, zq, ctr, part) : launch_v<true, false>(C, st, B, v, ze, zq, ctr, part);
+        else g += imag ? launch_v<false, true>(C, st, B, v, ze, zq, ctr, part) : launch_v<false, false>(C, st, B, v, ze, zq, ctr, part);
     }
     return g;
+}
+// Offsets of the staged parts in one upload buffer.
+struct StageLayout {
+    size_t oq, os, oo, tot;
+};
+StageLayout stage_layout(const BatchPlan& B) {
+    StageLayout L;
+    const size_t be = B.engs.size(), bq = B.qs.size(), bs = B.scale.size() * sizeof(double2);
+    L.oq = (be + 255) & ~size_t(255);
+    L.os = L.oq + ((bq + 255) & ~size_t(255));
+    L.oo = L.os + ((bs + 255) & ~size_t(255));
+    L.tot = L.oo + B.order.size() * 4;
+    return L;
+}
+void stage_copy(const BatchPlan& B, const StageLayout& L, uint8_t* h) {
+    std::memcpy(h, B.engs.data(), B.engs.size());
+    std::memcpy(h + L.oq, B.qs.data(), B.qs.size());
+    std::memcpy(h + L.os, B.scale.data(), B.scale.size() * sizeof(double2));
+    std::memcpy(h + L.oo, B.order.data(), B.order.size() * 4);
 }
 }  // namespace
 
@@ -794,31 +1121,29 @@ struct Z8Slot {
 
 std::shared_ptr<Z8Slot> dev_slot_create() { (void)ctx(); return std::make_shared<Z8Slot>(); }
 
-void dev_z8_launch(Z8Slot& S, const std::vector<Z8Job>& jobs, bool accurate) {
+void dev_z8_launch(Z8Slot& S, const std::vector<Z8Job>& jobs, int walk) {
     if (S.pending) throw std::logic_error("batch slot already has a batch in flight");
     Ctx& C = ctx();
-    BatchPlan B = plan_batch(jobs);
-    const uint32_t nq = (uint32_t)B.qs.size();
+    BatchPlan B = plan_batch(jobs, walk);
+    const uint32_t nq = B.nq();
     S.nq = nq; S.items = B.total_items; S.branches = B.branches;
     if (nq == 0) return;
     // one pinned staging buffer: engines | queries | scales | order
-    const size_t be = B.engs.size() * sizeof(Z8Dev), bq = nq * sizeof(Z8Q), bs = nq * sizeof(double2), bo = nq * 4;
-    const size_t oq = (be + 255) & ~size_t(255), os = oq + ((bq + 255) & ~size_t(255)), oo = os + ((bs + 255) & ~size_t(255));
-    const size_t tot = oo + bo;
-    uint8_t* h = (uint8_t*)S.h_in.get(tot);
-    std::memcpy(h, B.engs.data(), be); std::memcpy(h + oq, B.qs.data(), bq); std::memcpy(h + os, B.scale.data(), bs); std::memcpy(h + oo, B.order.data(), bo);
-    uint8_t* d = (uint8_t*)S.d_in.get(tot);
+    const StageLayout L = stage_layout(B);
+    uint8_t* h = (uint8_t*)S.h_in.get(L.tot);
+    stage_copy(B, L, h);
+    uint8_t* d = (uint8_t*)S.d_in.get(L.tot);
     unsigned long long* ctr = (unsigned long long*)S.d_ctr.get(64);
-    double* part = (double*)S.d_part.get(B.total_items * 32);
+    double* part = (double*)S.d_part.get(std::max<uint64_t>(B.total_items, 1) * 32);
     double* res = (double*)S.d_res.get(nq * 16);
     double* hres = (double*)S.h_res.get(nq * 16);
     HIPCHECK(hipEventRecord(S.e0, S.stream));
-    HIPCHECK(hipMemcpyAsync(d, h, tot, hipMemcpyHostToDevice, S.stream));
+    HIPCHECK(hipMemcpyAsync(d, h, L.tot, hipMemcpyHostToDevice, S.stream));
     hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(kVariants), 0, S.stream, ctr);
     HIPCHECK(hipEventRecord(S.e1, S.stream));
-    S.grid = launch_walks(C, S.stream, accurate, B, (const Z8Dev*)d, (const Z8Q*)(d + oq), ctr, part);
+    S.grid = launch_walks(C, S.stream, B, d, d + L.oq, ctr, part);
     HIPCHECK(hipEventRecord(S.e2, S.stream));
-    hipLaunchKernelGGL(k_z8_reduce, dim3(nq), dim3(256), 0, S.stream, (const Z8Q*)(d + oq), part, (const double2*)(d + os), (const uint32_t*)(d + oo), res);
+    hipLaunchKernelGGL(k_z8_reduce, dim3(nq), dim3(256), 0, S.stream, (const uint64_t*)(d + L.oq), part, (const double2*)(d + L.os), (const uint32_t*)(d + L.oo), res);
     HIPCHECK(hipGetLastError());
     HIPCHECK(hipMemcpyAsync(hres, res, nq * 16, hipMemcpyDeviceToHost, S.stream));
     HIPCHECK(hipEventRecord(S.e3, S.stream));
@@ -847,14 +1172,14 @@ void dev_z8_wait(Z8Slot& S, std::vector<std::complex<double>>& out, Z8Timing* ti
     }
 }
 
-void dev_z8_batch(const std::vector<Z8Job>& jobs, std::vector<std::complex<double>>& out, Z8Timing* timing, bool accurate) {
+void dev_z8_batch(const std::vector<Z8Job>& jobs, std::vector<std::complex<double>>& out, Z8Timing* timing, int walk) {
     out.assign(jobs.size(), {0, 0});
     if (jobs.empty()) return;
     Ctx& C = ctx();
     static std::shared_ptr<Z8Slot> slot;
     std::lock_guard<std::mutex> lk(C.mu);
     if (!slot) slot = std::make_shared<Z8Slot>();
-    dev_z8_launch(*slot, jobs, accurate);
+    dev_z8_launch(*slot, jobs, walk);
     dev_z8_wait(*slot, out, timing);
 }
 
@@ -877,31 +1202,29 @@ struct Z8Graph {
     }
 };
 
-std::shared_ptr<Z8Graph> dev_z8_graph(const std::vector<Z8Job>& jobs, bool accurate) {
+std::shared_ptr<Z8Graph> dev_z8_graph(const std::vector<Z8Job>& jobs, int walk) {
     Ctx& C = ctx();
     auto G = std::make_shared<Z8Graph>();
-    BatchPlan B = plan_batch(jobs);
-    const uint32_t nq = (uint32_t)B.qs.size();
+    BatchPlan B = plan_batch(jobs, walk);
+    const uint32_t nq = B.nq();
+    if (nq == 0) throw std::invalid_argument("a graph needs at least one nonzero query");
     G->nq = nq; G->items = B.total_items; G->branches = B.branches;
-    const size_t be = B.engs.size() * sizeof(Z8Dev), bq = nq * sizeof(Z8Q), bs = nq * sizeof(double2), bo = nq * 4;
-    const size_t oq = (be + 255) & ~size_t(255), os = oq + ((bq + 255) & ~size_t(255)), oo = os + ((bs + 255) & ~size_t(255));
-    const size_t tot = oo + bo;
+    const StageLayout L = stage_layout(B);
     std::lock_guard<std::mutex> lk(C.mu);
-    HIPCHECK(hipHostMalloc(&G->h_in, tot, hipHostMallocDefault));
+    HIPCHECK(hipHostMalloc(&G->h_in, L.tot, hipHostMallocDefault));
     HIPCHECK(hipHostMalloc(&G->h_out, nq * 16, hipHostMallocDefault));
-    HIPCHECK(hipMalloc(&G->d_in, tot));
+    HIPCHECK(hipMalloc(&G->d_in, L.tot));
     HIPCHECK(hipMalloc(&G->d_part, std::max<uint64_t>(B.total_items * 32, 32)));
     HIPCHECK(hipMalloc(&G->d_res, nq * 16));
     HIPCHECK(hipMalloc(&G->d_ctr, 64));
     HIPCHECK(hipEventCreate(&G->k0)); HIPCHECK(hipEventCreate(&G->k1));
-    uint8_t* h = (uint8_t*)G->h_in;
-    std::memcpy(h, B.engs.data(), be); std::memcpy(h + oq, B.qs.data(), bq); std::memcpy(h + os, B.scale.data(), bs); std::memcpy(h + oo, B.order.data(), bo);
+    stage_copy(B, L, (uint8_t*)G->h_in);
     uint8_t* d = (uint8_t*)G->d_in;
     HIPCHECK(hipStreamBeginCapture(C.stream, hipStreamCaptureModeThreadLocal));
-    HIPCHECK(hipMemcpyAsync(d, h, tot, hipMemcpyHostToDevice, C.stream));
+    HIPCHECK(hipMemcpyAsync(d, G->h_in, L.tot, hipMemcpyHostToDevice, C.stream));
     hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(kVariants), 0, C.stream, (unsigned long long*)G->d_ctr);   // a kernel, not a memset node
-    G->grid = launch_walks(C, C.stream, accurate, B, (const Z8Dev*)d, (const Z8Q*)(d + oq), (unsigned long long*)G->d_ctr, (double*)G->d_part);
-    hipLaunchKernelGGL(k_z8_reduce, dim3(nq), dim3(256), 0, C.stream, (const Z8Q*)(d + oq), (const double*)G->d_part, (const double2*)(d + os), (const uint32_t*)(d + oo), (double*)G->d_res);
+    G->grid = launch_walks(C, C.stream, B, d, d + L.oq, (unsigned long long*)G->d_ctr, (double*)G->d_part);
+    hipLaunchKernelGGL(k_z8_reduce, dim3(nq), dim3(256), 0, C.stream, (const uint64_t*)(d + L.oq), (const double*)G->d_part, (const double2*)(d + L.os), (const uint32_t*)(d + L.oo), (double*)G->d_res);
     HIPCHECK(hipMemcpyAsync(G->h_out, G->d_res, nq * 16, hipMemcpyDeviceToHost, C.stream));
     HIPCHECK(hipStreamEndCapture(C.stream, &G->graph));
     HIPCHECK(hipGraphInstantiate(&G->exec, G->graph, nullptr, nullptr, 0));
@@ -1001,6 +1324,46 @@ __global__ __launch_bounds__(256) void k_dump_walk(Z8Dev E, Z8Q Q, uint8_t* __re
             mout[idx] = F * gi * fabsf(wv);
         }
     }
+}
+
+// Integer state of every outer point of the segment walk for one query:
+// K mod 8, table-basis sign mask and inner pivots, indexed by (lane << M) + o,
+// computed by the same seed and flip code as k_seg_walk.
+__global__ __launch_bounds__(256) void k_dump_seg(SegDev E, SegQ Q, uint8_t* __restrict__ kout, uint32_t* __restrict__ hsout, uint32_t* __restrict__ pout) {
+    const uint64_t lane = (uint64_t)blockIdx.x * 256 + threadIdx.x;
+    if ((lane >> Q.H) != 0) return;
+    SegLaneDev L;
+    seg_seed(E, Q, lane, L);
+    const uint32_t npts = 1u << Q.M;
+    for (uint32_t o = 0; o < npts; ++o) {
+        if (o) seg_flip(E, Q, E.L6 + (uint32_t)__builtin_ctz(o), L);
+        const uint64_t idx = (lane << Q.M) + o;
+        kout[idx] = (uint8_t)(L.K & 7u);
+        hsout[idx] = L.hs;
+        pout[idx] = L.pio;
+    }
+}
+
+void dev_dump_seg(const DevEngine& D, const SegQueryHost& qh, uint32_t mid_bits, uint8_t* k, uint32_t* hs, uint32_t* pio) {
+    Ctx& C = ctx();
+    const SegImage& G = *D.seg;
+    SegQ q{};
+    q.delta = qh.delta; q.s0 = qh.s0; q.K0 = qh.K0; q.hs0 = qh.hs0; q.pio0 = qh.pio0;
+    q.M = std::min(mid_bits, std::min(G.Mcap, G.nout - G.L6));
+    q.H = G.nout - q.M;
+    if (G.nout > 24) throw std::invalid_argument("dump_seg is limited to 2^24 outer points");
+    const uint64_t n = 1ull << G.nout;
+    std::lock_guard<std::mutex> lk(C.mu);
+    uint8_t* dk = (uint8_t*)C.d_misc.get(n + 256);
+    uint32_t* dh = (uint32_t*)C.d_misc2.get(n * 4 + 256);
+    uint32_t* dp = (uint32_t*)C.d_misc3.get(n * 4 + 256);
+    const uint64_t lanes = 1ull << q.H;
+    hipLaunchKernelGGL(k_dump_seg, dim3((uint32_t)((lanes + 255) / 256)), dim3(256), 0, C.stream, D.sg, q, dk, dh, dp);
+    HIPCHECK(hipGetLastError());
+    HIPCHECK(hipMemcpyAsync(k, dk, n, hipMemcpyDeviceToHost, C.stream));
+    HIPCHECK(hipMemcpyAsync(hs, dh, n * 4, hipMemcpyDeviceToHost, C.stream));
+    HIPCHECK(hipMemcpyAsync(pio, dp, n * 4, hipMemcpyDeviceToHost, C.stream));
+    HIPCHECK(hipStreamSynchronize(C.stream));
 }
 
 void dev_dump_walk(const DevEngine& D, const Z8QueryHost& qh, uint32_t mid_bits, uint8_t* k, float* mag) {

@@ -1,0 +1,279 @@
+// Segment-table image of an engine: the data of the default device form.
+//
+// Over kernel coordinates c the per-branch term is
+//     omega^{k(c)} prod_k r_k[s(c)_k],   k(c) = K0 + sum_j c_j l_j + 4 sum_{j<k} c_j c_k Q_jk
+// (the Z8 quadratic form of pq_engine.cpp; s(c) = s0 + sum_j c_j N_j,
+// engine.rs:28-38).  Split the coordinates into I inner ones (c_in) and the
+// outer ones (c_out).  With c_out fixed, the sum over the 2^I inner branches is
+//     omega^{K(c_out)} F(c_out) * T[pio(c_out)][hs(c_out)],
+//     T[pio][h] = sum_{c_in} W(c_in) g(pio + pin(c_in)) (-1)^{c_in . h}
+// where W holds the inner-only phase and magnitudes, g the magnitudes of the
+// pivot terms the inner directions touch, and h the inner sign mask (query
+// bits of the inner coordinates plus their cross terms with c_out).  T is a
+// Walsh-Hadamard transform, the same for every query of the engine: it is
+// computed here once, in FP64, and stored as FP32 hi/lo pairs (16 bytes per
+// entry).  The device then walks only the outer coordinates in Gray order and
+// adds one table entry per outer point, rotated by omega^K: a segment of 2^I
+// branches per lookup.
+//
+// Outer slot order: the (up to) six lane-low slots first (lane bits 0..5 of a
+// wavefront), then pivot-free slots that a lane may walk (their magnitude
+// is a uniform table over the walk index), then the rest.  The inner sign
+// basis is changed so that the six lane-low cross masks span the low bits of
+// h: the 64 lanes of a wavefront then read one contiguous block of the table.
+#include "pq_host.hpp"
+#include <algorithm>
+#include <atomic>
+#include <cmath>
+#include <cstring>
+
+namespace pq {
+
+static std::atomic<uint32_t> g_seg_bits{14};
+void seg_set_bits(uint32_t b) {
+    if (b > 20) throw std::invalid_argument("segment table bits above 20");
+    g_seg_bits = b;
+}
+uint32_t seg_get_bits() { return g_seg_bits; }
+
+namespace {
+inline void split_df(double v, float& hi, float& lo) {
+    hi = (float)v;
+    lo = (float)(v - (double)hi);
+}
+// Inverse of an n x n matrix over F2 given by columns (n <= 32).
+std::vector<uint32_t> f2_inverse_cols(const std::vector<uint32_t>& cols, uint32_t n) {
+    // rows of [A | I]
+    std::vector<uint64_t> rows(n, 0);
+    for (uint32_t c = 0; c < n; ++c)
+        for (uint32_t r = 0; r < n; ++r)
+            if ((cols[c] >> r) & 1) rows[r] |= 1ull << c;
+    for (uint32_t r = 0; r < n; ++r) rows[r] |= 1ull << (32 + r);
+    for (uint32_t c = 0; c < n; ++c) {
+        uint32_t p = c;
+        while (p < n && !((rows[p] >> c) & 1)) ++p;
+        if (p == n) throw std::logic_error("singular lane basis");
+        std::swap(rows[p], rows[c]);
+        for (uint32_t r = 0; r < n; ++r) if (r != c && ((rows[r] >> c) & 1)) rows[r] ^= rows[c];
+    }
+    std::vector<uint32_t> inv(n, 0);   // columns of the inverse
+    for (uint32_t r = 0; r < n; ++r)
+        for (uint32_t c = 0; c < n; ++c)
+            if ((rows[r] >> (32 + c)) & 1) inv[c] |= 1u << r;
+    return inv;
+}
+}  // namespace
+
+SegImage build_seg_image(const HostEngine& E) {
+    SegImage S;
+    const uint32_t d = E.d, t = E.t;
+    if (d > 56) throw std::invalid_argument("branch dimension above 56 is not supported");
+    S.d = d; S.t = t;
+    std::vector<int32_t> piv_index(t, -1);
+    for (uint32_t i = 0; i < E.rk; ++i) piv_index[E.pivot_col[i]] = (int32_t)i;
+    auto pivots_of = [&](uint64_t N) { uint64_t pm = 0; for (uint32_t k = 0; k < t; ++k) if (((N >> k) & 1) && piv_index[k] >= 0) pm |= 1ull << piv_index[k]; return pm; };
+
+    // Inner directions within a table of 2^(I + npin) entries: pivot-free ones
+    // first, then those adding the fewest new pivots.
+    const uint32_t B = g_seg_bits;
+    std::vector<uint8_t> used(d, 0);
+    uint64_t pin_set = 0;
+    for (uint32_t j = 0; j < d && S.inner_j.size() < B; ++j)
+        if (pivots_of(E.kernel[j]) == 0) { S.inner_j.push_back(j); used[j] = 1; }
+    for (;;) {
+        int best = -1; uint32_t bestc = 99;
+        for (uint32_t j = 0; j < d; ++j) {
+            if (used[j]) continue;
+            const uint64_t pj = pivots_of(E.kernel[j]);
+            const uint32_t tot = (uint32_t)S.inner_j.size() + 1 + (uint32_t)__builtin_popcountll(pin_set | pj);
+            const uint32_t c = (uint32_t)__builtin_popcountll(pj & ~pin_set);
+            if (tot <= B && c < bestc) { bestc = c; best = (int)j; }
+        }
+        if (best < 0) break;
+        S.inner_j.push_back((uint32_t)best); used[best] = 1; pin_set |= pivots_of(E.kernel[best]);
+    }
+    S.I = (uint32_t)S.inner_j.size();
+    std::vector<int32_t> cpos(E.rk, -1);
+    for (uint32_t i = 0; i < E.rk; ++i) if ((pin_set >> i) & 1) { cpos[i] = (int32_t)S.pin_term.size(); S.pin_term.push_back(E.pivot_col[i]); }
+    S.npin = (uint32_t)S.pin_term.size();
+    auto pin_of = [&](uint64_t N) { uint32_t pm = 0; const uint64_t pv = pivots_of(N); for (uint32_t i = 0; i < E.rk; ++i) if (((pv >> i) & 1) && cpos[i] >= 0) pm |= 1u << cpos[i]; return pm; };
+
+    // Outer order: lane-low (slots touching pivots first), walkable pivot-free, rest.
+    std::vector<uint32_t> rest_p, rest_f;
+    for (uint32_t j = 0; j < d; ++j) if (!used[j]) (pivots_of(E.kernel[j]) ? rest_p : rest_f).push_back(j);
+    const uint32_t nout = d - S.I;
+    S.nout = nout;
+    S.L6 = std::min<uint32_t>(6, nout);
+    std::vector<uint32_t> order;
+    size_t ip = 0, jf = 0;
+    while (order.size() < S.L6) order.push_back(ip < rest_p.size() ? rest_p[ip++] : rest_f[jf++]);
+    S.Mcap = std::min<uint32_t>((uint32_t)(rest_f.size() - jf), kSegMaxMid);
+    for (uint32_t m = 0; m < S.Mcap; ++m) order.push_back(rest_f[jf++]);
+    while (ip < rest_p.size()) order.push_back(rest_p[ip++]);
+    while (jf < rest_f.size()) order.push_back(rest_f[jf++]);
+    S.outer_j = order;
+
+    // Lane basis of the inner sign mask: the lane-low cross masks go to unit vectors.
+    const uint32_t I = S.I;
+    auto cross_orig = [&](uint32_t j) { uint32_t x = 0; for (uint32_t p = 0; p < I; ++p) if ((E.Q[S.inner_j[p]] >> j) & 1) x |= 1u << p; return x; };
+    std::vector<uint32_t> bcols;     // columns of B = (A^T)^{-1}
+    {
+        std::vector<uint32_t> basis;  // reduced vectors for independence tests
+        auto reduce = [&](uint32_t v) { for (uint32_t b : basis) v = std::min(v, v ^ b); return v; };
+        auto add = [&](uint32_t v) {
+            const uint32_t r = reduce(v);
+            if (!r) return false;
+            basis.push_back(r);
+            std::sort(basis.begin(), basis.end(), std::greater<uint32_t>());
+            return true;
+        };
+        for (uint32_t b = 0; b < S.L6 && I; ++b) { const uint32_t x = cross_orig(S.outer_j[b]); if (add(x)) bcols.push_back(x); }
+        for (uint32_t p = 0; p < I; ++p) if (add(1u << p)) bcols.push_back(1u << p);
+    }
+    std::vector<uint32_t> at = I ? f2_inverse_cols(bcols, I) : std::vector<uint32_t>{};   // columns of A^T
+    S.hsmap.assign(I, 0);
+    for (uint32_t p = 0; p < I; ++p) S.hsmap[p] = at[p];
+    auto to_table_basis = [&](uint32_t x) { uint32_t y = 0; for (uint32_t p = 0; p < I; ++p) if ((x >> p) & 1) y ^= S.hsmap[p]; return y; };
+
+    // Outer slot data
+    S.oq.assign(nout, 0); S.ox.assign(nout, 0); S.opin.assign(nout, 0); S.olam.assign(nout, 0); S.on.assign(nout, 0);
+    for (uint32_t k = 0; k < nout; ++k) {
+        const uint32_t j = S.outer_j[k];
+        for (uint32_t l = 0; l < nout; ++l) if (l != k && ((E.Q[j] >> S.outer_j[l]) & 1)) S.oq[k] |= 1ull << l;
+        S.ox[k] = to_table_basis(cross_orig(j));
+        S.opin[k] = pin_of(E.kernel[j]);
+        S.olam[k] = E.lam[j];
+        S.on[k] = E.kernel[j];
+    }
+
+    // Terms of the lane factor: all but the inner free terms, the inner pivots
+    // and the free terms of the walkable slots.
+    const uint64_t all = t == 64 ? ~0ull : ((1ull << t) - 1);
+    uint64_t excl = 0;
+    for (uint32_t j : S.inner_j) excl |= 1ull << E.free_col[j];
+    for (uint32_t term : S.pin_term) excl |= 1ull << term;
+    for (uint32_t m = 0; m < S.Mcap; ++m) excl |= 1ull << E.free_col[S.outer_j[S.L6 + m]];
+    S.lane_terms = all & ~excl;
+    S.nbytes = (t + 7) / 8;
+    S.lt.assign((size_t)S.nbytes * 256 * 2, 0.f);
+    for (uint32_t b = 0; b < S.nbytes; ++b)
+        for (uint32_t v = 0; v < 256; ++v) {
+            double f = 1.0;
+            for (uint32_t q = 0; q < 8; ++q) {
+                const uint32_t k = 8 * b + q;
+                if (k >= t || !((S.lane_terms >> k) & 1)) continue;
+                f *= ((v >> q) & 1) ? E.r1[k] : E.r0[k];
+            }
+            split_df(f, S.lt[2 * ((size_t)b * 256 + v)], S.lt[2 * ((size_t)b * 256 + v) + 1]);
+        }
+    // Walk magnitudes: for M = 0..Mcap, entry o = prod_{m < M} r_{f_m}[gray(o)_m]
+    S.fmid.assign((size_t)2 * ((2u << S.Mcap) - 1), 0.f);
+    for (uint32_t M = 0; M <= S.Mcap; ++M)
+        for (uint32_t o = 0; o < (1u << M); ++o) {
+            const uint32_t g = o ^ (o >> 1);
+            double f = 1.0;
+            for (uint32_t m = 0; m < M; ++m) { const uint32_t k = E.free_col[S.outer_j[S.L6 + m]]; f *= ((g >> m) & 1) ? E.r1[k] : E.r0[k]; }
+            const size_t at2 = (size_t)2 * ((1u << M) - 1 + o);
+            split_df(f, S.fmid[at2], S.fmid[at2 + 1]);
+        }
+    S.rmid.assign((size_t)4 * std::max<uint32_t>(S.Mcap, 1), 0.f);
+    for (uint32_t m = 0; m < S.Mcap; ++m) {
+        const uint32_t k = E.free_col[S.outer_j[S.L6 + m]];
+        split_df(E.r0[k], S.rmid[4 * m], S.rmid[4 * m + 1]);
+        split_df(E.r1[k], S.rmid[4 * m + 2], S.rmid[4 * m + 3]);
+    }
+
+    // Inner weights W(c) g(pio + pin(c)) and their Walsh-Hadamard transform.
+    const uint32_t NI = 1u << I, NP = 1u << S.npin;
+    std::vector<double> fin(NI, 1.0);
+    std::vector<uint8_t> kin(NI, 0);
+    std::vector<uint32_t> pinc(NI, 0);
+    for (uint32_t p = 0; p < I; ++p) {
+        const uint32_t j = S.inner_j[p], k = E.free_col[j];
+        uint32_t qlow = 0;
+        for (uint32_t q = 0; q < p; ++q) if ((E.Q[j] >> S.inner_j[q]) & 1) qlow |= 1u << q;
+        const uint32_t pm = pin_of(E.kernel[j]);
+        for (uint32_t c = 0; c < (1u << p); ++c) {
+            const uint32_t c1 = c | (1u << p);
+            fin[c1] = fin[c] * E.r1[k];
+            fin[c] *= E.r0[k];
+            kin[c1] = (uint8_t)((kin[c] + E.lam[j] + 4 * (__builtin_popcount(qlow & c) & 1)) & 7);
+            pinc[c1] = pinc[c] ^ pm;
+        }
+    }
+    std::vector<double> gpin(NP, 1.0);
+    for (uint32_t pi = 0; pi < NP; ++pi)
+        for (uint32_t b = 0; b < S.npin; ++b) gpin[pi] *= ((pi >> b) & 1) ? E.r1[S.pin_term[b]] : E.r0[S.pin_term[b]];
+    static const double H = 0.70710678118654752440;
+    const double w8r[8] = {1, H, 0, -H, -1, -H, 0, H}, w8i[8] = {0, H, 1, H, 0, -H, -1, -H};
+    S.table.assign((size_t)NP * NI * 4, 0.f);
+    std::vector<double> re(NI), im(NI);
+    for (uint32_t pio = 0; pio < NP; ++pio) {
+        for (uint32_t c = 0; c < NI; ++c) {
+            const double w = fin[c] * gpin[pio ^ pinc[c]];
+            re[c] = w * w8r[kin[c]];
+            im[c] = w * w8i[kin[c]];
+        }
+        for (uint32_t len = 1; len < NI; len <<= 1)
+            for (uint32_t base = 0; base < NI; base += 2 * len)
+                for (uint32_t x = base; x < base + len; ++x) {
+                    const double ar = re[x], ai = im[x], br = re[x + len], bi = im[x + len];
+                    re[x] = ar + br; im[x] = ai + bi;
+                    re[x + len] = ar - br; im[x + len] = ai - bi;
+                }
+        for (uint32_t hp = 0; hp < NI; ++hp) {
+            uint32_t h = 0;
+            for (uint32_t i = 0; i < I; ++i) if ((hp >> i) & 1) h ^= bcols[i];
+            float* dst = &S.table[((size_t)pio * NI + hp) * 4];
+            split_df(re[h], dst[0], dst[1]);
+            split_df(im[h], dst[2], dst[3]);
+        }
+    }
+    return S;
+}
+
+SegQueryHost seg_query(const HostEngine& E, const SegImage& S, const Prepared& P) {
+    SegQueryHost q;
+    q.K0 = P.K0;
+    q.s0 = P.s0;
+    q.delta = 0;
+    for (uint32_t k = 0; k < S.nout; ++k) {
+        const uint32_t j = S.outer_j[k];
+        if ((((uint32_t)P.ell[j] - E.lam[j]) >> 2) & 1) q.delta |= 1ull << k;
+    }
+    q.hs0 = 0;
+    for (uint32_t p = 0; p < S.I; ++p) {
+        const uint32_t j = S.inner_j[p];
+        if ((((uint32_t)P.ell[j] - E.lam[j]) >> 2) & 1) q.hs0 ^= S.hsmap[p];
+    }
+    q.pio0 = 0;
+    for (uint32_t b = 0; b < S.npin; ++b) if ((P.s0 >> S.pin_term[b]) & 1) q.pio0 |= 1u << b;
+    return q;
+}
+
+uint32_t seg_mid_bits(const SegImage& S, uint32_t offset, uint32_t max_mid) {
+    const uint32_t room = S.nout - S.L6;
+    uint32_t M = S.nout > offset ? S.nout - offset : 0;
+    return std::min(std::min(M, max_mid), std::min(room, S.Mcap));
+}
+
+SegLane seg_lane_state(const SegImage& S, const SegQueryHost& q, uint32_t M, uint64_t lane, uint32_t o) {
+    // the lane's hi coordinates and the Gray point o of its walk
+    uint64_t c = 0;
+    const uint32_t H = S.nout - M;
+    for (uint32_t b = 0; b < H; ++b) if ((lane >> b) & 1) c |= 1ull << (b < S.L6 ? b : S.L6 + M + (b - S.L6));
+    const uint32_t g = o ^ (o >> 1);
+    for (uint32_t m = 0; m < M; ++m) if ((g >> m) & 1) c |= 1ull << (S.L6 + m);
+    SegLane L{q.K0, q.hs0, q.pio0, q.s0, c};
+    for (uint32_t k = 0; k < S.nout; ++k) {
+        if (!((c >> k) & 1)) continue;
+        L.K += S.olam[k] + 4 * (uint32_t)((q.delta >> k) & 1) + 4 * ((uint32_t)__builtin_popcountll(S.oq[k] & c & ((1ull << k) - 1)) & 1);
+        L.hs ^= S.ox[k];
+        L.pio ^= S.opin[k];
+        L.s ^= S.on[k];
+    }
+    L.K &= 7;
+    return L;
+}
+
+}  // namespace pq

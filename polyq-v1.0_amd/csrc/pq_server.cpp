@@ -73,6 +73,7 @@ uint64_t circuit_key(const Circuit& c, int strategy) {
 struct Server {
     uint64_t budget;
     int form;
+    int walk;
     std::atomic<uint64_t> resident{0};
     std::mutex mu;
     std::condition_variable cv_work, cv_ready, cv_done, cv_mem;
@@ -87,7 +88,7 @@ struct Server {
     uint64_t completed = 0, failed = 0, batches = 0, evictions = 0, builds = 0, max_batch = 0, overlapped = 0;
     double busy_kernel_ms = 0;
 
-    Server(uint64_t b, unsigned nworkers, int f) : budget(b), form(f) {
+    Server(uint64_t b, unsigned nworkers, int f) : budget(b), form(f), walk(walk_of_form(f)) {
         for (unsigned i = 0; i < std::max(1u, nworkers); ++i) workers.emplace_back([this] { worker(); });
         dispatcher = std::thread([this] { dispatch(); });
     }
@@ -148,7 +149,7 @@ struct Server {
 
     static size_t estimate_bytes(const RankEng& R) {
         size_t b = 0;
-        for (size_t i = 0; i < R.host.size(); ++i) b += dev_engine_image_bytes(*R.host[i], *R.z8[i]);
+        for (size_t i = 0; i < R.host.size(); ++i) b += dev_engine_image_bytes(*R.host[i], *R.z8[i], *R.seg[i]);
         return b;
     }
 
@@ -205,7 +206,7 @@ struct Server {
                 for (auto& J : r->jobs) {
                     const uint32_t d = dev_z8(*J.eng).d;
                     if (d > 40) throw std::runtime_error("branch dimension above 40: request rejected");
-                    J.mid_bits = z8_auto_mid_bits(d, 0);
+                    J.mid_bits = dev_mid_bits(*J.eng, walk);
                 }
                 std::lock_guard<std::mutex> lk(mu);
                 ready_q.push_back(r);
@@ -251,7 +252,6 @@ struct Server {
         } catch (std::exception&) {
             // no device: requests fail when they reach the dispatcher
         }
-        const bool accurate = form == PQ_FORM_Z8_ACCURATE;
         for (;;) {
             std::vector<std::shared_ptr<Req>> batch;
             int inflight = (order[0] >= 0) + (order[1] >= 0);
@@ -273,7 +273,7 @@ struct Server {
                 std::vector<Z8Job> jobs;
                 for (auto& r : batch) for (auto& J : r->jobs) jobs.push_back(J);
                 if (jobs.empty()) { complete(batch, {}); continue; }
-                dev_z8_launch(*slots[s], jobs, accurate);
+                dev_z8_launch(*slots[s], jobs, walk);
                 held[s] = std::move(batch);
                 if (order[0] < 0) order[0] = s; else { order[1] = s; overlapped++; }
                 std::lock_guard<std::mutex> lk(mu);
@@ -349,7 +349,7 @@ extern "C" {
 
 int pq_server_create(uint64_t hbm_budget_bytes, uint32_t workers, int form, pq_server** out) {
     try {
-        if (form != PQ_FORM_Z8 && form != PQ_FORM_Z8_ACCURATE) throw std::invalid_argument("the server runs the Z8 forms");
+        (void)walk_of_form(form);   // the server runs the batched forms only
         auto h = std::make_unique<pq_server>();
         h->s = std::make_unique<Server>(hbm_budget_bytes, workers, form);
         *out = h.release();

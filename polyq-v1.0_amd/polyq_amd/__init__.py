@@ -42,8 +42,14 @@ lib.pq_sim_marginal_probability.argtypes = [_sim_p, ctypes.c_uint64, _u32p, _u8p
 lib.pq_sim_amplitudes.argtypes = [_sim_p, ctypes.c_size_t, _u64p, _u64p, ctypes.c_int, _f64p]
 lib.pq_batch_amplitudes.argtypes = [ctypes.c_size_t, ctypes.POINTER(_sim_p), _u64p, _u64p, ctypes.c_int, _f64p]
 lib.pq_last_timing.argtypes = [_f64p]
-lib.pq_set_walk_tuning.argtypes = [ctypes.c_uint32, ctypes.c_uint32]
-lib.pq_get_walk_tuning.argtypes = [_u32p]
+lib.pq_set_walk_tuning.argtypes = [ctypes.c_int, ctypes.c_uint32, ctypes.c_uint32]
+lib.pq_get_walk_tuning.argtypes = [ctypes.c_int, _u32p]
+lib.pq_set_segment_bits.argtypes = [ctypes.c_uint32]
+lib.pq_get_segment_bits.restype = ctypes.c_uint32
+lib.pq_sim_seg_info.argtypes = [_sim_p, _u64p]
+lib.pq_sim_seg_dump.argtypes = [_sim_p, ctypes.c_uint64, ctypes.c_uint64, ctypes.c_uint32, ctypes.c_size_t, _u8p, _u32p, _u32p, ctypes.POINTER(ctypes.c_size_t)]
+lib.pq_sim_seg_lane_ref.argtypes = lib.pq_sim_seg_dump.argtypes
+lib.pq_sim_seg_selfcheck.argtypes = [_sim_p, ctypes.c_uint64, ctypes.c_uint64, ctypes.c_uint32, ctypes.c_uint64, _f64p]
 lib.pq_graph_create.argtypes = [ctypes.c_size_t, ctypes.POINTER(_sim_p), _u64p, _u64p, ctypes.c_int, ctypes.POINTER(ctypes.c_void_p)]
 lib.pq_graph_run.argtypes = [ctypes.c_void_p, _f64p, _f64p]
 lib.pq_graph_destroy.argtypes = [ctypes.c_void_p]
@@ -55,7 +61,7 @@ lib.pq_sim_z8_info.argtypes = [_sim_p, _u32p]
 lib.pq_sim_marginal_stats.argtypes = [_sim_p, _u32p, ctypes.c_size_t, _u64p]
 lib.pq_device_info.argtypes = [ctypes.c_char_p, ctypes.c_size_t]
 
-FORMS = {"z8": 0, "z8_accurate": 1, "plain_fp64": 2}
+FORMS = {"z8": 0, "branch_walk": 1, "plain_fp64": 2}
 STRATEGIES = {"auto": 0, "cached": 1, "gpu_auto": 3}
 
 # Gate codes (src/circuit.rs:25-42)
@@ -161,7 +167,9 @@ class Simulator:
     """Simulator::with_options (src/sim.rs:144-148) with the GPU branch evaluator.
 
     strategy: "auto" | "cached"; pairs=p selects Strategy::Pairs(p)
-    (pybind.rs:124-131).  form: "z8" (default), "z8_accurate", "plain_fp64".
+    (pybind.rs:124-131).  form: "z8" (default: segment tables + outer Gray
+    walk), "branch_walk" (per-branch Z8 walk), "plain_fp64" (FP64 port of the
+    reference walk, one query per launch).
     """
 
     def __init__(self, circuit, strategy="auto", pairs=None, form="z8"):
@@ -236,6 +244,34 @@ class Simulator:
         _check(lib.pq_sim_z8_info(self._h, _ptr(o, _u32p)))
         return dict(zip(("d", "inner", "slots", "inner_pivots", "rank_k", "imag_slot0", "ops_per_branch"), (int(v) for v in o)))
 
+    def seg_info(self):
+        o = np.zeros(8, dtype=np.uint64)
+        _check(lib.pq_sim_seg_info(self._h, _ptr(o, _u64p)))
+        return dict(zip(("d", "inner", "inner_pivots", "outer", "lane_low", "walkable", "table_bytes", "mid_bits"), (int(v) for v in o)))
+
+    def _seg_call(self, fn, x, y, mid_bits):
+        total = 1 << self.seg_info()["outer"]
+        k = np.zeros(total, np.uint8); hs = np.zeros(total, np.uint32); pio = np.zeros(total, np.uint32)
+        n = ctypes.c_size_t()
+        rc = _check(fn(self._h, int(x), int(y), int(mid_bits), total, _ptr(k, _u8p), _ptr(hs, _u32p), _ptr(pio, _u32p), ctypes.byref(n)))
+        if rc == 1:
+            return None
+        return k[: n.value], hs[: n.value], pio[: n.value]
+
+    def seg_dump(self, x, y, mid_bits=0):
+        """Device segment walk state (K mod 8, sign mask, inner pivots) at every outer point."""
+        return self._seg_call(lib.pq_sim_seg_dump, x, y, mid_bits)
+
+    def seg_lane_ref(self, x, y, mid_bits=0):
+        """Host restatement of seg_dump."""
+        return self._seg_call(lib.pq_sim_seg_lane_ref, x, y, mid_bits)
+
+    def seg_selfcheck(self, x, y, trials=8, seed=1):
+        """Host-only: max |table segment - direct inner sum| / inner magnitude mass."""
+        err = ctypes.c_double()
+        rc = _check(lib.pq_sim_seg_selfcheck(self._h, int(x), int(y), trials, seed, ctypes.byref(err)))
+        return None if rc == 1 else float(err.value)
+
     def selfcheck(self, x, y, trials=256, seed=1):
         bad = ctypes.c_uint64()
         rc = _check(lib.pq_sim_selfcheck(self._h, int(x), int(y), trials, seed, ctypes.byref(bad)))
@@ -300,35 +336,51 @@ class Graph:
             self._g = None
 
 
-def set_walk_tuning(offset, max_mid):
-    _check(lib.pq_set_walk_tuning(int(offset), int(max_mid)))
+def set_walk_tuning(offset, max_mid, form="z8"):
+    _check(lib.pq_set_walk_tuning(FORMS[form], int(offset), int(max_mid)))
 
 
-def get_walk_tuning():
+def get_walk_tuning(form="z8"):
     o = np.zeros(2, dtype=np.uint32)
-    lib.pq_get_walk_tuning(_ptr(o, _u32p))
+    _check(lib.pq_get_walk_tuning(FORMS[form], _ptr(o, _u32p)))
     return int(o[0]), int(o[1])
 
 
-def autotune(sims, inputs, outputs, offsets=(23, 24, 25, 26, 27, 28, 29), max_mids=(4, 6, 8), repeats=3, cache=True):
-    """Pick the segment split (offset, max_mid) that minimises the walk-kernel
-    time of this batch; apply it and, if cache, store it per device in
-    ~/.cache/polyq_amd/walk_tuning.json.  Returns (offset, max_mid, table)."""
+def set_segment_bits(bits):
+    """Table bits I + npin of the segment tables of simulators created afterwards."""
+    _check(lib.pq_set_segment_bits(int(bits)))
+
+
+def get_segment_bits():
+    return int(lib.pq_get_segment_bits())
+
+
+_TUNE_GRID = {"z8": ((8, 10, 11, 12, 13, 14, 16), (0, 2, 4, 6, 8)),
+              "branch_walk": ((23, 24, 25, 26, 27, 28, 29), (4, 6, 8))}
+
+
+def autotune(sims, inputs, outputs, form="z8", offsets=None, max_mids=None, repeats=3, cache=True):
+    """Pick the lane walk split (offset, max_mid) of a batched form that
+    minimises the walk-kernel time of this batch; apply it and, if cache,
+    store it per device and form in ~/.cache/polyq_amd/walk_tuning.json.
+    Returns (offset, max_mid, {"offset/max_mid": kernel ms})."""
     import json
+    offsets = offsets or _TUNE_GRID[form][0]
+    max_mids = max_mids or _TUNE_GRID[form][1]
     table = {}
     best = None
     for off in offsets:
         for mx in max_mids:
-            set_walk_tuning(off, mx)
+            set_walk_tuning(off, mx, form)
             ms = []
             for _ in range(repeats):
-                batch_amplitudes(sims, inputs, outputs)
+                batch_amplitudes(sims, inputs, outputs, form=form)
                 ms.append(last_timing()["kernel_ms"])
             t = sorted(ms)[len(ms) // 2]
             table["%d/%d" % (off, mx)] = t
             if best is None or t < best[0]:
                 best = (t, off, mx)
-    set_walk_tuning(best[1], best[2])
+    set_walk_tuning(best[1], best[2], form)
     if cache:
         try:
             path = os.path.join(os.path.expanduser("~"), ".cache", "polyq_amd", "walk_tuning.json")
@@ -337,7 +389,7 @@ def autotune(sims, inputs, outputs, offsets=(23, 24, 25, 26, 27, 28, 29), max_mi
             if os.path.exists(path):
                 with open(path) as f:
                     data = json.load(f)
-            data[device_info()] = {"offset": best[1], "max_mid": best[2], "table_ms": table}
+            data["%s|%s" % (device_info(), form)] = {"offset": best[1], "max_mid": best[2], "table_ms": table}
             with open(path, "w") as f:
                 json.dump(data, f)
         except OSError:
@@ -345,17 +397,17 @@ def autotune(sims, inputs, outputs, offsets=(23, 24, 25, 26, 27, 28, 29), max_mi
     return best[1], best[2], table
 
 
-def load_tuning():
-    """Apply a cached autotuner result for this device, if any."""
+def load_tuning(form="z8"):
+    """Apply a cached autotuner result for this device and form, if any."""
     import json
     path = os.path.join(os.path.expanduser("~"), ".cache", "polyq_amd", "walk_tuning.json")
     try:
         with open(path) as f:
-            rec = json.load(f).get(device_info())
+            rec = json.load(f).get("%s|%s" % (device_info(), form))
     except (OSError, ValueError):
         return None
     if rec:
-        set_walk_tuning(rec["offset"], rec["max_mid"])
+        set_walk_tuning(rec["offset"], rec["max_mid"], form)
         return rec["offset"], rec["max_mid"]
     return None
 

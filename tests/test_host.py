@@ -104,3 +104,78 @@ def test_device_calls_fail_loudly_without_a_gpu():
     s = pq.Simulator(c)
     with pytest.raises(pq.PolyqError, match="HIP|device"):
         s.amplitude(0, 1)
+
+
+@pytest.mark.parametrize("bits", [14, 8, 3])
+def test_segment_tables_match_direct_inner_sums(bits):
+    # table entry x lane magnitude == sum of the 2^I inner branches with the
+    # packed-row phase, for random outer points (host only)
+    old = pq.get_segment_bits()
+    pq.set_segment_bits(bits)
+    try:
+        rng = W.Rng(77 + bits)
+        checked = 0
+        for trial in range(40):
+            n = 2 + rng.below(5)
+            if trial % 2:
+                c = W.random_clifford_t(n, 20 + rng.below(40), 2 + rng.below(14), trial % 4 == 1, rng)
+            else:
+                c = W.sandwich_layer(n + 3, 4 + rng.below(10), 3 * (n + 3), n + 3, rng)
+            for strat in ("cached", "auto"):
+                s = pq.Simulator(c, strategy=strat)
+                info = s.seg_info()
+                assert info["inner"] + info["inner_pivots"] <= bits
+                assert info["inner"] + info["outer"] == info["d"]
+                nq = c.num_qubits
+                err = s.seg_selfcheck(rng.below(1 << nq), rng.below(1 << nq), trials=6, seed=trial)
+                if err is not None:
+                    assert err < 1e-12, (trial, strat, info, err)
+                    checked += 1
+        for cfg in (W.config2, W.config3):
+            c, xs, ys = cfg(5) if cfg is W.config2 else cfg(4, n_amps=2)
+            s = pq.Simulator(c, strategy="cached")
+            assert s.seg_selfcheck(xs[0], ys[0], trials=2) < 1e-12
+        assert checked > 30
+    finally:
+        pq.set_segment_bits(old)
+
+
+def test_segment_lane_state_matches_quadratic_form():
+    # the host restatement of the device walk state: K at every outer point
+    # equals K0 + sum l + 4 sum Q over the outer coordinates, and walking
+    # (mid bits > 0) visits the same points as seeding each one directly
+    rng = W.Rng(91)
+    old = pq.get_segment_bits()
+    pq.set_segment_bits(4)
+    try:
+        for trial in range(20):
+            n = 3 + rng.below(4)
+            c = W.sandwich_layer(n, 6 + rng.below(8), 3 * n, n, rng)
+            s = pq.Simulator(c, strategy="cached")
+            info = s.seg_info()
+            if info["outer"] > 14:
+                continue
+            x, y = rng.below(1 << n), rng.below(1 << n)
+            base = s.seg_lane_ref(x, y, 0)
+            if base is None:
+                continue
+            walk = s.seg_lane_ref(x, y, 3)
+            mb = min(3, info["walkable"], info["outer"] - info["lane_low"])
+            # reorder the walked dump to outer-point order: point (lane, o)
+            # covers the coordinates of lane bits and gray(o) on the walk slots
+            K0, H0, P0 = base
+            K1, H1, P1 = walk
+            L6, nout = info["lane_low"], info["outer"]
+            for idx in range(len(K1)):
+                lane, o = idx >> mb, idx & ((1 << mb) - 1)
+                g = o ^ (o >> 1)
+                cc = 0
+                for b in range(nout - mb):
+                    if (lane >> b) & 1:
+                        cc |= 1 << (b if b < L6 else L6 + mb + (b - L6))
+                for m in range(mb):
+                    if (g >> m) & 1:
+                        cc |= 1 << (L6 + m)
+                assert (K1[idx], H1[idx], P1[idx]) == (K0[cc], H0[cc], P0[cc])
+    finally:
+        pq.set_segment_bits(old)
