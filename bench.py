@@ -8,7 +8,8 @@ evaluated in one persistent segment-walk launch (2^14-branch Walsh-Hadamard
 segments built once per circuit, the other 16 coordinates walked in Gray
 order).  `value` counts the 2^d branches each amplitude sums.  Engine images
 are resident in HBM before timing starts.  Reported beside it:
-  * the segment-walk kernel on a saturating launch (4096 amplitudes),
+  * the segment-walk kernel on a saturating launch (4096 amplitudes, lane
+    walk length autotuned for it; --sections saturated runs only this),
   * the per-branch Z8 walk (every branch visited) on the same 16 amplitudes,
   * the plain FP64 form (one query per launch, FP64 port of the reference's
     Gray walk) on the same 16 amplitudes, and per-amplitude error against it,
@@ -127,22 +128,9 @@ def headline(args, pq, W, np):
         "timing": timing, "dev": dev, "tuning": list(pq.get_walk_tuning()),
     }
     # Roofline of k_seg_walk on this launch (16 queries: 2^20 outer points,
-    # too few lanes to fill 256 CUs) and on a saturating launch of the same
-    # engine (4096 queries <0|C|y>, random y).
+    # too few lanes to fill 256 CUs); the saturating launch is measured by
+    # saturated() below.
     res["roofline"] = _seg_roofline(dev, points_per_step, kernel_avg_ms)
-    rng = W.Rng(99)
-    big_x = [rng.below(1 << 40) for _ in range(4096)]
-    big_y = [0] * 4096
-    sim.amplitudes(big_x, big_y)
-    bms, bwall = [], []
-    for _ in range(5):
-        tb = time.perf_counter()
-        sim.amplitudes(big_x, big_y)
-        bwall.append(time.perf_counter() - tb)
-        bms.append(pq.last_timing()["kernel_ms"])
-    res["roofline_saturated"] = _seg_roofline(dev, 4096 * (1 << seg["outer"]), float(np.median(bms)),
-                                              {"queries": 4096, "amplitudes_per_s": 4096 / float(np.median(bwall)),
-                                               "branches_per_s": 4096 * (1 << d) / float(np.median(bwall))})
 
     # The per-branch Z8 walk on the same 16 amplitudes (every branch visited:
     # one sign XOR and one FMA per branch, one more LDS lookup with inner pivots).
@@ -196,6 +184,40 @@ def headline(args, pq, W, np):
             "amplitudes_per_s": amps / done_sec, "max_rel_err_z8_vs_cpu": cpu_err,
         }
     return res
+
+
+def saturated(args, pq, W, np, sim=None):
+    """k_seg_walk at full occupancy: 4096 amplitudes <0|C|y> (random y) of the
+    n=40 workload circuit in one launch, with the lane walk length autotuned
+    for this batch (then restored)."""
+    dev = pq.device_info()
+    if sim is None:
+        c, _, _ = W.config2(SEED)
+        sim = pq.Simulator(c, strategy="cached", form="z8")
+    seg = sim.seg_info()
+    d = seg["d"]
+    rng = W.Rng(99)
+    big_x = [rng.below(1 << 40) for _ in range(4096)]
+    big_y = [0] * 4096
+    default = pq.get_walk_tuning()
+    S = [sim] * 4096
+    if args.saturated_tuning:
+        off, mx = (int(v) for v in args.saturated_tuning.split(","))
+        pq.set_walk_tuning(off, mx)
+    else:
+        off, mx, table = pq.autotune(S, big_x, big_y, repeats=2, cache=False)
+    bms, bwall = [], []
+    for _ in range(args.saturated_reps):
+        tb = time.perf_counter()
+        sim.amplitudes(big_x, big_y)
+        bwall.append(time.perf_counter() - tb)
+        bms.append(pq.last_timing()["kernel_ms"])
+    tuned = sim.seg_info()["mid_bits"]
+    pq.set_walk_tuning(*default)
+    return _seg_roofline(dev, 4096 * (1 << seg["outer"]), float(np.median(bms)),
+                         {"queries": 4096, "walk_tuning": [off, mx], "mid_bits": tuned,
+                          "amplitudes_per_s": 4096 / float(np.median(bwall)),
+                          "branches_per_s": 4096 * (1 << d) / float(np.median(bwall))})
 
 
 def precision(args, pq, W, np):
@@ -371,6 +393,8 @@ def main():
     ap.add_argument("--serve-seconds", type=float, default=60.0)
     ap.add_argument("--serve-rate", type=float, default=200.0)
     ap.add_argument("--batch-circuits", type=int, default=256)
+    ap.add_argument("--saturated-reps", type=int, default=5)
+    ap.add_argument("--saturated-tuning", default=None, help="offset,max_mid for the saturating launch instead of autotuning")
     ap.add_argument("--precision-out", default=None)
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run three 256 MiB device copies (reference traffic for FETCH_SIZE/WRITE_SIZE counters)")
@@ -395,7 +419,12 @@ def main():
     from polyq_amd import workloads as W
 
     sections = set(args.sections.split(","))
+    if sections == {"saturated"}:
+        # profiling aid: only the saturating segment-walk launches
+        print(json.dumps({"roofline_saturated": saturated(args, pq, W, np)}))
+        return
     h = headline(args, pq, W, np)
+    h["roofline_saturated"] = saturated(args, pq, W, np, h["sim"])
     st, d, nq, elapsed = h["st"], h["d"], h["nq"], h["elapsed"]
     value = h["branches_per_step"] * args.steps / elapsed
     amps = nq * args.steps / elapsed

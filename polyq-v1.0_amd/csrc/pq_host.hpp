@@ -238,6 +238,15 @@ struct SegImage {
     std::vector<float> lt;                    // nbytes x 256 x {hi, lo}: lane-factor byte tables
     std::vector<float> fmid;                  // M = 0..Mcap: 2^M x {hi, lo} walk magnitudes at 2(2^M - 1)
     std::vector<float> rmid;                  // Mcap x {r0 hi, r0 lo, r1 hi, r1 lo}
+    // Walk table, o < 2^Mcap, c = gray(o) over the walkable slots: the sign-mask
+    // change X(o) (table basis, bits 0..23) and the engine part of the phase
+    // change Ue(o) = sum c_j lambda_j + 4 sum_{j<j'} c_j c_j' Q_jj' mod 8 (bits 24..26).
+    std::vector<uint32_t> wtab;
+    // The walk's points sorted by engine phase class Ue: for M = 0..Mcap, at
+    // entry offset 2^M - 1, 2^M records {X(o), gray(o), Fmid_M(o) hi, lo}, and
+    // class boundaries pcls[9 M + c] (c = 0..8, relative to the M block).
+    std::vector<uint32_t> pts;
+    std::vector<uint32_t> pcls;
 };
 SegImage build_seg_image(const HostEngine& E);
 void seg_set_bits(uint32_t bits);             // table bits I + npin of engines built afterwards

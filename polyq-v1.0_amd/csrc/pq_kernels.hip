@@ -144,6 +144,9 @@ struct SegDev {
     const float2* lt;        // nbytes x 256
     const float* fmid;       // walk magnitudes {hi, lo}, M = 0..Mcap at 2(2^M - 1)
     const float* rmid;       // Mcap x {r0 hi, r0 lo, r1 hi, r1 lo}
+    const uint32_t* wtab;    // 2^Mcap: sign-mask change | engine phase change << 24
+    const uint32_t* pts;     // walk points sorted by phase class, per M at 4(2^M - 1)
+    const uint32_t* pcls;    // class boundaries, 9 per M
     uint32_t I, npin, nout, L6, Mcap, nbytes, pad0, pad1;
 };
 struct SegQ {                // one query of a segment batch (64 B; item0 / nitems where Z8Q has them)
@@ -197,7 +200,7 @@ uint32_t pick_nw(uint32_t bits) {
 namespace {
 struct BlobLayout {
     size_t o_ku, o_kn, o_soff, o_supp, o_a0, o_a1, o_up, o_dn, o_diag, o_mrow, o_we, o_sq, o_sn, o_sp, o_sl, o_in, o_lt, o_gin;
-    size_t g_oq, g_ox, g_opin, g_olam, g_on, g_tab, g_lt, g_fmid, g_rmid;
+    size_t g_oq, g_ox, g_opin, g_olam, g_on, g_tab, g_lt, g_fmid, g_rmid, g_wtab, g_pts, g_pcls;
     uint32_t NW, NWe;
 };
 // Host image of one engine: the plain FP64 part and the Z8 walk part in one blob.
@@ -251,6 +254,9 @@ BlobLayout build_blob(const HostEngine& E, const Z8Image& Z, const SegImage& G, 
     O.g_on = B.put(G.on.data(), G.on.size() * 8); O.g_tab = B.put(G.table.data(), G.table.size() * 4);
     O.g_lt = B.put(G.lt.data(), G.lt.size() * 4); O.g_fmid = B.put(G.fmid.data(), G.fmid.size() * 4);
     O.g_rmid = B.put(G.rmid.data(), G.rmid.size() * 4);
+    O.g_wtab = B.put(G.wtab.data(), G.wtab.size() * 4);
+    O.g_pts = B.put(G.pts.data(), G.pts.size() * 4);
+    O.g_pcls = B.put(G.pcls.data(), G.pcls.size() * 4);
     return O;
 }
 }  // namespace
@@ -294,6 +300,8 @@ std::shared_ptr<DevEngine> dev_engine_upload(std::shared_ptr<const HostEngine> E
     g.oq = (const uint64_t*)at(O.g_oq); g.ox = (const uint32_t*)at(O.g_ox); g.opin = (const uint32_t*)at(O.g_opin);
     g.olam = (const uint32_t*)at(O.g_olam); g.on = (const uint64_t*)at(O.g_on); g.table = (const float4*)at(O.g_tab);
     g.lt = (const float2*)at(O.g_lt); g.fmid = (const float*)at(O.g_fmid); g.rmid = (const float*)at(O.g_rmid);
+    g.wtab = (const uint32_t*)at(O.g_wtab);
+    g.pts = (const uint32_t*)at(O.g_pts); g.pcls = (const uint32_t*)at(O.g_pcls);
     g.I = G.I; g.npin = G.npin; g.nout = G.nout; g.L6 = G.L6; g.Mcap = G.Mcap; g.nbytes = G.nbytes;
     if (G.nbytes > 8) throw std::invalid_argument("more than 64 terms");
     return D;
@@ -711,19 +719,7 @@ __device__ __forceinline__ void seg_seed(const SegDev& E, const SegQ& Q, uint64_
         }
     }
 }
-// Walk step onto a walkable (pivot-free) slot k: only K, hs and cout change.
-__device__ __forceinline__ void seg_flip(const SegDev& E, const SegQ& Q, uint32_t k, SegLaneDev& L) {
-    const uint32_t ell = cview(E.olam)[k] + (((uint32_t)(Q.delta >> k) & 1u) << 2);
-    const uint32_t par = (__popcll(cview(E.oq)[k] & L.cout) & 1u) << 2;
-    const bool was = (L.cout >> k) & 1;
-    L.K += (was ? (8u - ell) : ell) + par;
-    L.cout ^= 1ull << k;
-    L.hs ^= cview(E.ox)[k];
-}
-__device__ __forceinline__ uint32_t seg_index(const SegDev& E, const SegLaneDev& L) {
-    return (L.pio << E.I) | L.hs;
-}
-// Lane factor of the terms outside the table and the walk: byte tables in LDS.
+// Lane-factor of the terms outside the table and the walk: byte tables in LDS.
 __device__ __forceinline__ void seg_lane_factor(const float2* s_lt, uint32_t nbytes, uint64_t s, float& fh, float& fl) {
     for (uint32_t b = 0; b < nbytes; ++b) {
         const float2 v = s_lt[(b << 8) | (uint32_t)((s >> (8 * b)) & 0xFFu)];
@@ -732,31 +728,59 @@ __device__ __forceinline__ void seg_lane_factor(const float2* s_lt, uint32_t nby
         fh = ph; fl = pl;
     }
 }
+// Over a lane's segment of the M walkable slots, at point o (c_w = gray(o)):
+//   K(o)  = K_seed + Ue(o) + 4 (delta_w . c_w) + 4 (P . c_w)   (mod 8)
+//   hs(o) = hs_seed ^ X(o)
+// with Ue, X the XOR-accumulated Gray-walk data of the engine (one record per
+// point), delta_w the query bits of the walk slots (uniform) and
+// P_j = parity(Q_{walk j} . c_lane) the only lane-dependent part.  The walk
+// slots are pivot-free, so pio and the lane factor are fixed over the segment.
+__device__ __forceinline__ uint32_t seg_lane_parities(const SegDev& E, uint32_t M, uint64_t cout) {
+    uint32_t P = 0;
+    for (uint32_t j = 0; j < M; ++j) P |= (uint32_t)(__popcll(cview(E.oq)[E.L6 + j] & cout) & 1) << j;
+    return P;
+}
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+// A += f * T (double-float real times double-float complex).
+__device__ __forceinline__ void seg_point(float (&A)[4], float fh, float fl, const f32x4& T) {
+    float ph, pl, qh, ql;
+    df_mul(fh, fl, T.x, T.y, ph, pl);
+    df_mul(fh, fl, T.z, T.w, qh, ql);
+    df_add(A[0], A[1], ph, pl);
+    df_add(A[2], A[3], qh, ql);
+}
+// (x + i y) * i^r in double-float, r lane-dependent (selects).
+__device__ __forceinline__ void df_rot(float (&v)[4], uint32_t r) {
+    const bool sw = r & 1u;
+    const uint32_t sre = ((r ^ (r >> 1)) & 1u) << 31, sim = ((r >> 1) & 1u) << 31;
+    const float a0 = sw ? v[2] : v[0], a1 = sw ? v[3] : v[1], b0 = sw ? v[0] : v[2], b1 = sw ? v[1] : v[3];
+    v[0] = __uint_as_float(__float_as_uint(a0) ^ sre); v[1] = __uint_as_float(__float_as_uint(a1) ^ sre);
+    v[2] = __uint_as_float(__float_as_uint(b0) ^ sim); v[3] = __uint_as_float(__float_as_uint(b1) ^ sim);
+}
+// sum_q omega^q A[q] split as EV + omega OD (static rotations by i^m):
+// dst = A[P] + i A[P+2] - A[P+4] - i A[P+6] for P = 0 (EV) and 1 (OD).
+template <int P>
+__device__ __forceinline__ void seg_fold4(const float (&A)[8][4], float (&dst)[4]) {
+    dst[0] = A[P][0]; dst[1] = A[P][1]; dst[2] = A[P][2]; dst[3] = A[P][3];
+    df_add(dst[0], dst[1], -A[P + 2][2], -A[P + 2][3]);   // i (x + i y) = -y + i x
+    df_add(dst[2], dst[3], A[P + 2][0], A[P + 2][1]);
+    df_add(dst[0], dst[1], -A[P + 4][0], -A[P + 4][1]);   // -1
+    df_add(dst[2], dst[3], -A[P + 4][2], -A[P + 4][3]);
+    df_add(dst[0], dst[1], A[P + 6][2], A[P + 6][3]);     // -i (x + i y) = y - i x
+    df_add(dst[2], dst[3], -A[P + 6][0], -A[P + 6][1]);
+}
 
 // Persistent work-queue kernel over (query, 256-lane block) items.  A lane
-// seeds its state from its coset index, walks the 2^M outer points of its
-// segment in Gray order (one slot flip each) and adds, per point, the table
-// entry T[pio][hs] rotated by i^(K>>1) and scaled by the walk magnitude, in
-// double-float.  The parity of K is the lane's parity at the first point xor
-// the parity of the walked linear coefficients, which is uniform: points go
-// to one of two accumulators by a uniform branch, and the odd one is turned
-// by omega in the FP64 reduction.  The 64 lanes of a wavefront differ in the
-// lane-low coordinates, whose sign masks the table basis maps to the low six
-// index bits: one wavefront reads one contiguous 1 KB block per point.
-__device__ __forceinline__ void seg_add(float (&acc)[4], float fh, float fl, uint32_t K, const float4& T) {
-    const uint32_t r = (K >> 1) & 3u;
-    const bool sw = r & 1u;
-    const uint32_t sre = ((r ^ (r >> 1)) & 1u) << 31, sim = (r >> 1) << 31;
-    const float ah = __uint_as_float(__float_as_uint(sw ? T.z : T.x) ^ sre);
-    const float al = __uint_as_float(__float_as_uint(sw ? T.w : T.y) ^ sre);
-    const float bh = __uint_as_float(__float_as_uint(sw ? T.x : T.z) ^ sim);
-    const float bl = __uint_as_float(__float_as_uint(sw ? T.y : T.w) ^ sim);
-    float ph, pl, qh, ql;
-    df_mul(fh, fl, ah, al, ph, pl);
-    df_mul(fh, fl, bh, bl, qh, ql);
-    df_add(acc[0], acc[1], ph, pl);
-    df_add(acc[2], acc[3], qh, ql);
-}
+// seeds its state from its coset index (integer phase, sign mask, pivots,
+// lane factor), then covers the 2^M outer points of its segment.  The point
+// records come grouped by engine phase class Ue, so each group adds into its
+// own accumulator with no per-point branch.  Per point: one XOR for the table
+// index, one scalar-cache read of the record, one global 16-byte read of
+// T[pio][hs], the lane and query sign, and a double-float product and sum.
+// The 64 lanes of a wavefront differ in the lane-low coordinates, whose sign
+// masks the table basis maps to the low six index bits: one wavefront reads
+// one contiguous 1 KB block per point.
+typedef const __attribute__((address_space(1))) f32x4 GFloat4;
 __global__ __launch_bounds__(256) void k_seg_walk(const SegDev* __restrict__ engs, const SegQ* __restrict__ qs, uint32_t nq,
                                                   uint64_t item_end, unsigned long long* __restrict__ counter, double* __restrict__ partial) {
     __shared__ float2 s_lt[8 * 256];
@@ -780,7 +804,6 @@ __global__ __launch_bounds__(256) void k_seg_walk(const SegDev* __restrict__ eng
             __syncthreads();
             if (tid == 0) s_eng = (int)Q.eng;
         }
-        float a0[4] = {0.f, 0.f, 0.f, 0.f}, a1[4] = {0.f, 0.f, 0.f, 0.f};   // uniform walk parity 0 / 1
         float ev[4] = {0.f, 0.f, 0.f, 0.f}, od[4] = {0.f, 0.f, 0.f, 0.f};
         const uint64_t lane = (item - Q.item0) * 256 + tid;
         if ((lane >> Q.H) == 0) {
@@ -788,47 +811,67 @@ __global__ __launch_bounds__(256) void k_seg_walk(const SegDev* __restrict__ eng
             seg_seed(E, Q, lane, L);
             float fh = L.fh, fl = L.fl;
             seg_lane_factor(s_lt, E.nbytes, L.s, fh, fl);
-            const uint32_t pb = L.K & 1u;
-            const PQ_CONST float* fm = cview(E.fmid) + 2 * ((1u << Q.M) - 1u);
-            const PQ_CONST uint32_t* lam = cview(E.olam) + E.L6;
-            const float4* tab = E.table;
-            const uint32_t npts = 1u << Q.M;
-            uint32_t u = 0;
-            if (npts >= 4) {
-                // four points per step: the slot flips inside a block of four
-                // are always walk slots 0, 1, 0; all four table loads issue first
-                const uint32_t p0 = lam[0] & 1u, p1 = lam[1] & 1u;
-                for (uint32_t o0 = 0; o0 < npts; o0 += 4) {
-                    if (o0) { const uint32_t j = (uint32_t)__builtin_ctz(o0); seg_flip(E, Q, E.L6 + j, L); u ^= lam[j] & 1u; }
-                    const uint32_t K0 = L.K, i0 = seg_index(E, L);
-                    seg_flip(E, Q, E.L6, L);
-                    const uint32_t K1 = L.K, i1 = seg_index(E, L);
-                    seg_flip(E, Q, E.L6 + 1, L);
-                    const uint32_t K2 = L.K, i2 = seg_index(E, L);
-                    seg_flip(E, Q, E.L6, L);
-                    const uint32_t K3 = L.K, i3 = seg_index(E, L);
-                    const float4 T0 = tab[i0], T1 = tab[i1], T2 = tab[i2], T3 = tab[i3];
-                    const uint32_t u1 = u ^ p0, u2 = u1 ^ p1, u3 = u2 ^ p0;
-                    if (u) seg_add(a1, fm[2 * o0], fm[2 * o0 + 1], K0, T0); else seg_add(a0, fm[2 * o0], fm[2 * o0 + 1], K0, T0);
-                    if (u1) seg_add(a1, fm[2 * o0 + 2], fm[2 * o0 + 3], K1, T1); else seg_add(a0, fm[2 * o0 + 2], fm[2 * o0 + 3], K1, T1);
-                    if (u2) seg_add(a1, fm[2 * o0 + 4], fm[2 * o0 + 5], K2, T2); else seg_add(a0, fm[2 * o0 + 4], fm[2 * o0 + 5], K2, T2);
-                    if (u3) seg_add(a1, fm[2 * o0 + 6], fm[2 * o0 + 7], K3, T3); else seg_add(a0, fm[2 * o0 + 6], fm[2 * o0 + 7], K3, T3);
-                    u = u3;
+            const uint32_t M = Q.M;
+            const uint32_t P = seg_lane_parities(E, M, L.cout);
+            const uint32_t dw = (uint32_t)(Q.delta >> E.L6) & ((1u << M) - 1u);
+            const uint32_t base = (L.pio << E.I) | L.hs;
+            GFloat4* tab = (GFloat4*)E.table;
+            const PQ_CONST uint32_t* pt = cview(E.pts) + 4 * ((1u << M) - 1u);
+            const PQ_CONST uint32_t* cl = cview(E.pcls) + 9 * M;
+            float A[8][4];
+            // points grouped by their engine phase class c: the class is a
+            // loop constant, so each group adds into its own accumulator
+#pragma unroll
+            for (int c = 0; c < 8; ++c) {
+                A[c][0] = 0.f; A[c][1] = 0.f; A[c][2] = 0.f; A[c][3] = 0.f;
+                const uint32_t p1 = cl[c + 1];
+                uint32_t p = cl[c];
+                // four points per step, all table loads first
+                for (; p + 4 <= p1; p += 4) {
+                    f32x4 T[4];
+                    uint32_t sg[4];
+                    float fh4[4], fl4[4];
+#pragma unroll
+                    for (int k = 0; k < 4; ++k) {
+                        const PQ_CONST uint32_t* e = pt + 4 * (p + k);
+                        const uint32_t X = e[0], g = e[1];
+                        T[k] = tab[base ^ X];
+                        // query sign (uniform) and lane sign of the point
+                        const uint32_t us = (uint32_t)(__popc(dw & g) & 1) << 31;
+                        sg[k] = ((uint32_t)(__popc(P & g) & 1) << 31) ^ us;
+                        fh4[k] = __uint_as_float(e[2]); fl4[k] = __uint_as_float(e[3]);
+                    }
+#pragma unroll
+                    for (int k = 0; k < 4; ++k)
+                        seg_point(A[c], __uint_as_float(__float_as_uint(fh4[k]) ^ sg[k]), __uint_as_float(__float_as_uint(fl4[k]) ^ sg[k]), T[k]);
                 }
-            } else {
-                for (uint32_t o = 0; o < npts; ++o) {
-                    if (o) { const uint32_t j = (uint32_t)__builtin_ctz(o); seg_flip(E, Q, E.L6 + j, L); u ^= lam[j] & 1u; }
-                    const float4 T = tab[seg_index(E, L)];
-                    if (u) seg_add(a1, fm[2 * o], fm[2 * o + 1], L.K, T); else seg_add(a0, fm[2 * o], fm[2 * o + 1], L.K, T);
+                for (; p < p1; ++p) {
+                    const PQ_CONST uint32_t* e = pt + 4 * p;
+                    const uint32_t X = e[0], g = e[1];
+                    const f32x4 T = tab[base ^ X];
+                    const uint32_t sg = (((uint32_t)(__popc(P & g) & 1)) ^ ((uint32_t)(__popc(dw & g) & 1))) << 31;
+                    seg_point(A[c], __uint_as_float(e[2] ^ sg), __uint_as_float(e[3] ^ sg), T);
                 }
             }
-            // even / odd K by the lane's base parity; times the lane factor
+            // lane total = omega^K_seed F (EVu + omega ODu)
+            float eu[4], ou[4];
+            seg_fold4<0>(A, eu);
+            seg_fold4<1>(A, ou);
+            const uint32_t Ks = L.K & 7u;
+            float e2[4], o2[4];
+            if (Ks & 1u) {   // omega^(2a+1) (EVu + omega ODu) = i^a (i ODu + omega EVu)
+                e2[0] = -ou[2]; e2[1] = -ou[3]; e2[2] = ou[0]; e2[3] = ou[1];
+                o2[0] = eu[0]; o2[1] = eu[1]; o2[2] = eu[2]; o2[3] = eu[3];
+            } else {
+#pragma unroll
+                for (int c = 0; c < 4; ++c) { e2[c] = eu[c]; o2[c] = ou[c]; }
+            }
+            df_rot(e2, Ks >> 1);
+            df_rot(o2, Ks >> 1);
 #pragma unroll
             for (int c = 0; c < 4; c += 2) {
-                const float eh = pb ? a1[c] : a0[c], el = pb ? a1[c + 1] : a0[c + 1];
-                const float oh = pb ? a0[c] : a1[c], ol = pb ? a0[c + 1] : a1[c + 1];
-                df_mul(eh, el, fh, fl, ev[c], ev[c + 1]);
-                df_mul(oh, ol, fh, fl, od[c], od[c + 1]);
+                df_mul(e2[c], e2[c + 1], fh, fl, ev[c], ev[c + 1]);
+                df_mul(o2[c], o2[c + 1], fh, fl, od[c], od[c + 1]);
             }
         }
         // workgroup reduction of the four double-float sums
@@ -1329,18 +1372,30 @@ __global__ __launch_bounds__(256) void k_dump_walk(Z8Dev E, Z8Q Q, uint8_t* __re
 // Integer state of every outer point of the segment walk for one query:
 // K mod 8, table-basis sign mask and inner pivots, indexed by (lane << M) + o,
 // computed by the same seed and flip code as k_seg_walk.
+// Integer state (K mod 8, sign mask, inner pivots) of every outer point of
+// one query, computed from the same seed and class-sorted point records as
+// k_seg_walk, written at (lane << M) + o.
 __global__ __launch_bounds__(256) void k_dump_seg(SegDev E, SegQ Q, uint8_t* __restrict__ kout, uint32_t* __restrict__ hsout, uint32_t* __restrict__ pout) {
     const uint64_t lane = (uint64_t)blockIdx.x * 256 + threadIdx.x;
     if ((lane >> Q.H) != 0) return;
     SegLaneDev L;
     seg_seed(E, Q, lane, L);
-    const uint32_t npts = 1u << Q.M;
-    for (uint32_t o = 0; o < npts; ++o) {
-        if (o) seg_flip(E, Q, E.L6 + (uint32_t)__builtin_ctz(o), L);
-        const uint64_t idx = (lane << Q.M) + o;
-        kout[idx] = (uint8_t)(L.K & 7u);
-        hsout[idx] = L.hs;
-        pout[idx] = L.pio;
+    const uint32_t M = Q.M;
+    const uint32_t P = seg_lane_parities(E, M, L.cout);
+    const uint32_t dw = (uint32_t)(Q.delta >> E.L6) & ((1u << M) - 1u);
+    const uint32_t* pt = E.pts + 4 * ((1u << M) - 1u);
+    const uint32_t* cl = E.pcls + 9 * M;
+    for (uint32_t c = 0; c < 8; ++c) {
+        for (uint32_t p = cl[c]; p < cl[c + 1]; ++p) {
+            const uint32_t X = pt[4 * p], g = pt[4 * p + 1];
+            uint32_t o = g;
+            for (uint32_t sh = 1; sh < 32; sh <<= 1) o ^= o >> sh;   // inverse Gray code
+            const uint32_t lp = (uint32_t)((__popc(P & g) ^ __popc(dw & g)) & 1);
+            const uint64_t idx = (lane << M) + o;
+            kout[idx] = (uint8_t)((L.K + c + 4u * lp) & 7u);
+            hsout[idx] = L.hs ^ X;
+            pout[idx] = L.pio;
+        }
     }
 }
 

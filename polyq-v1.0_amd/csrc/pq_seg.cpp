@@ -176,6 +176,35 @@ SegImage build_seg_image(const HostEngine& E) {
             const size_t at2 = (size_t)2 * ((1u << M) - 1 + o);
             split_df(f, S.fmid[at2], S.fmid[at2 + 1]);
         }
+    S.wtab.assign((size_t)1 << S.Mcap, 0);
+    for (uint32_t o = 0; o < (1u << S.Mcap); ++o) {
+        const uint32_t g = o ^ (o >> 1);
+        uint32_t X = 0, U = 0;
+        for (uint32_t m = 0; m < S.Mcap; ++m) {
+            if (!((g >> m) & 1)) continue;
+            const uint32_t k = S.L6 + m;
+            X ^= S.ox[k];
+            U += S.olam[k] + 4 * ((uint32_t)__builtin_popcountll(S.oq[k] & ((uint64_t)(g & ((1u << m) - 1)) << S.L6)) & 1);
+        }
+        S.wtab[o] = X | ((U & 7u) << 24);
+    }
+    S.pts.assign((size_t)4 * ((2u << S.Mcap) - 1), 0);
+    S.pcls.assign((size_t)9 * (S.Mcap + 1), 0);
+    for (uint32_t M = 0; M <= S.Mcap; ++M) {
+        std::vector<uint32_t> idx(1u << M);
+        for (uint32_t o = 0; o < (1u << M); ++o) idx[o] = o;
+        std::stable_sort(idx.begin(), idx.end(), [&](uint32_t a, uint32_t b) { return (S.wtab[a] >> 24) < (S.wtab[b] >> 24); });
+        const size_t base = (size_t)4 * ((1u << M) - 1);
+        for (uint32_t p = 0; p < (1u << M); ++p) {
+            const uint32_t o = idx[p];
+            uint32_t* e = &S.pts[base + 4 * (size_t)p];
+            e[0] = S.wtab[o] & 0xFFFFFFu;
+            e[1] = o ^ (o >> 1);
+            std::memcpy(&e[2], &S.fmid[2 * ((1u << M) - 1 + o)], 8);
+            S.pcls[9 * M + 1 + (S.wtab[o] >> 24)]++;
+        }
+        for (uint32_t c = 1; c <= 8; ++c) S.pcls[9 * M + c] += S.pcls[9 * M + c - 1];
+    }
     S.rmid.assign((size_t)4 * std::max<uint32_t>(S.Mcap, 1), 0.f);
     for (uint32_t m = 0; m < S.Mcap; ++m) {
         const uint32_t k = E.free_col[S.outer_j[S.L6 + m]];
