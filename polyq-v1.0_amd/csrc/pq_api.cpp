@@ -4,6 +4,7 @@
 #include "pq_host.hpp"
 #include "pq_device.hpp"
 #include "pq_api_internal.hpp"
+#include "pq_pool.hpp"
 #include "../../include/polyq_amd.h"
 #include <mutex>
 #include <map>
@@ -54,23 +55,64 @@ std::shared_ptr<RankEng> build_rank(const PathSum& ps, const std::vector<uint32_
     return R;
 }
 
-// Append the device jobs of one (engine set, assignment) query; returns false
-// if every branch engine vanishes for it.
-void add_jobs(const RankEng& R, const std::vector<std::pair<uint32_t, uint8_t>>& asg, std::vector<Z8Job>& jobs, std::vector<size_t>& owner,
-              size_t qid, std::vector<Prepared>* keep) {
-    R.ensure_device();
+// Append the device jobs of one (engine set, assignment) query: one job per
+// branch engine whose prologue does not vanish.  Only the query record of the
+// given walk form is built.  The engine set must already be on the device.
+void add_query_jobs(const RankEng& R, const std::vector<std::pair<uint32_t, uint8_t>>& asg, int walk, std::vector<Z8Job>& jobs,
+                    std::vector<size_t>& owner, size_t qid) {
+    thread_local Prepared P;
     for (size_t b = 0; b < R.host.size(); ++b) {
-        Prepared P;
         if (!prepare_assignment(*R.host[b], asg, P)) continue;
         Z8Job J;
         J.eng = R.dev[b].get();
-        J.q = z8_query(*R.host[b], *R.z8[b], P);
-        J.sq = seg_query(*R.host[b], *R.seg[b], P);
+        if (walk == kWalkSeg) J.sq = seg_query(*R.host[b], *R.seg[b], P);
+        else J.q = z8_query(*R.host[b], *R.z8[b], P);
         J.scale = P.scale;
         J.mid_bits = 0;
         jobs.push_back(J);
         owner.push_back(qid);
-        if (keep) keep->push_back(std::move(P));
+    }
+}
+
+void add_jobs(const RankEng& R, const std::vector<std::pair<uint32_t, uint8_t>>& asg, int walk, std::vector<Z8Job>& jobs,
+              std::vector<size_t>& owner, size_t qid) {
+    R.ensure_device();
+    add_query_jobs(R, asg, walk, jobs, owner, qid);
+}
+
+// Jobs of n queries, query i on the engine set rank_of(i) with the boundary
+// assignment assign(i, asg) (false: the amplitude is exactly 0).  Large
+// batches run their prologues on the host thread pool; the jobs come out in
+// query order either way.
+void build_jobs(size_t n, int walk, const std::function<const RankEng*(size_t)>& rank_of,
+                const std::function<bool(size_t, std::vector<std::pair<uint32_t, uint8_t>>&)>& assign, std::vector<Z8Job>& jobs,
+                std::vector<size_t>& owner) {
+    const RankEng* last = nullptr;
+    for (size_t i = 0; i < n; ++i) {
+        const RankEng* R = rank_of(i);
+        if (R != last) { R->ensure_device(); last = R; }
+    }
+    ChunkPool& pool = ChunkPool::get();
+    std::vector<std::vector<Z8Job>> cj(pool.threads());
+    std::vector<std::vector<size_t>> co(pool.threads());
+    std::vector<std::string> err(pool.threads());
+    const unsigned nch = pool.run(n, 64, [&](size_t lo, size_t hi, unsigned c) {
+        try {
+            std::vector<std::pair<uint32_t, uint8_t>> asg;
+            for (size_t i = lo; i < hi; ++i)
+                if (assign(i, asg)) add_query_jobs(*rank_of(i), asg, walk, cj[c], co[c], i);
+        } catch (std::exception& e) {
+            err[c] = e.what();
+        }
+    });
+    for (unsigned c = 0; c < nch; ++c) if (!err[c].empty()) throw std::invalid_argument(err[c]);
+    if (nch == 1) { jobs.swap(cj[0]); owner.swap(co[0]); return; }
+    size_t tot = 0;
+    for (unsigned c = 0; c < nch; ++c) tot += cj[c].size();
+    jobs.reserve(tot); owner.reserve(tot);
+    for (unsigned c = 0; c < nch; ++c) {
+        jobs.insert(jobs.end(), cj[c].begin(), cj[c].end());
+        owner.insert(owner.end(), co[c].begin(), co[c].end());
     }
 }
 
@@ -219,11 +261,11 @@ int pq_sim_amplitudes(pq_sim* h, size_t n, const uint64_t* xs, const uint64_t* y
             const int walk = walk_of_form(form);
             std::vector<Z8Job> jobs;
             std::vector<size_t> owner;
-            for (size_t i = 0; i < n; ++i) {
+            const RankEng* R = h->s.eng.get();
+            build_jobs(n, walk, [&](size_t) { return R; }, [&](size_t i, std::vector<std::pair<uint32_t, uint8_t>>& a) {
                 if (!qubits_ok(h, xs[i]) || !qubits_ok(h, ys[i])) throw std::invalid_argument("basis index has bits beyond the qubit count");
-                if (!h->s.assignment(xs[i], ys[i], asg)) continue;   // contradictory boundary: exactly 0
-                add_jobs(*h->s.eng, asg, jobs, owner, i, nullptr);
-            }
+                return h->s.assignment(xs[i], ys[i], a);   // false: contradictory boundary, exactly 0
+            }, jobs, owner);
             run_jobs(jobs, owner, n, walk, res);
         }
         for (size_t i = 0; i < n; ++i) { out[2 * i] = res[i].real(); out[2 * i + 1] = res[i].imag(); }
@@ -257,9 +299,10 @@ int pq_sim_marginal_probability(pq_sim* h, uint64_t x, const uint32_t* qubits, c
         if (form == PQ_FORM_PLAIN_FP64) v = plain_eval(R, a);
         else {
             std::vector<Z8Job> jobs; std::vector<size_t> owner;
-            add_jobs(R, a, jobs, owner, 0, nullptr);
+            const int walk = walk_of_form(form);
+            add_jobs(R, a, walk, jobs, owner, 0);
             std::vector<std::complex<double>> res;
-            run_jobs(jobs, owner, 1, walk_of_form(form), res);
+            run_jobs(jobs, owner, 1, walk, res);
             v = res[0];
         }
         *out = std::max(v.real(), 0.0);   // sim.rs:230
@@ -271,12 +314,11 @@ int pq_batch_amplitudes(size_t n, pq_sim* const* sims, const uint64_t* xs, const
     PQ_TRY({
         const int walk = walk_of_form(form);
         std::vector<Z8Job> jobs; std::vector<size_t> owner;
-        std::vector<std::pair<uint32_t, uint8_t>> asg;
-        for (size_t i = 0; i < n; ++i) {
-            if (!qubits_ok(sims[i], xs[i]) || !qubits_ok(sims[i], ys[i])) throw std::invalid_argument("basis index has bits beyond the qubit count");
-            if (!sims[i]->s.assignment(xs[i], ys[i], asg)) continue;
-            add_jobs(*sims[i]->s.eng, asg, jobs, owner, i, nullptr);
-        }
+        build_jobs(n, walk, [&](size_t i) { return (const RankEng*)sims[i]->s.eng.get(); },
+                   [&](size_t i, std::vector<std::pair<uint32_t, uint8_t>>& a) {
+                       if (!qubits_ok(sims[i], xs[i]) || !qubits_ok(sims[i], ys[i])) throw std::invalid_argument("basis index has bits beyond the qubit count");
+                       return sims[i]->s.assignment(xs[i], ys[i], a);
+                   }, jobs, owner);
         std::vector<std::complex<double>> res;
         run_jobs(jobs, owner, n, walk, res);
         for (size_t i = 0; i < n; ++i) { out[2 * i] = res[i].real(); out[2 * i + 1] = res[i].imag(); }
@@ -308,12 +350,12 @@ int pq_graph_create(size_t n, pq_sim* const* sims, const uint64_t* xs, const uin
         const int walk = walk_of_form(form);
         auto g = std::make_unique<pq_graph>();
         std::vector<Z8Job> jobs;
-        std::vector<std::pair<uint32_t, uint8_t>> asg;
-        for (size_t i = 0; i < n; ++i) {
-            if (!sims[i]->s.assignment(xs[i], ys[i], asg)) continue;
-            add_jobs(*sims[i]->s.eng, asg, jobs, g->owner, i, nullptr);
-            g->keep.push_back(sims[i]->s.eng);
-        }
+        for (size_t i = 0; i < n; ++i) g->keep.push_back(sims[i]->s.eng);
+        build_jobs(n, walk, [&](size_t i) { return (const RankEng*)sims[i]->s.eng.get(); },
+                   [&](size_t i, std::vector<std::pair<uint32_t, uint8_t>>& a) {
+                       if (!qubits_ok(sims[i], xs[i]) || !qubits_ok(sims[i], ys[i])) throw std::invalid_argument("basis index has bits beyond the qubit count");
+                       return sims[i]->s.assignment(xs[i], ys[i], a);
+                   }, jobs, g->owner);
         if (jobs.empty()) throw std::invalid_argument("every query of the graph is identically zero");
         assign_mid_bits(jobs, walk);
         g->nq = n;

@@ -2,6 +2,7 @@
 #pragma once
 #include "pq_host.hpp"
 #include "pq_device.hpp"
+#include <functional>
 #include <map>
 #include <mutex>
 
@@ -63,8 +64,8 @@ struct Sim {
     bool marginal_assignment(uint64_t x, const uint32_t* qubits, const uint8_t* outcomes, size_t k, std::vector<std::pair<uint32_t, uint8_t>>& a);
 };
 
-void add_jobs(const RankEng& R, const std::vector<std::pair<uint32_t, uint8_t>>& asg, std::vector<Z8Job>& jobs, std::vector<size_t>& owner,
-              size_t qid, std::vector<Prepared>* keep);
+void add_jobs(const RankEng& R, const std::vector<std::pair<uint32_t, uint8_t>>& asg, int walk, std::vector<Z8Job>& jobs,
+              std::vector<size_t>& owner, size_t qid);
 void assign_mid_bits(std::vector<Z8Job>& jobs, int walk);
 void run_jobs(std::vector<Z8Job>& jobs, const std::vector<size_t>& owner, size_t nq, int walk, std::vector<std::complex<double>>& out);
 // Walk of a batched form (PQ_FORM_Z8 -> segment walk, PQ_FORM_BRANCH_WALK ->

@@ -340,18 +340,22 @@ bool prepare(const HostEngine& E, const uint8_t* values, Prepared& P) {
     uint32_t z = 0;
     Phase th = E.global;
     P.u0.assign(W, 0);
-    std::vector<uint32_t> ones;
-    for (uint32_t i = 0; i < E.nf; ++i) if (values[i]) ones.push_back(i);
-    for (size_t a = 0; a < ones.size(); ++a) {
-        uint32_t f = ones[a];
-        z += E.fixed_v[f]; th += E.fixed_theta[f];
-        for (size_t b2 = a + 1; b2 < ones.size(); ++b2) if (E.b_fixed[f].get(ones[b2])) z += 2;
+    // Z4 value of the fixed assignment: linear part plus 2 per adjacent pair of
+    // ones.  b_fixed is symmetric with a zero diagonal, so twice the pair count
+    // is sum over ones a of |B_a & ones|.
+    thread_local Bits ones;
+    if (ones.n != E.nf) ones = Bits(E.nf); else ones.clear();
+    for (uint32_t i = 0; i < E.nf; ++i) if (values[i]) ones.set(i);
+    ones.each([&](uint32_t f) {
+        z += E.fixed_v[f] + E.b_fixed[f].and_popc(ones);
+        th += E.fixed_theta[f];
         const uint64_t* c = &E.ucols[(size_t)f * W];
         for (uint32_t k = 0; k < W; ++k) P.u0[k] ^= c[k];
-    }
+    });
     P.phase = th + z4(z);
     // kappa = w_e + u0_e; particular solution of K s = kappa (engine.rs:188-201)
-    Bits kappa(L.neven);
+    thread_local Bits kappa;
+    if (kappa.n != L.neven) kappa = Bits(L.neven);
     for (uint32_t k = 0; k < L.We; ++k) kappa.w[k] = E.diag[L.off_e() + k] ^ P.u0[L.off_e() + k];
     for (auto& r : E.consistency) if (r.dot(kappa)) return false;
     uint64_t s0 = 0;
@@ -379,7 +383,8 @@ bool prepare(const HostEngine& E, const uint8_t* values, Prepared& P) {
 }
 
 bool prepare_assignment(const HostEngine& E, const std::vector<std::pair<uint32_t, uint8_t>>& asg, Prepared& P) {
-    std::vector<uint8_t> vals(E.nf, 0);
+    thread_local std::vector<uint8_t> vals;
+    vals.assign(E.nf, 0);
     size_t seen = 0;
     for (auto& a : asg) {
         int32_t p = a.first < E.fixed_pos.size() ? E.fixed_pos[a.first] : -1;

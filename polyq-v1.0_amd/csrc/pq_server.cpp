@@ -198,10 +198,10 @@ struct Server {
                 if (r->kind == 2) {
                     const RankEng& R = S.marginal_engine(r->mq);
                     make_resident(r->entry, R);
-                    if (S.marginal_assignment(r->x, r->mq.data(), r->mo.data(), r->mq.size(), asg)) add_jobs(R, asg, r->jobs, owner, 0, nullptr);
+                    if (S.marginal_assignment(r->x, r->mq.data(), r->mo.data(), r->mq.size(), asg)) add_jobs(R, asg, walk, r->jobs, owner, 0);
                 } else {
                     make_resident(r->entry, *S.eng);
-                    if (S.assignment(r->x, r->y, asg)) add_jobs(*S.eng, asg, r->jobs, owner, 0, nullptr);
+                    if (S.assignment(r->x, r->y, asg)) add_jobs(*S.eng, asg, walk, r->jobs, owner, 0);
                 }
                 for (auto& J : r->jobs) {
                     const uint32_t d = dev_z8(*J.eng).d;
