@@ -8,8 +8,8 @@ evaluated in one persistent segment-walk launch (2^14-branch Walsh-Hadamard
 segments built once per circuit, the other 16 coordinates walked in Gray
 order).  `value` counts the 2^d branches each amplitude sums.  Engine images
 are resident in HBM before timing starts.  Reported beside it:
-  * the segment-walk kernel on a saturating launch (4096 amplitudes, lane
-    walk length autotuned for it; --sections saturated runs only this),
+  * (section saturated) the segment-walk kernel on a saturating launch (4096
+    amplitudes, lane walk length autotuned for it unless --saturated-tuning),
   * the per-branch Z8 walk (every branch visited) on the same 16 amplitudes,
   * the plain FP64 form (one query per launch, FP64 port of the reference's
     Gray walk) on the same 16 amplitudes, and per-amplitude error against it,
@@ -17,9 +17,9 @@ are resident in HBM before timing starts.  Reported beside it:
     over 2^14-branch Gray chunks) on the host cores, on a bounded sample,
   * the roofline fraction of the walk kernel against the VALU issue rate.
 Further sections (each bounded; --sections picks them):
-  precision  n=48, 40 Rz(theta) (seed 4, d = 28): 1024 amplitudes in the Z8
-             and Z8-accurate forms against the FP64 GPU form, the FP64 form
-             against the CPU oracle on 16 of them; per-amplitude errors, with
+  precision  n=48, 40 Rz(theta) (seed 4, d = 28): 1024 amplitudes in the
+             segment (z8) and branch-walk forms against the FP64 GPU form, the
+             FP64 form against the CPU oracle on 16 of them; per-amplitude errors, with
              |a| < 2^-20 reported separately (--precision-out writes them all)
   batch      256 circuits (n ~ U[20,56], T ~ U[8,32]) x 16 (x, y) pairs: host
              builds, segment-split autotuner, one persistent launch, the same
@@ -30,7 +30,7 @@ Further sections (each bounded; --sections picks them):
              throughput, and results re-evaluated alone for comparison
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu] [--no-plain]
-                       [--sections headline,precision,batch,serve] [--serve-seconds S]
+                       [--sections headline,saturated,precision,batch,serve] [--serve-seconds S]
 Prints one JSON line.
 """
 import argparse
@@ -336,13 +336,16 @@ def serve(args, pq, W, np):
             time.sleep(ta - now)
         tickets.append(srv.submit(c, kind, x, y, meas))
     t_sub = time.perf_counter() - t0
-    lat, vals, failed = [], [], 0
-    for tk in tickets:
+    lat, vals, failed, errors = [], [], 0, []
+    for i, tk in enumerate(tickets):
         try:
             v, l = srv.wait(tk, timeout=120.0)
             lat.append(l); vals.append(v)
-        except pq.PolyqError:
+        except pq.PolyqError as e:
             failed += 1; vals.append(None)
+            if len(errors) < 5:
+                _, c, kind, x, y, meas = sched[i]
+                errors.append({"error": str(e), "kind": kind, "n": c.num_qubits, "measured": [list(m) for m in meas]})
     t_end = time.perf_counter() - t0
     st = srv.stats()
     srv.close()
@@ -369,7 +372,7 @@ def serve(args, pq, W, np):
     return {
         "workload": "Poisson %.0f req/s for %.0f s; 50%% new circuits (n ~ U[20,56], T ~ U[8,32]), 50%% repeats; "
                     "70/20/10 amplitude/probability/4-qubit marginal; 64 GB HBM budget" % (rate, secs),
-        "requests": len(sched), "new_circuits": len(circuits), "kinds": kinds, "failed": failed,
+        "requests": len(sched), "new_circuits": len(circuits), "kinds": kinds, "failed": failed, "errors": errors,
         "p50_ms": float(np.percentile(lat, 50)) if len(lat) else None,
         "p99_ms": float(np.percentile(lat, 99)) if len(lat) else None,
         "max_ms": float(lat.max()) if len(lat) else None,
@@ -389,7 +392,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
-    ap.add_argument("--sections", default="headline,precision,batch,serve")
+    ap.add_argument("--sections", default="headline,saturated,precision,batch,serve")
     ap.add_argument("--serve-seconds", type=float, default=60.0)
     ap.add_argument("--serve-rate", type=float, default=200.0)
     ap.add_argument("--batch-circuits", type=int, default=256)
@@ -419,12 +422,18 @@ def main():
     from polyq_amd import workloads as W
 
     sections = set(args.sections.split(","))
-    if sections == {"saturated"}:
-        # profiling aid: only the saturating segment-walk launches
-        print(json.dumps({"roofline_saturated": saturated(args, pq, W, np)}))
+    if "headline" not in sections:
+        # profiling aid: only the listed sections (e.g. the saturating launches)
+        out = {}
+        if "saturated" in sections:
+            out["roofline_saturated"] = saturated(args, pq, W, np)
+        for name, fn in (("precision", precision), ("batch", batch), ("serve", serve)):
+            if name in sections:
+                out[name] = fn(args, pq, W, np)
+        print(json.dumps(out))
         return
     h = headline(args, pq, W, np)
-    h["roofline_saturated"] = saturated(args, pq, W, np, h["sim"])
+    h["roofline_saturated"] = saturated(args, pq, W, np, h["sim"]) if "saturated" in sections else None
     st, d, nq, elapsed = h["st"], h["d"], h["nq"], h["elapsed"]
     value = h["branches_per_step"] * args.steps / elapsed
     amps = nq * args.steps / elapsed

@@ -207,34 +207,39 @@ struct BlobLayout {
 BlobLayout build_blob(const HostEngine& E, const Z8Image& Z, const SegImage& G, BlobBuilder& B) {
     const ULayout& L = E.L;
     BlobLayout O;
-    // plain image
-    const uint32_t NW = pick_nw(std::max(L.g, L.nodd)), NWe = std::max<uint32_t>(2, (L.neven + 31) / 32 + ((L.neven + 31) / 32) % 2);
+    // plain image: only when the normal form fits the plain kernel's packed
+    // rows (at most 1024 planes or lines); NW = 0 marks an engine without it
+    const bool has_plain = std::max(L.g, L.nodd) <= 1024;
+    const uint32_t NW = has_plain ? pick_nw(std::max(L.g, L.nodd)) : 0;
+    const uint32_t NWe = std::max<uint32_t>(2, (L.neven + 31) / 32 + ((L.neven + 31) / 32) % 2);
     const uint32_t W = L.words();
     std::vector<uint32_t> ku, diag, mrow, we;
-    for (uint32_t j = 0; j < E.d; ++j) {
-        const uint64_t* r = &E.kernel_u[(size_t)j * W];
-        put_u32(ku, r + L.off_a(), L.Wg, L.g, NW); put_u32(ku, r + L.off_b(), L.Wg, L.g, NW); put_u32(ku, r + L.off_o(), L.Wo, L.nodd, NW);
-    }
-    put_u32(diag, E.diag.data() + L.off_a(), L.Wg, L.g, NW); put_u32(diag, E.diag.data() + L.off_b(), L.Wg, L.g, NW);
-    put_u32(diag, E.diag.data() + L.off_o(), L.Wo, L.nodd, NW); put_u32(diag, E.o3.data(), L.Wo, L.nodd, NW);
-    for (uint32_t k = 0; k < E.t; ++k) {
-        const uint64_t* r = &E.mcols[(size_t)k * W];
-        put_u32(mrow, r + L.off_a(), L.Wg, L.g, NW); put_u32(mrow, r + L.off_b(), L.Wg, L.g, NW);
-        put_u32(mrow, r + L.off_o(), L.Wo, L.nodd, NW); put_u32(mrow, r + L.off_e(), L.We, L.neven, NWe);
-    }
-    put_u32(we, E.diag.data() + L.off_e(), L.We, L.neven, NWe);
     std::vector<uint32_t> soff{0};
     std::vector<uint8_t> supp;
-    for (uint32_t j = 0; j < E.d; ++j) {
-        for (uint32_t k = 0; k < E.t; ++k) if ((E.kernel[j] >> k) & 1) supp.push_back((uint8_t)k);
-        soff.push_back((uint32_t)supp.size());
-    }
     std::vector<double> a0, a1, up, dn;
-    for (uint32_t k = 0; k < E.t; ++k) {
-        a0.push_back(E.a0[k].real()); a0.push_back(E.a0[k].imag());
-        a1.push_back(E.a1[k].real()); a1.push_back(E.a1[k].imag());
-        up.push_back(E.up[k].real()); up.push_back(E.up[k].imag());
-        dn.push_back(E.down[k].real()); dn.push_back(E.down[k].imag());
+    if (has_plain) {
+        for (uint32_t j = 0; j < E.d; ++j) {
+            const uint64_t* r = &E.kernel_u[(size_t)j * W];
+            put_u32(ku, r + L.off_a(), L.Wg, L.g, NW); put_u32(ku, r + L.off_b(), L.Wg, L.g, NW); put_u32(ku, r + L.off_o(), L.Wo, L.nodd, NW);
+        }
+        put_u32(diag, E.diag.data() + L.off_a(), L.Wg, L.g, NW); put_u32(diag, E.diag.data() + L.off_b(), L.Wg, L.g, NW);
+        put_u32(diag, E.diag.data() + L.off_o(), L.Wo, L.nodd, NW); put_u32(diag, E.o3.data(), L.Wo, L.nodd, NW);
+        for (uint32_t k = 0; k < E.t; ++k) {
+            const uint64_t* r = &E.mcols[(size_t)k * W];
+            put_u32(mrow, r + L.off_a(), L.Wg, L.g, NW); put_u32(mrow, r + L.off_b(), L.Wg, L.g, NW);
+            put_u32(mrow, r + L.off_o(), L.Wo, L.nodd, NW); put_u32(mrow, r + L.off_e(), L.We, L.neven, NWe);
+        }
+        put_u32(we, E.diag.data() + L.off_e(), L.We, L.neven, NWe);
+        for (uint32_t j = 0; j < E.d; ++j) {
+            for (uint32_t k = 0; k < E.t; ++k) if ((E.kernel[j] >> k) & 1) supp.push_back((uint8_t)k);
+            soff.push_back((uint32_t)supp.size());
+        }
+        for (uint32_t k = 0; k < E.t; ++k) {
+            a0.push_back(E.a0[k].real()); a0.push_back(E.a0[k].imag());
+            a1.push_back(E.a1[k].real()); a1.push_back(E.a1[k].imag());
+            up.push_back(E.up[k].real()); up.push_back(E.up[k].imag());
+            dn.push_back(E.down[k].real()); dn.push_back(E.down[k].imag());
+        }
     }
     O.NW = NW; O.NWe = NWe;
     O.o_ku = B.put(ku.data(), ku.size() * 4); O.o_kn = B.put(E.kernel.data(), E.kernel.size() * 8);
@@ -416,6 +421,7 @@ __global__ __launch_bounds__(256) void k_sum_pairs(const double* __restrict__ in
 std::complex<double> dev_plain_sum(const DevEngine& D, const Prepared& P, double* kernel_ms) {
     Ctx& C = ctx();
     const PlainDev& E = D.plain;
+    if (E.NW == 0) throw std::invalid_argument("normal form too large for the plain kernel (more than 1024 planes or lines)");
     const HostEngine& H = *D.host;
     if (!P.nonzero) return {0, 0};
     PlainQ Q{};
@@ -1319,6 +1325,7 @@ __global__ __launch_bounds__(256) void k_dump_s(PlainDev E, const uint32_t* __re
 
 void dev_dump_s(const DevEngine& D, const Prepared& P, const uint64_t* s, size_t n, uint8_t* zero, uint8_t* k8, uint8_t* ktot) {
     Ctx& C = ctx();
+    if (D.plain.NW == 0) throw std::invalid_argument("normal form too large for the per-branch dump (more than 1024 planes or lines)");
     const HostEngine& H = *D.host;
     const ULayout& L = H.L;
     std::vector<uint32_t> u0;
