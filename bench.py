@@ -74,6 +74,10 @@ def _err_summary(got, ref):
 
 
 SEG_OPS_PER_POINT = 30.0
+# HBM bytes per k_seg_walk launch on these two shapes (FETCH_SIZE x 2 +
+# WRITE_SIZE, copy-calibrated; profiles/k_seg_walk_config2.md): the 256 KB
+# table once per XCD L2, descriptors and item partials.
+SEG_TRAFFIC = {"headline": 2.43e6, "saturated": 4.13e6}
 # Algorithmic lane-ops per outer point of k_seg_walk: the integer walk step
 # (phase update with the cross-term parity, sign-mask flip, table index: 6)
 # and the double-float arithmetic (walk magnitude times the complex table
@@ -88,11 +92,11 @@ def _peak(dev):
     return cus * 128 * 2.4e9
 
 
-def _seg_roofline(dev, points, kernel_ms, extra=None):
+def _seg_roofline(dev, points, kernel_ms, extra=None, traffic=None):
     peak = _peak(dev)
     achieved = points * SEG_OPS_PER_POINT / (kernel_ms * 1e-3)
     r = {"bound": "valu", "achieved": achieved / 1e12, "peak": peak / 1e12, "unit": "Tlane-op/s", "frac": achieved / peak,
-         "traffic": None, "kernel": "k_seg_walk", "kernel_avg_ms": kernel_ms, "ops_per_outer_point": SEG_OPS_PER_POINT,
+         "traffic": traffic, "kernel": "k_seg_walk", "kernel_avg_ms": kernel_ms, "ops_per_outer_point": SEG_OPS_PER_POINT,
          "outer_points_per_launch": points}
     if extra:
         r.update(extra)
@@ -130,7 +134,7 @@ def headline(args, pq, W, np):
     # Roofline of k_seg_walk on this launch (16 queries: 2^20 outer points,
     # too few lanes to fill 256 CUs); the saturating launch is measured by
     # saturated() below.
-    res["roofline"] = _seg_roofline(dev, points_per_step, kernel_avg_ms)
+    res["roofline"] = _seg_roofline(dev, points_per_step, kernel_avg_ms, traffic=SEG_TRAFFIC["headline"])
 
     # The per-branch Z8 walk on the same 16 amplitudes (every branch visited:
     # one sign XOR and one FMA per branch, one more LDS lookup with inner pivots).
@@ -217,7 +221,8 @@ def saturated(args, pq, W, np, sim=None):
     return _seg_roofline(dev, 4096 * (1 << seg["outer"]), float(np.median(bms)),
                          {"queries": 4096, "walk_tuning": [off, mx], "mid_bits": tuned,
                           "amplitudes_per_s": 4096 / float(np.median(bwall)),
-                          "branches_per_s": 4096 * (1 << d) / float(np.median(bwall))})
+                          "branches_per_s": 4096 * (1 << d) / float(np.median(bwall))},
+                         traffic=SEG_TRAFFIC["saturated"] if (off, mx) == (8, 8) else None)
 
 
 def precision(args, pq, W, np):
