@@ -192,6 +192,9 @@ std::shared_ptr<HostEngine> build_engine(const PathSum& ps, const std::vector<ui
     }
     E.b_fixed.assign(E.nf, Bits(E.nf));
     for (uint32_t i = 0; i < E.nf; ++i) ps.adj[fixed[i]].each([&](uint32_t j) { if (E.fixed_pos[j] >= 0) E.b_fixed[i].set((uint32_t)E.fixed_pos[j]); });
+    E.nfw = (E.nf + 63) / 64;
+    E.b_fixed_flat.assign((size_t)E.nf * E.nfw, 0);
+    for (uint32_t i = 0; i < E.nf; ++i) std::copy(E.b_fixed[i].w.begin(), E.b_fixed[i].w.end(), &E.b_fixed_flat[(size_t)i * E.nfw]);
 
     NormalForm nf = normal_form(bff, parity);
     ULayout& L = E.L;
@@ -343,15 +346,24 @@ bool prepare(const HostEngine& E, const uint8_t* values, Prepared& P) {
     // Z4 value of the fixed assignment: linear part plus 2 per adjacent pair of
     // ones.  b_fixed is symmetric with a zero diagonal, so twice the pair count
     // is sum over ones a of |B_a & ones|.
-    thread_local Bits ones;
-    if (ones.n != E.nf) ones = Bits(E.nf); else ones.clear();
-    for (uint32_t i = 0; i < E.nf; ++i) if (values[i]) ones.set(i);
-    ones.each([&](uint32_t f) {
-        z += E.fixed_v[f] + E.b_fixed[f].and_popc(ones);
-        th += E.fixed_theta[f];
-        const uint64_t* c = &E.ucols[(size_t)f * W];
-        for (uint32_t k = 0; k < W; ++k) P.u0[k] ^= c[k];
-    });
+    thread_local std::vector<uint64_t> ones;
+    const uint32_t nfw = E.nfw;
+    ones.assign(nfw, 0);
+    for (uint32_t i = 0; i < E.nf; ++i) ones[i >> 6] |= (uint64_t)(values[i] & 1) << (i & 63);
+    uint64_t* __restrict__ u0 = P.u0.data();
+    const uint64_t* __restrict__ on = ones.data();
+    for (uint32_t wi = 0; wi < nfw; ++wi) {
+        for (uint64_t x = on[wi]; x; x &= x - 1) {
+            const uint32_t f = wi * 64 + (uint32_t)__builtin_ctzll(x);
+            const uint64_t* __restrict__ b = &E.b_fixed_flat[(size_t)f * nfw];
+            uint32_t pc = 0;
+            for (uint32_t k = 0; k < nfw; ++k) pc += (uint32_t)__builtin_popcountll(b[k] & on[k]);
+            z += E.fixed_v[f] + pc;
+            th += E.fixed_theta[f];
+            const uint64_t* __restrict__ c = &E.ucols[(size_t)f * W];
+            for (uint32_t k = 0; k < W; ++k) u0[k] ^= c[k];
+        }
+    }
     P.phase = th + z4(z);
     // kappa = w_e + u0_e; particular solution of K s = kappa (engine.rs:188-201)
     thread_local Bits kappa;

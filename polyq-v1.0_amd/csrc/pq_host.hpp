@@ -131,6 +131,8 @@ struct HostEngine {
     std::vector<uint8_t> fixed_v;
     std::vector<Phase> fixed_theta;
     std::vector<Bits> b_fixed;
+    std::vector<uint64_t> b_fixed_flat;      // the same rows, nfw words each
+    uint32_t nfw = 0;
     // packed shift rows (L.words() u64 each)
     std::vector<uint64_t> ucols;    // nf rows
     std::vector<uint64_t> mcols;    // t rows

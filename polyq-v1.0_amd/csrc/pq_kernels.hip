@@ -61,7 +61,7 @@ struct HBuf {
             if (p) HIPCHECK(hipHostFree(p));
             size_t c = std::max<size_t>(n, cap * 2);
             c = std::max<size_t>(c, 4096);
-            HIPCHECK(hipHostMalloc(&p, c, hipHostMallocDefault));
+            HIPCHECK(hipHostMalloc(&p, c, hipHostMallocMapped));   // pinned, device-addressable
             cap = c;
         }
         return p;
@@ -674,6 +674,44 @@ __global__ __launch_bounds__(256) void k_z8_walk(const Z8Dev* __restrict__ engs,
     }
 }
 
+// FP64 reduction of one query's item partials in a fixed order (strided by
+// thread, then a fixed shuffle and LDS tree, so the result does not depend on
+// which workgroup runs it), EV + omega OD, and the FP64 scale.  All 256
+// threads of the workgroup take part; thread 0 writes the result.
+template <bool ATOMIC_READ>
+__device__ __forceinline__ void query_reduce(uint64_t item0, uint64_t nitems, const double* partial, double2 sc, double* dst,
+                                             double (&red)[4][4]) {
+    const uint32_t tid = threadIdx.x;
+    double v[4] = {0, 0, 0, 0};
+    for (uint64_t k = tid; k < nitems; k += 256) {
+#pragma unroll
+        for (int c = 0; c < 4; ++c) {
+            const double* p = partial + 4 * (item0 + k) + c;
+            v[c] += ATOMIC_READ ? __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) : *p;
+        }
+    }
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+#pragma unroll
+        for (int c = 0; c < 4; ++c) v[c] += __shfl_xor(v[c], off);
+    }
+    if ((tid & 63) == 0) {
+#pragma unroll
+        for (int c = 0; c < 4; ++c) red[tid >> 6][c] = v[c];
+    }
+    __syncthreads();
+    if (tid == 0) {
+        double t[4];
+        for (int c = 0; c < 4; ++c) t[c] = (red[0][c] + red[1][c]) + (red[2][c] + red[3][c]);
+        const double h = 0.70710678118654752440;
+        // EV + omega * OD
+        const double re = t[0] + h * (t[2] - t[3]);
+        const double im = t[1] + h * (t[2] + t[3]);
+        dst[0] = sc.x * re - sc.y * im;
+        dst[1] = sc.x * im + sc.y * re;
+    }
+}
+
 // ======================================================= segment walk ====
 // Double-float (FP32 hi/lo pair) products and sums.  ph + pl is the product
 // of (ah + al)(bh + bl) up to the dropped al*bl term; the sum keeps the
@@ -701,27 +739,36 @@ struct SegLaneDev {
 };
 // Lane bit b -> outer slot: the lane-low slots first, then past the M walked slots.
 __device__ __forceinline__ uint32_t seg_slot_of_bit(const SegDev& E, uint32_t M, uint32_t b) { return b < E.L6 ? b : E.L6 + M + (b - E.L6); }
-__device__ __forceinline__ void seg_seed(const SegDev& E, const SegQ& Q, uint64_t lane, SegLaneDev& L) {
+// Outer-slot arrays of an engine, read through pointers of the address space
+// they live in: LDS copies in k_seg_walk, the device image in the dumps.
+template <class P64, class P32, class PF>
+struct SegSlots {
+    P64 oq, on;
+    P32 ox, opin, olam;
+    PF rmid;
+};
+template <class P64, class P32, class PF>
+__device__ __forceinline__ void seg_seed(const SegDev& E, const SegSlots<P64, P32, PF>& T, const SegQ& Q, uint64_t lane, SegLaneDev& L) {
     L.cout = 0; L.s = Q.s0; L.K = Q.K0; L.hs = Q.hs0; L.pio = Q.pio0; L.fh = 1.f; L.fl = 0.f;
     const uint32_t M = Q.M;
     for (uint32_t b = 0; b < Q.H; ++b) {
         const uint32_t k = seg_slot_of_bit(E, M, b);
         const bool on = (lane >> b) & 1;
         if (k < E.L6 + E.Mcap && k >= E.L6 + M) {
-            const PQ_CONST float* r = cview(E.rmid) + 4 * (k - E.L6);
-            const float r0h = r[0], r0l = r[1], r1h = r[2], r1l = r[3];
+            const uint32_t m = 4 * (k - E.L6);
+            const float r0h = T.rmid[m], r0l = T.rmid[m + 1], r1h = T.rmid[m + 2], r1l = T.rmid[m + 3];
             const float rh = on ? r1h : r0h, rl = on ? r1l : r0l;
             float ph, pl;
             df_mul(L.fh, L.fl, rh, rl, ph, pl);
             L.fh = ph; L.fl = pl;
         }
         if (on) {
-            const uint32_t ell = cview(E.olam)[k] + (((uint32_t)(Q.delta >> k) & 1u) << 2);
-            L.K += ell + ((__popcll(cview(E.oq)[k] & L.cout) & 1u) << 2);
+            const uint32_t ell = T.olam[k] + (((uint32_t)(Q.delta >> k) & 1u) << 2);
+            L.K += ell + ((__popcll(T.oq[k] & L.cout) & 1u) << 2);
             L.cout |= 1ull << k;
-            L.hs ^= cview(E.ox)[k];
-            L.pio ^= cview(E.opin)[k];
-            L.s ^= cview(E.on)[k];
+            L.hs ^= T.ox[k];
+            L.pio ^= T.opin[k];
+            L.s ^= T.on[k];
         }
     }
 }
@@ -741,9 +788,10 @@ __device__ __forceinline__ void seg_lane_factor(const float2* s_lt, uint32_t nby
 // point), delta_w the query bits of the walk slots (uniform) and
 // P_j = parity(Q_{walk j} . c_lane) the only lane-dependent part.  The walk
 // slots are pivot-free, so pio and the lane factor are fixed over the segment.
-__device__ __forceinline__ uint32_t seg_lane_parities(const SegDev& E, uint32_t M, uint64_t cout) {
+template <class P64>
+__device__ __forceinline__ uint32_t seg_lane_parities(const SegDev& E, P64 oq, uint32_t M, uint64_t cout) {
     uint32_t P = 0;
-    for (uint32_t j = 0; j < M; ++j) P |= (uint32_t)(__popcll(cview(E.oq)[E.L6 + j] & cout) & 1) << j;
+    for (uint32_t j = 0; j < M; ++j) P |= (uint32_t)(__popcll(oq[E.L6 + j] & cout) & 1) << j;
     return P;
 }
 typedef float f32x4 __attribute__((ext_vector_type(4)));
@@ -787,26 +835,47 @@ __device__ __forceinline__ void seg_fold4(const float (&A)[8][4], float (&dst)[4
 // masks the table basis maps to the low six index bits: one wavefront reads
 // one contiguous 1 KB block per point.
 typedef const __attribute__((address_space(1))) f32x4 GFloat4;
-__global__ __launch_bounds__(256) void k_seg_walk(const SegDev* __restrict__ engs, const SegQ* __restrict__ qs, uint32_t nq,
-                                                  uint64_t item_end, unsigned long long* __restrict__ counter, double* __restrict__ partial) {
+typedef __attribute__((address_space(3))) const uint64_t LdsU64;
+typedef __attribute__((address_space(3))) const uint32_t LdsU32;
+typedef __attribute__((address_space(3))) const float LdsF32;
+// After its item partials are written, the last workgroup of a query (the
+// one whose increment of the query's done counter completes it) reduces the
+// query in FP64 and writes the scaled amplitude straight into the caller's
+// pinned host buffer: no separate reduction launch and no result copy.
+__global__ __launch_bounds__(256) void k_seg_walk(const SegDev* __restrict__ engs, const SegQ* __restrict__ qs, const uint32_t* __restrict__ item_q,
+                                                  uint64_t item_end, unsigned long long* __restrict__ counter, double* __restrict__ partial,
+                                                  unsigned* __restrict__ qdone, const double2* __restrict__ scale, const uint32_t* __restrict__ order,
+                                                  double* __restrict__ out, int fused) {
+    __shared__ double s_qred[4][4];
+    __shared__ int s_last;
     __shared__ float2 s_lt[8 * 256];
+    // the engine's outer-slot arrays, so the lane seed reads LDS, not a chain
+    // of scalar-cache round trips
+    __shared__ uint64_t s_oq[64], s_on[64];
+    __shared__ uint32_t s_ox[64], s_opin[64], s_olam[64];
+    __shared__ float s_rmid[4 * kSegMaxMid];
     __shared__ unsigned long long s_item;
     __shared__ int s_eng;
     __shared__ float s_red[4][8];
     const uint32_t tid = threadIdx.x;
     if (tid == 0) s_eng = -1;
+    const SegSlots<LdsU64*, LdsU32*, LdsF32*> T{(LdsU64*)s_oq, (LdsU64*)s_on, (LdsU32*)s_ox, (LdsU32*)s_opin, (LdsU32*)s_olam, (LdsF32*)s_rmid};
     for (;;) {
         __syncthreads();
         if (tid == 0) s_item = atomicAdd(counter, 1ull);
         __syncthreads();
         const uint64_t item = s_item;
         if (item >= item_end) break;
-        uint32_t lo = 0, hi = nq;
-        while (hi - lo > 1) { const uint32_t mid = (lo + hi) >> 1; if (cview(qs)[mid].item0 <= item) lo = mid; else hi = mid; }
-        const SegQ Q = cload(qs + lo);
+        const uint32_t qi = cview(item_q)[item];
+        const SegQ Q = cload(qs + qi);
         const SegDev E = cload(engs + Q.eng);
         if ((int)Q.eng != s_eng) {
             for (uint32_t k = tid; k < E.nbytes * 256u; k += 256) s_lt[k] = E.lt[k];
+            if (tid < E.nout) {
+                s_oq[tid] = E.oq[tid]; s_on[tid] = E.on[tid];
+                s_ox[tid] = E.ox[tid]; s_opin[tid] = E.opin[tid]; s_olam[tid] = E.olam[tid];
+            }
+            if (tid < 4 * E.Mcap) s_rmid[tid] = E.rmid[tid];
             __syncthreads();
             if (tid == 0) s_eng = (int)Q.eng;
         }
@@ -814,11 +883,11 @@ __global__ __launch_bounds__(256) void k_seg_walk(const SegDev* __restrict__ eng
         const uint64_t lane = (item - Q.item0) * 256 + tid;
         if ((lane >> Q.H) == 0) {
             SegLaneDev L;
-            seg_seed(E, Q, lane, L);
+            seg_seed(E, T, Q, lane, L);
             float fh = L.fh, fl = L.fl;
             seg_lane_factor(s_lt, E.nbytes, L.s, fh, fl);
             const uint32_t M = Q.M;
-            const uint32_t P = seg_lane_parities(E, M, L.cout);
+            const uint32_t P = seg_lane_parities(E, T.oq, M, L.cout);
             const uint32_t dw = (uint32_t)(Q.delta >> E.L6) & ((1u << M) - 1u);
             const uint32_t base = (L.pio << E.I) | L.hs;
             GFloat4* tab = (GFloat4*)E.table;
@@ -904,51 +973,47 @@ __global__ __launch_bounds__(256) void k_seg_walk(const SegDev* __restrict__ eng
                 v[2] += (double)s_red[w][4] + (double)s_red[w][5];
                 v[3] += (double)s_red[w][6] + (double)s_red[w][7];
             }
-            partial[4 * item + 0] = v[0];
-            partial[4 * item + 1] = v[1];
-            partial[4 * item + 2] = v[2];
-            partial[4 * item + 3] = v[3];
+            if (!fused) {
+                // large batches: item partials only; k_z8_reduce follows
+#pragma unroll
+                for (int c = 0; c < 4; ++c) partial[4 * item + c] = v[c];
+                s_last = 0;
+            } else if (Q.nitems == 1) {
+                // the query is this item alone: finish it here (the same
+                // arithmetic as query_reduce on a single item)
+                const double h = 0.70710678118654752440;
+                const double re = v[0] + h * (v[2] - v[3]), im = v[1] + h * (v[2] + v[3]);
+                const double2 sc = scale[qi];
+                double* dst = out + 2 * order[qi];
+                dst[0] = sc.x * re - sc.y * im;
+                dst[1] = sc.x * im + sc.y * re;
+                s_last = 0;
+            } else {
+                // Partials go out as device-scope atomic stores and the count
+                // as a release; the last workgroup reads them back with
+                // device-scope atomic loads.  No acquire fence: it would
+                // invalidate this XCD's L2, which holds the segment table.
+#pragma unroll
+                for (int c = 0; c < 4; ++c) __hip_atomic_store(partial + 4 * item + c, v[c], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                const unsigned prev = __hip_atomic_fetch_add(qdone + qi, 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+                s_last = (prev + 1u == (unsigned)Q.nitems) ? 1 : 0;
+            }
         }
+        __syncthreads();
+        if (s_last) query_reduce<true>(Q.item0, Q.nitems, partial, scale[qi], out + 2 * order[qi], s_qred);
     }
 }
 
 // FP64 cross-workgroup reduction (fixed order) and global scaling per query.
-// Both query records (Z8Q, SegQ) carry item0 and nitems in words 2 and 3.
+// Separate reduction launch (branch walk): one workgroup per query.  Both
+// query records (Z8Q, SegQ) carry item0 and nitems in words 2 and 3.
 __global__ __launch_bounds__(256) void k_z8_reduce(const uint64_t* __restrict__ qwords, const double* __restrict__ partial, const double2* __restrict__ scale,
                                                    const uint32_t* __restrict__ order, double* __restrict__ out) {
     __shared__ double red[4][4];
-    const uint32_t qi = blockIdx.x, tid = threadIdx.x;
-    struct { uint64_t item0, nitems; } Q = {qwords[8 * qi + 2], qwords[8 * qi + 3]};
-    double v[4] = {0, 0, 0, 0};
-    for (uint64_t k = tid; k < Q.nitems; k += 256) {
-#pragma unroll
-        for (int c = 0; c < 4; ++c) v[c] += partial[4 * (Q.item0 + k) + c];
-    }
-#pragma unroll
-    for (int off = 32; off > 0; off >>= 1) {
-#pragma unroll
-        for (int c = 0; c < 4; ++c) v[c] += __shfl_xor(v[c], off);
-    }
-    if ((tid & 63) == 0) {
-#pragma unroll
-        for (int c = 0; c < 4; ++c) red[tid >> 6][c] = v[c];
-    }
-    __syncthreads();
-    if (tid == 0) {
-        double s[4];
-        for (int c = 0; c < 4; ++c) s[c] = (red[0][c] + red[1][c]) + (red[2][c] + red[3][c]);
-        const double h = 0.70710678118654752440;
-        // EV + omega * OD
-        const double re = s[0] + h * (s[2] - s[3]);
-        const double im = s[1] + h * (s[2] + s[3]);
-        const double2 sc = scale[qi];
-        const uint32_t dst = order[qi];
-        out[2 * dst] = sc.x * re - sc.y * im;
-        out[2 * dst + 1] = sc.x * im + sc.y * re;
-    }
+    const uint32_t qi = blockIdx.x;
+    query_reduce<false>(qwords[8 * qi + 2], qwords[8 * qi + 3], partial, scale[qi], out + 2 * order[qi], red);
 }
 
-__global__ void k_zero_u64(unsigned long long* p) { p[threadIdx.x] = 0ull; }   // one thread per counter
 
 // Segment split: a lane walks 2^(8+M) branches and 2^H lanes cover a query
 // (H = d-8-M).  M depends on d alone (M = clamp(d - offset, 0, max)), so a
@@ -989,6 +1054,7 @@ int variant_of(const Z8Image& Z) { return (Z.npin ? 2 : 0) | (Z.imag0 ? 1 : 0); 
 struct BatchPlan {
     int walk = kWalkSeg;
     std::vector<uint8_t> engs, qs;
+    std::vector<uint32_t> item_q;      // segment walk: query of each item
     std::vector<double2> scale;
     std::vector<uint32_t> order;
     uint64_t total_items = 0, branches = 0;
@@ -1072,6 +1138,7 @@ BatchPlan plan_seg(const std::vector<Z8Job>& jobs) {
         q.item0 = B.total_items;
         q.nitems = ((1ull << q.H) + 255) / 256;
         B.total_items += q.nitems;
+        B.item_q.insert(B.item_q.end(), q.nitems, B.nq());
         B.branches += 1ull << G.d;
         push_rec(B.qs, q);
         B.scale.push_back(make_double2(J.scale.real(), J.scale.imag()));
@@ -1101,11 +1168,43 @@ uint32_t launch_v(Ctx& C, hipStream_t st, const BatchPlan& B, int v, const Z8Dev
 }
 // Launch the walk of a staged batch (one persistent grid per branch-walk
 // variant present, or one segment-walk grid); returns the summed grid.
-uint32_t launch_walks(Ctx& C, hipStream_t st, const BatchPlan& B, const void* de, const void* dq, unsigned long long* ctr, double* part) {
+// The work-queue counters come first in the upload and are zero in the host
+// copy, so the descriptor copy also clears them (no fill kernel and no memset
+// node in a captured graph).
+struct StageLayout {
+    size_t oe, oq, os, oo, oi, tot;
+};
+StageLayout stage_layout(const BatchPlan& B) {
+    StageLayout L;
+    const size_t be = B.engs.size(), bq = B.qs.size(), bs = B.scale.size() * sizeof(double2), bo = B.order.size() * 4;
+    L.oe = (64 + 4 * (size_t)B.nq() + 255) & ~size_t(255);   // 4 u64 queue counters at 0, u32 per-query done counts at 64
+    L.oq = L.oe + ((be + 255) & ~size_t(255));
+    L.os = L.oq + ((bq + 255) & ~size_t(255));
+    L.oo = L.os + ((bs + 255) & ~size_t(255));
+    L.oi = L.oo + ((bo + 255) & ~size_t(255));
+    L.tot = L.oi + B.item_q.size() * 4;
+    return L;
+}
+void stage_copy(const BatchPlan& B, const StageLayout& L, uint8_t* h) {
+    std::memset(h, 0, L.oe);
+    std::memcpy(h + L.oe, B.engs.data(), B.engs.size());
+    std::memcpy(h + L.oq, B.qs.data(), B.qs.size());
+    std::memcpy(h + L.os, B.scale.data(), B.scale.size() * sizeof(double2));
+    std::memcpy(h + L.oo, B.order.data(), B.order.size() * 4);
+    std::memcpy(h + L.oi, B.item_q.data(), B.item_q.size() * 4);
+}
+// Segment walk with out != null: the reduction is fused and results go to
+// out (host-mapped).  Otherwise the caller launches k_z8_reduce afterwards.
+uint32_t launch_walks(Ctx& C, hipStream_t st, const BatchPlan& B, uint8_t* dbase, const StageLayout& L, double* part, double* out) {
+    unsigned long long* ctr = (unsigned long long*)dbase;
+    const void* de = dbase + L.oe;
+    const void* dq = dbase + L.oq;
     if (B.walk == kWalkSeg) {
         if (B.total_items == 0) return 0;
         const uint32_t grid = grid_for(C, k_seg_walk, B.total_items);
-        hipLaunchKernelGGL(k_seg_walk, dim3(grid), dim3(256), 0, st, (const SegDev*)de, (const SegQ*)dq, B.nq(), B.total_items, ctr, part);
+        hipLaunchKernelGGL(k_seg_walk, dim3(grid), dim3(256), 0, st, (const SegDev*)de, (const SegQ*)dq, (const uint32_t*)(dbase + L.oi),
+                           B.total_items, ctr, part, (unsigned*)(dbase + 64), (const double2*)(dbase + L.os), (const uint32_t*)(dbase + L.oo), out,
+                           out ? 1 : 0);
         HIPCHECK(hipGetLastError());
         return grid;
     }
@@ -1121,32 +1220,22 @@ uint32_t launch_walks(Ctx& C, hipStream_t st, const BatchPlan& B, const void* de
     return g;
 }
 // Offsets of the staged parts in one upload buffer.
-struct StageLayout {
-    size_t oq, os, oo, tot;
-};
-StageLayout stage_layout(const BatchPlan& B) {
-    StageLayout L;
-    const size_t be = B.engs.size(), bq = B.qs.size(), bs = B.scale.size() * sizeof(double2);
-    L.oq = (be + 255) & ~size_t(255);
-    L.os = L.oq + ((bq + 255) & ~size_t(255));
-    L.oo = L.os + ((bs + 255) & ~size_t(255));
-    L.tot = L.oo + B.order.size() * 4;
-    return L;
-}
-void stage_copy(const BatchPlan& B, const StageLayout& L, uint8_t* h) {
-    std::memcpy(h, B.engs.data(), B.engs.size());
-    std::memcpy(h + L.oq, B.qs.data(), B.qs.size());
-    std::memcpy(h + L.os, B.scale.data(), B.scale.size() * sizeof(double2));
-    std::memcpy(h + L.oo, B.order.data(), B.order.size() * 4);
-}
 }  // namespace
+
+// The segment walk finishes queries inside the walk kernel (last workgroup of
+// a query reduces it, results written to pinned host memory) for batches up to
+// this many items, where launch latency dominates; larger batches use the
+// separate reduction launch, which avoids one L2 write-back per item.  The
+// reduction order, hence every result bit, is the same either way.
+constexpr uint64_t kFuseItems = 4096;
+bool fuse_reduce(const BatchPlan& B) { return B.walk == kWalkSeg && B.total_items <= kFuseItems; }
 
 // One in-flight batch: its own stream, events and buffers, so a second batch
 // can be queued while this one still runs (continuous batching).
 struct Z8Slot {
     hipStream_t stream = nullptr;
     hipEvent_t e0 = nullptr, e1 = nullptr, e2 = nullptr, e3 = nullptr;
-    DBuf d_in, d_ctr, d_part, d_res;
+    DBuf d_in, d_part, d_res;
     HBuf h_in, h_res;
     uint32_t nq = 0, grid = 0;
     uint64_t items = 0, branches = 0;
@@ -1160,7 +1249,6 @@ struct Z8Slot {
         for (hipEvent_t e : {e0, e1, e2, e3}) if (e) (void)hipEventDestroy(e);
         if (stream) (void)hipStreamDestroy(stream);
         if (d_in.p) (void)hipFree(d_in.p);
-        if (d_ctr.p) (void)hipFree(d_ctr.p);
         if (d_part.p) (void)hipFree(d_part.p);
         if (d_res.p) (void)hipFree(d_res.p);
         if (h_in.p) (void)hipHostFree(h_in.p);
@@ -1177,24 +1265,30 @@ void dev_z8_launch(Z8Slot& S, const std::vector<Z8Job>& jobs, int walk) {
     const uint32_t nq = B.nq();
     S.nq = nq; S.items = B.total_items; S.branches = B.branches;
     if (nq == 0) return;
-    // one pinned staging buffer: engines | queries | scales | order
+    // one pinned staging buffer: counters | engines | queries | scales | order | item queries
     const StageLayout L = stage_layout(B);
     uint8_t* h = (uint8_t*)S.h_in.get(L.tot);
     stage_copy(B, L, h);
     uint8_t* d = (uint8_t*)S.d_in.get(L.tot);
-    unsigned long long* ctr = (unsigned long long*)S.d_ctr.get(64);
     double* part = (double*)S.d_part.get(std::max<uint64_t>(B.total_items, 1) * 32);
-    double* res = (double*)S.d_res.get(nq * 16);
     double* hres = (double*)S.h_res.get(nq * 16);
     HIPCHECK(hipEventRecord(S.e0, S.stream));
     HIPCHECK(hipMemcpyAsync(d, h, L.tot, hipMemcpyHostToDevice, S.stream));
-    hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(kVariants), 0, S.stream, ctr);
     HIPCHECK(hipEventRecord(S.e1, S.stream));
-    S.grid = launch_walks(C, S.stream, B, d, d + L.oq, ctr, part);
-    HIPCHECK(hipEventRecord(S.e2, S.stream));
-    hipLaunchKernelGGL(k_z8_reduce, dim3(nq), dim3(256), 0, S.stream, (const uint64_t*)(d + L.oq), part, (const double2*)(d + L.os), (const uint32_t*)(d + L.oo), res);
-    HIPCHECK(hipGetLastError());
-    HIPCHECK(hipMemcpyAsync(hres, res, nq * 16, hipMemcpyDeviceToHost, S.stream));
+    if (fuse_reduce(B)) {
+        // results land in the pinned host buffer, written by the walk itself
+        double* dres = nullptr;
+        HIPCHECK(hipHostGetDevicePointer((void**)&dres, hres, 0));
+        S.grid = launch_walks(C, S.stream, B, d, L, part, dres);
+        HIPCHECK(hipEventRecord(S.e2, S.stream));
+    } else {
+        double* res = (double*)S.d_res.get(nq * 16);
+        S.grid = launch_walks(C, S.stream, B, d, L, part, nullptr);
+        HIPCHECK(hipEventRecord(S.e2, S.stream));
+        hipLaunchKernelGGL(k_z8_reduce, dim3(nq), dim3(256), 0, S.stream, (const uint64_t*)(d + L.oq), part, (const double2*)(d + L.os), (const uint32_t*)(d + L.oo), res);
+        HIPCHECK(hipGetLastError());
+        HIPCHECK(hipMemcpyAsync(hres, res, nq * 16, hipMemcpyDeviceToHost, S.stream));
+    }
     HIPCHECK(hipEventRecord(S.e3, S.stream));
     S.pending = true;
 }
@@ -1236,7 +1330,7 @@ void dev_z8_batch(const std::vector<Z8Job>& jobs, std::vector<std::complex<doubl
 struct Z8Graph {
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
-    void *h_in = nullptr, *h_out = nullptr, *d_in = nullptr, *d_part = nullptr, *d_res = nullptr, *d_ctr = nullptr;
+    void *h_in = nullptr, *h_out = nullptr, *d_in = nullptr, *d_part = nullptr, *d_res = nullptr;
     uint32_t nq = 0, grid = 0;
     uint64_t items = 0, branches = 0;
     hipEvent_t k0 = nullptr, k1 = nullptr;
@@ -1245,7 +1339,7 @@ struct Z8Graph {
         if (graph) (void)hipGraphDestroy(graph);
         if (h_in) (void)hipHostFree(h_in);
         if (h_out) (void)hipHostFree(h_out);
-        for (void* p : {d_in, d_part, d_res, d_ctr}) if (p) (void)hipFree(p);
+        for (void* p : {d_in, d_part, d_res}) if (p) (void)hipFree(p);
         if (k0) (void)hipEventDestroy(k0);
         if (k1) (void)hipEventDestroy(k1);
     }
@@ -1261,20 +1355,24 @@ std::shared_ptr<Z8Graph> dev_z8_graph(const std::vector<Z8Job>& jobs, int walk) 
     const StageLayout L = stage_layout(B);
     std::lock_guard<std::mutex> lk(C.mu);
     HIPCHECK(hipHostMalloc(&G->h_in, L.tot, hipHostMallocDefault));
-    HIPCHECK(hipHostMalloc(&G->h_out, nq * 16, hipHostMallocDefault));
+    HIPCHECK(hipHostMalloc(&G->h_out, nq * 16, hipHostMallocMapped));
     HIPCHECK(hipMalloc(&G->d_in, L.tot));
     HIPCHECK(hipMalloc(&G->d_part, std::max<uint64_t>(B.total_items * 32, 32)));
     HIPCHECK(hipMalloc(&G->d_res, nq * 16));
-    HIPCHECK(hipMalloc(&G->d_ctr, 64));
     HIPCHECK(hipEventCreate(&G->k0)); HIPCHECK(hipEventCreate(&G->k1));
     stage_copy(B, L, (uint8_t*)G->h_in);
     uint8_t* d = (uint8_t*)G->d_in;
+    double* dout = nullptr;
+    HIPCHECK(hipHostGetDevicePointer((void**)&dout, G->h_out, 0));
     HIPCHECK(hipStreamBeginCapture(C.stream, hipStreamCaptureModeThreadLocal));
-    HIPCHECK(hipMemcpyAsync(d, G->h_in, L.tot, hipMemcpyHostToDevice, C.stream));
-    hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(kVariants), 0, C.stream, (unsigned long long*)G->d_ctr);   // a kernel, not a memset node
-    G->grid = launch_walks(C, C.stream, B, d, d + L.oq, (unsigned long long*)G->d_ctr, (double*)G->d_part);
-    hipLaunchKernelGGL(k_z8_reduce, dim3(nq), dim3(256), 0, C.stream, (const uint64_t*)(d + L.oq), (const double*)G->d_part, (const double2*)(d + L.os), (const uint32_t*)(d + L.oo), (double*)G->d_res);
-    HIPCHECK(hipMemcpyAsync(G->h_out, G->d_res, nq * 16, hipMemcpyDeviceToHost, C.stream));
+    HIPCHECK(hipMemcpyAsync(d, G->h_in, L.tot, hipMemcpyHostToDevice, C.stream));   // also clears the counters
+    if (fuse_reduce(B)) {
+        G->grid = launch_walks(C, C.stream, B, d, L, (double*)G->d_part, dout);
+    } else {
+        G->grid = launch_walks(C, C.stream, B, d, L, (double*)G->d_part, nullptr);
+        hipLaunchKernelGGL(k_z8_reduce, dim3(nq), dim3(256), 0, C.stream, (const uint64_t*)(d + L.oq), (const double*)G->d_part, (const double2*)(d + L.os), (const uint32_t*)(d + L.oo), (double*)G->d_res);
+        HIPCHECK(hipMemcpyAsync(G->h_out, G->d_res, nq * 16, hipMemcpyDeviceToHost, C.stream));
+    }
     HIPCHECK(hipStreamEndCapture(C.stream, &G->graph));
     HIPCHECK(hipGraphInstantiate(&G->exec, G->graph, nullptr, nullptr, 0));
     return G;
@@ -1386,9 +1484,10 @@ __global__ __launch_bounds__(256) void k_dump_seg(SegDev E, SegQ Q, uint8_t* __r
     const uint64_t lane = (uint64_t)blockIdx.x * 256 + threadIdx.x;
     if ((lane >> Q.H) != 0) return;
     SegLaneDev L;
-    seg_seed(E, Q, lane, L);
+    const SegSlots<const uint64_t*, const uint32_t*, const float*> T{E.oq, E.on, E.ox, E.opin, E.olam, E.rmid};
+    seg_seed(E, T, Q, lane, L);
     const uint32_t M = Q.M;
-    const uint32_t P = seg_lane_parities(E, M, L.cout);
+    const uint32_t P = seg_lane_parities(E, T.oq, M, L.cout);
     const uint32_t dw = (uint32_t)(Q.delta >> E.L6) & ((1u << M) - 1u);
     const uint32_t* pt = E.pts + 4 * ((1u << M) - 1u);
     const uint32_t* cl = E.pcls + 9 * M;

@@ -38,7 +38,7 @@ def test_batch_results_equal_alone_results_bitwise():
     for s, x, y, a in list(zip(S, X, Y, mixed))[:24]:
         o = O.OSim(O.OCircuit.from_circuit(s.circuit), "cached")
         if s.stats()[5] <= 18:
-            assert abs(o.amplitude(x, y) - a) < 1e-6
+            assert abs(o.amplitude(x, y) - a) < 1e-10
 
 
 def test_graph_replays_match_eager():
@@ -77,7 +77,7 @@ def test_server_results_equal_alone_results():
         elif kind == "probability":
             assert val.real == s.probability(x, y)
         else:
-            assert abs(val.real - s.marginal(x, meas)) < 1e-12
+            assert val.real == s.marginal(x, meas)
         assert lat > 0
     st = srv.stats()
     assert st["completed"] == len(tickets) and st["failed"] == 0
@@ -114,3 +114,18 @@ def test_server_tiny_budget_evicts_and_stays_correct():
         if t in got:
             assert got[t] == complex(*pq.Simulator(c, strategy="cached").amplitude(x, y))
     srv.close()
+
+
+def test_large_batch_reduction_equals_fused_small_batches_bitwise():
+    # A batch above the fused-reduction size (here 320 queries x 16 items of
+    # the n=40 workload circuit) takes the separate reduction launch; small
+    # batches finish queries inside the walk.  Same reduction order: same bits.
+    c, _, _ = W.config2(5)
+    s = pq.Simulator(c, strategy="cached")
+    rng = W.Rng(123)
+    xs = [rng.below(1 << 40) for _ in range(320)]
+    ys = [0] * 320
+    big = s.amplitudes(xs, ys)
+    assert pq.last_timing()["items"] > 4096
+    small = np.concatenate([s.amplitudes(xs[i:i + 8], ys[i:i + 8]) for i in range(0, 320, 8)])
+    np.testing.assert_array_equal(big, small)
