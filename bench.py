@@ -115,6 +115,8 @@ def headline(args, pq, W, np):
     branches_per_step = nq * (1 << d)
     points_per_step = nq * (1 << seg["outer"])
 
+    # lane walk length autotuned for this batch shape (16 queries of one circuit)
+    tuned = pq.autotune([sim] * nq, xs, ys, repeats=3, cache=False)[:2]
     for _ in range(args.warmup):
         sim.amplitudes(xs, ys)
     kern_ms = []
@@ -129,7 +131,7 @@ def headline(args, pq, W, np):
     res = {
         "sim": sim, "c": c, "xs": xs, "ys": ys, "st": st, "info": info, "seg": seg, "d": d, "nq": nq, "got": got,
         "branches_per_step": branches_per_step, "elapsed": elapsed, "kernel_avg_ms": kernel_avg_ms,
-        "timing": timing, "dev": dev, "tuning": list(pq.get_walk_tuning()),
+        "timing": timing, "dev": dev, "tuning": list(tuned),
     }
     # Roofline of k_seg_walk on this launch (16 queries: 2^20 outer points,
     # too few lanes to fill 256 CUs); the saturating launch is measured by

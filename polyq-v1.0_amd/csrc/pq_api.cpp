@@ -96,7 +96,9 @@ void build_jobs(size_t n, int walk, const std::function<const RankEng*(size_t)>&
     std::vector<std::vector<Z8Job>> cj(pool.threads());
     std::vector<std::vector<size_t>> co(pool.threads());
     std::vector<std::string> err(pool.threads());
-    const unsigned nch = pool.run(n, 64, [&](size_t lo, size_t hi, unsigned c) {
+    // at least 4 queries per chunk (a prologue is about 1 us): small batches
+    // use the polling workers, large ones the whole pool
+    const unsigned nch = pool.run(n, 4, [&](size_t lo, size_t hi, unsigned c) {
         try {
             std::vector<std::pair<uint32_t, uint8_t>> asg;
             for (size_t i = lo; i < hi; ++i)

@@ -1,14 +1,19 @@
-// A small fixed pool of host threads for the per-query prologues of large
-// batches.  run(n, min_chunk, fn) splits [0, n) into contiguous chunks, calls
+// A small fixed pool of host threads for the per-query prologues of batches.
+// run(n, min_chunk, fn) splits [0, n) into contiguous chunks, calls
 // fn(lo, hi, chunk_index) for each (the caller takes part) and returns the
 // number of chunks; chunk c covers a range that precedes chunk c + 1, so
 // results gathered per chunk and concatenated in chunk order are in query
 // order whatever the thread timing.  One parallel region runs at a time; a
-// caller that finds the pool busy runs its chunks itself.
+// caller that finds the pool busy runs its chunks itself.  The first few
+// workers poll for a new region for a short while after each one before they
+// block, so a caller that issues small batches back to back does not pay a
+// thread wake-up per batch; the others block at once.
 #pragma once
 #include <algorithm>
 #include <atomic>
+#include <chrono>
 #include <condition_variable>
+#include <cstdlib>
 #include <functional>
 #include <memory>
 #include <mutex>
@@ -35,13 +40,13 @@ public:
         {
             std::lock_guard<std::mutex> lk(mu_);
             current_ = task;
+            gen_.fetch_add(1, std::memory_order_release);
         }
-        cv_.notify_all();
+        // polling workers see the new generation by themselves; wake at most
+        // as many blocked ones as there are chunks for others to take
+        for (unsigned i = 1; i < want; ++i) cv_.notify_one();
         work(*task);
-        {
-            std::unique_lock<std::mutex> lk(task->mu);
-            task->cv.wait(lk, [&] { return task->done == task->chunks; });
-        }
+        while (task->done.load(std::memory_order_acquire) != task->chunks) std::this_thread::yield();
         std::lock_guard<std::mutex> lk(mu_);
         if (current_ == task) current_.reset();
         return want;
@@ -52,41 +57,51 @@ private:
         const std::function<void(size_t, size_t, unsigned)>* fn = nullptr;
         size_t n = 0;
         unsigned chunks = 0;
-        std::atomic<unsigned> next{0};
-        std::mutex mu;
-        std::condition_variable cv;
-        unsigned done = 0;
+        std::atomic<unsigned> next{0}, done{0};
     };
+    // POLYQ_AMD_HOST_THREADS sets the pool size (caller included), default
+    // min(16, hardware threads); 1 runs every prologue on the calling thread.
     ChunkPool() {
         unsigned hw = std::thread::hardware_concurrency();
-        nworkers_ = std::min<unsigned>(hw ? hw : 1, 16) - 1;
-        for (unsigned i = 0; i < nworkers_; ++i) std::thread([this] { loop(); }).detach();
+        unsigned n = std::min<unsigned>(hw ? hw : 1, 16);
+        if (const char* e = std::getenv("POLYQ_AMD_HOST_THREADS")) {
+            const long v = std::strtol(e, nullptr, 10);
+            if (v >= 1 && v <= 256) n = (unsigned)v;
+        }
+        nworkers_ = n - 1;
+        for (unsigned i = 0; i < nworkers_; ++i) std::thread([this, i] { loop(i < kHot); }).detach();
     }
     static void work(Task& t) {
         for (;;) {
             const unsigned c = t.next.fetch_add(1);
             if (c >= t.chunks) return;
             (*t.fn)(t.n * c / t.chunks, t.n * (c + 1) / t.chunks, c);
-            std::lock_guard<std::mutex> lk(t.mu);
-            if (++t.done == t.chunks) t.cv.notify_all();
+            t.done.fetch_add(1, std::memory_order_release);
         }
     }
-    void loop() {
-        std::shared_ptr<Task> last;
+    static constexpr unsigned kHot = 3;
+    void loop(bool hot) {
+        uint64_t seen = 0;
         for (;;) {
+            // hot workers poll for up to 200 us, then block
+            const auto t0 = std::chrono::steady_clock::now();
+            while (hot && gen_.load(std::memory_order_acquire) == seen &&
+                   std::chrono::steady_clock::now() - t0 < std::chrono::microseconds(200))
+                std::this_thread::yield();
             std::shared_ptr<Task> t;
             {
                 std::unique_lock<std::mutex> lk(mu_);
-                cv_.wait(lk, [&] { return current_ && current_ != last; });
+                cv_.wait(lk, [&] { return gen_.load(std::memory_order_relaxed) != seen; });
+                seen = gen_.load(std::memory_order_relaxed);
                 t = current_;
             }
-            last = t;
-            work(*t);
+            if (t) work(*t);
         }
     }
     unsigned nworkers_ = 0;
     std::mutex region_mu_, mu_;
     std::condition_variable cv_;
+    std::atomic<uint64_t> gen_{0};
     std::shared_ptr<Task> current_;
 };
 

@@ -39,8 +39,9 @@ lib.pq_sim_num_qubits.restype = ctypes.c_uint32
 lib.pq_sim_amplitude_index.argtypes = [_sim_p, ctypes.c_uint64, ctypes.c_uint64, ctypes.c_int, _f64p]
 lib.pq_sim_probability.argtypes = [_sim_p, ctypes.c_uint64, ctypes.c_uint64, ctypes.c_int, _f64p]
 lib.pq_sim_marginal_probability.argtypes = [_sim_p, ctypes.c_uint64, _u32p, _u8p, ctypes.c_size_t, ctypes.c_int, _f64p]
-lib.pq_sim_amplitudes.argtypes = [_sim_p, ctypes.c_size_t, _u64p, _u64p, ctypes.c_int, _f64p]
-lib.pq_batch_amplitudes.argtypes = [ctypes.c_size_t, ctypes.POINTER(_sim_p), _u64p, _u64p, ctypes.c_int, _f64p]
+# hot paths take raw addresses (ndarray.ctypes.data) rather than typed pointers
+lib.pq_sim_amplitudes.argtypes = [_sim_p, ctypes.c_size_t, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p]
+lib.pq_batch_amplitudes.argtypes = [ctypes.c_size_t, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p]
 lib.pq_last_timing.argtypes = [_f64p]
 lib.pq_set_walk_tuning.argtypes = [ctypes.c_int, ctypes.c_uint32, ctypes.c_uint32]
 lib.pq_get_walk_tuning.argtypes = [ctypes.c_int, _u32p]
@@ -223,9 +224,9 @@ class Simulator:
     def amplitudes(self, inputs, outputs, form=None):
         xs = np.ascontiguousarray(inputs, dtype=np.uint64)
         ys = np.ascontiguousarray(outputs, dtype=np.uint64)
-        out = np.zeros(2 * len(xs), dtype=np.float64)
-        _check(lib.pq_sim_amplitudes(self._h, len(xs), _ptr(xs, _u64p), _ptr(ys, _u64p), self._form(form), _ptr(out, _f64p)))
-        return out[0::2] + 1j * out[1::2]
+        out = np.empty(len(xs), dtype=np.complex128)
+        _check(lib.pq_sim_amplitudes(self._h, len(xs), xs.ctypes.data, ys.ctypes.data, self._form(form), out.ctypes.data))
+        return out
 
     def stats(self):
         """(free_vars, planes, lines, radical, terms, branch_dim, pairs) (pybind.rs:177-180)."""
@@ -304,9 +305,9 @@ def batch_amplitudes(sims, inputs, outputs, form="z8"):
     hs = (_sim_p * n)(*[s.handle for s in sims])
     xs = np.ascontiguousarray(inputs, dtype=np.uint64)
     ys = np.ascontiguousarray(outputs, dtype=np.uint64)
-    out = np.zeros(2 * n, dtype=np.float64)
-    _check(lib.pq_batch_amplitudes(n, hs, _ptr(xs, _u64p), _ptr(ys, _u64p), FORMS[form], _ptr(out, _f64p)))
-    return out[0::2] + 1j * out[1::2]
+    out = np.empty(n, dtype=np.complex128)
+    _check(lib.pq_batch_amplitudes(n, ctypes.cast(hs, ctypes.c_void_p), xs.ctypes.data, ys.ctypes.data, FORMS[form], out.ctypes.data))
+    return out
 
 
 class Graph:
