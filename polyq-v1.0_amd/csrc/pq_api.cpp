@@ -159,9 +159,7 @@ std::complex<double> plain_eval(const RankEng& R, const std::vector<std::pair<ui
 }
 
 bool Sim::assignment(uint64_t x, uint64_t y, std::vector<std::pair<uint32_t, uint8_t>>& asg) const {
-    std::vector<uint8_t> xb(ps.nq), yb(ps.nq);
-    for (uint32_t q = 0; q < ps.nq; ++q) { xb[q] = (x >> q) & 1; yb[q] = (y >> q) & 1; }
-    return ps.boundary(xb.data(), yb.data(), asg);
+    return ps.boundary_bits(x, y, asg);
 }
 
 const RankEng& Sim::marginal_engine(const std::vector<uint32_t>& qubits) {
@@ -181,18 +179,12 @@ const RankEng& Sim::marginal_engine(const std::vector<uint32_t>& qubits) {
 }
 
 bool Sim::marginal_assignment(uint64_t x, const uint32_t* qubits, const uint8_t* outcomes, size_t k, std::vector<std::pair<uint32_t, uint8_t>>& a) {
-    // Sandwich::assignment (sim.rs:106-124)
-    a.clear();
-    auto put = [&](uint32_t var, uint8_t val) {
-        for (auto& e : a) if (e.first == var) return e.second == val;
-        a.push_back({var, val});
-        return true;
-    };
-    const PathSum& s = sw->ps;
-    for (uint32_t q = 0; q < s.nq; ++q) if (!put(q, (x >> q) & 1)) return false;
-    for (uint32_t q = 0; q < s.nq; ++q) if (!put(s.out_var[q], (uint8_t)(((x >> q) & 1) ^ s.out_neg[q]))) return false;
-    for (size_t i = 0; i < k; ++i) if (!put(sw->junction[qubits[i]], (uint8_t)((outcomes[i] ? 1 : 0) ^ sw->jneg[qubits[i]]))) return false;
-    return true;
+    // Sandwich::assignment (sim.rs:106-124): x on both ends, the measured
+    // outcomes on the junction variables
+    std::vector<uint32_t> jv(k);
+    std::vector<uint8_t> jval(k);
+    for (size_t i = 0; i < k; ++i) { jv[i] = sw->junction[qubits[i]]; jval[i] = (uint8_t)((outcomes[i] ? 1 : 0) ^ sw->jneg[qubits[i]]); }
+    return sw->ps.boundary_bits(x, x, a, k, jv.data(), jval.data());
 }
 
 }  // namespace pq

@@ -178,16 +178,36 @@ void PathSum::substitute(uint32_t a, uint32_t b, bool c) {
     if (l >= 0) { v[a] = (uint8_t)((v[a] + l) & 3); theta[a] = 0; }
 }
 
-bool PathSum::boundary(const uint8_t* x, const uint8_t* y, std::vector<std::pair<uint32_t, uint8_t>>& out) const {
+// (var, value) pairs put into out; false if a variable gets two values.
+// extra_var / extra_val append k more pairs (the marginal junction).
+bool PathSum::boundary_bits(uint64_t x, uint64_t y, std::vector<std::pair<uint32_t, uint8_t>>& out, size_t k, const uint32_t* extra_var,
+                            const uint8_t* extra_val) const {
     out.clear();
+    // value already given to a variable: 0 / 1, or 255 if none yet
+    thread_local std::vector<uint8_t> given_tls;
+    std::vector<uint8_t>& given_v = given_tls;          // one TLS lookup per call
+    if (given_v.size() < nv) given_v.assign(nv, 255);
+    uint8_t* given = given_v.data();
+    out.reserve(2 * nq + k);
+    bool ok = true;
     auto put = [&](uint32_t var, uint8_t val) {
-        for (auto& e : out) if (e.first == var) return e.second == val;
+        if (given[var] != 255) { ok = ok && given[var] == val; return; }
+        given[var] = val;
         out.push_back({var, val});
-        return true;
     };
-    for (uint32_t q = 0; q < nq; ++q) if (!put(q, x[q] ? 1 : 0)) return false;
-    for (uint32_t q = 0; q < nq; ++q) if (!put(out_var[q], (uint8_t)((y[q] ? 1 : 0) ^ out_neg[q]))) return false;
-    return true;
+    for (uint32_t q = 0; q < nq; ++q) put(q, (uint8_t)((x >> q) & 1));
+    for (uint32_t q = 0; q < nq; ++q) put(out_var[q], (uint8_t)(((y >> q) & 1) ^ out_neg[q]));
+    for (size_t i = 0; i < k; ++i) put(extra_var[i], extra_val[i]);
+    for (auto& e : out) given[e.first] = 255;
+    if (!ok) out.clear();
+    return ok;
+}
+
+bool PathSum::boundary(const uint8_t* x, const uint8_t* y, std::vector<std::pair<uint32_t, uint8_t>>& out) const {
+    if (nq > 64) throw std::invalid_argument("at most 64 qubits");
+    uint64_t xb = 0, yb = 0;
+    for (uint32_t q = 0; q < nq; ++q) { xb |= (uint64_t)(x[q] ? 1 : 0) << q; yb |= (uint64_t)(y[q] ? 1 : 0) << q; }
+    return boundary_bits(xb, yb, out, 0, nullptr, nullptr);
 }
 
 }  // namespace pq

@@ -346,7 +346,8 @@ bool prepare(const HostEngine& E, const uint8_t* values, Prepared& P) {
     // Z4 value of the fixed assignment: linear part plus 2 per adjacent pair of
     // ones.  b_fixed is symmetric with a zero diagonal, so twice the pair count
     // is sum over ones a of |B_a & ones|.
-    thread_local std::vector<uint64_t> ones;
+    thread_local std::vector<uint64_t> ones_tls;
+    std::vector<uint64_t>& ones = ones_tls;             // one TLS lookup per call
     const uint32_t nfw = E.nfw;
     ones.assign(nfw, 0);
     for (uint32_t i = 0; i < E.nf; ++i) ones[i >> 6] |= (uint64_t)(values[i] & 1) << (i & 63);
@@ -366,7 +367,8 @@ bool prepare(const HostEngine& E, const uint8_t* values, Prepared& P) {
     }
     P.phase = th + z4(z);
     // kappa = w_e + u0_e; particular solution of K s = kappa (engine.rs:188-201)
-    thread_local Bits kappa;
+    thread_local Bits kappa_tls;
+    Bits& kappa = kappa_tls;
     if (kappa.n != L.neven) kappa = Bits(L.neven);
     for (uint32_t k = 0; k < L.We; ++k) kappa.w[k] = E.diag[L.off_e() + k] ^ P.u0[L.off_e() + k];
     for (auto& r : E.consistency) if (r.dot(kappa)) return false;
@@ -395,7 +397,8 @@ bool prepare(const HostEngine& E, const uint8_t* values, Prepared& P) {
 }
 
 bool prepare_assignment(const HostEngine& E, const std::vector<std::pair<uint32_t, uint8_t>>& asg, Prepared& P) {
-    thread_local std::vector<uint8_t> vals;
+    thread_local std::vector<uint8_t> vals_tls;
+    std::vector<uint8_t>& vals = vals_tls;
     vals.assign(E.nf, 0);
     size_t seen = 0;
     for (auto& a : asg) {

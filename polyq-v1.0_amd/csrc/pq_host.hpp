@@ -87,6 +87,9 @@ struct PathSum {
     // Boundary (var, value) pairs for <y|C|x>; false if contradictory
     // (pathsum.rs:430-450).
     bool boundary(const uint8_t* x, const uint8_t* y, std::vector<std::pair<uint32_t, uint8_t>>& out) const;
+    // Same from basis indices (bit q = qubit q), with k extra (var, value) pairs.
+    bool boundary_bits(uint64_t x, uint64_t y, std::vector<std::pair<uint32_t, uint8_t>>& out, size_t k = 0,
+                       const uint32_t* extra_var = nullptr, const uint8_t* extra_val = nullptr) const;
 };
 PathSum compile(const Circuit& c);                                          // pathsum.rs:274-297
 struct Sandwich { PathSum ps; std::vector<uint32_t> junction; std::vector<uint8_t> jneg; };
