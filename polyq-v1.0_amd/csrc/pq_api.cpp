@@ -74,23 +74,45 @@ void add_query_jobs(const RankEng& R, const std::vector<std::pair<uint32_t, uint
     }
 }
 
+// Same for <y|C|x> on amplitude engines, straight from the basis indices.
+void add_query_jobs_xy(const RankEng& R, uint64_t x, uint64_t y, int walk, std::vector<Z8Job>& jobs, std::vector<size_t>& owner, size_t qid) {
+    thread_local Prepared P;
+    for (size_t b = 0; b < R.host.size(); ++b) {
+        bool contradictory = false;
+        if (!prepare_bits(*R.host[b], R.bprog, x, y, P, &contradictory)) {
+            if (contradictory) return;          // exactly 0 on every branch engine
+            continue;
+        }
+        Z8Job J;
+        J.eng = R.dev[b].get();
+        if (walk == kWalkSeg) J.sq = seg_query(*R.host[b], *R.seg[b], P);
+        else J.q = z8_query(*R.host[b], *R.z8[b], P);
+        J.scale = P.scale;
+        J.mid_bits = 0;
+        jobs.push_back(J);
+        owner.push_back(qid);
+    }
+}
+
 void add_jobs(const RankEng& R, const std::vector<std::pair<uint32_t, uint8_t>>& asg, int walk, std::vector<Z8Job>& jobs,
               std::vector<size_t>& owner, size_t qid) {
     R.ensure_device();
     add_query_jobs(R, asg, walk, jobs, owner, qid);
 }
 
-// Jobs of n queries, query i on the engine set rank_of(i) with the boundary
-// assignment assign(i, asg) (false: the amplitude is exactly 0).  Large
-// batches run their prologues on the host thread pool; the jobs come out in
-// query order either way.
-void build_jobs(size_t n, int walk, const std::function<const RankEng*(size_t)>& rank_of,
-                const std::function<bool(size_t, std::vector<std::pair<uint32_t, uint8_t>>&)>& assign, std::vector<Z8Job>& jobs,
-                std::vector<size_t>& owner) {
+// Jobs of n queries <ys[i]|C_i|xs[i]>, query i on the amplitude engine set
+// rank_of(i).  Large batches run their prologues on the host thread pool; the
+// jobs come out in query order either way.
+void build_jobs(size_t n, int walk, const std::function<const RankEng*(size_t)>& rank_of, const std::function<void(size_t)>& check,
+                const uint64_t* xs, const uint64_t* ys, std::vector<Z8Job>& jobs, std::vector<size_t>& owner) {
     const RankEng* last = nullptr;
     for (size_t i = 0; i < n; ++i) {
         const RankEng* R = rank_of(i);
-        if (R != last) { R->ensure_device(); last = R; }
+        if (R != last) {
+            if (!R->has_bprog) throw std::logic_error("engine set without a boundary map");
+            R->ensure_device();
+            last = R;
+        }
     }
     ChunkPool& pool = ChunkPool::get();
     std::vector<std::vector<Z8Job>> cj(pool.threads());
@@ -100,9 +122,10 @@ void build_jobs(size_t n, int walk, const std::function<const RankEng*(size_t)>&
     // use the polling workers, large ones the whole pool
     const unsigned nch = pool.run(n, 4, [&](size_t lo, size_t hi, unsigned c) {
         try {
-            std::vector<std::pair<uint32_t, uint8_t>> asg;
-            for (size_t i = lo; i < hi; ++i)
-                if (assign(i, asg)) add_query_jobs(*rank_of(i), asg, walk, cj[c], co[c], i);
+            for (size_t i = lo; i < hi; ++i) {
+                check(i);
+                add_query_jobs_xy(*rank_of(i), xs[i], ys[i], walk, cj[c], co[c], i);
+            }
         } catch (std::exception& e) {
             err[c] = e.what();
         }
@@ -224,6 +247,8 @@ int pq_sim_create(uint32_t nq, size_t ng, const uint8_t* ops, const uint32_t* q0
         h->s.strategy = strategy; h->s.pairs_opt = pairs;
         h->s.ps = compile(h->s.circ);
         h->s.eng = build_rank(h->s.ps, h->s.ps.io_vars(), strategy, pairs);
+        h->s.eng->bprog = boundary_prog(h->s.ps, *h->s.eng->host[0]);
+        h->s.eng->has_bprog = true;
         *out = h.release();
         return 0;
     })
@@ -256,10 +281,9 @@ int pq_sim_amplitudes(pq_sim* h, size_t n, const uint64_t* xs, const uint64_t* y
             std::vector<Z8Job> jobs;
             std::vector<size_t> owner;
             const RankEng* R = h->s.eng.get();
-            build_jobs(n, walk, [&](size_t) { return R; }, [&](size_t i, std::vector<std::pair<uint32_t, uint8_t>>& a) {
+            build_jobs(n, walk, [&](size_t) { return R; }, [&](size_t i) {
                 if (!qubits_ok(h, xs[i]) || !qubits_ok(h, ys[i])) throw std::invalid_argument("basis index has bits beyond the qubit count");
-                return h->s.assignment(xs[i], ys[i], a);   // false: contradictory boundary, exactly 0
-            }, jobs, owner);
+            }, xs, ys, jobs, owner);
             run_jobs(jobs, owner, n, walk, res);
         }
         for (size_t i = 0; i < n; ++i) { out[2 * i] = res[i].real(); out[2 * i + 1] = res[i].imag(); }
@@ -308,11 +332,9 @@ int pq_batch_amplitudes(size_t n, pq_sim* const* sims, const uint64_t* xs, const
     PQ_TRY({
         const int walk = walk_of_form(form);
         std::vector<Z8Job> jobs; std::vector<size_t> owner;
-        build_jobs(n, walk, [&](size_t i) { return (const RankEng*)sims[i]->s.eng.get(); },
-                   [&](size_t i, std::vector<std::pair<uint32_t, uint8_t>>& a) {
-                       if (!qubits_ok(sims[i], xs[i]) || !qubits_ok(sims[i], ys[i])) throw std::invalid_argument("basis index has bits beyond the qubit count");
-                       return sims[i]->s.assignment(xs[i], ys[i], a);
-                   }, jobs, owner);
+        build_jobs(n, walk, [&](size_t i) { return (const RankEng*)sims[i]->s.eng.get(); }, [&](size_t i) {
+            if (!qubits_ok(sims[i], xs[i]) || !qubits_ok(sims[i], ys[i])) throw std::invalid_argument("basis index has bits beyond the qubit count");
+        }, xs, ys, jobs, owner);
         std::vector<std::complex<double>> res;
         run_jobs(jobs, owner, n, walk, res);
         for (size_t i = 0; i < n; ++i) { out[2 * i] = res[i].real(); out[2 * i + 1] = res[i].imag(); }
@@ -345,11 +367,9 @@ int pq_graph_create(size_t n, pq_sim* const* sims, const uint64_t* xs, const uin
         auto g = std::make_unique<pq_graph>();
         std::vector<Z8Job> jobs;
         for (size_t i = 0; i < n; ++i) g->keep.push_back(sims[i]->s.eng);
-        build_jobs(n, walk, [&](size_t i) { return (const RankEng*)sims[i]->s.eng.get(); },
-                   [&](size_t i, std::vector<std::pair<uint32_t, uint8_t>>& a) {
-                       if (!qubits_ok(sims[i], xs[i]) || !qubits_ok(sims[i], ys[i])) throw std::invalid_argument("basis index has bits beyond the qubit count");
-                       return sims[i]->s.assignment(xs[i], ys[i], a);
-                   }, jobs, g->owner);
+        build_jobs(n, walk, [&](size_t i) { return (const RankEng*)sims[i]->s.eng.get(); }, [&](size_t i) {
+            if (!qubits_ok(sims[i], xs[i]) || !qubits_ok(sims[i], ys[i])) throw std::invalid_argument("basis index has bits beyond the qubit count");
+        }, xs, ys, jobs, g->owner);
         if (jobs.empty()) throw std::invalid_argument("every query of the graph is identically zero");
         assign_mid_bits(jobs, walk);
         g->nq = n;
@@ -459,6 +479,28 @@ int pq_sim_marginal_stats(pq_sim* h, const uint32_t* qubits, size_t k, uint64_t*
         const RankEng& R = h->s.marginal_engine(qs);
         EngineStats s = R.host[0]->stats();
         o[0] = s.free_vars; o[1] = s.planes; o[2] = s.lines; o[3] = s.radical; o[4] = s.terms; o[5] = s.branch_dim; o[6] = R.pairs;
+        return 0;
+    })
+}
+
+int pq_sim_prologue_selfcheck(pq_sim* h, size_t n, const uint64_t* xs, const uint64_t* ys, uint64_t* bad) {
+    PQ_TRY({
+        *bad = 0;
+        const RankEng& R = *h->s.eng;
+        std::vector<std::pair<uint32_t, uint8_t>> asg;
+        Prepared A, B;
+        for (size_t i = 0; i < n; ++i) {
+            const bool ok = h->s.assignment(xs[i], ys[i], asg);
+            for (size_t b = 0; b < R.host.size(); ++b) {
+                bool contra = false;
+                const bool nb = prepare_bits(*R.host[b], R.bprog, xs[i], ys[i], B, &contra);
+                if (!ok) { if (!contra || nb) (*bad)++; continue; }
+                const bool na = prepare_assignment(*R.host[b], asg, A);
+                if (contra || na != nb) { (*bad)++; continue; }
+                if (!na) continue;
+                if (A.phase != B.phase || A.s0 != B.s0 || A.K0 != B.K0 || A.u_start != B.u_start || A.ell != B.ell || A.scale != B.scale) (*bad)++;
+            }
+        }
         return 0;
     })
 }

@@ -18,6 +18,9 @@ struct RankEng {
     mutable std::vector<std::shared_ptr<DevEngine>> dev;
     mutable std::mutex dev_mu;
     uint32_t pairs = 0;
+    // boundary map of the amplitude engines (fixed = the circuit's io vars)
+    bool has_bprog = false;
+    BoundaryProg bprog;
     void ensure_device() const {
         std::lock_guard<std::mutex> lk(dev_mu);
         for (size_t i = dev.size(); i < host.size(); ++i) dev.push_back(dev_engine_upload(host[i], z8[i], seg[i]));
@@ -66,6 +69,7 @@ struct Sim {
 
 void add_jobs(const RankEng& R, const std::vector<std::pair<uint32_t, uint8_t>>& asg, int walk, std::vector<Z8Job>& jobs,
               std::vector<size_t>& owner, size_t qid);
+void add_query_jobs_xy(const RankEng& R, uint64_t x, uint64_t y, int walk, std::vector<Z8Job>& jobs, std::vector<size_t>& owner, size_t qid);
 void assign_mid_bits(std::vector<Z8Job>& jobs, int walk);
 void run_jobs(std::vector<Z8Job>& jobs, const std::vector<size_t>& owner, size_t nq, int walk, std::vector<std::complex<double>>& out);
 // Walk of a batched form (PQ_FORM_Z8 -> segment walk, PQ_FORM_BRANCH_WALK ->

@@ -335,7 +335,65 @@ std::shared_ptr<HostEngine> build_engine(const PathSum& ps, const std::vector<ui
 }
 
 // -------------------------------------------------------------- prologue --
+// The prologue from the fixed values packed as a bit row over the E.nf fixed
+// positions (nfw words).
+static bool prepare_ones(const HostEngine& E, const uint64_t* __restrict__ on, Prepared& P);
+
 bool prepare(const HostEngine& E, const uint8_t* values, Prepared& P) {
+    thread_local std::vector<uint64_t> ones_tls;
+    std::vector<uint64_t>& ones = ones_tls;             // one TLS lookup per call
+    ones.assign(E.nfw, 0);
+    for (uint32_t i = 0; i < E.nf; ++i) ones[i >> 6] |= (uint64_t)(values[i] & 1) << (i & 63);
+    return prepare_ones(E, ones.data(), P);
+}
+
+BoundaryProg boundary_prog(const PathSum& ps, const HostEngine& E) {
+    BoundaryProg B;
+    B.nq = ps.nq;
+    B.in_pos.assign(ps.nq, 0);
+    B.out_pos.assign(ps.nq, -1);
+    B.out_neg.assign(ps.nq, 0);
+    B.out_chk.assign(ps.nq, -1);
+    std::vector<int32_t> owner(ps.nv, -1);   // source of a variable's value: input q (q) or output q (64 + q)
+    for (uint32_t q = 0; q < ps.nq; ++q) {
+        if (E.fixed_pos[q] < 0) throw std::logic_error("input variable is not fixed in the engine");
+        B.in_pos[q] = (uint32_t)E.fixed_pos[q];
+        owner[q] = (int32_t)q;
+    }
+    for (uint32_t q = 0; q < ps.nq; ++q) {
+        const uint32_t v = ps.out_var[q];
+        B.out_neg[q] = ps.out_neg[q];
+        if (owner[v] >= 0) { B.out_chk[q] = owner[v]; continue; }
+        if (E.fixed_pos[v] < 0) throw std::logic_error("output variable is not fixed in the engine");
+        B.out_pos[q] = E.fixed_pos[v];
+        owner[v] = (int32_t)(64 + q);
+    }
+    return B;
+}
+
+bool prepare_bits(const HostEngine& E, const BoundaryProg& B, uint64_t x, uint64_t y, Prepared& P, bool* contradictory) {
+    // pathsum.rs:430-450 with the (variable, value) pairs written straight
+    // into the fixed-position bit row
+    thread_local std::vector<uint64_t> ones_tls;
+    std::vector<uint64_t>& ones = ones_tls;
+    ones.assign(E.nfw, 0);
+    uint64_t* on = ones.data();
+    for (uint32_t q = 0; q < B.nq; ++q) if ((x >> q) & 1) on[B.in_pos[q] >> 6] |= 1ull << (B.in_pos[q] & 63);
+    for (uint32_t q = 0; q < B.nq; ++q) {
+        const uint32_t v = (uint32_t)((y >> q) & 1) ^ B.out_neg[q];
+        const int32_t c = B.out_chk[q];
+        if (c >= 0) {
+            const uint32_t prev = c < 64 ? (uint32_t)((x >> c) & 1) : ((uint32_t)((y >> (c - 64)) & 1) ^ B.out_neg[c - 64]);
+            if (prev != v) { P.nonzero = false; if (contradictory) *contradictory = true; return false; }
+        } else if (v) {
+            on[B.out_pos[q] >> 6] |= 1ull << (B.out_pos[q] & 63);
+        }
+    }
+    if (contradictory) *contradictory = false;
+    return prepare_ones(E, on, P);
+}
+
+static bool prepare_ones(const HostEngine& E, const uint64_t* __restrict__ on, Prepared& P) {
     P.nonzero = false;
     if (E.zero) return false;                                   // engine.rs:570-572
     const ULayout& L = E.L;
@@ -346,13 +404,8 @@ bool prepare(const HostEngine& E, const uint8_t* values, Prepared& P) {
     // Z4 value of the fixed assignment: linear part plus 2 per adjacent pair of
     // ones.  b_fixed is symmetric with a zero diagonal, so twice the pair count
     // is sum over ones a of |B_a & ones|.
-    thread_local std::vector<uint64_t> ones_tls;
-    std::vector<uint64_t>& ones = ones_tls;             // one TLS lookup per call
     const uint32_t nfw = E.nfw;
-    ones.assign(nfw, 0);
-    for (uint32_t i = 0; i < E.nf; ++i) ones[i >> 6] |= (uint64_t)(values[i] & 1) << (i & 63);
     uint64_t* __restrict__ u0 = P.u0.data();
-    const uint64_t* __restrict__ on = ones.data();
     for (uint32_t wi = 0; wi < nfw; ++wi) {
         for (uint64_t x = on[wi]; x; x &= x - 1) {
             const uint32_t f = wi * 64 + (uint32_t)__builtin_ctzll(x);

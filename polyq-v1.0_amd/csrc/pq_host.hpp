@@ -183,6 +183,21 @@ struct Prepared {
     std::complex<double> scale;      // e^{i phase} 2^{log2mag - h/2} E B
 };
 bool prepare(const HostEngine& E, const uint8_t* values, Prepared& P);
+// Boundary of <y|C|x> mapped straight to the engine's fixed positions
+// (pathsum.rs:430-450): input qubit q fixes position in_pos[q] to x_q; output
+// qubit q fixes out_pos[q] to y_q ^ out_neg[q], or, when its variable already
+// has a value (out_chk[q] = q' for input q', 64 + q' for output q'), must
+// agree with it.
+struct BoundaryProg {
+    uint32_t nq = 0;
+    std::vector<uint32_t> in_pos;
+    std::vector<int32_t> out_pos, out_chk;
+    std::vector<uint8_t> out_neg;
+};
+BoundaryProg boundary_prog(const PathSum& ps, const HostEngine& E);
+// prepare() for <y|C|x> from basis indices; *contradictory tells a
+// contradictory boundary from a vanishing or inconsistent engine.
+bool prepare_bits(const HostEngine& E, const BoundaryProg& B, uint64_t x, uint64_t y, Prepared& P, bool* contradictory = nullptr);
 // Same, with a (var, value) assignment covering every fixed variable.
 bool prepare_assignment(const HostEngine& E, const std::vector<std::pair<uint32_t, uint8_t>>& asg, Prepared& P);
 

@@ -138,6 +138,8 @@ struct Server {
             s->strategy = r.strategy;
             s->ps = compile(s->circ);
             s->eng = build_rank(s->ps, s->ps.io_vars(), r.strategy, 0);
+            s->eng->bprog = boundary_prog(s->ps, *s->eng->host[0]);
+            s->eng->has_bprog = true;
             E->sim = std::move(s);
             E->built = true;
             std::lock_guard<std::mutex> lk(mu);
@@ -201,7 +203,8 @@ struct Server {
                     if (S.marginal_assignment(r->x, r->mq.data(), r->mo.data(), r->mq.size(), asg)) add_jobs(R, asg, walk, r->jobs, owner, 0);
                 } else {
                     make_resident(r->entry, *S.eng);
-                    if (S.assignment(r->x, r->y, asg)) add_jobs(*S.eng, asg, walk, r->jobs, owner, 0);
+                    S.eng->ensure_device();
+                    add_query_jobs_xy(*S.eng, r->x, r->y, walk, r->jobs, owner, 0);
                 }
                 for (auto& J : r->jobs) {
                     const uint32_t d = dev_z8(*J.eng).d;

@@ -48,6 +48,7 @@ lib.pq_get_walk_tuning.argtypes = [ctypes.c_int, _u32p]
 lib.pq_set_segment_bits.argtypes = [ctypes.c_uint32]
 lib.pq_get_segment_bits.restype = ctypes.c_uint32
 lib.pq_sim_seg_info.argtypes = [_sim_p, _u64p]
+lib.pq_sim_prologue_selfcheck.argtypes = [_sim_p, ctypes.c_size_t, _u64p, _u64p, _u64p]
 lib.pq_sim_seg_dump.argtypes = [_sim_p, ctypes.c_uint64, ctypes.c_uint64, ctypes.c_uint32, ctypes.c_size_t, _u8p, _u32p, _u32p, ctypes.POINTER(ctypes.c_size_t)]
 lib.pq_sim_seg_lane_ref.argtypes = lib.pq_sim_seg_dump.argtypes
 lib.pq_sim_seg_selfcheck.argtypes = [_sim_p, ctypes.c_uint64, ctypes.c_uint64, ctypes.c_uint32, ctypes.c_uint64, _f64p]
@@ -266,6 +267,14 @@ class Simulator:
     def seg_lane_ref(self, x, y, mid_bits=0):
         """Host restatement of seg_dump."""
         return self._seg_call(lib.pq_sim_seg_lane_ref, x, y, mid_bits)
+
+    def prologue_selfcheck(self, inputs, outputs):
+        """Host-only: mismatches between the direct and the assignment prologue."""
+        xs = np.ascontiguousarray(inputs, dtype=np.uint64)
+        ys = np.ascontiguousarray(outputs, dtype=np.uint64)
+        bad = np.zeros(1, dtype=np.uint64)
+        _check(lib.pq_sim_prologue_selfcheck(self._h, len(xs), _ptr(xs, _u64p), _ptr(ys, _u64p), _ptr(bad, _u64p)))
+        return int(bad[0])
 
     def seg_selfcheck(self, x, y, trials=8, seed=1):
         """Host-only: max |table segment - direct inner sum| / inner magnitude mass."""

@@ -179,3 +179,27 @@ def test_segment_lane_state_matches_quadratic_form():
                 assert (K1[idx], H1[idx], P1[idx]) == (K0[cc], H0[cc], P0[cc])
     finally:
         pq.set_segment_bits(old)
+
+
+def test_direct_prologue_matches_assignment_prologue():
+    # the batched forms map (x, y) straight to the fixed positions; the plain
+    # form goes through (variable, value) pairs as the reference does
+    rng = W.Rng(1234)
+    checked = 0
+    for trial in range(60):
+        n = 1 + rng.below(6)
+        if trial % 3 == 0:
+            c = W.random_clifford_t(n, 10 + rng.below(40), rng.below(12), trial % 2 == 0, rng)
+        elif trial % 3 == 1:
+            c = W.random_layered(n, 1 + rng.below(3), rng)
+        else:
+            c = W.uniform_gates(n, 5 + rng.below(30), rng) if n > 1 else W.random_layered(n, 2, rng)
+        for kw in ({"strategy": "cached"}, {"pairs": 2}):
+            s = pq.Simulator(c, **kw)
+            xs = [x for x in range(1 << n) for _ in range(1 << n)]
+            ys = [y for _ in range(1 << n) for y in range(1 << n)]
+            assert s.prologue_selfcheck(xs, ys) == 0, (trial, kw)
+            checked += 1
+    c, xs, ys = W.config2(5)
+    assert pq.Simulator(c, strategy="cached").prologue_selfcheck(xs, ys) == 0
+    assert checked >= 100
