@@ -119,15 +119,22 @@ def headline(args, pq, W, np):
     tuned = pq.autotune([sim] * nq, xs, ys, repeats=3, cache=False)[:2]
     for _ in range(args.warmup):
         sim.amplitudes(xs, ys)
-    kern_ms = []
     t0 = time.perf_counter()
     for _ in range(args.steps):
         got = sim.amplitudes(xs, ys)     # synchronous: returns after the device results are on the host
-        kern_ms.append(pq.last_timing()["kernel_ms"])
     t1 = time.perf_counter()
     elapsed = t1 - t0
+    # Kernel durations: the same steps again with HIP timing events on the
+    # launch stream (the events are queue barriers that lengthen a step, so
+    # the throughput loop above runs without them).
+    kern_ms = []
+    with pq.timed():
+        for _ in range(args.steps):
+            again = sim.amplitudes(xs, ys)
+            kern_ms.append(pq.last_timing()["kernel_ms"])
     kernel_avg_ms = float(np.mean(kern_ms))
     timing = pq.last_timing()
+    assert np.array_equal(again, got)
     res = {
         "sim": sim, "c": c, "xs": xs, "ys": ys, "st": st, "info": info, "seg": seg, "d": d, "nq": nq, "got": got,
         "branches_per_step": branches_per_step, "elapsed": elapsed, "kernel_avg_ms": kernel_avg_ms,
@@ -141,9 +148,10 @@ def headline(args, pq, W, np):
     # The per-branch Z8 walk on the same 16 amplitudes (every branch visited:
     # one sign XOR and one FMA per branch, one more LDS lookup with inner pivots).
     bw = []
-    for _ in range(3):
-        sim.amplitudes(xs, ys, form="branch_walk")
-        bw.append(pq.last_timing()["kernel_ms"])
+    with pq.timed():
+        for _ in range(3):
+            sim.amplitudes(xs, ys, form="branch_walk")
+            bw.append(pq.last_timing()["kernel_ms"])
     bw_ms = float(np.median(bw))
     ga = sim.amplitudes(xs, ys, form="branch_walk")
     peak = _peak(dev)
@@ -217,7 +225,10 @@ def saturated(args, pq, W, np, sim=None):
         tb = time.perf_counter()
         sim.amplitudes(big_x, big_y)
         bwall.append(time.perf_counter() - tb)
-        bms.append(pq.last_timing()["kernel_ms"])
+    with pq.timed():
+        for _ in range(args.saturated_reps):
+            sim.amplitudes(big_x, big_y)
+            bms.append(pq.last_timing()["kernel_ms"])
     tuned = sim.seg_info()["mid_bits"]
     pq.set_walk_tuning(*default)
     return _seg_roofline(dev, 4096 * (1 << seg["outer"]), float(np.median(bms)),
@@ -274,7 +285,9 @@ def batch(args, pq, W, np):
     eager_ms = []
     for _ in range(3):
         t0 = time.perf_counter(); eager = pq.batch_amplitudes(S, X, Y); eager_ms.append(1e3 * (time.perf_counter() - t0))
-    tm = pq.last_timing()
+    with pq.timed():
+        pq.batch_amplitudes(S, X, Y)
+        tm = pq.last_timing()
     g = pq.Graph(S, X, Y)
     reps, gms, wall = 10, [], []
     same = True

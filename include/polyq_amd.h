@@ -106,8 +106,14 @@ int pq_set_segment_bits(uint32_t bits);
 uint32_t pq_get_segment_bits(void);
 
 /* Timing of the last device evaluation on this thread: kernel_ms, total_ms,
- * branches, items, grid (doubles). */
+ * branches, items, grid (doubles).  kernel_ms and total_ms come from HIP
+ * events and are NaN unless timing is on. */
 int pq_last_timing(double out5[5]);
+/* Whether eager evaluations record HIP timing events (no counterpart in the
+ * reference; default off: each event is a queue barrier that adds several
+ * microseconds to a short batch). */
+int pq_set_timing(int on);
+int pq_get_timing(void);
 
 /* Replayable batch captured in a hipGraph (descriptor upload, counter clear,
  * persistent walk, FP64 reduction, result download). */

@@ -114,13 +114,23 @@ void build_jobs(size_t n, int walk, const std::function<const RankEng*(size_t)>&
             last = R;
         }
     }
+    // At least 16 queries per chunk (a prologue takes about 0.3 us, a
+    // hand-off to another thread about as much as a few of them): batches
+    // below two chunks stay on the calling thread.
+    constexpr size_t kChunk = 16;
+    if (n < 2 * kChunk) {
+        jobs.clear(); owner.clear();
+        for (size_t i = 0; i < n; ++i) {
+            check(i);
+            add_query_jobs_xy(*rank_of(i), xs[i], ys[i], walk, jobs, owner, i);
+        }
+        return;
+    }
     ChunkPool& pool = ChunkPool::get();
     std::vector<std::vector<Z8Job>> cj(pool.threads());
     std::vector<std::vector<size_t>> co(pool.threads());
     std::vector<std::string> err(pool.threads());
-    // at least 4 queries per chunk (a prologue is about 1 us): small batches
-    // use the polling workers, large ones the whole pool
-    const unsigned nch = pool.run(n, 4, [&](size_t lo, size_t hi, unsigned c) {
+    const unsigned nch = pool.run(n, kChunk, [&](size_t lo, size_t hi, unsigned c) {
         try {
             for (size_t i = lo; i < hi; ++i) {
                 check(i);
@@ -360,6 +370,8 @@ int pq_set_segment_bits(uint32_t bits) { PQ_TRY({ seg_set_bits(bits); return 0; 
 uint32_t pq_get_segment_bits(void) { return seg_get_bits(); }
 
 int pq_last_timing(double* o) { for (int i = 0; i < 5; ++i) o[i] = g_timing[i]; return 0; }
+int pq_set_timing(int on) { dev_set_timing(on != 0); return 0; }
+int pq_get_timing(void) { return dev_get_timing() ? 1 : 0; }
 
 int pq_graph_create(size_t n, pq_sim* const* sims, const uint64_t* xs, const uint64_t* ys, int form, pq_graph** out) {
     PQ_TRY({

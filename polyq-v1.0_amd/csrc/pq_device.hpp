@@ -38,18 +38,21 @@ struct Z8Job {
     uint32_t mid_bits;            // outer Gray bits per lane (autotuned)
 };
 struct Z8Timing {
-    double kernel_ms = 0;         // walk kernel, HIP events on the launch stream
-    double total_ms = 0;          // copies + launches + reduction, events
+    double kernel_ms = 0;         // walk kernel, HIP events on the launch stream (NaN when not timed)
+    double total_ms = 0;          // copies + launches + reduction, events (NaN when not timed)
     uint64_t branches = 0;        // sum over jobs of 2^d (real coordinates)
     uint64_t items = 0;
     uint32_t grid = 0;
 };
 // Results are scale * (EV + omega * OD) per job, FP64.
+// Eager batches record HIP timing events only while this is on (default off).
+void dev_set_timing(bool on);
+bool dev_get_timing();
 void dev_z8_batch(const std::vector<Z8Job>& jobs, std::vector<std::complex<double>>& out, Z8Timing* timing, int walk);
 // Asynchronous form: a slot holds one batch in flight on its own stream.
 struct Z8Slot;
 std::shared_ptr<Z8Slot> dev_slot_create();
-void dev_z8_launch(Z8Slot& S, const std::vector<Z8Job>& jobs, int walk);
+void dev_z8_launch(Z8Slot& S, const std::vector<Z8Job>& jobs, int walk, bool timed);
 bool dev_z8_ready(Z8Slot& S);
 void dev_z8_wait(Z8Slot& S, std::vector<std::complex<double>>& out, Z8Timing* timing);
 // Mid-bit heuristic: enough items to fill every CU several times, long

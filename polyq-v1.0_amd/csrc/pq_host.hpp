@@ -262,9 +262,10 @@ struct SegImage {
     // change X(o) (table basis, bits 0..23) and the engine part of the phase
     // change Ue(o) = sum c_j lambda_j + 4 sum_{j<j'} c_j c_j' Q_jj' mod 8 (bits 24..26).
     std::vector<uint32_t> wtab;
-    // The walk's points sorted by engine phase class Ue: for M = 0..Mcap, at
-    // entry offset 2^M - 1, 2^M records {X(o), gray(o), Fmid_M(o) hi, lo}, and
-    // class boundaries pcls[9 M + c] (c = 0..8, relative to the M block).
+    // The walk's points sorted by Ue mod 4: for M = 0..Mcap, at entry offset
+    // 2^M - 1, 2^M records {X(o), gray(o) | h << 31, (-1)^h Fmid_M(o) hi, lo}
+    // with h = Ue >> 2 (omega^4 = -1), and class boundaries pcls[5 M + c]
+    // (c = 0..4, relative to the M block).
     std::vector<uint32_t> pts;
     std::vector<uint32_t> pcls;
 };

@@ -28,6 +28,7 @@
 #include <cstring>
 #include <algorithm>
 #include <numeric>
+#include <cmath>
 
 namespace pq {
 
@@ -55,13 +56,16 @@ struct DBuf {
 };
 struct HBuf {
     void* p = nullptr;
+    void* dev = nullptr;     // the device's address of p
     size_t cap = 0;
     void* get(size_t n) {
         if (n > cap) {
             if (p) HIPCHECK(hipHostFree(p));
+            p = dev = nullptr;
             size_t c = std::max<size_t>(n, cap * 2);
             c = std::max<size_t>(c, 4096);
             HIPCHECK(hipHostMalloc(&p, c, hipHostMallocMapped));   // pinned, device-addressable
+            HIPCHECK(hipHostGetDevicePointer(&dev, p, 0));
             cap = c;
         }
         return p;
@@ -145,8 +149,8 @@ struct SegDev {
     const float* fmid;       // walk magnitudes {hi, lo}, M = 0..Mcap at 2(2^M - 1)
     const float* rmid;       // Mcap x {r0 hi, r0 lo, r1 hi, r1 lo}
     const uint32_t* wtab;    // 2^Mcap: sign-mask change | engine phase change << 24
-    const uint32_t* pts;     // walk points sorted by phase class, per M at 4(2^M - 1)
-    const uint32_t* pcls;    // class boundaries, 9 per M
+    const uint32_t* pts;     // walk points sorted by phase class mod 4, per M at 4(2^M - 1)
+    const uint32_t* pcls;    // class boundaries, 5 per M
     uint32_t I, npin, nout, L6, Mcap, nbytes, pad0, pad1;
 };
 struct SegQ {                // one query of a segment batch (64 B; item0 / nitems where Z8Q has them)
@@ -678,17 +682,14 @@ __global__ __launch_bounds__(256) void k_z8_walk(const Z8Dev* __restrict__ engs,
 // thread, then a fixed shuffle and LDS tree, so the result does not depend on
 // which workgroup runs it), EV + omega OD, and the FP64 scale.  All 256
 // threads of the workgroup take part; thread 0 writes the result.
-template <bool ATOMIC_READ>
+// host_query_reduce restates this order exactly.
 __device__ __forceinline__ void query_reduce(uint64_t item0, uint64_t nitems, const double* partial, double2 sc, double* dst,
                                              double (&red)[4][4]) {
     const uint32_t tid = threadIdx.x;
     double v[4] = {0, 0, 0, 0};
     for (uint64_t k = tid; k < nitems; k += 256) {
 #pragma unroll
-        for (int c = 0; c < 4; ++c) {
-            const double* p = partial + 4 * (item0 + k) + c;
-            v[c] += ATOMIC_READ ? __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) : *p;
-        }
+        for (int c = 0; c < 4; ++c) v[c] += partial[4 * (item0 + k) + c];
     }
 #pragma unroll
     for (int off = 32; off > 0; off >>= 1) {
@@ -747,38 +748,57 @@ struct SegSlots {
     P32 ox, opin, olam;
     PF rmid;
 };
+// The lane's outer coordinates c: lane bits below L6 on slots 0..L6-1, the
+// rest on slots L6+M.. (the M walked slots stay 0 at the seed).  Every slot
+// term depends on c alone, so the slots are independent: no carried chain.
+//   K  = K0 + sum_{k in c} (lambda_k + 4 delta_k + 4 |Q_k & c & below(k)|)
+//   hs = hs0 ^ X_k..., pio, s likewise; factor = prod of r_{c_k} over the
+//   walkable slots beyond M.
 template <class P64, class P32, class PF>
 __device__ __forceinline__ void seg_seed(const SegDev& E, const SegSlots<P64, P32, PF>& T, const SegQ& Q, uint64_t lane, SegLaneDev& L) {
-    L.cout = 0; L.s = Q.s0; L.K = Q.K0; L.hs = Q.hs0; L.pio = Q.pio0; L.fh = 1.f; L.fl = 0.f;
-    const uint32_t M = Q.M;
-    for (uint32_t b = 0; b < Q.H; ++b) {
-        const uint32_t k = seg_slot_of_bit(E, M, b);
-        const bool on = (lane >> b) & 1;
-        if (k < E.L6 + E.Mcap && k >= E.L6 + M) {
-            const uint32_t m = 4 * (k - E.L6);
-            const float r0h = T.rmid[m], r0l = T.rmid[m + 1], r1h = T.rmid[m + 2], r1l = T.rmid[m + 3];
-            const float rh = on ? r1h : r0h, rl = on ? r1l : r0l;
-            float ph, pl;
-            df_mul(L.fh, L.fl, rh, rl, ph, pl);
-            L.fh = ph; L.fl = pl;
-        }
-        if (on) {
-            const uint32_t ell = T.olam[k] + (((uint32_t)(Q.delta >> k) & 1u) << 2);
-            L.K += ell + ((__popcll(T.oq[k] & L.cout) & 1u) << 2);
-            L.cout |= 1ull << k;
-            L.hs ^= T.ox[k];
-            L.pio ^= T.opin[k];
-            L.s ^= T.on[k];
-        }
-    }
-}
-// Lane-factor of the terms outside the table and the walk: byte tables in LDS.
-__device__ __forceinline__ void seg_lane_factor(const float2* s_lt, uint32_t nbytes, uint64_t s, float& fh, float& fl) {
-    for (uint32_t b = 0; b < nbytes; ++b) {
-        const float2 v = s_lt[(b << 8) | (uint32_t)((s >> (8 * b)) & 0xFFu)];
+    const uint32_t M = Q.M, L6 = E.L6;
+    const uint64_t c = (lane & ((1ull << L6) - 1ull)) | ((lane >> L6) << (L6 + M));
+    uint32_t K = Q.K0, hs = Q.hs0, pio = Q.pio0;
+    uint64_t s = Q.s0;
+    auto slot = [&](uint32_t k) {
+        const uint32_t on = (uint32_t)(c >> k) & 1u, m = 0u - on;
+        const uint32_t ell = T.olam[k] + (((uint32_t)(Q.delta >> k) & 1u) << 2) +
+                             (((uint32_t)__popcll(T.oq[k] & c & ((1ull << k) - 1ull)) & 1u) << 2);
+        K += ell & m;
+        hs ^= T.ox[k] & m;
+        pio ^= T.opin[k] & m;
+        s ^= T.on[k] & (0ull - (uint64_t)on);
+    };
+#pragma unroll 4
+    for (uint32_t k = 0; k < L6; ++k) slot(k);
+#pragma unroll 4
+    for (uint32_t k = L6 + M; k < E.nout; ++k) slot(k);
+    float fh = 1.f, fl = 0.f;
+    for (uint32_t k = L6 + M; k < L6 + E.Mcap; ++k) {
+        const uint32_t m = 4 * (k - L6) + 2 * ((uint32_t)(c >> k) & 1u);
         float ph, pl;
-        df_mul(fh, fl, v.x, v.y, ph, pl);
+        df_mul(fh, fl, T.rmid[m], T.rmid[m + 1], ph, pl);
         fh = ph; fl = pl;
+    }
+    L.cout = c; L.K = K; L.hs = hs; L.pio = pio; L.s = s; L.fh = fh; L.fl = fl;
+}
+// Lane-factor of the terms outside the table and the walk: byte tables in LDS
+// (8 x 256 entries, so every index is in bounds), read four at a time.
+__device__ __forceinline__ void seg_lane_factor(const float2* s_lt, uint32_t nbytes, uint64_t s, float& fh, float& fl) {
+#pragma unroll
+    for (uint32_t b0 = 0; b0 < 8; b0 += 4) {
+        if (b0 >= nbytes) break;
+        float2 v[4];
+#pragma unroll
+        for (uint32_t j = 0; j < 4; ++j) v[j] = s_lt[((b0 + j) << 8) | (uint32_t)((s >> (8 * (b0 + j))) & 0xFFu)];
+#pragma unroll
+        for (uint32_t j = 0; j < 4; ++j) {
+            if (b0 + j < nbytes) {
+                float ph, pl;
+                df_mul(fh, fl, v[j].x, v[j].y, ph, pl);
+                fh = ph; fl = pl;
+            }
+        }
     }
 }
 // Over a lane's segment of the M walkable slots, at point o (c_w = gray(o)):
@@ -791,10 +811,13 @@ __device__ __forceinline__ void seg_lane_factor(const float2* s_lt, uint32_t nby
 template <class P64>
 __device__ __forceinline__ uint32_t seg_lane_parities(const SegDev& E, P64 oq, uint32_t M, uint64_t cout) {
     uint32_t P = 0;
+#pragma unroll 4
     for (uint32_t j = 0; j < M; ++j) P |= (uint32_t)(__popcll(oq[E.L6 + j] & cout) & 1) << j;
     return P;
 }
 typedef float f32x4 __attribute__((ext_vector_type(4)));
+// Walks of at most 2^kSegShortWalk points issue all their table reads at once.
+constexpr uint32_t kSegShortWalk = 4;
 // A += f * T (double-float real times double-float complex).
 __device__ __forceinline__ void seg_point(float (&A)[4], float fh, float fl, const f32x4& T) {
     float ph, pl, qh, ql;
@@ -811,17 +834,13 @@ __device__ __forceinline__ void df_rot(float (&v)[4], uint32_t r) {
     v[0] = __uint_as_float(__float_as_uint(a0) ^ sre); v[1] = __uint_as_float(__float_as_uint(a1) ^ sre);
     v[2] = __uint_as_float(__float_as_uint(b0) ^ sim); v[3] = __uint_as_float(__float_as_uint(b1) ^ sim);
 }
-// sum_q omega^q A[q] split as EV + omega OD (static rotations by i^m):
-// dst = A[P] + i A[P+2] - A[P+4] - i A[P+6] for P = 0 (EV) and 1 (OD).
+// sum_q omega^q A[q] over q = 0..3 split as EV + omega OD:
+// dst = A[P] + i A[P+2] for P = 0 (EV) and 1 (OD).
 template <int P>
-__device__ __forceinline__ void seg_fold4(const float (&A)[8][4], float (&dst)[4]) {
+__device__ __forceinline__ void seg_fold4(const float (&A)[4][4], float (&dst)[4]) {
     dst[0] = A[P][0]; dst[1] = A[P][1]; dst[2] = A[P][2]; dst[3] = A[P][3];
     df_add(dst[0], dst[1], -A[P + 2][2], -A[P + 2][3]);   // i (x + i y) = -y + i x
     df_add(dst[2], dst[3], A[P + 2][0], A[P + 2][1]);
-    df_add(dst[0], dst[1], -A[P + 4][0], -A[P + 4][1]);   // -1
-    df_add(dst[2], dst[3], -A[P + 4][2], -A[P + 4][3]);
-    df_add(dst[0], dst[1], A[P + 6][2], A[P + 6][3]);     // -i (x + i y) = y - i x
-    df_add(dst[2], dst[3], -A[P + 6][0], -A[P + 6][1]);
 }
 
 // Persistent work-queue kernel over (query, 256-lane block) items.  A lane
@@ -842,75 +861,116 @@ typedef __attribute__((address_space(3))) const float LdsF32;
 // one whose increment of the query's done counter completes it) reduces the
 // query in FP64 and writes the scaled amplitude straight into the caller's
 // pinned host buffer: no separate reduction launch and no result copy.
-__global__ __launch_bounds__(256) void k_seg_walk(const SegDev* __restrict__ engs, const SegQ* __restrict__ qs, const uint32_t* __restrict__ item_q,
-                                                  uint64_t item_end, unsigned long long* __restrict__ counter, double* __restrict__ partial,
-                                                  unsigned* __restrict__ qdone, const double2* __restrict__ scale, const uint32_t* __restrict__ order,
-                                                  double* __restrict__ out, int fused) {
-    __shared__ double s_qred[4][4];
-    __shared__ int s_last;
-    __shared__ float2 s_lt[8 * 256];
-    // the engine's outer-slot arrays, so the lane seed reads LDS, not a chain
-    // of scalar-cache round trips
-    __shared__ uint64_t s_oq[64], s_on[64];
-    __shared__ uint32_t s_ox[64], s_opin[64], s_olam[64];
-    __shared__ float s_rmid[4 * kSegMaxMid];
-    __shared__ unsigned long long s_item;
-    __shared__ int s_eng;
-    __shared__ float s_red[4][8];
+// The LDS a workgroup keeps across the items it walks: the engine's byte
+// tables and outer-slot arrays (so the lane seed reads LDS, not a chain of
+// scalar-cache round trips) and the wave sums.
+struct SegLds {
+    float2* lt;              // 8 x 256
+    uint64_t *oq, *on;       // 64 each
+    uint32_t *ox, *opin, *olam;
+    float* rmid;             // 4 kSegMaxMid
+    float (*red)[8];         // 4 waves
+};
+#define PQ_SEG_LDS(name)                                                   \
+    __shared__ float2 name##_lt[8 * 256];                                  \
+    __shared__ uint64_t name##_oq[64], name##_on[64];                      \
+    __shared__ uint32_t name##_ox[64], name##_opin[64], name##_olam[64];   \
+    __shared__ float name##_rmid[4 * kSegMaxMid];                          \
+    __shared__ float name##_red[4][8];                                     \
+    const SegLds name{name##_lt, name##_oq, name##_on, name##_ox, name##_opin, name##_olam, name##_rmid, name##_red}
+
+// One item (256 lanes of one query): reload the engine tables into LDS if
+// the workgroup's last item was of another engine (eng, uniform), seed every
+// lane, walk its segment, and write the item's four partial sums.
+template <bool SHORT>
+__device__ __forceinline__ void seg_item(const SegLds& S, const SegDev& E, const SegQ& Q, uint64_t item, int& eng, double* __restrict__ partial) {
     const uint32_t tid = threadIdx.x;
-    if (tid == 0) s_eng = -1;
+    float2* s_lt = S.lt;
+    uint64_t *s_oq = S.oq, *s_on = S.on;
+    uint32_t *s_ox = S.ox, *s_opin = S.opin, *s_olam = S.olam;
+    float* s_rmid = S.rmid;
+    float (*s_red)[8] = S.red;
     const SegSlots<LdsU64*, LdsU32*, LdsF32*> T{(LdsU64*)s_oq, (LdsU64*)s_on, (LdsU32*)s_ox, (LdsU32*)s_opin, (LdsU32*)s_olam, (LdsF32*)s_rmid};
-    for (;;) {
-        __syncthreads();
-        if (tid == 0) s_item = atomicAdd(counter, 1ull);
-        __syncthreads();
-        const uint64_t item = s_item;
-        if (item >= item_end) break;
-        const uint32_t qi = cview(item_q)[item];
-        const SegQ Q = cload(qs + qi);
-        const SegDev E = cload(engs + Q.eng);
-        if ((int)Q.eng != s_eng) {
-            for (uint32_t k = tid; k < E.nbytes * 256u; k += 256) s_lt[k] = E.lt[k];
-            if (tid < E.nout) {
-                s_oq[tid] = E.oq[tid]; s_on[tid] = E.on[tid];
-                s_ox[tid] = E.ox[tid]; s_opin[tid] = E.opin[tid]; s_olam[tid] = E.olam[tid];
-            }
-            if (tid < 4 * E.Mcap) s_rmid[tid] = E.rmid[tid];
-            __syncthreads();
-            if (tid == 0) s_eng = (int)Q.eng;
+    if ((int)Q.eng != eng) {
+        for (uint32_t k = tid; k < E.nbytes * 256u; k += 256) s_lt[k] = E.lt[k];
+        if (tid < E.nout) {
+            s_oq[tid] = E.oq[tid]; s_on[tid] = E.on[tid];
+            s_ox[tid] = E.ox[tid]; s_opin[tid] = E.opin[tid]; s_olam[tid] = E.olam[tid];
         }
-        float ev[4] = {0.f, 0.f, 0.f, 0.f}, od[4] = {0.f, 0.f, 0.f, 0.f};
-        const uint64_t lane = (item - Q.item0) * 256 + tid;
-        if ((lane >> Q.H) == 0) {
-            SegLaneDev L;
-            seg_seed(E, T, Q, lane, L);
-            float fh = L.fh, fl = L.fl;
-            seg_lane_factor(s_lt, E.nbytes, L.s, fh, fl);
-            const uint32_t M = Q.M;
-            const uint32_t P = seg_lane_parities(E, T.oq, M, L.cout);
-            const uint32_t dw = (uint32_t)(Q.delta >> E.L6) & ((1u << M) - 1u);
-            const uint32_t base = (L.pio << E.I) | L.hs;
-            GFloat4* tab = (GFloat4*)E.table;
-            const PQ_CONST uint32_t* pt = cview(E.pts) + 4 * ((1u << M) - 1u);
-            const PQ_CONST uint32_t* cl = cview(E.pcls) + 9 * M;
-            float A[8][4];
-            // points grouped by their engine phase class c: the class is a
-            // loop constant, so each group adds into its own accumulator
+        if (tid < 4 * E.Mcap) s_rmid[tid] = E.rmid[tid];
+        __syncthreads();
+        eng = (int)Q.eng;
+    }
+    float ev[4] = {0.f, 0.f, 0.f, 0.f}, od[4] = {0.f, 0.f, 0.f, 0.f};
+    const uint64_t lane = (item - Q.item0) * 256 + tid;
+    if ((lane >> Q.H) == 0) {
+        SegLaneDev L;
+        seg_seed(E, T, Q, lane, L);
+        float fh = L.fh, fl = L.fl;
+        seg_lane_factor(s_lt, E.nbytes, L.s, fh, fl);
+        const uint32_t M = Q.M;
+        const uint32_t P = seg_lane_parities(E, T.oq, M, L.cout);
+        const uint32_t dw = (uint32_t)(Q.delta >> E.L6) & ((1u << M) - 1u);
+        const uint32_t base = (L.pio << E.I) | L.hs;
+        GFloat4* tab = (GFloat4*)E.table;
+        const PQ_CONST uint32_t* pt = cview(E.pts) + 4 * ((1u << M) - 1u);
+        const PQ_CONST uint32_t* cl = cview(E.pcls) + 5 * M;
+        float A[4][4];
 #pragma unroll
-            for (int c = 0; c < 8; ++c) {
-                A[c][0] = 0.f; A[c][1] = 0.f; A[c][2] = 0.f; A[c][3] = 0.f;
+        for (int c = 0; c < 4; ++c) { A[c][0] = 0.f; A[c][1] = 0.f; A[c][2] = 0.f; A[c][3] = 0.f; }
+        if constexpr (SHORT) {
+            // Short walk (at most 2^kSegShortWalk points): every table
+            // read is issued before the first sum, so the walk costs one
+            // memory latency; point k adds into the accumulator of the
+            // class range that holds it (uniform tests, static registers).
+            constexpr int NP = 1 << kSegShortWalk;
+            const uint32_t np = 1u << M;
+            // The NP index words first, unconditionally (the record array is
+            // padded by NP records, so every read is in bounds), all waited
+            // for at once: otherwise each table read would wait for its own
+            // record word.
+            uint32_t Xk[NP];
+#pragma unroll
+            for (int k = 0; k < NP; ++k) Xk[k] = pt[4 * k];
+#pragma unroll
+            for (int k = 0; k < NP; ++k) asm volatile("" ::"s"(Xk[k]));
+            f32x4 Tk[NP];
+#pragma unroll
+            for (int k = 0; k < NP; ++k)
+                if ((uint32_t)k < np) Tk[k] = tab[base ^ Xk[k]];
+            const uint32_t c1 = cl[1], c2 = cl[2], c3 = cl[3];
+#pragma unroll
+            for (int k = 0; k < NP; ++k) {
+                if ((uint32_t)k < np) {
+                    const PQ_CONST uint32_t* e = pt + 4 * k;
+                    const uint32_t g = e[1];
+                    // lane and query sign of the point
+                    const uint32_t sg = (uint32_t)((__popc(P & g) ^ __popc(dw & g)) & 1) << 31;
+                    const float wh = __uint_as_float(e[2] ^ sg), wl = __uint_as_float(e[3] ^ sg);
+                    const uint32_t u = (uint32_t)k;
+                    if (u < c1) seg_point(A[0], wh, wl, Tk[k]);
+                    else if (u < c2) seg_point(A[1], wh, wl, Tk[k]);
+                    else if (u < c3) seg_point(A[2], wh, wl, Tk[k]);
+                    else seg_point(A[3], wh, wl, Tk[k]);
+                }
+            }
+        } else {
+            // points grouped by their engine phase class c: the class is
+            // a loop constant, so each group adds into its own accumulator
+#pragma unroll
+            for (int c = 0; c < 4; ++c) {
                 const uint32_t p1 = cl[c + 1];
                 uint32_t p = cl[c];
                 // four points per step, all table loads first
                 for (; p + 4 <= p1; p += 4) {
-                    f32x4 T[4];
+                    f32x4 Tq[4];
                     uint32_t sg[4];
                     float fh4[4], fl4[4];
 #pragma unroll
                     for (int k = 0; k < 4; ++k) {
                         const PQ_CONST uint32_t* e = pt + 4 * (p + k);
                         const uint32_t X = e[0], g = e[1];
-                        T[k] = tab[base ^ X];
+                        Tq[k] = tab[base ^ X];
                         // query sign (uniform) and lane sign of the point
                         const uint32_t us = (uint32_t)(__popc(dw & g) & 1) << 31;
                         sg[k] = ((uint32_t)(__popc(P & g) & 1) << 31) ^ us;
@@ -918,90 +978,114 @@ __global__ __launch_bounds__(256) void k_seg_walk(const SegDev* __restrict__ eng
                     }
 #pragma unroll
                     for (int k = 0; k < 4; ++k)
-                        seg_point(A[c], __uint_as_float(__float_as_uint(fh4[k]) ^ sg[k]), __uint_as_float(__float_as_uint(fl4[k]) ^ sg[k]), T[k]);
+                        seg_point(A[c], __uint_as_float(__float_as_uint(fh4[k]) ^ sg[k]), __uint_as_float(__float_as_uint(fl4[k]) ^ sg[k]), Tq[k]);
                 }
                 for (; p < p1; ++p) {
                     const PQ_CONST uint32_t* e = pt + 4 * p;
                     const uint32_t X = e[0], g = e[1];
-                    const f32x4 T = tab[base ^ X];
+                    const f32x4 Tp = tab[base ^ X];
                     const uint32_t sg = (((uint32_t)(__popc(P & g) & 1)) ^ ((uint32_t)(__popc(dw & g) & 1))) << 31;
-                    seg_point(A[c], __uint_as_float(e[2] ^ sg), __uint_as_float(e[3] ^ sg), T);
+                    seg_point(A[c], __uint_as_float(e[2] ^ sg), __uint_as_float(e[3] ^ sg), Tp);
                 }
             }
-            // lane total = omega^K_seed F (EVu + omega ODu)
-            float eu[4], ou[4];
-            seg_fold4<0>(A, eu);
-            seg_fold4<1>(A, ou);
-            const uint32_t Ks = L.K & 7u;
-            float e2[4], o2[4];
-            if (Ks & 1u) {   // omega^(2a+1) (EVu + omega ODu) = i^a (i ODu + omega EVu)
-                e2[0] = -ou[2]; e2[1] = -ou[3]; e2[2] = ou[0]; e2[3] = ou[1];
-                o2[0] = eu[0]; o2[1] = eu[1]; o2[2] = eu[2]; o2[3] = eu[3];
-            } else {
-#pragma unroll
-                for (int c = 0; c < 4; ++c) { e2[c] = eu[c]; o2[c] = ou[c]; }
-            }
-            df_rot(e2, Ks >> 1);
-            df_rot(o2, Ks >> 1);
-#pragma unroll
-            for (int c = 0; c < 4; c += 2) {
-                df_mul(e2[c], e2[c + 1], fh, fl, ev[c], ev[c + 1]);
-                df_mul(o2[c], o2[c + 1], fh, fl, od[c], od[c + 1]);
-            }
         }
-        // workgroup reduction of the four double-float sums
+        // lane total = omega^K_seed F (EVu + omega ODu)
+        float eu[4], ou[4];
+        seg_fold4<0>(A, eu);
+        seg_fold4<1>(A, ou);
+        const uint32_t Ks = L.K & 7u;
+        float e2[4], o2[4];
+        if (Ks & 1u) {   // omega^(2a+1) (EVu + omega ODu) = i^a (i ODu + omega EVu)
+            e2[0] = -ou[2]; e2[1] = -ou[3]; e2[2] = ou[0]; e2[3] = ou[1];
+            o2[0] = eu[0]; o2[1] = eu[1]; o2[2] = eu[2]; o2[3] = eu[3];
+        } else {
 #pragma unroll
-        for (int off = 32; off > 0; off >>= 1) {
-#pragma unroll
-            for (int c = 0; c < 4; c += 2) {
-                const float a = __shfl_xor(ev[c], off), b = __shfl_xor(ev[c + 1], off);
-                const float e = __shfl_xor(od[c], off), f = __shfl_xor(od[c + 1], off);
-                merge2(ev[c], ev[c + 1], a, b);
-                merge2(od[c], od[c + 1], e, f);
-            }
+            for (int c = 0; c < 4; ++c) { e2[c] = eu[c]; o2[c] = ou[c]; }
         }
-        if ((tid & 63) == 0) {
+        df_rot(e2, Ks >> 1);
+        df_rot(o2, Ks >> 1);
 #pragma unroll
-            for (int c = 0; c < 4; ++c) { s_red[tid >> 6][c] = ev[c]; s_red[tid >> 6][4 + c] = od[c]; }
+        for (int c = 0; c < 4; c += 2) {
+            df_mul(e2[c], e2[c + 1], fh, fl, ev[c], ev[c + 1]);
+            df_mul(o2[c], o2[c + 1], fh, fl, od[c], od[c + 1]);
         }
-        __syncthreads();
-        if (tid == 0) {
-            double v[4] = {0.0, 0.0, 0.0, 0.0};
-            for (int w = 0; w < 4; ++w) {
-                v[0] += (double)s_red[w][0] + (double)s_red[w][1];
-                v[1] += (double)s_red[w][2] + (double)s_red[w][3];
-                v[2] += (double)s_red[w][4] + (double)s_red[w][5];
-                v[3] += (double)s_red[w][6] + (double)s_red[w][7];
-            }
-            if (!fused) {
-                // large batches: item partials only; k_z8_reduce follows
-#pragma unroll
-                for (int c = 0; c < 4; ++c) partial[4 * item + c] = v[c];
-                s_last = 0;
-            } else if (Q.nitems == 1) {
-                // the query is this item alone: finish it here (the same
-                // arithmetic as query_reduce on a single item)
-                const double h = 0.70710678118654752440;
-                const double re = v[0] + h * (v[2] - v[3]), im = v[1] + h * (v[2] + v[3]);
-                const double2 sc = scale[qi];
-                double* dst = out + 2 * order[qi];
-                dst[0] = sc.x * re - sc.y * im;
-                dst[1] = sc.x * im + sc.y * re;
-                s_last = 0;
-            } else {
-                // Partials go out as device-scope atomic stores and the count
-                // as a release; the last workgroup reads them back with
-                // device-scope atomic loads.  No acquire fence: it would
-                // invalidate this XCD's L2, which holds the segment table.
-#pragma unroll
-                for (int c = 0; c < 4; ++c) __hip_atomic_store(partial + 4 * item + c, v[c], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-                const unsigned prev = __hip_atomic_fetch_add(qdone + qi, 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
-                s_last = (prev + 1u == (unsigned)Q.nitems) ? 1 : 0;
-            }
-        }
-        __syncthreads();
-        if (s_last) query_reduce<true>(Q.item0, Q.nitems, partial, scale[qi], out + 2 * order[qi], s_qred);
     }
+    // workgroup reduction of the four double-float sums
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+#pragma unroll
+        for (int c = 0; c < 4; c += 2) {
+            const float a = __shfl_xor(ev[c], off), b = __shfl_xor(ev[c + 1], off);
+            const float e = __shfl_xor(od[c], off), f = __shfl_xor(od[c + 1], off);
+            merge2(ev[c], ev[c + 1], a, b);
+            merge2(od[c], od[c + 1], e, f);
+        }
+    }
+    if ((tid & 63) == 0) {
+#pragma unroll
+        for (int c = 0; c < 4; ++c) { s_red[tid >> 6][c] = ev[c]; s_red[tid >> 6][4 + c] = od[c]; }
+    }
+    __syncthreads();
+    // The item's four partial sums (EV re, im, OD re, im), one per lane
+    // of lanes 0..3: a single 32-byte write, into pinned host memory for
+    // small batches (the host finishes each query as its items arrive)
+    // or device memory before k_z8_reduce.  Each word is one aligned
+    // 8-byte store.
+    if (tid < 4) {
+        double v = 0.0;
+        for (int w = 0; w < 4; ++w) v += (double)s_red[w][2 * tid] + (double)s_red[w][2 * tid + 1];
+        partial[4 * item + tid] = v;
+    }
+}
+
+// SHORT: every query of the launch walks at most 2^kSegShortWalk points
+// (kept apart from the long-walk variant, whose register budget it would
+// otherwise raise).  Items item_lo .. item_end - 1: a workgroup's first item
+// is its own index; later ones come from the queue (counter starts at 0), so
+// a batch of at most one item per workgroup takes no queue atomics at all.
+// The long-walk variant is held to 7 waves per SIMD (70 VGPRs, no spills):
+// its table reads are latency-bound and want every resident wave.
+template <bool SHORT>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(SHORT ? 4 : 7, 8))) void k_seg_walk(const SegDev* __restrict__ engs, const SegQ* __restrict__ qs, const uint32_t* __restrict__ item_q,
+                                                  uint64_t item_lo, uint64_t item_end, unsigned long long* __restrict__ counter, double* __restrict__ partial) {
+    PQ_SEG_LDS(lds);
+    __shared__ unsigned long long s_item;
+    uint64_t item = item_lo + blockIdx.x;
+    int eng = -1;                // engine whose tables are in LDS (uniform)
+    while (item < item_end) {
+        const uint32_t qi = cview(item_q)[item];
+        const SegQ Q = cload(qs + qi);
+        const SegDev E = cload(engs + Q.eng);
+        seg_item<SHORT>(lds, E, Q, item, eng, partial);
+        if (item_end - item_lo <= gridDim.x) break;    // every item was a first item
+        __syncthreads();                               // LDS free for the next item
+        if (threadIdx.x == 0) s_item = item_lo + gridDim.x + atomicAdd(counter, 1ull);
+        __syncthreads();
+        item = s_item;
+    }
+}
+
+// A small batch passed whole as kernel arguments: no staging copy before the
+// launch and no dependent loads of query and engine records in the kernel.
+constexpr uint32_t kArgQueries = 16, kArgEngines = 2;
+struct SegArgs {
+    uint32_t qstart[kArgQueries];    // first item of each query, ascending; unused: ~0
+    SegDev eng[kArgEngines];
+    SegQ q[kArgQueries];
+};
+// One workgroup per item, items item_lo.. of the batch (grid = item count).
+template <bool SHORT>
+__global__ __launch_bounds__(256) void k_seg_walk_args(const SegArgs A, uint32_t item_lo, double* __restrict__ partial) {
+    PQ_SEG_LDS(lds);
+    const uint32_t item = item_lo + blockIdx.x;
+    // the item's query: how many later queries start at or before it (all
+    // starts read at once, no chain of dependent loads)
+    uint32_t qi = 0;
+#pragma unroll
+    for (uint32_t k = 1; k < kArgQueries; ++k) qi += item >= A.qstart[k] ? 1u : 0u;
+    const SegQ& Q = A.q[qi];
+    int eng = -1;
+    seg_item<SHORT>(lds, A.eng[Q.eng], Q, item, eng, partial);
 }
 
 // FP64 cross-workgroup reduction (fixed order) and global scaling per query.
@@ -1011,7 +1095,7 @@ __global__ __launch_bounds__(256) void k_z8_reduce(const uint64_t* __restrict__ 
                                                    const uint32_t* __restrict__ order, double* __restrict__ out) {
     __shared__ double red[4][4];
     const uint32_t qi = blockIdx.x;
-    query_reduce<false>(qwords[8 * qi + 2], qwords[8 * qi + 3], partial, scale[qi], out + 2 * order[qi], red);
+    query_reduce(qwords[8 * qi + 2], qwords[8 * qi + 3], partial, scale[qi], out + 2 * order[qi], red);
 }
 
 
@@ -1116,13 +1200,22 @@ BatchPlan plan_seg(const std::vector<Z8Job>& jobs) {
     std::vector<const DevEngine*> seen;
     std::vector<uint32_t> idx(jobs.size());
     std::iota(idx.begin(), idx.end(), 0u);
-    // heavy items first, jobs of one engine together (the LDS byte tables
-    // are reloaded only when a workgroup moves to another engine)
+    auto walk_bits = [&](const Z8Job& J) {
+        const SegImage& G = *J.eng->seg;
+        return std::min(J.mid_bits, std::min(G.Mcap, G.nout - G.L6));
+    };
+    // Queries with long lane walks first (launch variant 0), short walks
+    // after (variant 1); within each, heavy items first and jobs of one
+    // engine together (the LDS tables are reloaded only when a workgroup
+    // moves to another engine).
     std::stable_sort(idx.begin(), idx.end(), [&](uint32_t a, uint32_t b) {
-        const uint32_t ha = jobs[a].eng->seg->nout - jobs[a].mid_bits, hb = jobs[b].eng->seg->nout - jobs[b].mid_bits;
+        const bool sa = walk_bits(jobs[a]) <= kSegShortWalk, sb = walk_bits(jobs[b]) <= kSegShortWalk;
+        if (sa != sb) return sb;
+        const uint32_t ha = jobs[a].eng->seg->nout - walk_bits(jobs[a]), hb = jobs[b].eng->seg->nout - walk_bits(jobs[b]);
         if (ha != hb) return ha > hb;
         return jobs[a].eng < jobs[b].eng;
     });
+    int cur = -1;
     for (uint32_t ji : idx) {
         const Z8Job& J = jobs[ji];
         const SegImage& G = *J.eng->seg;
@@ -1132,9 +1225,11 @@ BatchPlan plan_seg(const std::vector<Z8Job>& jobs) {
         else e = (uint32_t)(it - seen.begin());
         SegQ q{};
         q.delta = J.sq.delta; q.s0 = J.sq.s0; q.eng = e; q.K0 = J.sq.K0; q.hs0 = J.sq.hs0; q.pio0 = J.sq.pio0;
-        q.M = std::min(J.mid_bits, std::min(G.Mcap, G.nout - G.L6));
+        q.M = walk_bits(J);
         q.H = G.nout - q.M;
         if (q.H > 40) throw std::invalid_argument("branch dimension too large for one batch");
+        const int v = q.M <= kSegShortWalk ? 1 : 0;
+        if (v != cur) { B.q_lo[v] = B.nq(); B.it_lo[v] = B.total_items; cur = v; }
         q.item0 = B.total_items;
         q.nitems = ((1ull << q.H) + 255) / 256;
         B.total_items += q.nitems;
@@ -1143,6 +1238,7 @@ BatchPlan plan_seg(const std::vector<Z8Job>& jobs) {
         push_rec(B.qs, q);
         B.scale.push_back(make_double2(J.scale.real(), J.scale.imag()));
         B.order.push_back(ji);
+        B.q_hi[v] = B.nq(); B.it_hi[v] = B.total_items;
     }
     return B;
 }
@@ -1151,10 +1247,22 @@ BatchPlan plan_batch(const std::vector<Z8Job>& jobs, int walk) {
     if (walk == kWalkBranch) return plan_branch(jobs);
     throw std::invalid_argument("unknown walk form");
 }
+// Resident 256-thread workgroups per CU of a kernel, asked of the runtime
+// once per kernel.
 template <class K>
-uint32_t grid_for(Ctx& C, K kernel, uint64_t items) {
+int blocks_per_cu(K kernel) {
+    static std::mutex mu;
+    static std::vector<std::pair<const void*, int>> known;
+    std::lock_guard<std::mutex> lk(mu);
+    for (const auto& e : known) if (e.first == (const void*)kernel) return e.second;
     int per_cu = 0;
     HIPCHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, 256, 0));
+    known.emplace_back((const void*)kernel, per_cu);
+    return per_cu;
+}
+template <class K>
+uint32_t grid_for(Ctx& C, K kernel, uint64_t items) {
+    const int per_cu = blocks_per_cu(kernel);
     const uint64_t g = (uint64_t)std::max(per_cu, 1) * (uint64_t)C.cus;
     return (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(g, items));
 }
@@ -1177,7 +1285,7 @@ struct StageLayout {
 StageLayout stage_layout(const BatchPlan& B) {
     StageLayout L;
     const size_t be = B.engs.size(), bq = B.qs.size(), bs = B.scale.size() * sizeof(double2), bo = B.order.size() * 4;
-    L.oe = (64 + 4 * (size_t)B.nq() + 255) & ~size_t(255);   // 4 u64 queue counters at 0, u32 per-query done counts at 64
+    L.oe = 256;   // 4 u64 queue counters at 0
     L.oq = L.oe + ((be + 255) & ~size_t(255));
     L.os = L.oq + ((bq + 255) & ~size_t(255));
     L.oo = L.os + ((bs + 255) & ~size_t(255));
@@ -1193,20 +1301,28 @@ void stage_copy(const BatchPlan& B, const StageLayout& L, uint8_t* h) {
     std::memcpy(h + L.oo, B.order.data(), B.order.size() * 4);
     std::memcpy(h + L.oi, B.item_q.data(), B.item_q.size() * 4);
 }
-// Segment walk with out != null: the reduction is fused and results go to
-// out (host-mapped).  Otherwise the caller launches k_z8_reduce afterwards.
-uint32_t launch_walks(Ctx& C, hipStream_t st, const BatchPlan& B, uint8_t* dbase, const StageLayout& L, double* part, double* out) {
+// Launches the walk kernels of a batch; item partials go to part (device
+// memory, or pinned host memory seen through its device address).  With
+// one_pass every item gets a workgroup of its own (the hardware dispatcher
+// hands out the items in order; no queue, and no workgroup waits for its
+// host write before taking another item); otherwise the grid is persistent.
+uint32_t launch_walks(Ctx& C, hipStream_t st, const BatchPlan& B, uint8_t* dbase, const StageLayout& L, double* part, bool one_pass) {
     unsigned long long* ctr = (unsigned long long*)dbase;
     const void* de = dbase + L.oe;
     const void* dq = dbase + L.oq;
     if (B.walk == kWalkSeg) {
-        if (B.total_items == 0) return 0;
-        const uint32_t grid = grid_for(C, k_seg_walk, B.total_items);
-        hipLaunchKernelGGL(k_seg_walk, dim3(grid), dim3(256), 0, st, (const SegDev*)de, (const SegQ*)dq, (const uint32_t*)(dbase + L.oi),
-                           B.total_items, ctr, part, (unsigned*)(dbase + 64), (const double2*)(dbase + L.os), (const uint32_t*)(dbase + L.oo), out,
-                           out ? 1 : 0);
-        HIPCHECK(hipGetLastError());
-        return grid;
+        uint32_t g = 0;
+        for (int v = 0; v < 2; ++v) {
+            const uint64_t items = B.it_hi[v] - B.it_lo[v];
+            if (items == 0) continue;
+            auto kern = v ? k_seg_walk<true> : k_seg_walk<false>;
+            const uint32_t grid = one_pass ? (uint32_t)items : grid_for(C, kern, items);
+            hipLaunchKernelGGL(kern, dim3(grid), dim3(256), 0, st, (const SegDev*)de, (const SegQ*)dq, (const uint32_t*)(dbase + L.oi),
+                               B.it_lo[v], B.it_hi[v], ctr + v, part);
+            HIPCHECK(hipGetLastError());
+            g += grid;
+        }
+        return g;
     }
     uint32_t g = 0;
     const Z8Dev* ze = (const Z8Dev*)de;
@@ -1219,83 +1335,205 @@ uint32_t launch_walks(Ctx& C, hipStream_t st, const BatchPlan& B, uint8_t* dbase
     }
     return g;
 }
-// Offsets of the staged parts in one upload buffer.
+
+// A host-finished segment batch small enough to travel as kernel arguments.
+bool fits_args(const BatchPlan& B) {
+    return B.walk == kWalkSeg && B.nq() <= kArgQueries && B.engs.size() <= kArgEngines * sizeof(SegDev);
+}
+uint32_t launch_args(hipStream_t st, const BatchPlan& B, double* part) {
+    SegArgs A;
+    std::memset(&A, 0, sizeof(A));
+    std::memcpy(A.eng, B.engs.data(), B.engs.size());
+    std::memcpy(A.q, B.qs.data(), B.qs.size());
+    for (uint32_t k = 0; k < kArgQueries; ++k) A.qstart[k] = k < B.nq() ? (uint32_t)A.q[k].item0 : ~0u;
+    uint32_t g = 0;
+    for (int v = 0; v < 2; ++v) {
+        const uint64_t items = B.it_hi[v] - B.it_lo[v];
+        if (items == 0) continue;
+        auto kern = v ? k_seg_walk_args<true> : k_seg_walk_args<false>;
+        hipLaunchKernelGGL(kern, dim3((uint32_t)items), dim3(256), 0, st, A, (uint32_t)B.it_lo[v], part);
+        HIPCHECK(hipGetLastError());
+        g += (uint32_t)items;
+    }
+    return g;
+}
+
+// The per-query data the host needs to finish a batch itself.
+struct QueryTail {
+    std::vector<uint64_t> item0, nitems;
+    std::vector<double2> scale;
+    std::vector<uint32_t> order;
+    void set(const BatchPlan& B) {
+        const uint32_t nq = B.nq();
+        item0.resize(nq); nitems.resize(nq);
+        for (uint32_t i = 0; i < nq; ++i) {
+            std::memcpy(&item0[i], B.qs.data() + 64 * (size_t)i + 16, 8);
+            std::memcpy(&nitems[i], B.qs.data() + 64 * (size_t)i + 24, 8);
+        }
+        scale = B.scale; order = B.order;
+    }
+};
+
+// The device order of query_reduce restated on the host, bit for bit (IEEE
+// double additions and products in the same association; this file is built
+// without floating-point contraction): thread t of 256 adds items t, t + 256,
+// ... starting from 0; each 64-lane wave folds by the xor butterfly (lane 0
+// ends with lane 0 + lane 32 first, then with the lane-16 sum, ...); then
+// (w0 + w1) + (w2 + w3), EV + omega OD and the scale.
+void host_query_reduce(const double* part, uint64_t nitems, double2 sc, double* dst) {
+    double w[4][4];          // wave totals, [wave][component]
+    for (uint32_t wv = 0; wv < 4; ++wv) {
+        const uint64_t t0 = 64 * (uint64_t)wv;
+        if (t0 >= nitems) { for (int c = 0; c < 4; ++c) w[wv][c] = 0.0; continue; }
+        // lanes at or beyond n hold no item: +0.0, and so does every sum of
+        // them, so only lanes below n are computed (x + 0.0 is still done:
+        // it turns -0.0 into +0.0 exactly as the device does)
+        uint32_t n = (uint32_t)std::min<uint64_t>(64, nitems - t0);
+        double L[64][4];
+        for (uint32_t i = 0; i < n; ++i)
+            for (int c = 0; c < 4; ++c) {
+                double v = 0.0;
+                for (uint64_t k = t0 + i; k < nitems; k += 256) v += part[4 * k + c];
+                L[i][c] = v;
+            }
+        for (uint32_t off = 32; off > 0; off >>= 1) {
+            const uint32_t m = std::min(off, n);
+            for (uint32_t i = 0; i < m; ++i) {
+                if (i + off < n) { for (int c = 0; c < 4; ++c) L[i][c] = L[i][c] + L[i + off][c]; }
+                else { for (int c = 0; c < 4; ++c) L[i][c] = L[i][c] + 0.0; }
+            }
+            n = m;
+        }
+        for (int c = 0; c < 4; ++c) w[wv][c] = L[0][c];
+    }
+    double t[4];
+    for (int c = 0; c < 4; ++c) t[c] = (w[0][c] + w[1][c]) + (w[2][c] + w[3][c]);
+    const double h = 0.70710678118654752440;
+    const double re = t[0] + h * (t[2] - t[3]);
+    const double im = t[1] + h * (t[2] + t[3]);
+    dst[0] = sc.x * re - sc.y * im;
+    dst[1] = sc.x * im + sc.y * re;
+}
+
+// A pinned host word the device has not written yet holds this pattern (a
+// signalling NaN, which GPU arithmetic never produces); each partial is one
+// aligned 8-byte store, so a word holds either the pattern or the value.
+constexpr uint64_t kPending = 0x7FF4C0FFEE5107EDull;
+
+// Waits for the item partials of each query in turn (the walk may still be
+// running later items) and finishes the query on the host.  The stream's
+// event is polled now and then, so a failed launch is reported instead of
+// waited on.
+void host_finish(const QueryTail& T, const double* part, hipEvent_t done, double* res) {
+    const volatile uint64_t* w = (const volatile uint64_t*)part;
+    bool finished = false;   // the stream has completed: every store landed
+    uint64_t spin = 0;
+    for (size_t q = 0; q < T.item0.size(); ++q) {
+        const uint64_t lo = 4 * T.item0[q], hi = 4 * (T.item0[q] + T.nitems[q]);
+        for (uint64_t i = lo; i < hi;) {
+            if (w[i] != kPending) { ++i; continue; }
+            if (finished) throw std::runtime_error("segment walk ended without writing every item partial");
+            if ((++spin & 255) == 0) {
+                const hipError_t e = hipEventQuery(done);
+                if (e != hipErrorNotReady) { HIPCHECK(e); finished = true; }
+            }
+        }
+        std::atomic_thread_fence(std::memory_order_acquire);
+        host_query_reduce(part + 4 * T.item0[q], T.nitems[q], T.scale[q], res + 2 * (size_t)T.order[q]);
+    }
+}
 }  // namespace
 
-// The segment walk finishes queries inside the walk kernel (last workgroup of
-// a query reduces it, results written to pinned host memory) for batches up to
-// this many items, where launch latency dominates; larger batches use the
-// separate reduction launch, which avoids one L2 write-back per item.  The
-// reduction order, hence every result bit, is the same either way.
-constexpr uint64_t kFuseItems = 4096;
-bool fuse_reduce(const BatchPlan& B) { return B.walk == kWalkSeg && B.total_items <= kFuseItems; }
+// Small segment-walk batches are finished on the host: the walk writes its
+// item partials straight into pinned host memory and the host reduces each
+// query as soon as its items have landed (no cross-workgroup atomics on the
+// device, no reduction launch, no result copy).  Larger batches reduce on the
+// device (k_z8_reduce) and copy the results.  The reduction order, hence
+// every result bit, is the same either way.
+constexpr uint64_t kHostFinishItems = 4096;
+bool host_finishes(const BatchPlan& B) { return B.walk == kWalkSeg && B.total_items <= kHostFinishItems; }
 
 // One in-flight batch: its own stream, events and buffers, so a second batch
-// can be queued while this one still runs (continuous batching).
+// can be queued while this one still runs (continuous batching).  Timing
+// events (e0..e3) are recorded only for a timed batch: each one is a queue
+// barrier that costs several microseconds of a short batch's latency.
 struct Z8Slot {
     hipStream_t stream = nullptr;
-    hipEvent_t e0 = nullptr, e1 = nullptr, e2 = nullptr, e3 = nullptr;
+    hipEvent_t e0 = nullptr, e1 = nullptr, e2 = nullptr, e3 = nullptr, done = nullptr;
     DBuf d_in, d_part, d_res;
-    HBuf h_in, h_res;
+    HBuf h_in, h_res, h_part;
+    QueryTail tail;
     uint32_t nq = 0, grid = 0;
     uint64_t items = 0, branches = 0;
-    bool pending = false;
+    bool pending = false, timed = false, on_host = false;
     Z8Slot() {
         HIPCHECK(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));   // default priority
         HIPCHECK(hipEventCreate(&e0)); HIPCHECK(hipEventCreate(&e1)); HIPCHECK(hipEventCreate(&e2)); HIPCHECK(hipEventCreate(&e3));
+        HIPCHECK(hipEventCreateWithFlags(&done, hipEventDisableTiming));
     }
     ~Z8Slot() {
-        if (pending) (void)hipStreamSynchronize(stream);
-        for (hipEvent_t e : {e0, e1, e2, e3}) if (e) (void)hipEventDestroy(e);
+        if (stream) (void)hipStreamSynchronize(stream);
+        for (hipEvent_t e : {e0, e1, e2, e3, done}) if (e) (void)hipEventDestroy(e);
         if (stream) (void)hipStreamDestroy(stream);
-        if (d_in.p) (void)hipFree(d_in.p);
-        if (d_part.p) (void)hipFree(d_part.p);
-        if (d_res.p) (void)hipFree(d_res.p);
-        if (h_in.p) (void)hipHostFree(h_in.p);
-        if (h_res.p) (void)hipHostFree(h_res.p);
+        for (void* p : {d_in.p, d_part.p, d_res.p}) if (p) (void)hipFree(p);
+        for (void* p : {h_in.p, h_res.p, h_part.p}) if (p) (void)hipHostFree(p);
     }
 };
 
 std::shared_ptr<Z8Slot> dev_slot_create() { (void)ctx(); return std::make_shared<Z8Slot>(); }
 
-void dev_z8_launch(Z8Slot& S, const std::vector<Z8Job>& jobs, int walk) {
+void dev_z8_launch(Z8Slot& S, const std::vector<Z8Job>& jobs, int walk, bool timed) {
     if (S.pending) throw std::logic_error("batch slot already has a batch in flight");
     Ctx& C = ctx();
     BatchPlan B = plan_batch(jobs, walk);
     const uint32_t nq = B.nq();
-    S.nq = nq; S.items = B.total_items; S.branches = B.branches;
+    S.nq = nq; S.items = B.total_items; S.branches = B.branches; S.timed = timed; S.on_host = host_finishes(B);
     if (nq == 0) return;
+    double* hres = (double*)S.h_res.get(nq * 16);
+    if (S.on_host && fits_args(B)) {
+        // everything in the kernel arguments: no upload at all
+        uint64_t* hp = (uint64_t*)S.h_part.get(B.total_items * 32);
+        for (uint64_t i = 0; i < 4 * B.total_items; ++i) hp[i] = kPending;
+        S.tail.set(B);
+        if (timed) { HIPCHECK(hipEventRecord(S.e0, S.stream)); HIPCHECK(hipEventRecord(S.e1, S.stream)); }
+        S.grid = launch_args(S.stream, B, (double*)S.h_part.dev);
+        if (timed) { HIPCHECK(hipEventRecord(S.e2, S.stream)); HIPCHECK(hipEventRecord(S.e3, S.stream)); }
+        HIPCHECK(hipEventRecord(S.done, S.stream));
+        S.pending = true;
+        return;
+    }
     // one pinned staging buffer: counters | engines | queries | scales | order | item queries
     const StageLayout L = stage_layout(B);
     uint8_t* h = (uint8_t*)S.h_in.get(L.tot);
     stage_copy(B, L, h);
     uint8_t* d = (uint8_t*)S.d_in.get(L.tot);
-    double* part = (double*)S.d_part.get(std::max<uint64_t>(B.total_items, 1) * 32);
-    double* hres = (double*)S.h_res.get(nq * 16);
-    HIPCHECK(hipEventRecord(S.e0, S.stream));
+    if (timed) HIPCHECK(hipEventRecord(S.e0, S.stream));
     HIPCHECK(hipMemcpyAsync(d, h, L.tot, hipMemcpyHostToDevice, S.stream));
-    HIPCHECK(hipEventRecord(S.e1, S.stream));
-    if (fuse_reduce(B)) {
-        // results land in the pinned host buffer, written by the walk itself
-        double* dres = nullptr;
-        HIPCHECK(hipHostGetDevicePointer((void**)&dres, hres, 0));
-        S.grid = launch_walks(C, S.stream, B, d, L, part, dres);
-        HIPCHECK(hipEventRecord(S.e2, S.stream));
+    if (timed) HIPCHECK(hipEventRecord(S.e1, S.stream));
+    if (S.on_host) {
+        uint64_t* hp = (uint64_t*)S.h_part.get(B.total_items * 32);
+        for (uint64_t i = 0; i < 4 * B.total_items; ++i) hp[i] = kPending;
+        double* dp = (double*)S.h_part.dev;
+        S.tail.set(B);
+        S.grid = launch_walks(C, S.stream, B, d, L, dp, true);
+        if (timed) HIPCHECK(hipEventRecord(S.e2, S.stream));
     } else {
+        double* part = (double*)S.d_part.get(std::max<uint64_t>(B.total_items, 1) * 32);
         double* res = (double*)S.d_res.get(nq * 16);
-        S.grid = launch_walks(C, S.stream, B, d, L, part, nullptr);
-        HIPCHECK(hipEventRecord(S.e2, S.stream));
+        S.grid = launch_walks(C, S.stream, B, d, L, part, false);
+        if (timed) HIPCHECK(hipEventRecord(S.e2, S.stream));
         hipLaunchKernelGGL(k_z8_reduce, dim3(nq), dim3(256), 0, S.stream, (const uint64_t*)(d + L.oq), part, (const double2*)(d + L.os), (const uint32_t*)(d + L.oo), res);
         HIPCHECK(hipGetLastError());
         HIPCHECK(hipMemcpyAsync(hres, res, nq * 16, hipMemcpyDeviceToHost, S.stream));
     }
-    HIPCHECK(hipEventRecord(S.e3, S.stream));
+    if (timed) HIPCHECK(hipEventRecord(S.e3, S.stream));
+    HIPCHECK(hipEventRecord(S.done, S.stream));
     S.pending = true;
 }
 
 bool dev_z8_ready(Z8Slot& S) {
     if (!S.pending) return true;
-    hipError_t e = hipEventQuery(S.e3);
+    hipError_t e = hipEventQuery(S.done);
     if (e == hipErrorNotReady) return false;
     HIPCHECK(e);
     return true;
@@ -1304,16 +1542,36 @@ bool dev_z8_ready(Z8Slot& S) {
 void dev_z8_wait(Z8Slot& S, std::vector<std::complex<double>>& out, Z8Timing* timing) {
     out.assign(S.nq, {0, 0});
     if (!S.pending) return;
-    HIPCHECK(hipEventSynchronize(S.e3));
+    double* hres = (double*)S.h_res.p;
+    if (S.on_host) {
+        try {
+            host_finish(S.tail, (const double*)S.h_part.p, S.done, hres);
+        } catch (...) {
+            S.pending = false;
+            (void)hipStreamSynchronize(S.stream);
+            throw;
+        }
+        if (S.timed) HIPCHECK(hipEventSynchronize(S.done));
+    } else {
+        HIPCHECK(hipEventSynchronize(S.done));
+    }
     S.pending = false;
-    const double* hres = (const double*)S.h_res.p;
     for (uint32_t i = 0; i < S.nq; ++i) out[i] = {hres[2 * i], hres[2 * i + 1]};
     if (timing) {
-        float k, t;
-        HIPCHECK(hipEventElapsedTime(&k, S.e1, S.e2)); HIPCHECK(hipEventElapsedTime(&t, S.e0, S.e3));
-        timing->kernel_ms = k; timing->total_ms = t; timing->branches = S.branches; timing->items = S.items; timing->grid = S.grid;
+        timing->branches = S.branches; timing->items = S.items; timing->grid = S.grid;
+        timing->kernel_ms = timing->total_ms = std::nan("");
+        if (S.timed) {
+            float k, t;
+            HIPCHECK(hipEventElapsedTime(&k, S.e1, S.e2)); HIPCHECK(hipEventElapsedTime(&t, S.e0, S.e3));
+            timing->kernel_ms = k; timing->total_ms = t;
+        }
     }
 }
+
+// Whether eager batches record timing events (pq_set_timing).
+static std::atomic<bool> g_timed{false};
+void dev_set_timing(bool on) { g_timed = on; }
+bool dev_get_timing() { return g_timed; }
 
 void dev_z8_batch(const std::vector<Z8Job>& jobs, std::vector<std::complex<double>>& out, Z8Timing* timing, int walk) {
     out.assign(jobs.size(), {0, 0});
@@ -1322,7 +1580,7 @@ void dev_z8_batch(const std::vector<Z8Job>& jobs, std::vector<std::complex<doubl
     static std::shared_ptr<Z8Slot> slot;
     std::lock_guard<std::mutex> lk(C.mu);
     if (!slot) slot = std::make_shared<Z8Slot>();
-    dev_z8_launch(*slot, jobs, walk);
+    dev_z8_launch(*slot, jobs, walk, g_timed);
     dev_z8_wait(*slot, out, timing);
 }
 
@@ -1330,15 +1588,16 @@ void dev_z8_batch(const std::vector<Z8Job>& jobs, std::vector<std::complex<doubl
 struct Z8Graph {
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
-    void *h_in = nullptr, *h_out = nullptr, *d_in = nullptr, *d_part = nullptr, *d_res = nullptr;
+    void *h_in = nullptr, *h_out = nullptr, *h_part = nullptr, *d_in = nullptr, *d_part = nullptr, *d_res = nullptr;
+    QueryTail tail;
+    bool on_host = false;
     uint32_t nq = 0, grid = 0;
     uint64_t items = 0, branches = 0;
     hipEvent_t k0 = nullptr, k1 = nullptr;
     ~Z8Graph() {
         if (exec) (void)hipGraphExecDestroy(exec);
         if (graph) (void)hipGraphDestroy(graph);
-        if (h_in) (void)hipHostFree(h_in);
-        if (h_out) (void)hipHostFree(h_out);
+        for (void* p : {h_in, h_out, h_part}) if (p) (void)hipHostFree(p);
         for (void* p : {d_in, d_part, d_res}) if (p) (void)hipFree(p);
         if (k0) (void)hipEventDestroy(k0);
         if (k1) (void)hipEventDestroy(k1);
@@ -1351,27 +1610,41 @@ std::shared_ptr<Z8Graph> dev_z8_graph(const std::vector<Z8Job>& jobs, int walk) 
     BatchPlan B = plan_batch(jobs, walk);
     const uint32_t nq = B.nq();
     if (nq == 0) throw std::invalid_argument("a graph needs at least one nonzero query");
-    G->nq = nq; G->items = B.total_items; G->branches = B.branches;
+    G->nq = nq; G->items = B.total_items; G->branches = B.branches; G->on_host = host_finishes(B);
     const StageLayout L = stage_layout(B);
     std::lock_guard<std::mutex> lk(C.mu);
     HIPCHECK(hipHostMalloc(&G->h_in, L.tot, hipHostMallocDefault));
     HIPCHECK(hipHostMalloc(&G->h_out, nq * 16, hipHostMallocMapped));
     HIPCHECK(hipMalloc(&G->d_in, L.tot));
-    HIPCHECK(hipMalloc(&G->d_part, std::max<uint64_t>(B.total_items * 32, 32)));
-    HIPCHECK(hipMalloc(&G->d_res, nq * 16));
     HIPCHECK(hipEventCreate(&G->k0)); HIPCHECK(hipEventCreate(&G->k1));
     stage_copy(B, L, (uint8_t*)G->h_in);
     uint8_t* d = (uint8_t*)G->d_in;
-    double* dout = nullptr;
-    HIPCHECK(hipHostGetDevicePointer((void**)&dout, G->h_out, 0));
-    HIPCHECK(hipStreamBeginCapture(C.stream, hipStreamCaptureModeThreadLocal));
-    HIPCHECK(hipMemcpyAsync(d, G->h_in, L.tot, hipMemcpyHostToDevice, C.stream));   // also clears the counters
-    if (fuse_reduce(B)) {
-        G->grid = launch_walks(C, C.stream, B, d, L, (double*)G->d_part, dout);
+    // every allocation before the capture: none is allowed inside it
+    double* dp = nullptr;
+    if (G->on_host) {
+        HIPCHECK(hipHostMalloc(&G->h_part, B.total_items * 32, hipHostMallocMapped));
+        HIPCHECK(hipHostGetDevicePointer((void**)&dp, G->h_part, 0));
+        G->tail.set(B);
     } else {
-        G->grid = launch_walks(C, C.stream, B, d, L, (double*)G->d_part, nullptr);
-        hipLaunchKernelGGL(k_z8_reduce, dim3(nq), dim3(256), 0, C.stream, (const uint64_t*)(d + L.oq), (const double*)G->d_part, (const double2*)(d + L.os), (const uint32_t*)(d + L.oo), (double*)G->d_res);
-        HIPCHECK(hipMemcpyAsync(G->h_out, G->d_res, nq * 16, hipMemcpyDeviceToHost, C.stream));
+        HIPCHECK(hipMalloc(&G->d_part, std::max<uint64_t>(B.total_items * 32, 32)));
+        HIPCHECK(hipMalloc(&G->d_res, nq * 16));
+        dp = (double*)G->d_part;
+    }
+    HIPCHECK(hipStreamBeginCapture(C.stream, hipStreamCaptureModeThreadLocal));
+    try {
+        HIPCHECK(hipMemcpyAsync(d, G->h_in, L.tot, hipMemcpyHostToDevice, C.stream));   // also clears the counters
+        G->grid = launch_walks(C, C.stream, B, d, L, dp, G->on_host);
+        if (!G->on_host) {
+            hipLaunchKernelGGL(k_z8_reduce, dim3(nq), dim3(256), 0, C.stream, (const uint64_t*)(d + L.oq), (const double*)G->d_part, (const double2*)(d + L.os), (const uint32_t*)(d + L.oo), (double*)G->d_res);
+            HIPCHECK(hipGetLastError());
+            HIPCHECK(hipMemcpyAsync(G->h_out, G->d_res, nq * 16, hipMemcpyDeviceToHost, C.stream));
+        }
+    } catch (...) {
+        // leave the stream usable: end the capture and drop what it holds
+        hipGraph_t partial = nullptr;
+        (void)hipStreamEndCapture(C.stream, &partial);
+        if (partial) (void)hipGraphDestroy(partial);
+        throw;
     }
     HIPCHECK(hipStreamEndCapture(C.stream, &G->graph));
     HIPCHECK(hipGraphInstantiate(&G->exec, G->graph, nullptr, nullptr, 0));
@@ -1385,7 +1658,10 @@ void dev_z8_graph_run(Z8Graph& G, std::vector<std::complex<double>>& out, Z8Timi
     HIPCHECK(hipGraphLaunch(G.exec, C.stream));
     HIPCHECK(hipEventRecord(G.k1, C.stream));
     HIPCHECK(hipStreamSynchronize(C.stream));
-    const double* r = (const double*)G.h_out;
+    double* r = (double*)G.h_out;
+    if (G.on_host)   // the stream is done: every partial is in place
+        for (size_t q = 0; q < G.nq; ++q)
+            host_query_reduce((const double*)G.h_part + 4 * G.tail.item0[q], G.tail.nitems[q], G.tail.scale[q], r + 2 * (size_t)G.tail.order[q]);
     out.resize(G.nq);
     for (uint32_t i = 0; i < G.nq; ++i) out[i] = {r[2 * i], r[2 * i + 1]};
     if (timing) {
@@ -1490,15 +1766,15 @@ __global__ __launch_bounds__(256) void k_dump_seg(SegDev E, SegQ Q, uint8_t* __r
     const uint32_t P = seg_lane_parities(E, T.oq, M, L.cout);
     const uint32_t dw = (uint32_t)(Q.delta >> E.L6) & ((1u << M) - 1u);
     const uint32_t* pt = E.pts + 4 * ((1u << M) - 1u);
-    const uint32_t* cl = E.pcls + 9 * M;
-    for (uint32_t c = 0; c < 8; ++c) {
+    const uint32_t* cl = E.pcls + 5 * M;
+    for (uint32_t c = 0; c < 4; ++c) {
         for (uint32_t p = cl[c]; p < cl[c + 1]; ++p) {
-            const uint32_t X = pt[4 * p], g = pt[4 * p + 1];
+            const uint32_t X = pt[4 * p], g = pt[4 * p + 1] & 0x7FFFFFFFu, h = pt[4 * p + 1] >> 31;
             uint32_t o = g;
             for (uint32_t sh = 1; sh < 32; sh <<= 1) o ^= o >> sh;   // inverse Gray code
             const uint32_t lp = (uint32_t)((__popc(P & g) ^ __popc(dw & g)) & 1);
             const uint64_t idx = (lane << M) + o;
-            kout[idx] = (uint8_t)((L.K + c + 4u * lp) & 7u);
+            kout[idx] = (uint8_t)((L.K + c + 4u * (lp ^ h)) & 7u);
             hsout[idx] = L.hs ^ X;
             pout[idx] = L.pio;
         }

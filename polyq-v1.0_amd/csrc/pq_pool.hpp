@@ -42,9 +42,12 @@ public:
             current_ = task;
             gen_.fetch_add(1, std::memory_order_release);
         }
-        // polling workers see the new generation by themselves; wake at most
-        // as many blocked ones as there are chunks for others to take
-        for (unsigned i = 1; i < want; ++i) cv_.notify_one();
+        // Polling workers see the new generation by themselves (a worker
+        // counted here checks the generation again before it blocks), so
+        // only the chunks they cannot cover wake a blocked worker: in steady
+        // back-to-back use no wake-up system call is made at all.
+        const unsigned helpers = want - 1, polling = polling_.load(std::memory_order_acquire);
+        for (unsigned i = polling; i < helpers; ++i) cv_.notify_one();
         work(*task);
         while (task->done.load(std::memory_order_acquire) != task->chunks) std::this_thread::yield();
         std::lock_guard<std::mutex> lk(mu_);
@@ -84,10 +87,14 @@ private:
         uint64_t seen = 0;
         for (;;) {
             // hot workers poll for up to 200 us, then block
-            const auto t0 = std::chrono::steady_clock::now();
-            while (hot && gen_.load(std::memory_order_acquire) == seen &&
-                   std::chrono::steady_clock::now() - t0 < std::chrono::microseconds(200))
-                std::this_thread::yield();
+            if (hot) {
+                polling_.fetch_add(1, std::memory_order_acq_rel);
+                const auto t0 = std::chrono::steady_clock::now();
+                while (gen_.load(std::memory_order_acquire) == seen &&
+                       std::chrono::steady_clock::now() - t0 < std::chrono::microseconds(200))
+                    std::this_thread::yield();
+                polling_.fetch_sub(1, std::memory_order_acq_rel);
+            }
             std::shared_ptr<Task> t;
             {
                 std::unique_lock<std::mutex> lk(mu_);
@@ -102,6 +109,7 @@ private:
     std::mutex region_mu_, mu_;
     std::condition_variable cv_;
     std::atomic<uint64_t> gen_{0};
+    std::atomic<unsigned> polling_{0};   // hot workers in their polling loop
     std::shared_ptr<Task> current_;
 };
 

@@ -188,22 +188,30 @@ SegImage build_seg_image(const HostEngine& E) {
         }
         S.wtab[o] = X | ((U & 7u) << 24);
     }
-    S.pts.assign((size_t)4 * ((2u << S.Mcap) - 1), 0);
-    S.pcls.assign((size_t)9 * (S.Mcap + 1), 0);
+    // Point records per walk length M, grouped by class Ue mod 4: omega^4 =
+    // -1, so a point of class c + 4 is a point of class c with its magnitude
+    // negated, and four accumulators cover the eight phases.  Record
+    // {X, gray | h << 31, f hi, f lo} with h = Ue >> 2 and f = (-1)^h fmid.
+    // (plus 16 zero records, so a short walk may read 16 records from any block)
+    S.pts.assign((size_t)4 * ((2u << S.Mcap) - 1 + 16), 0);
+    S.pcls.assign((size_t)5 * (S.Mcap + 1), 0);
     for (uint32_t M = 0; M <= S.Mcap; ++M) {
         std::vector<uint32_t> idx(1u << M);
         for (uint32_t o = 0; o < (1u << M); ++o) idx[o] = o;
-        std::stable_sort(idx.begin(), idx.end(), [&](uint32_t a, uint32_t b) { return (S.wtab[a] >> 24) < (S.wtab[b] >> 24); });
+        std::stable_sort(idx.begin(), idx.end(), [&](uint32_t a, uint32_t b) { return ((S.wtab[a] >> 24) & 3u) < ((S.wtab[b] >> 24) & 3u); });
         const size_t base = (size_t)4 * ((1u << M) - 1);
         for (uint32_t p = 0; p < (1u << M); ++p) {
             const uint32_t o = idx[p];
+            const uint32_t U = S.wtab[o] >> 24, h = U >> 2;
             uint32_t* e = &S.pts[base + 4 * (size_t)p];
             e[0] = S.wtab[o] & 0xFFFFFFu;
-            e[1] = o ^ (o >> 1);
-            std::memcpy(&e[2], &S.fmid[2 * ((1u << M) - 1 + o)], 8);
-            S.pcls[9 * M + 1 + (S.wtab[o] >> 24)]++;
+            e[1] = (o ^ (o >> 1)) | (h << 31);
+            float fm[2] = {S.fmid[2 * ((1u << M) - 1 + o)], S.fmid[2 * ((1u << M) - 1 + o) + 1]};
+            if (h) { fm[0] = -fm[0]; fm[1] = -fm[1]; }
+            std::memcpy(&e[2], fm, 8);
+            S.pcls[5 * M + 1 + (U & 3u)]++;
         }
-        for (uint32_t c = 1; c <= 8; ++c) S.pcls[9 * M + c] += S.pcls[9 * M + c - 1];
+        for (uint32_t c = 1; c <= 4; ++c) S.pcls[5 * M + c] += S.pcls[5 * M + c - 1];
     }
     S.rmid.assign((size_t)4 * std::max<uint32_t>(S.Mcap, 1), 0.f);
     for (uint32_t m = 0; m < S.Mcap; ++m) {

@@ -276,7 +276,7 @@ struct Server {
                 std::vector<Z8Job> jobs;
                 for (auto& r : batch) for (auto& J : r->jobs) jobs.push_back(J);
                 if (jobs.empty()) { complete(batch, {}); continue; }
-                dev_z8_launch(*slots[s], jobs, walk);
+                dev_z8_launch(*slots[s], jobs, walk, true);
                 held[s] = std::move(batch);
                 if (order[0] < 0) order[0] = s; else { order[1] = s; overlapped++; }
                 std::lock_guard<std::mutex> lk(mu);

@@ -43,6 +43,8 @@ lib.pq_sim_marginal_probability.argtypes = [_sim_p, ctypes.c_uint64, _u32p, _u8p
 lib.pq_sim_amplitudes.argtypes = [_sim_p, ctypes.c_size_t, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p]
 lib.pq_batch_amplitudes.argtypes = [ctypes.c_size_t, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p]
 lib.pq_last_timing.argtypes = [_f64p]
+lib.pq_set_timing.argtypes = [ctypes.c_int]
+lib.pq_get_timing.argtypes = []
 lib.pq_set_walk_tuning.argtypes = [ctypes.c_int, ctypes.c_uint32, ctypes.c_uint32]
 lib.pq_get_walk_tuning.argtypes = [ctypes.c_int, _u32p]
 lib.pq_set_segment_bits.argtypes = [ctypes.c_uint32]
@@ -379,17 +381,18 @@ def autotune(sims, inputs, outputs, form="z8", offsets=None, max_mids=None, repe
     max_mids = max_mids or _TUNE_GRID[form][1]
     table = {}
     best = None
-    for off in offsets:
-        for mx in max_mids:
-            set_walk_tuning(off, mx, form)
-            ms = []
-            for _ in range(repeats):
-                batch_amplitudes(sims, inputs, outputs, form=form)
-                ms.append(last_timing()["kernel_ms"])
-            t = sorted(ms)[len(ms) // 2]
-            table["%d/%d" % (off, mx)] = t
-            if best is None or t < best[0]:
-                best = (t, off, mx)
+    with timed():
+        for off in offsets:
+            for mx in max_mids:
+                set_walk_tuning(off, mx, form)
+                ms = []
+                for _ in range(repeats):
+                    batch_amplitudes(sims, inputs, outputs, form=form)
+                    ms.append(last_timing()["kernel_ms"])
+                t = sorted(ms)[len(ms) // 2]
+                table["%d/%d" % (off, mx)] = t
+                if best is None or t < best[0]:
+                    best = (t, off, mx)
     set_walk_tuning(best[1], best[2], form)
     if cache:
         try:
@@ -422,7 +425,32 @@ def load_tuning(form="z8"):
     return None
 
 
+def set_timing(on):
+    """Record HIP timing events around eager batches (off by default: each
+    event is a queue barrier that adds microseconds to a short batch)."""
+    lib.pq_set_timing(1 if on else 0)
+
+
+def get_timing():
+    return bool(lib.pq_get_timing())
+
+
+class timed:
+    """Context manager: timing events on inside, previous setting after."""
+
+    def __enter__(self):
+        self._prev = get_timing()
+        set_timing(True)
+        return self
+
+    def __exit__(self, *exc):
+        set_timing(self._prev)
+        return False
+
+
 def last_timing():
+    """kernel_ms and total_ms (HIP events; NaN unless timing is on), branches,
+    items and grid of the last evaluation on this thread."""
     o = np.zeros(5, dtype=np.float64)
     lib.pq_last_timing(_ptr(o, _f64p))
     return dict(zip(("kernel_ms", "total_ms", "branches", "items", "grid"), o.tolist()))

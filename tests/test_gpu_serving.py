@@ -129,3 +129,24 @@ def test_large_batch_reduction_equals_fused_small_batches_bitwise():
     assert pq.last_timing()["items"] > 4096
     small = np.concatenate([s.amplitudes(xs[i:i + 8], ys[i:i + 8]) for i in range(0, 320, 8)])
     np.testing.assert_array_equal(big, small)
+
+
+def test_host_finish_equals_device_reduction_many_items_per_query():
+    # 1024 items per query (2^18 lanes of one point each): more items than
+    # reduction threads, finished on the host in a batch of one query and on
+    # the device in a batch of five.  Same order: same bits.
+    c, _, _ = W.config2(26)
+    s = pq.Simulator(c, strategy="cached")
+    assert s.seg_info()["outer"] == 18
+    prev = pq.get_walk_tuning()
+    try:
+        pq.set_walk_tuning(30, 0)
+        rng = W.Rng(321)
+        xs = [rng.below(1 << 40) for _ in range(5)]
+        ys = [0] * 5
+        big = s.amplitudes(xs, ys)
+        assert pq.last_timing()["items"] == 5 * 1024
+        small = np.concatenate([s.amplitudes(xs[i:i + 1], ys[i:i + 1]) for i in range(5)])
+        np.testing.assert_array_equal(big, small)
+    finally:
+        pq.set_walk_tuning(*prev)
