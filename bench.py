@@ -74,10 +74,11 @@ def _err_summary(got, ref):
 
 
 SEG_OPS_PER_POINT = 30.0
-# HBM bytes per k_seg_walk launch on these two shapes (FETCH_SIZE x 2 +
-# WRITE_SIZE, copy-calibrated; profiles/k_seg_walk_config2.md): the 256 KB
-# table once per XCD L2, descriptors and item partials.
-SEG_TRAFFIC = {"headline": 2.43e6, "saturated": 4.13e6}
+# HBM bytes per walk launch on these two shapes at the named lane walk
+# tuning (FETCH_SIZE x 2 + WRITE_SIZE, copy-calibrated;
+# profiles/k_seg_walk_config2.md): the 256 KB table once per XCD L2 and the
+# item partials.
+SEG_TRAFFIC = {("headline", (12, 8)): 2.50e6, ("saturated", (8, 8)): 3.34e6}
 # Algorithmic lane-ops per outer point of k_seg_walk: the integer walk step
 # (phase update with the cross-term parity, sign-mask flip, table index: 6)
 # and the double-float arithmetic (walk magnitude times the complex table
@@ -92,11 +93,11 @@ def _peak(dev):
     return cus * 128 * 2.4e9
 
 
-def _seg_roofline(dev, points, kernel_ms, extra=None, traffic=None):
+def _seg_roofline(dev, points, kernel_ms, extra=None, traffic=None, kernel="k_seg_walk"):
     peak = _peak(dev)
     achieved = points * SEG_OPS_PER_POINT / (kernel_ms * 1e-3)
     r = {"bound": "valu", "achieved": achieved / 1e12, "peak": peak / 1e12, "unit": "Tlane-op/s", "frac": achieved / peak,
-         "traffic": traffic, "kernel": "k_seg_walk", "kernel_avg_ms": kernel_ms, "ops_per_outer_point": SEG_OPS_PER_POINT,
+         "traffic": traffic, "kernel": kernel, "kernel_avg_ms": kernel_ms, "ops_per_outer_point": SEG_OPS_PER_POINT,
          "outer_points_per_launch": points}
     if extra:
         r.update(extra)
@@ -116,7 +117,11 @@ def headline(args, pq, W, np):
     points_per_step = nq * (1 << seg["outer"])
 
     # lane walk length autotuned for this batch shape (16 queries of one circuit)
-    tuned = pq.autotune([sim] * nq, xs, ys, repeats=3, cache=False)[:2]
+    if args.headline_tuning:
+        tuned = tuple(int(v) for v in args.headline_tuning.split(","))
+        pq.set_walk_tuning(*tuned)
+    else:
+        tuned = pq.autotune([sim] * nq, xs, ys, repeats=7, cache=False)[:2]
     for _ in range(args.warmup):
         sim.amplitudes(xs, ys)
     t0 = time.perf_counter()
@@ -143,7 +148,10 @@ def headline(args, pq, W, np):
     # Roofline of k_seg_walk on this launch (16 queries: 2^20 outer points,
     # too few lanes to fill 256 CUs); the saturating launch is measured by
     # saturated() below.
-    res["roofline"] = _seg_roofline(dev, points_per_step, kernel_avg_ms, traffic=SEG_TRAFFIC["headline"])
+    # 16 queries over one engine travel as kernel arguments; walks of at most
+    # 16 points run the short-walk instance
+    kern = "k_seg_walk_args<%s>" % ("true" if sim.seg_info()["mid_bits"] <= 4 else "false")
+    res["roofline"] = _seg_roofline(dev, points_per_step, kernel_avg_ms, traffic=SEG_TRAFFIC.get(("headline", tuple(tuned))), kernel=kern)
 
     # The per-branch Z8 walk on the same 16 amplitudes (every branch visited:
     # one sign XOR and one FMA per branch, one more LDS lookup with inner pivots).
@@ -219,7 +227,7 @@ def saturated(args, pq, W, np, sim=None):
         off, mx = (int(v) for v in args.saturated_tuning.split(","))
         pq.set_walk_tuning(off, mx)
     else:
-        off, mx, table = pq.autotune(S, big_x, big_y, repeats=2, cache=False)
+        off, mx, table = pq.autotune(S, big_x, big_y, repeats=2, cache=False, objective="kernel")
     bms, bwall = [], []
     for _ in range(args.saturated_reps):
         tb = time.perf_counter()
@@ -235,7 +243,7 @@ def saturated(args, pq, W, np, sim=None):
                          {"queries": 4096, "walk_tuning": [off, mx], "mid_bits": tuned,
                           "amplitudes_per_s": 4096 / float(np.median(bwall)),
                           "branches_per_s": 4096 * (1 << d) / float(np.median(bwall))},
-                         traffic=SEG_TRAFFIC["saturated"] if (off, mx) == (8, 8) else None)
+                         traffic=SEG_TRAFFIC.get(("saturated", (off, mx))), kernel="k_seg_walk<false>")
 
 
 def precision(args, pq, W, np):
@@ -419,6 +427,8 @@ def main():
     ap.add_argument("--saturated-reps", type=int, default=5)
     ap.add_argument("--saturated-tuning", default=None, help="offset,max_mid for the saturating launch instead of autotuning")
     ap.add_argument("--precision-out", default=None)
+    ap.add_argument("--headline-tuning", default="",
+                    help="offset,max_mid of the headline lane walk instead of autotuning it (profiling runs)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run three 256 MiB device copies (reference traffic for FETCH_SIZE/WRITE_SIZE counters)")
     args = ap.parse_args()

@@ -371,28 +371,40 @@ _TUNE_GRID = {"z8": ((8, 10, 11, 12, 13, 14, 16), (0, 2, 4, 6, 8)),
               "branch_walk": ((23, 24, 25, 26, 27, 28, 29), (4, 6, 8))}
 
 
-def autotune(sims, inputs, outputs, form="z8", offsets=None, max_mids=None, repeats=3, cache=True):
+def autotune(sims, inputs, outputs, form="z8", offsets=None, max_mids=None, repeats=3, cache=True, objective="wall"):
     """Pick the lane walk split (offset, max_mid) of a batched form that
-    minimises the walk-kernel time of this batch; apply it and, if cache,
-    store it per device and form in ~/.cache/polyq_amd/walk_tuning.json.
-    Returns (offset, max_mid, {"offset/max_mid": kernel ms})."""
+    minimises the time of this batch: its wall time per call (objective
+    "wall", the default: host prologue, launch and wait included) or its walk
+    kernel time from HIP events ("kernel").  Applies it and, if cache, stores
+    it per device and form in ~/.cache/polyq_amd/walk_tuning.json.
+    Returns (offset, max_mid, {"offset/max_mid": ms})."""
     import json
+    import time
     offsets = offsets or _TUNE_GRID[form][0]
     max_mids = max_mids or _TUNE_GRID[form][1]
+    xs = np.ascontiguousarray(inputs, dtype=np.uint64)
+    ys = np.ascontiguousarray(outputs, dtype=np.uint64)
     table = {}
     best = None
-    with timed():
+    prev = get_timing()
+    set_timing(objective == "kernel")
+    try:
         for off in offsets:
             for mx in max_mids:
                 set_walk_tuning(off, mx, form)
+                batch_amplitudes(sims, xs, ys, form=form)     # first call of a split: not timed
                 ms = []
                 for _ in range(repeats):
-                    batch_amplitudes(sims, inputs, outputs, form=form)
-                    ms.append(last_timing()["kernel_ms"])
+                    t0 = time.perf_counter()
+                    batch_amplitudes(sims, xs, ys, form=form)
+                    wall = 1e3 * (time.perf_counter() - t0)
+                    ms.append(last_timing()["kernel_ms"] if objective == "kernel" else wall)
                 t = sorted(ms)[len(ms) // 2]
                 table["%d/%d" % (off, mx)] = t
                 if best is None or t < best[0]:
                     best = (t, off, mx)
+    finally:
+        set_timing(prev)
     set_walk_tuning(best[1], best[2], form)
     if cache:
         try:
