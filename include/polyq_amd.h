@@ -100,8 +100,10 @@ int pq_batch_amplitudes(size_t n, pq_sim* const* sims, const uint64_t* inputs, c
  * independent of the batch it runs in. */
 int pq_set_walk_tuning(int form, uint32_t offset, uint32_t max_mid);
 int pq_get_walk_tuning(int form, uint32_t out2[2]);
-/* Table bits I + npin of the segment tables of engines built afterwards
- * (default 14: 2^14 entries x 8 rotations x 16 B = 2 MiB per engine). */
+/* Cap on the table bits I + npin of the segment tables of engines built
+ * afterwards (default 16).  An engine of branch dimension d gets
+ * min(cap, max(14, d - 14)) bits: 2^bits entries x 16 B, so 256 KB at 14 bits
+ * and 1 MB at 16. */
 int pq_set_segment_bits(uint32_t bits);
 uint32_t pq_get_segment_bits(void);
 

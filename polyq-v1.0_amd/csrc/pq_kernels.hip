@@ -501,6 +501,30 @@ __device__ __forceinline__ void merge2(float& s, float& c, float s2, float c2) {
     c = (c + c2) + e;
 }
 
+// One step of a wave reduction of four double-float pairs over DPP: every
+// lane takes the pairs of the lane CTRL selects (0 in rows ROWS leaves out)
+// and merges them.  Register-to-register, no LDS round trip.
+template <int CTRL, int ROWS>
+__device__ __forceinline__ void dpp_merge(float (&v)[8]) {
+    float p[8];
+#pragma unroll
+    for (int i = 0; i < 8; ++i) p[i] = __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v[i]), CTRL, ROWS, 0xF, false));
+#pragma unroll
+    for (int i = 0; i < 8; i += 2) merge2(v[i], v[i + 1], p[i], p[i + 1]);
+}
+// Sum of the four pairs over the 64 lanes, in a fixed order, left in lane 63:
+// pairs and quads (quad_perm), 8 and 16 lanes (half-row and row mirror),
+// then rows 0+1 and 2+3 (row_bcast:15 into rows 1, 3) and all four
+// (row_bcast:31 into rows 2, 3).  All 64 lanes must be active.
+__device__ __forceinline__ void wave_sum_dpp(float (&v)[8]) {
+    dpp_merge<0xB1, 0xF>(v);     // quad_perm [1,0,3,2]
+    dpp_merge<0x4E, 0xF>(v);     // quad_perm [2,3,0,1]
+    dpp_merge<0x141, 0xF>(v);    // row_half_mirror
+    dpp_merge<0x140, 0xF>(v);    // row_mirror
+    dpp_merge<0x142, 0xA>(v);    // row_bcast:15
+    dpp_merge<0x143, 0xC>(v);    // row_bcast:31
+}
+
 // Rotate (xr, xi) by i^((K>>1)&3) and add into the even or odd accumulator
 // (both pairs are updated, the other with zero, so the accumulator index stays
 // static and the array stays in registers).
@@ -915,14 +939,14 @@ __device__ __forceinline__ void seg_item(const SegLds& S, const SegDev& E, const
         GFloat4* tab = (GFloat4*)E.table;
         const PQ_CONST uint32_t* pt = cview(E.pts) + 4 * ((1u << M) - 1u);
         const PQ_CONST uint32_t* cl = cview(E.pcls) + 5 * M;
-        float A[4][4];
-#pragma unroll
-        for (int c = 0; c < 4; ++c) { A[c][0] = 0.f; A[c][1] = 0.f; A[c][2] = 0.f; A[c][3] = 0.f; }
+        // the segment's sums EVu and ODu (double-float re, im)
+        float eu[4] = {0.f, 0.f, 0.f, 0.f}, ou[4] = {0.f, 0.f, 0.f, 0.f};
         if constexpr (SHORT) {
             // Short walk (at most 2^kSegShortWalk points): every table
             // read is issued before the first sum, so the walk costs one
-            // memory latency; point k adds into the accumulator of the
-            // class range that holds it (uniform tests, static registers).
+            // memory latency.  A point of class c (uniform) adds i^(c>>1)
+            // times its term into EVu (c even) or ODu (c odd): one compact
+            // path per point, no per-class code.
             constexpr int NP = 1 << kSegShortWalk;
             const uint32_t np = 1u << M;
             // The NP index words first, unconditionally (the record array is
@@ -948,13 +972,20 @@ __device__ __forceinline__ void seg_item(const SegLds& S, const SegDev& E, const
                     const uint32_t sg = (uint32_t)((__popc(P & g) ^ __popc(dw & g)) & 1) << 31;
                     const float wh = __uint_as_float(e[2] ^ sg), wl = __uint_as_float(e[3] ^ sg);
                     const uint32_t u = (uint32_t)k;
-                    if (u < c1) seg_point(A[0], wh, wl, Tk[k]);
-                    else if (u < c2) seg_point(A[1], wh, wl, Tk[k]);
-                    else if (u < c3) seg_point(A[2], wh, wl, Tk[k]);
-                    else seg_point(A[3], wh, wl, Tk[k]);
+                    const uint32_t c = (u >= c1 ? 1u : 0u) + (u >= c2 ? 1u : 0u) + (u >= c3 ? 1u : 0u);
+                    float ph, pl, qh, ql;
+                    df_mul(wh, wl, Tk[k].x, Tk[k].y, ph, pl);
+                    df_mul(wh, wl, Tk[k].z, Tk[k].w, qh, ql);
+                    const bool r = c >= 2;       // times i: (x, y) -> (-y, x)
+                    const float xh = r ? -qh : ph, xl = r ? -ql : pl, yh = r ? ph : qh, yl = r ? pl : ql;
+                    if (c & 1u) { df_add(ou[0], ou[1], xh, xl); df_add(ou[2], ou[3], yh, yl); }
+                    else { df_add(eu[0], eu[1], xh, xl); df_add(eu[2], eu[3], yh, yl); }
                 }
             }
         } else {
+            float A[4][4];
+#pragma unroll
+            for (int c = 0; c < 4; ++c) { A[c][0] = 0.f; A[c][1] = 0.f; A[c][2] = 0.f; A[c][3] = 0.f; }
             // points grouped by their engine phase class c: the class is
             // a loop constant, so each group adds into its own accumulator
 #pragma unroll
@@ -988,11 +1019,10 @@ __device__ __forceinline__ void seg_item(const SegLds& S, const SegDev& E, const
                     seg_point(A[c], __uint_as_float(e[2] ^ sg), __uint_as_float(e[3] ^ sg), Tp);
                 }
             }
+            seg_fold4<0>(A, eu);
+            seg_fold4<1>(A, ou);
         }
         // lane total = omega^K_seed F (EVu + omega ODu)
-        float eu[4], ou[4];
-        seg_fold4<0>(A, eu);
-        seg_fold4<1>(A, ou);
         const uint32_t Ks = L.K & 7u;
         float e2[4], o2[4];
         if (Ks & 1u) {   // omega^(2a+1) (EVu + omega ODu) = i^a (i ODu + omega EVu)
@@ -1010,20 +1040,13 @@ __device__ __forceinline__ void seg_item(const SegLds& S, const SegDev& E, const
             df_mul(o2[c], o2[c + 1], fh, fl, od[c], od[c + 1]);
         }
     }
-    // workgroup reduction of the four double-float sums
+    // workgroup reduction of the four double-float sums: each wave over DPP
+    // (total in lane 63), then the four waves through LDS
+    float wv[8] = {ev[0], ev[1], ev[2], ev[3], od[0], od[1], od[2], od[3]};
+    wave_sum_dpp(wv);
+    if ((tid & 63) == 63) {
 #pragma unroll
-    for (int off = 32; off > 0; off >>= 1) {
-#pragma unroll
-        for (int c = 0; c < 4; c += 2) {
-            const float a = __shfl_xor(ev[c], off), b = __shfl_xor(ev[c + 1], off);
-            const float e = __shfl_xor(od[c], off), f = __shfl_xor(od[c + 1], off);
-            merge2(ev[c], ev[c + 1], a, b);
-            merge2(od[c], od[c + 1], e, f);
-        }
-    }
-    if ((tid & 63) == 0) {
-#pragma unroll
-        for (int c = 0; c < 4; ++c) { s_red[tid >> 6][c] = ev[c]; s_red[tid >> 6][4 + c] = od[c]; }
+        for (int c = 0; c < 8; ++c) s_red[tid >> 6][c] = wv[c];
     }
     __syncthreads();
     // The item's four partial sums (EV re, im, OD re, im), one per lane

@@ -29,7 +29,15 @@
 
 namespace pq {
 
-static std::atomic<uint32_t> g_seg_bits{14};
+// Cap on the table bits I + npin.  An engine of branch dimension d gets
+// min(cap, max(14, d - 14)) bits: its table is no larger than the outer walk
+// of one query, so engines queried a few times (served, batched) keep cheap
+// 2^14-entry tables while large resident engines walk 4x fewer points.
+static std::atomic<uint32_t> g_seg_bits{16};
+uint32_t seg_bits_for(uint32_t d) {
+    const uint32_t cap = g_seg_bits;
+    return std::min(cap, std::max<uint32_t>(14, d > 14 ? d - 14 : 0));
+}
 void seg_set_bits(uint32_t b) {
     if (b > 20) throw std::invalid_argument("segment table bits above 20");
     g_seg_bits = b;
@@ -64,6 +72,28 @@ std::vector<uint32_t> f2_inverse_cols(const std::vector<uint32_t>& cols, uint32_
 }
 }  // namespace
 
+namespace {
+// In-place Walsh-Hadamard transform of n = 2^k doubles, two radix-2 stages
+// per pass (the same additions in the same order as stage-by-stage radix 2,
+// in half the passes over memory).
+void fwht(double* v, uint32_t n) {
+    uint32_t len = 1;
+    for (; 4 * len <= n; len *= 4)
+        for (uint32_t base = 0; base < n; base += 4 * len)
+            for (uint32_t x = base; x < base + len; ++x) {
+                const double a0 = v[x], a1 = v[x + len], a2 = v[x + 2 * len], a3 = v[x + 3 * len];
+                const double s01 = a0 + a1, d01 = a0 - a1, s23 = a2 + a3, d23 = a2 - a3;
+                v[x] = s01 + s23; v[x + len] = d01 + d23;
+                v[x + 2 * len] = s01 - s23; v[x + 3 * len] = d01 - d23;
+            }
+    if (len < n)
+        for (uint32_t x = 0; x < len; ++x) {
+            const double a = v[x], b = v[x + len];
+            v[x] = a + b; v[x + len] = a - b;
+        }
+}
+}  // namespace
+
 SegImage build_seg_image(const HostEngine& E) {
     SegImage S;
     const uint32_t d = E.d, t = E.t;
@@ -75,7 +105,7 @@ SegImage build_seg_image(const HostEngine& E) {
 
     // Inner directions within a table of 2^(I + npin) entries: pivot-free ones
     // first, then those adding the fewest new pivots.
-    const uint32_t B = g_seg_bits;
+    const uint32_t B = seg_bits_for(d);
     std::vector<uint8_t> used(d, 0);
     uint64_t pin_set = 0;
     for (uint32_t j = 0; j < d && S.inner_j.size() < B; ++j)
@@ -244,6 +274,10 @@ SegImage build_seg_image(const HostEngine& E) {
     static const double H = 0.70710678118654752440;
     const double w8r[8] = {1, H, 0, -H, -1, -H, 0, H}, w8i[8] = {0, H, 1, H, 0, -H, -1, -H};
     S.table.assign((size_t)NP * NI * 4, 0.f);
+    // table index hp -> transform index h = B hp (B: the columns bcols), one
+    // XOR per entry from the entry with the lowest set bit cleared
+    std::vector<uint32_t> hof(NI, 0);
+    for (uint32_t hp = 1; hp < NI; ++hp) hof[hp] = hof[hp & (hp - 1)] ^ bcols[__builtin_ctz(hp)];
     std::vector<double> re(NI), im(NI);
     for (uint32_t pio = 0; pio < NP; ++pio) {
         for (uint32_t c = 0; c < NI; ++c) {
@@ -251,16 +285,10 @@ SegImage build_seg_image(const HostEngine& E) {
             re[c] = w * w8r[kin[c]];
             im[c] = w * w8i[kin[c]];
         }
-        for (uint32_t len = 1; len < NI; len <<= 1)
-            for (uint32_t base = 0; base < NI; base += 2 * len)
-                for (uint32_t x = base; x < base + len; ++x) {
-                    const double ar = re[x], ai = im[x], br = re[x + len], bi = im[x + len];
-                    re[x] = ar + br; im[x] = ai + bi;
-                    re[x + len] = ar - br; im[x + len] = ai - bi;
-                }
+        fwht(re.data(), NI);
+        fwht(im.data(), NI);
         for (uint32_t hp = 0; hp < NI; ++hp) {
-            uint32_t h = 0;
-            for (uint32_t i = 0; i < I; ++i) if ((hp >> i) & 1) h ^= bcols[i];
+            const uint32_t h = hof[hp];
             float* dst = &S.table[((size_t)pio * NI + hp) * 4];
             split_df(re[h], dst[0], dst[1]);
             split_df(im[h], dst[2], dst[3]);

@@ -359,7 +359,9 @@ def get_walk_tuning(form="z8"):
 
 
 def set_segment_bits(bits):
-    """Table bits I + npin of the segment tables of simulators created afterwards."""
+    """Cap on the table bits I + npin of the segment tables of simulators
+    created afterwards (default 16); an engine of branch dimension d gets
+    min(cap, max(14, d - 14))."""
     _check(lib.pq_set_segment_bits(int(bits)))
 
 

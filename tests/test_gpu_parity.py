@@ -258,6 +258,19 @@ def test_segment_form_config3_shape_per_amplitude():
     assert rel.max() < 1e-9
 
 
+def test_headline_workload_per_amplitude_with_large_tables():
+    # n = 40, 36 T (d = 30: 2^16-entry tables by default): the 16 amplitudes
+    # of a headline step within 1e-9 of FP64, each relative to itself
+    c, xs, ys = W.config2(5)
+    z = pq.Simulator(c, strategy="cached")
+    assert z.seg_info()["inner"] == 16
+    got = z.amplitudes(xs, ys)
+    want = pq.Simulator(c, strategy="cached", form="plain_fp64").amplitudes(xs, ys)
+    rel = np.abs(got - want) / np.abs(want)
+    print("headline workload: max rel", rel.max(), "median", np.median(rel))
+    assert rel.max() < 1e-9
+
+
 def test_marginal_with_a_normal_form_beyond_the_plain_kernel():
     # A 56-qubit circuit of the batched-workload generator whose 4-qubit
     # marginal sandwich has 1030 lines: the plain FP64 image (<= 1024 planes or

@@ -136,7 +136,12 @@ def test_host_finish_equals_device_reduction_many_items_per_query():
     # reduction threads, finished on the host in a batch of one query and on
     # the device in a batch of five.  Same order: same bits.
     c, _, _ = W.config2(26)
-    s = pq.Simulator(c, strategy="cached")
+    bits = pq.get_segment_bits()
+    pq.set_segment_bits(14)
+    try:
+        s = pq.Simulator(c, strategy="cached")
+    finally:
+        pq.set_segment_bits(bits)
     assert s.seg_info()["outer"] == 18
     prev = pq.get_walk_tuning()
     try:

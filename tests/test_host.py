@@ -140,6 +140,23 @@ def test_segment_tables_match_direct_inner_sums(bits):
         pq.set_segment_bits(old)
 
 
+@pytest.mark.parametrize("cap", [16, 17])
+def test_large_segment_tables_match_direct_inner_sums(cap):
+    # engines of branch dimension 30 and 32 get min(cap, d - 14) table bits
+    old = pq.get_segment_bits()
+    pq.set_segment_bits(cap)
+    try:
+        for seed, d in ((5, 30), (26, 32)):
+            c, xs, ys = W.config2(seed)
+            s = pq.Simulator(c, strategy="cached")
+            info = s.seg_info()
+            assert info["d"] == d
+            assert info["inner"] + info["inner_pivots"] == min(cap, d - 14)
+            assert s.seg_selfcheck(xs[0], ys[0], trials=3) < 1e-12
+    finally:
+        pq.set_segment_bits(old)
+
+
 def test_segment_lane_state_matches_quadratic_form():
     # the host restatement of the device walk state: K at every outer point
     # equals K0 + sum l + 4 sum Q over the outer coordinates, and walking
