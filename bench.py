@@ -74,11 +74,11 @@ def _err_summary(got, ref):
 
 
 SEG_OPS_PER_POINT = 30.0
-# HBM bytes per walk launch on these two shapes at the named lane walk
-# tuning (FETCH_SIZE x 2 + WRITE_SIZE, copy-calibrated;
-# profiles/k_seg_walk_config2.md): the 256 KB table once per XCD L2 and the
-# item partials.
-SEG_TRAFFIC = {("headline", (12, 8)): 2.50e6, ("saturated", (8, 8)): 3.34e6}
+# HBM bytes per walk launch, keyed by launch shape, table bits and lane walk
+# length (FETCH_SIZE x 2 + WRITE_SIZE, copy-calibrated;
+# profiles/k_seg_walk_config2.md): the table once per XCD L2 and the item
+# partials.
+SEG_TRAFFIC = {("headline", 16, 2): 1.90e6, ("saturated", 16, 6): 5.38e6}
 # Algorithmic lane-ops per outer point of k_seg_walk: the integer walk step
 # (phase update with the cross-term parity, sign-mask flip, table index: 6)
 # and the double-float arithmetic (walk magnitude times the complex table
@@ -150,8 +150,11 @@ def headline(args, pq, W, np):
     # saturated() below.
     # 16 queries over one engine travel as kernel arguments; walks of at most
     # 16 points run the short-walk instance
-    kern = "k_seg_walk_args<%s>" % ("true" if sim.seg_info()["mid_bits"] <= 4 else "false")
-    res["roofline"] = _seg_roofline(dev, points_per_step, kernel_avg_ms, traffic=SEG_TRAFFIC.get(("headline", tuple(tuned))), kernel=kern)
+    seg = sim.seg_info()            # with the tuned lane walk length
+    res["seg"] = seg
+    kern = "k_seg_walk_args<%s>" % ("true" if seg["mid_bits"] <= 4 else "false")
+    res["roofline"] = _seg_roofline(dev, points_per_step, kernel_avg_ms, kernel=kern,
+                                    traffic=SEG_TRAFFIC.get(("headline", seg["inner"], seg["mid_bits"])))
 
     # The per-branch Z8 walk on the same 16 amplitudes (every branch visited:
     # one sign XOR and one FMA per branch, one more LDS lookup with inner pivots).
@@ -243,7 +246,7 @@ def saturated(args, pq, W, np, sim=None):
                          {"queries": 4096, "walk_tuning": [off, mx], "mid_bits": tuned,
                           "amplitudes_per_s": 4096 / float(np.median(bwall)),
                           "branches_per_s": 4096 * (1 << d) / float(np.median(bwall))},
-                         traffic=SEG_TRAFFIC.get(("saturated", (off, mx))), kernel="k_seg_walk<false>")
+                         traffic=SEG_TRAFFIC.get(("saturated", seg["inner"], tuned)), kernel="k_seg_walk<false>")
 
 
 def precision(args, pq, W, np):
