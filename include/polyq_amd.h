@@ -84,6 +84,13 @@ int pq_sim_probability(pq_sim* sim, uint64_t input, uint64_t output, int form, d
  * rebuilds it on every call). */
 int pq_sim_marginal_probability(pq_sim* sim, uint64_t input, const uint32_t* qubits, const uint8_t* outcomes, size_t k,
                                 int form, double* out);
+/* Simulator::statevector (src/sim.rs:201-213): all 2^n amplitudes <y|C|x>,
+ * out holds 2 * 2^n doubles (re, im by y); n < 48. */
+int pq_sim_statevector(pq_sim* sim, uint64_t input, int form, double* out);
+/* Simulator::sample_many (src/sim.rs:253-284, pybind.rs:169-172): shots
+ * measurement outcomes of all qubits as basis indices, by the chain rule over
+ * qubit prefixes with the reference's per-shot SplitMix64 streams. */
+int pq_sim_sample_many(pq_sim* sim, uint64_t input, size_t shots, uint64_t seed, int form, uint64_t* out);
 /* Many amplitudes <outputs[i]|C|inputs[i]> of one simulator in one launch
  * (the query loop of Simulator::statevector, src/sim.rs:201-213). */
 int pq_sim_amplitudes(pq_sim* sim, size_t n, const uint64_t* inputs, const uint64_t* outputs, int form, double* out);

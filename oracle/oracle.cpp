@@ -20,7 +20,10 @@
 //                      closed-form Gauss sum vs 2^m brute force (normal_form.rs:608-625)
 //   engine / rank /    engine vs brute force, GHZ, HTZH (engine.rs:745-802),
 //   simulator          rank == cached (rank.rs:243-276), statevector and
-//                      marginals vs dense (sim.rs:329-379)
+//                      marginals vs dense (sim.rs:329-379); sample_many: every
+//                      drawn outcome has a nonzero dense probability, shot
+//                      frequencies follow the dense distribution, and the
+//                      chain-rule prefix marginals equal dense sums
 //   per-branch phase8  2^m brute-force Gauss sum at the same shift
 // Parity unpinned (implementation-defined in the reference and not observable
 // through any mathematical check): the exact basis vectors chosen by the
@@ -974,22 +977,83 @@ struct Simulator {  // sim.rs:128-135
         if (!ps.boundary(in, out, bd)) return C64(0, 0);
         return engine.eval_assignment(bd);
     }
-    // sim.rs:225-231 with Sandwich::fixed_for / assignment (sim.rs:95-124)
-    double marginal(const std::vector<bool>& in, const std::vector<std::pair<size_t, bool>>& measured) {
+    // Sandwich::assignment (sim.rs:106-124); false when contradictory
+    bool sw_assignment(const std::vector<bool>& in, const std::vector<std::pair<size_t, bool>>& measured,
+                       std::vector<std::pair<size_t, bool>>& a) {
         if (!have_sw) { sw = sandwich(circuit); have_sw = true; }
-        std::vector<std::pair<size_t, bool>> a;
+        a.clear();
         auto put = [&](size_t var, bool val) {
             for (auto& e : a) if (e.first == var) return e.second == val;
             a.push_back({var, val}); return true;
         };
-        for (size_t q = 0; q < sw.ps.num_qubits; ++q) if (!put(q, in[q])) return 0.0;
-        for (size_t q = 0; q < sw.ps.num_qubits; ++q) if (!put(sw.ps.outputs[q], in[q] ^ sw.ps.out_neg[q])) return 0.0;
-        for (auto& m : measured) if (!put(sw.junction[m.first], m.second ^ sw.jneg[m.first])) return 0.0;
+        for (size_t q = 0; q < sw.ps.num_qubits; ++q) if (!put(q, in[q])) return false;
+        for (size_t q = 0; q < sw.ps.num_qubits; ++q) if (!put(sw.ps.outputs[q], in[q] ^ sw.ps.out_neg[q])) return false;
+        for (auto& m : measured) if (!put(sw.junction[m.first], m.second ^ sw.jneg[m.first])) return false;
+        return true;
+    }
+    // Sandwich::fixed_for (sim.rs:95-103)
+    std::vector<size_t> sw_fixed_for(const std::vector<size_t>& qubits) {
         std::vector<size_t> f = sw.ps.io_vars();
-        for (auto& m : measured) if (std::find(f.begin(), f.end(), sw.junction[m.first]) == f.end()) f.push_back(sw.junction[m.first]);
-        RankEngine eng = build_engine(sw.ps, f, strategy, pairs_opt);
+        for (size_t q : qubits) if (std::find(f.begin(), f.end(), sw.junction[q]) == f.end()) f.push_back(sw.junction[q]);
+        return f;
+    }
+    // sim.rs:225-231 with Sandwich::fixed_for / assignment (sim.rs:95-124)
+    double marginal(const std::vector<bool>& in, const std::vector<std::pair<size_t, bool>>& measured) {
+        std::vector<std::pair<size_t, bool>> a;
+        if (!sw_assignment(in, measured, a)) return 0.0;
+        std::vector<size_t> qs;
+        for (auto& m : measured) qs.push_back(m.first);
+        RankEngine eng = build_engine(sw.ps, sw_fixed_for(qs), strategy, pairs_opt);
         eng.set_threads(threads);
         return std::max(eng.eval_assignment(a).re, 0.0);
+    }
+    // sim.rs:234-239: engines with the first l qubits' junctions fixed, l = 1..n
+    std::vector<RankEngine> prefix;
+    void prefix_engines() {
+        if (!prefix.empty()) return;
+        if (!have_sw) { sw = sandwich(circuit); have_sw = true; }
+        std::vector<size_t> qs;
+        for (size_t l = 1; l <= ps.num_qubits; ++l) {
+            qs.push_back(l - 1);
+            prefix.push_back(build_engine(sw.ps, sw_fixed_for(qs), strategy, pairs_opt));
+            prefix.back().set_threads(threads);
+        }
+    }
+    // sim.rs:241-250
+    double prefix_marginal(const std::vector<bool>& in, const std::vector<bool>& y) {
+        if (y.empty()) return 1.0;
+        std::vector<std::pair<size_t, bool>> measured, a;
+        for (size_t q = 0; q < y.size(); ++q) measured.push_back({q, y[q]});
+        if (!sw_assignment(in, measured, a)) return 0.0;
+        prefix_engines();
+        return std::max(prefix[y.size() - 1].eval_assignment(a).re, 0.0);
+    }
+    // sim.rs:253-269: chain rule over qubit prefixes
+    std::vector<bool> sample(const std::vector<bool>& in, Rng& rng) {
+        const size_t n = ps.num_qubits;
+        std::vector<bool> y;
+        double p_prev = 1.0;
+        for (size_t q = 0; q < n; ++q) {
+            y.push_back(false);
+            const double p0 = std::min(prefix_marginal(in, y), p_prev);
+            const bool one = rng.next_f64() * p_prev >= p0;
+            if (one) { y.back() = true; p_prev = std::max(p_prev - p0, 0.0); }
+            else p_prev = p0;
+        }
+        return y;
+    }
+    // sim.rs:272-284 (sequential over shots: each shot has its own stream)
+    std::vector<uint64_t> sample_many(const std::vector<bool>& in, size_t shots, uint64_t seed) {
+        Rng rng(seed);
+        prefix_engines();
+        std::vector<uint64_t> seeds(shots), out(shots, 0);
+        for (auto& s : seeds) s = rng.next_u64();
+        for (size_t i = 0; i < shots; ++i) {
+            Rng r(seeds[i]);
+            const std::vector<bool> y = sample(in, r);
+            for (size_t q = 0; q < y.size(); ++q) if (y[q]) out[i] |= 1ull << q;
+        }
+        return out;
     }
 };
 
@@ -1169,6 +1233,16 @@ int orc_sim_amplitude(const orc_sim* h, const uint8_t* x, const uint8_t* y, doub
         size_t n = h->s.ps.num_qubits;
         C64 a = h->s.amplitude(bits_of(n, x), bits_of(n, y));
         out[0] = a.re; out[1] = a.im; return 0;
+    } catch (std::exception& e) { g_err = e.what(); return -1; }
+}
+// Simulator::sample_many (sim.rs:272-284); out holds shots basis indices.
+int orc_sim_sample_many(orc_sim* h, const uint8_t* x, size_t shots, uint64_t seed, uint64_t* out) {
+    try {
+        size_t n = h->s.ps.num_qubits;
+        if (n > 64) throw std::invalid_argument("sampling more than 64 qubits");
+        auto r = h->s.sample_many(bits_of(n, x), shots, seed);
+        std::copy(r.begin(), r.end(), out);
+        return 0;
     } catch (std::exception& e) { g_err = e.what(); return -1; }
 }
 int orc_sim_marginal(orc_sim* h, const uint8_t* x, const uint32_t* qubits, const uint8_t* outcomes, size_t k, double* out) {

@@ -338,6 +338,112 @@ int pq_sim_marginal_probability(pq_sim* h, uint64_t x, const uint32_t* qubits, c
     })
 }
 
+// Simulator::statevector (sim.rs:201-213): all 2^n amplitudes <y|C|x>, in
+// batches of at most 2^16 queries through the batched form.
+int pq_sim_statevector(pq_sim* h, uint64_t x, int form, double* out) {
+    PQ_TRY({
+        const uint32_t n = h->s.ps.nq;
+        if (n >= 48) throw std::invalid_argument("statevector of 48 or more qubits");
+        if (!qubits_ok(h, x)) throw std::invalid_argument("basis index has bits beyond the qubit count");
+        const uint64_t N = 1ull << n;
+        constexpr uint64_t kChunk = 1ull << 16;
+        std::vector<uint64_t> xs, ys;
+        for (uint64_t y0 = 0; y0 < N; y0 += kChunk) {
+            const uint64_t m = std::min(kChunk, N - y0);
+            xs.assign(m, x); ys.resize(m);
+            for (uint64_t i = 0; i < m; ++i) ys[i] = y0 + i;
+            const int rc = pq_sim_amplitudes(h, m, xs.data(), ys.data(), form, out + 2 * y0);
+            if (rc != 0) return rc;
+        }
+        return 0;
+    })
+}
+
+namespace {
+// SplitMix64 as the reference's Rng (rng.rs:9-27): state = seed + gamma,
+// next adds gamma to the state and mixes the new state; next_f64 takes the
+// top 53 bits.
+struct SplitMix {
+    uint64_t s;
+    explicit SplitMix(uint64_t seed) : s(seed + 0x9E3779B97F4A7C15ull) {}
+    uint64_t next_u64() {
+        s += 0x9E3779B97F4A7C15ull;
+        uint64_t z = s;
+        z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+        z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+        return z ^ (z >> 31);
+    }
+    double next_f64() { return (double)(next_u64() >> 11) / (double)(1ull << 53); }
+};
+}  // namespace
+
+// Simulator::sample_many (sim.rs:253-284): every shot draws its qubits in
+// order by the chain rule over qubit prefixes, p0 = P(prefix, 0) from the
+// cached prefix (marginal) engines, with the shot's own SplitMix64 stream
+// seeded from the run's.  All shots advance one qubit at a time: each qubit
+// is one batched evaluation over the distinct prefixes of the shots (a
+// query's value does not depend on its batch, so shots with the same prefix
+// share it).
+int pq_sim_sample_many(pq_sim* h, uint64_t x, size_t shots, uint64_t seed, int form, uint64_t* out) {
+    PQ_TRY({
+        const uint32_t n = h->s.ps.nq;
+        if (n > 64) throw std::invalid_argument("sampling more than 64 qubits");
+        if (!qubits_ok(h, x)) throw std::invalid_argument("basis index has bits beyond the qubit count");
+        SplitMix top(seed);
+        std::vector<SplitMix> rng;
+        rng.reserve(shots);
+        for (size_t i = 0; i < shots; ++i) rng.emplace_back(top.next_u64());
+        std::vector<uint64_t> y(shots, 0);
+        std::vector<double> pprev(shots, 1.0);
+        std::vector<uint32_t> qs;
+        std::vector<uint8_t> outc;
+        std::vector<std::pair<uint32_t, uint8_t>> a;
+        for (uint32_t l = 0; l < n; ++l) {
+            qs.push_back(l);
+            const RankEng& R = h->s.marginal_engine(qs);   // prefix of l + 1 qubits
+            // distinct prefixes y & (2^l - 1), with qubit l = 0
+            const uint64_t mask = l == 0 ? 0 : (l >= 64 ? ~0ull : ((1ull << l) - 1));
+            std::map<uint64_t, size_t> slot;
+            std::vector<uint64_t> prefixes;
+            std::vector<size_t> of(shots);
+            for (size_t i = 0; i < shots; ++i) {
+                auto it = slot.emplace(y[i] & mask, prefixes.size());
+                if (it.second) prefixes.push_back(y[i] & mask);
+                of[i] = it.first->second;
+            }
+            std::vector<double> p0(prefixes.size(), 0.0);
+            std::vector<Z8Job> jobs;
+            std::vector<size_t> owner;
+            const int walk = form == PQ_FORM_PLAIN_FP64 ? 0 : walk_of_form(form);
+            if (form != PQ_FORM_PLAIN_FP64) R.ensure_device();
+            outc.assign(l + 1, 0);
+            for (size_t p = 0; p < prefixes.size(); ++p) {
+                for (uint32_t j = 0; j < l; ++j) outc[j] = (uint8_t)((prefixes[p] >> j) & 1);
+                outc[l] = 0;
+                if (!h->s.marginal_assignment(x, qs.data(), outc.data(), l + 1, a)) continue;   // contradictory: 0
+                if (form == PQ_FORM_PLAIN_FP64) p0[p] = std::max(plain_eval(R, a).real(), 0.0);
+                else add_query_jobs(R, a, walk, jobs, owner, p);
+            }
+            if (form != PQ_FORM_PLAIN_FP64) {
+                std::vector<std::complex<double>> res;
+                run_jobs(jobs, owner, prefixes.size(), walk, res);
+                for (size_t p = 0; p < prefixes.size(); ++p) p0[p] = std::max(res[p].real(), 0.0);   // sim.rs:248
+            }
+            for (size_t i = 0; i < shots; ++i) {
+                const double q0 = std::min(p0[of[i]], pprev[i]);
+                if (rng[i].next_f64() * pprev[i] >= q0) {
+                    y[i] |= 1ull << l;
+                    pprev[i] = std::max(pprev[i] - q0, 0.0);
+                } else {
+                    pprev[i] = q0;
+                }
+            }
+        }
+        for (size_t i = 0; i < shots; ++i) out[i] = y[i];
+        return 0;
+    })
+}
+
 int pq_batch_amplitudes(size_t n, pq_sim* const* sims, const uint64_t* xs, const uint64_t* ys, int form, double* out) {
     PQ_TRY({
         const int walk = walk_of_form(form);

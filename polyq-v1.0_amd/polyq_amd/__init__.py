@@ -42,6 +42,8 @@ lib.pq_sim_marginal_probability.argtypes = [_sim_p, ctypes.c_uint64, _u32p, _u8p
 # hot paths take raw addresses (ndarray.ctypes.data) rather than typed pointers
 lib.pq_sim_amplitudes.argtypes = [_sim_p, ctypes.c_size_t, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p]
 lib.pq_batch_amplitudes.argtypes = [ctypes.c_size_t, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p]
+lib.pq_sim_statevector.argtypes = [_sim_p, ctypes.c_uint64, ctypes.c_int, ctypes.c_void_p]
+lib.pq_sim_sample_many.argtypes = [_sim_p, ctypes.c_uint64, ctypes.c_size_t, ctypes.c_uint64, ctypes.c_int, ctypes.c_void_p]
 lib.pq_last_timing.argtypes = [_f64p]
 lib.pq_set_timing.argtypes = [ctypes.c_int]
 lib.pq_get_timing.argtypes = []
@@ -230,6 +232,23 @@ class Simulator:
         out = np.empty(len(xs), dtype=np.complex128)
         _check(lib.pq_sim_amplitudes(self._h, len(xs), xs.ctypes.data, ys.ctypes.data, self._form(form), out.ctypes.data))
         return out
+
+    def statevector(self, input, form=None):
+        """All 2^n amplitudes <y|C|input> as a complex128 array indexed by y
+        (sim.rs:201-213; pybind.rs:154-164 returns the same values as raw
+        little-endian complex128 bytes)."""
+        n = self.num_qubits
+        out = np.empty(1 << n, dtype=np.complex128)
+        _check(lib.pq_sim_statevector(self._h, int(input), self._form(form), out.ctypes.data))
+        return out
+
+    def sample_many(self, input, shots, seed, form=None):
+        """shots measurement outcomes of all qubits, as basis indices
+        (sim.rs:272-284, pybind.rs:169-172): the chain rule over qubit
+        prefixes with the reference's per-shot SplitMix64 streams."""
+        out = np.empty(int(shots), dtype=np.uint64)
+        _check(lib.pq_sim_sample_many(self._h, int(input), int(shots), int(seed), self._form(form), out.ctypes.data))
+        return [int(v) for v in out]
 
     def stats(self):
         """(free_vars, planes, lines, radical, terms, branch_dim, pairs) (pybind.rs:177-180)."""

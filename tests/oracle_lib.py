@@ -48,6 +48,7 @@ def lib():
         L.orc_sim_stats.argtypes = [ctypes.c_void_p, _u64p]
         L.orc_sim_amplitude.argtypes = [ctypes.c_void_p, _u8p, _u8p, _f64p]
         L.orc_sim_marginal.argtypes = [ctypes.c_void_p, _u8p, _u32p, _u8p, ctypes.c_size_t, _f64p]
+        L.orc_sim_sample_many.argtypes = [ctypes.c_void_p, _u8p, ctypes.c_size_t, ctypes.c_uint64, _u64p]
         L.orc_branch_dump.argtypes = [ctypes.c_void_p, _u8p, _u8p, ctypes.c_size_t, _u64p, _u8p, _u16p, _u8p, _u8p]
         L.orc_surviving_branches.restype = ctypes.c_long
         L.orc_surviving_branches.argtypes = [ctypes.c_void_p, _u8p, _u8p, _u64p, ctypes.c_long]
@@ -154,6 +155,11 @@ class OSim:
         out = np.zeros(2, np.float64)
         _check(lib().orc_sim_amplitude(self.h, _p(bits(self.c.n, x), _u8p), _p(bits(self.c.n, y), _u8p), _p(out, _f64p)))
         return complex(out[0], out[1])
+
+    def sample_many(self, x, shots, seed):
+        out = np.zeros(shots, np.uint64)
+        _check(lib().orc_sim_sample_many(self.h, _p(bits(self.c.n, x), _u8p), shots, seed, _p(out, _u64p)))
+        return out
 
     def marginal(self, x, measured):
         qs = np.asarray([m[0] for m in measured], np.uint32)

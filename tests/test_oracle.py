@@ -271,6 +271,42 @@ def test_marginals_match_dense():
             assert abs(s.marginal(0, meas) - want) < 1e-9
 
 
+def _sampling_circuit():
+    # sim.rs:383-398 (sampling_follows_the_distribution)
+    import polyq_amd as pq
+    c = pq.Circuit(3)
+    c.h(0).t(0).h(0).cx(0, 1).h(2).t(2).h(2).cz(1, 2).h(1).t(1).h(1)
+    return c
+
+
+def test_sampling_follows_the_distribution():
+    # sim.rs:383-398: 40000 shots (seed 1234) within 0.012 of |amplitude|^2
+    oc = O.OCircuit.from_circuit(_sampling_circuit())
+    sv = O.dense(oc, 0)
+    shots = 40000
+    got = O.OSim(oc, "auto").sample_many(0, shots, 1234)
+    hist = np.bincount(got.astype(np.int64), minlength=8)
+    for y in range(8):
+        assert abs(abs(sv[y]) ** 2 - hist[y] / shots) < 0.012, y
+
+
+def test_sampling_draws_only_possible_outcomes_and_is_reproducible():
+    # every drawn outcome has nonzero probability; the same seed gives the
+    # same shots, another seed other shots (wider random circuits)
+    rng = W.Rng(17)
+    for trial in range(6):
+        n = 3 + rng.below(4)
+        c = W.random_layered(n, 2, rng)
+        oc = O.OCircuit.from_circuit(c)
+        x = rng.below(1 << n)
+        sv = O.dense(oc, x)
+        s = O.OSim(oc, "cached")
+        a = s.sample_many(x, 300, 5 + trial)
+        assert all(abs(sv[int(y)]) ** 2 > 1e-12 for y in a)
+        np.testing.assert_array_equal(a, s.sample_many(x, 300, 5 + trial))
+        assert not np.array_equal(a, s.sample_many(x, 300, 6 + trial)) or np.max(np.abs(sv) ** 2) > 0.999
+
+
 def test_toffoli_and_circuit_known_answers():
     # circuit.rs:233-244 and sim.rs:344-351
     from polyq_amd import Circuit
