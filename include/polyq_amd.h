@@ -84,6 +84,12 @@ int pq_sim_probability(pq_sim* sim, uint64_t input, uint64_t output, int form, d
  * rebuilds it on every call). */
 int pq_sim_marginal_probability(pq_sim* sim, uint64_t input, const uint32_t* qubits, const uint8_t* outcomes, size_t k,
                                 int form, double* out);
+/* Simulator::amplitude_estimate (src/sim.rs:187-193, pybind.rs:149-153):
+ * unbiased Monte-Carlo estimate of <y|C|x> from `samples` points
+ * (Engine::estimate, src/engine.rs:612-659, or random branch engines,
+ * src/rank.rs:149-171); out = {re, im, standard error}.  The exact sum when
+ * 2^d <= samples. */
+int pq_sim_amplitude_estimate(pq_sim* sim, uint64_t input, uint64_t output, size_t samples, uint64_t seed, int form, double* out3);
 /* Simulator::statevector (src/sim.rs:201-213): all 2^n amplitudes <y|C|x>,
  * out holds 2 * 2^n doubles (re, im by y); n < 48. */
 int pq_sim_statevector(pq_sim* sim, uint64_t input, int form, double* out);

@@ -845,6 +845,42 @@ struct Engine {  // engine.rs:240-284 (Core + Branching + Engine flattened)
         for (size_t hi = 0; hi < nhi; ++hi) tot = tot + chunk(hi);
         return tot;
     }
+    // engine.rs:612-659: Monte-Carlo estimate over uniform points of the
+    // branch subspace; chunks of 1024 samples, chunk ci drawing from
+    // Rng(seed ^ ci * 0x9E3779B97F4A7C15) (sequential here, Rayon there)
+    std::pair<C64, double> estimate(const std::vector<bool>& values, size_t samples, uint64_t seed) const {
+        Phase phase; C64 c_start; UVec u_start; BitVec s_start;
+        if (!prepare(values, phase, c_start, u_start, s_start)) return {C64(0, 0), 0.0};
+        const size_t d = elim.kernel.size();
+        const C64 norm = to_complex(phase) * C64(magnitude(), 0);
+        if (d < 63 && ((size_t)1 << d) <= samples) return {norm * sum_subspace(u_start, s_start, c_start), 0.0};
+        const size_t chunks = (samples + 1023) / 1024;
+        double sr = 0, si = 0, sq = 0;
+        size_t cnt = 0;
+        for (size_t ci = 0; ci < chunks; ++ci) {
+            Rng rng(seed ^ ((uint64_t)ci * 0x9E3779B97F4A7C15ull));
+            const size_t n_here = std::min<size_t>(1024, samples - ci * 1024);
+            double cr = 0, cim = 0, cq = 0;
+            for (size_t k = 0; k < n_here; ++k) {
+                UVec u = u_start; BitVec sv = s_start;
+                uint64_t word = 0;
+                for (size_t j = 0; j < d; ++j) {
+                    if (j % 64 == 0) word = rng.next_u64();
+                    if ((word >> (j % 64)) & 1) { u.xor_assign(elim.kernel_u[j]); sv.xor_assign(elim.kernel[j]); }
+                }
+                const C64 x = weight(sv) * ROOT8[Structure::phase8(diag, u)];
+                cr += x.re; cim += x.im; cq += x.re * x.re + x.im * x.im;
+            }
+            sr += cr; si += cim; sq += cq; cnt += n_here;
+        }
+        const double n = (double)cnt;
+        const C64 mean(sr / n, si / n);
+        const double var = std::max(sq / n - (mean.re * mean.re + mean.im * mean.im), 0.0);
+        const double scale = std::pow(2.0, (double)d);
+        const C64 est = norm * mean * scale;
+        const double nn = std::sqrt(norm.re * norm.re + norm.im * norm.im);
+        return {est, nn * scale * std::sqrt(var / n)};
+    }
     // engine.rs:557-561
     C64 eval(const std::vector<bool>& values) const {
         Phase ph; C64 cs; UVec u; BitVec s0;
@@ -950,6 +986,30 @@ struct RankEngine {  // rank.rs:47-51
         return tot;
     }
     void set_threads(int t) { for (auto& e : branches) e.threads = t; }
+    // rank.rs:149-171: one branch: Engine::estimate; several: exact values of
+    // uniformly drawn branches (sample i draws from Rng(seed ^ i * gamma))
+    std::pair<C64, double> estimate(const std::vector<std::pair<size_t, bool>>& asg, size_t samples, uint64_t seed) const {
+        if (branches.size() == 1) {
+            std::vector<bool> vals;
+            branches[0].values_of(asg, vals);
+            return branches[0].estimate(vals, samples, seed);
+        }
+        const size_t nb = branches.size();
+        std::vector<C64> val(nb);
+        std::vector<bool> have(nb, false);
+        double sr = 0, si = 0, sq = 0;
+        for (size_t i = 0; i < samples; ++i) {
+            Rng rng(seed ^ ((uint64_t)i * 0x9E3779B97F4A7C15ull));
+            const size_t b = rng.below(nb);
+            if (!have[b]) { val[b] = branches[b].eval_assignment(asg); have[b] = true; }
+            const C64 a = val[b];
+            sr += a.re; si += a.im; sq += a.re * a.re + a.im * a.im;
+        }
+        const double n = (double)samples;
+        const C64 mean(sr / n, si / n);
+        const double var = std::max(sq / n - (mean.re * mean.re + mean.im * mean.im), 0.0);
+        return {mean * (double)nb, (double)nb * std::sqrt(var / n)};
+    }
 };
 
 // ----------------------------------------------------------------- sim.rs ---
@@ -1233,6 +1293,18 @@ int orc_sim_amplitude(const orc_sim* h, const uint8_t* x, const uint8_t* y, doub
         size_t n = h->s.ps.num_qubits;
         C64 a = h->s.amplitude(bits_of(n, x), bits_of(n, y));
         out[0] = a.re; out[1] = a.im; return 0;
+    } catch (std::exception& e) { g_err = e.what(); return -1; }
+}
+// Simulator::amplitude_estimate (sim.rs:187-193): out = {re, im, standard error}.
+int orc_sim_amplitude_estimate(const orc_sim* h, const uint8_t* x, const uint8_t* y, size_t samples, uint64_t seed, double* out) {
+    try {
+        size_t n = h->s.ps.num_qubits;
+        std::vector<std::pair<size_t, bool>> bd;
+        out[0] = out[1] = out[2] = 0;
+        if (!h->s.ps.boundary(bits_of(n, x), bits_of(n, y), bd)) return 0;
+        auto r = h->s.engine.estimate(bd, samples, seed);
+        out[0] = r.first.re; out[1] = r.first.im; out[2] = r.second;
+        return 0;
     } catch (std::exception& e) { g_err = e.what(); return -1; }
 }
 // Simulator::sample_many (sim.rs:272-284); out holds shots basis indices.

@@ -338,6 +338,113 @@ int pq_sim_marginal_probability(pq_sim* h, uint64_t x, const uint32_t* qubits, c
     })
 }
 
+namespace {
+// SplitMix64 as the reference's Rng (rng.rs:9-27): state = seed + gamma,
+// next adds gamma to the state and mixes the new state; next_f64 takes the
+// top 53 bits.
+struct SplitMix {
+    uint64_t s;
+    explicit SplitMix(uint64_t seed) : s(seed + 0x9E3779B97F4A7C15ull) {}
+    uint64_t next_u64() {
+        s += 0x9E3779B97F4A7C15ull;
+        uint64_t z = s;
+        z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+        z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+        return z ^ (z >> 31);
+    }
+    double next_f64() { return (double)(next_u64() >> 11) / (double)(1ull << 53); }
+};
+}  // namespace
+
+// Simulator::amplitude_estimate (sim.rs:187-193) -> RankEngine::estimate
+// (rank.rs:149-171) -> Engine::estimate (engine.rs:612-659).  out = {re, im,
+// standard error}.  One branch engine: the exact sum when 2^d <= samples,
+// else the Monte-Carlo mean over `samples` uniform points of the branch
+// subspace (k_estimate).  Several: exact values of uniformly drawn branch
+// engines; each drawn engine is evaluated once (in one batch) and counted as
+// often as it was drawn.
+int pq_sim_amplitude_estimate(pq_sim* h, uint64_t x, uint64_t y, size_t samples, uint64_t seed, int form, double* out) {
+    PQ_TRY({
+        out[0] = out[1] = out[2] = 0;
+        if (!qubits_ok(h, x) || !qubits_ok(h, y)) throw std::invalid_argument("basis index has bits beyond the qubit count");
+        if (samples == 0) throw std::invalid_argument("estimate with zero samples");
+        std::vector<std::pair<uint32_t, uint8_t>> asg;
+        if (!h->s.assignment(x, y, asg)) return 0;                 // sim.rs:190
+        const RankEng& R = *h->s.eng;
+        R.ensure_device();
+        const size_t nb = R.host.size();
+        auto exact_values = [&](const std::vector<size_t>& which, std::vector<std::complex<double>>& val) {
+            val.assign(nb, {0, 0});
+            if (form == PQ_FORM_PLAIN_FP64) {
+                for (size_t b : which) {
+                    Prepared P;
+                    if (!prepare_assignment(*R.host[b], asg, P)) continue;
+                    const HostEngine& E = *R.host[b];
+                    val[b] = phase_complex(P.phase) * dev_plain_sum(*R.dev[b], P, nullptr) * std::pow(2.0, E.log2mag - (double)E.nh / 2.0);
+                }
+                return;
+            }
+            const int walk = walk_of_form(form);
+            std::vector<Z8Job> jobs;
+            std::vector<size_t> owner;
+            Prepared P;
+            for (size_t b : which) {
+                if (!prepare_assignment(*R.host[b], asg, P)) continue;
+                Z8Job J;
+                J.eng = R.dev[b].get();
+                if (walk == kWalkSeg) J.sq = seg_query(*R.host[b], *R.seg[b], P);
+                else J.q = z8_query(*R.host[b], *R.z8[b], P);
+                J.scale = P.scale;
+                J.mid_bits = 0;
+                jobs.push_back(J);
+                owner.push_back(b);
+            }
+            run_jobs(jobs, owner, nb, walk, val);
+        };
+        if (nb == 1) {
+            const HostEngine& E = *R.host[0];
+            Prepared P;
+            if (!prepare_assignment(E, asg, P)) return 0;           // engine.rs:618
+            const uint32_t d = E.d;
+            if (d < 63 && (1ull << d) <= samples) {                  // engine.rs:622-624
+                std::vector<std::complex<double>> val;
+                exact_values({0}, val);
+                out[0] = val[0].real(); out[1] = val[0].imag();
+                return 0;
+            }
+            const McSums S = dev_plain_estimate(*R.dev[0], P, samples, seed);
+            const std::complex<double> norm = phase_complex(P.phase) * std::pow(2.0, E.log2mag - (double)E.nh / 2.0);
+            const double n = (double)S.n;
+            const std::complex<double> mean(S.re / n, S.im / n);
+            const double var = std::max(S.sq / n - std::norm(mean), 0.0);
+            const double scale = std::pow(2.0, (double)d);
+            const std::complex<double> est = norm * mean * scale;
+            out[0] = est.real(); out[1] = est.imag();
+            out[2] = std::abs(norm) * scale * std::sqrt(var / n);
+            return 0;
+        }
+        // rank.rs:158-170: sample i draws its branch from Rng(seed ^ i * gamma)
+        std::vector<uint64_t> count(nb, 0);
+        for (size_t i = 0; i < samples; ++i) {
+            SplitMix r(seed ^ ((uint64_t)i * 0x9E3779B97F4A7C15ull));
+            ++count[r.next_u64() % nb];
+        }
+        std::vector<size_t> which;
+        for (size_t b = 0; b < nb; ++b) if (count[b]) which.push_back(b);
+        std::vector<std::complex<double>> val;
+        exact_values(which, val);
+        std::complex<double> sum(0, 0);
+        double sq = 0;
+        for (size_t b : which) { sum += (double)count[b] * val[b]; sq += (double)count[b] * std::norm(val[b]); }
+        const double n = (double)samples;
+        const std::complex<double> mean = sum / n;
+        const double var = std::max(sq / n - std::norm(mean), 0.0);
+        out[0] = mean.real() * (double)nb; out[1] = mean.imag() * (double)nb;
+        out[2] = (double)nb * std::sqrt(var / n);
+        return 0;
+    })
+}
+
 // Simulator::statevector (sim.rs:201-213): all 2^n amplitudes <y|C|x>, in
 // batches of at most 2^16 queries through the batched form.
 int pq_sim_statevector(pq_sim* h, uint64_t x, int form, double* out) {
@@ -359,23 +466,6 @@ int pq_sim_statevector(pq_sim* h, uint64_t x, int form, double* out) {
     })
 }
 
-namespace {
-// SplitMix64 as the reference's Rng (rng.rs:9-27): state = seed + gamma,
-// next adds gamma to the state and mixes the new state; next_f64 takes the
-// top 53 bits.
-struct SplitMix {
-    uint64_t s;
-    explicit SplitMix(uint64_t seed) : s(seed + 0x9E3779B97F4A7C15ull) {}
-    uint64_t next_u64() {
-        s += 0x9E3779B97F4A7C15ull;
-        uint64_t z = s;
-        z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
-        z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
-        return z ^ (z >> 31);
-    }
-    double next_f64() { return (double)(next_u64() >> 11) / (double)(1ull << 53); }
-};
-}  // namespace
 
 // Simulator::sample_many (sim.rs:253-284): every shot draws its qubits in
 // order by the chain rule over qubit prefixes, p0 = P(prefix, 0) from the

@@ -28,6 +28,10 @@ const SegImage& dev_seg(const DevEngine& D);
 // Returns the raw FP64 sum over the affine subspace (before the boundary phase
 // and the 2^{log2mag - h/2} scale).  kernel_ms receives the walk kernel time.
 std::complex<double> dev_plain_sum(const DevEngine& D, const Prepared& P, double* kernel_ms);
+// Monte-Carlo sums of Engine::estimate (engine.rs:612-659) over `samples`
+// uniform points of the branch subspace: sum of values (re, im) and of |x|^2.
+struct McSums { double re = 0, im = 0, sq = 0; uint64_t n = 0; };
+McSums dev_plain_estimate(const DevEngine& D, const Prepared& P, uint64_t samples, uint64_t seed);
 
 // Batched Z8 walk: one persistent launch over all (query, lane-block) items.
 struct Z8Job {

@@ -411,6 +411,68 @@ __global__ __launch_bounds__(256) void k_plain(PlainDev E, PlainQ Q, double* __r
     }
 }
 
+// Monte-Carlo samples of Engine::estimate (engine.rs:612-659), one per
+// thread: sample g is sample g mod 1024 of chunk g / 1024, drawn from
+// Rng(seed ^ chunk * gamma).  SplitMix64's state after m steps is
+// start + m * gamma, so each thread computes its own random words directly.
+// The point s = s0 + sum of the drawn kernel vectors, its value
+// weight(s) * omega^phase8(u) in FP64 (the reference's product order), and
+// per workgroup the sums of re, im and |x|^2 in a fixed order.
+template <int NW>
+__global__ __launch_bounds__(256) void k_estimate(PlainDev E, PlainQ Q, uint64_t samples, uint64_t seed, double* __restrict__ partial) {
+    __shared__ double r8[16];
+    __shared__ double red[12];
+    const uint32_t tid = threadIdx.x;
+    if (tid < 8) {
+        const double h = 0.70710678118654752440;
+        const double re[8] = {1, h, 0, -h, -1, -h, 0, h}, im[8] = {0, h, 1, h, 0, -h, -1, -h};
+        r8[2 * tid] = re[tid]; r8[2 * tid + 1] = im[tid];
+    }
+    __syncthreads();
+    constexpr uint64_t kGamma = 0x9E3779B97F4A7C15ull;
+    const uint64_t g = (uint64_t)blockIdx.x * 256 + tid;
+    double xr = 0.0, xi = 0.0, xq = 0.0;
+    if (g < samples) {
+        const uint64_t ci = g >> 10, k = g & 1023u;
+        const uint64_t start = (seed ^ (ci * kGamma)) + kGamma;       // Rng::new
+        uint32_t ua[NW], ub[NW], uo[NW];
+#pragma unroll
+        for (int w = 0; w < NW; ++w) { ua[w] = Q.ustart[w]; ub[w] = Q.ustart[NW + w]; uo[w] = Q.ustart[2 * NW + w]; }
+        uint64_t sv = Q.s0;
+        // d <= 64 (kernel vectors over at most 64 terms): one word per sample
+        uint64_t z = start + (k + 1) * kGamma;
+        z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+        z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+        z ^= z >> 31;
+        for (uint32_t j = 0; j < E.d; ++j) {
+            if ((z >> j) & 1) {
+                const uint32_t* row = E.ku + (size_t)j * 3 * NW;
+#pragma unroll
+                for (int w = 0; w < NW; ++w) { ua[w] ^= row[w]; ub[w] ^= row[NW + w]; uo[w] ^= row[2 * NW + w]; }
+                sv ^= E.kn[j];
+            }
+        }
+        double cr, ci2;
+        plain_weight(E, sv, cr, ci2);
+        int pc1 = 0, pc3 = 0, cnt = 0;
+#pragma unroll
+        for (int w = 0; w < NW; ++w) {
+            pc1 += __popc(uo[w] & E.diag[2 * NW + w]);
+            pc3 += __popc(uo[w] & E.diag[3 * NW + w]);
+            cnt += __popc((E.diag[w] ^ ua[w]) & (E.diag[NW + w] ^ ub[w]));
+        }
+        const int k8 = (E.sigma - 2 * (pc1 - pc3) + ((cnt & 1) << 2)) & 7;
+        const double rr = r8[2 * k8], ri = r8[2 * k8 + 1];
+        xr = cr * rr - ci2 * ri;
+        xi = cr * ri + ci2 * rr;
+        xq = xr * xr + xi * xi;
+    }
+    for (int off = 32; off > 0; off >>= 1) { xr += __shfl_xor(xr, off); xi += __shfl_xor(xi, off); xq += __shfl_xor(xq, off); }
+    if ((tid & 63) == 0) { red[3 * (tid >> 6)] = xr; red[3 * (tid >> 6) + 1] = xi; red[3 * (tid >> 6) + 2] = xq; }
+    __syncthreads();
+    if (tid < 3) partial[3 * (uint64_t)blockIdx.x + tid] = (red[tid] + red[3 + tid]) + (red[6 + tid] + red[9 + tid]);
+}
+
 __global__ __launch_bounds__(256) void k_sum_pairs(const double* __restrict__ in, uint64_t n, double* __restrict__ out) {
     __shared__ double red[8];
     const uint32_t tid = threadIdx.x;
@@ -422,12 +484,10 @@ __global__ __launch_bounds__(256) void k_sum_pairs(const double* __restrict__ in
     if (tid == 0) { out[0] = (red[0] + red[2]) + (red[4] + red[6]); out[1] = (red[1] + red[3]) + (red[5] + red[7]); }
 }
 
-std::complex<double> dev_plain_sum(const DevEngine& D, const Prepared& P, double* kernel_ms) {
-    Ctx& C = ctx();
+namespace {
+PlainQ plain_query(const DevEngine& D, const Prepared& P) {
     const PlainDev& E = D.plain;
-    if (E.NW == 0) throw std::invalid_argument("normal form too large for the plain kernel (more than 1024 planes or lines)");
     const HostEngine& H = *D.host;
-    if (!P.nonzero) return {0, 0};
     PlainQ Q{};
     const ULayout& L = H.L;
     std::vector<uint32_t> us;
@@ -437,6 +497,44 @@ std::complex<double> dev_plain_sum(const DevEngine& D, const Prepared& P, double
     if (us.size() > 96) throw std::invalid_argument("plain kernel state above 96 dwords");
     std::copy(us.begin(), us.end(), Q.ustart);
     Q.s0 = P.s0; Q.cre = P.c_start.real(); Q.cim = P.c_start.imag();
+    return Q;
+}
+}  // namespace
+
+McSums dev_plain_estimate(const DevEngine& D, const Prepared& P, uint64_t samples, uint64_t seed) {
+    Ctx& C = ctx();
+    const PlainDev& E = D.plain;
+    if (E.NW == 0) throw std::invalid_argument("normal form too large for the plain kernel (more than 1024 planes or lines)");
+    if (E.d > 64) throw std::invalid_argument("branch dimension above 64");
+    McSums S;
+    if (samples == 0 || !P.nonzero) return S;
+    const PlainQ Q = plain_query(D, P);
+    const uint64_t blocks = (samples + 255) / 256;
+    if (blocks > (1ull << 31)) throw std::invalid_argument("too many samples for one launch");
+    std::lock_guard<std::mutex> lk(C.mu);
+    double* part = (double*)C.d_misc.get(blocks * 24);
+    switch (E.NW) {
+        case 2: hipLaunchKernelGGL(k_estimate<2>, dim3((uint32_t)blocks), dim3(256), 0, C.stream, E, Q, samples, seed, part); break;
+        case 4: hipLaunchKernelGGL(k_estimate<4>, dim3((uint32_t)blocks), dim3(256), 0, C.stream, E, Q, samples, seed, part); break;
+        case 8: hipLaunchKernelGGL(k_estimate<8>, dim3((uint32_t)blocks), dim3(256), 0, C.stream, E, Q, samples, seed, part); break;
+        case 16: hipLaunchKernelGGL(k_estimate<16>, dim3((uint32_t)blocks), dim3(256), 0, C.stream, E, Q, samples, seed, part); break;
+        default: hipLaunchKernelGGL(k_estimate<32>, dim3((uint32_t)blocks), dim3(256), 0, C.stream, E, Q, samples, seed, part); break;
+    }
+    HIPCHECK(hipGetLastError());
+    std::vector<double> h(blocks * 3);
+    HIPCHECK(hipMemcpyAsync(h.data(), part, blocks * 24, hipMemcpyDeviceToHost, C.stream));
+    HIPCHECK(hipStreamSynchronize(C.stream));
+    for (uint64_t b = 0; b < blocks; ++b) { S.re += h[3 * b]; S.im += h[3 * b + 1]; S.sq += h[3 * b + 2]; }
+    S.n = samples;
+    return S;
+}
+
+std::complex<double> dev_plain_sum(const DevEngine& D, const Prepared& P, double* kernel_ms) {
+    Ctx& C = ctx();
+    const PlainDev& E = D.plain;
+    if (E.NW == 0) throw std::invalid_argument("normal form too large for the plain kernel (more than 1024 planes or lines)");
+    if (!P.nonzero) return {0, 0};
+    PlainQ Q = plain_query(D, P);
     // Gray chunk length: the reference's 2^14, shortened so that at least 2^16
     // threads exist when d allows (one query has to fill the device alone).
     const uint32_t d = E.d;

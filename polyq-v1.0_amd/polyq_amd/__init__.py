@@ -43,6 +43,7 @@ lib.pq_sim_marginal_probability.argtypes = [_sim_p, ctypes.c_uint64, _u32p, _u8p
 lib.pq_sim_amplitudes.argtypes = [_sim_p, ctypes.c_size_t, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p]
 lib.pq_batch_amplitudes.argtypes = [ctypes.c_size_t, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p]
 lib.pq_sim_statevector.argtypes = [_sim_p, ctypes.c_uint64, ctypes.c_int, ctypes.c_void_p]
+lib.pq_sim_amplitude_estimate.argtypes = [_sim_p, ctypes.c_uint64, ctypes.c_uint64, ctypes.c_size_t, ctypes.c_uint64, ctypes.c_int, _f64p]
 lib.pq_sim_sample_many.argtypes = [_sim_p, ctypes.c_uint64, ctypes.c_size_t, ctypes.c_uint64, ctypes.c_int, ctypes.c_void_p]
 lib.pq_last_timing.argtypes = [_f64p]
 lib.pq_set_timing.argtypes = [ctypes.c_int]
@@ -232,6 +233,14 @@ class Simulator:
         out = np.empty(len(xs), dtype=np.complex128)
         _check(lib.pq_sim_amplitudes(self._h, len(xs), xs.ctypes.data, ys.ctypes.data, self._form(form), out.ctypes.data))
         return out
+
+    def amplitude_estimate(self, input, output, samples, seed, form=None):
+        """Unbiased Monte-Carlo estimate of <output|C|input>: (re, im,
+        standard_error) (sim.rs:187-193, pybind.rs:149-153).  The exact value
+        with error 0 when the branch subspace has at most `samples` points."""
+        o = np.zeros(3, dtype=np.float64)
+        _check(lib.pq_sim_amplitude_estimate(self._h, int(input), int(output), int(samples), int(seed), self._form(form), _ptr(o, _f64p)))
+        return float(o[0]), float(o[1]), float(o[2])
 
     def statevector(self, input, form=None):
         """All 2^n amplitudes <y|C|input> as a complex128 array indexed by y

@@ -49,6 +49,7 @@ def lib():
         L.orc_sim_amplitude.argtypes = [ctypes.c_void_p, _u8p, _u8p, _f64p]
         L.orc_sim_marginal.argtypes = [ctypes.c_void_p, _u8p, _u32p, _u8p, ctypes.c_size_t, _f64p]
         L.orc_sim_sample_many.argtypes = [ctypes.c_void_p, _u8p, ctypes.c_size_t, ctypes.c_uint64, _u64p]
+        L.orc_sim_amplitude_estimate.argtypes = [ctypes.c_void_p, _u8p, _u8p, ctypes.c_size_t, ctypes.c_uint64, _f64p]
         L.orc_branch_dump.argtypes = [ctypes.c_void_p, _u8p, _u8p, ctypes.c_size_t, _u64p, _u8p, _u16p, _u8p, _u8p]
         L.orc_surviving_branches.restype = ctypes.c_long
         L.orc_surviving_branches.argtypes = [ctypes.c_void_p, _u8p, _u8p, _u64p, ctypes.c_long]
@@ -155,6 +156,11 @@ class OSim:
         out = np.zeros(2, np.float64)
         _check(lib().orc_sim_amplitude(self.h, _p(bits(self.c.n, x), _u8p), _p(bits(self.c.n, y), _u8p), _p(out, _f64p)))
         return complex(out[0], out[1])
+
+    def amplitude_estimate(self, x, y, samples, seed):
+        out = np.zeros(3, np.float64)
+        _check(lib().orc_sim_amplitude_estimate(self.h, _p(bits(self.c.n, x), _u8p), _p(bits(self.c.n, y), _u8p), samples, seed, _p(out, _f64p)))
+        return complex(out[0], out[1]), float(out[2])
 
     def sample_many(self, x, shots, seed):
         out = np.zeros(shots, np.uint64)

@@ -67,3 +67,43 @@ def test_statevector_across_batches():
     assert abs(np.sum(np.abs(got) ** 2) - 1) < 1e-9
     want = O.dense(O.OCircuit.from_circuit(c), 0)
     np.testing.assert_allclose(got, want, atol=1e-10)
+
+
+def _estimate_cases():
+    # as tests/test_oracle.py: H-sandwiched Clifford+T layers, d up to ~12
+    rng = W.Rng(23)
+    out = []
+    for trial in range(6):
+        n = 9 + rng.below(4)
+        c = W.sandwich_layer(n, 11 + rng.below(5), (15 * n) // 2, (5 * n) // 2, rng)
+        out.append((c, 0, rng.below(1 << n)))
+    return out
+
+
+@pytest.mark.parametrize("form", ["z8", "plain_fp64"])
+@pytest.mark.parametrize("strategy", ["cached", "pairs"])
+def test_amplitude_estimate_equals_oracle(strategy, form):
+    # the same random points (per-chunk SplitMix64 streams, jumped to on the
+    # device) or the same random branch engines: the same estimate and error
+    for c, x, y in _estimate_cases():
+        oc = O.OCircuit.from_circuit(c)
+        o = O.OSim(oc, strategy, 3)
+        s = pq.Simulator(c, strategy=strategy, pairs=3 if strategy == "pairs" else None, form=form)
+        for samples, seed in ((200, 1), (3000, 9)):
+            re, im, se = s.amplitude_estimate(x, y, samples, seed)
+            want, wse = o.amplitude_estimate(x, y, samples, seed)
+            scale = abs(want) + wse + 1e-300
+            assert abs(complex(re, im) - want) <= 1e-9 * scale, (samples, seed, complex(re, im), want)
+            assert abs(se - wse) <= 1e-9 * scale
+
+
+def test_amplitude_estimate_on_the_headline_workload():
+    # n = 40, 36 T (d = 30): 2^20 uniform points; within 6 standard errors of
+    # the exact amplitude
+    c, xs, ys = W.config2(5)
+    s = pq.Simulator(c, strategy="cached")
+    exact = s.amplitudes(xs[:1], ys[:1])[0]
+    re, im, se = s.amplitude_estimate(xs[0], ys[0], 1 << 20, 12345)
+    print("estimate", complex(re, im), "+-", se, "exact", exact)
+    assert se > 0
+    assert abs(complex(re, im) - exact) < 6 * se

@@ -307,6 +307,49 @@ def test_sampling_draws_only_possible_outcomes_and_is_reproducible():
         assert not np.array_equal(a, s.sample_many(x, 300, 6 + trial)) or np.max(np.abs(sv) ** 2) > 0.999
 
 
+def _estimate_cases():
+    # H-sandwiched Clifford+T layers (the config-2 generator shape): the branch
+    # dimension follows the T count and the amplitudes are nonzero
+    rng = W.Rng(23)
+    out = []
+    for trial in range(6):
+        n = 9 + rng.below(4)
+        c = W.sandwich_layer(n, 11 + rng.below(5), (15 * n) // 2, (5 * n) // 2, rng)
+        out.append((c, 0, rng.below(1 << n)))
+    return out
+
+
+def test_estimate_is_exact_when_the_subspace_fits():
+    # engine.rs:622-624: 2^d <= samples gives the exact sum and error 0
+    for c, x, y in _estimate_cases()[:4]:
+        oc = O.OCircuit.from_circuit(c)
+        s = O.OSim(oc, "cached")
+        d = s.stats()[5]
+        est, se = s.amplitude_estimate(x, y, 1 << d, 3)
+        assert se == 0.0
+        assert abs(est - s.amplitude(x, y)) < 1e-12
+
+
+@pytest.mark.parametrize("strategy", ["cached", "pairs"])
+def test_estimate_is_unbiased_within_its_error(strategy):
+    # engine.rs:612-659 / rank.rs:149-171: over many seeds the estimates
+    # scatter around the exact amplitude with the stated standard error
+    for c, x, y in _estimate_cases():
+        oc = O.OCircuit.from_circuit(c)
+        s = O.OSim(oc, strategy, 3)
+        exact = s.amplitude(x, y)
+        d = s.stats()[5]
+        if d < 8:
+            continue
+        zs = []
+        for seed in range(12):
+            est, se = s.amplitude_estimate(x, y, 64, seed)
+            if se > 0:
+                zs.append(abs(est - exact) / se)
+        if zs:
+            assert max(zs) < 6 and np.mean(zs) < 2.5, zs
+
+
 def test_toffoli_and_circuit_known_answers():
     # circuit.rs:233-244 and sim.rs:344-351
     from polyq_amd import Circuit
