@@ -114,10 +114,12 @@ void build_jobs(size_t n, int walk, const std::function<const RankEng*(size_t)>&
             last = R;
         }
     }
-    // At least 16 queries per chunk (a prologue takes about 0.3 us, a
-    // hand-off to another thread about as much as a few of them): batches
-    // below two chunks stay on the calling thread.
-    constexpr size_t kChunk = 16;
+    // At least 8 queries per chunk (a prologue takes about 0.3 us, a hand-off
+    // to another thread about as much as a few of them): batches below two
+    // chunks stay on the calling thread, and batches below 256 queries use
+    // at most the caller and the pool's polling workers, so no blocked
+    // worker has to be woken.
+    constexpr size_t kChunk = 8;
     if (n < 2 * kChunk) {
         jobs.clear(); owner.clear();
         for (size_t i = 0; i < n; ++i) {
@@ -130,7 +132,7 @@ void build_jobs(size_t n, int walk, const std::function<const RankEng*(size_t)>&
     std::vector<std::vector<Z8Job>> cj(pool.threads());
     std::vector<std::vector<size_t>> co(pool.threads());
     std::vector<std::string> err(pool.threads());
-    const unsigned nch = pool.run(n, kChunk, [&](size_t lo, size_t hi, unsigned c) {
+    const unsigned nch = pool.run(n, kChunk, n < 256 ? ChunkPool::kHot + 1 : pool.threads(), [&](size_t lo, size_t hi, unsigned c) {
         try {
             for (size_t i = lo; i < hi; ++i) {
                 check(i);

@@ -1,5 +1,5 @@
 // A small fixed pool of host threads for the per-query prologues of batches.
-// run(n, min_chunk, fn) splits [0, n) into contiguous chunks, calls
+// run(n, min_chunk, max_chunks, fn) splits [0, n) into contiguous chunks, calls
 // fn(lo, hi, chunk_index) for each (the caller takes part) and returns the
 // number of chunks; chunk c covers a range that precedes chunk c + 1, so
 // results gathered per chunk and concatenated in chunk order are in query
@@ -30,8 +30,13 @@ public:
     }
     unsigned threads() const { return nworkers_ + 1; }
 
-    unsigned run(size_t n, size_t min_chunk, const std::function<void(size_t, size_t, unsigned)>& fn) {
-        const unsigned want = (unsigned)std::max<size_t>(1, std::min<size_t>(threads(), n / std::max<size_t>(min_chunk, 1)));
+    // Workers that poll for new regions (the rest block between regions).
+    static constexpr unsigned kHot = 3;
+
+    // At most max_chunks chunks of at least min_chunk items each.
+    unsigned run(size_t n, size_t min_chunk, unsigned max_chunks, const std::function<void(size_t, size_t, unsigned)>& fn) {
+        const size_t cap = std::max<size_t>(1, std::min<size_t>(threads(), max_chunks));
+        const unsigned want = (unsigned)std::max<size_t>(1, std::min<size_t>(cap, n / std::max<size_t>(min_chunk, 1)));
         if (want <= 1) { fn(0, n, 0); return 1; }
         std::unique_lock<std::mutex> region(region_mu_, std::try_to_lock);
         if (!region.owns_lock()) { fn(0, n, 0); return 1; }
@@ -82,7 +87,6 @@ private:
             t.done.fetch_add(1, std::memory_order_release);
         }
     }
-    static constexpr unsigned kHot = 3;
     void loop(bool hot) {
         uint64_t seen = 0;
         for (;;) {
