@@ -122,6 +122,9 @@ def headline(args, pq, W, np):
         pq.set_walk_tuning(*tuned)
     else:
         tuned = pq.autotune([sim] * nq, xs, ys, repeats=7, cache=False)[:2]
+    # the basis indices as uint64 arrays, the form the C ABI takes them in
+    xs = np.asarray(xs, dtype=np.uint64)
+    ys = np.asarray(ys, dtype=np.uint64)
     for _ in range(args.warmup):
         sim.amplitudes(xs, ys)
     t0 = time.perf_counter()
@@ -197,7 +200,7 @@ def headline(args, pq, W, np):
         done_br, done_sec, amps = 0, 0.0, 0
         cpu_err = 0.0
         for i in range(nq):
-            a, sec, br = osim.chunk_range_timed(xs[i], ys[i], 0, nhi, threads)
+            a, sec, br = osim.chunk_range_timed(int(xs[i]), int(ys[i]), 0, nhi, threads)
             done_br += br; done_sec += sec; amps += 1
             cpu_err = max(cpu_err, abs(a - got[i]) / max(abs(a), 1e-300))
             if done_sec > args.cpu_seconds:

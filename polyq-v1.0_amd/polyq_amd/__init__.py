@@ -89,6 +89,35 @@ def _ptr(a, t):
     return a.ctypes.data_as(t)
 
 
+# Hot-path buffers: ctypes arrays over the caller's data (an ndarray's
+# .ctypes.data costs about 1 us per access; these a fraction of that).
+_U64_ARRAYS = {}
+_F64_ARRAYS = {}
+
+
+def _u64_arg(v):
+    """A ctypes uint64 array holding v (a list, tuple or ndarray of basis
+    indices), sharing memory with a writable C-contiguous uint64 ndarray."""
+    n = len(v)
+    t = _U64_ARRAYS.get(n)
+    if t is None:
+        t = _U64_ARRAYS[n] = ctypes.c_uint64 * n
+    if isinstance(v, np.ndarray):
+        if v.dtype != np.uint64 or not v.flags.c_contiguous or not v.flags.writeable:
+            v = np.array(v, dtype=np.uint64)
+        return t.from_buffer(v)
+    return t(*v)
+
+
+def _complex_out(n):
+    """An n-element complex128 result: (ctypes buffer for the call, ndarray view)."""
+    t = _F64_ARRAYS.get(n)
+    if t is None:
+        t = _F64_ARRAYS[n] = ctypes.c_double * (2 * n)
+    buf = t()
+    return buf, np.frombuffer(buf, dtype=np.complex128)
+
+
 class Phase:
     """Angle 2*pi*k/2^32 (src/phase.rs:22-52)."""
 
@@ -228,10 +257,11 @@ class Simulator:
         return out.value
 
     def amplitudes(self, inputs, outputs, form=None):
-        xs = np.ascontiguousarray(inputs, dtype=np.uint64)
-        ys = np.ascontiguousarray(outputs, dtype=np.uint64)
-        out = np.empty(len(xs), dtype=np.complex128)
-        _check(lib.pq_sim_amplitudes(self._h, len(xs), xs.ctypes.data, ys.ctypes.data, self._form(form), out.ctypes.data))
+        n = len(inputs)
+        if len(outputs) != n:
+            raise ValueError("inputs and outputs differ in length")
+        buf, out = _complex_out(n)
+        _check(lib.pq_sim_amplitudes(self._h, n, _u64_arg(inputs), _u64_arg(outputs), self._form(form), buf))
         return out
 
     def amplitude_estimate(self, input, output, samples, seed, form=None):
@@ -341,11 +371,11 @@ class Simulator:
 def batch_amplitudes(sims, inputs, outputs, form="z8"):
     """One persistent launch over queries of many simulators."""
     n = len(sims)
+    if len(inputs) != n or len(outputs) != n:
+        raise ValueError("sims, inputs and outputs differ in length")
     hs = (_sim_p * n)(*[s.handle for s in sims])
-    xs = np.ascontiguousarray(inputs, dtype=np.uint64)
-    ys = np.ascontiguousarray(outputs, dtype=np.uint64)
-    out = np.empty(n, dtype=np.complex128)
-    _check(lib.pq_batch_amplitudes(n, ctypes.cast(hs, ctypes.c_void_p), xs.ctypes.data, ys.ctypes.data, FORMS[form], out.ctypes.data))
+    buf, out = _complex_out(n)
+    _check(lib.pq_batch_amplitudes(n, ctypes.cast(hs, ctypes.c_void_p), _u64_arg(inputs), _u64_arg(outputs), FORMS[form], buf))
     return out
 
 
