@@ -262,6 +262,11 @@ struct SegImage {
     // change X(o) (table basis, bits 0..23) and the engine part of the phase
     // change Ue(o) = sum c_j lambda_j + 4 sum_{j<j'} c_j c_j' Q_jj' mod 8 (bits 24..26).
     std::vector<uint32_t> wtab;
+    // Lane-low table: for each pattern c of the L6 lane-low slots, the seed
+    // terms those slots contribute (query part delta excluded):
+    // {sum lambda_k + 4 |Q_k & c & below(k)| mod 8, XOR ox, XOR opin, 0,
+    //  XOR on (lo, hi)}, 6 words per entry, 2^L6 entries.
+    std::vector<uint32_t> lowtab;
     // The walk's points sorted by Ue mod 4: for M = 0..Mcap, at entry offset
     // 2^M - 1, 2^M records {X(o), gray(o) | h << 31, (-1)^h Fmid_M(o) hi, lo}
     // with h = Ue >> 2 (omega^4 = -1), and class boundaries pcls[5 M + c]

@@ -151,6 +151,7 @@ struct SegDev {
     const uint32_t* wtab;    // 2^Mcap: sign-mask change | engine phase change << 24
     const uint32_t* pts;     // walk points sorted by phase class mod 4, per M at 4(2^M - 1)
     const uint32_t* pcls;    // class boundaries, 5 per M
+    const uint32_t* lowtab;  // 2^L6 x 6 words: seed terms of the lane-low slots
     uint32_t I, npin, nout, L6, Mcap, nbytes, pad0, pad1;
 };
 struct SegQ {                // one query of a segment batch (64 B; item0 / nitems where Z8Q has them)
@@ -204,7 +205,7 @@ uint32_t pick_nw(uint32_t bits) {
 namespace {
 struct BlobLayout {
     size_t o_ku, o_kn, o_soff, o_supp, o_a0, o_a1, o_up, o_dn, o_diag, o_mrow, o_we, o_sq, o_sn, o_sp, o_sl, o_in, o_lt, o_gin;
-    size_t g_oq, g_ox, g_opin, g_olam, g_on, g_tab, g_lt, g_fmid, g_rmid, g_wtab, g_pts, g_pcls;
+    size_t g_oq, g_ox, g_opin, g_olam, g_on, g_tab, g_lt, g_fmid, g_rmid, g_wtab, g_pts, g_pcls, g_low;
     uint32_t NW, NWe;
 };
 // Host image of one engine: the plain FP64 part and the Z8 walk part in one blob.
@@ -264,6 +265,7 @@ BlobLayout build_blob(const HostEngine& E, const Z8Image& Z, const SegImage& G, 
     O.g_lt = B.put(G.lt.data(), G.lt.size() * 4); O.g_fmid = B.put(G.fmid.data(), G.fmid.size() * 4);
     O.g_rmid = B.put(G.rmid.data(), G.rmid.size() * 4);
     O.g_wtab = B.put(G.wtab.data(), G.wtab.size() * 4);
+    O.g_low = B.put(G.lowtab.data(), G.lowtab.size() * 4);
     O.g_pts = B.put(G.pts.data(), G.pts.size() * 4);
     O.g_pcls = B.put(G.pcls.data(), G.pcls.size() * 4);
     return O;
@@ -310,6 +312,7 @@ std::shared_ptr<DevEngine> dev_engine_upload(std::shared_ptr<const HostEngine> E
     g.olam = (const uint32_t*)at(O.g_olam); g.on = (const uint64_t*)at(O.g_on); g.table = (const float4*)at(O.g_tab);
     g.lt = (const float2*)at(O.g_lt); g.fmid = (const float*)at(O.g_fmid); g.rmid = (const float*)at(O.g_rmid);
     g.wtab = (const uint32_t*)at(O.g_wtab);
+    g.lowtab = (const uint32_t*)at(O.g_low);
     g.pts = (const uint32_t*)at(O.g_pts); g.pcls = (const uint32_t*)at(O.g_pcls);
     g.I = G.I; g.npin = G.npin; g.nout = G.nout; g.L6 = G.L6; g.Mcap = G.Mcap; g.nbytes = G.nbytes;
     if (G.nbytes > 8) throw std::invalid_argument("more than 64 terms");
@@ -869,6 +872,7 @@ struct SegSlots {
     P64 oq, on;
     P32 ox, opin, olam;
     PF rmid;
+    P32 low;                 // lane-low table, 6 words per pattern
 };
 // The lane's outer coordinates c: lane bits below L6 on slots 0..L6-1, the
 // rest on slots L6+M.. (the M walked slots stay 0 at the seed).  Every slot
@@ -880,8 +884,12 @@ template <class P64, class P32, class PF>
 __device__ __forceinline__ void seg_seed(const SegDev& E, const SegSlots<P64, P32, PF>& T, const SegQ& Q, uint64_t lane, SegLaneDev& L) {
     const uint32_t M = Q.M, L6 = E.L6;
     const uint64_t c = (lane & ((1ull << L6) - 1ull)) | ((lane >> L6) << (L6 + M));
-    uint32_t K = Q.K0, hs = Q.hs0, pio = Q.pio0;
-    uint64_t s = Q.s0;
+    // the lane-low slots from the engine's table (their query part is the
+    // parity of delta over them), the rest slot by slot
+    const uint32_t cl = (uint32_t)(c & ((1ull << L6) - 1ull)), e = 6 * cl;
+    uint32_t K = Q.K0 + T.low[e] + (((uint32_t)__popcll(Q.delta & (uint64_t)cl) & 1u) << 2);
+    uint32_t hs = Q.hs0 ^ T.low[e + 1], pio = Q.pio0 ^ T.low[e + 2];
+    uint64_t s = Q.s0 ^ ((uint64_t)T.low[e + 4] | ((uint64_t)T.low[e + 5] << 32));
     auto slot = [&](uint32_t k) {
         const uint32_t on = (uint32_t)(c >> k) & 1u, m = 0u - on;
         const uint32_t ell = T.olam[k] + (((uint32_t)(Q.delta >> k) & 1u) << 2) +
@@ -891,8 +899,6 @@ __device__ __forceinline__ void seg_seed(const SegDev& E, const SegSlots<P64, P3
         pio ^= T.opin[k] & m;
         s ^= T.on[k] & (0ull - (uint64_t)on);
     };
-#pragma unroll 4
-    for (uint32_t k = 0; k < L6; ++k) slot(k);
 #pragma unroll 4
     for (uint32_t k = L6 + M; k < E.nout; ++k) slot(k);
     float fh = 1.f, fl = 0.f;
@@ -992,6 +998,7 @@ struct SegLds {
     uint32_t *ox, *opin, *olam;
     float* rmid;             // 4 kSegMaxMid
     float (*red)[8];         // 4 waves
+    uint32_t* low;           // 6 x 64: the lane-low table
 };
 #define PQ_SEG_LDS(name)                                                   \
     __shared__ float2 name##_lt[8 * 256];                                  \
@@ -999,7 +1006,8 @@ struct SegLds {
     __shared__ uint32_t name##_ox[64], name##_opin[64], name##_olam[64];   \
     __shared__ float name##_rmid[4 * kSegMaxMid];                          \
     __shared__ float name##_red[4][8];                                     \
-    const SegLds name{name##_lt, name##_oq, name##_on, name##_ox, name##_opin, name##_olam, name##_rmid, name##_red}
+    __shared__ uint32_t name##_low[6 * 64];                                \
+    const SegLds name{name##_lt, name##_oq, name##_on, name##_ox, name##_opin, name##_olam, name##_rmid, name##_red, name##_low}
 
 // One item (256 lanes of one query): reload the engine tables into LDS if
 // the workgroup's last item was of another engine (eng, uniform), seed every
@@ -1012,14 +1020,46 @@ __device__ __forceinline__ void seg_item(const SegLds& S, const SegDev& E, const
     uint32_t *s_ox = S.ox, *s_opin = S.opin, *s_olam = S.olam;
     float* s_rmid = S.rmid;
     float (*s_red)[8] = S.red;
-    const SegSlots<LdsU64*, LdsU32*, LdsF32*> T{(LdsU64*)s_oq, (LdsU64*)s_on, (LdsU32*)s_ox, (LdsU32*)s_opin, (LdsU32*)s_olam, (LdsF32*)s_rmid};
-    if ((int)Q.eng != eng) {
-        for (uint32_t k = tid; k < E.nbytes * 256u; k += 256) s_lt[k] = E.lt[k];
-        if (tid < E.nout) {
-            s_oq[tid] = E.oq[tid]; s_on[tid] = E.on[tid];
-            s_ox[tid] = E.ox[tid]; s_opin[tid] = E.opin[tid]; s_olam[tid] = E.olam[tid];
+    uint32_t* s_low = S.low;
+    const SegSlots<LdsU64*, LdsU32*, LdsF32*> T{(LdsU64*)s_oq, (LdsU64*)s_on, (LdsU32*)s_ox, (LdsU32*)s_opin, (LdsU32*)s_olam, (LdsF32*)s_rmid,
+                                                 (LdsU32*)s_low};
+    if constexpr (!SHORT) {
+        // once per engine per workgroup of a persistent launch: the plain copy
+        if ((int)Q.eng != eng) {
+            for (uint32_t k = tid; k < E.nbytes * 256u; k += 256) s_lt[k] = E.lt[k];
+            if (tid < E.nout) {
+                s_oq[tid] = E.oq[tid]; s_on[tid] = E.on[tid];
+                s_ox[tid] = E.ox[tid]; s_opin[tid] = E.opin[tid]; s_olam[tid] = E.olam[tid];
+            }
+            if (tid < 4 * E.Mcap) s_rmid[tid] = E.rmid[tid];
+            for (uint32_t k = tid; k < (6u << E.L6); k += 256) s_low[k] = E.lowtab[k];
+            __syncthreads();
+            eng = (int)Q.eng;
         }
-        if (tid < 4 * E.Mcap) s_rmid[tid] = E.rmid[tid];
+    } else if ((int)Q.eng != eng) {
+        // latency-bound launches: every load first, then every LDS write,
+        // one memory latency rather than one per table row
+        const uint32_t nlt = E.nbytes * 256u, nout = E.nout, nrm = 4 * E.Mcap;
+        float2 lv[8];
+#pragma unroll
+        for (uint32_t j = 0; j < 8; ++j)
+            if (tid + 256u * j < nlt) lv[j] = E.lt[tid + 256u * j];
+        uint64_t vq = 0, vn = 0;
+        uint32_t vx = 0, vp = 0, vl = 0;
+        float vr = 0.f;
+        if (tid < nout) { vq = E.oq[tid]; vn = E.on[tid]; vx = E.ox[tid]; vp = E.opin[tid]; vl = E.olam[tid]; }
+        if (tid < nrm) vr = E.rmid[tid];
+        const uint32_t nlow = 6u << E.L6;
+        uint32_t w0 = 0, w1 = 0;
+        if (tid < nlow) w0 = E.lowtab[tid];
+        if (tid + 256u < nlow) w1 = E.lowtab[tid + 256u];
+#pragma unroll
+        for (uint32_t j = 0; j < 8; ++j)
+            if (tid + 256u * j < nlt) s_lt[tid + 256u * j] = lv[j];
+        if (tid < nout) { s_oq[tid] = vq; s_on[tid] = vn; s_ox[tid] = vx; s_opin[tid] = vp; s_olam[tid] = vl; }
+        if (tid < nrm) s_rmid[tid] = vr;
+        if (tid < nlow) s_low[tid] = w0;
+        if (tid + 256u < nlow) s_low[tid + 256u] = w1;
         __syncthreads();
         eng = (int)Q.eng;
     }
@@ -1881,7 +1921,7 @@ __global__ __launch_bounds__(256) void k_dump_seg(SegDev E, SegQ Q, uint8_t* __r
     const uint64_t lane = (uint64_t)blockIdx.x * 256 + threadIdx.x;
     if ((lane >> Q.H) != 0) return;
     SegLaneDev L;
-    const SegSlots<const uint64_t*, const uint32_t*, const float*> T{E.oq, E.on, E.ox, E.opin, E.olam, E.rmid};
+    const SegSlots<const uint64_t*, const uint32_t*, const float*> T{E.oq, E.on, E.ox, E.opin, E.olam, E.rmid, E.lowtab};
     seg_seed(E, T, Q, lane, L);
     const uint32_t M = Q.M;
     const uint32_t P = seg_lane_parities(E, T.oq, M, L.cout);

@@ -206,6 +206,18 @@ SegImage build_seg_image(const HostEngine& E) {
             const size_t at2 = (size_t)2 * ((1u << M) - 1 + o);
             split_df(f, S.fmid[at2], S.fmid[at2 + 1]);
         }
+    S.lowtab.assign((size_t)6 << S.L6, 0);
+    for (uint32_t c = 0; c < (1u << S.L6); ++c) {
+        uint32_t K = 0, hs = 0, pio = 0;
+        uint64_t sm = 0;
+        for (uint32_t k = 0; k < S.L6; ++k) {
+            if (!((c >> k) & 1)) continue;
+            K += S.olam[k] + 4 * ((uint32_t)__builtin_popcountll(S.oq[k] & (uint64_t)c & ((1ull << k) - 1)) & 1);
+            hs ^= S.ox[k]; pio ^= S.opin[k]; sm ^= S.on[k];
+        }
+        uint32_t* e = &S.lowtab[(size_t)6 * c];
+        e[0] = K & 7u; e[1] = hs; e[2] = pio; e[3] = 0; e[4] = (uint32_t)sm; e[5] = (uint32_t)(sm >> 32);
+    }
     S.wtab.assign((size_t)1 << S.Mcap, 0);
     for (uint32_t o = 0; o < (1u << S.Mcap); ++o) {
         const uint32_t g = o ^ (o >> 1);
