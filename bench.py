@@ -4,8 +4,8 @@ Headline (the JSON line's `value`): the plain-GPU-at-scale workload, one step
 = the 16 amplitudes <0|C|y> of the n=40 circuit "H on all qubits; 36 T + 300
 CX/CZ + 100 S interleaved (SplitMix64 seed 5); H on all qubits" (t = 30 terms
 after T merging, branch dimension d = 30: 16 x 2^30 branches), all 16
-evaluated in one persistent segment-walk launch (2^14-branch Walsh-Hadamard
-segments built once per circuit, the other 16 coordinates walked in Gray
+evaluated in one segment-walk launch (a 2^16-entry Walsh-Hadamard table at
+d = 30, built once per circuit; the other 14 coordinates walked in Gray
 order).  `value` counts the 2^d branches each amplitude sums.  Engine images
 are resident in HBM before timing starts.  Reported beside it:
   * (section saturated) the segment-walk kernel on a saturating launch (4096
@@ -14,7 +14,11 @@ are resident in HBM before timing starts.  Reported beside it:
   * the plain FP64 form (one query per launch, FP64 port of the reference's
     Gray walk) on the same 16 amplitudes, and per-amplitude error against it,
   * the CPU oracle (a C++ restatement of the reference's Rayon engine, OpenMP
-    over 2^14-branch Gray chunks) on the host cores, on a bounded sample,
+    over 2^14-branch Gray chunks) on the host cores, on a bounded sample, with
+    the same 2^d branches per amplitude (Cached), and with the oracle's own
+    best strategy (Auto: pair decomposition into 2^p smaller engines, built
+    and evaluated in parallel) for an amplitudes/s comparison; engine build
+    times are reported apart from evaluation,
   * the roofline fraction of the walk kernel against the VALU issue rate.
 Further sections (each bounded; --sections picks them):
   precision  n=48, 40 Rz(theta) (seed 4, d = 28): 1024 amplitudes in the
@@ -195,7 +199,8 @@ def headline(args, pq, W, np):
     if not args.no_cpu:
         O = _oracle()
         threads = max(1, min(16, os.cpu_count() or 1))
-        osim = O.OSim(O.OCircuit.from_circuit(c), "cached")
+        oc = O.OCircuit.from_circuit(c)
+        osim = O.OSim(oc, "cached", threads=threads)
         nhi = 1 << (d - min(d, 14))
         done_br, done_sec, amps = 0, 0.0, 0
         cpu_err = 0.0
@@ -210,6 +215,25 @@ def headline(args, pq, W, np):
             "sample": "%d full amplitudes (%d x 2^%d branches, 2^14-branch Gray chunks, OpenMP dynamic) of the same workload"
                       % (amps, amps, d),
             "amplitudes_per_s": amps / done_sec, "max_rel_err_z8_vs_cpu": cpu_err,
+            "engine_new_s": osim.build_seconds,
+        }
+        # the oracle's own best strategy: Auto (pair decomposition by the
+        # cost model, branch engines built and evaluated in parallel); eval
+        # timed alone, median of 5 passes over the 16 amplitudes after one
+        auto = O.OSim(oc, "auto", threads=threads)
+        st = auto.stats()
+        auto.amplitudes_timed(xs[:1], ys[:1], threads)
+        secs, auto_err = [], 0.0
+        for _ in range(5):
+            aa, sec = auto.amplitudes_timed(xs, ys, threads)
+            secs.append(sec)
+            auto_err = max(auto_err, float(np.max(np.abs(aa - got) / np.maximum(np.abs(aa), 1e-300))))
+        res["cpu_auto"] = {
+            "strategy": "auto", "pairs": st[6], "engines": 1 << st[6], "engine_branch_dim": st[5],
+            "branches_per_amplitude": (1 << st[6]) << st[5], "engine_new_s": auto.build_seconds,
+            "amplitudes_per_s": nq / float(np.median(secs)), "cores": threads,
+            "sample": "5 passes over the %d amplitudes of one step after one warm-up amplitude; median pass" % nq,
+            "max_rel_err_z8_vs_cpu_auto": auto_err,
         }
     return res
 
@@ -507,6 +531,8 @@ def main():
         "branch_walk": h["branch_walk"],
         "cpu_baseline": cpu,
         "speedup_vs_cpu": value / cpu["value"] if cpu else None,
+        "cpu_auto": h.get("cpu_auto"),
+        "speedup_vs_cpu_auto_amps": amps / h["cpu_auto"]["amplitudes_per_s"] if h.get("cpu_auto") else None,
     }
     line.update(extra)
     if "serve" in extra:

@@ -835,7 +835,7 @@ struct Engine {  // engine.rs:240-284 (Core + Branching + Engine flattened)
             }
             return gray(u, sv, c, dl);
         };
-        if (d >= parallel_min_dim && nhi > 1 && threads > 1) {
+        if (d >= parallel_min_dim && nhi > 1 && threads > 1 && !omp_in_parallel()) {
             double sr = 0, si = 0;
             #pragma omp parallel for schedule(dynamic, 1) reduction(+:sr, si) num_threads(threads)
             for (long hi = 0; hi < (long)nhi; ++hi) { C64 r = chunk((size_t)hi); sr += r.re; si += r.im; }
@@ -958,34 +958,58 @@ struct RankEngine {  // rank.rs:47-51
     std::vector<Engine> branches;
     size_t pairs = 0;
     // rank.rs:88-103
-    static RankEngine build(const PathSum& ps, const std::vector<size_t>& fixed, size_t pairs, PivotRule rule) {
+    // The 2^p branch engines are independent (into_par_iter there); with
+    // threads > 1 they are built by that many OpenMP workers, each into its
+    // own slot, so the result does not depend on the thread count.
+    static RankEngine build(const PathSum& ps, const std::vector<size_t>& fixed, size_t pairs, PivotRule rule, int threads = 1) {
         size_t nt;
         auto cands = pair_candidates(ps, fixed, nt);
         size_t p = std::min(std::min(pairs, cands.size()), MAX_PAIRS);
         RankEngine R; R.pairs = p;
-        for (size_t pattern = 0; pattern < ((size_t)1 << p); ++pattern) {
-            PathSum b = ps;
-            for (size_t i = 0; i < p; ++i) b.substitute(cands[i].first, cands[i].second, (pattern >> i) & 1);
-            R.branches.push_back(Engine::build(b, fixed, rule));
+        const long nb = 1l << p;
+        R.branches.resize((size_t)nb);
+        std::string err;
+        #pragma omp parallel for schedule(dynamic, 1) num_threads(threads) if (threads > 1 && nb > 1)
+        for (long pattern = 0; pattern < nb; ++pattern) {
+            try {
+                PathSum b = ps;
+                for (size_t i = 0; i < p; ++i) b.substitute(cands[i].first, cands[i].second, (pattern >> i) & 1);
+                R.branches[(size_t)pattern] = Engine::build(b, fixed, rule);
+            } catch (std::exception& e) {
+                #pragma omp critical(rank_build_err)
+                err = e.what();
+            }
         }
+        if (!err.empty()) throw std::runtime_error(err);
         return R;
     }
     // rank.rs:115-123
-    static RankEngine automatic(const PathSum& ps, const std::vector<size_t>& fixed, PivotRule rule) {
+    static RankEngine automatic(const PathSum& ps, const std::vector<size_t>& fixed, PivotRule rule, int threads = 1) {
         Engine base = Engine::build(ps, fixed, rule);
         size_t nt;
         auto cands = pair_candidates(ps, fixed, nt);
         size_t p = best_pairs(base.stats(), cands.size(), nt);
         if (p == 0) { RankEngine R; R.branches.push_back(std::move(base)); R.pairs = 0; return R; }
-        return build(ps, fixed, p, rule);
+        return build(ps, fixed, p, rule, threads);
     }
-    // rank.rs:184-190
+    // rank.rs:184-190: 8 or more branch engines are evaluated in parallel
+    // (par_iter there).  Here each engine's value lands in its own slot and
+    // the slots are added in engine order, so the sum is the sequential one.
+    int threads = 1;
     C64 eval_assignment(const std::vector<std::pair<size_t, bool>>& asg) const {
         C64 tot(0, 0);
+        if (branches.size() >= 8 && threads > 1) {
+            std::vector<C64> v(branches.size(), C64(0, 0));
+            const long nb = (long)branches.size();
+            #pragma omp parallel for schedule(dynamic, 1) num_threads(threads)
+            for (long i = 0; i < nb; ++i) v[(size_t)i] = branches[(size_t)i].eval_assignment(asg);
+            for (auto& x : v) tot = tot + x;
+            return tot;
+        }
         for (auto& e : branches) tot = tot + e.eval_assignment(asg);
         return tot;
     }
-    void set_threads(int t) { for (auto& e : branches) e.threads = t; }
+    void set_threads(int t) { threads = t; for (auto& e : branches) e.threads = t; }
     // rank.rs:149-171: one branch: Engine::estimate; several: exact values of
     // uniformly drawn branches (sample i draws from Rng(seed ^ i * gamma))
     std::pair<C64, double> estimate(const std::vector<std::pair<size_t, bool>>& asg, size_t samples, uint64_t seed) const {
@@ -1015,10 +1039,10 @@ struct RankEngine {  // rank.rs:47-51
 // ----------------------------------------------------------------- sim.rs ---
 enum Strategy { Auto = 0, Cached = 1, Pairs = 2 };  // sim.rs:40-48
 
-static RankEngine build_engine(const PathSum& ps, const std::vector<size_t>& fixed, int strategy, size_t pairs) {  // sim.rs:65-71
-    if (strategy == Auto) return RankEngine::automatic(ps, fixed, MinWeight);
+static RankEngine build_engine(const PathSum& ps, const std::vector<size_t>& fixed, int strategy, size_t pairs, int threads = 1) {  // sim.rs:65-71
+    if (strategy == Auto) return RankEngine::automatic(ps, fixed, MinWeight, threads);
     if (strategy == Cached) return RankEngine::build(ps, fixed, 0, MinWeight);
-    return RankEngine::build(ps, fixed, pairs, MinWeight);
+    return RankEngine::build(ps, fixed, pairs, MinWeight, threads);
 }
 
 struct Simulator {  // sim.rs:128-135
@@ -1274,6 +1298,26 @@ int orc_sim_create(uint32_t n, size_t ng, const uint8_t* ops, const uint32_t* q0
         h->s.strategy = strategy; h->s.pairs_opt = pairs;
         h->s.ps = from_circuit(h->s.circuit);
         h->s.engine = build_engine(h->s.ps, h->s.ps.io_vars(), strategy, pairs);
+        *out = h;
+        return 0;
+    } catch (std::exception& e) { g_err = e.what(); return -1; }
+}
+// orc_sim_create with the branch engines built by `threads` OpenMP workers;
+// *build_seconds is the wall time of the engine build (Engine::new, or
+// RankEngine::auto / new), reported apart from evaluation.
+int orc_sim_create_threaded(uint32_t n, size_t ng, const uint8_t* ops, const uint32_t* q0, const uint32_t* q1, const uint32_t* ph,
+                            int strategy, uint32_t pairs, int threads, orc_sim** out, double* build_seconds) {
+    try {
+        auto* h = new orc_sim;
+        h->s.circuit = circuit_from(n, ng, ops, q0, q1, ph);
+        h->s.strategy = strategy; h->s.pairs_opt = pairs;
+        h->s.ps = from_circuit(h->s.circuit);
+        auto t0 = std::chrono::steady_clock::now();
+        try {
+            h->s.engine = build_engine(h->s.ps, h->s.ps.io_vars(), strategy, pairs, threads);
+        } catch (...) { delete h; throw; }
+        *build_seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+        h->s.threads = threads; h->s.engine.set_threads(threads);
         *out = h;
         return 0;
     } catch (std::exception& e) { g_err = e.what(); return -1; }

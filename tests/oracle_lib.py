@@ -41,6 +41,7 @@ def lib():
         L.orc_pathsum_info.argtypes = circ + [_u64p]
         L.orc_normal_form_check.argtypes = [ctypes.c_uint32, _u8p, _u8p, _u8p, ctypes.c_int, ctypes.c_int, _u64p, _f64p, _f64p, ctypes.POINTER(ctypes.c_int)]
         L.orc_sim_create.argtypes = circ + [ctypes.c_int, ctypes.c_uint32, ctypes.POINTER(ctypes.c_void_p)]
+        L.orc_sim_create_threaded.argtypes = circ + [ctypes.c_int, ctypes.c_uint32, ctypes.c_int, ctypes.POINTER(ctypes.c_void_p), ctypes.POINTER(ctypes.c_double)]
         L.orc_sim_destroy.argtypes = [ctypes.c_void_p]
         L.orc_sim_destroy.restype = None
         L.orc_sim_set_threads.argtypes = [ctypes.c_void_p, ctypes.c_int]
@@ -133,10 +134,19 @@ STRAT = {"auto": 0, "cached": 1, "pairs": 2}
 
 
 class OSim:
-    def __init__(self, c, strategy="auto", pairs=0):
+    def __init__(self, c, strategy="auto", pairs=0, threads=None):
+        """threads=None: single-threaded build and evaluation.  threads=k: the
+        branch engines are built and evaluated by k OpenMP workers and the
+        build's wall time is kept in `build_seconds`."""
         self.c = c
         h = ctypes.c_void_p()
-        _check(lib().orc_sim_create(*c.args(), STRAT[strategy], pairs, ctypes.byref(h)))
+        self.build_seconds = None
+        if threads is None:
+            _check(lib().orc_sim_create(*c.args(), STRAT[strategy], pairs, ctypes.byref(h)))
+        else:
+            sec = ctypes.c_double()
+            _check(lib().orc_sim_create_threaded(*c.args(), STRAT[strategy], pairs, int(threads), ctypes.byref(h), ctypes.byref(sec)))
+            self.build_seconds = sec.value
         self.h = h
 
     def __del__(self):

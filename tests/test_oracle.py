@@ -376,3 +376,26 @@ def test_timed_full_range_is_the_amplitude():
         a, sec, br = o.chunk_range_timed(x, y, 0, 1 << (d - min(d, 14)), 2)
         assert abs(a - o.amplitude(x, y)) < 1e-12
         assert br == (1 << d) or br == 0
+
+
+def test_parallel_rank_engine_equals_serial():
+    # rank.rs:92-99 / :184-186 build and evaluate 8 or more branch engines in
+    # parallel; the oracle's threaded build and evaluation must give the same
+    # engines and (engine-ordered) sums, bit for bit, as the serial ones
+    rng = W.Rng(77)
+    checked = 0
+    for trial in range(8):
+        c = W.sandwich_layer(10, 10 + trial, 40, 12, rng)
+        oc = O.OCircuit.from_circuit(c)
+        for strat, p in (("pairs", 4), ("auto", 0)):
+            serial = O.OSim(oc, strat, p)
+            par = O.OSim(oc, strat, p, threads=4)
+            assert par.stats() == serial.stats()
+            assert par.build_seconds is not None and par.build_seconds >= 0
+            xs = [rng.below(1 << 10) for _ in range(4)]
+            ys = [rng.below(1 << 10) for _ in range(4)]
+            got, _ = par.amplitudes_timed(xs, ys, 4)
+            for i in range(4):
+                assert got[i] == serial.amplitude(xs[i], ys[i])
+            checked += serial.stats()[6] >= 3
+    assert checked >= 4
