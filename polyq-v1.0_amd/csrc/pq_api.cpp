@@ -114,13 +114,13 @@ void build_jobs(size_t n, int walk, const std::function<const RankEng*(size_t)>&
             last = R;
         }
     }
-    // At least 8 queries per chunk (a prologue takes about 0.3 us, a hand-off
-    // to another thread about as much as a few of them): batches below two
-    // chunks stay on the calling thread, and batches below 256 queries use
-    // at most the caller and the pool's polling workers, so no blocked
-    // worker has to be woken.
-    constexpr size_t kChunk = 8;
-    if (n < 2 * kChunk) {
+    // A prologue takes about 0.15 us and handing chunks to the pool costs a
+    // few microseconds, so batches below kInline queries stay on the calling
+    // thread.  Larger ones go out in chunks of at least 8 queries; below 256
+    // queries only to the caller and the pool's polling workers, so no
+    // blocked worker has to be woken.
+    constexpr size_t kChunk = 8, kInline = 48;
+    if (n < kInline) {
         jobs.clear(); owner.clear();
         for (size_t i = 0; i < n; ++i) {
             check(i);

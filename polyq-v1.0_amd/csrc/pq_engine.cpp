@@ -331,6 +331,15 @@ std::shared_ptr<HostEngine> build_engine(const PathSum& ps, const std::vector<ui
 
     for (uint32_t f : fixed) { E.fixed_v.push_back(ps.v[f]); E.fixed_theta.push_back(ps.theta[f]); }
     E.nh = ps.nh; E.global = ps.global; E.zero = ps.zero;
+    E.mag = std::pow(2.0, E.log2mag - (double)E.nh / 2.0) * E.Bmag;
+    E.ucol_dq.assign(E.nf, 0);
+    for (uint32_t f = 0; f < E.nf; ++f)
+        for (uint32_t j = 0; j < d; ++j)
+            if (row_parity(&E.ucols[(size_t)f * W], &E.dual[(size_t)j * W], W)) E.ucol_dq[f] |= 1ull << j;
+    E.mcol_dq.assign(t, 0);
+    for (uint32_t k = 0; k < t; ++k)
+        for (uint32_t j = 0; j < d; ++j)
+            if (row_parity(&E.mcols[(size_t)k * W], &E.dual[(size_t)j * W], W) ^ (((E.kernel[j] >> k) & 1) != 0)) E.mcol_dq[k] |= 1ull << j;
     return Ep;
 }
 
@@ -368,25 +377,52 @@ BoundaryProg boundary_prog(const PathSum& ps, const HostEngine& E) {
         B.out_pos[q] = E.fixed_pos[v];
         owner[v] = (int32_t)(64 + q);
     }
+    // runs of consecutive qubits on consecutive positions within one word
+    auto add_bit = [&](uint8_t src, uint32_t q, uint32_t pos) {
+        if (!B.runs.empty()) {
+            auto& r = B.runs.back();
+            if (r.src == src && r.q0 + r.len == q && r.word == pos >> 6 && r.shift + r.len == (pos & 63)) { ++r.len; return; }
+        }
+        B.runs.push_back(BoundaryProg::Run{src, (uint8_t)q, 1, (uint8_t)(pos >> 6), (uint8_t)(pos & 63)});
+    };
+    for (uint32_t q = 0; q < ps.nq; ++q) add_bit(0, q, B.in_pos[q]);
+    for (uint32_t q = 0; q < ps.nq; ++q) {
+        if (B.out_neg[q]) B.yneg |= 1ull << q;
+        if (B.out_chk[q] >= 0) B.checks.push_back({q, B.out_chk[q]});
+        else add_bit(1, q, (uint32_t)B.out_pos[q]);
+    }
     return B;
 }
 
 bool prepare_bits(const HostEngine& E, const BoundaryProg& B, uint64_t x, uint64_t y, Prepared& P, bool* contradictory) {
     // pathsum.rs:430-450 with the (variable, value) pairs written straight
     // into the fixed-position bit row
+    if (E.nfw <= 2 && B.nq <= 64) {
+        // the fixed row from the runs (at most two words: nf <= 2 nq)
+        uint64_t on[2] = {0, 0};
+        const uint64_t yy = y ^ B.yneg;
+        for (const auto& r : B.runs) on[r.word] |= (((r.src ? yy : x) >> r.q0) & (~0ull >> (64 - r.len))) << r.shift;
+        for (const auto& c : B.checks) {
+            const uint64_t prev = c.second < 64 ? ((x >> c.second) & 1) : ((yy >> (c.second - 64)) & 1);
+            if (prev != ((yy >> c.first) & 1)) { P.nonzero = false; if (contradictory) *contradictory = true; return false; }
+        }
+        if (contradictory) *contradictory = false;
+        return prepare_ones(E, on, P);
+    }
     thread_local std::vector<uint64_t> ones_tls;
     std::vector<uint64_t>& ones = ones_tls;
     ones.assign(E.nfw, 0);
     uint64_t* on = ones.data();
-    for (uint32_t q = 0; q < B.nq; ++q) if ((x >> q) & 1) on[B.in_pos[q] >> 6] |= 1ull << (B.in_pos[q] & 63);
+    // data-dependent bits are OR-ed in without branches
+    for (uint32_t q = 0; q < B.nq; ++q) on[B.in_pos[q] >> 6] |= ((x >> q) & 1) << (B.in_pos[q] & 63);
     for (uint32_t q = 0; q < B.nq; ++q) {
-        const uint32_t v = (uint32_t)((y >> q) & 1) ^ B.out_neg[q];
+        const uint64_t v = ((y >> q) & 1) ^ B.out_neg[q];
         const int32_t c = B.out_chk[q];
         if (c >= 0) {
-            const uint32_t prev = c < 64 ? (uint32_t)((x >> c) & 1) : ((uint32_t)((y >> (c - 64)) & 1) ^ B.out_neg[c - 64]);
+            const uint64_t prev = c < 64 ? ((x >> c) & 1) : (((y >> (c - 64)) & 1) ^ B.out_neg[c - 64]);
             if (prev != v) { P.nonzero = false; if (contradictory) *contradictory = true; return false; }
-        } else if (v) {
-            on[B.out_pos[q] >> 6] |= 1ull << (B.out_pos[q] & 63);
+        } else {
+            on[B.out_pos[q] >> 6] |= v << (B.out_pos[q] & 63);
         }
     }
     if (contradictory) *contradictory = false;
@@ -399,6 +435,7 @@ static bool prepare_ones(const HostEngine& E, const uint64_t* __restrict__ on, P
     const ULayout& L = E.L;
     const uint32_t W = L.words();
     uint32_t z = 0;
+    uint64_t dq = 0;
     Phase th = E.global;
     P.u0.assign(W, 0);
     // Z4 value of the fixed assignment: linear part plus 2 per adjacent pair of
@@ -414,6 +451,7 @@ static bool prepare_ones(const HostEngine& E, const uint64_t* __restrict__ on, P
             for (uint32_t k = 0; k < nfw; ++k) pc += (uint32_t)__builtin_popcountll(b[k] & on[k]);
             z += E.fixed_v[f] + pc;
             th += E.fixed_theta[f];
+            dq ^= E.ucol_dq[f];
             const uint64_t* __restrict__ c = &E.ucols[(size_t)f * W];
             for (uint32_t k = 0; k < W; ++k) u0[k] ^= c[k];
         }
@@ -429,24 +467,27 @@ static bool prepare_ones(const HostEngine& E, const uint64_t* __restrict__ on, P
     for (uint32_t i = 0; i < E.rk; ++i) if (E.pivot_row[i].dot(kappa)) s0 |= 1ull << E.pivot_col[i];
     P.s0 = s0;
     P.u_start = P.u0;
-    P.c_start = {1, 0};
-    for (uint32_t k = 0; k < E.t; ++k) {
-        if ((s0 >> k) & 1) {
-            const uint64_t* c = &E.mcols[(size_t)k * W];
-            for (uint32_t w = 0; w < W; ++w) P.u_start[w] ^= c[w];
-            P.c_start *= E.a1[k];
-        } else P.c_start *= E.a0[k];
+    uint64_t* __restrict__ us = P.u_start.data();
+    for (uint64_t b = s0; b; b &= b - 1) {
+        const uint32_t k = (uint32_t)__builtin_ctzll(b);
+        const uint64_t* __restrict__ c = &E.mcols[(size_t)k * W];
+        for (uint32_t w = 0; w < W; ++w) us[w] ^= c[w];
+        dq ^= E.mcol_dq[k];
     }
-    P.K0 = total_phase(E, P.u_start.data(), s0);
+    P.K0 = total_phase(E, us, s0);
+    // ell_j = lam_j + 4 (parity(u_start . dual_j) ^ parity(s0 . kernel_j))
+    P.dq = dq;
     P.ell.resize(E.d);
-    for (uint32_t j = 0; j < E.d; ++j) {
-        bool dq = row_parity(P.u_start.data(), &E.dual[(size_t)j * W], W) ^ ((__builtin_popcountll(s0 & E.kernel[j]) & 1) != 0);
-        P.ell[j] = (uint8_t)((E.lam[j] + (dq ? 4 : 0)) & 7);
-    }
-    double mag = std::pow(2.0, E.log2mag - (double)E.nh / 2.0) * E.Bmag;   // engine.rs:563-565
-    P.scale = phase_complex(P.phase) * E.Ephase * mag;
+    for (uint32_t j = 0; j < E.d; ++j) P.ell[j] = (uint8_t)((E.lam[j] + (uint32_t)((dq >> j) & 1) * 4) & 7);
+    P.scale = phase_complex(P.phase) * E.Ephase * E.mag;
     P.nonzero = true;
     return true;
+}
+
+std::complex<double> plain_c_start(const HostEngine& E, uint64_t s0) {
+    std::complex<double> c(1, 0);
+    for (uint32_t k = 0; k < E.t; ++k) c *= ((s0 >> k) & 1) ? E.a1[k] : E.a0[k];
+    return c;
 }
 
 bool prepare_assignment(const HostEngine& E, const std::vector<std::pair<uint32_t, uint8_t>>& asg, Prepared& P) {

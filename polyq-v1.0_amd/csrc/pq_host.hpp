@@ -162,6 +162,13 @@ struct HostEngine {
     std::vector<uint64_t> Q;                 // d rows (bit k of row j = Q_jk), symmetric, zero diagonal
     std::vector<uint8_t> lam;                // query-independent part of the linear coefficients
     std::vector<uint64_t> dual;              // d packed rows: (b_j | a_j | y_j | 0) for the query-dependent parity
+    // The query-dependent parities of the d linear coefficients, as d-bit
+    // masks: bit j of ucol_dq[f] = parity(ucols[f] . dual_j); bit j of
+    // mcol_dq[k] = parity(mcols[k] . dual_j) ^ bit k of kernel_j.  A query's
+    // mask is the XOR over its ones and the terms of s0 (parity is linear).
+    std::vector<uint64_t> ucol_dq;           // nf
+    std::vector<uint64_t> mcol_dq;           // t
+    double mag = 1.0;                        // 2^(log2mag - nh/2) * Bmag (engine.rs:563-565)
     // reference-style FP64 term multipliers for the plain kernel (engine.rs:221-238)
     std::vector<std::complex<double>> a0, a1, up, down;
     EngineStats stats() const { return EngineStats{m, L.g, L.nodd, L.neven, t, d}; }
@@ -177,12 +184,16 @@ struct Prepared {
     uint64_t s0 = 0;                 // particular solution over the t terms
     std::vector<uint64_t> u_start;   // packed shift row at c = 0
     std::vector<uint64_t> u0;        // shift row of the boundary alone (s = 0)
-    std::complex<double> c_start;    // reference-style weight at s0 (plain kernel)
     uint32_t K0 = 0;                 // total Z8 phase at c = 0
     std::vector<uint8_t> ell;        // d linear Z8 coefficients
+    uint64_t dq = 0;                 // bit j: ell[j] - lam[j] = 4 (the query-dependent part)
     std::complex<double> scale;      // e^{i phase} 2^{log2mag - h/2} E B
 };
 bool prepare(const HostEngine& E, const uint8_t* values, Prepared& P);
+// Reference-style FP64 weight at s0 (engine.rs:229-238: the product of the
+// term multipliers a1[k] for k in s0, a0[k] otherwise, in term order); only
+// the plain FP64 form uses it.
+std::complex<double> plain_c_start(const HostEngine& E, uint64_t s0);
 // Boundary of <y|C|x> mapped straight to the engine's fixed positions
 // (pathsum.rs:430-450): input qubit q fixes position in_pos[q] to x_q; output
 // qubit q fixes out_pos[q] to y_q ^ out_neg[q], or, when its variable already
@@ -193,6 +204,13 @@ struct BoundaryProg {
     std::vector<uint32_t> in_pos;
     std::vector<int32_t> out_pos, out_chk;
     std::vector<uint8_t> out_neg;
+    // The same map as runs: bits q0 .. q0+len-1 of x (src 0) or of y ^ yneg
+    // (src 1) land on bits shift .. of fixed-row word `word`.  Inputs and
+    // outputs usually map to a few such runs.
+    struct Run { uint8_t src, q0, len, word, shift; };
+    std::vector<Run> runs;
+    uint64_t yneg = 0;                                  // out_neg as a mask
+    std::vector<std::pair<uint32_t, int32_t>> checks;  // (q, out_chk[q]) for out_chk[q] >= 0
 };
 BoundaryProg boundary_prog(const PathSum& ps, const HostEngine& E);
 // prepare() for <y|C|x> from basis indices; *contradictory tells a

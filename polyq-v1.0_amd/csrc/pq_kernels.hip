@@ -499,7 +499,8 @@ PlainQ plain_query(const DevEngine& D, const Prepared& P) {
     put_u32(us, P.u_start.data() + L.off_o(), L.Wo, L.nodd, E.NW);
     if (us.size() > 96) throw std::invalid_argument("plain kernel state above 96 dwords");
     std::copy(us.begin(), us.end(), Q.ustart);
-    Q.s0 = P.s0; Q.cre = P.c_start.real(); Q.cim = P.c_start.imag();
+    const std::complex<double> cs = plain_c_start(H, P.s0);
+    Q.s0 = P.s0; Q.cre = cs.real(); Q.cim = cs.imag();
     return Q;
 }
 }  // namespace

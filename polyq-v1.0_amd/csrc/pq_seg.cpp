@@ -314,15 +314,9 @@ SegQueryHost seg_query(const HostEngine& E, const SegImage& S, const Prepared& P
     q.K0 = P.K0;
     q.s0 = P.s0;
     q.delta = 0;
-    for (uint32_t k = 0; k < S.nout; ++k) {
-        const uint32_t j = S.outer_j[k];
-        if ((((uint32_t)P.ell[j] - E.lam[j]) >> 2) & 1) q.delta |= 1ull << k;
-    }
+    for (uint32_t k = 0; k < S.nout; ++k) q.delta |= ((P.dq >> S.outer_j[k]) & 1) << k;
     q.hs0 = 0;
-    for (uint32_t p = 0; p < S.I; ++p) {
-        const uint32_t j = S.inner_j[p];
-        if ((((uint32_t)P.ell[j] - E.lam[j]) >> 2) & 1) q.hs0 ^= S.hsmap[p];
-    }
+    for (uint32_t p = 0; p < S.I; ++p) q.hs0 ^= S.hsmap[p] & (0u - (uint32_t)((P.dq >> S.inner_j[p]) & 1));
     q.pio0 = 0;
     for (uint32_t b = 0; b < S.npin; ++b) if ((P.s0 >> S.pin_term[b]) & 1) q.pio0 |= 1u << b;
     return q;
