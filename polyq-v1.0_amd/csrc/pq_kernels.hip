@@ -1010,11 +1010,210 @@ struct SegLds {
     __shared__ uint32_t name##_low[6 * 64];                                \
     const SegLds name{name##_lt, name##_oq, name##_on, name##_ox, name##_opin, name##_olam, name##_rmid, name##_red, name##_low}
 
+// Lane total omega^K (EVu + omega ODu) times the lane factor f (double-float).
+__device__ __forceinline__ void seg_lane_total(const float (&eu)[4], const float (&ou)[4], uint32_t K, float fh, float fl,
+                                               float (&ev)[4], float (&od)[4]) {
+    const uint32_t Ks = K & 7u;
+    float e2[4], o2[4];
+    if (Ks & 1u) {   // omega^(2a+1) (EVu + omega ODu) = i^a (i ODu + omega EVu)
+        e2[0] = -ou[2]; e2[1] = -ou[3]; e2[2] = ou[0]; e2[3] = ou[1];
+        o2[0] = eu[0]; o2[1] = eu[1]; o2[2] = eu[2]; o2[3] = eu[3];
+    } else {
+#pragma unroll
+        for (int c = 0; c < 4; ++c) { e2[c] = eu[c]; o2[c] = ou[c]; }
+    }
+    df_rot(e2, Ks >> 1);
+    df_rot(o2, Ks >> 1);
+#pragma unroll
+    for (int c = 0; c < 4; c += 2) {
+        df_mul(e2[c], e2[c + 1], fh, fl, ev[c], ev[c + 1]);
+        df_mul(o2[c], o2[c + 1], fh, fl, od[c], od[c + 1]);
+    }
+}
+// Workgroup reduction of the four double-float sums: each wave over DPP
+// (total in lane 63), then the four waves through LDS.  The item's four
+// partial sums (EV re, im, OD re, im), one per lane of lanes 0..3: a single
+// 32-byte write, into pinned host memory for small batches (the host
+// finishes each query as its items arrive) or device memory before
+// k_z8_reduce.  Each word is one aligned 8-byte store.
+__device__ __forceinline__ void seg_publish(float (*s_red)[8], const float (&ev)[4], const float (&od)[4], uint64_t item,
+                                            double* __restrict__ partial) {
+    const uint32_t tid = threadIdx.x;
+    float wv[8] = {ev[0], ev[1], ev[2], ev[3], od[0], od[1], od[2], od[3]};
+    wave_sum_dpp(wv);
+    if ((tid & 63) == 63) {
+#pragma unroll
+        for (int c = 0; c < 8; ++c) s_red[tid >> 6][c] = wv[c];
+    }
+    __syncthreads();
+    if (tid < 4) {
+        double v = 0.0;
+        for (int w = 0; w < 4; ++w) v += (double)s_red[w][2 * tid] + (double)s_red[w][2 * tid + 1];
+        partial[4 * item + tid] = v;
+    }
+}
+// XOR and sum of a value over the 64 lanes of the wave, broadcast to all
+// (the DPP order of wave_sum_dpp; all 64 lanes must be active).
+template <int CTRL, int ROWS>
+__device__ __forceinline__ uint32_t dpp_src(uint32_t v) { return (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, CTRL, ROWS, 0xF, false); }
+__device__ __forceinline__ uint32_t wave_xor(uint32_t v) {
+    v ^= dpp_src<0xB1, 0xF>(v);
+    v ^= dpp_src<0x4E, 0xF>(v);
+    v ^= dpp_src<0x141, 0xF>(v);
+    v ^= dpp_src<0x140, 0xF>(v);
+    v ^= dpp_src<0x142, 0xA>(v);
+    v ^= dpp_src<0x143, 0xC>(v);
+    return (uint32_t)__builtin_amdgcn_readlane((int)v, 63);
+}
+__device__ __forceinline__ uint32_t wave_add(uint32_t v) {
+    v += dpp_src<0xB1, 0xF>(v);
+    v += dpp_src<0x4E, 0xF>(v);
+    v += dpp_src<0x141, 0xF>(v);
+    v += dpp_src<0x140, 0xF>(v);
+    v += dpp_src<0x142, 0xA>(v);
+    v += dpp_src<0x143, 0xC>(v);
+    return (uint32_t)__builtin_amdgcn_readlane((int)v, 63);
+}
+
+// Short walks (2^MW points, MW <= kSegShortWalk), for latency-bound
+// launches: every load that depends only on the engine and the query is
+// issued first, the lane seed is formed in registers (no LDS tables, no
+// workgroup barrier before the walk), and the table reads and the lane-factor
+// byte lookups go out together.
+//   Lane coordinates c = (c_low on the L6 lane-low slots, c_hi on the slots
+// past the walk).  c_low is the lane's index in its wave; c_hi = lane >> L6
+// is the same for the whole wave (L6 = 6 whenever a slot past the lane-low
+// ones exists), so the slot terms of c_hi are summed once per wave, one slot
+// per lane, and the lane-low terms come from the engine's lane-low table:
+//   K  = K0 + low_K(c_low) + 4 (delta . c_low) + sum_{k in c_hi} (lambda_k
+//        + 4 delta_k + 4 (Q_k . c_hi below k)) + 4 ((XOR_{k in c_hi} Q_k) . c_low)
+//   hs, pio, s: the query's value XOR the lane-low entry XOR the c_hi slots.
+// The results equal the slot-by-slot seed bit for bit (integers mod 8 and
+// XORs), and the magnitudes multiply in the same slot order.
+template <uint32_t MW>
+__device__ __forceinline__ void seg_short(float (*s_red)[8], const SegDev& E, const SegQ& Q, uint64_t item, double* __restrict__ partial) {
+    constexpr uint32_t NP = 1u << MW;
+    const uint32_t tid = threadIdx.x;
+    const uint32_t L6 = E.L6, nout = E.nout, I = E.I, nbytes = E.nbytes, Mcap = E.Mcap;
+    const uint64_t lane = (item - Q.item0) * 256 + tid;
+    const bool live = (lane >> Q.H) == 0;
+    const uint32_t lowm = (1u << L6) - 1u;
+    const uint32_t cl = tid & lowm;
+    const uint64_t chi = L6 == 6 ? (((item - Q.item0) << 2) | (uint64_t)__builtin_amdgcn_readfirstlane(tid >> 6)) : 0ull;
+    const uint64_t cplace = chi << (L6 + MW);
+    // ---- loads, none behind a branch (indices clamped into the arrays)
+    const uint32_t* lw = E.lowtab + 6 * cl;
+    uint32_t w0 = lw[0], w1 = lw[1], w2 = lw[2], w4 = lw[4], w5 = lw[5];
+    const uint32_t j = tid & 63u, k = L6 + MW + j;
+    const bool onj = k < nout && ((chi >> j) & 1u);        // lane j holds slot k, and c_hi has it
+    const uint32_t ks = k < nout ? k : 0u;
+    uint64_t vq0 = E.oq[ks], vn0 = E.on[ks];
+    uint32_t vx0 = E.ox[ks], vp0 = E.opin[ks], vl0 = E.olam[ks];
+    const uint32_t wj = MW + j;
+    const uint32_t mj = wj < Mcap ? 4 * wj + 2 * ((uint32_t)(chi >> j) & 1u) : 0u;
+    float rh0 = E.rmid[mj], rl0 = E.rmid[mj + 1];
+    uint64_t oqw[MW > 0 ? MW : 1];
+#pragma unroll
+    for (uint32_t w = 0; w < MW; ++w) oqw[w] = cview(E.oq)[L6 + w];
+    const PQ_CONST uint32_t* pt = cview(E.pts) + 4 * (NP - 1u);
+    uint32_t rec[4 * NP];
+#pragma unroll
+    for (uint32_t i = 0; i < 4 * NP; ++i) rec[i] = pt[i];
+    const PQ_CONST uint32_t* pc = cview(E.pcls) + 5 * MW;
+    const uint32_t c1 = pc[1], c2 = pc[2], c3 = pc[3];
+    // every load above is in flight; from here on they are used (the empty
+    // asm statements keep the compiler from sinking a load into the
+    // branch that first uses it, which would cost one memory latency each)
+    asm volatile("" : "+v"(w0), "+v"(w1), "+v"(w2), "+v"(w4), "+v"(w5));
+    asm volatile("" : "+v"(vq0), "+v"(vn0), "+v"(vx0), "+v"(vp0), "+v"(vl0), "+v"(rh0), "+v"(rl0));
+    if constexpr (NP <= 8) {
+#pragma unroll
+        for (uint32_t i = 0; i < 4 * NP; ++i) asm volatile("" ::"s"(rec[i]));
+    }
+    asm volatile("" ::"s"(c1), "s"(c2), "s"(c3));
+    // ---- the wave's slots past the walk (one slot per lane), summed over the wave
+    const uint64_t vq = onj ? vq0 : 0ull, vn = onj ? vn0 : 0ull;
+    const uint32_t tK = onj ? vl0 + (((uint32_t)(Q.delta >> ks) & 1u) << 2) + (((uint32_t)__popcll(vq0 & cplace & ((1ull << ks) - 1ull)) & 1u) << 2) : 0u;
+    const uint32_t RK = wave_add(tK), Rhs = wave_xor(onj ? vx0 : 0u), Rpio = wave_xor(onj ? vp0 : 0u);
+    const uint32_t Rs0 = wave_xor((uint32_t)vn), Rs1 = wave_xor((uint32_t)(vn >> 32)), Rq = wave_xor((uint32_t)vq & lowm);
+    // magnitudes of the walkable slots past the walk, in slot order
+    const float rh = wj < Mcap ? rh0 : 1.f, rl = wj < Mcap ? rl0 : 0.f;
+    float fh = 1.f, fl = 0.f;
+#pragma unroll
+    for (uint32_t w = MW; w < kSegMaxMid; ++w) {
+        if (w < Mcap) {
+            float ph, pl;
+            df_mul(fh, fl, __int_as_float(__builtin_amdgcn_readlane(__float_as_int(rh), (int)(w - MW))),
+                   __int_as_float(__builtin_amdgcn_readlane(__float_as_int(rl), (int)(w - MW))), ph, pl);
+            fh = ph; fl = pl;
+        }
+    }
+    // ---- the lane's seed
+    const uint32_t K = Q.K0 + w0 + (((uint32_t)__popc((uint32_t)Q.delta & cl) & 1u) << 2) + RK + (((uint32_t)__popc(Rq & cl) & 1u) << 2);
+    const uint32_t hs = Q.hs0 ^ w1 ^ Rhs, pio = Q.pio0 ^ w2 ^ Rpio;
+    const uint64_t sv = Q.s0 ^ ((uint64_t)w4 | ((uint64_t)w5 << 32)) ^ ((uint64_t)Rs0 | ((uint64_t)Rs1 << 32));
+    const uint64_t c = (uint64_t)cl | cplace;
+    uint32_t P = 0;
+#pragma unroll
+    for (uint32_t w = 0; w < MW; ++w) P |= (uint32_t)(__popcll(oqw[w] & c) & 1) << w;
+    const uint32_t dw = (uint32_t)(Q.delta >> L6) & ((1u << MW) - 1u);
+    float ev[4] = {0.f, 0.f, 0.f, 0.f}, od[4] = {0.f, 0.f, 0.f, 0.f};
+    if (live) {
+        // ---- table reads and lane-factor byte lookups together
+        const uint32_t base = (pio << I) | hs;
+        GFloat4* tab = (GFloat4*)E.table;
+        f32x4 Tk[NP];
+#pragma unroll
+        for (uint32_t q = 0; q < NP; ++q) Tk[q] = tab[base ^ rec[4 * q]];
+        float2 lb[8];
+#pragma unroll
+        for (uint32_t b = 0; b < 8; ++b)
+            if (b < nbytes) lb[b] = E.lt[(b << 8) | (uint32_t)((sv >> (8 * b)) & 0xFFu)];
+        // a point of class cc (uniform) adds i^(cc>>1) times its term into
+        // EVu (cc even) or ODu (cc odd)
+        float eu[4] = {0.f, 0.f, 0.f, 0.f}, ou[4] = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+        for (uint32_t q = 0; q < NP; ++q) {
+            const uint32_t g = rec[4 * q + 1];
+            const uint32_t sg = (uint32_t)((__popc(P & g) ^ __popc(dw & g)) & 1) << 31;
+            const float wh = __uint_as_float(rec[4 * q + 2] ^ sg), wl = __uint_as_float(rec[4 * q + 3] ^ sg);
+            const uint32_t cc = (q >= c1 ? 1u : 0u) + (q >= c2 ? 1u : 0u) + (q >= c3 ? 1u : 0u);
+            float ph, pl, qh, ql;
+            df_mul(wh, wl, Tk[q].x, Tk[q].y, ph, pl);
+            df_mul(wh, wl, Tk[q].z, Tk[q].w, qh, ql);
+            const bool r = cc >= 2;       // times i: (x, y) -> (-y, x)
+            const float xh = r ? -qh : ph, xl = r ? -ql : pl, yh = r ? ph : qh, yl = r ? pl : ql;
+            if (cc & 1u) { df_add(ou[0], ou[1], xh, xl); df_add(ou[2], ou[3], yh, yl); }
+            else { df_add(eu[0], eu[1], xh, xl); df_add(eu[2], eu[3], yh, yl); }
+        }
+#pragma unroll
+        for (uint32_t b = 0; b < 8; ++b) {
+            if (b < nbytes) {
+                float ph, pl;
+                df_mul(fh, fl, lb[b].x, lb[b].y, ph, pl);
+                fh = ph; fl = pl;
+            }
+        }
+        seg_lane_total(eu, ou, K, fh, fl, ev, od);
+    }
+    seg_publish(s_red, ev, od, item, partial);
+}
+
 // One item (256 lanes of one query): reload the engine tables into LDS if
 // the workgroup's last item was of another engine (eng, uniform), seed every
 // lane, walk its segment, and write the item's four partial sums.
 template <bool SHORT>
 __device__ __forceinline__ void seg_item(const SegLds& S, const SegDev& E, const SegQ& Q, uint64_t item, int& eng, double* __restrict__ partial) {
+    if constexpr (SHORT) {
+        switch (Q.M) {
+        case 0: seg_short<0>(S.red, E, Q, item, partial); break;
+        case 1: seg_short<1>(S.red, E, Q, item, partial); break;
+        case 2: seg_short<2>(S.red, E, Q, item, partial); break;
+        case 3: seg_short<3>(S.red, E, Q, item, partial); break;
+        default: seg_short<kSegShortWalk>(S.red, E, Q, item, partial); break;
+        }
+        return;
+    }
     const uint32_t tid = threadIdx.x;
     float2* s_lt = S.lt;
     uint64_t *s_oq = S.oq, *s_on = S.on;
@@ -1024,43 +1223,15 @@ __device__ __forceinline__ void seg_item(const SegLds& S, const SegDev& E, const
     uint32_t* s_low = S.low;
     const SegSlots<LdsU64*, LdsU32*, LdsF32*> T{(LdsU64*)s_oq, (LdsU64*)s_on, (LdsU32*)s_ox, (LdsU32*)s_opin, (LdsU32*)s_olam, (LdsF32*)s_rmid,
                                                  (LdsU32*)s_low};
-    if constexpr (!SHORT) {
-        // once per engine per workgroup of a persistent launch: the plain copy
-        if ((int)Q.eng != eng) {
-            for (uint32_t k = tid; k < E.nbytes * 256u; k += 256) s_lt[k] = E.lt[k];
-            if (tid < E.nout) {
-                s_oq[tid] = E.oq[tid]; s_on[tid] = E.on[tid];
-                s_ox[tid] = E.ox[tid]; s_opin[tid] = E.opin[tid]; s_olam[tid] = E.olam[tid];
-            }
-            if (tid < 4 * E.Mcap) s_rmid[tid] = E.rmid[tid];
-            for (uint32_t k = tid; k < (6u << E.L6); k += 256) s_low[k] = E.lowtab[k];
-            __syncthreads();
-            eng = (int)Q.eng;
+    // once per engine per workgroup of a persistent launch
+    if ((int)Q.eng != eng) {
+        for (uint32_t k = tid; k < E.nbytes * 256u; k += 256) s_lt[k] = E.lt[k];
+        if (tid < E.nout) {
+            s_oq[tid] = E.oq[tid]; s_on[tid] = E.on[tid];
+            s_ox[tid] = E.ox[tid]; s_opin[tid] = E.opin[tid]; s_olam[tid] = E.olam[tid];
         }
-    } else if ((int)Q.eng != eng) {
-        // latency-bound launches: every load first, then every LDS write,
-        // one memory latency rather than one per table row
-        const uint32_t nlt = E.nbytes * 256u, nout = E.nout, nrm = 4 * E.Mcap;
-        float2 lv[8];
-#pragma unroll
-        for (uint32_t j = 0; j < 8; ++j)
-            if (tid + 256u * j < nlt) lv[j] = E.lt[tid + 256u * j];
-        uint64_t vq = 0, vn = 0;
-        uint32_t vx = 0, vp = 0, vl = 0;
-        float vr = 0.f;
-        if (tid < nout) { vq = E.oq[tid]; vn = E.on[tid]; vx = E.ox[tid]; vp = E.opin[tid]; vl = E.olam[tid]; }
-        if (tid < nrm) vr = E.rmid[tid];
-        const uint32_t nlow = 6u << E.L6;
-        uint32_t w0 = 0, w1 = 0;
-        if (tid < nlow) w0 = E.lowtab[tid];
-        if (tid + 256u < nlow) w1 = E.lowtab[tid + 256u];
-#pragma unroll
-        for (uint32_t j = 0; j < 8; ++j)
-            if (tid + 256u * j < nlt) s_lt[tid + 256u * j] = lv[j];
-        if (tid < nout) { s_oq[tid] = vq; s_on[tid] = vn; s_ox[tid] = vx; s_opin[tid] = vp; s_olam[tid] = vl; }
-        if (tid < nrm) s_rmid[tid] = vr;
-        if (tid < nlow) s_low[tid] = w0;
-        if (tid + 256u < nlow) s_low[tid + 256u] = w1;
+        if (tid < 4 * E.Mcap) s_rmid[tid] = E.rmid[tid];
+        for (uint32_t k = tid; k < (6u << E.L6); k += 256) s_low[k] = E.lowtab[k];
         __syncthreads();
         eng = (int)Q.eng;
     }
@@ -1080,48 +1251,7 @@ __device__ __forceinline__ void seg_item(const SegLds& S, const SegDev& E, const
         const PQ_CONST uint32_t* cl = cview(E.pcls) + 5 * M;
         // the segment's sums EVu and ODu (double-float re, im)
         float eu[4] = {0.f, 0.f, 0.f, 0.f}, ou[4] = {0.f, 0.f, 0.f, 0.f};
-        if constexpr (SHORT) {
-            // Short walk (at most 2^kSegShortWalk points): every table
-            // read is issued before the first sum, so the walk costs one
-            // memory latency.  A point of class c (uniform) adds i^(c>>1)
-            // times its term into EVu (c even) or ODu (c odd): one compact
-            // path per point, no per-class code.
-            constexpr int NP = 1 << kSegShortWalk;
-            const uint32_t np = 1u << M;
-            // The NP index words first, unconditionally (the record array is
-            // padded by NP records, so every read is in bounds), all waited
-            // for at once: otherwise each table read would wait for its own
-            // record word.
-            uint32_t Xk[NP];
-#pragma unroll
-            for (int k = 0; k < NP; ++k) Xk[k] = pt[4 * k];
-#pragma unroll
-            for (int k = 0; k < NP; ++k) asm volatile("" ::"s"(Xk[k]));
-            f32x4 Tk[NP];
-#pragma unroll
-            for (int k = 0; k < NP; ++k)
-                if ((uint32_t)k < np) Tk[k] = tab[base ^ Xk[k]];
-            const uint32_t c1 = cl[1], c2 = cl[2], c3 = cl[3];
-#pragma unroll
-            for (int k = 0; k < NP; ++k) {
-                if ((uint32_t)k < np) {
-                    const PQ_CONST uint32_t* e = pt + 4 * k;
-                    const uint32_t g = e[1];
-                    // lane and query sign of the point
-                    const uint32_t sg = (uint32_t)((__popc(P & g) ^ __popc(dw & g)) & 1) << 31;
-                    const float wh = __uint_as_float(e[2] ^ sg), wl = __uint_as_float(e[3] ^ sg);
-                    const uint32_t u = (uint32_t)k;
-                    const uint32_t c = (u >= c1 ? 1u : 0u) + (u >= c2 ? 1u : 0u) + (u >= c3 ? 1u : 0u);
-                    float ph, pl, qh, ql;
-                    df_mul(wh, wl, Tk[k].x, Tk[k].y, ph, pl);
-                    df_mul(wh, wl, Tk[k].z, Tk[k].w, qh, ql);
-                    const bool r = c >= 2;       // times i: (x, y) -> (-y, x)
-                    const float xh = r ? -qh : ph, xl = r ? -ql : pl, yh = r ? ph : qh, yl = r ? pl : ql;
-                    if (c & 1u) { df_add(ou[0], ou[1], xh, xl); df_add(ou[2], ou[3], yh, yl); }
-                    else { df_add(eu[0], eu[1], xh, xl); df_add(eu[2], eu[3], yh, yl); }
-                }
-            }
-        } else {
+        {
             float A[4][4];
 #pragma unroll
             for (int c = 0; c < 4; ++c) { A[c][0] = 0.f; A[c][1] = 0.f; A[c][2] = 0.f; A[c][3] = 0.f; }
@@ -1162,42 +1292,9 @@ __device__ __forceinline__ void seg_item(const SegLds& S, const SegDev& E, const
             seg_fold4<1>(A, ou);
         }
         // lane total = omega^K_seed F (EVu + omega ODu)
-        const uint32_t Ks = L.K & 7u;
-        float e2[4], o2[4];
-        if (Ks & 1u) {   // omega^(2a+1) (EVu + omega ODu) = i^a (i ODu + omega EVu)
-            e2[0] = -ou[2]; e2[1] = -ou[3]; e2[2] = ou[0]; e2[3] = ou[1];
-            o2[0] = eu[0]; o2[1] = eu[1]; o2[2] = eu[2]; o2[3] = eu[3];
-        } else {
-#pragma unroll
-            for (int c = 0; c < 4; ++c) { e2[c] = eu[c]; o2[c] = ou[c]; }
-        }
-        df_rot(e2, Ks >> 1);
-        df_rot(o2, Ks >> 1);
-#pragma unroll
-        for (int c = 0; c < 4; c += 2) {
-            df_mul(e2[c], e2[c + 1], fh, fl, ev[c], ev[c + 1]);
-            df_mul(o2[c], o2[c + 1], fh, fl, od[c], od[c + 1]);
-        }
+        seg_lane_total(eu, ou, L.K, fh, fl, ev, od);
     }
-    // workgroup reduction of the four double-float sums: each wave over DPP
-    // (total in lane 63), then the four waves through LDS
-    float wv[8] = {ev[0], ev[1], ev[2], ev[3], od[0], od[1], od[2], od[3]};
-    wave_sum_dpp(wv);
-    if ((tid & 63) == 63) {
-#pragma unroll
-        for (int c = 0; c < 8; ++c) s_red[tid >> 6][c] = wv[c];
-    }
-    __syncthreads();
-    // The item's four partial sums (EV re, im, OD re, im), one per lane
-    // of lanes 0..3: a single 32-byte write, into pinned host memory for
-    // small batches (the host finishes each query as its items arrive)
-    // or device memory before k_z8_reduce.  Each word is one aligned
-    // 8-byte store.
-    if (tid < 4) {
-        double v = 0.0;
-        for (int w = 0; w < 4; ++w) v += (double)s_red[w][2 * tid] + (double)s_red[w][2 * tid + 1];
-        partial[4 * item + tid] = v;
-    }
+    seg_publish(s_red, ev, od, item, partial);
 }
 
 // SHORT: every query of the launch walks at most 2^kSegShortWalk points
@@ -1232,6 +1329,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(SHORT ? 4 :
 constexpr uint32_t kArgQueries = 16, kArgEngines = 2;
 struct SegArgs {
     uint32_t qstart[kArgQueries];    // first item of each query, ascending; unused: ~0
+    uint32_t qeng;                   // bit q: engine of query q
+    uint32_t pad[3];
     SegDev eng[kArgEngines];
     SegQ q[kArgQueries];
 };
@@ -1245,9 +1344,15 @@ __global__ __launch_bounds__(256) void k_seg_walk_args(const SegArgs A, uint32_t
     uint32_t qi = 0;
 #pragma unroll
     for (uint32_t k = 1; k < kArgQueries; ++k) qi += item >= A.qstart[k] ? 1u : 0u;
-    const SegQ& Q = A.q[qi];
+    // the query and its engine descriptor, every field loaded at once (the
+    // engine index comes with the query starts, not from the query record)
+    const SegQ Q = A.q[qi];
+    const SegDev E = A.eng[(A.qeng >> qi) & 1u];
+    asm volatile("" ::"s"(Q.delta), "s"(Q.s0), "s"(Q.item0), "s"(Q.K0), "s"(Q.hs0), "s"(Q.pio0), "s"(Q.M), "s"(Q.H));
+    asm volatile("" ::"s"(E.oq), "s"(E.ox), "s"(E.opin), "s"(E.olam), "s"(E.on), "s"(E.table), "s"(E.lt), "s"(E.rmid));
+    asm volatile("" ::"s"(E.pts), "s"(E.pcls), "s"(E.lowtab), "s"(E.I), "s"(E.nout), "s"(E.L6), "s"(E.Mcap), "s"(E.nbytes));
     int eng = -1;
-    seg_item<SHORT>(lds, A.eng[Q.eng], Q, item, eng, partial);
+    seg_item<SHORT>(lds, E, Q, item, eng, partial);
 }
 
 // FP64 cross-workgroup reduction (fixed order) and global scaling per query.
@@ -1508,6 +1613,7 @@ uint32_t launch_args(hipStream_t st, const BatchPlan& B, double* part) {
     std::memcpy(A.eng, B.engs.data(), B.engs.size());
     std::memcpy(A.q, B.qs.data(), B.qs.size());
     for (uint32_t k = 0; k < kArgQueries; ++k) A.qstart[k] = k < B.nq() ? (uint32_t)A.q[k].item0 : ~0u;
+    for (uint32_t k = 0; k < B.nq(); ++k) A.qeng |= (A.q[k].eng & 1u) << k;
     uint32_t g = 0;
     for (int v = 0; v < 2; ++v) {
         const uint64_t items = B.it_hi[v] - B.it_lo[v];

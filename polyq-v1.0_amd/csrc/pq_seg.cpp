@@ -166,7 +166,9 @@ SegImage build_seg_image(const HostEngine& E) {
     auto to_table_basis = [&](uint32_t x) { uint32_t y = 0; for (uint32_t p = 0; p < I; ++p) if ((x >> p) & 1) y ^= S.hsmap[p]; return y; };
 
     // Outer slot data
-    S.oq.assign(nout, 0); S.ox.assign(nout, 0); S.opin.assign(nout, 0); S.olam.assign(nout, 0); S.on.assign(nout, 0);
+    // (at least one entry each, so a clamped index is always in bounds)
+    const uint32_t nslot = std::max<uint32_t>(nout, 1);
+    S.oq.assign(nslot, 0); S.ox.assign(nslot, 0); S.opin.assign(nslot, 0); S.olam.assign(nslot, 0); S.on.assign(nslot, 0);
     for (uint32_t k = 0; k < nout; ++k) {
         const uint32_t j = S.outer_j[k];
         for (uint32_t l = 0; l < nout; ++l) if (l != k && ((E.Q[j] >> S.outer_j[l]) & 1)) S.oq[k] |= 1ull << l;
