@@ -125,7 +125,7 @@ def headline(args, pq, W, np):
         tuned = tuple(int(v) for v in args.headline_tuning.split(","))
         pq.set_walk_tuning(*tuned)
     else:
-        tuned = pq.autotune([sim] * nq, xs, ys, repeats=7, cache=False)[:2]
+        tuned = pq.autotune([sim] * nq, xs, ys, repeats=7, rounds=5, cache=False)[:2]
     # the basis indices as uint64 arrays, the form the C ABI takes them in
     xs = np.asarray(xs, dtype=np.uint64)
     ys = np.asarray(ys, dtype=np.uint64)

@@ -1326,33 +1326,31 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(SHORT ? 4 :
 
 // A small batch passed whole as kernel arguments: no staging copy before the
 // launch and no dependent loads of query and engine records in the kernel.
-constexpr uint32_t kArgQueries = 16, kArgEngines = 2;
-struct SegArgs {
-    uint32_t qstart[kArgQueries];    // first item of each query, ascending; unused: ~0
-    uint32_t qeng;                   // bit q: engine of query q
-    uint32_t pad[3];
-    SegDev eng[kArgEngines];
-    SegQ q[kArgQueries];
+constexpr uint32_t kArgQueries = 16;
+struct SegArgQ {
+    SegQ q;
+    SegDev e;                        // the query's engine descriptor
 };
-// One workgroup per item, items item_lo.. of the batch (grid = item count).
+struct SegArgs {
+    SegArgQ a[kArgQueries];          // the queries of one launch, longest first
+};
+// One workgroup per (item, query): blockIdx.y is the query and blockIdx.x
+// the item within it (queries come longest first, so the grid's x extent is
+// the first query's item count; a shorter query leaves the rest of its row
+// idle).  The query record and its engine descriptor sit together at an
+// offset given by blockIdx.y alone: the kernel's only batch of argument
+// loads is issued as the wave starts.
 template <bool SHORT>
-__global__ __launch_bounds__(256) void k_seg_walk_args(const SegArgs A, uint32_t item_lo, double* __restrict__ partial) {
+__global__ __launch_bounds__(256) void k_seg_walk_args(const SegArgs A, double* __restrict__ partial) {
     PQ_SEG_LDS(lds);
-    const uint32_t item = item_lo + blockIdx.x;
-    // the item's query: how many later queries start at or before it (all
-    // starts read at once, no chain of dependent loads)
-    uint32_t qi = 0;
-#pragma unroll
-    for (uint32_t k = 1; k < kArgQueries; ++k) qi += item >= A.qstart[k] ? 1u : 0u;
-    // the query and its engine descriptor, every field loaded at once (the
-    // engine index comes with the query starts, not from the query record)
-    const SegQ Q = A.q[qi];
-    const SegDev E = A.eng[(A.qeng >> qi) & 1u];
-    asm volatile("" ::"s"(Q.delta), "s"(Q.s0), "s"(Q.item0), "s"(Q.K0), "s"(Q.hs0), "s"(Q.pio0), "s"(Q.M), "s"(Q.H));
+    const SegQ Q = A.a[blockIdx.y].q;
+    const SegDev E = A.a[blockIdx.y].e;
+    asm volatile("" ::"s"(Q.delta), "s"(Q.s0), "s"(Q.item0), "s"(Q.nitems), "s"(Q.K0), "s"(Q.hs0), "s"(Q.pio0), "s"(Q.M), "s"(Q.H));
     asm volatile("" ::"s"(E.oq), "s"(E.ox), "s"(E.opin), "s"(E.olam), "s"(E.on), "s"(E.table), "s"(E.lt), "s"(E.rmid));
     asm volatile("" ::"s"(E.pts), "s"(E.pcls), "s"(E.lowtab), "s"(E.I), "s"(E.nout), "s"(E.L6), "s"(E.Mcap), "s"(E.nbytes));
+    if (blockIdx.x >= Q.nitems) return;          // uniform: the whole workgroup
     int eng = -1;
-    seg_item<SHORT>(lds, E, Q, item, eng, partial);
+    seg_item<SHORT>(lds, E, Q, Q.item0 + blockIdx.x, eng, partial);
 }
 
 // FP64 cross-workgroup reduction (fixed order) and global scaling per query.
@@ -1605,23 +1603,32 @@ uint32_t launch_walks(Ctx& C, hipStream_t st, const BatchPlan& B, uint8_t* dbase
 
 // A host-finished segment batch small enough to travel as kernel arguments.
 bool fits_args(const BatchPlan& B) {
-    return B.walk == kWalkSeg && B.nq() <= kArgQueries && B.engs.size() <= kArgEngines * sizeof(SegDev);
+    return B.walk == kWalkSeg && B.nq() <= kArgQueries;
+}
+// The arguments of variant v of a kernel-argument batch and its grid:
+// (items of its first, longest query) x (its queries).
+void make_args(const BatchPlan& B, int v, SegArgs& A, uint32_t& gx, uint32_t& gy) {
+    std::memset(&A, 0, sizeof(A));
+    gy = B.q_hi[v] - B.q_lo[v];
+    gx = 0;
+    for (uint32_t k = 0; k < gy; ++k) {
+        SegArgQ& r = A.a[k];
+        std::memcpy(&r.q, B.qs.data() + 64 * (size_t)(B.q_lo[v] + k), sizeof(SegQ));
+        std::memcpy(&r.e, B.engs.data() + sizeof(SegDev) * (size_t)r.q.eng, sizeof(SegDev));
+        gx = std::max<uint32_t>(gx, (uint32_t)r.q.nitems);
+    }
 }
 uint32_t launch_args(hipStream_t st, const BatchPlan& B, double* part) {
     SegArgs A;
-    std::memset(&A, 0, sizeof(A));
-    std::memcpy(A.eng, B.engs.data(), B.engs.size());
-    std::memcpy(A.q, B.qs.data(), B.qs.size());
-    for (uint32_t k = 0; k < kArgQueries; ++k) A.qstart[k] = k < B.nq() ? (uint32_t)A.q[k].item0 : ~0u;
-    for (uint32_t k = 0; k < B.nq(); ++k) A.qeng |= (A.q[k].eng & 1u) << k;
     uint32_t g = 0;
     for (int v = 0; v < 2; ++v) {
-        const uint64_t items = B.it_hi[v] - B.it_lo[v];
-        if (items == 0) continue;
+        uint32_t gx, gy;
+        make_args(B, v, A, gx, gy);
+        if (gy == 0) continue;
         auto kern = v ? k_seg_walk_args<true> : k_seg_walk_args<false>;
-        hipLaunchKernelGGL(kern, dim3((uint32_t)items), dim3(256), 0, st, A, (uint32_t)B.it_lo[v], part);
+        hipLaunchKernelGGL(kern, dim3(gx, gy), dim3(256), 0, st, A, part);
         HIPCHECK(hipGetLastError());
-        g += (uint32_t)items;
+        g += gx * gy;
     }
     return g;
 }

@@ -431,12 +431,17 @@ _TUNE_GRID = {"z8": ((8, 10, 11, 12, 13, 14, 16), (0, 2, 4, 6, 8)),
               "branch_walk": ((23, 24, 25, 26, 27, 28, 29), (4, 6, 8))}
 
 
-def autotune(sims, inputs, outputs, form="z8", offsets=None, max_mids=None, repeats=3, cache=True, objective="wall"):
+def autotune(sims, inputs, outputs, form="z8", offsets=None, max_mids=None, repeats=3, cache=True, objective="wall", rounds=3):
     """Pick the lane walk split (offset, max_mid) of a batched form that
     minimises the time of this batch: its wall time per call (objective
     "wall", the default: host prologue, launch and wait included) or its walk
-    kernel time from HIP events ("kernel").  Applies it and, if cache, stores
-    it per device and form in ~/.cache/polyq_amd/walk_tuning.json.
+    kernel time from HIP events ("kernel").  Splits that give every engine
+    of the batch the same walk length are one configuration and are timed
+    once; the configurations are timed in `rounds` interleaved rounds of
+    `repeats` calls (after one untimed call each), so slow drifts of the host
+    do not favour one of them, and each is scored by its median.  Applies
+    the best and, if cache, stores it per device and form in
+    ~/.cache/polyq_amd/walk_tuning.json.
     Returns (offset, max_mid, {"offset/max_mid": ms})."""
     import json
     import time
@@ -444,27 +449,36 @@ def autotune(sims, inputs, outputs, form="z8", offsets=None, max_mids=None, repe
     max_mids = max_mids or _TUNE_GRID[form][1]
     xs = np.ascontiguousarray(inputs, dtype=np.uint64)
     ys = np.ascontiguousarray(outputs, dtype=np.uint64)
-    table = {}
-    best = None
+    distinct = list({id(s): s for s in sims}.values())
+    configs = {}                      # walk lengths per engine -> [(offset, max_mid), ...]
+    for off in offsets:
+        for mx in max_mids:
+            set_walk_tuning(off, mx, form)
+            key = tuple(s.seg_info()["mid_bits"] for s in distinct) if form == "z8" else (off, mx)
+            configs.setdefault(key, []).append((off, mx))
+    samples = {k: [] for k in configs}
     prev = get_timing()
     set_timing(objective == "kernel")
     try:
-        for off in offsets:
-            for mx in max_mids:
-                set_walk_tuning(off, mx, form)
-                batch_amplitudes(sims, xs, ys, form=form)     # first call of a split: not timed
-                ms = []
+        for _ in range(rounds):
+            for key, splits in configs.items():
+                set_walk_tuning(splits[0][0], splits[0][1], form)
+                batch_amplitudes(sims, xs, ys, form=form)     # untimed
                 for _ in range(repeats):
                     t0 = time.perf_counter()
                     batch_amplitudes(sims, xs, ys, form=form)
                     wall = 1e3 * (time.perf_counter() - t0)
-                    ms.append(last_timing()["kernel_ms"] if objective == "kernel" else wall)
-                t = sorted(ms)[len(ms) // 2]
-                table["%d/%d" % (off, mx)] = t
-                if best is None or t < best[0]:
-                    best = (t, off, mx)
+                    samples[key].append(last_timing()["kernel_ms"] if objective == "kernel" else wall)
     finally:
         set_timing(prev)
+    table = {}
+    best = None
+    for key, splits in configs.items():
+        t = float(np.median(samples[key]))
+        for off, mx in splits:
+            table["%d/%d" % (off, mx)] = t
+        if best is None or t < best[0]:
+            best = (t, splits[0][0], splits[0][1])
     set_walk_tuning(best[1], best[2], form)
     if cache:
         try:
