@@ -50,6 +50,12 @@ def build(jobs=8, verbose=False):
     tmp = OUT + ".tmp"
     _run(["hipcc", "--offload-arch=gfx950", "-shared", "-fPIC", "-pthread", "-o", tmp] + objs)
     os.replace(tmp, OUT)
+    # the CPython fast path of Simulator.amplitudes, linked against the library
+    import sysconfig
+    fast = os.path.join(HERE, "polyq_amd", "_pqfast" + sysconfig.get_config_var("EXT_SUFFIX"))
+    _run(["gcc", "-O2", "-shared", "-fPIC", "-I", sysconfig.get_paths()["include"], "-I", os.path.join(HERE, "..", "include"),
+          os.path.join(CSRC, "pq_pyfast.c"), "-o", fast + ".tmp", "-L", os.path.dirname(OUT), "-lpolyq_amd", "-Wl,-rpath,$ORIGIN"])
+    os.replace(fast + ".tmp", fast)
     return OUT
 
 

@@ -21,6 +21,13 @@ _LIB_PATH = os.path.join(_HERE, "libpolyq_amd.so")
 if not os.path.exists(_LIB_PATH):
     raise ImportError("libpolyq_amd.so is not built: run polyq-v1.0_amd/build_lib.py (or __graft_entry__.build())")
 lib = ctypes.CDLL(_LIB_PATH)
+try:
+    # CPython fast path of Simulator.amplitudes (buffer protocol, no ctypes
+    # argument conversion); built beside the library
+    from . import _pqfast
+except ImportError:
+    _pqfast = None
+_U64 = np.dtype(np.uint64)
 
 _u8p = ctypes.POINTER(ctypes.c_uint8)
 _u32p = ctypes.POINTER(ctypes.c_uint32)
@@ -223,6 +230,7 @@ class Simulator:
         h = _sim_p()
         _check(lib.pq_sim_create(circuit.num_qubits, len(ops), _ptr(ops, _u8p), _ptr(q0, _u32p), _ptr(q1, _u32p), _ptr(ph, _u32p), st, pr, ctypes.byref(h)))
         self._h = h
+        self._hv = h.value
         self.num_qubits = lib.pq_sim_num_qubits(h)
 
     def __del__(self):
@@ -230,6 +238,7 @@ class Simulator:
         if h:
             lib.pq_sim_destroy(h)
             self._h = None
+            self._hv = 0
 
     @property
     def handle(self):
@@ -260,6 +269,12 @@ class Simulator:
         n = len(inputs)
         if len(outputs) != n:
             raise ValueError("inputs and outputs differ in length")
+        if (_pqfast is not None and type(inputs) is np.ndarray and type(outputs) is np.ndarray and inputs.dtype == _U64
+                and outputs.dtype == _U64 and inputs.ndim == 1 and outputs.ndim == 1
+                and inputs.flags.c_contiguous and outputs.flags.c_contiguous):
+            out = np.empty(n, dtype=np.complex128)
+            _check(_pqfast.amplitudes(self._hv, inputs, outputs, out, FORMS[form or self.form]))
+            return out
         buf, out = _complex_out(n)
         _check(lib.pq_sim_amplitudes(self._h, n, _u64_arg(inputs), _u64_arg(outputs), self._form(form), buf))
         return out

@@ -56,3 +56,18 @@ def test_argument_batch_of_many_engines_and_both_walk_variants(restore_tuning):
     np.testing.assert_array_equal(a, _staged(S, X, Y))
     alone = np.array([S[i].amplitudes([X[i]], [Y[i]])[0] for i in range(16)])
     np.testing.assert_array_equal(a, alone)
+
+
+def test_buffer_protocol_call_equals_ctypes_call():
+    # Simulator.amplitudes passes uint64 arrays through the CPython fast path
+    # and anything else through ctypes: the same C ABI call either way
+    assert pq._pqfast is not None, "the CPython fast path was not built"
+    c, xs, ys = W.config2(5)
+    sim = pq.Simulator(c, strategy="cached")
+    X = np.asarray(xs, np.uint64)
+    Y = np.asarray(ys, np.uint64)
+    a = sim.amplitudes(X, Y)
+    b = sim.amplitudes([int(x) for x in xs], [int(y) for y in ys])
+    np.testing.assert_array_equal(a, b)
+    with pytest.raises(ValueError):
+        sim.amplitudes(X, Y[:3])
