@@ -37,42 +37,56 @@ NormalForm normal_form(const std::vector<Bits>& b, const Bits& parity) {
     Bits alive(m);
     for (uint32_t i = 0; i < m; ++i) alive.set(i);
 
-    typedef std::pair<uint32_t, uint32_t> Key;  // (weight, row)
-    std::priority_queue<Key, std::vector<Key>, std::greater<Key>> hd, ho;  // diagonal / off-diagonal classes
+    // Rows by class (1: diagonal entry set, 2: off-diagonal only) and weight
+    // (alive neighbours), as bucket bitsets: the next pivot of a class is the
+    // lowest row of its lightest non-empty bucket, the (weight, row) order of
+    // a min-heap, with O(1) updates and no stale entries.
+    struct Buckets {
+        std::vector<Bits> rows;          // per weight
+        std::vector<uint32_t> cnt;
+        uint32_t lo = 0;                 // no non-empty bucket below lo
+        explicit Buckets(uint32_t m) : rows(m + 1, Bits(m)), cnt(m + 1, 0), lo(m + 1) {}
+        void add(uint32_t w, uint32_t a) { rows[w].set(a); cnt[w]++; if (w < lo) lo = w; }
+        void del(uint32_t w, uint32_t a) { rows[w].put(a, false); cnt[w]--; }
+        long top() {
+            for (; lo < cnt.size(); ++lo) if (cnt[lo]) return rows[lo].lowest();
+            return -1;
+        }
+    };
+    Buckets bd(m), bo(m);              // diagonal / off-diagonal classes
     std::vector<uint8_t> cls(m, 0);
     std::vector<uint32_t> wt(m, 0);
     auto reclass = [&](uint32_t a) {
+        if (cls[a] == 1) bd.del(wt[a], a);
+        else if (cls[a] == 2) bo.del(wt[a], a);
         if (!alive.get(a)) { cls[a] = 0; return; }
         uint32_t w = gram[a].and_popc(alive);
-        if (gram[a].get(a)) { cls[a] = 1; wt[a] = w; hd.push({w, a}); }
-        else if (w > 0) { cls[a] = 2; wt[a] = w; ho.push({w, a}); }
+        if (gram[a].get(a)) { cls[a] = 1; wt[a] = w; bd.add(w, a); }
+        else if (w > 0) { cls[a] = 2; wt[a] = w; bo.add(w, a); }
         else cls[a] = 0;
     };
-    auto pop_current = [&](std::priority_queue<Key, std::vector<Key>, std::greater<Key>>& h, uint8_t want) -> long {
-        while (!h.empty()) {
-            Key k = h.top();
-            if (cls[k.second] == want && wt[k.second] == k.first && alive.get(k.second)) return (long)k.second;
-            h.pop();
-        }
-        return -1;
+    auto unclass = [&](uint32_t a) {
+        if (cls[a] == 1) bd.del(wt[a], a);
+        else if (cls[a] == 2) bo.del(wt[a], a);
+        cls[a] = 0;
     };
     for (uint32_t a = 0; a < m; ++a) reclass(a);
 
     NormalForm nf; nf.m = m;
     for (;;) {
-        long a = pop_current(hd, 1);
+        long a = bd.top();
         if (a >= 0) {
             // 1x1 pivot: A'_bc = A_bc + A_ba A_ac, p_b += p_a for neighbours b
             Bits s = gram[a]; s &= alive; s.put((uint32_t)a, false);
             const Bits pa = basis[a];
             s.each([&](uint32_t k) { gram[k] ^= s; basis[k] ^= pa; });
+            unclass((uint32_t)a);
             alive.put((uint32_t)a, false);
             nf.odd.push_back(pa);
-            cls[a] = 0;
             s.each([&](uint32_t k) { reclass(k); });
             continue;
         }
-        a = pop_current(ho, 2);
+        a = bo.top();
         if (a < 0) break;
         // 2x2 (hyperbolic) pivot on a and its first alive neighbour
         Bits ra = gram[a]; ra &= alive;
@@ -86,9 +100,9 @@ NormalForm normal_form(const std::vector<Bits>& b, const Bits& parity) {
             if (sa.get(k)) { gram[k] ^= sb; basis[k] ^= pb; }
             if (sb.get(k)) { gram[k] ^= sa; basis[k] ^= pa; }
         });
+        unclass((uint32_t)a); unclass(bb);
         alive.put((uint32_t)a, false); alive.put(bb, false);
         nf.pa.push_back(pa); nf.pb.push_back(pb);
-        cls[a] = 0; cls[bb] = 0;
         touched.each([&](uint32_t k) { reclass(k); });
     }
     alive.each([&](uint32_t r) { nf.even.push_back(basis[r]); });
@@ -97,10 +111,22 @@ NormalForm normal_form(const std::vector<Bits>& b, const Bits& parity) {
 
 // d(p) = q(p) mod 4 for a basis vector (normal_form.rs:339-348)
 static uint32_t z4_of(const std::vector<Bits>& b, const Bits& v0, const Bits& v1, const Bits& p) {
+    // linear part plus 2 per edge of b inside p; b is symmetric with a zero
+    // diagonal, so the edges are counted once each as the pairs (i < j):
+    // row j against the bits of p below j
     uint32_t lin = p.and_popc(v0) + 2 * p.and_popc(v1);
-    uint32_t t = 0;
-    p.each([&](uint32_t j) { t += b[j].and_popc(p); });
-    return (lin + 2 * ((t / 2) & 1)) & 3;
+    uint32_t edges = 0;
+    const uint64_t* pw = p.w.data();
+    const size_t W = p.w.size();
+    for (size_t k = 0; k < W; ++k) {
+        for (uint64_t x = pw[k]; x; x &= x - 1) {
+            const uint32_t j = (uint32_t)(k * 64 + __builtin_ctzll(x));
+            const uint64_t* r = b[j].w.data();
+            for (size_t q = 0; q < k; ++q) edges += (uint32_t)__builtin_popcountll(r[q] & pw[q]);
+            edges += (uint32_t)__builtin_popcountll(r[k] & pw[k] & ((1ull << (j & 63)) - 1ull));
+        }
+    }
+    return (lin + 2 * (edges & 1)) & 3;
 }
 
 static inline bool row_parity(const uint64_t* x, const uint64_t* y, uint32_t nw) {
