@@ -366,7 +366,7 @@ def serve(args, pq, W, np):
     rng = W.Rng(77)
     rate, secs = args.serve_rate, args.serve_seconds
     # Pre-generate the schedule so the submitting thread only sleeps and submits.
-    t, sched, circuits = 0.0, [], []
+    t, sched, circuits, fresh = 0.0, [], [], []
     while True:
         t += rnd.expovariate(rate)
         if t >= secs:
@@ -374,8 +374,10 @@ def serve(args, pq, W, np):
         if not circuits or rnd.random() < 0.5:
             c = W.config4_circuit(rng)
             circuits.append(c)
+            fresh.append(True)
         else:
             c = circuits[rnd.randrange(len(circuits))]
+            fresh.append(False)
         u = rnd.random()
         n = c.num_qubits
         x, y = rnd.getrandbits(n), rnd.getrandbits(n)
@@ -394,11 +396,12 @@ def serve(args, pq, W, np):
             time.sleep(ta - now)
         tickets.append(srv.submit(c, kind, x, y, meas))
     t_sub = time.perf_counter() - t0
-    lat, vals, failed, errors = [], [], 0, []
+    lat, vals, failed, errors, lat_new, lat_rep = [], [], 0, [], [], []
     for i, tk in enumerate(tickets):
         try:
             v, l = srv.wait(tk, timeout=120.0)
             lat.append(l); vals.append(v)
+            (lat_new if fresh[i] else lat_rep).append(l)
         except pq.PolyqError as e:
             failed += 1; vals.append(None)
             if len(errors) < 5:
@@ -434,6 +437,11 @@ def serve(args, pq, W, np):
         "p50_ms": float(np.percentile(lat, 50)) if len(lat) else None,
         "p99_ms": float(np.percentile(lat, 99)) if len(lat) else None,
         "max_ms": float(lat.max()) if len(lat) else None,
+        # first request of a circuit (host build and upload included) / repeats of resident circuits
+        "new_circuit_p50_ms": float(np.percentile(lat_new, 50)) if lat_new else None,
+        "new_circuit_p99_ms": float(np.percentile(lat_new, 99)) if lat_new else None,
+        "repeat_p50_ms": float(np.percentile(lat_rep, 50)) if lat_rep else None,
+        "repeat_p99_ms": float(np.percentile(lat_rep, 99)) if lat_rep else None,
         "throughput_req_per_s": len(lat) / t_end, "submit_span_s": t_sub, "drain_s": t_end - t_sub,
         "batches": st["batches"], "overlapped_batches": st["overlapped_batches"], "max_batch": st["max_batch"],
         "builds": st["builds"], "evictions": st["evictions"], "resident_bytes": st["resident_bytes"],

@@ -83,6 +83,7 @@ struct Server {
     std::vector<std::thread> workers;
     std::thread dispatcher;
     bool stop = false;
+    std::atomic<size_t> nready{0};   // requests in ready_q (read by the dispatcher's poll without the lock)
     uint64_t next_ticket = 1, use_clock = 0;
     // statistics
     uint64_t completed = 0, failed = 0, batches = 0, evictions = 0, builds = 0, max_batch = 0, overlapped = 0;
@@ -180,6 +181,15 @@ struct Server {
         if (r.entry) { r.entry->pinned--; r.entry.reset(); }
     }
 
+    // Walk length and size check of a request's device jobs.
+    void finish_jobs(Req& r) {
+        for (auto& J : r.jobs) {
+            const uint32_t d = dev_z8(*J.eng).d;
+            if (d > 40) throw std::runtime_error("branch dimension above 40: request rejected");
+            J.mid_bits = dev_mid_bits(*J.eng, walk);
+        }
+    }
+
     void worker() {
         for (;;) {
             std::shared_ptr<Req> r;
@@ -206,13 +216,10 @@ struct Server {
                     S.eng->ensure_device();
                     add_query_jobs_xy(*S.eng, r->x, r->y, walk, r->jobs, owner, 0);
                 }
-                for (auto& J : r->jobs) {
-                    const uint32_t d = dev_z8(*J.eng).d;
-                    if (d > 40) throw std::runtime_error("branch dimension above 40: request rejected");
-                    J.mid_bits = dev_mid_bits(*J.eng, walk);
-                }
+                finish_jobs(*r);
                 std::lock_guard<std::mutex> lk(mu);
                 ready_q.push_back(r);
+                nready++;
             } catch (std::exception& e) {
                 std::lock_guard<std::mutex> lk(mu);
                 r->status = -1; r->err = e.what(); r->t_done = now_s();
@@ -262,7 +269,10 @@ struct Server {
                 std::unique_lock<std::mutex> lk(mu);
                 if (inflight == 0) cv_ready.wait(lk, [&] { return stop || !ready_q.empty(); });
                 if (stop && ready_q.empty() && inflight == 0) return;
-                if (inflight < 2) while (!ready_q.empty()) { batch.push_back(ready_q.front()); ready_q.pop_front(); }
+                if (inflight < 2) {
+                    while (!ready_q.empty()) { batch.push_back(ready_q.front()); ready_q.pop_front(); }
+                    nready -= batch.size();
+                }
             }
             if (!batch.empty()) {
                 if (!slots[0]) {
@@ -289,6 +299,17 @@ struct Server {
                 const int s = order[0];
                 bool done = order[1] >= 0 || dev_z8_ready(*slots[s]);
                 if (!done) {
+                    // poll the batch without sleeping for up to 2 ms (a
+                    // batch of a few requests takes tens of microseconds),
+                    // going back to launch as soon as a request is ready;
+                    // after that, wait in 200 us steps
+                    const double t0 = now_s();
+                    while (!(done = dev_z8_ready(*slots[s]))) {
+                        if (nready.load(std::memory_order_acquire) > 0 || now_s() - t0 > 2e-3) break;
+                        std::this_thread::yield();
+                    }
+                }
+                if (!done) {
                     std::unique_lock<std::mutex> lk(mu);
                     if (ready_q.empty() && !stop) cv_ready.wait_for(lk, std::chrono::microseconds(200));
                     continue;
@@ -310,6 +331,11 @@ struct Server {
         r->mq.assign(mq, mq + k); r->mo.assign(mo, mo + k);
         r->t_submit = now_s();
         const uint64_t key = circuit_key(c, strategy);
+        // An amplitude or probability request on a circuit whose engines are
+        // built and on the device takes its prologue (about 0.2 us) here, on
+        // the submitting thread, and goes straight to the dispatcher; the
+        // rest go through a worker (host build, upload, marginal engines).
+        bool here = false;
         {
             std::lock_guard<std::mutex> lk(mu);
             auto it = entries.find(key);
@@ -321,9 +347,40 @@ struct Server {
             r->entry = it->second;
             r->ticket = next_ticket++;
             reqs[r->ticket] = r;
-            work_q.push_back(r);
+            Entry& E = *r->entry;
+            // try_lock: the build lock is taken before mu elsewhere
+            if (kind != 2 && E.built && E.err.empty() && E.build_mu.try_lock()) {
+                const RankEng& R = *E.sim->eng;
+                here = R.dev.size() == R.host.size();
+                E.build_mu.unlock();
+                if (here) { E.pinned++; E.last_use = ++use_clock; }
+            }
+            if (!here) work_q.push_back(r);
         }
-        cv_work.notify_one();
+        if (!here) {
+            cv_work.notify_one();
+            return r->ticket;
+        }
+        try {
+            std::vector<size_t> owner;
+            add_query_jobs_xy(*r->entry->sim->eng, r->x, r->y, walk, r->jobs, owner, 0);
+            finish_jobs(*r);
+        } catch (std::exception& e) {
+            std::lock_guard<std::mutex> lk(mu);
+            r->status = -1; r->err = e.what(); r->t_done = now_s();
+            r->jobs.clear();
+            unpin(*r);
+            failed++;
+            cv_done.notify_all();
+            cv_mem.notify_all();
+            return r->ticket;
+        }
+        {
+            std::lock_guard<std::mutex> lk(mu);
+            ready_q.push_back(r);
+            nready++;
+        }
+        cv_ready.notify_one();
         return r->ticket;
     }
 
