@@ -179,7 +179,7 @@ int pq_sim_walk_dump(pq_sim* sim, uint64_t x, uint64_t y, uint32_t mid_bits, siz
 int pq_sim_selfcheck(pq_sim* sim, uint64_t x, uint64_t y, uint32_t trials, uint64_t seed, uint64_t* bad);
 /* Host-only check of the direct boundary-to-prologue path used by the batched
  * forms against the (variable, value) assignment path (pathsum.rs:430-450,
- * engine.rs:569-610): same zero/contradiction verdict, phase, s0, shift row,
+ * engine.rs:569-610): same zero/contradiction outcome, phase, s0, shift row,
  * Z8 data and scale for each query and branch engine.  *bad counts mismatches. */
 int pq_sim_prologue_selfcheck(pq_sim* sim, size_t n, const uint64_t* xs, const uint64_t* ys, uint64_t* bad);
 /* Host-only: stats of the marginal (sandwich) engine for a measured-qubit set,
