@@ -52,6 +52,17 @@ SEED = 5
 SMALL = 2.0 ** -20
 
 
+def _cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return None
+
+
 def _oracle():
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     import oracle_lib as O
@@ -215,7 +226,7 @@ def headline(args, pq, W, np):
             "sample": "%d full amplitudes (%d x 2^%d branches, 2^14-branch Gray chunks, OpenMP dynamic) of the same workload"
                       % (amps, amps, d),
             "amplitudes_per_s": amps / done_sec, "max_rel_err_z8_vs_cpu": cpu_err,
-            "engine_new_s": osim.build_seconds,
+            "engine_new_s": osim.build_seconds, "host_nproc": os.cpu_count(), "host_cpu": _cpu_model(),
         }
         # the oracle's own best strategy: Auto (pair decomposition by the
         # cost model, branch engines built and evaluated in parallel); eval
