@@ -661,6 +661,7 @@ int pq_sim_selfcheck(pq_sim* h, uint64_t x, uint64_t y, uint32_t trials, uint64_
             const HostEngine& E = *h->s.eng->host[b];
             Prepared P;
             if (!prepare_assignment(E, asg, P)) continue;
+            prepared_ell(E, P);
             const uint32_t W = E.L.words();
             uint64_t st = seed + 0x9E3779B97F4A7C15ull * (b + 1);
             for (uint32_t tr = 0; tr < trials; ++tr) {
@@ -708,7 +709,7 @@ int pq_sim_prologue_selfcheck(pq_sim* h, size_t n, const uint64_t* xs, const uin
                 const bool na = prepare_assignment(*R.host[b], asg, A);
                 if (contra || na != nb) { (*bad)++; continue; }
                 if (!na) continue;
-                if (A.phase != B.phase || A.s0 != B.s0 || A.K0 != B.K0 || A.u_start != B.u_start || A.ell != B.ell || A.scale != B.scale) (*bad)++;
+                if (A.phase != B.phase || A.s0 != B.s0 || A.K0 != B.K0 || A.u_start != B.u_start || A.dq != B.dq || A.scale != B.scale) (*bad)++;
             }
         }
         return 0;

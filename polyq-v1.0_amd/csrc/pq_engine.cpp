@@ -501,13 +501,17 @@ static bool prepare_ones(const HostEngine& E, const uint64_t* __restrict__ on, P
         dq ^= E.mcol_dq[k];
     }
     P.K0 = total_phase(E, us, s0);
-    // ell_j = lam_j + 4 (parity(u_start . dual_j) ^ parity(s0 . kernel_j))
+    // ell_j = lam_j + 4 (parity(u_start . dual_j) ^ parity(s0 . kernel_j)):
+    // the query part is the mask dq (prepared_ell spells out ell)
     P.dq = dq;
-    P.ell.resize(E.d);
-    for (uint32_t j = 0; j < E.d; ++j) P.ell[j] = (uint8_t)((E.lam[j] + (uint32_t)((dq >> j) & 1) * 4) & 7);
     P.scale = phase_complex(P.phase) * E.Ephase * E.mag;
     P.nonzero = true;
     return true;
+}
+
+void prepared_ell(const HostEngine& E, Prepared& P) {
+    P.ell.resize(E.d);
+    for (uint32_t j = 0; j < E.d; ++j) P.ell[j] = (uint8_t)((E.lam[j] + (uint32_t)((P.dq >> j) & 1) * 4) & 7);
 }
 
 std::complex<double> plain_c_start(const HostEngine& E, uint64_t s0) {

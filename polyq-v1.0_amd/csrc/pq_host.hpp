@@ -185,7 +185,7 @@ struct Prepared {
     std::vector<uint64_t> u_start;   // packed shift row at c = 0
     std::vector<uint64_t> u0;        // shift row of the boundary alone (s = 0)
     uint32_t K0 = 0;                 // total Z8 phase at c = 0
-    std::vector<uint8_t> ell;        // d linear Z8 coefficients
+    std::vector<uint8_t> ell;        // d linear Z8 coefficients (filled by prepared_ell only)
     uint64_t dq = 0;                 // bit j: ell[j] - lam[j] = 4 (the query-dependent part)
     std::complex<double> scale;      // e^{i phase} 2^{log2mag - h/2} E B
 };
@@ -194,6 +194,9 @@ bool prepare(const HostEngine& E, const uint8_t* values, Prepared& P);
 // term multipliers a1[k] for k in s0, a0[k] otherwise, in term order); only
 // the plain FP64 form uses it.
 std::complex<double> plain_c_start(const HostEngine& E, uint64_t s0);
+// The d linear Z8 coefficients of a prepared query, ell_j = lam_j + 4 dq_j
+// (the prologue itself keeps only the mask dq).
+void prepared_ell(const HostEngine& E, Prepared& P);
 // Boundary of <y|C|x> mapped straight to the engine's fixed positions
 // (pathsum.rs:430-450): input qubit q fixes position in_pos[q] to x_q; output
 // qubit q fixes out_pos[q] to y_q ^ out_neg[q], or, when its variable already
