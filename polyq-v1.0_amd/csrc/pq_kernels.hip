@@ -998,7 +998,7 @@ struct SegLds {
     uint64_t *oq, *on;       // 64 each
     uint32_t *ox, *opin, *olam;
     float* rmid;             // 4 kSegMaxMid
-    float (*red)[8];         // 4 waves
+    double (*red)[4];        // 4 waves x (EV re, im, OD re, im)
     uint32_t* low;           // 6 x 64: the lane-low table
 };
 #define PQ_SEG_LDS(name)                                                   \
@@ -1006,7 +1006,7 @@ struct SegLds {
     __shared__ uint64_t name##_oq[64], name##_on[64];                      \
     __shared__ uint32_t name##_ox[64], name##_opin[64], name##_olam[64];   \
     __shared__ float name##_rmid[4 * kSegMaxMid];                          \
-    __shared__ float name##_red[4][8];                                     \
+    __shared__ double name##_red[4][4];                                    \
     __shared__ uint32_t name##_low[6 * 64];                                \
     const SegLds name{name##_lt, name##_oq, name##_on, name##_ox, name##_opin, name##_olam, name##_rmid, name##_red, name##_low}
 
@@ -1030,25 +1030,45 @@ __device__ __forceinline__ void seg_lane_total(const float (&eu)[4], const float
         df_mul(o2[c], o2[c + 1], fh, fl, od[c], od[c + 1]);
     }
 }
-// Workgroup reduction of the four double-float sums: each wave over DPP
-// (total in lane 63), then the four waves through LDS.  The item's four
-// partial sums (EV re, im, OD re, im), one per lane of lanes 0..3: a single
-// 32-byte write, into pinned host memory for small batches (the host
-// finishes each query as its items arrive) or device memory before
+// A double's two halves moved by one DPP pattern (rows the mask leaves out
+// read 0.0).
+template <int CTRL, int ROWS>
+__device__ __forceinline__ double dpp_f64(double x) {
+    const uint64_t u = (uint64_t)__double_as_longlong(x);
+    const uint32_t lo = (uint32_t)__builtin_amdgcn_update_dpp(0, (int)(uint32_t)u, CTRL, ROWS, 0xF, false);
+    const uint32_t hi = (uint32_t)__builtin_amdgcn_update_dpp(0, (int)(uint32_t)(u >> 32), CTRL, ROWS, 0xF, false);
+    return __longlong_as_double((long long)(((uint64_t)hi << 32) | lo));
+}
+template <int CTRL, int ROWS>
+__device__ __forceinline__ void dpp_add4(double (&v)[4]) {
+#pragma unroll
+    for (int i = 0; i < 4; ++i) v[i] += dpp_f64<CTRL, ROWS>(v[i]);
+}
+// Workgroup reduction of the four sums in FP64: a lane's double-float pair
+// hi + lo is exact in FP64; each wave adds over DPP in the fixed order of
+// wave_sum_dpp (total in lane 63), then the four waves through LDS.  The
+// item's four partial sums (EV re, im, OD re, im), one per lane of lanes
+// 0..3: a single 32-byte write, into pinned host memory for small batches
+// (the host finishes each query as its items arrive) or device memory before
 // k_z8_reduce.  Each word is one aligned 8-byte store.
-__device__ __forceinline__ void seg_publish(float (*s_red)[8], const float (&ev)[4], const float (&od)[4], uint64_t item,
+__device__ __forceinline__ void seg_publish(double (*s_red)[4], const float (&ev)[4], const float (&od)[4], uint64_t item,
                                             double* __restrict__ partial) {
     const uint32_t tid = threadIdx.x;
-    float wv[8] = {ev[0], ev[1], ev[2], ev[3], od[0], od[1], od[2], od[3]};
-    wave_sum_dpp(wv);
+    double wv[4] = {(double)ev[0] + (double)ev[1], (double)ev[2] + (double)ev[3], (double)od[0] + (double)od[1], (double)od[2] + (double)od[3]};
+    dpp_add4<0xB1, 0xF>(wv);     // quad_perm [1,0,3,2]
+    dpp_add4<0x4E, 0xF>(wv);     // quad_perm [2,3,0,1]
+    dpp_add4<0x141, 0xF>(wv);    // row_half_mirror
+    dpp_add4<0x140, 0xF>(wv);    // row_mirror
+    dpp_add4<0x142, 0xA>(wv);    // row_bcast:15
+    dpp_add4<0x143, 0xC>(wv);    // row_bcast:31
     if ((tid & 63) == 63) {
 #pragma unroll
-        for (int c = 0; c < 8; ++c) s_red[tid >> 6][c] = wv[c];
+        for (int c = 0; c < 4; ++c) s_red[tid >> 6][c] = wv[c];
     }
     __syncthreads();
     if (tid < 4) {
         double v = 0.0;
-        for (int w = 0; w < 4; ++w) v += (double)s_red[w][2 * tid] + (double)s_red[w][2 * tid + 1];
+        for (int w = 0; w < 4; ++w) v += s_red[w][tid];
         partial[4 * item + tid] = v;
     }
 }
@@ -1091,7 +1111,7 @@ __device__ __forceinline__ uint32_t wave_add(uint32_t v) {
 // The results equal the slot-by-slot seed bit for bit (integers mod 8 and
 // XORs), and the magnitudes multiply in the same slot order.
 template <uint32_t MW>
-__device__ __forceinline__ void seg_short(float (*s_red)[8], const SegDev& E, const SegQ& Q, uint64_t item, double* __restrict__ partial) {
+__device__ __forceinline__ void seg_short(double (*s_red)[4], const SegDev& E, const SegQ& Q, uint64_t item, double* __restrict__ partial) {
     constexpr uint32_t NP = 1u << MW;
     const uint32_t tid = threadIdx.x;
     const uint32_t L6 = E.L6, nout = E.nout, I = E.I, nbytes = E.nbytes, Mcap = E.Mcap;
@@ -1219,7 +1239,7 @@ __device__ __forceinline__ void seg_item(const SegLds& S, const SegDev& E, const
     uint64_t *s_oq = S.oq, *s_on = S.on;
     uint32_t *s_ox = S.ox, *s_opin = S.opin, *s_olam = S.olam;
     float* s_rmid = S.rmid;
-    float (*s_red)[8] = S.red;
+    double (*s_red)[4] = S.red;
     uint32_t* s_low = S.low;
     const SegSlots<LdsU64*, LdsU32*, LdsF32*> T{(LdsU64*)s_oq, (LdsU64*)s_on, (LdsU32*)s_ox, (LdsU32*)s_opin, (LdsU32*)s_olam, (LdsF32*)s_rmid,
                                                  (LdsU32*)s_low};
