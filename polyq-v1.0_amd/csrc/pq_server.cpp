@@ -348,10 +348,14 @@ struct Server {
             r->ticket = next_ticket++;
             reqs[r->ticket] = r;
             Entry& E = *r->entry;
-            // try_lock: the build lock is taken before mu elsewhere
-            if (kind != 2 && E.built && E.err.empty() && E.build_mu.try_lock()) {
-                const RankEng& R = *E.sim->eng;
-                here = R.dev.size() == R.host.size();
+            // try_lock: the build lock is taken before mu elsewhere.  The
+            // build state and the device images are read under it; a pinned
+            // entry is never evicted, so the images stay after it is released.
+            if (kind != 2 && E.build_mu.try_lock()) {
+                if (E.built && E.err.empty()) {
+                    const RankEng& R = *E.sim->eng;
+                    here = R.dev.size() == R.host.size();
+                }
                 E.build_mu.unlock();
                 if (here) { E.pinned++; E.last_use = ++use_clock; }
             }
