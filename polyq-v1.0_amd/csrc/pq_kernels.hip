@@ -1675,35 +1675,35 @@ struct QueryTail {
 // ... starting from 0; each 64-lane wave folds by the xor butterfly (lane 0
 // ends with lane 0 + lane 32 first, then with the lane-16 sum, ...); then
 // (w0 + w1) + (w2 + w3), EV + omega OD and the scale.
+typedef double HostV4 __attribute__((vector_size(32)));
 void host_query_reduce(const double* part, uint64_t nitems, double2 sc, double* dst) {
-    double w[4][4];          // wave totals, [wave][component]
+    // the four components (EV re, im, OD re, im) as one 4-wide vector: the
+    // same IEEE additions, component by component
+    HostV4 w[4];
+    const HostV4 z = {0.0, 0.0, 0.0, 0.0};
     for (uint32_t wv = 0; wv < 4; ++wv) {
         const uint64_t t0 = 64 * (uint64_t)wv;
-        if (t0 >= nitems) { for (int c = 0; c < 4; ++c) w[wv][c] = 0.0; continue; }
+        if (t0 >= nitems) { w[wv] = z; continue; }
         // lanes at or beyond n hold no item: +0.0, and so does every sum of
         // them, so only lanes below n are computed (x + 0.0 is still done:
         // it turns -0.0 into +0.0 exactly as the device does)
         uint32_t n = (uint32_t)std::min<uint64_t>(64, nitems - t0);
-        double L[64][4];
-        for (uint32_t i = 0; i < n; ++i)
-            for (int c = 0; c < 4; ++c) {
-                double v = 0.0;
-                for (uint64_t k = t0 + i; k < nitems; k += 256) v += part[4 * k + c];
-                L[i][c] = v;
-            }
+        HostV4 L[64];
+        for (uint32_t i = 0; i < n; ++i) {
+            HostV4 v = z;
+            for (uint64_t k = t0 + i; k < nitems; k += 256) { HostV4 x; std::memcpy(&x, part + 4 * k, 32); v = v + x; }
+            L[i] = v;
+        }
         for (uint32_t off = 32; off > 0; off >>= 1) {
             const uint32_t m = std::min(off, n);
-            for (uint32_t i = 0; i < m; ++i) {
-                if (i + off < n) { for (int c = 0; c < 4; ++c) L[i][c] = L[i][c] + L[i + off][c]; }
-                else { for (int c = 0; c < 4; ++c) L[i][c] = L[i][c] + 0.0; }
-            }
+            for (uint32_t i = 0; i < m; ++i) L[i] = L[i] + (i + off < n ? L[i + off] : z);
             n = m;
         }
-        for (int c = 0; c < 4; ++c) w[wv][c] = L[0][c];
+        w[wv] = L[0];
     }
-    double t[4];
-    for (int c = 0; c < 4; ++c) t[c] = (w[0][c] + w[1][c]) + (w[2][c] + w[3][c]);
+    const HostV4 t = (w[0] + w[1]) + (w[2] + w[3]);
     const double h = 0.70710678118654752440;
+    // EV + omega * OD
     const double re = t[0] + h * (t[2] - t[3]);
     const double im = t[1] + h * (t[2] + t[3]);
     dst[0] = sc.x * re - sc.y * im;
