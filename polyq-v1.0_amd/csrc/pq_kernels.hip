@@ -24,6 +24,7 @@
 #include <hip/hip_runtime.h>
 #include "pq_device.hpp"
 #include <mutex>
+#include <thread>
 #include <atomic>
 #include <cstring>
 #include <algorithm>
@@ -1728,6 +1729,11 @@ void host_finish(const QueryTail& T, const double* part, hipEvent_t done, double
         for (uint64_t i = lo; i < hi;) {
             if (w[i] != kPending) { ++i; continue; }
             if (finished) throw std::runtime_error("segment walk ended without writing every item partial");
+            // Yield now and then: the runtime's own threads (completion
+            // handling) may share this core, and a spin that never lets them
+            // run gets the calling thread descheduled for long stretches
+            // (every host phase of a step then runs several times slower).
+            if ((spin & 63) == 63) std::this_thread::yield();
             if ((++spin & 255) == 0) {
                 const hipError_t e = hipEventQuery(done);
                 if (e != hipErrorNotReady) { HIPCHECK(e); finished = true; }
