@@ -24,6 +24,7 @@
 #include <hip/hip_runtime.h>
 #include "pq_device.hpp"
 #include <mutex>
+#include <chrono>
 #include <thread>
 #include <atomic>
 #include <cstring>
@@ -1717,13 +1718,16 @@ void host_query_reduce(const double* part, uint64_t nitems, double2 sc, double* 
 constexpr uint64_t kPending = 0x7FF4C0FFEE5107EDull;
 
 // Waits for the item partials of each query in turn (the walk may still be
-// running later items) and finishes the query on the host.  The stream's
-// event is polled now and then, so a failed launch is reported instead of
-// waited on.
-void host_finish(const QueryTail& T, const double* part, hipEvent_t done, double* res) {
+// running later items) and finishes the query on the host.  A launch that
+// failed leaves words pending: with an event (`done`) the event is polled now
+// and then; without one the stream is asked after every 2 s of waiting (a
+// batch small enough to finish here takes milliseconds at most), so the
+// failure is reported instead of waited on.
+void host_finish(const QueryTail& T, const double* part, hipEvent_t done, hipStream_t stream, double* res) {
     const volatile uint64_t* w = (const volatile uint64_t*)part;
     bool finished = false;   // the stream has completed: every store landed
     uint64_t spin = 0;
+    auto t_ask = std::chrono::steady_clock::now();
     for (size_t q = 0; q < T.item0.size(); ++q) {
         const uint64_t lo = 4 * T.item0[q], hi = 4 * (T.item0[q] + T.nitems[q]);
         for (uint64_t i = lo; i < hi;) {
@@ -1734,9 +1738,15 @@ void host_finish(const QueryTail& T, const double* part, hipEvent_t done, double
             // run gets the calling thread descheduled for long stretches
             // (every host phase of a step then runs several times slower).
             if ((spin & 63) == 63) std::this_thread::yield();
-            if ((++spin & 255) == 0) {
-                const hipError_t e = hipEventQuery(done);
+            if (done) {
+                if ((++spin & 255) == 0) {
+                    const hipError_t e = hipEventQuery(done);
+                    if (e != hipErrorNotReady) { HIPCHECK(e); finished = true; }
+                }
+            } else if ((++spin & 4095) == 0 && std::chrono::steady_clock::now() - t_ask > std::chrono::seconds(2)) {
+                const hipError_t e = hipStreamQuery(stream);
                 if (e != hipErrorNotReady) { HIPCHECK(e); finished = true; }
+                t_ask = std::chrono::steady_clock::now();
             }
         }
         std::atomic_thread_fence(std::memory_order_acquire);
@@ -1767,6 +1777,7 @@ struct Z8Slot {
     uint32_t nq = 0, grid = 0;
     uint64_t items = 0, branches = 0;
     bool pending = false, timed = false, on_host = false;
+    bool has_done = false;           // `done` was recorded after the batch
     Z8Slot() {
         HIPCHECK(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));   // default priority
         HIPCHECK(hipEventCreate(&e0)); HIPCHECK(hipEventCreate(&e1)); HIPCHECK(hipEventCreate(&e2)); HIPCHECK(hipEventCreate(&e3));
@@ -1798,8 +1809,11 @@ void dev_z8_launch(Z8Slot& S, const std::vector<Z8Job>& jobs, int walk, bool tim
         S.tail.set(B);
         if (timed) { HIPCHECK(hipEventRecord(S.e0, S.stream)); HIPCHECK(hipEventRecord(S.e1, S.stream)); }
         S.grid = launch_args(S.stream, B, (double*)S.h_part.dev);
-        if (timed) { HIPCHECK(hipEventRecord(S.e2, S.stream)); HIPCHECK(hipEventRecord(S.e3, S.stream)); }
-        HIPCHECK(hipEventRecord(S.done, S.stream));
+        // An untimed batch records no completion event: the host waits for
+        // its partials (an event record costs about 1 us of host time per
+        // batch and wakes the runtime's completion handling)
+        if (timed) { HIPCHECK(hipEventRecord(S.e2, S.stream)); HIPCHECK(hipEventRecord(S.e3, S.stream)); HIPCHECK(hipEventRecord(S.done, S.stream)); }
+        S.has_done = timed;
         S.pending = true;
         return;
     }
@@ -1829,12 +1843,13 @@ void dev_z8_launch(Z8Slot& S, const std::vector<Z8Job>& jobs, int walk, bool tim
     }
     if (timed) HIPCHECK(hipEventRecord(S.e3, S.stream));
     HIPCHECK(hipEventRecord(S.done, S.stream));
+    S.has_done = true;
     S.pending = true;
 }
 
 bool dev_z8_ready(Z8Slot& S) {
     if (!S.pending) return true;
-    hipError_t e = hipEventQuery(S.done);
+    hipError_t e = S.has_done ? hipEventQuery(S.done) : hipStreamQuery(S.stream);
     if (e == hipErrorNotReady) return false;
     HIPCHECK(e);
     return true;
@@ -1846,13 +1861,13 @@ void dev_z8_wait(Z8Slot& S, std::vector<std::complex<double>>& out, Z8Timing* ti
     double* hres = (double*)S.h_res.p;
     if (S.on_host) {
         try {
-            host_finish(S.tail, (const double*)S.h_part.p, S.done, hres);
+            host_finish(S.tail, (const double*)S.h_part.p, S.has_done ? S.done : nullptr, S.stream, hres);
         } catch (...) {
             S.pending = false;
             (void)hipStreamSynchronize(S.stream);
             throw;
         }
-        if (S.timed) HIPCHECK(hipEventSynchronize(S.done));
+        if (S.has_done && S.timed) HIPCHECK(hipEventSynchronize(S.done));
     } else {
         HIPCHECK(hipEventSynchronize(S.done));
     }
