@@ -156,8 +156,8 @@ int pq_server_submit(pq_server* srv, uint32_t num_qubits, size_t num_gates, cons
  * request failed (message in pq_last_error()). */
 int pq_server_wait(pq_server* srv, uint64_t ticket, double timeout_s, double out[2], double* latency_ms);
 /* completed, failed, batches, evictions, resident bytes, builds, largest
- * batch, batches launched while another was in flight, walk-kernel ms,
- * resident circuits. */
+ * batch, batches launched while another was in flight, walk-kernel ms (timed
+ * on every 8th batch and scaled to all batches), resident circuits. */
 int pq_server_stats(pq_server* srv, double out10[10]);
 
 /* ---- parity and diagnostics (no counterpart in the reference) ---- */

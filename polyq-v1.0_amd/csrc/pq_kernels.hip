@@ -1849,6 +1849,14 @@ void dev_z8_launch(Z8Slot& S, const std::vector<Z8Job>& jobs, int walk, bool tim
 
 bool dev_z8_ready(Z8Slot& S) {
     if (!S.pending) return true;
+    if (S.on_host && !S.has_done) {
+        // finished on the host: ready once every partial word has landed
+        // (or once the stream is idle without them: the wait reports that)
+        const volatile uint64_t* w = (const volatile uint64_t*)S.h_part.p;
+        bool all = true;
+        for (uint64_t i = 0; i < 4 * S.items && all; ++i) all = w[i] != kPending;
+        if (all) return true;
+    }
     hipError_t e = S.has_done ? hipEventQuery(S.done) : hipStreamQuery(S.stream);
     if (e == hipErrorNotReady) return false;
     HIPCHECK(e);
