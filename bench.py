@@ -93,7 +93,7 @@ SEG_OPS_PER_POINT = 30.0
 # length (FETCH_SIZE x 2 + WRITE_SIZE, copy-calibrated;
 # profiles/k_seg_walk_config2.md): the table once per XCD L2 and the item
 # partials.
-SEG_TRAFFIC = {("headline", 16, 2): 1.85e6, ("saturated", 16, 6): 5.37e6}
+SEG_TRAFFIC = {("headline", 16, 2): 1.84e6, ("saturated", 16, 6): 5.37e6}
 # Algorithmic lane-ops per outer point of k_seg_walk: the integer walk step
 # (phase update with the cross-term parity, sign-mask flip, table index: 6)
 # and the double-float arithmetic (walk magnitude times the complex table
