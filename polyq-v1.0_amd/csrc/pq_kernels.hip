@@ -22,6 +22,7 @@
 // through the scalar cache; lane-indexed tables (lane-factor byte tables, the
 // inner pivot table) are staged in LDS.
 #include <hip/hip_runtime.h>
+#include <hip/hip_ext.h>
 #include "pq_device.hpp"
 #include <mutex>
 #include <chrono>
@@ -1640,17 +1641,29 @@ void make_args(const BatchPlan& B, int v, SegArgs& A, uint32_t& gx, uint32_t& gy
         gx = std::max<uint32_t>(gx, (uint32_t)r.q.nitems);
     }
 }
-uint32_t launch_args(hipStream_t st, const BatchPlan& B, double* part) {
-    SegArgs A;
+// With t0/t1 set, the two events take their timestamps from the walk
+// kernels' own dispatches (start of the first, end of the last), so the
+// interval holds no marker packets and reads what a kernel trace reads.
+uint32_t launch_args(hipStream_t st, const BatchPlan& B, double* part, hipEvent_t t0 = nullptr, hipEvent_t t1 = nullptr) {
+    SegArgs A[2];
+    uint32_t gx[2], gy[2];
+    int last = -1, first = -1;
+    for (int v = 0; v < 2; ++v) {
+        make_args(B, v, A[v], gx[v], gy[v]);
+        if (gy[v] == 0) continue;
+        if (first < 0) first = v;
+        last = v;
+    }
     uint32_t g = 0;
     for (int v = 0; v < 2; ++v) {
-        uint32_t gx, gy;
-        make_args(B, v, A, gx, gy);
-        if (gy == 0) continue;
+        if (gy[v] == 0) continue;
         auto kern = v ? k_seg_walk_args<true> : k_seg_walk_args<false>;
-        hipLaunchKernelGGL(kern, dim3(gx, gy), dim3(256), 0, st, A, part);
+        if (t0 || t1)
+            hipExtLaunchKernelGGL(kern, dim3(gx[v], gy[v]), dim3(256), 0, st, v == first ? t0 : nullptr, v == last ? t1 : nullptr, 0, A[v], part);
+        else
+            hipLaunchKernelGGL(kern, dim3(gx[v], gy[v]), dim3(256), 0, st, A[v], part);
         HIPCHECK(hipGetLastError());
-        g += gx * gy;
+        g += gx[v] * gy[v];
     }
     return g;
 }
@@ -1807,12 +1820,12 @@ void dev_z8_launch(Z8Slot& S, const std::vector<Z8Job>& jobs, int walk, bool tim
         uint64_t* hp = (uint64_t*)S.h_part.get(B.total_items * 32);
         for (uint64_t i = 0; i < 4 * B.total_items; ++i) hp[i] = kPending;
         S.tail.set(B);
-        if (timed) { HIPCHECK(hipEventRecord(S.e0, S.stream)); HIPCHECK(hipEventRecord(S.e1, S.stream)); }
-        S.grid = launch_args(S.stream, B, (double*)S.h_part.dev);
+        if (timed) HIPCHECK(hipEventRecord(S.e0, S.stream));
+        S.grid = launch_args(S.stream, B, (double*)S.h_part.dev, timed ? S.e1 : nullptr, timed ? S.e2 : nullptr);
         // An untimed batch records no completion event: the host waits for
         // its partials (an event record costs about 1 us of host time per
         // batch and wakes the runtime's completion handling)
-        if (timed) { HIPCHECK(hipEventRecord(S.e2, S.stream)); HIPCHECK(hipEventRecord(S.e3, S.stream)); HIPCHECK(hipEventRecord(S.done, S.stream)); }
+        if (timed) { HIPCHECK(hipEventRecord(S.e3, S.stream)); HIPCHECK(hipEventRecord(S.done, S.stream)); }
         S.has_done = timed;
         S.pending = true;
         return;
