@@ -71,3 +71,26 @@ def test_buffer_protocol_call_equals_ctypes_call():
     np.testing.assert_array_equal(a, b)
     with pytest.raises(ValueError):
         sim.amplitudes(X, Y[:3])
+
+
+def test_timed_argument_batches_equal_untimed(restore_tuning):
+    # timing events bound to the walk dispatches (one launch, then two: the
+    # short-walk and the long-walk kernel) change no result and give a
+    # kernel time inside the batch's total
+    c1, xs1, ys1 = W.config2(5)
+    c2, xs2, ys2 = W.config2(2)
+    s1, s2 = pq.Simulator(c1, strategy="cached"), pq.Simulator(c2, strategy="cached")
+    for tuning, S, X, Y in (((12, 8), [s1] * 16, xs1[:16], ys1[:16]),
+                            ((8, 8), [s1] * 3 + [s2] * 3, list(xs1[:3]) + list(xs2[:3]), list(ys1[:3]) + list(ys2[:3]))):
+        pq.set_walk_tuning(*tuning)
+        X = [int(x) for x in X]
+        Y = [int(y) for y in Y]
+        plain = pq.batch_amplitudes(S, X, Y)
+        with pq.timed():
+            a = pq.batch_amplitudes(S, X, Y)
+            t = pq.last_timing()
+        np.testing.assert_array_equal(a, plain)
+        assert np.isfinite(t["kernel_ms"]) and t["kernel_ms"] > 0
+        assert t["kernel_ms"] <= t["total_ms"] + 1e-6
+    pq.batch_amplitudes(S, X, Y)
+    assert np.isnan(pq.last_timing()["kernel_ms"])      # untimed again outside the block
