@@ -17,6 +17,7 @@
 // shift rows.
 #include "pq_host.hpp"
 #include <algorithm>
+#include <cstring>
 #include <cmath>
 #include <queue>
 #include <map>
@@ -27,41 +28,66 @@ namespace pq {
 // Symmetric LDL^T over F2 with 1x1 and 2x2 pivots.  Pivot choice follows the
 // reference's MinWeight rule (normal_form.rs:269-330): first the lightest
 // alive row with a diagonal bit, else the lightest alive row with a neighbour,
-// ties by row index.  A lazy min-heap with per-row stamps replaces the two
-// ordered sets: popping the smallest (weight, index) entry whose stamp is
-// current gives the same pivot sequence.
+// ties by row index.  Bucket bitsets per (class, weight) replace the two
+// ordered sets: the lowest row of the lightest non-empty bucket is the
+// smallest (weight, index) entry, so the pivot sequence is the same.
 NormalForm normal_form(const std::vector<Bits>& b, const Bits& parity) {
     const uint32_t m = (uint32_t)b.size();
-    std::vector<Bits> gram(b), basis(m, Bits(m));
-    for (uint32_t i = 0; i < m; ++i) { gram[i].put(i, parity.get(i)); basis[i].set(i); }
-    Bits alive(m);
-    for (uint32_t i = 0; i < m; ++i) alive.set(i);
+    const size_t W = (m + 63) / 64;
+    // Gram and basis rows in two flat arrays (row r at r * W), scratch rows
+    // allocated once: a pivot step allocates nothing.
+    std::vector<uint64_t> gram(std::max<size_t>(m * W, 1), 0), basis(std::max<size_t>(m * W, 1), 0);
+    for (uint32_t i = 0; i < m; ++i) {
+        std::memcpy(&gram[i * W], b[i].w.data(), W * 8);
+        uint64_t& g = gram[i * W + (i >> 6)];
+        g = (g & ~(1ull << (i & 63))) | ((uint64_t)parity.get(i) << (i & 63));
+        basis[i * W + (i >> 6)] |= 1ull << (i & 63);
+    }
+    auto G = [&](uint32_t r) { return &gram[(size_t)r * W]; };
+    auto P = [&](uint32_t r) { return &basis[(size_t)r * W]; };
+    auto bit = [](const uint64_t* x, uint32_t i) { return (x[i >> 6] >> (i & 63)) & 1; };
+    auto clr = [](uint64_t* x, uint32_t i) { x[i >> 6] &= ~(1ull << (i & 63)); };
+    auto each = [&](const uint64_t* x, auto f) {
+        for (size_t k = 0; k < W; ++k) for (uint64_t v = x[k]; v; v &= v - 1) f((uint32_t)(k * 64 + __builtin_ctzll(v)));
+    };
+    auto lowest = [&](const uint64_t* x) -> long {
+        for (size_t k = 0; k < W; ++k) if (x[k]) return (long)(k * 64 + __builtin_ctzll(x[k]));
+        return -1;
+    };
+    auto to_bits = [&](const uint64_t* x) { Bits r(m); std::memcpy(r.w.data(), x, W * 8); return r; };
+    std::vector<uint64_t> alive(std::max<size_t>(W, 1), 0);
+    for (uint32_t i = 0; i < m; ++i) alive[i >> 6] |= 1ull << (i & 63);
 
     // Rows by class (1: diagonal entry set, 2: off-diagonal only) and weight
     // (alive neighbours), as bucket bitsets: the next pivot of a class is the
     // lowest row of its lightest non-empty bucket, the (weight, row) order of
     // a min-heap, with O(1) updates and no stale entries.
     struct Buckets {
-        std::vector<Bits> rows;          // per weight
+        std::vector<uint64_t> rows;      // per weight, W words each
         std::vector<uint32_t> cnt;
-        uint32_t lo = 0;                 // no non-empty bucket below lo
-        explicit Buckets(uint32_t m) : rows(m + 1, Bits(m)), cnt(m + 1, 0), lo(m + 1) {}
-        void add(uint32_t w, uint32_t a) { rows[w].set(a); cnt[w]++; if (w < lo) lo = w; }
-        void del(uint32_t w, uint32_t a) { rows[w].put(a, false); cnt[w]--; }
+        size_t W;
+        uint32_t lo;                     // no non-empty bucket below lo
+        Buckets(uint32_t m, size_t W) : rows((m + 1) * std::max<size_t>(W, 1), 0), cnt(m + 1, 0), W(W), lo(m + 1) {}
+        void add(uint32_t w, uint32_t a) { rows[w * W + (a >> 6)] |= 1ull << (a & 63); cnt[w]++; if (w < lo) lo = w; }
+        void del(uint32_t w, uint32_t a) { rows[w * W + (a >> 6)] &= ~(1ull << (a & 63)); cnt[w]--; }
         long top() {
-            for (; lo < cnt.size(); ++lo) if (cnt[lo]) return rows[lo].lowest();
+            for (; lo < cnt.size(); ++lo)
+                if (cnt[lo])
+                    for (size_t k = 0; k < W; ++k) if (rows[lo * W + k]) return (long)(k * 64 + __builtin_ctzll(rows[lo * W + k]));
             return -1;
         }
     };
-    Buckets bd(m), bo(m);              // diagonal / off-diagonal classes
+    Buckets bd(m, W), bo(m, W);        // diagonal / off-diagonal classes
     std::vector<uint8_t> cls(m, 0);
     std::vector<uint32_t> wt(m, 0);
     auto reclass = [&](uint32_t a) {
         if (cls[a] == 1) bd.del(wt[a], a);
         else if (cls[a] == 2) bo.del(wt[a], a);
-        if (!alive.get(a)) { cls[a] = 0; return; }
-        uint32_t w = gram[a].and_popc(alive);
-        if (gram[a].get(a)) { cls[a] = 1; wt[a] = w; bd.add(w, a); }
+        if (!bit(alive.data(), a)) { cls[a] = 0; return; }
+        const uint64_t* g = G(a);
+        uint32_t w = 0;
+        for (size_t k = 0; k < W; ++k) w += (uint32_t)__builtin_popcountll(g[k] & alive[k]);
+        if (bit(g, a)) { cls[a] = 1; wt[a] = w; bd.add(w, a); }
         else if (w > 0) { cls[a] = 2; wt[a] = w; bo.add(w, a); }
         else cls[a] = 0;
     };
@@ -72,40 +98,52 @@ NormalForm normal_form(const std::vector<Bits>& b, const Bits& parity) {
     };
     for (uint32_t a = 0; a < m; ++a) reclass(a);
 
+    std::vector<uint64_t> scratch(6 * std::max<size_t>(W, 1));
+    uint64_t* s = &scratch[0];
+    uint64_t* sa = &scratch[W];
+    uint64_t* sb = &scratch[2 * W];
+    uint64_t* pa = &scratch[3 * W];
+    uint64_t* pb = &scratch[4 * W];
+    uint64_t* touched = &scratch[5 * W];
     NormalForm nf; nf.m = m;
     for (;;) {
         long a = bd.top();
         if (a >= 0) {
             // 1x1 pivot: A'_bc = A_bc + A_ba A_ac, p_b += p_a for neighbours b
-            Bits s = gram[a]; s &= alive; s.put((uint32_t)a, false);
-            const Bits pa = basis[a];
-            s.each([&](uint32_t k) { gram[k] ^= s; basis[k] ^= pa; });
-            unclass((uint32_t)a);
-            alive.put((uint32_t)a, false);
-            nf.odd.push_back(pa);
-            s.each([&](uint32_t k) { reclass(k); });
+            const uint32_t ua = (uint32_t)a;
+            for (size_t k = 0; k < W; ++k) { s[k] = G(ua)[k] & alive[k]; pa[k] = P(ua)[k]; }
+            clr(s, ua);
+            each(s, [&](uint32_t r) {
+                uint64_t* g = G(r); uint64_t* p = P(r);
+                for (size_t k = 0; k < W; ++k) { g[k] ^= s[k]; p[k] ^= pa[k]; }
+            });
+            unclass(ua);
+            clr(alive.data(), ua);
+            nf.odd.push_back(to_bits(pa));
+            each(s, [&](uint32_t r) { reclass(r); });
             continue;
         }
         a = bo.top();
         if (a < 0) break;
         // 2x2 (hyperbolic) pivot on a and its first alive neighbour
-        Bits ra = gram[a]; ra &= alive;
-        uint32_t bb = (uint32_t)ra.lowest();
-        Bits sa = ra; sa.put(bb, false);
-        Bits sb = gram[bb]; sb &= alive; sb.put((uint32_t)a, false);
-        const Bits pa = basis[a], pb = basis[bb];
-        Bits touched = sa;
-        for (size_t k = 0; k < touched.w.size(); ++k) touched.w[k] |= sb.w[k];
-        touched.each([&](uint32_t k) {
-            if (sa.get(k)) { gram[k] ^= sb; basis[k] ^= pb; }
-            if (sb.get(k)) { gram[k] ^= sa; basis[k] ^= pa; }
+        const uint32_t ua = (uint32_t)a;
+        for (size_t k = 0; k < W; ++k) sa[k] = G(ua)[k] & alive[k];
+        const uint32_t bb = (uint32_t)lowest(sa);
+        clr(sa, bb);
+        for (size_t k = 0; k < W; ++k) { sb[k] = G(bb)[k] & alive[k]; pa[k] = P(ua)[k]; pb[k] = P(bb)[k]; }
+        clr(sb, ua);
+        for (size_t k = 0; k < W; ++k) touched[k] = sa[k] | sb[k];
+        each(touched, [&](uint32_t r) {
+            uint64_t* g = G(r); uint64_t* p = P(r);
+            if (bit(sa, r)) for (size_t k = 0; k < W; ++k) { g[k] ^= sb[k]; p[k] ^= pb[k]; }
+            if (bit(sb, r)) for (size_t k = 0; k < W; ++k) { g[k] ^= sa[k]; p[k] ^= pa[k]; }
         });
-        unclass((uint32_t)a); unclass(bb);
-        alive.put((uint32_t)a, false); alive.put(bb, false);
-        nf.pa.push_back(pa); nf.pb.push_back(pb);
-        touched.each([&](uint32_t k) { reclass(k); });
+        unclass(ua); unclass(bb);
+        clr(alive.data(), ua); clr(alive.data(), bb);
+        nf.pa.push_back(to_bits(pa)); nf.pb.push_back(to_bits(pb));
+        each(touched, [&](uint32_t r) { reclass(r); });
     }
-    alive.each([&](uint32_t r) { nf.even.push_back(basis[r]); });
+    each(alive.data(), [&](uint32_t r) { nf.even.push_back(to_bits(P(r))); });
     return nf;
 }
 
