@@ -181,8 +181,18 @@ const Z8Image& dev_z8(const DevEngine& D) { return *D.z8; }
 const SegImage& dev_seg(const DevEngine& D) { return *D.seg; }
 
 namespace {
+// The blob is assembled in a per-thread buffer that the next blob on the
+// thread reuses (a blob holds the segment tables, hundreds of KB: a fresh
+// vector of that size is mmap'd and page-faulted on every build).  At most
+// one builder is alive per thread at a time.
+std::vector<uint8_t>& blob_scratch() {
+    thread_local std::vector<uint8_t> b;
+    return b;
+}
 struct BlobBuilder {
-    std::vector<uint8_t> bytes;
+    std::vector<uint8_t>& bytes;
+    BlobBuilder() : bytes(blob_scratch()) { bytes.clear(); }
+    BlobBuilder(const BlobBuilder&) = delete;
     size_t put(const void* p, size_t n) {
         size_t off = (bytes.size() + 255) & ~size_t(255);
         bytes.resize(off + std::max<size_t>(n, 16), 0);

@@ -76,7 +76,17 @@ namespace {
 // In-place Walsh-Hadamard transform of n = 2^k doubles, two radix-2 stages
 // per pass (the same additions in the same order as stage-by-stage radix 2,
 // in half the passes over memory).
-void fwht(double* v, uint32_t n) {
+struct SegScratch {
+    std::vector<double> fin, re, im;
+    std::vector<uint8_t> kin;
+    std::vector<uint32_t> pinc, hof;
+};
+SegScratch& seg_scratch() {
+    thread_local SegScratch s;
+    return s;
+}
+
+void fwht_scalar(double* v, uint32_t n) {
     uint32_t len = 1;
     for (; 4 * len <= n; len *= 4)
         for (uint32_t base = 0; base < n; base += 4 * len)
@@ -90,6 +100,51 @@ void fwht(double* v, uint32_t n) {
         for (uint32_t x = 0; x < len; ++x) {
             const double a = v[x], b = v[x + len];
             v[x] = a + b; v[x + len] = a - b;
+        }
+}
+typedef double V4 __attribute__((vector_size(32)));
+typedef long long M4 __attribute__((vector_size(32)));
+inline V4 ld4(const double* p) { V4 r; std::memcpy(&r, p, 32); return r; }
+inline void st4(double* p, V4 r) { std::memcpy(p, &r, 32); }
+inline void bfly4(V4& a0, V4& a1, V4& a2, V4& a3) {
+    const V4 s01 = a0 + a1, d01 = a0 - a1, s23 = a2 + a3, d23 = a2 - a3;
+    a0 = s01 + s23; a1 = d01 + d23; a2 = s01 - s23; a3 = d01 - d23;
+}
+inline void transpose4(V4& a, V4& b, V4& c, V4& d) {
+    const V4 t0 = __builtin_shuffle(a, b, (M4){0, 4, 2, 6}), t1 = __builtin_shuffle(a, b, (M4){1, 5, 3, 7});
+    const V4 t2 = __builtin_shuffle(c, d, (M4){0, 4, 2, 6}), t3 = __builtin_shuffle(c, d, (M4){1, 5, 3, 7});
+    a = __builtin_shuffle(t0, t2, (M4){0, 1, 4, 5}); c = __builtin_shuffle(t0, t2, (M4){2, 3, 6, 7});
+    b = __builtin_shuffle(t1, t3, (M4){0, 1, 4, 5}); d = __builtin_shuffle(t1, t3, (M4){2, 3, 6, 7});
+}
+// Walsh-Hadamard transform in place: radix-4 passes over the index bit
+// pairs (0,1), (2,3), ..., then one radix-2 pass if log2 n is odd.  Four
+// lanes at a time: the passes of bits (0,1) and (2,3) on 16-entry blocks
+// (a 4x4 transpose puts the four entries of a butterfly in four vectors),
+// the later ones on four consecutive butterflies.  Every entry sees the same
+// additions in the same order as the one-at-a-time loop, so the result is
+// the same bit for bit.
+void fwht(double* v, uint32_t n) {
+    if (n < 16) { fwht_scalar(v, n); return; }
+    for (uint32_t b = 0; b < n; b += 16) {
+        V4 r0 = ld4(v + b), r1 = ld4(v + b + 4), r2 = ld4(v + b + 8), r3 = ld4(v + b + 12);
+        transpose4(r0, r1, r2, r3);
+        bfly4(r0, r1, r2, r3);          // bits (0,1)
+        transpose4(r0, r1, r2, r3);
+        bfly4(r0, r1, r2, r3);          // bits (2,3)
+        st4(v + b, r0); st4(v + b + 4, r1); st4(v + b + 8, r2); st4(v + b + 12, r3);
+    }
+    uint32_t len = 16;
+    for (; 4 * len <= n; len *= 4)
+        for (uint32_t base = 0; base < n; base += 4 * len)
+            for (uint32_t x = base; x < base + len; x += 4) {
+                V4 a0 = ld4(v + x), a1 = ld4(v + x + len), a2 = ld4(v + x + 2 * len), a3 = ld4(v + x + 3 * len);
+                bfly4(a0, a1, a2, a3);
+                st4(v + x, a0); st4(v + x + len, a1); st4(v + x + 2 * len, a2); st4(v + x + 3 * len, a3);
+            }
+    if (len < n)
+        for (uint32_t x = 0; x < len; x += 4) {
+            const V4 a = ld4(v + x), c = ld4(v + x + len);
+            st4(v + x, a + c); st4(v + x + len, a - c);
         }
 }
 }  // namespace
@@ -266,9 +321,17 @@ SegImage build_seg_image(const HostEngine& E) {
 
     // Inner weights W(c) g(pio + pin(c)) and their Walsh-Hadamard transform.
     const uint32_t NI = 1u << I, NP = 1u << S.npin;
-    std::vector<double> fin(NI, 1.0);
-    std::vector<uint8_t> kin(NI, 0);
-    std::vector<uint32_t> pinc(NI, 0);
+    // Work arrays of up to 2^16 entries (about 2 MB) live in per-thread
+    // buffers that later builds on the thread reuse: fresh blocks this large
+    // come from mmap, and touching their pages cost about as much as the
+    // transforms themselves.
+    SegScratch& W = seg_scratch();
+    W.fin.assign(NI, 1.0);
+    W.kin.assign(NI, 0);
+    W.pinc.assign(NI, 0);
+    std::vector<double>& fin = W.fin;
+    std::vector<uint8_t>& kin = W.kin;
+    std::vector<uint32_t>& pinc = W.pinc;
     for (uint32_t p = 0; p < I; ++p) {
         const uint32_t j = S.inner_j[p], k = E.free_col[j];
         uint32_t qlow = 0;
@@ -290,9 +353,12 @@ SegImage build_seg_image(const HostEngine& E) {
     S.table.assign((size_t)NP * NI * 4, 0.f);
     // table index hp -> transform index h = B hp (B: the columns bcols), one
     // XOR per entry from the entry with the lowest set bit cleared
-    std::vector<uint32_t> hof(NI, 0);
+    W.hof.assign(NI, 0);
+    std::vector<uint32_t>& hof = W.hof;
     for (uint32_t hp = 1; hp < NI; ++hp) hof[hp] = hof[hp & (hp - 1)] ^ bcols[__builtin_ctz(hp)];
-    std::vector<double> re(NI), im(NI);
+    W.re.resize(NI); W.im.resize(NI);
+    std::vector<double>& re = W.re;
+    std::vector<double>& im = W.im;
     for (uint32_t pio = 0; pio < NP; ++pio) {
         for (uint32_t c = 0; c < NI; ++c) {
             const double w = fin[c] * gpin[pio ^ pinc[c]];
