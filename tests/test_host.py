@@ -220,3 +220,14 @@ def test_direct_prologue_matches_assignment_prologue():
     c, xs, ys = W.config2(5)
     assert pq.Simulator(c, strategy="cached").prologue_selfcheck(xs, ys) == 0
     assert checked >= 100
+
+
+def test_index_marshalling_to_uint64():
+    # basis indices reach the C-ABI as one uint64 array, from lists, tuples
+    # or arrays; a negative index is an error, not a wrapped value
+    for v in ([1, 2, 2**64 - 1], (0, 7), np.array([3, 4], dtype=np.int64), np.arange(5, dtype=np.uint64)[::2]):
+        a = pq._u64_arg(v)
+        assert list(a) == [int(x) for x in v]
+    assert len(pq._u64_arg([])) == 0
+    with pytest.raises(OverflowError):
+        pq._u64_arg([5, -1])
