@@ -1,27 +1,33 @@
 """Benchmark of the MI355X branch evaluator.
 
-Headline (the JSON line's `value`): the plain-GPU-at-scale workload, one step
-= the 16 amplitudes <0|C|y> of the n=40 circuit "H on all qubits; 36 T + 300
-CX/CZ + 100 S interleaved (SplitMix64 seed 5); H on all qubits" (t = 30 terms
-after T merging, branch dimension d = 30: 16 x 2^30 branches), all 16
-evaluated in one segment-walk launch (a 2^16-entry Walsh-Hadamard table at
-d = 30, built once per circuit; the other 14 coordinates walked in Gray
-order).  `value` counts the 2^d branches each amplitude sums.  Engine images
-are resident in HBM before timing starts.  Reported beside it:
-  * (section saturated) the segment-walk kernel on a saturating launch (4096
-    amplitudes, lane walk length autotuned for it unless --saturated-tuning),
-  * the per-branch Z8 walk (every branch visited) on the same 16 amplitudes,
+Headline (the JSON line's `value`, Gray-code branches/s at T=36): the
+plain-GPU-at-scale workload, one step = the 16 amplitudes <0|C|y> of the n=40
+circuit "H on all qubits; 36 T + 300 CX/CZ + 100 S interleaved (SplitMix64
+seed 5); H on all qubits" (t = 30 terms after T merging, branch dimension
+d = 30: 16 x 2^30 branches), all 16 evaluated in one launch of the per-branch
+walk (`k_z8_walk`): every one of the 2^30 branches of each amplitude is
+visited on the GPU, one sign XOR and one FP64 FMA per branch, with integer Z8
+phases and FP64 accumulation.  Engine images are resident in HBM before
+timing starts; the host build of the engine (compile, normal form, branch
+tables) is timed apart and reported in `config`.  Reported beside it:
+  * the default served form (segment walk: per-engine FP64 Walsh-Hadamard
+    tables over 14-16 inner coordinates, outer Gray walk in double-float
+    lanes) on the same 16 amplitudes: amplitudes/s, its kernel's roofline on
+    the outer points it walks, and a saturating launch (4096 amplitudes),
   * the plain FP64 form (one query per launch, FP64 port of the reference's
-    Gray walk) on the same 16 amplitudes, and per-amplitude error against it,
+    Gray walk) on the same 16 amplitudes, and the per-amplitude error of the
+    other forms against it,
   * the CPU oracle (a C++ restatement of the reference's Rayon engine, OpenMP
     over 2^14-branch Gray chunks) on the host cores, on a bounded sample, with
     the same 2^d branches per amplitude (Cached), and with the oracle's own
     best strategy (Auto: pair decomposition into 2^p smaller engines, built
     and evaluated in parallel) for an amplitudes/s comparison; engine build
     times are reported apart from evaluation,
-  * the roofline fraction of the walk kernel against the VALU issue rate.
+  * a VALU issue-rate probe (pq_valu_probe) run first on the same device:
+    the roofline peaks are its measured lanes per clock per CU x CUs x the
+    2.4 GHz peak engine clock.
 Further sections (each bounded; --sections picks them):
-  precision  n=48, 40 Rz(theta) (seed 4, d = 28): 1024 amplitudes in the
+  precision  n=48, 40 Rz(theta) (seed 4, d = 30): 1024 amplitudes in the
              segment (z8) and branch-walk forms against the FP64 GPU form, the
              FP64 form against the CPU oracle on 16 of them; per-amplitude errors, with
              |a| < 2^-20 reported separately (--precision-out writes them all)
@@ -89,107 +95,155 @@ def _err_summary(got, ref):
 
 
 SEG_OPS_PER_POINT = 30.0
-# HBM bytes per walk launch, keyed by launch shape, table bits and lane walk
-# length (FETCH_SIZE x 2 + WRITE_SIZE, copy-calibrated;
-# profiles/k_seg_walk_config2.md): the table once per XCD L2 and the item
-# partials.
-SEG_TRAFFIC = {("headline", 16, 2): 1.84e6, ("saturated", 16, 6): 5.37e6}
 # Algorithmic lane-ops per outer point of k_seg_walk: the integer walk step
 # (phase update with the cross-term parity, sign-mask flip, table index: 6)
 # and the double-float arithmetic (walk magnitude times the complex table
 # entry: 2 x 4; compensated complex accumulation: 2 x 8).  The lane seed, the
 # lane factor and the reductions are overhead, not counted.
+# HBM bytes per walk launch, keyed by launch shape, table bits and lane walk
+# length (FETCH_SIZE x 2 + WRITE_SIZE, copy-calibrated;
+# profiles/k_seg_walk_config2.md): the table once per XCD L2 and the item
+# partials.
+SEG_TRAFFIC = {("headline", 16, 2): 1.84e6, ("saturated", 16, 6): 5.37e6}
+# HBM bytes per branch-walk launch on the headline step (FETCH_SIZE x 2 +
+# WRITE_SIZE, copy-calibrated; profiles/k_z8_walk_config2.md): the item
+# partials (4 doubles per 256-lane item) and the tables once per XCD.
+BRANCH_TRAFFIC = {}
+PEAK_CLOCK_HZ = 2.4e9       # MI355X peak engine clock
 
 
-def _peak(dev):
-    cus = int(dev.split("/")[-1].split()[0]) if "CUs" in dev else 256
-    # 128 lanes/clk/CU for FP32/int VALU ops (157.3 TFLOP/s FP32 / 2 per FMA);
-    # rocprof counters on the walk kernels confirm issue above 64 lanes/clk/CU.
-    return cus * 128 * 2.4e9
+def _valu_probe(pq):
+    """Measured VALU issue rates (lanes per clock per CU) of this device."""
+    try:
+        return pq.valu_probe(blocks_per_cu=8, iters=16384)
+    except Exception as e:      # the probe is a measurement aid; the bench still runs
+        return {"error": str(e)}
 
 
-def _seg_roofline(dev, points, kernel_ms, extra=None, traffic=None, kernel="k_seg_walk"):
-    peak = _peak(dev)
+def _peak(probe, op, fallback_lanes):
+    """Peak lane-ops/s for an instruction mix: the probe's lanes per clock per
+    CU for that mix x CUs x the peak engine clock (fallback_lanes per clock
+    when the probe did not run)."""
+    ops = probe.get("ops", {})
+    cus = probe.get("cus", 256)
+    lanes = ops[op]["lanes_per_clk_per_cu"] if op in ops else fallback_lanes
+    return cus * lanes * PEAK_CLOCK_HZ, lanes, op if op in ops else "assumed"
+
+
+def _seg_roofline(probe, points, kernel_ms, extra=None, traffic=None, kernel="k_seg_walk"):
+    # the segment walk issues int32 and FP32 ops: price it at the faster of
+    # the two measured rates
+    p_int, l_int, _ = _peak(probe, "v_xor_b32", 64)
+    p_f32, l_f32, _ = _peak(probe, "v_fma_f32", 64)
+    peak = max(p_int, p_f32)
     achieved = points * SEG_OPS_PER_POINT / (kernel_ms * 1e-3)
     r = {"bound": "valu", "achieved": achieved / 1e12, "peak": peak / 1e12, "unit": "Tlane-op/s", "frac": achieved / peak,
          "traffic": traffic, "kernel": kernel, "kernel_avg_ms": kernel_ms, "ops_per_outer_point": SEG_OPS_PER_POINT,
-         "outer_points_per_launch": points}
+         "outer_points_per_launch": points, "peak_lanes_per_clk_per_cu": max(l_int, l_f32)}
     if extra:
         r.update(extra)
     return r
 
 
-def headline(args, pq, W, np):
+def _s8d_ops(st):
+    """Lane-ops per branch of the reference's per-branch algorithm (popcount
+    phase8 on the packed shift row, term update, combine): 4 W_g + 4 W_o +
+    4 W_t + 14 with W = 32-bit words of planes, lines and terms."""
+    w = lambda n: (n + 31) // 32
+    return 4 * w(st[1]) + 4 * w(st[2]) + 4 * w(st[4]) + 14
+
+
+def _timed_kernel_ms(pq, np, fn, reps):
+    ms = []
+    with pq.timed():
+        for _ in range(reps):
+            out = fn()
+            ms.append(pq.last_timing()["kernel_ms"])
+    return float(np.mean(ms)), out
+
+
+def headline(args, pq, W, np, probe):
     dev = pq.device_info()
     c, xs, ys = W.config2(SEED)
-    sim = pq.Simulator(c, strategy="cached", form="z8")
+    tb = time.perf_counter()
+    sim = pq.Simulator(c, strategy="cached", form="branch_walk")
+    build_s = time.perf_counter() - tb          # compile + normal form + elimination + walk images + segment tables
     st = sim.stats()
-    seg = sim.seg_info()
     info = sim.z8_info()
     d = st[5]
     nq = len(ys)
     branches_per_step = nq * (1 << d)
-    points_per_step = nq * (1 << seg["outer"])
-
-    # lane walk length autotuned for this batch shape (16 queries of one circuit)
-    if args.headline_tuning:
-        tuned = tuple(int(v) for v in args.headline_tuning.split(","))
-        pq.set_walk_tuning(*tuned)
-    else:
-        tuned = pq.autotune([sim] * nq, xs, ys, repeats=7, rounds=5, cache=False)[:2]
     # the basis indices as uint64 arrays, the form the C ABI takes them in
     xs = np.asarray(xs, dtype=np.uint64)
     ys = np.asarray(ys, dtype=np.uint64)
+    tu = time.perf_counter()
+    sim.amplitudes(xs[:1], ys[:1])
+    upload_s = time.perf_counter() - tu          # first device use: engine images to HBM (one amplitude included)
+
+    # lane walk length of the branch walk, autotuned for this batch shape
+    if args.headline_tuning:
+        tuned = tuple(int(v) for v in args.headline_tuning.split(","))
+        pq.set_walk_tuning(*tuned, form="branch_walk")
+    else:
+        tuned = pq.autotune([sim] * nq, xs, ys, form="branch_walk", repeats=3, rounds=2, cache=False)[:2]
     for _ in range(args.warmup):
         sim.amplitudes(xs, ys)
     t0 = time.perf_counter()
     for _ in range(args.steps):
         got = sim.amplitudes(xs, ys)     # synchronous: returns after the device results are on the host
-    t1 = time.perf_counter()
-    elapsed = t1 - t0
+    elapsed = time.perf_counter() - t0
     # Kernel durations: the same steps again with HIP timing events on the
-    # launch stream (the events are queue barriers that lengthen a step, so
-    # the throughput loop above runs without them).
-    kern_ms = []
-    with pq.timed():
-        for _ in range(args.steps):
-            again = sim.amplitudes(xs, ys)
-            kern_ms.append(pq.last_timing()["kernel_ms"])
-    kernel_avg_ms = float(np.mean(kern_ms))
+    # launch stream (the events are queue barriers, so the throughput loop
+    # above runs without them).
+    kernel_avg_ms, again = _timed_kernel_ms(pq, np, lambda: sim.amplitudes(xs, ys), min(args.steps, 50))
     timing = pq.last_timing()
     assert np.array_equal(again, got)
+    ops = info["ops_per_branch"]
+    # without inner pivots the inner step is one FP64 FMA per branch (the sign
+    # XOR comes once per 8 branches): priced at the measured v_fma_f64 rate;
+    # with pivots at the xor/FMA mix
+    peak, lanes, peak_op = _peak(probe, "v_fma_f64" if ops == 1 else "v_xor_b32+v_fma_f64", 64)
+    achieved = branches_per_step * ops / (kernel_avg_ms * 1e-3)
+    s8d = _s8d_ops(st)
+    kern = "k_z8_walk<%s, %s>" % ("true" if info["inner_pivots"] else "false", "true" if info["imag_slot0"] else "false")
     res = {
-        "sim": sim, "c": c, "xs": xs, "ys": ys, "st": st, "info": info, "seg": seg, "d": d, "nq": nq, "got": got,
+        "sim": sim, "c": c, "xs": xs, "ys": ys, "st": st, "info": info, "d": d, "nq": nq, "got": got,
         "branches_per_step": branches_per_step, "elapsed": elapsed, "kernel_avg_ms": kernel_avg_ms,
-        "timing": timing, "dev": dev, "tuning": list(tuned),
+        "timing": timing, "dev": dev, "tuning": list(tuned), "build_s": build_s, "upload_s": upload_s,
+        "roofline": {"bound": "valu", "achieved": achieved / 1e12, "peak": peak / 1e12, "unit": "Tlane-op/s",
+                     "frac": achieved / peak, "traffic": BRANCH_TRAFFIC.get(tuple(tuned)), "kernel": kern,
+                     "kernel_avg_ms": kernel_avg_ms, "ops_per_branch": ops, "branches_per_launch": branches_per_step,
+                     "peak_mix": peak_op, "peak_lanes_per_clk_per_cu": lanes,
+                     "kernel_branches_per_s": branches_per_step / (kernel_avg_ms * 1e-3),
+                     # the same branch rate priced at the reference algorithm's per-branch work
+                     "ref_algorithm_ops_per_branch": s8d,
+                     "ref_algorithm_equiv_frac": branches_per_step * s8d / (kernel_avg_ms * 1e-3) / peak},
     }
-    # Roofline of k_seg_walk on this launch (16 queries: 2^20 outer points,
-    # too few lanes to fill 256 CUs); the saturating launch is measured by
-    # saturated() below.
-    # 16 queries over one engine travel as kernel arguments; walks of at most
-    # 16 points run the short-walk instance
-    seg = sim.seg_info()            # with the tuned lane walk length
-    res["seg"] = seg
-    kern = "k_seg_walk_args<%s>" % ("true" if seg["mid_bits"] <= 4 else "false")
-    res["roofline"] = _seg_roofline(dev, points_per_step, kernel_avg_ms, kernel=kern,
-                                    traffic=SEG_TRAFFIC.get(("headline", seg["inner"], seg["mid_bits"])))
 
-    # The per-branch Z8 walk on the same 16 amplitudes (every branch visited:
-    # one sign XOR and one FMA per branch, one more LDS lookup with inner pivots).
-    bw = []
-    with pq.timed():
-        for _ in range(3):
-            sim.amplitudes(xs, ys, form="branch_walk")
-            bw.append(pq.last_timing()["kernel_ms"])
-    bw_ms = float(np.median(bw))
-    ga = sim.amplitudes(xs, ys, form="branch_walk")
-    peak = _peak(dev)
-    res["branch_walk"] = {
-        "kernel": "k_z8_walk", "kernel_ms": bw_ms, "branches_per_s": branches_per_step / (bw_ms * 1e-3),
-        "roofline": {"bound": "valu", "achieved": branches_per_step * info["ops_per_branch"] / (bw_ms * 1e-3) / 1e12,
-                     "peak": peak / 1e12, "unit": "Tlane-op/s",
-                     "frac": branches_per_step * info["ops_per_branch"] / (bw_ms * 1e-3) / peak,
-                     "ops_per_branch": info["ops_per_branch"]},
+    # The default served form (segment walk) on the same 16 amplitudes.
+    seg = sim.seg_info()
+    points_per_step = nq * (1 << seg["outer"])
+    if args.segment_tuning:
+        pq.set_walk_tuning(*(int(v) for v in args.segment_tuning.split(",")))
+        stune = tuple(int(v) for v in args.segment_tuning.split(","))
+    else:
+        stune = pq.autotune([sim] * nq, xs, ys, repeats=7, rounds=5, cache=False)[:2]
+    for _ in range(args.warmup):
+        sim.amplitudes(xs, ys, form="z8")
+    ts = time.perf_counter()
+    for _ in range(args.steps):
+        zgot = sim.amplitudes(xs, ys, form="z8")
+    z_elapsed = time.perf_counter() - ts
+    z_ms, zagain = _timed_kernel_ms(pq, np, lambda: sim.amplitudes(xs, ys, form="z8"), min(args.steps, 50))
+    assert np.array_equal(zagain, zgot)
+    seg = sim.seg_info()            # with the tuned lane walk length
+    zkern = "k_seg_walk_args<%s>" % ("true" if seg["mid_bits"] <= 4 else "false")
+    res["segment"] = {
+        "form": "z8 (default: per-engine FP64 Walsh-Hadamard tables, outer Gray walk, double-float lanes)",
+        "amplitudes_per_s": nq * args.steps / z_elapsed, "ms_per_step": 1e3 * z_elapsed / args.steps,
+        "walk_tuning": list(stune), "segment": seg, "got": zgot,
+        "roofline": _seg_roofline(probe, points_per_step, z_ms, kernel=zkern,
+                                  traffic=SEG_TRAFFIC.get(("headline", seg["inner"], seg["mid_bits"]))),
     }
 
     if not args.no_plain:
@@ -198,13 +252,11 @@ def headline(args, pq, W, np):
         tp0 = time.perf_counter()
         pa = psim.amplitudes(xs, ys)
         tp1 = time.perf_counter()
-        scale = float(np.max(np.abs(pa)))
         res["plain"] = {
             "amplitudes_per_s": nq / (tp1 - tp0),
             "branches_per_s": branches_per_step / (tp1 - tp0),
-            "max_abs_err_z8_over_max_amp": float(np.max(np.abs(got - pa)) / scale),
-            "z8_vs_fp64": _err_summary(got, pa),
-            "branch_walk_vs_fp64": _err_summary(ga, pa),
+            "branch_walk_vs_fp64": _err_summary(got, pa),
+            "z8_vs_fp64": _err_summary(zgot, pa),
         }
 
     if not args.no_cpu:
@@ -225,23 +277,23 @@ def headline(args, pq, W, np):
             "value": done_br / done_sec, "unit": "branches/s", "cores": threads, "kind": "port",
             "sample": "%d full amplitudes (%d x 2^%d branches, 2^14-branch Gray chunks, OpenMP dynamic) of the same workload"
                       % (amps, amps, d),
-            "amplitudes_per_s": amps / done_sec, "max_rel_err_z8_vs_cpu": cpu_err,
+            "amplitudes_per_s": amps / done_sec, "max_rel_err_branch_walk_vs_cpu": cpu_err,
             "engine_new_s": osim.build_seconds, "host_nproc": os.cpu_count(), "host_cpu": _cpu_model(),
         }
         # the oracle's own best strategy: Auto (pair decomposition by the
         # cost model, branch engines built and evaluated in parallel); eval
         # timed alone, median of 5 passes over the 16 amplitudes after one
         auto = O.OSim(oc, "auto", threads=threads)
-        st = auto.stats()
+        ast = auto.stats()
         auto.amplitudes_timed(xs[:1], ys[:1], threads)
         secs, auto_err = [], 0.0
         for _ in range(5):
             aa, sec = auto.amplitudes_timed(xs, ys, threads)
             secs.append(sec)
-            auto_err = max(auto_err, float(np.max(np.abs(aa - got) / np.maximum(np.abs(aa), 1e-300))))
+            auto_err = max(auto_err, float(np.max(np.abs(aa - zgot) / np.maximum(np.abs(aa), 1e-300))))
         res["cpu_auto"] = {
-            "strategy": "auto", "pairs": st[6], "engines": 1 << st[6], "engine_branch_dim": st[5],
-            "branches_per_amplitude": (1 << st[6]) << st[5], "engine_new_s": auto.build_seconds,
+            "strategy": "auto", "pairs": ast[6], "engines": 1 << ast[6], "engine_branch_dim": ast[5],
+            "branches_per_amplitude": (1 << ast[6]) << ast[5], "engine_new_s": auto.build_seconds,
             "amplitudes_per_s": nq / float(np.median(secs)), "cores": threads,
             "sample": "5 passes over the %d amplitudes of one step after one warm-up amplitude; median pass" % nq,
             "max_rel_err_z8_vs_cpu_auto": auto_err,
@@ -249,11 +301,10 @@ def headline(args, pq, W, np):
     return res
 
 
-def saturated(args, pq, W, np, sim=None):
+def saturated(args, pq, W, np, probe, sim=None):
     """k_seg_walk at full occupancy: 4096 amplitudes <0|C|y> (random y) of the
     n=40 workload circuit in one launch, with the lane walk length autotuned
     for this batch (then restored)."""
-    dev = pq.device_info()
     if sim is None:
         c, _, _ = W.config2(SEED)
         sim = pq.Simulator(c, strategy="cached", form="z8")
@@ -272,15 +323,15 @@ def saturated(args, pq, W, np, sim=None):
     bms, bwall = [], []
     for _ in range(args.saturated_reps):
         tb = time.perf_counter()
-        sim.amplitudes(big_x, big_y)
+        sim.amplitudes(big_x, big_y, form="z8")
         bwall.append(time.perf_counter() - tb)
     with pq.timed():
         for _ in range(args.saturated_reps):
-            sim.amplitudes(big_x, big_y)
+            sim.amplitudes(big_x, big_y, form="z8")
             bms.append(pq.last_timing()["kernel_ms"])
     tuned = sim.seg_info()["mid_bits"]
     pq.set_walk_tuning(*default)
-    return _seg_roofline(dev, 4096 * (1 << seg["outer"]), float(np.median(bms)),
+    return _seg_roofline(probe, 4096 * (1 << seg["outer"]), float(np.median(bms)),
                          {"queries": 4096, "walk_tuning": [off, mx], "mid_bits": tuned,
                           "amplitudes_per_s": 4096 / float(np.median(bwall)),
                           "branches_per_s": 4096 * (1 << d) / float(np.median(bwall))},
@@ -296,7 +347,7 @@ def precision(args, pq, W, np):
     aa = z.amplitudes(xs, ys, form="branch_walk"); t2 = time.perf_counter()
     p = pq.Simulator(c, strategy="cached", form="plain_fp64")
     t3 = time.perf_counter(); ap = p.amplitudes(xs, ys); t4 = time.perf_counter()
-    out = {"workload": "n=48: H^48; 40 Rz(theta ~ U[0,2pi)) + 360 CX/CZ + 120 S (seed 4); H^48; 1024 amplitudes <0|C|y>",
+    out = {"workload": "n=48: H^48; 40 Rz(theta ~ U[0,2pi)) + 300 CX/CZ + 100 S (seed 4); H^48; 1024 amplitudes <0|C|y>",
            "branch_dim": d, "z8_vs_fp64": _err_summary(az, ap), "branch_walk_vs_fp64": _err_summary(aa, ap),
            "z8_s": t1 - t0, "branch_walk_s": t2 - t1, "plain_fp64_s": t4 - t3}
     if not args.no_cpu:
@@ -477,7 +528,9 @@ def main():
     ap.add_argument("--saturated-tuning", default=None, help="offset,max_mid for the saturating launch instead of autotuning")
     ap.add_argument("--precision-out", default=None)
     ap.add_argument("--headline-tuning", default="",
-                    help="offset,max_mid of the headline lane walk instead of autotuning it (profiling runs)")
+                    help="offset,max_mid of the headline (branch walk) lane walk instead of autotuning it (profiling runs)")
+    ap.add_argument("--segment-tuning", default="",
+                    help="offset,max_mid of the segment walk on the headline step instead of autotuning it")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run three 256 MiB device copies (reference traffic for FETCH_SIZE/WRITE_SIZE counters)")
     args = ap.parse_args()
@@ -501,27 +554,32 @@ def main():
     from polyq_amd import workloads as W
 
     sections = set(args.sections.split(","))
+    probe = _valu_probe(pq)
     if "headline" not in sections:
         # profiling aid: only the listed sections (e.g. the saturating launches)
-        out = {}
+        out = {"valu_probe": probe}
         if "saturated" in sections:
-            out["roofline_saturated"] = saturated(args, pq, W, np)
+            out["roofline_saturated"] = saturated(args, pq, W, np, probe)
         for name, fn in (("precision", precision), ("batch", batch), ("serve", serve)):
             if name in sections:
                 out[name] = fn(args, pq, W, np)
         print(json.dumps(out))
         return
-    h = headline(args, pq, W, np)
-    h["roofline_saturated"] = saturated(args, pq, W, np, h["sim"]) if "saturated" in sections else None
+    h = headline(args, pq, W, np, probe)
+    print("bench: headline done", file=sys.stderr, flush=True)
+    h["segment"]["saturated"] = saturated(args, pq, W, np, probe, h["sim"]) if "saturated" in sections else None
     st, d, nq, elapsed = h["st"], h["d"], h["nq"], h["elapsed"]
     value = h["branches_per_step"] * args.steps / elapsed
-    amps = nq * args.steps / elapsed
+    bw_amps = nq * args.steps / elapsed
     extra = {}
     for name, fn in (("precision", precision), ("batch", batch), ("serve", serve)):
         if name in sections:
             print("bench: section %s" % name, file=sys.stderr, flush=True)
             extra[name] = fn(args, pq, W, np)
     plain, cpu = h.get("plain"), h.get("cpu")
+    segd = dict(h["segment"])
+    segd.pop("got", None)
+    seg_amps = segd["amplitudes_per_s"]
     line = {
         "metric": METRIC,
         "value": value,
@@ -533,25 +591,29 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "int32 Z8 phases + fp32 hi/lo pairs (fp64 tables built once per circuit; fp64 cross-workgroup)",
+        "dtype": "f64 (FP64 FMA accumulation per branch; int32 Z8 phases)",
         "data": "synthetic (seeded SplitMix64 circuits)",
         "config": {
-            "workload": "n=40: H^40; 36 T + 300 CX/CZ + 100 S interleaved (seed %d); H^40; 16 amplitudes <0|C|y>" % SEED,
+            "workload": "n=40: H^40; 36 T + 300 CX/CZ + 100 S interleaved (seed %d); H^40; 16 amplitudes <0|C|y>, "
+                        "every branch visited (per-branch walk)" % SEED,
             "terms": st[4], "branch_dim": d, "free_vars": st[0], "planes": st[1], "lines": st[2], "radical": st[3],
-            "branches_per_step": h["branches_per_step"], "segment": h["seg"],
+            "branches_per_step": h["branches_per_step"], "branch_walk": h["info"],
             "items": h["timing"]["items"], "grid": h["timing"]["grid"], "device": h["dev"],
             "walk_tuning": h["tuning"],
+            "engine_build_ms": 1e3 * h["build_s"], "engine_upload_ms": 1e3 * h["upload_s"],
         },
-        "amplitudes_per_s": amps,
+        "amplitudes_per_s_branch_walk": bw_amps,
+        "amplitudes_per_s": seg_amps,
+        "segment_walk": segd,
         "plain_fp64": plain,
-        "speedup_vs_plain_amps": amps / plain["amplitudes_per_s"] if plain else None,
+        "speedup_vs_plain_amps": seg_amps / plain["amplitudes_per_s"] if plain else None,
+        "speedup_vs_plain_branches": value / plain["branches_per_s"] if plain else None,
         "roofline": h["roofline"],
-        "roofline_saturated": h["roofline_saturated"],
-        "branch_walk": h["branch_walk"],
+        "valu_probe": probe,
         "cpu_baseline": cpu,
         "speedup_vs_cpu": value / cpu["value"] if cpu else None,
         "cpu_auto": h.get("cpu_auto"),
-        "speedup_vs_cpu_auto_amps": amps / h["cpu_auto"]["amplitudes_per_s"] if h.get("cpu_auto") else None,
+        "speedup_vs_cpu_auto_amps": seg_amps / h["cpu_auto"]["amplitudes_per_s"] if h.get("cpu_auto") else None,
     }
     line.update(extra)
     if "serve" in extra:

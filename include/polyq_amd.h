@@ -138,9 +138,11 @@ int pq_graph_run(pq_graph* g, double* out, double* total_ms);
 void pq_graph_destroy(pq_graph* g);
 
 /* ---- request loop (no counterpart in the reference; the serving layer of
- * the GPU strategy).  Circuits are keyed by content, compiled and built on
- * the host once, and stay resident in HBM within hbm_budget_bytes (LRU
- * eviction of idle entries, back-pressure when everything is in flight).
+ * the GPU strategy).  Circuits are keyed by content (hash, then the gate list
+ * compared), compiled and built on the host once, and stay resident in HBM
+ * within hbm_budget_bytes (LRU eviction of idle entries, back-pressure while
+ * requests in flight hold the rest; a request that cannot fit fails with an
+ * "HBM budget" error instead of waiting).
  * Requests that arrive while a batch runs are launched as the next batch on a
  * second stream.  kind: 0 amplitude <y|C|x>, 1 probability |<y|C|x>|^2,
  * 2 marginal probability of the k listed (qubit, outcome) pairs for input x
@@ -159,6 +161,12 @@ int pq_server_wait(pq_server* srv, uint64_t ticket, double timeout_s, double out
  * batch, batches launched while another was in flight, walk-kernel ms (timed
  * on every 8th batch and scaled to all batches), resident circuits. */
 int pq_server_stats(pq_server* srv, double out10[10]);
+/* Host bytes of built engine images kept for idle circuits (default 8 GiB):
+ * past it the least recently used circuits with no request in flight are
+ * dropped (device images included) and rebuilt if they come back. */
+int pq_server_set_host_budget(pq_server* srv, uint64_t bytes);
+/* host image bytes held, host budget, circuits dropped by the host LRU. */
+int pq_server_host_stats(pq_server* srv, double out3[3]);
 
 /* ---- parity and diagnostics (no counterpart in the reference) ---- */
 /* Per-branch Gauss-sum outputs on the first branch engine of the cached
@@ -206,6 +214,16 @@ int pq_sim_seg_selfcheck(pq_sim* sim, uint64_t x, uint64_t y, uint32_t trials, u
  * rank K, whether inner entries alternate real/imaginary, VALU lane-ops per
  * branch of the inner step. */
 int pq_sim_z8_info(pq_sim* sim, uint32_t out7[7]);
+
+/* VALU issue-rate probe (no counterpart in the reference): one kernel per
+ * instruction (int32 xor, and, add, popcount; FP32 FMA; FP64 FMA; one int32
+ * xor per FP64 FMA, the branch walk's mix), each lane running 8 independent
+ * chains of iters x 64 instructions on blocks_per_cu 256-thread workgroups
+ * per CU.  out (3 + 4 x 7 doubles): CUs, wall-clock MHz, blocks per CU, then
+ * per op {lane-ops/s, lanes per clock per CU at the in-kernel clock, that
+ * clock in GHz, kernel ms}.  Returns the number of ops (7) or -1. */
+int pq_valu_probe(double* out, size_t cap, uint32_t blocks_per_cu, uint32_t iters);
+const char* pq_valu_probe_name(int op);
 
 const char* pq_last_error(void);
 /* Device name, architecture and CU count; returns -1 without a device. */

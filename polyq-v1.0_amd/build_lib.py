@@ -14,7 +14,7 @@ OBJ = os.path.join(HERE, "_build")
 OUT = os.path.join(HERE, "polyq_amd", "libpolyq_amd.so")
 
 HOST_SOURCES = ["pq_compile.cpp", "pq_engine.cpp", "pq_z8.cpp", "pq_seg.cpp", "pq_api.cpp", "pq_server.cpp"]
-HIP_SOURCES = ["pq_kernels.hip"]
+HIP_SOURCES = ["pq_kernels.hip", "pq_valu.hip"]
 
 # x86-64-v3: the library built here also runs on the GPU host's CPU.
 HOST_FLAGS = ["-O3", "-std=c++17", "-fPIC", "-march=x86-64-v3", "-Wall", "-Wno-unused-function", "-pthread"]

@@ -21,6 +21,7 @@ std::shared_ptr<RankEng> build_rank(const PathSum& ps, const std::vector<uint32_
     auto add = [&](std::shared_ptr<HostEngine> he) {
         auto z = std::make_shared<Z8Image>(build_z8_image(*he));
         auto g = std::make_shared<SegImage>(build_seg_image(*he));
+        R->image_bytes += dev_engine_image_bytes(*he, *z, *g);
         R->host.push_back(he);
         R->z8.push_back(z);
         R->seg.push_back(g);
@@ -197,20 +198,35 @@ bool Sim::assignment(uint64_t x, uint64_t y, std::vector<std::pair<uint32_t, uin
     return ps.boundary_bits(x, y, asg);
 }
 
-const RankEng& Sim::marginal_engine(const std::vector<uint32_t>& qubits) {
+std::shared_ptr<const RankEng> Sim::marginal_engine(const std::vector<uint32_t>& qubits) {
     std::lock_guard<std::mutex> lk(mu);
     if (!sw) sw = std::make_unique<Sandwich>(compile_sandwich(circ));
     std::vector<uint32_t> key(qubits);
     std::sort(key.begin(), key.end());
     key.erase(std::unique(key.begin(), key.end()), key.end());
     auto it = marg.find(key);
-    if (it != marg.end()) return *it->second;
+    if (it != marg.end()) {
+        it->second.last_use = ++marg_clock;
+        return it->second.r;
+    }
     // Sandwich::fixed_for (sim.rs:95-103)
     std::vector<uint32_t> f = sw->ps.io_vars();
     for (uint32_t q : qubits) if (std::find(f.begin(), f.end(), sw->junction[q]) == f.end()) f.push_back(sw->junction[q]);
     auto R = build_rank(sw->ps, f, strategy, pairs_opt);
-    marg[key] = R;
-    return *R;
+    if (marg.size() >= kMargCap) {
+        auto lru = marg.begin();
+        for (auto i = marg.begin(); i != marg.end(); ++i) if (i->second.last_use < lru->second.last_use) lru = i;
+        marg.erase(lru);
+    }
+    marg[key] = MargSet{R, ++marg_clock};
+    return R;
+}
+
+size_t Sim::host_bytes() {
+    std::lock_guard<std::mutex> lk(mu);
+    size_t b = eng ? eng->image_bytes : 0;
+    for (auto& kv : marg) b += kv.second.r->image_bytes;
+    return b;
 }
 
 bool Sim::marginal_assignment(uint64_t x, const uint32_t* qubits, const uint8_t* outcomes, size_t k, std::vector<std::pair<uint32_t, uint8_t>>& a) {
@@ -322,7 +338,8 @@ int pq_sim_marginal_probability(pq_sim* h, uint64_t x, const uint32_t* qubits, c
     PQ_TRY({
         for (size_t i = 0; i < k; ++i) if (qubits[i] >= h->s.ps.nq) throw std::invalid_argument("measured qubit out of range");
         std::vector<uint32_t> qs(qubits, qubits + k);
-        const RankEng& R = h->s.marginal_engine(qs);
+        const auto Rp = h->s.marginal_engine(qs);
+        const RankEng& R = *Rp;
         std::vector<std::pair<uint32_t, uint8_t>> a;
         if (!h->s.marginal_assignment(x, qubits, outcomes, k, a)) { *out = 0; return 0; }
         std::complex<double> v(0, 0);
@@ -492,7 +509,8 @@ int pq_sim_sample_many(pq_sim* h, uint64_t x, size_t shots, uint64_t seed, int f
         std::vector<std::pair<uint32_t, uint8_t>> a;
         for (uint32_t l = 0; l < n; ++l) {
             qs.push_back(l);
-            const RankEng& R = h->s.marginal_engine(qs);   // prefix of l + 1 qubits
+            const auto Rp = h->s.marginal_engine(qs);       // prefix of l + 1 qubits
+            const RankEng& R = *Rp;
             // distinct prefixes y & (2^l - 1), with qubit l = 0
             const uint64_t mask = l == 0 ? 0 : (l >= 64 ? ~0ull : ((1ull << l) - 1));
             std::map<uint64_t, size_t> slot;
@@ -687,7 +705,8 @@ int pq_sim_marginal_stats(pq_sim* h, const uint32_t* qubits, size_t k, uint64_t*
     PQ_TRY({
         for (size_t i = 0; i < k; ++i) if (qubits[i] >= h->s.ps.nq) throw std::invalid_argument("measured qubit out of range");
         std::vector<uint32_t> qs(qubits, qubits + k);
-        const RankEng& R = h->s.marginal_engine(qs);
+        const auto Rp = h->s.marginal_engine(qs);
+        const RankEng& R = *Rp;
         EngineStats s = R.host[0]->stats();
         o[0] = s.free_vars; o[1] = s.planes; o[2] = s.lines; o[3] = s.radical; o[4] = s.terms; o[5] = s.branch_dim; o[6] = R.pairs;
         return 0;

@@ -18,6 +18,7 @@ struct RankEng {
     mutable std::vector<std::shared_ptr<DevEngine>> dev;
     mutable std::mutex dev_mu;
     uint32_t pairs = 0;
+    size_t image_bytes = 0;        // device bytes the upload of every engine takes (set by build_rank)
     // boundary map of the amplitude engines (fixed = the circuit's io vars)
     bool has_bprog = false;
     BoundaryProg bprog;
@@ -48,22 +49,31 @@ struct Sim {
     uint32_t pairs_opt = 0;
     std::mutex mu;
     std::unique_ptr<Sandwich> sw;
-    std::map<std::vector<uint32_t>, std::shared_ptr<RankEng>> marg;
+    // Marginal (sandwich) engine sets per measured-qubit set, least recently
+    // used dropped beyond kMargCap sets.  Callers hold the shared_ptr while
+    // they use a set, so a dropped set lives until its last user is done.
+    static constexpr size_t kMargCap = 32;
+    struct MargSet { std::shared_ptr<RankEng> r; uint64_t last_use = 0; };
+    std::map<std::vector<uint32_t>, MargSet> marg;
+    uint64_t marg_clock = 0;
 
     bool assignment(uint64_t x, uint64_t y, std::vector<std::pair<uint32_t, uint8_t>>& asg) const;
     size_t device_bytes() {
         std::lock_guard<std::mutex> lk(mu);
         size_t b = eng ? eng->device_bytes() : 0;
-        for (auto& kv : marg) b += kv.second->device_bytes();
+        for (auto& kv : marg) b += kv.second.r->device_bytes();
         return b;
     }
+    // Host bytes of the built images (the engine set and the cached marginal
+    // sets), the size their device images take.
+    size_t host_bytes();
     // Free every device image (host engines stay; the next use re-uploads).
     void drop_device() {
         std::lock_guard<std::mutex> lk(mu);
         if (eng) eng->drop_device();
-        for (auto& kv : marg) kv.second->drop_device();
+        for (auto& kv : marg) kv.second.r->drop_device();
     }
-    const RankEng& marginal_engine(const std::vector<uint32_t>& qubits);
+    std::shared_ptr<const RankEng> marginal_engine(const std::vector<uint32_t>& qubits);
     bool marginal_assignment(uint64_t x, const uint32_t* qubits, const uint8_t* outcomes, size_t k, std::vector<std::pair<uint32_t, uint8_t>>& a);
 };
 

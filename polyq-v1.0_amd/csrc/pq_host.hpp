@@ -248,12 +248,19 @@ struct Z8Image {
     std::vector<uint32_t> slot_pin;    // inner-touched pivot bits of N, compacted
     std::vector<uint8_t> slot_lam;     // query-independent linear coefficient
     std::vector<uint8_t> slot_real;    // slot holds a direction (not a phantom)
-    std::vector<float> inner;          // 256 signed values, 256 x pin*4 (bits), 256 x kin (bits)
-    std::vector<float> lt;             // nbytes x 256 lane-factor byte tables
-    std::vector<float> gin;            // 2^npin inner pivot factor table
+    std::vector<double> inner;         // 256 signed FP64 values W[i] (entry i covers c_in = gray(i))
+    // The same 256 values in the order of the accumulator walk (no inner
+    // pivots): entry 8 q + l covers c_in = (gray5(q) << 3) | l.
+    std::vector<double> inner_acc;
+    std::vector<uint32_t> inner_aux;   // 256 x pin*8 (LDS byte offset of the inner pivots), then 256 x kin
+    std::vector<double> lt;            // nbytes x 256 lane-factor byte tables (FP64)
+    std::vector<double> gin;           // 2^npin inner pivot factor table (FP64)
     std::vector<uint32_t> pin_term;    // term index of each compact inner pivot bit
     uint64_t lane_terms = 0;           // terms whose ratio is carried by the lane factor
-    uint32_t ops_per_branch() const { return npin ? 4u : 2u; }
+    // VALU lane-ops per branch of the inner step: with inner pivots a sign
+    // XOR, an address XOR, an LDS read and one FP64 FMA; without, one FP64 FMA
+    // (the sign is applied once per 8 branches).
+    uint32_t ops_per_branch() const { return npin ? 4u : 1u; }
 };
 Z8Image build_z8_image(const HostEngine& E);
 // Query part of the Z8 walk: K0, delta bits over slots, s0, compact pivots.

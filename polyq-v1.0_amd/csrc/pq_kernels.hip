@@ -11,10 +11,10 @@
 //                 the outer coordinates, walks the mid coordinates in Gray
 //                 order by single-slot updates of its integer state, and sums
 //                 the 256 inner branches of each outer point from a uniform
-//                 table (scalar loads): per branch one sign flip and two FP32
-//                 FMAs (plus one LDS lookup when inner directions touch
-//                 pivots).  Partials fold into per-lane two-sum compensated
-//                 FP32 accumulators split by the parity of the lane phase.
+//                 FP64 table (scalar loads): per branch one sign flip and one
+//                 FP64 FMA (plus one LDS lookup when inner directions touch
+//                 pivots).  Every branch is visited; partials accumulate in
+//                 FP64, split by the parity of the lane phase.
 //  k_z8_reduce    FP64 cross-workgroup reduction and global scaling.
 //  k_dump_s, k_dump_walk   per-branch outputs for parity tests.
 //
@@ -130,9 +130,11 @@ struct Z8Dev {
     const uint64_t* slot_n;
     const uint32_t* slot_pin;
     const uint32_t* slot_lam;
-    const float* inner;      // 256 signed values, then 256 x pin*4, then 256 x kin
-    const float* lt;         // nbytes x 256
-    const float* gin;        // 2^npin
+    const double* inner;     // 256 signed FP64 values W[i]
+    const double* inner_acc; // the same in accumulator-walk order
+    const uint32_t* inaux;   // 256 x pin*8, then 256 x kin
+    const double* lt;        // nbytes x 256
+    const double* gin;       // 2^npin
     uint32_t nslots, nbytes, npin, imag0;
 };
 struct Z8Q {                 // one query of a batch (64 B)
@@ -217,7 +219,7 @@ uint32_t pick_nw(uint32_t bits) {
 
 namespace {
 struct BlobLayout {
-    size_t o_ku, o_kn, o_soff, o_supp, o_a0, o_a1, o_up, o_dn, o_diag, o_mrow, o_we, o_sq, o_sn, o_sp, o_sl, o_in, o_lt, o_gin;
+    size_t o_ku, o_kn, o_soff, o_supp, o_a0, o_a1, o_up, o_dn, o_diag, o_mrow, o_we, o_sq, o_sn, o_sp, o_sl, o_in, o_inacc, o_inaux, o_lt, o_gin;
     size_t g_oq, g_ox, g_opin, g_olam, g_on, g_tab, g_lt, g_fmid, g_rmid, g_wtab, g_pts, g_pcls, g_low;
     uint32_t NW, NWe;
 };
@@ -269,8 +271,9 @@ BlobLayout build_blob(const HostEngine& E, const Z8Image& Z, const SegImage& G, 
     std::vector<uint32_t> lam32(Z.slot_lam.begin(), Z.slot_lam.end());
     O.o_sq = B.put(Z.slot_q.data(), Z.slot_q.size() * 8); O.o_sn = B.put(Z.slot_n.data(), Z.slot_n.size() * 8);
     O.o_sp = B.put(Z.slot_pin.data(), Z.slot_pin.size() * 4); O.o_sl = B.put(lam32.data(), lam32.size() * 4);
-    O.o_in = B.put(Z.inner.data(), Z.inner.size() * 4); O.o_lt = B.put(Z.lt.data(), Z.lt.size() * 4);
-    O.o_gin = B.put(Z.gin.data(), Z.gin.size() * 4);
+    O.o_in = B.put(Z.inner.data(), Z.inner.size() * 8); O.o_inacc = B.put(Z.inner_acc.data(), Z.inner_acc.size() * 8);
+    O.o_inaux = B.put(Z.inner_aux.data(), Z.inner_aux.size() * 4);
+    O.o_lt = B.put(Z.lt.data(), Z.lt.size() * 8); O.o_gin = B.put(Z.gin.data(), Z.gin.size() * 8);
     // segment-table image
     O.g_oq = B.put(G.oq.data(), G.oq.size() * 8); O.g_ox = B.put(G.ox.data(), G.ox.size() * 4);
     O.g_opin = B.put(G.opin.data(), G.opin.size() * 4); O.g_olam = B.put(G.olam.data(), G.olam.size() * 4);
@@ -318,7 +321,8 @@ std::shared_ptr<DevEngine> dev_engine_upload(std::shared_ptr<const HostEngine> E
     P.rbits = 2 * L.g + 2 * L.neven + L.nodd; P.neg = E.neg_mask;
     Z8Dev& z = D->z;
     z.slot_q = (const uint64_t*)at(O.o_sq); z.slot_n = (const uint64_t*)at(O.o_sn); z.slot_pin = (const uint32_t*)at(O.o_sp); z.slot_lam = (const uint32_t*)at(O.o_sl);
-    z.inner = (const float*)at(O.o_in); z.lt = (const float*)at(O.o_lt); z.gin = (const float*)at(O.o_gin);
+    z.inner = (const double*)at(O.o_in); z.inner_acc = (const double*)at(O.o_inacc); z.inaux = (const uint32_t*)at(O.o_inaux);
+    z.lt = (const double*)at(O.o_lt); z.gin = (const double*)at(O.o_gin);
     z.nslots = Z.nslots; z.nbytes = Z.nbytes; z.npin = Z.npin; z.imag0 = Z.imag0 ? 1u : 0u;
     SegDev& g = D->sg;
     g.oq = (const uint64_t*)at(O.g_oq); g.ox = (const uint32_t*)at(O.g_ox); g.opin = (const uint32_t*)at(O.g_opin);
@@ -601,13 +605,6 @@ template <class T> __device__ __forceinline__ T cload(const T* p) {
     return out;
 }
 
-__device__ __forceinline__ void two_sum(float& s, float& c, float x) {
-    const float t = s + x;
-    const float bp = t - s;
-    const float e = (s - (t - bp)) + (x - bp);
-    s = t;
-    c += e;
-}
 __device__ __forceinline__ void merge2(float& s, float& c, float s2, float c2) {
     const float t = s + s2;
     const float bp = t - s;
@@ -640,20 +637,20 @@ __device__ __forceinline__ void wave_sum_dpp(float (&v)[8]) {
     dpp_merge<0x143, 0xC>(v);    // row_bcast:31
 }
 
-// Rotate (xr, xi) by i^((K>>1)&3) and add into the even or odd accumulator
-// (both pairs are updated, the other with zero, so the accumulator index stays
-// static and the array stays in registers).
-__device__ __forceinline__ void fold(float xr, float xi, uint32_t K, float (&acc)[8]) {
+// Rotate (xr, xi) by i^((K>>1)&3) and add it, in FP64, into the even or odd
+// accumulator (both are updated, the other with zero, so the accumulator
+// index stays static and the array stays in registers).
+__device__ __forceinline__ void fold64(double xr, double xi, uint32_t K, double (&acc)[4]) {
     const uint32_t r2 = (K >> 1) & 3u;
-    const float a = (r2 & 1u) ? xi : xr;
-    const float b = (r2 & 1u) ? xr : xi;
-    const float yr = (r2 == 1u || r2 == 2u) ? -a : a;
-    const float yi = (r2 >= 2u) ? -b : b;
+    const double a = (r2 & 1u) ? xi : xr;
+    const double b = (r2 & 1u) ? xr : xi;
+    const double yr = (r2 == 1u || r2 == 2u) ? -a : a;
+    const double yi = (r2 >= 2u) ? -b : b;
     const bool odd = K & 1u;
-    two_sum(acc[0], acc[1], odd ? 0.f : yr);
-    two_sum(acc[2], acc[3], odd ? 0.f : yi);
-    two_sum(acc[4], acc[5], odd ? yr : 0.f);
-    two_sum(acc[6], acc[7], odd ? yi : 0.f);
+    acc[0] += odd ? 0.0 : yr;
+    acc[1] += odd ? 0.0 : yi;
+    acc[2] += odd ? yr : 0.0;
+    acc[3] += odd ? yi : 0.0;
 }
 
 // Lane state over the outer coordinates: integer phase K at c_in = 0, the
@@ -692,73 +689,101 @@ __device__ __forceinline__ void flip_mid(const Z8Dev& E, const Z8Q& Q, uint32_t 
     S.pio ^= cview(E.slot_pin)[p];
     S.sout ^= cview(E.slot_n)[p];
 }
-__device__ __forceinline__ float lane_factor(const float* s_lt, uint32_t nbytes, uint64_t sout) {
-    float F = 1.f;
+__device__ __forceinline__ double lane_factor(const double* s_lt, uint32_t nbytes, uint64_t sout) {
+    double F = 1.0;
     for (uint32_t b = 0; b < nbytes; ++b) F *= s_lt[(b << 8) | (uint32_t)((sout >> (8 * b)) & 0xFFu)];
     return F;
 }
 
-// The 256 inner branches of one outer point, folded every FOLD steps.  Each
-// table entry is real or imaginary; with IMAG the class is bit 0 of gray(i)
-// (the builder arranges the inner basis so), without it every entry is real.
-// Per branch: one XOR for the sign (and, with HASG, an LDS lookup of the
-// inner pivot factor) and one FMA into the accumulator of the entry's class.
-// The uniform table is read 16 steps at a time as wide scalar loads.
-template <bool HASG, bool IMAG, int FOLD>
-__device__ __forceinline__ void inner_walk(const float* Wg, const uint32_t (&ps)[8], const float* s_gin, const LaneState& S, float F, float (&acc)[8]) {
-    const PQ_CONST float* Wv = cview(Wg);                                    // 256 signed values
-    uint32_t sm[8];
+// The 256 inner branches of one outer point.  Each table entry is real or
+// imaginary; with IMAG the class is bit 0 of c_in (the builder arranges the
+// inner basis so), without it every entry is real.  The sign of branch c_in is
+// (-1)^|c_in & hs|, hs being the lane's inner sign mask.
+//
+// Without inner pivots (HASG false) the walk keeps 8 accumulators, one per
+// pattern l of the three low inner slots, and walks the five high slots in
+// Gray order: per branch one FP64 FMA of the +-1 multiplier of the high slots
+// with the uniform table entry (16-dword scalar loads, 8 entries per step),
+// and one XOR on the multiplier's sign word per 8 branches.  The low slots'
+// signs are applied once, when the 8 accumulators are combined (7 FMAs).
+//
+// With inner pivots every branch also reads its pivot factor
+// gin[pio ^ pin(c_in)] from LDS (the address follows the same Gray recurrence
+// as the sign) and flips that factor's sign word: per branch a sign XOR, an
+// address XOR, an LDS read and one FP64 FMA.
+template <bool HASG, bool IMAG>
+__device__ __forceinline__ void inner_walk(const Z8Dev& E, const uint32_t (&ps)[8], const double* s_gin, const LaneState& S, double F,
+                                           double (&acc)[4]) {
+    double tr = 0.0, ti = 0.0;
+    if constexpr (!HASG) {
+        const PQ_CONST double* Wv = cview(E.inner_acc);
+        uint32_t sm[5];
 #pragma unroll
-    for (int j = 0; j < 8; ++j) sm[j] = (S.hs << (31 - j)) & 0x80000000u;
-    uint32_t sgn = 0;
-    uint32_t addr = S.pio << 2;      // LDS byte offset of gin[pio ^ pin(gray(i))], same Gray recurrence as the sign
-    float tr = 0.f, ti = 0.f;
+        for (int j = 0; j < 5; ++j) sm[j] = (S.hs << (28 - j)) & 0x80000000u;      // bit 3 + j of hs
+        double m = 1.0;
+        double a[8] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
 #pragma unroll
-    for (int blk = 0; blk < 16; ++blk) {
-        float w[16];
-        // An empty asm on the block pointer keeps each block's scalar loads
-        // next to their use: the table lives in the constant address space,
-        // so otherwise all 256 entries are hoisted and the SGPRs spill.
-        const PQ_CONST float* bw = Wv + 16 * blk;
-        asm volatile("" : "+s"(bw) : : "memory");
+        for (int q = 0; q < 32; ++q) {
+            if (q) m = __hiloint2double(__double2hiint(m) ^ (int)sm[__builtin_ctz(q)], __double2loint(m));
+            // An empty asm on the block pointer keeps each step's scalar load
+            // next to its use (the table is in the constant address space,
+            // so otherwise all 256 entries are hoisted and the SGPRs spill).
+            const PQ_CONST double* bw = Wv + 8 * q;
+            asm volatile("" : "+s"(bw) : : "memory");
+            double w[8];
 #pragma unroll
-        for (int k = 0; k < 16; ++k) w[k] = bw[k];
+            for (int l = 0; l < 8; ++l) w[l] = bw[l];
 #pragma unroll
-        for (int ii = 0; ii < 16; ++ii) {
-            const int i = 16 * blk + ii;
-            if (i) {
-                sgn ^= sm[__builtin_ctz(i)];
-                if constexpr (HASG) addr ^= ps[__builtin_ctz(i)];
-            }
-            float m;
-            if constexpr (HASG) {
-                const float g = *(const float*)((const char*)s_gin + addr);
-                m = __uint_as_float(__float_as_uint(g) ^ sgn);
-            } else {
-                m = __uint_as_float(0x3f800000u ^ sgn);
-            }
-            if (IMAG && ((i ^ (i >> 1)) & 1)) ti = fmaf(m, w[ii], ti);
-            else tr = fmaf(m, w[ii], tr);
-            if (((i + 1) % FOLD) == 0) {
-                fold(tr * F, ti * F, S.K, acc);
-                tr = 0.f; ti = 0.f;
+            for (int l = 0; l < 8; ++l) a[l] = fma(m, w[l], a[l]);
+        }
+        const double s0 = (S.hs & 1u) ? -1.0 : 1.0, s1 = (S.hs & 2u) ? -1.0 : 1.0, s2 = (S.hs & 4u) ? -1.0 : 1.0;
+        const double ev = fma(s2, fma(s1, a[6], a[4]), fma(s1, a[2], a[0]));          // l = 0, 2, 4, 6
+        const double od = fma(s2, fma(s1, a[7], a[5]), fma(s1, a[3], a[1]));          // l = 1, 3, 5, 7
+        if constexpr (IMAG) { tr = ev; ti = s0 * od; }
+        else tr = fma(s0, od, ev);
+    } else {
+        const PQ_CONST double* Wv = cview(E.inner);
+        uint32_t sm[8];
+#pragma unroll
+        for (int j = 0; j < 8; ++j) sm[j] = (S.hs << (31 - j)) & 0x80000000u;
+        uint32_t sgn = 0;
+        uint32_t addr = S.pio << 3;      // LDS byte offset of gin[pio ^ pin(gray(i))]
+#pragma unroll
+        for (int blk = 0; blk < 16; ++blk) {
+            double w[16];
+            const PQ_CONST double* bw = Wv + 16 * blk;
+            asm volatile("" : "+s"(bw) : : "memory");
+#pragma unroll
+            for (int k = 0; k < 16; ++k) w[k] = bw[k];
+#pragma unroll
+            for (int ii = 0; ii < 16; ++ii) {
+                const int i = 16 * blk + ii;
+                if (i) {
+                    sgn ^= sm[__builtin_ctz(i)];
+                    addr ^= ps[__builtin_ctz(i)];
+                }
+                const double g = *(const double*)((const char*)s_gin + addr);
+                const double m = __hiloint2double(__double2hiint(g) ^ (int)sgn, __double2loint(g));
+                if (IMAG && ((i ^ (i >> 1)) & 1)) ti = fma(m, w[ii], ti);
+                else tr = fma(m, w[ii], tr);
             }
         }
     }
+    fold64(tr * F, ti * F, S.K, acc);
 }
 
 // One instantiation per inner-step variant (pivot lookups or not, imaginary
 // class pattern or not); a batch launches one persistent grid per variant
 // present, each over its own contiguous range of queries and items.
-template <int FOLD, bool HASG, bool IMAG>
+template <bool HASG, bool IMAG>
 __global__ __launch_bounds__(256) void k_z8_walk(const Z8Dev* __restrict__ engs, const Z8Q* __restrict__ qs, uint32_t q_lo, uint32_t q_hi,
                                                  uint64_t item_begin, uint64_t item_end, unsigned long long* __restrict__ counter,
                                                  double* __restrict__ partial) {
-    __shared__ float s_lt[8 * 256];
-    __shared__ float s_gin[4096];
+    __shared__ double s_lt[8 * 256];
+    __shared__ double s_gin[HASG ? 4096 : 1];
     __shared__ unsigned long long s_item;
     __shared__ int s_eng;
-    __shared__ float s_red[4][8];
+    __shared__ double s_red[4][4];
     const uint32_t tid = threadIdx.x;
     if (tid == 0) s_eng = -1;
     for (;;) {
@@ -777,43 +802,33 @@ __global__ __launch_bounds__(256) void k_z8_walk(const Z8Dev* __restrict__ engs,
             __syncthreads();
             if (tid == 0) s_eng = (int)Q.eng;
         }
-        float acc[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+        double acc[4] = {0.0, 0.0, 0.0, 0.0};
         const uint64_t lane = (item - Q.item0) * 256 + tid;
         if ((lane >> Q.H) == 0) {
             LaneState S;
             seed_lane(E, Q, lane, S);
             uint32_t ps[8];
 #pragma unroll
-            for (int j = 0; j < 8; ++j) ps[j] = HASG ? (cview(E.slot_pin)[j] << 2) : 0u;
+            for (int j = 0; j < 8; ++j) ps[j] = HASG ? (cview(E.slot_pin)[j] << 3) : 0u;
             const uint32_t nout = 1u << Q.M;
             for (uint32_t o = 0; o < nout; ++o) {
                 if (o) flip_mid(E, Q, 8u + (uint32_t)__builtin_ctz(o), S);
-                const float F = lane_factor(s_lt, E.nbytes, S.sout);
-                inner_walk<HASG, IMAG, FOLD>(E.inner, ps, s_gin, S, F, acc);
+                const double F = lane_factor(s_lt, E.nbytes, S.sout);
+                inner_walk<HASG, IMAG>(E, ps, s_gin, S, F, acc);
             }
         }
-        // workgroup reduction of the four compensated pairs (FP32 two-sum)
+        // workgroup reduction in FP64, in a fixed order
 #pragma unroll
         for (int off = 32; off > 0; off >>= 1) {
 #pragma unroll
-            for (int k = 0; k < 4; ++k) {
-                const float s2 = __shfl_xor(acc[2 * k], off), c2 = __shfl_xor(acc[2 * k + 1], off);
-                merge2(acc[2 * k], acc[2 * k + 1], s2, c2);
-            }
+            for (int k = 0; k < 4; ++k) acc[k] += __shfl_xor(acc[k], off);
         }
         if ((tid & 63) == 0) {
 #pragma unroll
-            for (int k = 0; k < 8; ++k) s_red[tid >> 6][k] = acc[k];
+            for (int k = 0; k < 4; ++k) s_red[tid >> 6][k] = acc[k];
         }
         __syncthreads();
-        if (tid == 0) {
-#pragma unroll
-            for (int k = 0; k < 4; ++k) {
-                float s = s_red[0][2 * k], c = s_red[0][2 * k + 1];
-                for (int w = 1; w < 4; ++w) merge2(s, c, s_red[w][2 * k], s_red[w][2 * k + 1]);
-                partial[4 * item + k] = (double)s + (double)c;
-            }
-        }
+        if (tid < 4) partial[4 * item + tid] = (s_red[0][tid] + s_red[1][tid]) + (s_red[2][tid] + s_red[3][tid]);
     }
 }
 
@@ -1567,8 +1582,8 @@ uint32_t grid_for(Ctx& C, K kernel, uint64_t items) {
 template <bool HASG, bool IMAG>
 uint32_t launch_v(Ctx& C, hipStream_t st, const BatchPlan& B, int v, const Z8Dev* de, const Z8Q* dq, unsigned long long* ctr, double* part) {
     const uint64_t items = B.it_hi[v] - B.it_lo[v];
-    const uint32_t grid = grid_for(C, k_z8_walk<256, HASG, IMAG>, items);
-    hipLaunchKernelGGL((k_z8_walk<256, HASG, IMAG>), dim3(grid), dim3(256), 0, st, de, dq, B.q_lo[v], B.q_hi[v], B.it_lo[v], B.it_hi[v], ctr + v, part);
+    const uint32_t grid = grid_for(C, k_z8_walk<HASG, IMAG>, items);
+    hipLaunchKernelGGL((k_z8_walk<HASG, IMAG>), dim3(grid), dim3(256), 0, st, de, dq, B.q_lo[v], B.q_hi[v], B.it_lo[v], B.it_hi[v], ctr + v, part);
     HIPCHECK(hipGetLastError());
     return grid;
 }
@@ -2068,8 +2083,8 @@ void dev_dump_s(const DevEngine& D, const Prepared& P, const uint64_t* s, size_t
 
 // Every branch of the walk for one query, with the walk's own lane state.
 __global__ __launch_bounds__(256) void k_dump_walk(Z8Dev E, Z8Q Q, uint8_t* __restrict__ kout, float* __restrict__ mout) {
-    __shared__ float s_lt[8 * 256];
-    __shared__ float s_gin[4096];
+    __shared__ double s_lt[8 * 256];
+    __shared__ double s_gin[4096];
     const uint32_t tid = threadIdx.x;
     for (uint32_t k = tid; k < E.nbytes * 256u; k += 256) s_lt[k] = E.lt[k];
     if (E.npin) for (uint32_t k = tid; k < (1u << E.npin); k += 256) s_gin[k] = E.gin[k];
@@ -2081,18 +2096,18 @@ __global__ __launch_bounds__(256) void k_dump_walk(Z8Dev E, Z8Q Q, uint8_t* __re
     const uint32_t nout = 1u << Q.M;
     for (uint32_t o = 0; o < nout; ++o) {
         if (o) flip_mid(E, Q, 8u + (uint32_t)__builtin_ctz(o), S);
-        const float F = lane_factor(s_lt, E.nbytes, S.sout);
+        const double F = lane_factor(s_lt, E.nbytes, S.sout);
         for (uint32_t i = 0; i < 256; ++i) {
             const uint32_t g = i ^ (i >> 1);
-            const float wv = E.inner[i];
-            const uint32_t pin4 = __float_as_uint(E.inner[256 + i]), kin = __float_as_uint(E.inner[512 + i]);
+            const double wv = E.inner[i];
+            const uint32_t pin8 = E.inaux[i], kin = E.inaux[256 + i];
             const uint64_t idx = (((lane << Q.M) + o) << 8) + i;
-            if (wv == 0.f) { kout[idx] = 255; mout[idx] = 0.f; continue; }
+            if (wv == 0.0) { kout[idx] = 255; mout[idx] = 0.f; continue; }
             const uint32_t par = (uint32_t)__popc(g & S.hs) & 1u;
             const uint32_t k = (S.K + kin + 4u * par) & 7u;
-            const float gi = E.npin ? s_gin[(S.pio ^ (pin4 >> 2))] : 1.f;
+            const double gi = E.npin ? s_gin[(S.pio ^ (pin8 >> 3))] : 1.0;
             kout[idx] = (uint8_t)k;
-            mout[idx] = F * gi * fabsf(wv);
+            mout[idx] = (float)(F * gi * fabs(wv));
         }
     }
 }

@@ -1,10 +1,13 @@
 // Request loop of the branch evaluator: resident engines in HBM, admission by
 // device memory, continuous batching.
 //
-//  * Circuits are keyed by a content hash; an engine set is compiled and built
-//    on the host once (src/sim.rs:144-148 + src/rank.rs:88-123), uploaded and
-//    kept resident until evicted (LRU, only when the HBM budget is exceeded
-//    and the entry has no request in flight).
+//  * Circuits are keyed by content (a hash, confirmed by comparing the gate
+//    list); an engine set is compiled and built on the host once
+//    (src/sim.rs:144-148 + src/rank.rs:88-123), uploaded and kept resident
+//    until evicted (LRU, only when the HBM budget is exceeded and the entry
+//    has no request in flight).  Host builds are bounded too: past the host
+//    byte budget the least recently used idle circuits are dropped and are
+//    rebuilt if they come back.
 //  * Worker threads run every request's host prologue (boundary, prepare,
 //    Z8 query bits) and any build it needs, off the dispatch path.
 //  * The dispatcher keeps up to two batches in flight on separate streams:
@@ -28,12 +31,16 @@ namespace {
 
 double now_s() { return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count(); }
 
-// One circuit: the host build is kept for the server's lifetime, the device
-// images only while resident.  `pinned` counts requests whose device jobs
-// exist (prepared, queued or running); only unpinned entries are evictable,
-// and requests still waiting for a worker pin nothing.
+// One circuit: the host build is kept while the entry is in the table (host
+// LRU), the device images only while resident.  `pinned` counts requests whose
+// device jobs exist (prepared, queued or running); only unpinned entries are
+// evictable, and requests still waiting for a worker pin nothing.
 struct Entry {
     uint64_t key = 0;
+    Circuit circ;              // the content the key stands for
+    int strategy = 0;
+    bool in_table = true;      // false once the host LRU dropped it
+    size_t host_bytes = 0;     // host images accounted against the host budget
     std::mutex build_mu;
     bool built = false;
     std::string err;
@@ -55,10 +62,20 @@ struct Req {
     int strategy = PQ_CACHED;
     double t_submit = 0, t_done = 0;
     std::vector<Z8Job> jobs;       // prepared device work
+    std::shared_ptr<const RankEng> keep;   // the engine set the jobs point into (marginals)
     std::complex<double> val{0, 0};
     int status = 0;                // 0 pending, 1 done, -1 failed
     std::string err;
 };
+
+bool same_circuit(const Circuit& a, const Circuit& b) {
+    if (a.n != b.n || a.gates.size() != b.gates.size()) return false;
+    for (size_t i = 0; i < a.gates.size(); ++i) {
+        const Gate &x = a.gates[i], &y = b.gates[i];
+        if (x.op != y.op || x.a != y.a || x.b != y.b || x.ph != y.ph) return false;
+    }
+    return true;
+}
 
 uint64_t circuit_key(const Circuit& c, int strategy) {
     uint64_t h = 1469598103934665603ull;
@@ -72,6 +89,12 @@ uint64_t circuit_key(const Circuit& c, int strategy) {
 
 struct Server {
     uint64_t budget;
+    uint64_t host_budget = 8ull << 30;   // host bytes of built engine images kept in the table
+    std::atomic<uint64_t> host_resident{0};
+    uint64_t host_drops = 0;
+    // requests whose jobs are queued for or running on the device: they
+    // finish (and unpin) without further admission
+    uint64_t inflight = 0;
     int form;
     int walk;
     std::atomic<uint64_t> resident{0};
@@ -105,29 +128,77 @@ struct Server {
     }
 
     // ---- admission: make room for `bytes` of engine images in the HBM budget.
-    // Called with the requesting entry pinned, so it is never its own victim.
-    bool admit(size_t bytes) {
+    // Called with the requesting entry `self` pinned, so it is never its own
+    // victim.  Gives up (false) when the request can never fit (its own
+    // resident images plus `bytes` exceed the budget), when every other
+    // resident entry is pinned and no queued or running request will release
+    // a pin, or after kAdmitWait seconds of back-pressure.
+    static constexpr double kAdmitWait = 30.0;
+    bool admit(size_t bytes, const Entry* self) {
         std::unique_lock<std::mutex> lk(mu);
+        const double t0 = now_s();
         for (;;) {
+            if (self->bytes + bytes > budget || stop) return false;
             if (resident.load() + bytes <= budget) { resident += bytes; return true; }
             // evict the least recently used resident entry that nothing pins
             std::shared_ptr<Entry> victim;
             for (auto& kv : entries) {
                 auto& e = kv.second;
-                if (!e->resident || e->pinned > 0) continue;
+                if (!e->resident || e->pinned > 0 || e.get() == self) continue;
                 if (!victim || e->last_use < victim->last_use) victim = e;
             }
             if (victim) {
-                victim->sim->drop_device();
-                resident -= victim->bytes;
-                victim->bytes = 0;
-                victim->resident = false;
+                drop_device_locked(*victim);
                 evictions++;
                 continue;
             }
-            if (bytes > budget || stop) return false;      // can never fit / shutting down
+            if (inflight == 0 || now_s() - t0 > kAdmitWait) return false;
             cv_mem.wait_for(lk, std::chrono::milliseconds(5));   // back-pressure until a batch retires
         }
+    }
+
+    void drop_device_locked(Entry& e) {     // with mu held, e unpinned
+        if (e.sim) e.sim->drop_device();
+        resident -= e.bytes;
+        e.bytes = 0;
+        e.resident = false;
+    }
+
+    // Host LRU: drop the least recently used idle entries from the table while
+    // the host images exceed the host budget.  A dropped entry lives on while
+    // requests still hold it; its device images go when its last pin does.
+    void trim_host_locked(const Entry* keep) {
+        while (host_resident.load() > host_budget) {
+            std::shared_ptr<Entry> victim;
+            for (auto& kv : entries) {
+                auto& e = kv.second;
+                if (e.get() == keep || e->pinned > 0 || !e->built) continue;
+                if (!victim || e->last_use < victim->last_use) victim = e;
+            }
+            if (!victim) return;
+            if (victim->resident) drop_device_locked(*victim);
+            host_resident -= victim->host_bytes;
+            victim->host_bytes = 0;
+            victim->in_table = false;
+            entries.erase(victim->key);
+            host_drops++;
+        }
+    }
+
+    // The entry's host images as they are now (engine set and cached marginal
+    // sets), accounted against the host budget.
+    void account_host(const std::shared_ptr<Entry>& E) {
+        size_t hb = 0;
+        {
+            std::lock_guard<std::mutex> bl(E->build_mu);
+            if (E->sim) hb = E->sim->host_bytes();
+        }
+        std::lock_guard<std::mutex> lk(mu);
+        if (!E->in_table) return;
+        host_resident += hb;
+        host_resident -= E->host_bytes;
+        E->host_bytes = hb;
+        trim_host_locked(E.get());
     }
 
     // Host build once per circuit (src/sim.rs:144-148, src/rank.rs:88-123).
@@ -143,19 +214,15 @@ struct Server {
             s->eng->bprog = boundary_prog(s->ps, *s->eng->host[0]);
             s->eng->has_bprog = true;
             E->sim = std::move(s);
-            E->built = true;
             std::lock_guard<std::mutex> lk(mu);
+            E->built = true;
             builds++;
         } catch (std::exception& e) {
             E->err = e.what();
         }
     }
 
-    static size_t estimate_bytes(const RankEng& R) {
-        size_t b = 0;
-        for (size_t i = 0; i < R.host.size(); ++i) b += dev_engine_image_bytes(*R.host[i], *R.z8[i], *R.seg[i]);
-        return b;
-    }
+    static size_t estimate_bytes(const RankEng& R) { return R.image_bytes; }
 
     // Upload the engine set `R` of entry E if it is not on the device,
     // within the budget.  E is pinned by the caller.
@@ -163,23 +230,32 @@ struct Server {
         std::lock_guard<std::mutex> bl(E->build_mu);
         if (R.dev.size() == R.host.size()) return;
         const size_t est = estimate_bytes(R);
-        if (!admit(est)) throw std::runtime_error("engine does not fit the HBM budget");
+        if (!admit(est, E.get())) throw std::runtime_error("engine does not fit the HBM budget");
         try {
             R.ensure_device();
         } catch (...) {
             resident -= est;
             throw;
         }
-        const size_t real = R.device_bytes();
+        // the entry's device bytes as they are now (marginal sets the
+        // simulator's cache dropped no longer count)
+        const size_t now_bytes = E->sim->device_bytes();
         std::lock_guard<std::mutex> lk(mu);
-        resident += real;
         resident -= est;
-        E->bytes += real;
+        resident += now_bytes;
+        resident -= E->bytes;
+        E->bytes = now_bytes;
         E->resident = true;
     }
 
     void unpin(Req& r) {   // with mu held
-        if (r.entry) { r.entry->pinned--; r.entry.reset(); }
+        if (!r.entry) return;
+        Entry& e = *r.entry;
+        e.pinned--;
+        // an entry the host LRU dropped leaves the device with its last pin
+        if (e.pinned == 0 && !e.in_table && e.resident) drop_device_locked(e);
+        r.entry.reset();
+        r.keep.reset();
     }
 
     // Walk length and size check of a request's device jobs.
@@ -203,16 +279,19 @@ struct Server {
                 r->entry->last_use = ++use_clock;
             }
             try {
+                const bool was_built = r->entry->built;
                 build_entry(r->entry, *r);
                 if (!r->entry->err.empty()) throw std::runtime_error(r->entry->err);
                 Sim& S = *r->entry->sim;
                 std::vector<std::pair<uint32_t, uint8_t>> asg;
                 std::vector<size_t> owner;
                 if (r->kind == 2) {
-                    const RankEng& R = S.marginal_engine(r->mq);
-                    make_resident(r->entry, R);
-                    if (S.marginal_assignment(r->x, r->mq.data(), r->mo.data(), r->mq.size(), asg)) add_jobs(R, asg, walk, r->jobs, owner, 0);
+                    r->keep = S.marginal_engine(r->mq);
+                    account_host(r->entry);
+                    make_resident(r->entry, *r->keep);
+                    if (S.marginal_assignment(r->x, r->mq.data(), r->mo.data(), r->mq.size(), asg)) add_jobs(*r->keep, asg, walk, r->jobs, owner, 0);
                 } else {
+                    if (!was_built) account_host(r->entry);
                     make_resident(r->entry, *S.eng);
                     S.eng->ensure_device();
                     add_query_jobs_xy(*S.eng, r->x, r->y, walk, r->jobs, owner, 0);
@@ -221,6 +300,7 @@ struct Server {
                 std::lock_guard<std::mutex> lk(mu);
                 ready_q.push_back(r);
                 nready++;
+                inflight++;
             } catch (std::exception& e) {
                 std::lock_guard<std::mutex> lk(mu);
                 r->status = -1; r->err = e.what(); r->t_done = now_s();
@@ -248,6 +328,7 @@ struct Server {
             r->status = 1; r->t_done = t;
             r->jobs.clear();
             unpin(*r);
+            inflight--;
             completed++;
         }
         cv_done.notify_all();
@@ -279,7 +360,7 @@ struct Server {
                 if (!slots[0]) {
                     std::vector<std::complex<double>> none;
                     std::lock_guard<std::mutex> lk(mu);
-                    for (auto& r : batch) { r->status = -1; r->err = "no HIP device available"; r->t_done = now_s(); r->jobs.clear(); unpin(*r); failed++; }
+                    for (auto& r : batch) { r->status = -1; r->err = "no HIP device available"; r->t_done = now_s(); r->jobs.clear(); unpin(*r); inflight--; failed++; }
                     cv_done.notify_all();
                     continue;
                 }
@@ -345,11 +426,17 @@ struct Server {
         bool here = false;
         {
             std::lock_guard<std::mutex> lk(mu);
-            auto it = entries.find(key);
+            // open addressing on the 64-bit key: a colliding circuit with
+            // other content takes the next free key
+            uint64_t k = key;
+            auto it = entries.find(k);
+            while (it != entries.end() && !(it->second->strategy == strategy && same_circuit(it->second->circ, c))) it = entries.find(++k);
             if (it == entries.end()) {
                 auto e = std::make_shared<Entry>();
-                e->key = key;
-                it = entries.emplace(key, e).first;
+                e->key = k;
+                e->circ = c;
+                e->strategy = strategy;
+                it = entries.emplace(k, e).first;
             }
             r->entry = it->second;
             r->ticket = next_ticket++;
@@ -390,6 +477,7 @@ struct Server {
             std::lock_guard<std::mutex> lk(mu);
             ready_q.push_back(r);
             nready++;
+            inflight++;
         }
         cv_ready.notify_one();
         return r->ticket;
@@ -457,6 +545,21 @@ int pq_server_stats(pq_server* h, double* o) {
     o[4] = (double)S.resident.load(); o[5] = (double)S.builds; o[6] = (double)S.max_batch; o[7] = (double)S.overlapped;
     o[8] = S.timed_batches ? S.timed_kernel_ms * (double)S.batches / (double)S.timed_batches : 0.0;
     o[9] = (double)S.entries.size();
+    return 0;
+}
+
+int pq_server_set_host_budget(pq_server* h, uint64_t bytes) {
+    Server& S = *h->s;
+    std::lock_guard<std::mutex> lk(S.mu);
+    S.host_budget = bytes;
+    S.trim_host_locked(nullptr);
+    return 0;
+}
+
+int pq_server_host_stats(pq_server* h, double* o) {
+    Server& S = *h->s;
+    std::lock_guard<std::mutex> lk(S.mu);
+    o[0] = (double)S.host_resident.load(); o[1] = (double)S.host_budget; o[2] = (double)S.host_drops;
     return 0;
 }
 

@@ -27,7 +27,7 @@ Z8Image build_z8_image(const HostEngine& E) {
 
     // Choose up to 8 inner directions: pivot-free kernel vectors first, then
     // the ones adding the fewest new pivots while at most 12 pivots are
-    // touched (the gin table stays within 16 KB of LDS).
+    // touched (the FP64 gin table stays within 32 KB of LDS).
     std::vector<uint8_t> used(d, 0);
     std::vector<uint32_t> inner;
     uint64_t pin_set = 0;
@@ -104,7 +104,8 @@ Z8Image build_z8_image(const HostEngine& E) {
     Z.lane_terms = all & ~excl;
 
     // Inner table: entry i covers c_in = gray(i); phantom bits give zero.
-    Z.inner.assign(256 * 3, 0.f);
+    Z.inner.assign(256, 0.0);
+    Z.inner_aux.assign(512, 0u);
     for (uint32_t i = 0; i < 256; ++i) {
         const uint32_t c = i ^ (i >> 1);
         uint32_t pin = 0, kin = 0;
@@ -123,15 +124,23 @@ Z8Image build_z8_image(const HostEngine& E) {
             if (cls != (Z.imag0 ? (c & 1u) : 0u)) throw std::logic_error("inner real/imaginary classes do not follow slot 0");
             double fin = 1.0;
             for (uint32_t k = 0; k < t; ++k) if ((inner_free >> k) & 1) fin *= ((sin >> k) & 1) ? E.r1[k] : E.r0[k];
-            Z.inner[i] = (float)((kin & 4) ? -fin : fin);
+            Z.inner[i] = (kin & 4) ? -fin : fin;
         }
-        const uint32_t pin4 = pin << 2;
-        std::memcpy(&Z.inner[256 + i], &pin4, 4);
-        std::memcpy(&Z.inner[512 + i], &kin, 4);
+        Z.inner_aux[i] = pin << 3;
+        Z.inner_aux[256 + i] = kin;
     }
+    // Accumulator-walk order: c_in = (gray5(q) << 3) | l at 8 q + l; entry i
+    // of the Gray order covers gray(i), so c_in sits at i = gray^-1(c_in).
+    Z.inner_acc.assign(256, 0.0);
+    for (uint32_t q = 0; q < 32; ++q)
+        for (uint32_t l = 0; l < 8; ++l) {
+            uint32_t c = ((q ^ (q >> 1)) << 3) | l, i = c;
+            i ^= i >> 1; i ^= i >> 2; i ^= i >> 4;
+            Z.inner_acc[8 * q + l] = Z.inner[i];
+        }
     // Lane-factor byte tables
     Z.nbytes = (t + 7) / 8;
-    Z.lt.assign((size_t)Z.nbytes * 256, 1.f);
+    Z.lt.assign((size_t)Z.nbytes * 256, 1.0);
     for (uint32_t b = 0; b < Z.nbytes; ++b)
         for (uint32_t v = 0; v < 256; ++v) {
             double f = 1.0;
@@ -140,14 +149,14 @@ Z8Image build_z8_image(const HostEngine& E) {
                 if (k >= t || !((Z.lane_terms >> k) & 1)) continue;
                 f *= ((v >> q) & 1) ? E.r1[k] : E.r0[k];
             }
-            Z.lt[(size_t)b * 256 + v] = (float)f;
+            Z.lt[(size_t)b * 256 + v] = f;
         }
     // inner pivot table
-    Z.gin.assign((size_t)1 << Z.npin, 1.f);
+    Z.gin.assign((size_t)1 << Z.npin, 1.0);
     for (uint32_t pi = 0; pi < (1u << Z.npin); ++pi) {
         double f = 1.0;
         for (uint32_t b = 0; b < Z.npin; ++b) f *= ((pi >> b) & 1) ? E.r1[Z.pin_term[b]] : E.r0[Z.pin_term[b]];
-        Z.gin[pi] = (float)f;
+        Z.gin[pi] = f;
     }
     return Z;
 }

@@ -74,6 +74,9 @@ lib.pq_sim_selfcheck.argtypes = [_sim_p, ctypes.c_uint64, ctypes.c_uint64, ctype
 lib.pq_sim_z8_info.argtypes = [_sim_p, _u32p]
 lib.pq_sim_marginal_stats.argtypes = [_sim_p, _u32p, ctypes.c_size_t, _u64p]
 lib.pq_device_info.argtypes = [ctypes.c_char_p, ctypes.c_size_t]
+lib.pq_valu_probe.argtypes = [_f64p, ctypes.c_size_t, ctypes.c_uint32, ctypes.c_uint32]
+lib.pq_valu_probe_name.argtypes = [ctypes.c_int]
+lib.pq_valu_probe_name.restype = ctypes.c_char_p
 
 FORMS = {"z8": 0, "branch_walk": 1, "plain_fp64": 2}
 STRATEGIES = {"auto": 0, "cached": 1, "gpu_auto": 3}
@@ -102,18 +105,47 @@ _U64_ARRAYS = {}
 _F64_ARRAYS = {}
 
 
+def _index(v):
+    """One basis index as a Python int in [0, 2^64): integers only (Python
+    ints, numpy integer scalars); a negative or non-integer index raises
+    instead of wrapping or truncating on its way to a C uint64."""
+    if isinstance(v, (bool, np.bool_)) or not isinstance(v, (int, np.integer)):
+        raise TypeError("basis index must be an integer, not %s" % type(v).__name__)
+    v = int(v)
+    if v < 0 or v >> 64:
+        raise OverflowError("basis index %d outside [0, 2^64)" % v)
+    return v
+
+
+def _u64_array(v):
+    """v (a list, tuple or ndarray of basis indices) as a C-contiguous uint64
+    ndarray.  Integer arrays only: a float or bool array raises TypeError, and
+    a signed array with a negative entry raises OverflowError (a plain cast
+    would wrap -1 to 2^64 - 1)."""
+    if isinstance(v, np.ndarray):
+        if v.dtype == np.uint64:
+            return np.ascontiguousarray(v)
+        if v.dtype.kind not in "iu":
+            raise TypeError("basis indices must be an integer array, not %s" % v.dtype)
+        if v.dtype.kind == "i" and v.size and int(v.min()) < 0:
+            raise OverflowError("negative basis index")
+        return np.ascontiguousarray(v, dtype=np.uint64)
+    # one numpy conversion of the checked Python ints (about 3x faster than
+    # ctypes' per-element one for thousands of them)
+    return np.array([_index(x) for x in v], dtype=np.uint64)
+
+
 def _u64_arg(v):
-    """A ctypes uint64 array holding v (a list, tuple or ndarray of basis
-    indices), sharing memory with a writable C-contiguous uint64 ndarray."""
+    """A ctypes uint64 array holding v (see _u64_array), sharing memory with a
+    writable C-contiguous uint64 ndarray."""
     n = len(v)
     t = _U64_ARRAYS.get(n)
     if t is None:
         t = _U64_ARRAYS[n] = ctypes.c_uint64 * n
     if not isinstance(v, np.ndarray) or v.dtype != np.uint64 or not v.flags.c_contiguous or not v.flags.writeable:
-        # one numpy conversion (about 3x faster than ctypes' per-element
-        # one for thousands of Python ints; a negative index raises
-        # OverflowError instead of wrapping)
-        v = np.array(v, dtype=np.uint64)
+        v = _u64_array(v)
+        if not v.flags.writeable:
+            v = v.copy()
     return t.from_buffer(v)
 
 
@@ -255,12 +287,12 @@ class Simulator:
 
     def amplitude(self, input, output, form=None):
         out = (ctypes.c_double * 2)()
-        _check(lib.pq_sim_amplitude_index(self._h, int(input), int(output), self._form(form), out))
+        _check(lib.pq_sim_amplitude_index(self._h, _index(input), _index(output), self._form(form), out))
         return (out[0], out[1])
 
     def probability(self, input, output, form=None):
         out = ctypes.c_double()
-        _check(lib.pq_sim_probability(self._h, int(input), int(output), self._form(form), ctypes.byref(out)))
+        _check(lib.pq_sim_probability(self._h, _index(input), _index(output), self._form(form), ctypes.byref(out)))
         return out.value
 
     def marginal(self, input, measured, form=None):
@@ -268,7 +300,7 @@ class Simulator:
         qs = np.asarray([m[0] for m in measured], dtype=np.uint32)
         os_ = np.asarray([1 if m[1] else 0 for m in measured], dtype=np.uint8)
         out = ctypes.c_double()
-        _check(lib.pq_sim_marginal_probability(self._h, int(input), _ptr(qs, _u32p), _ptr(os_, _u8p), len(qs), self._form(form), ctypes.byref(out)))
+        _check(lib.pq_sim_marginal_probability(self._h, _index(input), _ptr(qs, _u32p), _ptr(os_, _u8p), len(qs), self._form(form), ctypes.byref(out)))
         return out.value
 
     def amplitudes(self, inputs, outputs, form=None):
@@ -290,7 +322,7 @@ class Simulator:
         standard_error) (sim.rs:187-193, pybind.rs:149-153).  The exact value
         with error 0 when the branch subspace has at most `samples` points."""
         o = np.zeros(3, dtype=np.float64)
-        _check(lib.pq_sim_amplitude_estimate(self._h, int(input), int(output), int(samples), int(seed), self._form(form), _ptr(o, _f64p)))
+        _check(lib.pq_sim_amplitude_estimate(self._h, _index(input), _index(output), int(samples), int(seed), self._form(form), _ptr(o, _f64p)))
         return float(o[0]), float(o[1]), float(o[2])
 
     def statevector(self, input, form=None):
@@ -299,7 +331,7 @@ class Simulator:
         little-endian complex128 bytes)."""
         n = self.num_qubits
         out = np.empty(1 << n, dtype=np.complex128)
-        _check(lib.pq_sim_statevector(self._h, int(input), self._form(form), out.ctypes.data))
+        _check(lib.pq_sim_statevector(self._h, _index(input), self._form(form), out.ctypes.data))
         return out
 
     def sample_many(self, input, shots, seed, form=None):
@@ -307,7 +339,7 @@ class Simulator:
         (sim.rs:272-284, pybind.rs:169-172): the chain rule over qubit
         prefixes with the reference's per-shot SplitMix64 streams."""
         out = np.empty(int(shots), dtype=np.uint64)
-        _check(lib.pq_sim_sample_many(self._h, int(input), int(shots), int(seed), self._form(form), out.ctypes.data))
+        _check(lib.pq_sim_sample_many(self._h, _index(input), int(shots), int(seed), self._form(form), out.ctypes.data))
         return [int(v) for v in out]
 
     def stats(self):
@@ -468,8 +500,8 @@ def autotune(sims, inputs, outputs, form="z8", offsets=None, max_mids=None, repe
     import time
     offsets = offsets or _TUNE_GRID[form][0]
     max_mids = max_mids or _TUNE_GRID[form][1]
-    xs = np.ascontiguousarray(inputs, dtype=np.uint64)
-    ys = np.ascontiguousarray(outputs, dtype=np.uint64)
+    xs = _u64_array(inputs)
+    ys = _u64_array(outputs)
     distinct = list({id(s): s for s in sims}.values())
     configs = {}                      # walk lengths per engine -> [(offset, max_mid), ...]
     for off in offsets:
@@ -569,6 +601,21 @@ def device_info():
     return buf.value.decode()
 
 
+def valu_probe(blocks_per_cu=8, iters=16384):
+    """VALU issue rates of the device (pq_valu_probe): per instruction,
+    lane-ops/s, lanes per clock per CU at the in-kernel clock, that clock (GHz)
+    and the kernel time."""
+    n = 3 + 4 * 7
+    o = np.zeros(n, dtype=np.float64)
+    k = _check(lib.pq_valu_probe(_ptr(o, _f64p), n, blocks_per_cu, iters))
+    ops = {}
+    for i in range(k):
+        r = o[3 + 4 * i: 7 + 4 * i]
+        ops[lib.pq_valu_probe_name(i).decode()] = {"lane_ops_per_s": float(r[0]), "lanes_per_clk_per_cu": float(r[1]),
+                                                   "clock_ghz": float(r[2]), "ms": float(r[3])}
+    return {"cus": int(o[0]), "wall_clock_mhz": float(o[1]), "blocks_per_cu": int(o[2]), "ops": ops}
+
+
 def rng_u64(seed):
     """First SplitMix64 output of Rng::new(seed) (src/rng.rs:13-23, pybind.rs:221-224)."""
     from .workloads import Rng
@@ -582,6 +629,8 @@ lib.pq_server_submit.argtypes = [ctypes.c_void_p, ctypes.c_uint32, ctypes.c_size
                                  ctypes.c_uint64, ctypes.c_uint64, _u32p, _u8p, ctypes.c_size_t, _u64p]
 lib.pq_server_wait.argtypes = [ctypes.c_void_p, ctypes.c_uint64, ctypes.c_double, _f64p, _f64p]
 lib.pq_server_stats.argtypes = [ctypes.c_void_p, _f64p]
+lib.pq_server_set_host_budget.argtypes = [ctypes.c_void_p, ctypes.c_uint64]
+lib.pq_server_host_stats.argtypes = [ctypes.c_void_p, _f64p]
 
 KINDS = {"amplitude": 0, "probability": 1, "marginal": 2}
 
@@ -589,11 +638,22 @@ KINDS = {"amplitude": 0, "probability": 1, "marginal": 2}
 class Server:
     """Request loop with resident engines, HBM admission and continuous batching."""
 
-    def __init__(self, hbm_budget_bytes=64 << 30, workers=8, form="z8", strategy="cached"):
+    def __init__(self, hbm_budget_bytes=64 << 30, workers=8, form="z8", strategy="cached", host_budget_bytes=None):
         h = ctypes.c_void_p()
         _check(lib.pq_server_create(int(hbm_budget_bytes), workers, FORMS[form], ctypes.byref(h)))
         self._h = h
         self.strategy = STRATEGIES[strategy]
+        if host_budget_bytes is not None:
+            self.set_host_budget(host_budget_bytes)
+
+    def set_host_budget(self, nbytes):
+        """Host bytes of built engine images kept for idle circuits (default 8 GiB)."""
+        _check(lib.pq_server_set_host_budget(self._h, int(nbytes)))
+
+    def host_stats(self):
+        o = np.zeros(3, dtype=np.float64)
+        _check(lib.pq_server_host_stats(self._h, _ptr(o, _f64p)))
+        return dict(zip(("host_bytes", "host_budget", "host_drops"), o.tolist()))
 
     def submit(self, circuit, kind, x=0, y=0, measured=()):
         ops, q0, q1, ph = circuit.arrays()
@@ -601,7 +661,7 @@ class Server:
         mo = np.asarray([1 if m[1] else 0 for m in measured], dtype=np.uint8)
         t = ctypes.c_uint64()
         _check(lib.pq_server_submit(self._h, circuit.num_qubits, len(ops), _ptr(ops, _u8p), _ptr(q0, _u32p), _ptr(q1, _u32p), _ptr(ph, _u32p),
-                                    self.strategy, KINDS[kind], int(x), int(y), _ptr(mq, _u32p), _ptr(mo, _u8p), len(mq), ctypes.byref(t)))
+                                    self.strategy, KINDS[kind], _index(x), _index(y), _ptr(mq, _u32p), _ptr(mo, _u8p), len(mq), ctypes.byref(t)))
         return t.value
 
     def wait(self, ticket, timeout=-1.0):

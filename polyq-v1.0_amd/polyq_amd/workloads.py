@@ -180,10 +180,11 @@ def config2(seed=1):
 
 
 def config3(seed=1, n_amps=1024):
-    """n=48, the config-2 shape with 40 Rz(theta), theta ~ U[0, 2pi);
-    amplitudes <0|C|y> for random inputs y.  Returns (circuit, inputs, outputs)."""
+    """n=48, the config-2 generator (300 CX/CZ and 100 S interleaved) with 40
+    Rz(theta), theta ~ U[0, 2pi), in place of the T gates; amplitudes <0|C|y>
+    for random inputs y.  Returns (circuit, inputs, outputs)."""
     rng = Rng(seed)
-    c = sandwich_layer(48, 40, 360, 120, rng, angle="uniform")
+    c = sandwich_layer(48, 40, 300, 100, rng, angle="uniform")
     ys = [rng.below(1 << 48) for _ in range(n_amps)]
     return c, ys, [0] * n_amps
 

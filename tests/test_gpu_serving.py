@@ -1,6 +1,7 @@
 """Batched, graph-replayed and served evaluation on the GPU: every result is
 the same alone and in the mix."""
 import random
+import time
 
 import numpy as np
 import pytest
@@ -155,3 +156,83 @@ def test_host_finish_equals_device_reduction_many_items_per_query():
         np.testing.assert_array_equal(big, small)
     finally:
         pq.set_walk_tuning(*prev)
+
+
+def _resident_bytes_of(c, x, y):
+    probe = pq.Server(hbm_budget_bytes=64 << 30, workers=1)
+    probe.wait(probe.submit(c, "amplitude", x, y), timeout=120.0)
+    one = probe.stats()["resident_bytes"]
+    probe.close()
+    assert one > 0
+    return one
+
+
+def test_server_marginal_on_resident_circuit_beyond_budget_fails_fast():
+    # The amplitude engine is resident; the marginal (sandwich) engine of the
+    # same circuit would need about as much again, which the budget cannot
+    # hold next to it while the request pins its own circuit: the request
+    # fails with the HBM error at once instead of waiting for room.
+    (c, pairs), = _mixed_circuits(15, 1, nmax=14)
+    one = _resident_bytes_of(c, *pairs[0])
+    srv = pq.Server(hbm_budget_bytes=int(1.5 * one), workers=1)
+    a, _ = srv.wait(srv.submit(c, "amplitude", *pairs[0]), timeout=60.0)
+    assert a == complex(*pq.Simulator(c, strategy="cached").amplitude(*pairs[0]))
+    t0 = time.time()
+    with pytest.raises(pq.PolyqError, match="HBM budget"):
+        srv.wait(srv.submit(c, "marginal", pairs[0][0], 0, [(0, True), (1, False)]), timeout=60.0)
+    assert time.time() - t0 < 30.0
+    # the server keeps serving afterwards
+    a, _ = srv.wait(srv.submit(c, "amplitude", *pairs[1]), timeout=60.0)
+    assert a == complex(*pq.Simulator(c, strategy="cached").amplitude(*pairs[1]))
+    srv.close()
+
+
+def test_server_concurrent_marginals_under_tight_budget_never_hang():
+    # Two resident circuits and a budget with room for one more engine: two
+    # marginals submitted together each pin their own circuit and want the
+    # other evicted.  Neither waits forever: each either runs or fails with
+    # the HBM error, and the ones that run are right.
+    work = _mixed_circuits(16, 2, nmax=14)
+    one = max(_resident_bytes_of(c, *p[0]) for c, p in work)
+    srv = pq.Server(hbm_budget_bytes=int(2.6 * one), workers=2)
+    for c, pairs in work:
+        srv.wait(srv.submit(c, "amplitude", *pairs[0]), timeout=60.0)
+    meas = [(0, True), (1, True)]
+    tickets = [(srv.submit(c, "marginal", pairs[0][0], 0, meas), c, pairs[0][0]) for c, pairs in work for _ in range(3)]
+    ok = 0
+    for t, c, x in tickets:
+        try:
+            v, _ = srv.wait(t, timeout=60.0)
+            assert v.real == pq.Simulator(c, strategy="cached").marginal(x, meas)
+            ok += 1
+        except pq.PolyqError as e:
+            assert "HBM budget" in str(e)
+    st = srv.stats()
+    assert st["completed"] + st["failed"] == len(tickets) + len(work)
+    assert st["resident_bytes"] <= 2.6 * one
+    srv.close()
+
+
+def test_server_host_lru_drops_idle_builds_and_rebuilds():
+    # A host budget below one build: every idle circuit is dropped when the
+    # next one is built, comes back as a fresh build, and answers the same.
+    work = _mixed_circuits(17, 6, nmax=14)
+    srv = pq.Server(hbm_budget_bytes=64 << 30, workers=2, host_budget_bytes=1)
+    got = []
+    for rep in range(2):
+        for c, pairs in work:
+            for kind in ("amplitude", "marginal"):
+                x, y = pairs[rep]
+                meas = [(0, rep == 0), (1, True)] if kind == "marginal" else ()
+                v, _ = srv.wait(srv.submit(c, kind, x, y, meas), timeout=60.0)
+                got.append((c, kind, x, y, meas, v))
+    hs = srv.host_stats()
+    st = srv.stats()
+    assert hs["host_drops"] >= len(work)
+    assert st["builds"] >= 2 * len(work) - 1
+    assert st["resident_circuits"] <= 2
+    for c, kind, x, y, meas, v in got:
+        s = pq.Simulator(c, strategy="cached")
+        want = complex(*s.amplitude(x, y)) if kind == "amplitude" else complex(s.marginal(x, meas), 0.0)
+        assert v == want
+    srv.close()

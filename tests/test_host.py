@@ -231,3 +231,21 @@ def test_index_marshalling_to_uint64():
     assert len(pq._u64_arg([])) == 0
     with pytest.raises(OverflowError):
         pq._u64_arg([5, -1])
+    # numpy inputs: a signed array or scalar with a negative value raises
+    # instead of wrapping to 2^64 - 1, and floats are not truncated
+    with pytest.raises(OverflowError):
+        pq._u64_arg(np.array([-1], np.int64))
+    with pytest.raises(OverflowError):
+        pq._u64_arg([np.int64(-1)])
+    with pytest.raises(OverflowError):
+        pq._index(np.int64(-1))
+    with pytest.raises(OverflowError):
+        pq._index(1 << 64)
+    with pytest.raises(TypeError):
+        pq._u64_arg(np.array([1.5, 2.0]))
+    with pytest.raises(TypeError):
+        pq._index(2.0)
+    with pytest.raises(TypeError):
+        pq._u64_arg(np.array([True]))
+    assert pq._index(np.uint64(2**64 - 1)) == 2**64 - 1
+    assert list(pq._u64_arg(np.array([7, 0], np.int32))) == [7, 0]
