@@ -105,10 +105,11 @@ SEG_OPS_PER_POINT = 30.0
 # profiles/k_seg_walk_config2.md): the table once per XCD L2 and the item
 # partials.
 SEG_TRAFFIC = {("headline", 16, 2): 1.84e6, ("saturated", 16, 6): 5.37e6}
-# HBM bytes per branch-walk launch on the headline step (FETCH_SIZE x 2 +
-# WRITE_SIZE, copy-calibrated; profiles/k_z8_walk_config2.md): the item
-# partials (4 doubles per 256-lane item) and the tables once per XCD.
-BRANCH_TRAFFIC = {}
+# HBM bytes per branch-walk launch on the headline step, keyed by the lane
+# walk tuning (FETCH_SIZE x 2 + WRITE_SIZE, copy-calibrated;
+# profiles/k_z8_walk_config2.md): the item partials (4 doubles per 256-lane
+# item) and the tables once per XCD.
+BRANCH_TRAFFIC = {(26, 4): 1.31e6}
 PEAK_CLOCK_HZ = 2.4e9       # MI355X peak engine clock
 
 

@@ -101,6 +101,69 @@ int pq_sim_sample_many(pq_sim* sim, uint64_t input, size_t shots, uint64_t seed,
  * (the query loop of Simulator::statevector, src/sim.rs:201-213). */
 int pq_sim_amplitudes(pq_sim* sim, size_t n, const uint64_t* inputs, const uint64_t* outputs, int form, double* out);
 
+/* ---- engine-level seam: one Engine (src/engine.rs:273-284) handed over by
+ * the fields it holds, so a Strategy::Gpu arm in src/rank.rs keeps its own
+ * path-sum compiler and normal form and evaluates its branch engines here
+ * (the call chain Strategy -> RankEngine::eval_assignment -> Engine::eval_with
+ * -> sum_subspace, src/sim.rs:65-71, src/rank.rs:184-190,
+ * src/engine.rs:557-561, :669).
+ *
+ * Rows are packed F2 vectors, LSB-first in u64 words, tail bits zero (BitVec,
+ * src/bits.rs:10-19).  A shift row (UVec, src/normal_form.rs:161-201) is its
+ * four parts a | b | o | e, each in ceil(len/64) words, for part lengths
+ * planes, planes, lines, radical.  The caller owns every array; create copies
+ * what it needs. */
+typedef struct pq_engine_desc {
+    uint32_t nf;               /* fixed variables (Engine::fixed order) */
+    uint32_t planes, lines, radical;   /* Structure: g, #odd, #even */
+    uint32_t terms;            /* rotation terms t (Branching::term_vars) */
+    uint32_t branch_dim;       /* kernel vectors d (Elim::kernel) */
+    uint32_t rank;             /* RREF pivots (Elim::pivots) */
+    uint32_t consistency_rows; /* Elim::consistency */
+    uint32_t num_h;            /* Engine::num_h */
+    uint32_t global_phase;     /* Engine::global (Phase) */
+    int32_t zero;              /* Engine::zero */
+    int32_t sigma;             /* Diag::sigma = #o1 - #o3 */
+    uint32_t free_vars;        /* Core::free.len() (statistics only) */
+    const uint64_t* ucols;     /* nf shift rows (Core::ucols) */
+    const uint64_t* mcols;     /* terms shift rows (Branching::mcols) */
+    const uint64_t* diag;      /* one shift-shaped row: Diag wa | wb | o1 | we */
+    const uint64_t* o3;        /* Diag::o3 over the lines */
+    const uint8_t* fixed_v;    /* nf Z4 linear coefficients (Engine::fixed_v) */
+    const uint32_t* fixed_theta;   /* nf phases (Engine::fixed_theta) */
+    const uint64_t* b_fixed;   /* nf rows of ceil(nf/64) words (Core::b_fixed) */
+    const uint32_t* theta;     /* terms phases (the Term angles) */
+    const uint32_t* pivot_col; /* rank increasing pivot columns (Elim::pivots .0) */
+    const uint64_t* pivot_row; /* rank rows over the radical (Elim::pivots .1) */
+    const uint64_t* consistency;   /* consistency_rows rows over the radical */
+    const uint64_t* kernel;    /* branch_dim rows of ceil(terms/64) words (Elim::kernel) */
+    const uint64_t* kernel_u;  /* branch_dim shift rows (Elim::kernel_u), or NULL: then derived from mcols */
+} pq_engine_desc;
+typedef struct pq_engine pq_engine;
+
+/* Replaces the Engine the Rust side built (src/engine.rs:289-390) by a GPU
+ * engine built from its fields; the descriptor is checked for consistency
+ * (RREF shape, kernel inside ker K, kernel_u = xor of mcols, Diag lines,
+ * symmetric b_fixed, zero tails).  The device images are uploaded on first
+ * evaluation and stay until pq_engine_destroy.  At most 64 terms. */
+int pq_engine_create(const pq_engine_desc* desc, pq_engine** out);
+void pq_engine_destroy(pq_engine* engine);
+/* Engine::stats (src/engine.rs:415-425): free_vars, planes, lines, radical,
+ * terms, branch_dim. */
+int pq_engine_stats(const pq_engine* engine, uint64_t out6[6]);
+/* Engine::eval (src/engine.rs:551-561): values aligned with the fixed
+ * variables, one byte each (0/1). */
+int pq_engine_eval(pq_engine* engine, const uint8_t* values, size_t nf, int form, double out[2]);
+/* Engine::eval for n (engine, values) queries in one device launch (batched
+ * forms; the plain form runs them one by one).  A RankEngine's eval_assignment
+ * is the sum of its 2^p branch engines' values for the same fixed values
+ * (src/rank.rs:184-190).  out has 2n doubles. */
+int pq_engine_eval_batch(size_t n, pq_engine* const* engines, const uint8_t* const* values, int form, double* out);
+/* Per-branch Gauss-sum outputs of the engine for explicit t-bit branch vectors
+ * s, as pq_sim_branch_dump (normal_form.rs:472-493). */
+int pq_engine_branch_dump(pq_engine* engine, const uint8_t* values, size_t n, const uint64_t* s, uint8_t* zero, uint8_t* k8,
+                          uint8_t* ktot, uint32_t* r);
+
 /* Heterogeneous batch: query i is <outputs[i]|sims[i]|inputs[i]>; all
  * branch-engine sums of all queries run in one persistent launch (Z8 forms
  * only).  out has 2n doubles. */

@@ -1407,6 +1407,80 @@ int orc_branch_dump(const orc_sim* h, const uint8_t* x, const uint8_t* y, size_t
     } catch (std::exception& e) { g_err = e.what(); return -1; }
 }
 
+// ---- engine fields in the layout of the library's engine descriptor
+// (pq_engine_desc): what a Rust Engine holds (engine.rs:240-284: Core,
+// Branching, Engine) for branch engine `branch` of the simulator.  Rows are
+// packed LSB-first in u64 words (bits.rs:10-19); a UVec row is a | b | o | e,
+// each part in ceil(len/64) words; Diag as one such row wa | wb | o1 | we plus
+// o3 over the lines.
+// sizes (13): nf, planes, lines, radical, terms, kernel vectors, RREF rank,
+// consistency rows, num_h, global phase, zero, sigma, free variables.
+static size_t words_of(size_t n) { return (n + 63) / 64; }
+static void put_bits(const BitVec& b, uint64_t*& p, size_t nw) { for (size_t k = 0; k < nw; ++k) *p++ = k < b.w.size() ? b.w[k] : 0; }
+static void put_uvec(const UVec& u, const Structure& st, uint64_t*& p) {
+    put_bits(u.a, p, words_of(st.g())); put_bits(u.b, p, words_of(st.g()));
+    put_bits(u.o, p, words_of(st.odd.size())); put_bits(u.e, p, words_of(st.even.size()));
+}
+int orc_engine_sizes(const orc_sim* h, size_t branch, uint64_t* o) {
+    try {
+        if (branch >= h->s.engine.branches.size()) { g_err = "no such branch engine"; return -1; }
+        const Engine& E = h->s.engine.branches[branch];
+        o[0] = E.fixed.size(); o[1] = E.st.g(); o[2] = E.st.odd.size(); o[3] = E.st.even.size(); o[4] = E.terms.size();
+        o[5] = E.elim.kernel.size(); o[6] = E.elim.pivots.size(); o[7] = E.elim.consistency.size(); o[8] = E.num_h;
+        o[9] = E.global; o[10] = E.zero ? 1 : 0; o[11] = (uint64_t)(int64_t)E.diag.sigma; o[12] = E.free_.size();
+        return 0;
+    } catch (std::exception& e) { g_err = e.what(); return -1; }
+}
+int orc_engine_export(const orc_sim* h, size_t branch, uint64_t* ucols, uint64_t* mcols, uint64_t* diag, uint64_t* o3, uint8_t* fixed_v,
+                      uint32_t* fixed_theta, uint64_t* b_fixed, uint32_t* theta, uint32_t* pivot_col, uint64_t* pivot_row,
+                      uint64_t* consistency, uint64_t* kernel, uint64_t* kernel_u) {
+    try {
+        if (branch >= h->s.engine.branches.size()) { g_err = "no such branch engine"; return -1; }
+        const Engine& E = h->s.engine.branches[branch];
+        const Structure& st = E.st;
+        const size_t nf = E.fixed.size(), t = E.terms.size(), ne = st.even.size();
+        for (size_t i = 0; i < nf; ++i) put_uvec(E.ucols[i], st, ucols);
+        for (size_t k = 0; k < t; ++k) put_uvec(E.mcols[k], st, mcols);
+        put_bits(E.diag.wa, diag, words_of(st.g())); put_bits(E.diag.wb, diag, words_of(st.g()));
+        put_bits(E.diag.o1, diag, words_of(st.odd.size())); put_bits(E.diag.we, diag, words_of(ne));
+        put_bits(E.diag.o3, o3, words_of(st.odd.size()));
+        for (size_t i = 0; i < nf; ++i) { fixed_v[i] = E.fixed_v[i]; fixed_theta[i] = E.fixed_theta[i]; put_bits(E.b_fixed.rows[i], b_fixed, words_of(nf)); }
+        for (size_t k = 0; k < t; ++k) theta[k] = E.term_theta[k];
+        for (size_t i = 0; i < E.elim.pivots.size(); ++i) { pivot_col[i] = (uint32_t)E.elim.pivots[i].first; put_bits(E.elim.pivots[i].second, pivot_row, words_of(ne)); }
+        for (auto& r : E.elim.consistency) put_bits(r, consistency, words_of(ne));
+        for (auto& n : E.elim.kernel) put_bits(n, kernel, words_of(t));
+        for (auto& u : E.elim.kernel_u) put_uvec(u, st, kernel_u);
+        return 0;
+    } catch (std::exception& e) { g_err = e.what(); return -1; }
+}
+// The fixed values of <y|C|x> aligned with fixed() (pathsum.rs:430-450,
+// engine.rs:515-531); 1 if the boundary is contradictory.
+int orc_engine_values(const orc_sim* h, const uint8_t* x, const uint8_t* y, uint8_t* vals) {
+    try {
+        const PathSum& ps = h->s.ps;
+        const size_t n = ps.num_qubits;
+        std::vector<std::pair<size_t, bool>> bd;
+        if (!ps.boundary(bits_of(n, x), bits_of(n, y), bd)) return 1;
+        std::vector<bool> v;
+        h->s.engine.branches[0].values_of(bd, v);
+        for (size_t i = 0; i < v.size(); ++i) vals[i] = v[i] ? 1 : 0;
+        return 0;
+    } catch (std::exception& e) { g_err = e.what(); return -1; }
+}
+// Engine::eval (engine.rs:551-561) of branch engine `branch` on fixed values.
+int orc_engine_eval(const orc_sim* h, size_t branch, const uint8_t* vals, size_t nf, double* out) {
+    try {
+        if (branch >= h->s.engine.branches.size()) { g_err = "no such branch engine"; return -1; }
+        const Engine& E = h->s.engine.branches[branch];
+        if (nf != E.fixed.size()) { g_err = "values do not match the fixed variables"; return -1; }
+        std::vector<bool> v(nf);
+        for (size_t i = 0; i < nf; ++i) v[i] = vals[i] != 0;
+        const C64 a = E.eval(v);
+        out[0] = a.re; out[1] = a.im;
+        return 0;
+    } catch (std::exception& e) { g_err = e.what(); return -1; }
+}
+
 // Brute-force Gauss sum of one branch, independent of the normal form: the
 // sum over all 2^m free assignments z of i^{q_free(z)} (-1)^{shift(s).z} with
 // shift(s) = B_{free,F} x_F + sum_{k in s} e_{term var k} (engine.rs:8-31,

@@ -13,7 +13,7 @@ CSRC = os.path.join(HERE, "csrc")
 OBJ = os.path.join(HERE, "_build")
 OUT = os.path.join(HERE, "polyq_amd", "libpolyq_amd.so")
 
-HOST_SOURCES = ["pq_compile.cpp", "pq_engine.cpp", "pq_z8.cpp", "pq_seg.cpp", "pq_api.cpp", "pq_server.cpp"]
+HOST_SOURCES = ["pq_compile.cpp", "pq_engine.cpp", "pq_z8.cpp", "pq_seg.cpp", "pq_api.cpp", "pq_engine_abi.cpp", "pq_server.cpp"]
 HIP_SOURCES = ["pq_kernels.hip", "pq_valu.hip"]
 
 # x86-64-v3: the library built here also runs on the GPU host's CPU.

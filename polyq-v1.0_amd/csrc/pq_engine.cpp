@@ -256,16 +256,11 @@ std::shared_ptr<HostEngine> build_engine(const PathSum& ps, const std::vector<ui
     }
     E.b_fixed.assign(E.nf, Bits(E.nf));
     for (uint32_t i = 0; i < E.nf; ++i) ps.adj[fixed[i]].each([&](uint32_t j) { if (E.fixed_pos[j] >= 0) E.b_fixed[i].set((uint32_t)E.fixed_pos[j]); });
-    E.nfw = (E.nf + 63) / 64;
-    E.b_fixed_flat.assign((size_t)E.nf * E.nfw, 0);
-    for (uint32_t i = 0; i < E.nf; ++i) std::copy(E.b_fixed[i].w.begin(), E.b_fixed[i].w.end(), &E.b_fixed_flat[(size_t)i * E.nfw]);
-
     NormalForm nf = normal_form(bff, parity);
     ULayout& L = E.L;
     L.g = (uint32_t)nf.pa.size(); L.nodd = (uint32_t)nf.odd.size(); L.neven = (uint32_t)nf.even.size();
     L.Wg = (L.g + 63) / 64; L.Wo = (L.nodd + 63) / 64; L.We = (L.neven + 63) / 64;
     const uint32_t W = L.words();
-    E.log2mag = (double)L.g + (double)L.neven + (double)L.nodd / 2.0;   // normal_form.rs:465-467
 
     // Column rows: for each needed free variable j, the packed coordinates
     // (p_l[j])_l (normal_form.rs:413-433).  Needed = neighbours of fixed
@@ -348,13 +343,33 @@ std::shared_ptr<HostEngine> build_engine(const PathSum& ps, const std::vector<ui
         for (uint32_t k = 0; k < t; ++k) if ((E.kernel[j] >> k) & 1)
             for (uint32_t w = 0; w < W; ++w) E.kernel_u[(size_t)j * W + w] ^= E.mcols[(size_t)k * W + w];
 
+    for (uint32_t k = 0; k < t; ++k) E.theta.push_back(ps.theta[freev[term_vars[k]]]);
+    for (uint32_t f : fixed) { E.fixed_v.push_back(ps.v[f]); E.fixed_theta.push_back(ps.theta[f]); }
+    E.nh = ps.nh; E.global = ps.global; E.zero = ps.zero;
+    finish_engine(E);
+    return Ep;
+}
+
+// Everything the evaluator derives from the engine's core fields (fixed
+// values and edges, shift rows, Diag, RREF and kernel, term angles): the
+// packed fixed-edge rows, the magnitude, the term multipliers and the Z8
+// quadratic form.
+void finish_engine(HostEngine& E) {
+    const ULayout& L = E.L;
+    const uint32_t W = L.words(), t = E.t;
+    if (t > 64) throw std::invalid_argument("more than 64 non-Clifford terms in one engine");
+    E.nfw = (E.nf + 63) / 64;
+    E.b_fixed_flat.assign((size_t)E.nf * E.nfw, 0);
+    for (uint32_t i = 0; i < E.nf; ++i) std::copy(E.b_fixed[i].w.begin(), E.b_fixed[i].w.end(), &E.b_fixed_flat[(size_t)i * E.nfw]);
+    E.log2mag = (double)L.g + (double)L.neven + (double)L.nodd / 2.0;   // normal_form.rs:465-467
+
     // Rotation terms.  Reference-style FP64 multipliers (engine.rs:221-238) for
     // the plain kernel, and the factorised form for the Z8 kernel.
-    double half_sum = 0;   // sum theta_k / 2 in units of 2pi/2^32
+    E.a0.clear(); E.a1.clear(); E.up.clear(); E.down.clear(); E.r0.clear(); E.r1.clear();
+    E.Bmag = 1.0; E.neg_mask = 0;
     uint64_t tsum = 0;
     for (uint32_t k = 0; k < t; ++k) {
-        Phase th = ps.theta[freev[term_vars[k]]];
-        E.theta.push_back(th);
+        const Phase th = E.theta[k];
         std::complex<double> e = phase_complex(th), one(1, 0);
         std::complex<double> a0 = (one + e) * 0.5, a1 = (one - e) * 0.5;
         E.a0.push_back(a0); E.a1.push_back(a1);
@@ -372,7 +387,6 @@ std::shared_ptr<HostEngine> build_engine(const PathSum& ps, const std::vector<ui
         if (th > kPi) E.neg_mask |= 1ull << k;
         tsum += th;
     }
-    (void)half_sum;
     {
         // E = e^{i sum theta_k / 2}: the half-angle sum is periodic mod 2^33 units
         uint64_t hs = tsum & ((1ull << 33) - 1);
@@ -393,8 +407,6 @@ std::shared_ptr<HostEngine> build_engine(const PathSum& ps, const std::vector<ui
             if (z8_cross(E, U, E.kernel[j], &E.kernel_u[(size_t)k * W], E.kernel[k])) { E.Q[j] |= 1ull << k; E.Q[k] |= 1ull << j; }
     }
 
-    for (uint32_t f : fixed) { E.fixed_v.push_back(ps.v[f]); E.fixed_theta.push_back(ps.theta[f]); }
-    E.nh = ps.nh; E.global = ps.global; E.zero = ps.zero;
     E.mag = std::pow(2.0, E.log2mag - (double)E.nh / 2.0) * E.Bmag;
     E.ucol_dq.assign(E.nf, 0);
     for (uint32_t f = 0; f < E.nf; ++f)
@@ -404,7 +416,6 @@ std::shared_ptr<HostEngine> build_engine(const PathSum& ps, const std::vector<ui
     for (uint32_t k = 0; k < t; ++k)
         for (uint32_t j = 0; j < d; ++j)
             if (row_parity(&E.mcols[(size_t)k * W], &E.dual[(size_t)j * W], W) ^ (((E.kernel[j] >> k) & 1) != 0)) E.mcol_dq[k] |= 1ull << j;
-    return Ep;
 }
 
 // -------------------------------------------------------------- prologue --

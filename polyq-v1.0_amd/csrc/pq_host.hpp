@@ -176,6 +176,12 @@ struct HostEngine {
 
 // Build an engine for `ps` with the given fixed variables (engine.rs:289-370).
 std::shared_ptr<HostEngine> build_engine(const PathSum& ps, const std::vector<uint32_t>& fixed);
+// Derive the evaluator's data from an engine's core fields (fixed values,
+// fixed edges b_fixed, layout L, ucols, mcols, diag, o3, sigma, the RREF
+// pivot_col / pivot_row / consistency, kernel, free_col, kernel_u, theta, nh,
+// global, zero); build_engine ends with it, and so does an engine built from
+// the fields of an existing one (pq_engine_create).
+void finish_engine(HostEngine& E);
 
 // Per-query prologue (engine.rs:569-610) extended with the Z8 form data.
 struct Prepared {

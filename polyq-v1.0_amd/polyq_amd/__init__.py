@@ -13,7 +13,8 @@ import os
 
 import numpy as np
 
-__all__ = ["Circuit", "Simulator", "Graph", "batch_amplitudes", "rng_u64", "Phase", "lib", "last_timing", "device_info"]
+__all__ = ["Circuit", "Simulator", "Engine", "Graph", "batch_amplitudes", "engine_eval_batch", "rng_u64", "Phase", "lib", "last_timing",
+           "device_info"]
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
 _LIB_PATH = os.path.join(_HERE, "libpolyq_amd.so")
@@ -74,6 +75,13 @@ lib.pq_sim_selfcheck.argtypes = [_sim_p, ctypes.c_uint64, ctypes.c_uint64, ctype
 lib.pq_sim_z8_info.argtypes = [_sim_p, _u32p]
 lib.pq_sim_marginal_stats.argtypes = [_sim_p, _u32p, ctypes.c_size_t, _u64p]
 lib.pq_device_info.argtypes = [ctypes.c_char_p, ctypes.c_size_t]
+lib.pq_engine_create.argtypes = [ctypes.c_void_p, ctypes.POINTER(ctypes.c_void_p)]
+lib.pq_engine_destroy.argtypes = [ctypes.c_void_p]
+lib.pq_engine_destroy.restype = None
+lib.pq_engine_stats.argtypes = [ctypes.c_void_p, _u64p]
+lib.pq_engine_eval.argtypes = [ctypes.c_void_p, _u8p, ctypes.c_size_t, ctypes.c_int, _f64p]
+lib.pq_engine_eval_batch.argtypes = [ctypes.c_size_t, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p]
+lib.pq_engine_branch_dump.argtypes = [ctypes.c_void_p, _u8p, ctypes.c_size_t, _u64p, _u8p, _u8p, _u8p, _u32p]
 lib.pq_valu_probe.argtypes = [_f64p, ctypes.c_size_t, ctypes.c_uint32, ctypes.c_uint32]
 lib.pq_valu_probe_name.argtypes = [ctypes.c_int]
 lib.pq_valu_probe_name.restype = ctypes.c_char_p
@@ -419,6 +427,86 @@ class Simulator:
         if rc == 1:
             return None
         return k[: n.value], mag[: n.value], s[: n.value]
+
+
+class _EngineDesc(ctypes.Structure):
+    _fields_ = [(n, ctypes.c_uint32) for n in ("nf", "planes", "lines", "radical", "terms", "branch_dim", "rank", "consistency_rows",
+                                            "num_h", "global_phase")] + \
+               [("zero", ctypes.c_int32), ("sigma", ctypes.c_int32), ("free_vars", ctypes.c_uint32)] + \
+               [("ucols", _u64p), ("mcols", _u64p), ("diag", _u64p), ("o3", _u64p), ("fixed_v", _u8p), ("fixed_theta", _u32p),
+                ("b_fixed", _u64p), ("theta", _u32p), ("pivot_col", _u32p), ("pivot_row", _u64p), ("consistency", _u64p),
+                ("kernel", _u64p), ("kernel_u", _u64p)]
+
+
+_DESC_ARRAYS = {"ucols": (np.uint64, _u64p), "mcols": (np.uint64, _u64p), "diag": (np.uint64, _u64p), "o3": (np.uint64, _u64p),
+                "fixed_v": (np.uint8, _u8p), "fixed_theta": (np.uint32, _u32p), "b_fixed": (np.uint64, _u64p),
+                "theta": (np.uint32, _u32p), "pivot_col": (np.uint32, _u32p), "pivot_row": (np.uint64, _u64p),
+                "consistency": (np.uint64, _u64p), "kernel": (np.uint64, _u64p), "kernel_u": (np.uint64, _u64p)}
+
+
+class Engine:
+    """One path-sum Engine (src/engine.rs:273-284) handed to the GPU by its
+    fields (pq_engine_create): the engine-level seam below the Simulator.
+    `fields` holds the sizes and arrays of pq_engine_desc (include/polyq_amd.h)
+    under the same names; kernel_u may be None (derived from mcols)."""
+
+    def __init__(self, fields):
+        d = _EngineDesc()
+        for name, _ in _EngineDesc._fields_[:13]:
+            setattr(d, name, int(fields[name]))
+        self._keep = []
+        for name, (dt, pt) in _DESC_ARRAYS.items():
+            a = fields.get(name)
+            if a is None:
+                setattr(d, name, None)
+                continue
+            a = np.ascontiguousarray(a, dtype=dt)
+            self._keep.append(a)
+            setattr(d, name, a.ctypes.data_as(pt))
+        h = ctypes.c_void_p()
+        _check(lib.pq_engine_create(ctypes.byref(d), ctypes.byref(h)))
+        self._h = h
+        self.nf = int(fields["nf"])
+        self._keep = None
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h:
+            lib.pq_engine_destroy(h)
+            self._h = None
+
+    def stats(self):
+        o = np.zeros(6, np.uint64)
+        _check(lib.pq_engine_stats(self._h, _ptr(o, _u64p)))
+        return tuple(int(v) for v in o)
+
+    def eval(self, values, form="z8"):
+        """Engine::eval (src/engine.rs:551-561): values aligned with fixed()."""
+        v = np.ascontiguousarray(values, dtype=np.uint8)
+        out = (ctypes.c_double * 2)()
+        _check(lib.pq_engine_eval(self._h, _ptr(v, _u8p), len(v), FORMS[form], out))
+        return complex(out[0], out[1])
+
+    def branch_dump(self, values, s_list):
+        v = np.ascontiguousarray(values, dtype=np.uint8)
+        s = np.ascontiguousarray(s_list, dtype=np.uint64)
+        n = len(s)
+        zero = np.zeros(n, np.uint8); k8 = np.zeros(n, np.uint8); kt = np.zeros(n, np.uint8); r = ctypes.c_uint32()
+        _check(lib.pq_engine_branch_dump(self._h, _ptr(v, _u8p), n, _ptr(s, _u64p), _ptr(zero, _u8p), _ptr(k8, _u8p), _ptr(kt, _u8p), ctypes.byref(r)))
+        return zero, k8, kt, r.value
+
+
+def engine_eval_batch(engines, values, form="z8"):
+    """Engine::eval of many (engine, values) queries in one launch
+    (pq_engine_eval_batch); summing the branch engines of a RankEngine for the
+    same values gives RankEngine::eval_assignment (src/rank.rs:184-190)."""
+    n = len(engines)
+    vs = [np.ascontiguousarray(v, dtype=np.uint8) for v in values]
+    hs = (ctypes.c_void_p * n)(*[e._h.value for e in engines])
+    ps = (ctypes.c_void_p * n)(*[v.ctypes.data for v in vs])
+    buf, out = _complex_out(n)
+    _check(lib.pq_engine_eval_batch(n, hs, ps, FORMS[form], buf))
+    return out.copy()
 
 
 def batch_amplitudes(sims, inputs, outputs, form="z8"):

@@ -57,6 +57,11 @@ def lib():
         L.orc_sim_amplitudes_timed.argtypes = [ctypes.c_void_p, ctypes.c_size_t, _u8p, _u8p, ctypes.c_int, _f64p, _f64p]
         L.orc_sim_chunk_range_timed.argtypes = [ctypes.c_void_p, _u8p, _u8p, ctypes.c_uint64, ctypes.c_uint64, ctypes.c_int, _f64p, _f64p, _u64p]
         L.orc_max_threads.restype = ctypes.c_int
+        L.orc_engine_sizes.argtypes = [ctypes.c_void_p, ctypes.c_size_t, _u64p]
+        L.orc_engine_export.argtypes = [ctypes.c_void_p, ctypes.c_size_t, _u64p, _u64p, _u64p, _u64p, _u8p, _u32p, _u64p, _u32p, _u32p,
+                                        _u64p, _u64p, _u64p, _u64p]
+        L.orc_engine_values.argtypes = [ctypes.c_void_p, _u8p, _u8p, _u8p]
+        L.orc_engine_eval.argtypes = [ctypes.c_void_p, ctypes.c_size_t, _u8p, ctypes.c_size_t, _f64p]
         _lib = L
     return _lib
 
@@ -236,3 +241,51 @@ def branch_brute(sim, x, y, s_list):
     if rc == 1:
         return None
     return out[0::2] + 1j * out[1::2]
+
+
+def _w(n):
+    return (n + 63) // 64
+
+
+def engine_fields(osim, branch=0):
+    """The fields of branch engine `branch` of an oracle simulator, in the
+    layout of the library's pq_engine_desc (a dict of sizes and arrays)."""
+    sz = np.zeros(13, np.uint64)
+    _check(lib().orc_engine_sizes(osim.h, branch, _p(sz, _u64p)))
+    nf, g, nodd, ne, t, d, rk, nc, nh, glob, zero, sigma, m = (int(v) for v in sz)
+    sigma = int(np.int64(np.uint64(sigma)))
+    W = 2 * _w(g) + _w(nodd) + _w(ne)
+    a = {
+        "ucols": np.zeros(max(1, nf * W), np.uint64), "mcols": np.zeros(max(1, t * W), np.uint64),
+        "diag": np.zeros(max(1, W), np.uint64), "o3": np.zeros(max(1, _w(nodd)), np.uint64),
+        "fixed_v": np.zeros(max(1, nf), np.uint8), "fixed_theta": np.zeros(max(1, nf), np.uint32),
+        "b_fixed": np.zeros(max(1, nf * _w(nf)), np.uint64), "theta": np.zeros(max(1, t), np.uint32),
+        "pivot_col": np.zeros(max(1, rk), np.uint32), "pivot_row": np.zeros(max(1, rk * _w(ne)), np.uint64),
+        "consistency": np.zeros(max(1, nc * _w(ne)), np.uint64), "kernel": np.zeros(max(1, d * _w(t)), np.uint64),
+        "kernel_u": np.zeros(max(1, d * W), np.uint64),
+    }
+    _check(lib().orc_engine_export(osim.h, branch, _p(a["ucols"], _u64p), _p(a["mcols"], _u64p), _p(a["diag"], _u64p), _p(a["o3"], _u64p),
+                                   _p(a["fixed_v"], _u8p), _p(a["fixed_theta"], _u32p), _p(a["b_fixed"], _u64p), _p(a["theta"], _u32p),
+                                   _p(a["pivot_col"], _u32p), _p(a["pivot_row"], _u64p), _p(a["consistency"], _u64p), _p(a["kernel"], _u64p),
+                                   _p(a["kernel_u"], _u64p)))
+    a.update(nf=nf, planes=g, lines=nodd, radical=ne, terms=t, branch_dim=d, rank=rk, consistency_rows=nc, num_h=nh,
+             global_phase=glob, zero=zero, sigma=sigma, free_vars=m)
+    return a
+
+
+def engine_values(osim, x, y):
+    """The fixed values of <y|C|x> aligned with the engines' fixed variables,
+    or None for a contradictory boundary."""
+    sz = np.zeros(13, np.uint64)
+    _check(lib().orc_engine_sizes(osim.h, 0, _p(sz, _u64p)))
+    vals = np.zeros(max(1, int(sz[0])), np.uint8)
+    rc = _check(lib().orc_engine_values(osim.h, _p(bits(osim.c.n, x), _u8p), _p(bits(osim.c.n, y), _u8p), _p(vals, _u8p)))
+    return None if rc == 1 else vals[:int(sz[0])]
+
+
+def engine_eval(osim, branch, vals):
+    """Engine::eval (engine.rs:551-561) of one branch engine."""
+    out = np.zeros(2, np.float64)
+    v = np.ascontiguousarray(vals, np.uint8)
+    _check(lib().orc_engine_eval(osim.h, branch, _p(v, _u8p), len(v), _p(out, _f64p)))
+    return complex(out[0], out[1])
