@@ -164,6 +164,21 @@ int pq_engine_eval_batch(size_t n, pq_engine* const* engines, const uint8_t* con
 int pq_engine_branch_dump(pq_engine* engine, const uint8_t* values, size_t n, const uint64_t* s, uint8_t* zero, uint8_t* k8,
                           uint8_t* ktot, uint32_t* r);
 
+/* Many amplitudes, as pq_sim_amplitudes (batched forms only), allocating
+ * nothing: no counterpart in the reference; for callers that cannot allocate
+ * device memory (code under HIP graph capture, tight serving loops, harnesses
+ * that forbid hipMalloc).  The staging records, work-queue counters, item
+ * partials and results live in a fixed device array of
+ * pq_noalloc_scratch_bytes() (8 MiB) per GPU, held until the process ends; a
+ * batch that needs more is rejected with an error.  The simulator's engines
+ * must already be resident (any earlier evaluation through an allocating entry
+ * point uploads them), else an error.  No device or pinned host memory is
+ * allocated (host-side planning uses ordinary heap memory).  Calls on one GPU
+ * share the array: the library serialises them, and each call overwrites the
+ * previous call's contents.  Results are bitwise those of pq_sim_amplitudes. */
+int pq_sim_amplitudes_noalloc(pq_sim* sim, size_t n, const uint64_t* inputs, const uint64_t* outputs, int form, double* out);
+size_t pq_noalloc_scratch_bytes(void);
+
 /* Heterogeneous batch: query i is <outputs[i]|sims[i]|inputs[i]>; all
  * branch-engine sums of all queries run in one persistent launch (Z8 forms
  * only).  out has 2n doubles. */

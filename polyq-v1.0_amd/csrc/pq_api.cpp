@@ -121,8 +121,8 @@ void build_jobs(size_t n, int walk, const std::function<const RankEng*(size_t)>&
     // queries only to the caller and the pool's polling workers, so no
     // blocked worker has to be woken.
     constexpr size_t kChunk = 8, kInline = 48;
+    jobs.clear(); owner.clear();       // the jobs of these n queries only, whatever the vectors held
     if (n < kInline) {
-        jobs.clear(); owner.clear();
         for (size_t i = 0; i < n; ++i) {
             check(i);
             add_query_jobs_xy(*rank_of(i), xs[i], ys[i], walk, jobs, owner, i);
@@ -318,6 +318,34 @@ int pq_sim_amplitudes(pq_sim* h, size_t n, const uint64_t* xs, const uint64_t* y
         return 0;
     })
 }
+
+int pq_sim_amplitudes_noalloc(pq_sim* h, size_t n, const uint64_t* xs, const uint64_t* ys, int form, double* out) {
+    PQ_TRY({
+        const int walk = walk_of_form(form);     // the batched forms only
+        const RankEng* R = h->s.eng.get();
+        {
+            std::lock_guard<std::mutex> lk(R->dev_mu);
+            if (R->dev.size() != R->host.size())
+                throw std::invalid_argument("engines are not resident on the device: evaluate once through an allocating entry point first");
+        }
+        thread_local std::vector<Z8Job> jobs;
+        thread_local std::vector<size_t> owner;
+        thread_local std::vector<std::complex<double>> r, res;
+        build_jobs(n, walk, [&](size_t) { return R; }, [&](size_t i) {
+            if (!qubits_ok(h, xs[i]) || !qubits_ok(h, ys[i])) throw std::invalid_argument("basis index has bits beyond the qubit count");
+        }, xs, ys, jobs, owner);
+        res.assign(n, {0, 0});
+        if (!jobs.empty()) {
+            assign_mid_bits(jobs, walk);
+            dev_z8_batch_noalloc(jobs, r, walk);
+            for (size_t i = 0; i < jobs.size(); ++i) res[owner[i]] += r[i];
+        }
+        for (size_t i = 0; i < n; ++i) { out[2 * i] = res[i].real(); out[2 * i + 1] = res[i].imag(); }
+        return 0;
+    })
+}
+
+size_t pq_noalloc_scratch_bytes(void) { return dev_noalloc_scratch_bytes(); }
 
 int pq_sim_amplitude_index(pq_sim* h, uint64_t x, uint64_t y, int form, double* out) { return pq_sim_amplitudes(h, 1, &x, &y, form, out); }
 

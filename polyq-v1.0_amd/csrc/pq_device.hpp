@@ -53,6 +53,12 @@ struct Z8Timing {
 void dev_set_timing(bool on);
 bool dev_get_timing();
 void dev_z8_batch(const std::vector<Z8Job>& jobs, std::vector<std::complex<double>>& out, Z8Timing* timing, int walk);
+// The same batch with every device buffer (staging, queue counters, item
+// partials, results) in a fixed file-scope device array of
+// dev_noalloc_scratch_bytes(): allocates no device or pinned host memory.
+// Throws if the batch needs more.  Results are bitwise those of dev_z8_batch.
+void dev_z8_batch_noalloc(const std::vector<Z8Job>& jobs, std::vector<std::complex<double>>& out, int walk);
+size_t dev_noalloc_scratch_bytes();
 // Asynchronous form: a slot holds one batch in flight on its own stream.
 struct Z8Slot;
 std::shared_ptr<Z8Slot> dev_slot_create();

@@ -1946,6 +1946,74 @@ void dev_z8_batch(const std::vector<Z8Job>& jobs, std::vector<std::complex<doubl
     dev_z8_wait(*slot, out, timing);
 }
 
+// ------------------------------------------------- allocation-free batch ----
+// A batch whose device working memory (staging: queue counters, engine
+// descriptors, query records, scales, output order, item -> query table; the
+// item partials; the query results) lives in one file-scope device array, so
+// the call allocates no device or pinned host memory.  Each device holds its
+// own copy of the array; the host reaches it only through the address
+// hipGetSymbolAddress returns for that device (the array's name is a host-side
+// stand-in, not a device address).  Calls are serialised on the context's
+// lock, so two calls never share the array at the same time; each call
+// overwrites what the previous one left (the staging copy rewrites the
+// counters with zeros, every item writes all its partials, every query its
+// result).
+constexpr size_t kNoAllocScratch = 8u << 20;
+static __device__ __align__(256) uint8_t scratch[kNoAllocScratch];
+
+namespace {
+constexpr int kMaxDevices = 64;
+uint8_t* g_scratch_ptr[kMaxDevices] = {};
+std::mutex g_scratch_mu;
+uint8_t* scratch_on(int device) {
+    if (device < 0 || device >= kMaxDevices) throw std::invalid_argument("device index beyond the scratch table");
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (!g_scratch_ptr[device]) {
+        void* p = nullptr;
+        HIPCHECK(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));   // the current device's copy
+        g_scratch_ptr[device] = (uint8_t*)p;
+    }
+    return g_scratch_ptr[device];
+}
+}  // namespace
+
+size_t dev_noalloc_scratch_bytes() { return kNoAllocScratch; }
+
+void dev_z8_batch_noalloc(const std::vector<Z8Job>& jobs, std::vector<std::complex<double>>& out, int walk) {
+    out.assign(jobs.size(), {0, 0});
+    if (jobs.empty()) return;
+    Ctx& C = ctx();
+    const BatchPlan B = plan_batch(jobs, walk);
+    const uint32_t nq = B.nq();
+    if (nq == 0) return;
+    const StageLayout L = stage_layout(B);
+    const size_t o_part = (L.tot + 255) & ~size_t(255);
+    const size_t o_res = (o_part + B.total_items * 32 + 255) & ~size_t(255);
+    const size_t need = o_res + (size_t)nq * 16;
+    if (need > kNoAllocScratch)
+        throw std::invalid_argument("batch needs " + std::to_string(need) + " bytes of device scratch; the allocation-free entry point has " +
+                                    std::to_string(kNoAllocScratch));
+    // host staging: the batch's own records (ordinary heap memory, no
+    // device or pinned allocation)
+    thread_local std::vector<uint8_t> h;
+    thread_local std::vector<double> hres;
+    h.resize(L.tot);
+    stage_copy(B, L, h.data());
+    hres.resize(2 * (size_t)nq);
+    std::lock_guard<std::mutex> lk(C.mu);
+    uint8_t* d = scratch_on(C.device);
+    double* part = (double*)(d + o_part);
+    double* res = (double*)(d + o_res);
+    HIPCHECK(hipMemcpyAsync(d, h.data(), L.tot, hipMemcpyHostToDevice, C.stream));
+    launch_walks(C, C.stream, B, d, L, part, false);
+    hipLaunchKernelGGL(k_z8_reduce, dim3(nq), dim3(256), 0, C.stream, (const uint64_t*)(d + L.oq), part, (const double2*)(d + L.os),
+                       (const uint32_t*)(d + L.oo), res);
+    HIPCHECK(hipGetLastError());
+    HIPCHECK(hipMemcpyAsync(hres.data(), res, (size_t)nq * 16, hipMemcpyDeviceToHost, C.stream));
+    HIPCHECK(hipStreamSynchronize(C.stream));
+    for (uint32_t i = 0; i < nq; ++i) out[i] = {hres[2 * i], hres[2 * i + 1]};
+}
+
 // ------------------------------------------------------------- hipGraph ----
 struct Z8Graph {
     hipGraph_t graph = nullptr;
