@@ -38,9 +38,12 @@ Further sections (each bounded; --sections picks them):
              batch generator, 50% repeats; 70% amplitude, 20% probability,
              10% 4-qubit marginal; 64 GB HBM budget; p50/p99 latency,
              throughput, and results re-evaluated alone for comparison
+  serve_sweep the serve mix at Poisson rates 200, 400, ... req/s (each up to
+             --sweep-seconds) until p99 bends: p50/p99, throughput, largest
+             batch and overlapped batches per rate
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu] [--no-plain]
-                       [--sections headline,saturated,precision,batch,serve] [--serve-seconds S]
+                       [--sections headline,saturated,precision,batch,serve,serve_sweep] [--serve-seconds S]
 Prints one JSON line.
 """
 import argparse
@@ -423,14 +426,14 @@ def batch(args, pq, W, np):
     return out
 
 
-def serve(args, pq, W, np):
-    """Open-loop Poisson load against the request loop."""
-    rnd = random.Random(2024)
-    rng = W.Rng(77)
-    rate, secs = args.serve_rate, args.serve_seconds
-    # Pre-generate the schedule so the submitting thread only sleeps and submits.
+def _serve_schedule(W, rate, secs, rseed=2024, cseed=77, max_requests=None):
+    """Poisson arrivals at `rate` for `secs` seconds: 50% new circuits of the
+    batch generator, 50% repeats; 70% amplitude, 20% probability, 10% 4-qubit
+    marginal."""
+    rnd = random.Random(rseed)
+    rng = W.Rng(cseed)
     t, sched, circuits, fresh = 0.0, [], [], []
-    while True:
+    while max_requests is None or len(sched) < max_requests:
         t += rnd.expovariate(rate)
         if t >= secs:
             break
@@ -450,6 +453,12 @@ def serve(args, pq, W, np):
             sched.append((t, c, "probability", x, y, ()))
         else:
             sched.append((t, c, "marginal", x, 0, tuple((q, rnd.random() < 0.5) for q in rnd.sample(range(n), 4))))
+    for c in circuits:
+        c.arrays()                  # gate arrays built before the clock starts
+    return sched, circuits, fresh
+
+
+def _serve_run(pq, np, sched, fresh):
     srv = pq.Server(hbm_budget_bytes=64 << 30, workers=8)
     tickets = []
     t0 = time.perf_counter()
@@ -472,9 +481,31 @@ def serve(args, pq, W, np):
                 errors.append({"error": str(e), "kind": kind, "n": c.num_qubits, "measured": [list(m) for m in meas]})
     t_end = time.perf_counter() - t0
     st = srv.stats()
+    hs = srv.host_stats()
     srv.close()
     lat = np.array(lat)
+    pct = lambda a, q: float(np.percentile(a, q)) if len(a) else None
+    out = {
+        "requests": len(sched), "failed": failed, "errors": errors,
+        "p50_ms": pct(lat, 50), "p99_ms": pct(lat, 99), "max_ms": float(lat.max()) if len(lat) else None,
+        # first request of a circuit (host build and upload included) / repeats of resident circuits
+        "new_circuit_p50_ms": pct(lat_new, 50), "new_circuit_p99_ms": pct(lat_new, 99),
+        "repeat_p50_ms": pct(lat_rep, 50), "repeat_p99_ms": pct(lat_rep, 99),
+        "throughput_req_per_s": len(lat) / t_end, "submit_span_s": t_sub, "drain_s": t_end - t_sub,
+        "batches": st["batches"], "overlapped_batches": st["overlapped_batches"], "max_batch": st["max_batch"],
+        "builds": st["builds"], "evictions": st["evictions"], "resident_bytes": st["resident_bytes"],
+        "host_bytes": hs["host_bytes"], "host_drops": hs["host_drops"], "walk_kernel_ms_total": st["kernel_ms"],
+    }
+    return out, vals
+
+
+def serve(args, pq, W, np):
+    """Open-loop Poisson load against the request loop."""
+    rate, secs = args.serve_rate, args.serve_seconds
+    sched, circuits, fresh = _serve_schedule(W, rate, secs)
+    out, vals = _serve_run(pq, np, sched, fresh)
     # Alone vs in the mix: re-evaluate a sample of served requests one by one.
+    rnd = random.Random(7)
     sample = rnd.sample(range(len(sched)), min(96, len(sched)))
     equal, maxdiff, cache = 0, 0.0, {}
     for i in sample:
@@ -493,24 +524,38 @@ def serve(args, pq, W, np):
         equal += int(want == vals[i])
         maxdiff = max(maxdiff, abs(want - vals[i]))
     kinds = {k: sum(1 for r in sched if r[2] == k) for k in ("amplitude", "probability", "marginal")}
-    return {
+    out.update({
         "workload": "Poisson %.0f req/s for %.0f s; 50%% new circuits (n ~ U[20,56], T ~ U[8,32]), 50%% repeats; "
                     "70/20/10 amplitude/probability/4-qubit marginal; 64 GB HBM budget" % (rate, secs),
-        "requests": len(sched), "new_circuits": len(circuits), "kinds": kinds, "failed": failed, "errors": errors,
-        "p50_ms": float(np.percentile(lat, 50)) if len(lat) else None,
-        "p99_ms": float(np.percentile(lat, 99)) if len(lat) else None,
-        "max_ms": float(lat.max()) if len(lat) else None,
-        # first request of a circuit (host build and upload included) / repeats of resident circuits
-        "new_circuit_p50_ms": float(np.percentile(lat_new, 50)) if lat_new else None,
-        "new_circuit_p99_ms": float(np.percentile(lat_new, 99)) if lat_new else None,
-        "repeat_p50_ms": float(np.percentile(lat_rep, 50)) if lat_rep else None,
-        "repeat_p99_ms": float(np.percentile(lat_rep, 99)) if lat_rep else None,
-        "throughput_req_per_s": len(lat) / t_end, "submit_span_s": t_sub, "drain_s": t_end - t_sub,
-        "batches": st["batches"], "overlapped_batches": st["overlapped_batches"], "max_batch": st["max_batch"],
-        "builds": st["builds"], "evictions": st["evictions"], "resident_bytes": st["resident_bytes"],
-        "walk_kernel_ms_total": st["kernel_ms"],
+        "new_circuits": len(circuits), "kinds": kinds,
         "alone_vs_mix": {"checked": len(sample), "bitwise_equal": equal, "max_abs_diff": maxdiff},
-    }
+    })
+    return out
+
+
+def serve_sweep(args, pq, W, np):
+    """The same request mix at rising Poisson rates (each for up to
+    --sweep-seconds, at most 40000 requests) until the p99 latency bends:
+    the served capacity of one GPU, with the largest batch and the number of
+    batches that overlapped another on the device."""
+    rows = []
+    base_p99 = None
+    for rate in (200, 400, 800, 1600, 3200, 6400, 12800, 25600):
+        secs = min(args.sweep_seconds, 40000.0 / rate)
+        sched, circuits, fresh = _serve_schedule(W, rate, secs, rseed=rate, cseed=rate + 1)
+        out, _ = _serve_run(pq, np, sched, fresh)
+        row = {"rate_req_per_s": rate, "seconds": secs}
+        row.update({k: out[k] for k in ("requests", "failed", "p50_ms", "p99_ms", "throughput_req_per_s", "max_batch",
+                                        "overlapped_batches", "batches", "builds", "host_drops")})
+        rows.append(row)
+        print("bench: serve_sweep rate %d p50 %.3f p99 %.3f thr %.0f" % (rate, out["p50_ms"], out["p99_ms"], out["throughput_req_per_s"]),
+              file=sys.stderr, flush=True)
+        base_p99 = base_p99 or out["p99_ms"]
+        if out["p99_ms"] > 10 * base_p99 or out["throughput_req_per_s"] < 0.8 * rate or out["failed"]:
+            break
+    return {"workload": "the serve mix (50% new config-4 circuits, 70/20/10 amplitude/probability/4-qubit marginal, 64 GB HBM, "
+                        "8 host workers) at rising Poisson rates; stops once p99 exceeds 10x its value at 200 req/s or "
+                        "throughput falls below 80% of the offered rate", "rows": rows}
 
 
 def main():
@@ -521,9 +566,10 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
-    ap.add_argument("--sections", default="headline,saturated,precision,batch,serve")
+    ap.add_argument("--sections", default="headline,saturated,precision,batch,serve,serve_sweep")
     ap.add_argument("--serve-seconds", type=float, default=60.0)
     ap.add_argument("--serve-rate", type=float, default=200.0)
+    ap.add_argument("--sweep-seconds", type=float, default=5.0)
     ap.add_argument("--batch-circuits", type=int, default=256)
     ap.add_argument("--saturated-reps", type=int, default=5)
     ap.add_argument("--saturated-tuning", default=None, help="offset,max_mid for the saturating launch instead of autotuning")
@@ -561,7 +607,7 @@ def main():
         out = {"valu_probe": probe}
         if "saturated" in sections:
             out["roofline_saturated"] = saturated(args, pq, W, np, probe)
-        for name, fn in (("precision", precision), ("batch", batch), ("serve", serve)):
+        for name, fn in (("precision", precision), ("batch", batch), ("serve", serve), ("serve_sweep", serve_sweep)):
             if name in sections:
                 out[name] = fn(args, pq, W, np)
         print(json.dumps(out))
@@ -573,7 +619,7 @@ def main():
     value = h["branches_per_step"] * args.steps / elapsed
     bw_amps = nq * args.steps / elapsed
     extra = {}
-    for name, fn in (("precision", precision), ("batch", batch), ("serve", serve)):
+    for name, fn in (("precision", precision), ("batch", batch), ("serve", serve), ("serve_sweep", serve_sweep)):
         if name in sections:
             print("bench: section %s" % name, file=sys.stderr, flush=True)
             extra[name] = fn(args, pq, W, np)

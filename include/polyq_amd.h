@@ -72,8 +72,15 @@ int pq_sim_stats(const pq_sim* sim, uint64_t out7[7]);
 uint32_t pq_sim_num_qubits(const pq_sim* sim);
 
 /* Simulator::amplitude (src/sim.rs:171-176): <output|C|input>, one byte per
- * qubit (0/1). */
+ * qubit (0/1), any number of qubits.  Circuits above 64 qubits are served by
+ * this and the other byte-array entry points only: the u64 basis-index entry
+ * points refuse them. */
 int pq_sim_amplitude(pq_sim* sim, const uint8_t* input, const uint8_t* output, int form, double out[2]);
+/* Simulator::probability (src/sim.rs:196-198) with byte-array basis states. */
+int pq_sim_probability_bytes(pq_sim* sim, const uint8_t* input, const uint8_t* output, int form, double* out);
+/* Simulator::marginal_probability (src/sim.rs:225-231) with a byte-array input. */
+int pq_sim_marginal_probability_bytes(pq_sim* sim, const uint8_t* input, const uint32_t* qubits, const uint8_t* outcomes, size_t k,
+                                      int form, double* out);
 /* Simulator::amplitude_index (src/sim.rs:179-182). */
 int pq_sim_amplitude_index(pq_sim* sim, uint64_t input, uint64_t output, int form, double out[2]);
 /* Simulator::probability (src/sim.rs:196-198) = |amplitude|^2. */

@@ -108,6 +108,7 @@ void build_jobs(size_t n, int walk, const std::function<const RankEng*(size_t)>&
                 const uint64_t* xs, const uint64_t* ys, std::vector<Z8Job>& jobs, std::vector<size_t>& owner) {
     const RankEng* last = nullptr;
     for (size_t i = 0; i < n; ++i) {
+        check(i);                       // every index first: a bad one fails the call before any work
         const RankEng* R = rank_of(i);
         if (R != last) {
             if (!R->has_bprog) throw std::logic_error("engine set without a boundary map");
@@ -123,10 +124,7 @@ void build_jobs(size_t n, int walk, const std::function<const RankEng*(size_t)>&
     constexpr size_t kChunk = 8, kInline = 48;
     jobs.clear(); owner.clear();       // the jobs of these n queries only, whatever the vectors held
     if (n < kInline) {
-        for (size_t i = 0; i < n; ++i) {
-            check(i);
-            add_query_jobs_xy(*rank_of(i), xs[i], ys[i], walk, jobs, owner, i);
-        }
+        for (size_t i = 0; i < n; ++i) add_query_jobs_xy(*rank_of(i), xs[i], ys[i], walk, jobs, owner, i);
         return;
     }
     ChunkPool& pool = ChunkPool::get();
@@ -135,10 +133,7 @@ void build_jobs(size_t n, int walk, const std::function<const RankEng*(size_t)>&
     std::vector<std::string> err(pool.threads());
     const unsigned nch = pool.run(n, kChunk, n < 256 ? ChunkPool::kHot + 1 : pool.threads(), [&](size_t lo, size_t hi, unsigned c) {
         try {
-            for (size_t i = lo; i < hi; ++i) {
-                check(i);
-                add_query_jobs_xy(*rank_of(i), xs[i], ys[i], walk, cj[c], co[c], i);
-            }
+            for (size_t i = lo; i < hi; ++i) add_query_jobs_xy(*rank_of(i), xs[i], ys[i], walk, cj[c], co[c], i);
         } catch (std::exception& e) {
             err[c] = e.what();
         }
@@ -195,6 +190,7 @@ std::complex<double> plain_eval(const RankEng& R, const std::vector<std::pair<ui
 }
 
 bool Sim::assignment(uint64_t x, uint64_t y, std::vector<std::pair<uint32_t, uint8_t>>& asg) const {
+    if (ps.nq > 64) throw std::invalid_argument(kWideCircuit);
     return ps.boundary_bits(x, y, asg);
 }
 
@@ -230,12 +226,21 @@ size_t Sim::host_bytes() {
 }
 
 bool Sim::marginal_assignment(uint64_t x, const uint32_t* qubits, const uint8_t* outcomes, size_t k, std::vector<std::pair<uint32_t, uint8_t>>& a) {
+    if (ps.nq > 64) throw std::invalid_argument(kWideCircuit);
     // Sandwich::assignment (sim.rs:106-124): x on both ends, the measured
     // outcomes on the junction variables
     std::vector<uint32_t> jv(k);
     std::vector<uint8_t> jval(k);
     for (size_t i = 0; i < k; ++i) { jv[i] = sw->junction[qubits[i]]; jval[i] = (uint8_t)((outcomes[i] ? 1 : 0) ^ sw->jneg[qubits[i]]); }
     return sw->ps.boundary_bits(x, x, a, k, jv.data(), jval.data());
+}
+
+bool Sim::marginal_assignment_bytes(const uint8_t* x, const uint32_t* qubits, const uint8_t* outcomes, size_t k,
+                                    std::vector<std::pair<uint32_t, uint8_t>>& a) {
+    std::vector<uint32_t> jv(k);
+    std::vector<uint8_t> jval(k);
+    for (size_t i = 0; i < k; ++i) { jv[i] = sw->junction[qubits[i]]; jval[i] = (uint8_t)((outcomes[i] ? 1 : 0) ^ sw->jneg[qubits[i]]); }
+    return sw->ps.boundary(x, x, a, k, jv.data(), jval.data());
 }
 
 }  // namespace pq
@@ -248,7 +253,31 @@ struct pq_graph { std::shared_ptr<Z8Graph> g; std::vector<size_t> owner; size_t 
 #define PQ_TRY(...)                                                                                   \
     try { __VA_ARGS__ } catch (std::exception & e) { g_err = e.what(); return -1; } catch (...) { g_err = "unknown error"; return -1; }
 
-static bool qubits_ok(const pq_sim* h, uint64_t x) { return h->s.ps.nq >= 64 || (x >> h->s.ps.nq) == 0; }
+// The per-branch diagnostics take t-bit branch vectors as u64 words: engines
+// with more than 64 terms are evaluated, not dumped.
+static void diag_terms_ok(const HostEngine& E) {
+    if (E.tdev != E.t) throw std::invalid_argument("per-branch diagnostics need at most 64 non-Clifford terms");
+}
+
+// A u64 basis index addresses at most 64 qubits: wider circuits take the
+// byte-array entry points.
+static bool qubits_ok(const pq_sim* h, uint64_t x) {
+    if (h->s.ps.nq > 64) throw std::invalid_argument(kWideCircuit);
+    return h->s.ps.nq == 64 || (x >> h->s.ps.nq) == 0;
+}
+
+// Sum over the branch engines of R for one (variable, value) assignment
+// (RankEngine::eval_assignment, src/rank.rs:184-190), in the given form.
+static std::complex<double> eval_assignment_form(const RankEng& R, const std::vector<std::pair<uint32_t, uint8_t>>& asg, int form) {
+    if (form == PQ_FORM_PLAIN_FP64) return plain_eval(R, asg);
+    std::vector<Z8Job> jobs;
+    std::vector<size_t> owner;
+    const int walk = walk_of_form(form);
+    add_jobs(R, asg, walk, jobs, owner, 0);
+    std::vector<std::complex<double>> res;
+    run_jobs(jobs, owner, 1, walk, res);
+    return res[0];
+}
 
 extern "C" {
 
@@ -267,16 +296,18 @@ int pq_sim_create(uint32_t nq, size_t ng, const uint8_t* ops, const uint32_t* q0
                   uint32_t pairs, pq_sim** out) {
     PQ_TRY({
         if (!out) throw std::invalid_argument("null output handle");
-        if (nq > 64) throw std::invalid_argument("at most 64 qubits (basis indices are u64)");
         auto h = std::make_unique<pq_sim>();
         h->s.circ.n = nq;
         for (size_t i = 0; i < ng; ++i) h->s.circ.add(ops[i], q0[i], q1 ? q1[i] : 0, ph ? ph[i] : 0);
-        if (h->s.circ.n > 64) throw std::invalid_argument("gate on a qubit beyond 63");
         h->s.strategy = strategy; h->s.pairs_opt = pairs;
         h->s.ps = compile(h->s.circ);
         h->s.eng = build_rank(h->s.ps, h->s.ps.io_vars(), strategy, pairs);
-        h->s.eng->bprog = boundary_prog(h->s.ps, *h->s.eng->host[0]);
-        h->s.eng->has_bprog = true;
+        // the basis-index boundary map serves circuits of at most 64 qubits;
+        // wider ones go through the (variable, value) assignment
+        if (h->s.ps.nq <= 64) {
+            h->s.eng->bprog = boundary_prog(h->s.ps, *h->s.eng->host[0]);
+            h->s.eng->has_bprog = true;
+        }
         *out = h.release();
         return 0;
     })
@@ -350,9 +381,26 @@ size_t pq_noalloc_scratch_bytes(void) { return dev_noalloc_scratch_bytes(); }
 int pq_sim_amplitude_index(pq_sim* h, uint64_t x, uint64_t y, int form, double* out) { return pq_sim_amplitudes(h, 1, &x, &y, form, out); }
 
 int pq_sim_amplitude(pq_sim* h, const uint8_t* input, const uint8_t* output, int form, double* out) {
-    uint64_t x = 0, y = 0;
-    for (uint32_t q = 0; q < h->s.ps.nq; ++q) { x |= (uint64_t)(input[q] ? 1 : 0) << q; y |= (uint64_t)(output[q] ? 1 : 0) << q; }
-    return pq_sim_amplitudes(h, 1, &x, &y, form, out);
+    if (h->s.ps.nq <= 64) {
+        uint64_t x = 0, y = 0;
+        for (uint32_t q = 0; q < h->s.ps.nq; ++q) { x |= (uint64_t)(input[q] ? 1 : 0) << q; y |= (uint64_t)(output[q] ? 1 : 0) << q; }
+        return pq_sim_amplitudes(h, 1, &x, &y, form, out);
+    }
+    PQ_TRY({
+        // sim.rs:171-176 for any number of qubits
+        std::vector<std::pair<uint32_t, uint8_t>> asg;
+        std::complex<double> v(0, 0);
+        if (h->s.ps.boundary(input, output, asg)) v = eval_assignment_form(*h->s.eng, asg, form);
+        out[0] = v.real(); out[1] = v.imag();
+        return 0;
+    })
+}
+
+int pq_sim_probability_bytes(pq_sim* h, const uint8_t* input, const uint8_t* output, int form, double* out) {
+    double a[2];
+    const int r = pq_sim_amplitude(h, input, output, form, a);
+    if (r == 0) *out = a[0] * a[0] + a[1] * a[1];
+    return r;
 }
 
 int pq_sim_probability(pq_sim* h, uint64_t x, uint64_t y, int form, double* out) {
@@ -364,23 +412,27 @@ int pq_sim_probability(pq_sim* h, uint64_t x, uint64_t y, int form, double* out)
 
 int pq_sim_marginal_probability(pq_sim* h, uint64_t x, const uint32_t* qubits, const uint8_t* outcomes, size_t k, int form, double* out) {
     PQ_TRY({
+        if (!qubits_ok(h, x)) throw std::invalid_argument("basis index has bits beyond the qubit count");
         for (size_t i = 0; i < k; ++i) if (qubits[i] >= h->s.ps.nq) throw std::invalid_argument("measured qubit out of range");
         std::vector<uint32_t> qs(qubits, qubits + k);
         const auto Rp = h->s.marginal_engine(qs);
         const RankEng& R = *Rp;
         std::vector<std::pair<uint32_t, uint8_t>> a;
         if (!h->s.marginal_assignment(x, qubits, outcomes, k, a)) { *out = 0; return 0; }
-        std::complex<double> v(0, 0);
-        if (form == PQ_FORM_PLAIN_FP64) v = plain_eval(R, a);
-        else {
-            std::vector<Z8Job> jobs; std::vector<size_t> owner;
-            const int walk = walk_of_form(form);
-            add_jobs(R, a, walk, jobs, owner, 0);
-            std::vector<std::complex<double>> res;
-            run_jobs(jobs, owner, 1, walk, res);
-            v = res[0];
-        }
-        *out = std::max(v.real(), 0.0);   // sim.rs:230
+        *out = std::max(eval_assignment_form(R, a, form).real(), 0.0);   // sim.rs:230
+        return 0;
+    })
+}
+
+int pq_sim_marginal_probability_bytes(pq_sim* h, const uint8_t* input, const uint32_t* qubits, const uint8_t* outcomes, size_t k, int form,
+                                      double* out) {
+    PQ_TRY({
+        for (size_t i = 0; i < k; ++i) if (qubits[i] >= h->s.ps.nq) throw std::invalid_argument("measured qubit out of range");
+        std::vector<uint32_t> qs(qubits, qubits + k);
+        const auto Rp = h->s.marginal_engine(qs);
+        std::vector<std::pair<uint32_t, uint8_t>> a;
+        if (!h->s.marginal_assignment_bytes(input, qubits, outcomes, k, a)) { *out = 0; return 0; }
+        *out = std::max(eval_assignment_form(*Rp, a, form).real(), 0.0);   // sim.rs:230
         return 0;
     })
 }
@@ -657,6 +709,7 @@ int pq_sim_branch_dump(pq_sim* h, uint64_t x, uint64_t y, size_t n, const uint64
         const RankEng& R = *h->s.eng;
         const HostEngine& E = *R.host[0];
         if (E.zero) throw std::invalid_argument("path sum vanishes identically");
+        diag_terms_ok(E);
         Prepared P;
         prepare_assignment(E, asg, P);   // u0 is filled even when K s = kappa is inconsistent
         if (P.u0.empty()) throw std::logic_error("prepare did not produce the boundary shift");
@@ -675,6 +728,7 @@ int pq_sim_walk_dump(pq_sim* h, uint64_t x, uint64_t y, uint32_t mid_bits, size_
         const RankEng& R = *h->s.eng;
         const HostEngine& E = *R.host[0];
         const Z8Image& Z = *R.z8[0];
+        diag_terms_ok(E);
         Prepared P;
         if (!prepare_assignment(E, asg, P)) return 1;
         Z8QueryHost q = z8_query(E, Z, P);
@@ -705,6 +759,7 @@ int pq_sim_selfcheck(pq_sim* h, uint64_t x, uint64_t y, uint32_t trials, uint64_
         if (!h->s.assignment(x, y, asg)) return 1;
         for (size_t b = 0; b < h->s.eng->host.size(); ++b) {
             const HostEngine& E = *h->s.eng->host[b];
+            diag_terms_ok(E);
             Prepared P;
             if (!prepare_assignment(E, asg, P)) continue;
             prepared_ell(E, P);
@@ -756,7 +811,9 @@ int pq_sim_prologue_selfcheck(pq_sim* h, size_t n, const uint64_t* xs, const uin
                 const bool na = prepare_assignment(*R.host[b], asg, A);
                 if (contra || na != nb) { (*bad)++; continue; }
                 if (!na) continue;
-                if (A.phase != B.phase || A.s0 != B.s0 || A.K0 != B.K0 || A.u_start != B.u_start || A.dq != B.dq || A.scale != B.scale) (*bad)++;
+                if (A.phase != B.phase || A.s0 != B.s0 || A.s0_rest != B.s0_rest || A.K0 != B.K0 || A.u_start != B.u_start || A.dq != B.dq ||
+                    A.scale != B.scale)
+                    (*bad)++;
             }
         }
         return 0;
@@ -829,6 +886,7 @@ int pq_sim_seg_selfcheck(pq_sim* h, uint64_t x, uint64_t y, uint32_t trials, uin
         for (size_t b = 0; b < h->s.eng->host.size(); ++b) {
             const HostEngine& E = *h->s.eng->host[b];
             const SegImage& G = *h->s.eng->seg[b];
+            diag_terms_ok(E);
             Prepared P;
             if (!prepare_assignment(E, asg, P)) continue;
             const SegQueryHost q = seg_query(E, G, P);

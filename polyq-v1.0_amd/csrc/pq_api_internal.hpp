@@ -75,7 +75,15 @@ struct Sim {
     }
     std::shared_ptr<const RankEng> marginal_engine(const std::vector<uint32_t>& qubits);
     bool marginal_assignment(uint64_t x, const uint32_t* qubits, const uint8_t* outcomes, size_t k, std::vector<std::pair<uint32_t, uint8_t>>& a);
+    // The same with the input as one byte per qubit (any number of qubits).
+    bool marginal_assignment_bytes(const uint8_t* x, const uint32_t* qubits, const uint8_t* outcomes, size_t k,
+                                   std::vector<std::pair<uint32_t, uint8_t>>& a);
 };
+
+// Error of the u64 basis-index entry points on circuits above 64 qubits.
+constexpr const char* kWideCircuit =
+    "circuit has more than 64 qubits: basis indices do not fit u64 (use pq_sim_amplitude, pq_sim_probability_bytes or "
+    "pq_sim_marginal_probability_bytes)";
 
 void add_jobs(const RankEng& R, const std::vector<std::pair<uint32_t, uint8_t>>& asg, int walk, std::vector<Z8Job>& jobs,
               std::vector<size_t>& owner, size_t qid);

@@ -203,11 +203,28 @@ bool PathSum::boundary_bits(uint64_t x, uint64_t y, std::vector<std::pair<uint32
     return ok;
 }
 
-bool PathSum::boundary(const uint8_t* x, const uint8_t* y, std::vector<std::pair<uint32_t, uint8_t>>& out) const {
-    if (nq > 64) throw std::invalid_argument("at most 64 qubits");
-    uint64_t xb = 0, yb = 0;
-    for (uint32_t q = 0; q < nq; ++q) { xb |= (uint64_t)(x[q] ? 1 : 0) << q; yb |= (uint64_t)(y[q] ? 1 : 0) << q; }
-    return boundary_bits(xb, yb, out, 0, nullptr, nullptr);
+bool PathSum::boundary(const uint8_t* x, const uint8_t* y, std::vector<std::pair<uint32_t, uint8_t>>& out, size_t k, const uint32_t* extra_var,
+                       const uint8_t* extra_val) const {
+    if (nq <= 64) {
+        uint64_t xb = 0, yb = 0;
+        for (uint32_t q = 0; q < nq; ++q) { xb |= (uint64_t)(x[q] ? 1 : 0) << q; yb |= (uint64_t)(y[q] ? 1 : 0) << q; }
+        return boundary_bits(xb, yb, out, k, extra_var, extra_val);
+    }
+    // any number of qubits: the same (variable, value) pairs from the bytes
+    out.clear();
+    std::vector<uint8_t> given(nv, 255);
+    out.reserve(2 * nq + k);
+    bool ok = true;
+    auto put = [&](uint32_t var, uint8_t val) {
+        if (given[var] != 255) { ok = ok && given[var] == val; return; }
+        given[var] = val;
+        out.push_back({var, val});
+    };
+    for (uint32_t q = 0; q < nq; ++q) put(q, x[q] ? 1 : 0);
+    for (uint32_t q = 0; q < nq; ++q) put(out_var[q], (uint8_t)((y[q] ? 1 : 0) ^ out_neg[q]));
+    for (size_t i = 0; i < k; ++i) put(extra_var[i], extra_val[i]);
+    if (!ok) out.clear();
+    return ok;
 }
 
 }  // namespace pq

@@ -284,7 +284,6 @@ std::shared_ptr<HostEngine> build_engine(const PathSum& ps, const std::vector<ui
             if (fidx[j] >= 0) { const uint64_t* c = &cols[(size_t)crow[fidx[j]] * W]; for (uint32_t k = 0; k < W; ++k) E.ucols[(size_t)i * W + k] ^= c[k]; }
         });
     E.t = (uint32_t)term_vars.size();
-    if (E.t > 64) throw std::invalid_argument("more than 64 non-Clifford terms in one engine");
     E.mcols.assign((size_t)E.t * W, 0);
     for (uint32_t k = 0; k < E.t; ++k) std::copy_n(&cols[(size_t)crow[term_vars[k]] * W], W, &E.mcols[(size_t)k * W]);
 
@@ -305,23 +304,24 @@ std::shared_ptr<HostEngine> build_engine(const PathSum& ps, const std::vector<ui
     for (uint32_t l = 0; l < L.neven; ++l)
         if (z4_of(bff, v0, v1, nf.even[l]) == 2) E.diag[L.off_e() + (l >> 6)] |= 1ull << (l & 63);
 
-    // RREF of K = M restricted to the radical coordinates (engine.rs:119-184)
+    // RREF of K = M restricted to the radical coordinates (engine.rs:119-184),
+    // rows over all t terms
     const uint32_t t = E.t, ne = L.neven;
-    std::vector<uint64_t> krow(ne, 0);
+    std::vector<Bits> krow(ne, Bits(t));
     std::vector<Bits> rrow(ne, Bits(ne));
     for (uint32_t e = 0; e < ne; ++e) {
-        for (uint32_t c = 0; c < t; ++c) if ((E.mcols[(size_t)c * W + L.off_e() + (e >> 6)] >> (e & 63)) & 1) krow[e] |= 1ull << c;
+        for (uint32_t c = 0; c < t; ++c) if ((E.mcols[(size_t)c * W + L.off_e() + (e >> 6)] >> (e & 63)) & 1) krow[e].set(c);
         rrow[e].set(e);
     }
     uint32_t rank = 0;
     std::vector<uint8_t> is_piv(t, 0);
     for (uint32_t col = 0; col < t; ++col) {
         uint32_t r = rank;
-        while (r < ne && !((krow[r] >> col) & 1)) ++r;
+        while (r < ne && !krow[r].get(col)) ++r;
         if (r == ne) continue;
         std::swap(krow[rank], krow[r]); std::swap(rrow[rank], rrow[r]);
         for (uint32_t r2 = 0; r2 < ne; ++r2)
-            if (r2 != rank && ((krow[r2] >> col) & 1)) { krow[r2] ^= krow[rank]; rrow[r2] ^= rrow[rank]; }
+            if (r2 != rank && krow[r2].get(col)) { krow[r2] ^= krow[rank]; rrow[r2] ^= rrow[rank]; }
         E.pivot_col.push_back(col);
         is_piv[col] = 1;
         rank++;
@@ -330,24 +330,77 @@ std::shared_ptr<HostEngine> build_engine(const PathSum& ps, const std::vector<ui
     // the R rows are read after the elimination finished (engine.rs:149)
     for (uint32_t i = 0; i < rank; ++i) E.pivot_row.push_back(rrow[i]);
     for (uint32_t r = rank; r < ne; ++r) E.consistency.push_back(rrow[r]);
+    std::vector<Bits> kernel_full;
+    std::vector<uint32_t> free_full;
     for (uint32_t j = 0; j < t; ++j) {
         if (is_piv[j]) continue;
-        uint64_t nvec = 1ull << j;
-        for (uint32_t i = 0; i < rank; ++i) if ((krow[i] >> j) & 1) nvec |= 1ull << E.pivot_col[i];
-        E.kernel.push_back(nvec);
-        E.free_col.push_back(j);
+        Bits nvec(t);
+        nvec.set(j);
+        for (uint32_t i = 0; i < rank; ++i) if (krow[i].get(j)) nvec.set(E.pivot_col[i]);
+        kernel_full.push_back(nvec);
+        free_full.push_back(j);
     }
-    E.d = (uint32_t)E.kernel.size();
-    E.kernel_u.assign((size_t)E.d * W, 0);
-    for (uint32_t j = 0; j < E.d; ++j)
-        for (uint32_t k = 0; k < t; ++k) if ((E.kernel[j] >> k) & 1)
-            for (uint32_t w = 0; w < W; ++w) E.kernel_u[(size_t)j * W + w] ^= E.mcols[(size_t)k * W + w];
-
     for (uint32_t k = 0; k < t; ++k) E.theta.push_back(ps.theta[freev[term_vars[k]]]);
+    place_terms(E, kernel_full, free_full);
+
     for (uint32_t f : fixed) { E.fixed_v.push_back(ps.v[f]); E.fixed_theta.push_back(ps.theta[f]); }
     E.nh = ps.nh; E.global = ps.global; E.zero = ps.zero;
     finish_engine(E);
     return Ep;
+}
+
+void place_terms(HostEngine& E, const std::vector<Bits>& kernel_full, const std::vector<uint32_t>& free_full) {
+    const uint32_t t = E.t, W = E.L.words();
+    // perm[new position] = term; device terms (those a kernel vector flips)
+    // first when t > 64, else the reference's order
+    std::vector<uint32_t> perm;
+    if (t <= 64) {
+        for (uint32_t k = 0; k < t; ++k) perm.push_back(k);
+        E.tdev = t;
+    } else {
+        std::vector<uint8_t> active(t, 0);
+        for (const Bits& n : kernel_full) n.each([&](uint32_t k) { active[k] = 1; });
+        for (uint32_t k = 0; k < t; ++k) if (active[k]) perm.push_back(k);
+        E.tdev = (uint32_t)perm.size();
+        if (E.tdev > 64) throw std::invalid_argument("more than 64 non-Clifford terms vary over the branch subspace");
+        for (uint32_t k = 0; k < t; ++k) if (!active[k]) perm.push_back(k);
+    }
+    std::vector<uint32_t> pos(t);
+    for (uint32_t i = 0; i < t; ++i) pos[perm[i]] = i;
+    std::vector<uint64_t> mc((size_t)t * W);
+    std::vector<Phase> th(t);
+    for (uint32_t i = 0; i < t; ++i) {
+        std::copy_n(&E.mcols[(size_t)perm[i] * W], W, &mc[(size_t)i * W]);
+        th[i] = E.theta[perm[i]];
+    }
+    E.mcols.swap(mc);
+    E.theta.swap(th);
+    // pivots on device terms first (their index is a bit of a u64 set)
+    std::vector<uint32_t> order(E.rk);
+    for (uint32_t i = 0; i < E.rk; ++i) order[i] = i;
+    std::stable_sort(order.begin(), order.end(), [&](uint32_t a, uint32_t b) { return pos[E.pivot_col[a]] < pos[E.pivot_col[b]]; });
+    std::vector<uint32_t> pc;
+    std::vector<Bits> pr;
+    E.rkdev = 0;
+    for (uint32_t i : order) {
+        pc.push_back(pos[E.pivot_col[i]]);
+        pr.push_back(E.pivot_row[i]);
+        if (pc.back() < E.tdev) E.rkdev++;
+    }
+    E.pivot_col.swap(pc);
+    E.pivot_row.swap(pr);
+    E.kernel.clear(); E.free_col.clear();
+    for (size_t j = 0; j < kernel_full.size(); ++j) {
+        uint64_t nv = 0;
+        kernel_full[j].each([&](uint32_t k) { nv |= 1ull << pos[k]; });
+        E.kernel.push_back(nv);
+        E.free_col.push_back(pos[free_full[j]]);
+    }
+    E.d = (uint32_t)E.kernel.size();
+    E.kernel_u.assign((size_t)E.d * W, 0);
+    for (uint32_t j = 0; j < E.d; ++j)
+        for (uint32_t k = 0; k < E.tdev; ++k) if ((E.kernel[j] >> k) & 1)
+            for (uint32_t w = 0; w < W; ++w) E.kernel_u[(size_t)j * W + w] ^= E.mcols[(size_t)k * W + w];
 }
 
 // Everything the evaluator derives from the engine's core fields (fixed
@@ -357,7 +410,7 @@ std::shared_ptr<HostEngine> build_engine(const PathSum& ps, const std::vector<ui
 void finish_engine(HostEngine& E) {
     const ULayout& L = E.L;
     const uint32_t W = L.words(), t = E.t;
-    if (t > 64) throw std::invalid_argument("more than 64 non-Clifford terms in one engine");
+    if (E.tdev > 64 || E.tdev > t) throw std::logic_error("terms not placed");
     E.nfw = (E.nf + 63) / 64;
     E.b_fixed_flat.assign((size_t)E.nf * E.nfw, 0);
     for (uint32_t i = 0; i < E.nf; ++i) std::copy(E.b_fixed[i].w.begin(), E.b_fixed[i].w.end(), &E.b_fixed_flat[(size_t)i * E.nfw]);
@@ -367,6 +420,7 @@ void finish_engine(HostEngine& E) {
     // the plain kernel, and the factorised form for the Z8 kernel.
     E.a0.clear(); E.a1.clear(); E.up.clear(); E.down.clear(); E.r0.clear(); E.r1.clear();
     E.Bmag = 1.0; E.neg_mask = 0;
+    E.neg_rest.assign(t - E.tdev, 0);
     uint64_t tsum = 0;
     for (uint32_t k = 0; k < t; ++k) {
         const Phase th = E.theta[k];
@@ -384,7 +438,10 @@ void finish_engine(HostEngine& E) {
         double mx = std::max(cs, sn);
         E.r0.push_back(cs / mx); E.r1.push_back(sn / mx);
         E.Bmag *= mx;
-        if (th > kPi) E.neg_mask |= 1ull << k;
+        if (th > kPi) {
+            if (k < E.tdev) E.neg_mask |= 1ull << k;
+            else E.neg_rest[k - E.tdev] = 1;
+        }
         tsum += th;
     }
     {
@@ -415,7 +472,8 @@ void finish_engine(HostEngine& E) {
     E.mcol_dq.assign(t, 0);
     for (uint32_t k = 0; k < t; ++k)
         for (uint32_t j = 0; j < d; ++j)
-            if (row_parity(&E.mcols[(size_t)k * W], &E.dual[(size_t)j * W], W) ^ (((E.kernel[j] >> k) & 1) != 0)) E.mcol_dq[k] |= 1ull << j;
+            if (row_parity(&E.mcols[(size_t)k * W], &E.dual[(size_t)j * W], W) ^ (k < E.tdev && ((E.kernel[j] >> k) & 1) != 0))
+                E.mcol_dq[k] |= 1ull << j;
 }
 
 // -------------------------------------------------------------- prologue --
@@ -539,21 +597,44 @@ static bool prepare_ones(const HostEngine& E, const uint64_t* __restrict__ on, P
     for (uint32_t k = 0; k < L.We; ++k) kappa.w[k] = E.diag[L.off_e() + k] ^ P.u0[L.off_e() + k];
     for (auto& r : E.consistency) if (r.dot(kappa)) return false;
     uint64_t s0 = 0;
-    for (uint32_t i = 0; i < E.rk; ++i) if (E.pivot_row[i].dot(kappa)) s0 |= 1ull << E.pivot_col[i];
+    P.s0_rest.clear();
+    for (uint32_t i = 0; i < E.rk; ++i)
+        if (E.pivot_row[i].dot(kappa)) {
+            if (i < E.rkdev) s0 |= 1ull << E.pivot_col[i];
+            else P.s0_rest.push_back(E.pivot_col[i]);
+        }
     P.s0 = s0;
     P.u_start = P.u0;
     uint64_t* __restrict__ us = P.u_start.data();
-    for (uint64_t b = s0; b; b &= b - 1) {
-        const uint32_t k = (uint32_t)__builtin_ctzll(b);
+    auto add_term = [&](uint32_t k) {
         const uint64_t* __restrict__ c = &E.mcols[(size_t)k * W];
         for (uint32_t w = 0; w < W; ++w) us[w] ^= c[w];
         dq ^= E.mcol_dq[k];
-    }
+    };
+    for (uint64_t b = s0; b; b &= b - 1) add_term((uint32_t)__builtin_ctzll(b));
+    for (uint32_t k : P.s0_rest) add_term(k);
     P.K0 = total_phase(E, us, s0);
     // ell_j = lam_j + 4 (parity(u_start . dual_j) ^ parity(s0 . kernel_j)):
     // the query part is the mask dq (prepared_ell spells out ell)
     P.dq = dq;
     P.scale = phase_complex(P.phase) * E.Ephase * E.mag;
+    if (E.tdev < E.t) {
+        // terms no kernel vector flips keep their s0 value on every branch:
+        // their phase 6 s_k + 4 [theta_k > pi, s_k = 0] and magnitude r_k[s_k]
+        // are constants of the query
+        thread_local std::vector<uint8_t> in_s;
+        in_s.assign(E.t - E.tdev, 0);
+        for (uint32_t k : P.s0_rest) in_s[k - E.tdev] = 1;
+        uint32_t kr = 0;
+        double mr = 1.0;
+        for (uint32_t k = E.tdev; k < E.t; ++k) {
+            const bool sk = in_s[k - E.tdev];
+            kr += sk ? 6u : (E.neg_rest[k - E.tdev] ? 4u : 0u);
+            mr *= sk ? E.r1[k] : E.r0[k];
+        }
+        P.K0 = (P.K0 + kr) & 7u;
+        P.scale *= mr;
+    }
     P.nonzero = true;
     return true;
 }
@@ -565,7 +646,16 @@ void prepared_ell(const HostEngine& E, Prepared& P) {
 
 std::complex<double> plain_c_start(const HostEngine& E, uint64_t s0) {
     std::complex<double> c(1, 0);
-    for (uint32_t k = 0; k < E.t; ++k) c *= ((s0 >> k) & 1) ? E.a1[k] : E.a0[k];
+    for (uint32_t k = 0; k < E.tdev; ++k) c *= ((s0 >> k) & 1) ? E.a1[k] : E.a0[k];
+    return c;
+}
+
+std::complex<double> plain_c_rest(const HostEngine& E, const Prepared& P) {
+    std::complex<double> c(1, 0);
+    if (E.tdev == E.t) return c;
+    std::vector<uint8_t> in_s(E.t - E.tdev, 0);
+    for (uint32_t k : P.s0_rest) in_s[k - E.tdev] = 1;
+    for (uint32_t k = E.tdev; k < E.t; ++k) c *= in_s[k - E.tdev] ? E.a1[k] : E.a0[k];
     return c;
 }
 

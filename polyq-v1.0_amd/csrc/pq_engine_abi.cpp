@@ -35,7 +35,6 @@ std::shared_ptr<HostEngine> engine_from_desc(const pq_engine_desc& D) {
     auto Ep = std::make_shared<HostEngine>();
     HostEngine& E = *Ep;
     const uint32_t nf = D.nf, t = D.terms, d = D.branch_dim, rk = D.rank, ne = D.radical;
-    if (t > 64) throw std::invalid_argument("more than 64 non-Clifford terms in one engine");
     if (rk + d != t) throw std::invalid_argument("RREF rank plus kernel dimension differs from the number of terms");
     if (rk + D.consistency_rows != ne) throw std::invalid_argument("RREF rank plus consistency rows differs from the radical size");
     if (!D.ucols && nf) throw std::invalid_argument("ucols missing");
@@ -105,31 +104,36 @@ std::shared_ptr<HostEngine> engine_from_desc(const pq_engine_desc& D) {
         if (!tail_clear(D.consistency + (size_t)r * L.We, ne)) throw std::invalid_argument("consistency row has bits beyond the radical");
         E.consistency.push_back(bits_from(D.consistency + (size_t)r * L.We, ne));
     }
-    E.kernel_u.assign((size_t)d * W, 0);
+    // kernel vectors over all t terms, each with one non-pivot column, in
+    // increasing order of that column (engine.rs:159-171)
+    std::vector<Bits> kernel_full;
+    std::vector<uint32_t> free_full;
     for (uint32_t j = 0; j < d; ++j) {
-        const uint64_t nvec = t ? D.kernel[(size_t)j * tw] : 0;
-        if (t < 64 && (nvec >> t)) throw std::invalid_argument("kernel vector has bits beyond the terms");
+        const uint64_t* row = D.kernel + (size_t)j * tw;
+        if (!tail_clear(row, t)) throw std::invalid_argument("kernel vector has bits beyond the terms");
+        Bits n = bits_from(row, t);
         int32_t free_col = -1;
-        for (uint32_t k = 0; k < t; ++k)
-            if (((nvec >> k) & 1) && !is_piv[k]) {
-                if (free_col >= 0) throw std::invalid_argument("kernel vector with two non-pivot columns");
-                free_col = (int32_t)k;
-            }
-        if (free_col < 0 || (j && (uint32_t)free_col <= E.free_col.back()))
+        n.each([&](uint32_t k) {
+            if (is_piv[k]) return;
+            if (free_col >= 0) throw std::invalid_argument("kernel vector with two non-pivot columns");
+            free_col = (int32_t)k;
+        });
+        if (free_col < 0 || (j && (uint32_t)free_col <= free_full.back()))
             throw std::invalid_argument("kernel vectors must carry increasing non-pivot columns");
-        E.kernel.push_back(nvec);
-        E.free_col.push_back((uint32_t)free_col);
-        // kernel_u[j] = xor of mcols over the support (engine.rs:172-182)
-        uint64_t* u = &E.kernel_u[(size_t)j * W];
-        for (uint32_t k = 0; k < t; ++k)
-            if ((nvec >> k) & 1)
-                for (uint32_t w = 0; w < W; ++w) u[w] ^= E.mcols[(size_t)k * W + w];
+        kernel_full.push_back(n);
+        free_full.push_back((uint32_t)free_col);
+    }
+    E.theta.assign(D.theta, D.theta + t);
+    place_terms(E, kernel_full, free_full);
+    // kernel_u[j] = xor of mcols over the support (engine.rs:172-182): the
+    // e part vanishes inside ker K, and a supplied kernel_u must agree
+    for (uint32_t j = 0; j < d; ++j) {
+        const uint64_t* u = &E.kernel_u[(size_t)j * W];
         for (uint32_t w = 0; w < L.We; ++w)
             if (u[L.off_e() + w]) throw std::invalid_argument("kernel vector is not in the kernel of K");
         if (D.kernel_u && std::memcmp(u, D.kernel_u + (size_t)j * W, (size_t)W * 8) != 0)
             throw std::invalid_argument("kernel_u differs from the xor of mcols over the kernel support");
     }
-    E.theta.assign(D.theta, D.theta + t);
     finish_engine(E);
     return Ep;
 }
@@ -241,6 +245,7 @@ int pq_engine_branch_dump(pq_engine* e, const uint8_t* values, size_t n, const u
         const RankEng& R = *e->r;
         const HostEngine& E = *R.host[0];
         if (E.zero) throw std::invalid_argument("path sum vanishes identically");
+        if (E.tdev != E.t) throw std::invalid_argument("per-branch diagnostics need at most 64 non-Clifford terms");
         Prepared P;
         prepare(E, values, P);          // u0 is filled even when K s = kappa is inconsistent
         if (P.u0.empty()) throw std::logic_error("prepare did not produce the boundary shift");

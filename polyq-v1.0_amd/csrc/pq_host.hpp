@@ -84,9 +84,11 @@ struct PathSum {
 
     std::vector<uint32_t> io_vars() const;                                  // pathsum.rs:328-336
     void substitute(uint32_t a, uint32_t b, bool c);                        // pathsum.rs:360-394
-    // Boundary (var, value) pairs for <y|C|x>; false if contradictory
+    // Boundary (var, value) pairs for <y|C|x>, one byte per qubit, any number
+    // of qubits, with k extra (var, value) pairs; false if contradictory
     // (pathsum.rs:430-450).
-    bool boundary(const uint8_t* x, const uint8_t* y, std::vector<std::pair<uint32_t, uint8_t>>& out) const;
+    bool boundary(const uint8_t* x, const uint8_t* y, std::vector<std::pair<uint32_t, uint8_t>>& out, size_t k = 0,
+                  const uint32_t* extra_var = nullptr, const uint8_t* extra_val = nullptr) const;
     // Same from basis indices (bit q = qubit q), with k extra (var, value) pairs.
     bool boundary_bits(uint64_t x, uint64_t y, std::vector<std::pair<uint32_t, uint8_t>>& out, size_t k = 0,
                        const uint32_t* extra_var = nullptr, const uint8_t* extra_val = nullptr) const;
@@ -123,6 +125,13 @@ struct EngineStats { uint64_t free_vars, planes, lines, radical, terms, branch_d
 struct HostEngine {
     // sizes
     uint32_t m = 0, t = 0, d = 0, nf = 0, rk = 0;   // rk = rank K
+    // Device terms: the terms a kernel vector can flip, placed first (term
+    // indices 0..tdev-1), at most 64, so a branch vector over them is one u64.
+    // With t <= 64 every term is a device term and the order is the
+    // reference's; above 64, the terms no kernel vector touches come after
+    // them and are constant over the walk (folded into each query's prologue).
+    // rkdev pivots (the first ones) sit on device terms.
+    uint32_t tdev = 0, rkdev = 0;
     uint32_t nh = 0;
     Phase global = 0;
     bool zero = false;
@@ -148,12 +157,13 @@ struct HostEngine {
     std::vector<uint32_t> pivot_col;         // rk
     std::vector<Bits> pivot_row;             // rk rows over neven
     std::vector<Bits> consistency;           // rows over neven
-    std::vector<uint64_t> kernel;            // d kernel vectors over t (t <= 64)
+    std::vector<uint64_t> kernel;            // d kernel vectors over the device terms
     std::vector<uint32_t> free_col;          // d: the non-pivot column of kernel j
     std::vector<uint64_t> kernel_u;          // d packed shift rows
     // rotation terms
     std::vector<Phase> theta;                // t
-    uint64_t neg_mask = 0;                   // theta_k > pi
+    uint64_t neg_mask = 0;                   // theta_k > pi, device terms
+    std::vector<uint8_t> neg_rest;           // theta_k > pi for terms tdev..t-1
     // factorised weights: weight(s) = E * B * omega^{6|s| + 4|neg\s|} * prod_k r_k[s_k]
     std::vector<double> r0, r1;              // per term, max(r0, r1) = 1
     double Bmag = 1.0;                       // prod_k max(|cos|, sin)
@@ -176,6 +186,10 @@ struct HostEngine {
 
 // Build an engine for `ps` with the given fixed variables (engine.rs:289-370).
 std::shared_ptr<HostEngine> build_engine(const PathSum& ps, const std::vector<uint32_t>& fixed);
+// Place the terms (device terms first, see HostEngine::tdev) given the RREF
+// kernel vectors over all t terms and their non-pivot columns: permutes mcols,
+// theta and the pivots, and sets kernel, free_col, kernel_u, tdev and rkdev.
+void place_terms(HostEngine& E, const std::vector<Bits>& kernel_full, const std::vector<uint32_t>& free_full);
 // Derive the evaluator's data from an engine's core fields (fixed values,
 // fixed edges b_fixed, layout L, ucols, mcols, diag, o3, sigma, the RREF
 // pivot_col / pivot_row / consistency, kernel, free_col, kernel_u, theta, nh,
@@ -187,7 +201,8 @@ void finish_engine(HostEngine& E);
 struct Prepared {
     bool nonzero = false;            // false: the amplitude is exactly 0
     Phase phase = 0;                 // boundary phase (integer exact)
-    uint64_t s0 = 0;                 // particular solution over the t terms
+    uint64_t s0 = 0;                 // particular solution over the device terms
+    std::vector<uint32_t> s0_rest;   // its terms beyond the device terms (t > 64 only)
     std::vector<uint64_t> u_start;   // packed shift row at c = 0
     std::vector<uint64_t> u0;        // shift row of the boundary alone (s = 0)
     uint32_t K0 = 0;                 // total Z8 phase at c = 0
@@ -197,9 +212,11 @@ struct Prepared {
 };
 bool prepare(const HostEngine& E, const uint8_t* values, Prepared& P);
 // Reference-style FP64 weight at s0 (engine.rs:229-238: the product of the
-// term multipliers a1[k] for k in s0, a0[k] otherwise, in term order); only
-// the plain FP64 form uses it.
+// term multipliers a1[k] for k in s0, a0[k] otherwise, in term order) over the
+// device terms, and the constant product over the other terms; only the plain
+// FP64 form uses them.
 std::complex<double> plain_c_start(const HostEngine& E, uint64_t s0);
+std::complex<double> plain_c_rest(const HostEngine& E, const Prepared& P);
 // The d linear Z8 coefficients of a prepared query, ell_j = lam_j + 4 dq_j
 // (the prologue itself keeps only the mask dq).
 void prepared_ell(const HostEngine& E, Prepared& P);

@@ -244,17 +244,17 @@ BlobLayout build_blob(const HostEngine& E, const Z8Image& Z, const SegImage& G, 
         }
         put_u32(diag, E.diag.data() + L.off_a(), L.Wg, L.g, NW); put_u32(diag, E.diag.data() + L.off_b(), L.Wg, L.g, NW);
         put_u32(diag, E.diag.data() + L.off_o(), L.Wo, L.nodd, NW); put_u32(diag, E.o3.data(), L.Wo, L.nodd, NW);
-        for (uint32_t k = 0; k < E.t; ++k) {
+        for (uint32_t k = 0; k < E.tdev; ++k) {
             const uint64_t* r = &E.mcols[(size_t)k * W];
             put_u32(mrow, r + L.off_a(), L.Wg, L.g, NW); put_u32(mrow, r + L.off_b(), L.Wg, L.g, NW);
             put_u32(mrow, r + L.off_o(), L.Wo, L.nodd, NW); put_u32(mrow, r + L.off_e(), L.We, L.neven, NWe);
         }
         put_u32(we, E.diag.data() + L.off_e(), L.We, L.neven, NWe);
         for (uint32_t j = 0; j < E.d; ++j) {
-            for (uint32_t k = 0; k < E.t; ++k) if ((E.kernel[j] >> k) & 1) supp.push_back((uint8_t)k);
+            for (uint32_t k = 0; k < E.tdev; ++k) if ((E.kernel[j] >> k) & 1) supp.push_back((uint8_t)k);
             soff.push_back((uint32_t)supp.size());
         }
-        for (uint32_t k = 0; k < E.t; ++k) {
+        for (uint32_t k = 0; k < E.tdev; ++k) {
             a0.push_back(E.a0[k].real()); a0.push_back(E.a0[k].imag());
             a1.push_back(E.a1[k].real()); a1.push_back(E.a1[k].imag());
             up.push_back(E.up[k].real()); up.push_back(E.up[k].imag());
@@ -317,7 +317,7 @@ std::shared_ptr<DevEngine> dev_engine_upload(std::shared_ptr<const HostEngine> E
     P.ku = (const uint32_t*)at(O.o_ku); P.kn = (const uint64_t*)at(O.o_kn); P.soff = (const uint32_t*)at(O.o_soff); P.supp = at(O.o_supp);
     P.a0 = (const double2*)at(O.o_a0); P.a1 = (const double2*)at(O.o_a1); P.up = (const double2*)at(O.o_up); P.down = (const double2*)at(O.o_dn);
     P.diag = (const uint32_t*)at(O.o_diag); P.mrow = (const uint32_t*)at(O.o_mrow); P.we = (const uint32_t*)at(O.o_we);
-    P.d = E.d; P.t = E.t; P.NW = NW; P.NWe = NWe; P.sigma = E.sigma;
+    P.d = E.d; P.t = E.tdev; P.NW = NW; P.NWe = NWe; P.sigma = E.sigma;
     P.rbits = 2 * L.g + 2 * L.neven + L.nodd; P.neg = E.neg_mask;
     Z8Dev& z = D->z;
     z.slot_q = (const uint64_t*)at(O.o_sq); z.slot_n = (const uint64_t*)at(O.o_sn); z.slot_pin = (const uint32_t*)at(O.o_sp); z.slot_lam = (const uint32_t*)at(O.o_sl);
@@ -341,6 +341,7 @@ struct PlainQ {
     uint32_t ustart[96];     // [a | b | o] padded to NW dwords each
     uint64_t s0;
     double cre, cim;
+    double fre, fim;         // product of the term multipliers of the terms beyond the device terms (1 when t <= 64)
     uint32_t L;
 };
 
@@ -349,8 +350,8 @@ __device__ __forceinline__ void cmul(double& r, double& i, double mr, double mi)
     const double ni = r * mi + i * mr;
     r = nr; i = ni;
 }
-__device__ __forceinline__ void plain_weight(const PlainDev& E, uint64_t sv, double& cr, double& ci) {
-    cr = 1.0; ci = 0.0;
+__device__ __forceinline__ void plain_weight(const PlainDev& E, const PlainQ& Q, uint64_t sv, double& cr, double& ci) {
+    cr = Q.fre; ci = Q.fim;
     for (uint32_t k = 0; k < E.t; ++k) {
         const double2 a = ((sv >> k) & 1) ? E.a1[k] : E.a0[k];
         cmul(cr, ci, a.x, a.y);
@@ -386,7 +387,7 @@ __global__ __launch_bounds__(256) void k_plain(PlainDev E, PlainQ Q, double* __r
                     sv ^= E.kn[L + j];
                 }
             }
-            plain_weight(E, sv, cr, ci);
+            plain_weight(E, Q, sv, cr, ci);
         }
         const uint32_t steps = 1u << L;
         for (uint32_t i = 0; i < steps; ++i) {
@@ -409,7 +410,7 @@ __global__ __launch_bounds__(256) void k_plain(PlainDev E, PlainQ Q, double* __r
                 for (int w = 0; w < NW; ++w) { ua[w] ^= row[w]; ub[w] ^= row[NW + w]; uo[w] ^= row[2 * NW + w]; }
                 if (((i + 1) & 2047u) == 0) {
                     sv ^= E.kn[j];
-                    plain_weight(E, sv, cr, ci);
+                    plain_weight(E, Q, sv, cr, ci);
                 } else {
                     for (uint32_t e = E.soff[j]; e < E.soff[j + 1]; ++e) {
                         const uint32_t k = E.supp[e];
@@ -473,7 +474,7 @@ __global__ __launch_bounds__(256) void k_estimate(PlainDev E, PlainQ Q, uint64_t
             }
         }
         double cr, ci2;
-        plain_weight(E, sv, cr, ci2);
+        plain_weight(E, Q, sv, cr, ci2);
         int pc1 = 0, pc3 = 0, cnt = 0;
 #pragma unroll
         for (int w = 0; w < NW; ++w) {
@@ -516,8 +517,10 @@ PlainQ plain_query(const DevEngine& D, const Prepared& P) {
     put_u32(us, P.u_start.data() + L.off_o(), L.Wo, L.nodd, E.NW);
     if (us.size() > 96) throw std::invalid_argument("plain kernel state above 96 dwords");
     std::copy(us.begin(), us.end(), Q.ustart);
-    const std::complex<double> cs = plain_c_start(H, P.s0);
+    const std::complex<double> cf = plain_c_rest(H, P);
+    const std::complex<double> cs = plain_c_start(H, P.s0) * cf;
     Q.s0 = P.s0; Q.cre = cs.real(); Q.cim = cs.imag();
+    Q.fre = cf.real(); Q.fim = cf.imag();
     return Q;
 }
 }  // namespace

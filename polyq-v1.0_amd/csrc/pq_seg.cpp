@@ -151,11 +151,11 @@ void fwht(double* v, uint32_t n) {
 
 SegImage build_seg_image(const HostEngine& E) {
     SegImage S;
-    const uint32_t d = E.d, t = E.t;
+    const uint32_t d = E.d, t = E.tdev;   // device terms (see HostEngine::tdev)
     if (d > 56) throw std::invalid_argument("branch dimension above 56 is not supported");
     S.d = d; S.t = t;
     std::vector<int32_t> piv_index(t, -1);
-    for (uint32_t i = 0; i < E.rk; ++i) piv_index[E.pivot_col[i]] = (int32_t)i;
+    for (uint32_t i = 0; i < E.rkdev; ++i) piv_index[E.pivot_col[i]] = (int32_t)i;
     auto pivots_of = [&](uint64_t N) { uint64_t pm = 0; for (uint32_t k = 0; k < t; ++k) if (((N >> k) & 1) && piv_index[k] >= 0) pm |= 1ull << piv_index[k]; return pm; };
 
     // Inner directions within a table of 2^(I + npin) entries: pivot-free ones
@@ -178,10 +178,10 @@ SegImage build_seg_image(const HostEngine& E) {
         S.inner_j.push_back((uint32_t)best); used[best] = 1; pin_set |= pivots_of(E.kernel[best]);
     }
     S.I = (uint32_t)S.inner_j.size();
-    std::vector<int32_t> cpos(E.rk, -1);
-    for (uint32_t i = 0; i < E.rk; ++i) if ((pin_set >> i) & 1) { cpos[i] = (int32_t)S.pin_term.size(); S.pin_term.push_back(E.pivot_col[i]); }
+    std::vector<int32_t> cpos(E.rkdev, -1);
+    for (uint32_t i = 0; i < E.rkdev; ++i) if ((pin_set >> i) & 1) { cpos[i] = (int32_t)S.pin_term.size(); S.pin_term.push_back(E.pivot_col[i]); }
     S.npin = (uint32_t)S.pin_term.size();
-    auto pin_of = [&](uint64_t N) { uint32_t pm = 0; const uint64_t pv = pivots_of(N); for (uint32_t i = 0; i < E.rk; ++i) if (((pv >> i) & 1) && cpos[i] >= 0) pm |= 1u << cpos[i]; return pm; };
+    auto pin_of = [&](uint64_t N) { uint32_t pm = 0; const uint64_t pv = pivots_of(N); for (uint32_t i = 0; i < E.rkdev; ++i) if (((pv >> i) & 1) && cpos[i] >= 0) pm |= 1u << cpos[i]; return pm; };
 
     // Outer order: lane-low (slots touching pivots first), walkable pivot-free, rest.
     std::vector<uint32_t> rest_p, rest_f;

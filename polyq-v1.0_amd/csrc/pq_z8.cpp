@@ -18,11 +18,11 @@ namespace pq {
 
 Z8Image build_z8_image(const HostEngine& E) {
     Z8Image Z;
-    const uint32_t d = E.d, t = E.t, W = E.L.words();
+    const uint32_t d = E.d, t = E.tdev, W = E.L.words();   // device terms (see HostEngine::tdev)
     if (d > 56) throw std::invalid_argument("branch dimension above 56 is not supported by the Z8 walk");
     Z.d = d; Z.t = t;
     std::vector<int32_t> piv_index(t, -1);
-    for (uint32_t i = 0; i < E.rk; ++i) piv_index[E.pivot_col[i]] = (int32_t)i;
+    for (uint32_t i = 0; i < E.rkdev; ++i) piv_index[E.pivot_col[i]] = (int32_t)i;
     auto pivots_of = [&](uint64_t N) { uint64_t pm = 0; for (uint32_t k = 0; k < t; ++k) if (((N >> k) & 1) && piv_index[k] >= 0) pm |= 1ull << piv_index[k]; return pm; };
 
     // Choose up to 8 inner directions: pivot-free kernel vectors first, then
@@ -76,8 +76,8 @@ Z8Image build_z8_image(const HostEngine& E) {
     // compact numbering of the pivots touched by inner directions
     uint64_t touched = 0;
     for (uint32_t p = 0; p < Z.nin; ++p) touched |= pivots_of(N[p]);
-    std::vector<int32_t> cpos(E.rk, -1);
-    for (uint32_t i = 0; i < E.rk; ++i) if ((touched >> i) & 1) { cpos[i] = (int32_t)Z.pin_term.size(); Z.pin_term.push_back(E.pivot_col[i]); }
+    std::vector<int32_t> cpos(E.rkdev, -1);
+    for (uint32_t i = 0; i < E.rkdev; ++i) if ((touched >> i) & 1) { cpos[i] = (int32_t)Z.pin_term.size(); Z.pin_term.push_back(E.pivot_col[i]); }
     Z.npin = (uint32_t)Z.pin_term.size();
 
     Z.slot_q.assign(S, 0); Z.slot_n.assign(S, 0); Z.slot_pin.assign(S, 0); Z.slot_lam.assign(S, 0);
@@ -89,7 +89,7 @@ Z8Image build_z8_image(const HostEngine& E) {
         Z.slot_lam[p] = (uint8_t)z8_lambda(E, U[p].data(), N[p]);
         z8_dual(E, U[p].data(), &Z.slot_dual[(size_t)p * W]);
         const uint64_t pm = pivots_of(N[p]);
-        for (uint32_t i = 0; i < E.rk; ++i) if (((pm >> i) & 1) && cpos[i] >= 0) Z.slot_pin[p] |= 1u << cpos[i];
+        for (uint32_t i = 0; i < E.rkdev; ++i) if (((pm >> i) & 1) && cpos[i] >= 0) Z.slot_pin[p] |= 1u << cpos[i];
         for (uint32_t q = 0; q < S; ++q)
             if (q != p && Z.slot_real[q] && z8_cross(E, U[p].data(), N[p], U[q].data(), N[q])) Z.slot_q[p] |= 1ull << q;
     }

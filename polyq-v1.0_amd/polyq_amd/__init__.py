@@ -47,6 +47,9 @@ lib.pq_sim_num_qubits.restype = ctypes.c_uint32
 lib.pq_sim_amplitude_index.argtypes = [_sim_p, ctypes.c_uint64, ctypes.c_uint64, ctypes.c_int, _f64p]
 lib.pq_sim_probability.argtypes = [_sim_p, ctypes.c_uint64, ctypes.c_uint64, ctypes.c_int, _f64p]
 lib.pq_sim_marginal_probability.argtypes = [_sim_p, ctypes.c_uint64, _u32p, _u8p, ctypes.c_size_t, ctypes.c_int, _f64p]
+lib.pq_sim_amplitude.argtypes = [_sim_p, _u8p, _u8p, ctypes.c_int, _f64p]
+lib.pq_sim_probability_bytes.argtypes = [_sim_p, _u8p, _u8p, ctypes.c_int, _f64p]
+lib.pq_sim_marginal_probability_bytes.argtypes = [_sim_p, _u8p, _u32p, _u8p, ctypes.c_size_t, ctypes.c_int, _f64p]
 # hot paths take raw addresses (ndarray.ctypes.data) rather than typed pointers
 lib.pq_sim_amplitudes.argtypes = [_sim_p, ctypes.c_size_t, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p]
 lib.pq_batch_amplitudes.argtypes = [ctypes.c_size_t, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p]
@@ -125,6 +128,16 @@ def _index(v):
     return v
 
 
+def _wide_index(v, n):
+    """A basis index of an n-qubit circuit as a Python int in [0, 2^n)."""
+    if isinstance(v, (bool, np.bool_)) or not isinstance(v, (int, np.integer)):
+        raise TypeError("basis index must be an integer, not %s" % type(v).__name__)
+    v = int(v)
+    if v < 0 or v >> n:
+        raise OverflowError("basis index %d outside [0, 2^%d)" % (v, n))
+    return v
+
+
 def _u64_array(v):
     """v (a list, tuple or ndarray of basis indices) as a C-contiguous uint64
     ndarray.  Integer arrays only: a float or bool array raises TypeError, and
@@ -196,6 +209,7 @@ class Circuit:
         self.ops, self.q0, self.q1, self.ph = [], [], [], []
 
     def _push(self, op, a, b=0, ph=0):
+        self._arr = None
         if op != GPHASE:
             self.num_qubits = max(self.num_qubits, a + 1)
         if op in (CZ, CX, SWAP):
@@ -247,9 +261,27 @@ class Circuit:
     def __len__(self):
         return len(self.ops)
 
+    _EXTEND_OPS = {"H": (H, 1), "X": (X, 1), "Y": (Y, 1), "Z": (Z, 1), "S": (S, 1), "SDG": (SDG, 1), "T": (T, 1), "TDG": (TDG, 1),
+                   "CX": (CX, 2), "CZ": (CZ, 2), "SWAP": (SWAP, 2)}
+
+    def extend(self, ops):
+        """Append gates from a flat list of (name, qubits) pairs, names H, X,
+        Y, Z, S, SDG, T, TDG, CX, CZ, SWAP (pybind.rs:101-121); a bad name or
+        qubit count raises ValueError and appends nothing of that entry on."""
+        for name, q in ops:
+            q = [int(v) for v in q]
+            spec = self._EXTEND_OPS.get(name)
+            if spec is None or len(q) != spec[1] or any(v < 0 for v in q):
+                raise ValueError("bad op %s %s" % (name, q))
+            self._push(spec[0], *q)
+        return self
+
     def arrays(self):
-        return (np.asarray(self.ops, dtype=np.uint8), np.asarray(self.q0, dtype=np.uint32),
-                np.asarray(self.q1, dtype=np.uint32), np.asarray(self.ph, dtype=np.uint32))
+        """(ops, q0, q1, phase) as uint8/uint32 arrays; cached until the next gate."""
+        if getattr(self, "_arr", None) is None:
+            self._arr = (np.asarray(self.ops, dtype=np.uint8), np.asarray(self.q0, dtype=np.uint32),
+                         np.asarray(self.q1, dtype=np.uint32), np.asarray(self.ph, dtype=np.uint32))
+        return self._arr
 
 
 class Simulator:
@@ -293,14 +325,27 @@ class Simulator:
     def _form(self, form):
         return FORMS[form or self.form]
 
+    def _qubit_bytes(self, v):
+        """Basis index v (bit q = qubit q) as one byte per qubit."""
+        v = _wide_index(v, self.num_qubits)
+        return np.frombuffer(bytes(((v >> q) & 1) for q in range(self.num_qubits)), dtype=np.uint8)
+
     def amplitude(self, input, output, form=None):
         out = (ctypes.c_double * 2)()
-        _check(lib.pq_sim_amplitude_index(self._h, _index(input), _index(output), self._form(form), out))
+        if self.num_qubits > 64:
+            x, y = self._qubit_bytes(input), self._qubit_bytes(output)
+            _check(lib.pq_sim_amplitude(self._h, _ptr(x, _u8p), _ptr(y, _u8p), self._form(form), out))
+        else:
+            _check(lib.pq_sim_amplitude_index(self._h, _index(input), _index(output), self._form(form), out))
         return (out[0], out[1])
 
     def probability(self, input, output, form=None):
         out = ctypes.c_double()
-        _check(lib.pq_sim_probability(self._h, _index(input), _index(output), self._form(form), ctypes.byref(out)))
+        if self.num_qubits > 64:
+            x, y = self._qubit_bytes(input), self._qubit_bytes(output)
+            _check(lib.pq_sim_probability_bytes(self._h, _ptr(x, _u8p), _ptr(y, _u8p), self._form(form), ctypes.byref(out)))
+        else:
+            _check(lib.pq_sim_probability(self._h, _index(input), _index(output), self._form(form), ctypes.byref(out)))
         return out.value
 
     def marginal(self, input, measured, form=None):
@@ -308,7 +353,13 @@ class Simulator:
         qs = np.asarray([m[0] for m in measured], dtype=np.uint32)
         os_ = np.asarray([1 if m[1] else 0 for m in measured], dtype=np.uint8)
         out = ctypes.c_double()
-        _check(lib.pq_sim_marginal_probability(self._h, _index(input), _ptr(qs, _u32p), _ptr(os_, _u8p), len(qs), self._form(form), ctypes.byref(out)))
+        if self.num_qubits > 64:
+            x = self._qubit_bytes(input)
+            _check(lib.pq_sim_marginal_probability_bytes(self._h, _ptr(x, _u8p), _ptr(qs, _u32p), _ptr(os_, _u8p), len(qs), self._form(form),
+                                                         ctypes.byref(out)))
+        else:
+            _check(lib.pq_sim_marginal_probability(self._h, _index(input), _ptr(qs, _u32p), _ptr(os_, _u8p), len(qs), self._form(form),
+                                                   ctypes.byref(out)))
         return out.value
 
     def amplitudes(self, inputs, outputs, form=None):
@@ -333,14 +384,20 @@ class Simulator:
         _check(lib.pq_sim_amplitude_estimate(self._h, _index(input), _index(output), int(samples), int(seed), self._form(form), _ptr(o, _f64p)))
         return float(o[0]), float(o[1]), float(o[2])
 
-    def statevector(self, input, form=None):
+    def statevector_array(self, input, form=None):
         """All 2^n amplitudes <y|C|input> as a complex128 array indexed by y
-        (sim.rs:201-213; pybind.rs:154-164 returns the same values as raw
-        little-endian complex128 bytes)."""
+        (sim.rs:201-213)."""
         n = self.num_qubits
         out = np.empty(1 << n, dtype=np.complex128)
         _check(lib.pq_sim_statevector(self._h, _index(input), self._form(form), out.ctypes.data))
         return out
+
+    def statevector(self, input, form=None):
+        """The statevector as raw little-endian complex128 bytes (re, im per
+        amplitude, indexed by y), as the reference's binding returns it
+        (pybind.rs:158-168); np.frombuffer(b, np.complex128) reads it back."""
+        a = self.statevector_array(input, form)
+        return a.astype("<c16", copy=False).tobytes()
 
     def sample_many(self, input, shots, seed, form=None):
         """shots measurement outcomes of all qubits, as basis indices

@@ -6,7 +6,8 @@ import os
 import numpy as np
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
-PATH = os.path.join(ROOT, "oracle", "liboracle.so")
+# POLYQ_ORACLE_LIB selects another build of the oracle (the sanitizer run)
+PATH = os.environ.get("POLYQ_ORACLE_LIB") or os.path.join(ROOT, "oracle", "liboracle.so")
 
 _u8p = ctypes.POINTER(ctypes.c_uint8)
 _u16p = ctypes.POINTER(ctypes.c_uint16)

@@ -57,4 +57,3 @@ def test_descriptor_checks():
     bad("kernel vector", kernel=kn)
     o3 = f["o3"].copy(); o3[0] ^= 1
     bad("line", o3=o3)
-    bad("more than 64", terms=65)

@@ -54,7 +54,7 @@ def test_statevector_matches_dense(form):
         c = W.random_clifford_t(n, 25, rng.below(8), trial % 4 == 0, rng) if trial % 2 == 0 else W.random_layered(n, 2 + rng.below(2), rng)
         x = rng.below(1 << n)
         want = O.dense(O.OCircuit.from_circuit(c), x)
-        got = pq.Simulator(c, strategy="cached", form=form).statevector(x)
+        got = pq.Simulator(c, strategy="cached", form=form).statevector_array(x)
         np.testing.assert_allclose(got, want, atol=1e-6 if form == "branch_walk" else 1e-10)
         assert abs(np.sum(np.abs(got) ** 2) - 1) < 1e-6
 
@@ -63,7 +63,7 @@ def test_statevector_across_batches():
     # 2^17 outputs: two batches of 2^16 queries; normalisation and a dense spot check
     rng = W.Rng(5)
     c = W.random_layered(17, 2, rng)
-    got = pq.Simulator(c, strategy="cached").statevector(0)
+    got = pq.Simulator(c, strategy="cached").statevector_array(0)
     assert abs(np.sum(np.abs(got) ** 2) - 1) < 1e-9
     want = O.dense(O.OCircuit.from_circuit(c), 0)
     np.testing.assert_allclose(got, want, atol=1e-10)
@@ -107,3 +107,13 @@ def test_amplitude_estimate_on_the_headline_workload():
     print("estimate", complex(re, im), "+-", se, "exact", exact)
     assert se > 0
     assert abs(complex(re, im) - exact) < 6 * se
+
+
+def test_statevector_bytes_as_the_reference_binding():
+    # pybind.rs:158-168: raw little-endian complex128 bytes, re then im
+    c, _, _ = W.config1(2)
+    s = pq.Simulator(c, strategy="cached")
+    b = s.statevector(5)
+    assert isinstance(b, bytes) and len(b) == 16 << c.num_qubits
+    a = s.statevector_array(5)
+    np.testing.assert_array_equal(np.frombuffer(b, dtype="<f8").reshape(-1, 2), np.stack([a.real, a.imag], axis=1))

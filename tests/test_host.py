@@ -249,3 +249,63 @@ def test_index_marshalling_to_uint64():
         pq._u64_arg(np.array([True]))
     assert pq._index(np.uint64(2**64 - 1)) == 2**64 - 1
     assert list(pq._u64_arg(np.array([7, 0], np.int32))) == [7, 0]
+
+
+def test_circuit_extend_matches_builders_and_rejects_bad_ops():
+    # pybind.rs:101-121: (name, qubits) pairs
+    a = pq.Circuit(3).extend([("H", [0]), ("CX", [0, 1]), ("T", [2]), ("SDG", [1]), ("SWAP", [0, 2]), ("CZ", (1, 2)), ("TDG", [0])])
+    b = pq.Circuit(3).h(0).cx(0, 1).t(2).sdg(1).swap(0, 2).cz(1, 2).tdg(0)
+    for x, y in zip(a.arrays(), b.arrays()):
+        np.testing.assert_array_equal(x, y)
+    assert a.num_qubits == 3 and len(a) == 7
+    for bad in ([("CCX", [0, 1, 2])], [("H", [0, 1])], [("CX", [0])], [("X", [-1])]):
+        with pytest.raises(ValueError, match="bad op"):
+            pq.Circuit(3).extend(bad)
+    # the cached arrays follow later gates
+    n0 = len(a.arrays()[0])
+    a.h(1)
+    assert len(a.arrays()[0]) == n0 + 1
+
+
+def test_wide_circuits_build_and_refuse_u64_indices():
+    # above 64 qubits the engine builds (no device needed), and the u64
+    # basis-index entry points refuse instead of truncating the index
+    c = pq.Circuit(80)
+    c.h(70).cx(70, 3).t(3).h(79).cz(79, 70).t(70).h(70)
+    s = pq.Simulator(c, strategy="cached")
+    small = pq.Circuit(3)
+    small.h(1).cx(1, 0).t(0).h(2).cz(2, 1).t(1).h(1)
+    assert s.stats()[:6] == pq.Simulator(small, strategy="cached").stats()[:6]
+    with pytest.raises(pq.PolyqError, match="more than 64 qubits"):
+        pq._check(pq.lib.pq_sim_amplitude_index(s.handle, 0, 0, 0, (ctypes.c_double * 2)()))
+    with pytest.raises(OverflowError):
+        s._qubit_bytes(1 << 80)
+
+
+def test_engines_with_more_than_64_terms_build_on_the_host():
+    # 60 isolated H P(theta) H gadgets beside an 8-qubit Clifford+T circuit:
+    # 68 terms, of which the gadgets' are constant over the branch subspace
+    rng = W.Rng(3)
+    main = W.random_clifford_t(8, 60, 12, False, rng)
+    main.ccx(0, 1, 2)
+    c = pq.Circuit(68)
+    for op, a, b, ph in zip(main.ops, main.q0, main.q1, main.ph):
+        c._push(op, a, b, ph)
+    for q in range(8, 68):
+        c.h(q).t(q).h(q)
+    st = pq.Simulator(c, strategy="cached").stats()
+    assert st[4] == 68 and st[5] == pq.Simulator(main, strategy="cached").stats()[5]
+    assert st == O.OSim(O.OCircuit.from_circuit(c), "cached").stats()
+    # the prologue of the basis-index path and of the assignment path agree
+    # (the constant terms folded into each query)
+    core = W.sandwich_layer(12, 40, 90, 30, rng)
+    small = pq.Circuit(64)
+    for op, a, b, ph in zip(core.ops, core.q0, core.q1, core.ph):
+        small._push(op, a, b, ph)
+    for q in range(12, 64):
+        small.h(q).tdg(q).h(q)
+    s = pq.Simulator(small, strategy="cached")
+    assert s.stats()[4] > 64
+    xs = [rng.next_u64() for _ in range(32)]
+    ys = [rng.next_u64() for _ in range(32)]
+    assert s.prologue_selfcheck(xs, ys) == 0
