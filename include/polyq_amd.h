@@ -191,6 +191,12 @@ size_t pq_noalloc_scratch_bytes(void);
  * only).  out has 2n doubles. */
 int pq_batch_amplitudes(size_t n, pq_sim* const* sims, const uint64_t* inputs, const uint64_t* outputs, int form, double* out);
 
+/* The number of T-pairs Strategy::Auto decomposes (src/rank.rs:205-222) for
+ * an engine with the given stats (free_vars, planes, lines, radical, terms,
+ * branch_dim), available pairs and T-class pairs; gpu_model != 0 evaluates
+ * the refit of PQ_GPU_AUTO instead.  Returns p >= 0, or -1. */
+int pq_best_pairs(const uint64_t st6[6], uint32_t avail, uint32_t tclass, int gpu_model);
+
 /* Lane walk length of a batched form (no counterpart in the reference).  A
  * lane walks 2^M outer Gray points, M = clamp(n - offset, 0, max_mid), with n
  * the outer coordinates of the query (PQ_FORM_Z8: d - I; PQ_FORM_BRANCH_WALK:

@@ -19,6 +19,7 @@ CSRC = os.path.join(HERE, "csrc")
 OBJ = os.path.join(HERE, "_build")
 OUT = os.path.join(HERE, "polyq_amd", "libpolyq_amd.so")
 
+ROCM_LLVM = "/opt/rocm/llvm/bin"
 HOST_SOURCES = ["pq_compile.cpp", "pq_engine.cpp", "pq_z8.cpp", "pq_seg.cpp", "pq_api.cpp", "pq_engine_abi.cpp", "pq_server.cpp"]
 HIP_SOURCES = ["pq_kernels.hip", "pq_valu.hip"]
 
@@ -54,10 +55,14 @@ def build(jobs=8, verbose=False, sanitize=None, out_dir=None):
                 shutil.copy(os.path.join(HERE, "polyq_amd", f), pkg)
         out = os.path.join(pkg, "libpolyq_amd.so")
         host_flags = [f for f in HOST_FLAGS if f != "-O3"] + ["-O1", "-g", "-fno-omit-frame-pointer", "-fsanitize=" + sanitize]
+    # ThreadSanitizer builds use the ROCm clang and its runtime (GCC 11's
+    # libtsan does not intercept pthread_cond_clockwait, which
+    # condition_variable::wait_for calls, and reports false double locks)
+    host_cxx = os.path.join(ROCM_LLVM, "clang++") if sanitize == "thread" else "g++"
     for s in HOST_SOURCES:
         o = os.path.join(host_obj, s + ".o")
         objs.append(o)
-        cmds.append(["g++"] + host_flags + ["-c", os.path.join(CSRC, s), "-o", o])
+        cmds.append([host_cxx] + host_flags + ["-c", os.path.join(CSRC, s), "-o", o])
     for s in HIP_SOURCES:
         o = os.path.join(OBJ, s + ".o")
         objs.append(o)
@@ -68,9 +73,10 @@ def build(jobs=8, verbose=False, sanitize=None, out_dir=None):
                 print(text)
     tmp = out + ".tmp"
     runtime = []
-    if sanitize:
+    if sanitize and sanitize != "thread":
         # the GCC sanitizer runtimes the instrumented host objects call into
-        runtime = ["-l" + {"address": "asan", "undefined": "ubsan", "thread": "tsan"}[k] for k in sanitize.split(",")]
+        # (a TSan library takes its runtime from the executable that loads it)
+        runtime = ["-l" + {"address": "asan", "undefined": "ubsan"}[k] for k in sanitize.split(",")]
     _run(["hipcc", "--offload-arch=gfx950", "-shared", "-fPIC", "-pthread", "-o", tmp] + objs + runtime)
     os.replace(tmp, out)
     # the CPython fast path of Simulator.amplitudes, linked against the library

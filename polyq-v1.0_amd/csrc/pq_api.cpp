@@ -354,11 +354,8 @@ int pq_sim_amplitudes_noalloc(pq_sim* h, size_t n, const uint64_t* xs, const uin
     PQ_TRY({
         const int walk = walk_of_form(form);     // the batched forms only
         const RankEng* R = h->s.eng.get();
-        {
-            std::lock_guard<std::mutex> lk(R->dev_mu);
-            if (R->dev.size() != R->host.size())
-                throw std::invalid_argument("engines are not resident on the device: evaluate once through an allocating entry point first");
-        }
+        if (!R->on_device())
+            throw std::invalid_argument("engines are not resident on the device: evaluate once through an allocating entry point first");
         thread_local std::vector<Z8Job> jobs;
         thread_local std::vector<size_t> owner;
         thread_local std::vector<std::complex<double>> r, res;
@@ -377,6 +374,13 @@ int pq_sim_amplitudes_noalloc(pq_sim* h, size_t n, const uint64_t* xs, const uin
 }
 
 size_t pq_noalloc_scratch_bytes(void) { return dev_noalloc_scratch_bytes(); }
+
+int pq_best_pairs(const uint64_t* st6, uint32_t avail, uint32_t tclass, int gpu_model) {
+    PQ_TRY({
+        const EngineStats st{st6[0], st6[1], st6[2], st6[3], st6[4], st6[5]};
+        return (int)(gpu_model ? best_pairs_gpu(st, avail, tclass, kGpuBranchSeconds) : best_pairs(st, avail, tclass));
+    })
+}
 
 int pq_sim_amplitude_index(pq_sim* h, uint64_t x, uint64_t y, int form, double* out) { return pq_sim_amplitudes(h, 1, &x, &y, form, out); }
 

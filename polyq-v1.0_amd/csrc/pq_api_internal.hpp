@@ -30,6 +30,10 @@ struct RankEng {
         std::lock_guard<std::mutex> lk(dev_mu);
         dev.clear();
     }
+    bool on_device() const {
+        std::lock_guard<std::mutex> lk(dev_mu);
+        return dev.size() == host.size();
+    }
     size_t device_bytes() const {
         std::lock_guard<std::mutex> lk(dev_mu);
         size_t b = 0;

@@ -285,7 +285,7 @@ std::shared_ptr<HostEngine> build_engine(const PathSum& ps, const std::vector<ui
         });
     E.t = (uint32_t)term_vars.size();
     E.mcols.assign((size_t)E.t * W, 0);
-    for (uint32_t k = 0; k < E.t; ++k) std::copy_n(&cols[(size_t)crow[term_vars[k]] * W], W, &E.mcols[(size_t)k * W]);
+    for (uint32_t k = 0; k < E.t; ++k) std::copy_n(&cols[(size_t)crow[term_vars[k]] * W], W, (E.mcols.data() + (size_t)k * W));
 
     // Diag (normal_form.rs:354-391)
     Bits v0(m), v1(m);
@@ -370,7 +370,7 @@ void place_terms(HostEngine& E, const std::vector<Bits>& kernel_full, const std:
     std::vector<uint64_t> mc((size_t)t * W);
     std::vector<Phase> th(t);
     for (uint32_t i = 0; i < t; ++i) {
-        std::copy_n(&E.mcols[(size_t)perm[i] * W], W, &mc[(size_t)i * W]);
+        std::copy_n((E.mcols.data() + (size_t)perm[i] * W), W, (mc.data() + (size_t)i * W));
         th[i] = E.theta[perm[i]];
     }
     E.mcols.swap(mc);
@@ -413,7 +413,7 @@ void finish_engine(HostEngine& E) {
     if (E.tdev > 64 || E.tdev > t) throw std::logic_error("terms not placed");
     E.nfw = (E.nf + 63) / 64;
     E.b_fixed_flat.assign((size_t)E.nf * E.nfw, 0);
-    for (uint32_t i = 0; i < E.nf; ++i) std::copy(E.b_fixed[i].w.begin(), E.b_fixed[i].w.end(), &E.b_fixed_flat[(size_t)i * E.nfw]);
+    for (uint32_t i = 0; i < E.nf; ++i) std::copy(E.b_fixed[i].w.begin(), E.b_fixed[i].w.end(), (E.b_fixed_flat.data() + (size_t)i * E.nfw));
     E.log2mag = (double)L.g + (double)L.neven + (double)L.nodd / 2.0;   // normal_form.rs:465-467
 
     // Rotation terms.  Reference-style FP64 multipliers (engine.rs:221-238) for
@@ -457,22 +457,22 @@ void finish_engine(HostEngine& E) {
     E.lam.assign(d, 0);
     E.dual.assign((size_t)d * W, 0);
     for (uint32_t j = 0; j < d; ++j) {
-        const uint64_t* U = &E.kernel_u[(size_t)j * W];
+        const uint64_t* U = (E.kernel_u.data() + (size_t)j * W);
         E.lam[j] = z8_lambda(E, U, E.kernel[j]);
-        z8_dual(E, U, &E.dual[(size_t)j * W]);
+        z8_dual(E, U, (E.dual.data() + (size_t)j * W));
         for (uint32_t k = j + 1; k < d; ++k)
-            if (z8_cross(E, U, E.kernel[j], &E.kernel_u[(size_t)k * W], E.kernel[k])) { E.Q[j] |= 1ull << k; E.Q[k] |= 1ull << j; }
+            if (z8_cross(E, U, E.kernel[j], (E.kernel_u.data() + (size_t)k * W), E.kernel[k])) { E.Q[j] |= 1ull << k; E.Q[k] |= 1ull << j; }
     }
 
     E.mag = std::pow(2.0, E.log2mag - (double)E.nh / 2.0) * E.Bmag;
     E.ucol_dq.assign(E.nf, 0);
     for (uint32_t f = 0; f < E.nf; ++f)
         for (uint32_t j = 0; j < d; ++j)
-            if (row_parity(&E.ucols[(size_t)f * W], &E.dual[(size_t)j * W], W)) E.ucol_dq[f] |= 1ull << j;
+            if (row_parity((E.ucols.data() + (size_t)f * W), (E.dual.data() + (size_t)j * W), W)) E.ucol_dq[f] |= 1ull << j;
     E.mcol_dq.assign(t, 0);
     for (uint32_t k = 0; k < t; ++k)
         for (uint32_t j = 0; j < d; ++j)
-            if (row_parity(&E.mcols[(size_t)k * W], &E.dual[(size_t)j * W], W) ^ (k < E.tdev && ((E.kernel[j] >> k) & 1) != 0))
+            if (row_parity((E.mcols.data() + (size_t)k * W), (E.dual.data() + (size_t)j * W), W) ^ (k < E.tdev && ((E.kernel[j] >> k) & 1) != 0))
                 E.mcol_dq[k] |= 1ull << j;
 }
 
@@ -579,13 +579,13 @@ static bool prepare_ones(const HostEngine& E, const uint64_t* __restrict__ on, P
     for (uint32_t wi = 0; wi < nfw; ++wi) {
         for (uint64_t x = on[wi]; x; x &= x - 1) {
             const uint32_t f = wi * 64 + (uint32_t)__builtin_ctzll(x);
-            const uint64_t* __restrict__ b = &E.b_fixed_flat[(size_t)f * nfw];
+            const uint64_t* __restrict__ b = (E.b_fixed_flat.data() + (size_t)f * nfw);
             uint32_t pc = 0;
             for (uint32_t k = 0; k < nfw; ++k) pc += (uint32_t)__builtin_popcountll(b[k] & on[k]);
             z += E.fixed_v[f] + pc;
             th += E.fixed_theta[f];
             dq ^= E.ucol_dq[f];
-            const uint64_t* __restrict__ c = &E.ucols[(size_t)f * W];
+            const uint64_t* __restrict__ c = (E.ucols.data() + (size_t)f * W);
             for (uint32_t k = 0; k < W; ++k) u0[k] ^= c[k];
         }
     }
@@ -607,7 +607,7 @@ static bool prepare_ones(const HostEngine& E, const uint64_t* __restrict__ on, P
     P.u_start = P.u0;
     uint64_t* __restrict__ us = P.u_start.data();
     auto add_term = [&](uint32_t k) {
-        const uint64_t* __restrict__ c = &E.mcols[(size_t)k * W];
+        const uint64_t* __restrict__ c = (E.mcols.data() + (size_t)k * W);
         for (uint32_t w = 0; w < W; ++w) us[w] ^= c[w];
         dq ^= E.mcol_dq[k];
     };

@@ -72,9 +72,9 @@ std::shared_ptr<HostEngine> engine_from_desc(const pq_engine_desc& D) {
         for (uint32_t j = 0; j < nf; ++j)
             if (E.b_fixed[i].get(j) != E.b_fixed[j].get(i)) throw std::invalid_argument("b_fixed is not symmetric");
     E.ucols.assign(D.ucols, D.ucols + (size_t)nf * W);
-    for (uint32_t i = 0; i < nf; ++i) check_urow(&E.ucols[(size_t)i * W], "ucols row");
+    for (uint32_t i = 0; i < nf; ++i) check_urow((E.ucols.data() + (size_t)i * W), "ucols row");
     E.mcols.assign(D.mcols, D.mcols + (size_t)t * W);
-    for (uint32_t k = 0; k < t; ++k) check_urow(&E.mcols[(size_t)k * W], "mcols row");
+    for (uint32_t k = 0; k < t; ++k) check_urow((E.mcols.data() + (size_t)k * W), "mcols row");
     E.diag.assign(D.diag, D.diag + W);
     check_urow(E.diag.data(), "diag row");
     E.o3.assign(D.o3, D.o3 + L.Wo);
@@ -128,7 +128,7 @@ std::shared_ptr<HostEngine> engine_from_desc(const pq_engine_desc& D) {
     // kernel_u[j] = xor of mcols over the support (engine.rs:172-182): the
     // e part vanishes inside ker K, and a supplied kernel_u must agree
     for (uint32_t j = 0; j < d; ++j) {
-        const uint64_t* u = &E.kernel_u[(size_t)j * W];
+        const uint64_t* u = (E.kernel_u.data() + (size_t)j * W);
         for (uint32_t w = 0; w < L.We; ++w)
             if (u[L.off_e() + w]) throw std::invalid_argument("kernel vector is not in the kernel of K");
         if (D.kernel_u && std::memcmp(u, D.kernel_u + (size_t)j * W, (size_t)W * 8) != 0)

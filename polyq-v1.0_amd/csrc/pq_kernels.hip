@@ -83,22 +83,29 @@ struct Ctx {
     DBuf d_engs, d_qs, d_counter, d_partial, d_out, d_misc, d_misc2, d_misc3;
     HBuf h_in, h_out;
 };
+// The context is created by the first call that finds a device; a call that
+// fails (no device) throws and leaves the next call to try again.  A plain
+// mutex rather than std::call_once: an exception leaving call_once's callable
+// is not reset by every pthread_once (ThreadSanitizer's interceptor leaves the
+// flag taken, and every later caller waits for ever).
 Ctx& ctx() {
-    static Ctx* c = nullptr;
-    static std::once_flag once;
-    std::call_once(once, [] {
-        int n = 0;
-        if (hipGetDeviceCount(&n) != hipSuccess || n == 0) throw std::runtime_error("no HIP device available");
-        c = new Ctx();
-        HIPCHECK(hipGetDevice(&c->device));
-        HIPCHECK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
-        HIPCHECK(hipEventCreate(&c->e0)); HIPCHECK(hipEventCreate(&c->e1));
-        HIPCHECK(hipEventCreate(&c->e2)); HIPCHECK(hipEventCreate(&c->e3));
-        hipDeviceProp_t prop;
-        HIPCHECK(hipGetDeviceProperties(&prop, c->device));
-        c->cus = prop.multiProcessorCount;
-    });
-    return *c;
+    static std::atomic<Ctx*> made{nullptr};
+    static std::mutex init_mu;
+    if (Ctx* c = made.load(std::memory_order_acquire)) return *c;
+    std::lock_guard<std::mutex> lk(init_mu);
+    if (Ctx* c = made.load(std::memory_order_relaxed)) return *c;
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess || n == 0) throw std::runtime_error("no HIP device available");
+    std::unique_ptr<Ctx> c(new Ctx());
+    HIPCHECK(hipGetDevice(&c->device));
+    HIPCHECK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+    HIPCHECK(hipEventCreate(&c->e0)); HIPCHECK(hipEventCreate(&c->e1));
+    HIPCHECK(hipEventCreate(&c->e2)); HIPCHECK(hipEventCreate(&c->e3));
+    hipDeviceProp_t prop;
+    HIPCHECK(hipGetDeviceProperties(&prop, c->device));
+    c->cus = prop.multiProcessorCount;
+    made.store(c.get(), std::memory_order_release);
+    return *c.release();
 }
 }  // namespace
 
@@ -239,13 +246,13 @@ BlobLayout build_blob(const HostEngine& E, const Z8Image& Z, const SegImage& G, 
     std::vector<double> a0, a1, up, dn;
     if (has_plain) {
         for (uint32_t j = 0; j < E.d; ++j) {
-            const uint64_t* r = &E.kernel_u[(size_t)j * W];
+            const uint64_t* r = (E.kernel_u.data() + (size_t)j * W);
             put_u32(ku, r + L.off_a(), L.Wg, L.g, NW); put_u32(ku, r + L.off_b(), L.Wg, L.g, NW); put_u32(ku, r + L.off_o(), L.Wo, L.nodd, NW);
         }
         put_u32(diag, E.diag.data() + L.off_a(), L.Wg, L.g, NW); put_u32(diag, E.diag.data() + L.off_b(), L.Wg, L.g, NW);
         put_u32(diag, E.diag.data() + L.off_o(), L.Wo, L.nodd, NW); put_u32(diag, E.o3.data(), L.Wo, L.nodd, NW);
         for (uint32_t k = 0; k < E.tdev; ++k) {
-            const uint64_t* r = &E.mcols[(size_t)k * W];
+            const uint64_t* r = (E.mcols.data() + (size_t)k * W);
             put_u32(mrow, r + L.off_a(), L.Wg, L.g, NW); put_u32(mrow, r + L.off_b(), L.Wg, L.g, NW);
             put_u32(mrow, r + L.off_o(), L.Wo, L.nodd, NW); put_u32(mrow, r + L.off_e(), L.We, L.neven, NWe);
         }

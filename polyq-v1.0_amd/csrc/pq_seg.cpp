@@ -104,6 +104,14 @@ void fwht_scalar(double* v, uint32_t n) {
 }
 typedef double V4 __attribute__((vector_size(32)));
 typedef long long M4 __attribute__((vector_size(32)));
+// Two-source 4-lane shuffle: GCC's builtin takes an index vector, clang's a
+// list of constants (the library is built with g++; the ThreadSanitizer build
+// of the host code uses clang).
+#if defined(__clang__)
+#define PQ_SHUF4(x, y, i0, i1, i2, i3) __builtin_shufflevector(x, y, i0, i1, i2, i3)
+#else
+#define PQ_SHUF4(x, y, i0, i1, i2, i3) __builtin_shuffle(x, y, (M4){i0, i1, i2, i3})
+#endif
 inline V4 ld4(const double* p) { V4 r; std::memcpy(&r, p, 32); return r; }
 inline void st4(double* p, V4 r) { std::memcpy(p, &r, 32); }
 inline void bfly4(V4& a0, V4& a1, V4& a2, V4& a3) {
@@ -111,10 +119,10 @@ inline void bfly4(V4& a0, V4& a1, V4& a2, V4& a3) {
     a0 = s01 + s23; a1 = d01 + d23; a2 = s01 - s23; a3 = d01 - d23;
 }
 inline void transpose4(V4& a, V4& b, V4& c, V4& d) {
-    const V4 t0 = __builtin_shuffle(a, b, (M4){0, 4, 2, 6}), t1 = __builtin_shuffle(a, b, (M4){1, 5, 3, 7});
-    const V4 t2 = __builtin_shuffle(c, d, (M4){0, 4, 2, 6}), t3 = __builtin_shuffle(c, d, (M4){1, 5, 3, 7});
-    a = __builtin_shuffle(t0, t2, (M4){0, 1, 4, 5}); c = __builtin_shuffle(t0, t2, (M4){2, 3, 6, 7});
-    b = __builtin_shuffle(t1, t3, (M4){0, 1, 4, 5}); d = __builtin_shuffle(t1, t3, (M4){2, 3, 6, 7});
+    const V4 t0 = PQ_SHUF4(a, b, 0, 4, 2, 6), t1 = PQ_SHUF4(a, b, 1, 5, 3, 7);
+    const V4 t2 = PQ_SHUF4(c, d, 0, 4, 2, 6), t3 = PQ_SHUF4(c, d, 1, 5, 3, 7);
+    a = PQ_SHUF4(t0, t2, 0, 1, 4, 5); c = PQ_SHUF4(t0, t2, 2, 3, 6, 7);
+    b = PQ_SHUF4(t1, t3, 0, 1, 4, 5); d = PQ_SHUF4(t1, t3, 2, 3, 6, 7);
 }
 // Walsh-Hadamard transform in place: radix-4 passes over the index bit
 // pairs (0,1), (2,3), ..., then one radix-2 pass if log2 n is odd.  Four

@@ -202,9 +202,10 @@ struct Server {
     }
 
     // Host build once per circuit (src/sim.rs:144-148, src/rank.rs:88-123).
-    void build_entry(const std::shared_ptr<Entry>& E, const Req& r) {
+    // True when this call built it.
+    bool build_entry(const std::shared_ptr<Entry>& E, const Req& r) {
         std::lock_guard<std::mutex> bl(E->build_mu);
-        if (E->built || !E->err.empty()) return;
+        if (E->built || !E->err.empty()) return false;
         try {
             auto s = std::make_unique<Sim>();
             s->circ = r.circ;
@@ -217,9 +218,11 @@ struct Server {
             std::lock_guard<std::mutex> lk(mu);
             E->built = true;
             builds++;
+            return true;
         } catch (std::exception& e) {
             E->err = e.what();
         }
+        return false;
     }
 
     static size_t estimate_bytes(const RankEng& R) { return R.image_bytes; }
@@ -228,7 +231,7 @@ struct Server {
     // within the budget.  E is pinned by the caller.
     void make_resident(const std::shared_ptr<Entry>& E, const RankEng& R) {
         std::lock_guard<std::mutex> bl(E->build_mu);
-        if (R.dev.size() == R.host.size()) return;
+        if (R.on_device()) return;
         const size_t est = estimate_bytes(R);
         if (!admit(est, E.get())) throw std::runtime_error("engine does not fit the HBM budget");
         try {
@@ -279,8 +282,7 @@ struct Server {
                 r->entry->last_use = ++use_clock;
             }
             try {
-                const bool was_built = r->entry->built;
-                build_entry(r->entry, *r);
+                const bool built_now = build_entry(r->entry, *r);
                 if (!r->entry->err.empty()) throw std::runtime_error(r->entry->err);
                 Sim& S = *r->entry->sim;
                 std::vector<std::pair<uint32_t, uint8_t>> asg;
@@ -291,7 +293,7 @@ struct Server {
                     make_resident(r->entry, *r->keep);
                     if (S.marginal_assignment(r->x, r->mq.data(), r->mo.data(), r->mq.size(), asg)) add_jobs(*r->keep, asg, walk, r->jobs, owner, 0);
                 } else {
-                    if (!was_built) account_host(r->entry);
+                    if (built_now) account_host(r->entry);
                     make_resident(r->entry, *S.eng);
                     S.eng->ensure_device();
                     add_query_jobs_xy(*S.eng, r->x, r->y, walk, r->jobs, owner, 0);
@@ -447,8 +449,7 @@ struct Server {
             // entry is never evicted, so the images stay after it is released.
             if (kind != 2 && E.build_mu.try_lock()) {
                 if (E.built && E.err.empty()) {
-                    const RankEng& R = *E.sim->eng;
-                    here = R.dev.size() == R.host.size();
+                    here = E.sim->eng->on_device();
                 }
                 E.build_mu.unlock();
                 if (here) { E.pinned++; E.last_use = ++use_clock; }

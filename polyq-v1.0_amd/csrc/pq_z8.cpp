@@ -53,7 +53,7 @@ Z8Image build_z8_image(const HostEngine& E) {
     // bit 0 of gray(i), known at compile time in the kernel.
     std::vector<uint64_t> N;
     std::vector<std::vector<uint64_t>> U;
-    for (uint32_t j : inner) { N.push_back(E.kernel[j]); U.emplace_back(&E.kernel_u[(size_t)j * W], &E.kernel_u[(size_t)j * W] + W); }
+    for (uint32_t j : inner) { N.push_back(E.kernel[j]); U.emplace_back((E.kernel_u.data() + (size_t)j * W), (E.kernel_u.data() + (size_t)j * W) + W); }
     int p1 = -1;
     for (uint32_t p = 0; p < Z.nin; ++p) if ((z8_lambda(E, U[p].data(), N[p]) >> 1) & 1) { p1 = (int)p; break; }
     if (p1 >= 0) {
@@ -66,7 +66,7 @@ Z8Image build_z8_image(const HostEngine& E) {
         Z.imag0 = true;
     }
     while (N.size() < kInner) { N.push_back(0); U.emplace_back(W, 0ull); }
-    for (uint32_t j = 0; j < d; ++j) if (!used[j]) { N.push_back(E.kernel[j]); U.emplace_back(&E.kernel_u[(size_t)j * W], &E.kernel_u[(size_t)j * W] + W); }
+    for (uint32_t j = 0; j < d; ++j) if (!used[j]) { N.push_back(E.kernel[j]); U.emplace_back((E.kernel_u.data() + (size_t)j * W), (E.kernel_u.data() + (size_t)j * W) + W); }
     Z.nslots = (uint32_t)N.size();
     if (Z.nslots > 64) throw std::invalid_argument("too many slots");
     const uint32_t S = Z.nslots;
@@ -85,9 +85,9 @@ Z8Image build_z8_image(const HostEngine& E) {
     for (uint32_t p = 0; p < S; ++p) {
         if (!Z.slot_real[p]) continue;
         Z.slot_n[p] = N[p];
-        std::copy(U[p].begin(), U[p].end(), &Z.slot_u[(size_t)p * W]);
+        std::copy(U[p].begin(), U[p].end(), (Z.slot_u.data() + (size_t)p * W));
         Z.slot_lam[p] = (uint8_t)z8_lambda(E, U[p].data(), N[p]);
-        z8_dual(E, U[p].data(), &Z.slot_dual[(size_t)p * W]);
+        z8_dual(E, U[p].data(), (Z.slot_dual.data() + (size_t)p * W));
         const uint64_t pm = pivots_of(N[p]);
         for (uint32_t i = 0; i < E.rkdev; ++i) if (((pm >> i) & 1) && cpos[i] >= 0) Z.slot_pin[p] |= 1u << cpos[i];
         for (uint32_t q = 0; q < S; ++q)
@@ -170,7 +170,7 @@ Z8QueryHost z8_query(const HostEngine& E, const Z8Image& Z, const Prepared& P) {
     for (uint32_t p = 0; p < Z.nslots; ++p) {
         if (!Z.slot_real[p]) continue;
         uint64_t a = 0;
-        const uint64_t* dr = &Z.slot_dual[(size_t)p * W];
+        const uint64_t* dr = (Z.slot_dual.data() + (size_t)p * W);
         for (uint32_t w = 0; w < W; ++w) a ^= P.u_start[w] & dr[w];
         const bool bit = ((__builtin_popcountll(a) + __builtin_popcountll(P.s0 & Z.slot_n[p])) & 1) != 0;
         if (bit) q.delta |= 1ull << p;

@@ -85,6 +85,7 @@ lib.pq_engine_stats.argtypes = [ctypes.c_void_p, _u64p]
 lib.pq_engine_eval.argtypes = [ctypes.c_void_p, _u8p, ctypes.c_size_t, ctypes.c_int, _f64p]
 lib.pq_engine_eval_batch.argtypes = [ctypes.c_size_t, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p]
 lib.pq_engine_branch_dump.argtypes = [ctypes.c_void_p, _u8p, ctypes.c_size_t, _u64p, _u8p, _u8p, _u8p, _u32p]
+lib.pq_best_pairs.argtypes = [_u64p, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_int]
 lib.pq_valu_probe.argtypes = [_f64p, ctypes.c_size_t, ctypes.c_uint32, ctypes.c_uint32]
 lib.pq_valu_probe_name.argtypes = [ctypes.c_int]
 lib.pq_valu_probe_name.restype = ctypes.c_char_p
@@ -744,6 +745,14 @@ def device_info():
     buf = ctypes.create_string_buffer(256)
     _check(lib.pq_device_info(buf, 256))
     return buf.value.decode()
+
+
+def best_pairs(stats, available_pairs, t_class_pairs, gpu_model=False):
+    """Pairs Strategy::Auto decomposes for an engine of the given stats
+    (free_vars, planes, lines, radical, terms, branch_dim): the cost model of
+    src/rank.rs:205-222, or its GPU refit."""
+    st = np.asarray(list(stats)[:6], dtype=np.uint64)
+    return _check(lib.pq_best_pairs(_ptr(st, _u64p), int(available_pairs), int(t_class_pairs), 1 if gpu_model else 0))
 
 
 def valu_probe(blocks_per_cu=8, iters=16384):

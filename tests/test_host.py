@@ -309,3 +309,17 @@ def test_engines_with_more_than_64_terms_build_on_the_host():
     xs = [rng.next_u64() for _ in range(32)]
     ys = [rng.next_u64() for _ in range(32)]
     assert s.prologue_selfcheck(xs, ys) == 0
+
+
+def test_best_pairs_at_hand_evaluated_points():
+    # rank.rs:205-222 evaluated by hand at config-2 sizes (m = 340 free
+    # variables, t = 34 terms, 17 T-class pairs available): p = 5, 7, 8, 10
+    # for branch dimensions 24, 28, 30, 34
+    for d, p in ((24, 5), (28, 7), (30, 8), (34, 10)):
+        assert pq.best_pairs((340, 0, 0, 0, 34, d), 17, 17) == p
+    # small subspaces are cheaper to walk than to split; the cap is MAX_PAIRS
+    # (rank.rs:44) however large d is
+    assert pq.best_pairs((340, 0, 0, 0, 34, 10), 17, 17) == 0
+    assert pq.best_pairs((40, 0, 0, 0, 120, 120), 60, 60) == 14
+    # the GPU refit keeps p = 0 on the headline engine
+    assert pq.best_pairs((342, 2, 338, 0, 30, 30), 15, 15, gpu_model=True) == 0
