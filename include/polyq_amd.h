@@ -308,11 +308,12 @@ int pq_sim_z8_info(pq_sim* sim, uint32_t out7[7]);
 
 /* VALU issue-rate probe (no counterpart in the reference): one kernel per
  * instruction (int32 xor, and, add, popcount; FP32 FMA; FP64 FMA; one int32
- * xor per FP64 FMA, the branch walk's mix), each lane running 8 independent
+ * xor per FP64 FMA, the branch walk's mix; FP64 FMA with an SGPR operand; FP64
+ * FMA with a DPP row broadcast operand), each lane running 8 independent
  * chains of iters x 64 instructions on blocks_per_cu 256-thread workgroups
- * per CU.  out (3 + 4 x 7 doubles): CUs, wall-clock MHz, blocks per CU, then
+ * per CU.  out (3 + 4 x 9 doubles): CUs, wall-clock MHz, blocks per CU, then
  * per op {lane-ops/s, lanes per clock per CU at the in-kernel clock, that
- * clock in GHz, kernel ms}.  Returns the number of ops (7) or -1. */
+ * clock in GHz, kernel ms}.  Returns the number of ops (9) or -1. */
 int pq_valu_probe(double* out, size_t cap, uint32_t blocks_per_cu, uint32_t iters);
 const char* pq_valu_probe_name(int op);
 

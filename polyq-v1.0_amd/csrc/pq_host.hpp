@@ -261,6 +261,7 @@ uint32_t total_phase(const HostEngine& E, const uint64_t* u, uint64_t s);
 // phantoms whose table entries are zero), then the outer slots, whose first M
 // are walked in Gray order per lane and the rest index the lane's coset.
 constexpr uint32_t kInner = 8;
+constexpr uint32_t kMidFree = 8;     // pivot-free outer slots whose magnitudes a lane takes from a per-query table
 struct Z8Image {
     uint32_t d = 0, nslots = 0, nin = 0, t = 0, npin = 0, nbytes = 0;
     bool imag0 = false;                // inner entries alternate real / imaginary with slot 0
@@ -277,6 +278,16 @@ struct Z8Image {
     std::vector<double> inner_acc;
     std::vector<uint32_t> inner_aux;   // 256 x pin*8 (LDS byte offset of the inner pivots), then 256 x kin
     std::vector<double> lt;            // nbytes x 256 lane-factor byte tables (FP64)
+    // Pivot-free outer slots come first after the inner ones: slots
+    // kInner .. kInner + nx - 1 are single free terms that no other slot
+    // touches (nx <= kMidFree), so as mid slots their magnitudes are the same
+    // for every lane of a query.  mid_term[j] / mid_r[2 j + b] are the term
+    // of slot kInner + j and its magnitude r_b; lt_rest are the lane-factor
+    // byte tables without those nx terms.
+    uint32_t nx = 0;
+    std::vector<uint32_t> mid_term;
+    std::vector<double> mid_r;
+    std::vector<double> lt_rest;
     std::vector<double> gin;           // 2^npin inner pivot factor table (FP64)
     std::vector<uint32_t> pin_term;    // term index of each compact inner pivot bit
     uint64_t lane_terms = 0;           // terms whose ratio is carried by the lane factor

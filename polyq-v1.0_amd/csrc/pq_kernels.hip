@@ -142,7 +142,10 @@ struct Z8Dev {
     const uint32_t* inaux;   // 256 x pin*8, then 256 x kin
     const double* lt;        // nbytes x 256
     const double* gin;       // 2^npin
-    uint32_t nslots, nbytes, npin, imag0;
+    const double* lt_rest;   // nbytes x 256, without the terms of the pivot-free outer slots 8 .. 8 + nx - 1
+    const double* mid_r;     // nx x {r0, r1}: magnitudes of those slots' terms
+    const uint32_t* mid_term;  // nx: their term indices
+    uint32_t nslots, nbytes, npin, imag0, nx, pad;
 };
 struct Z8Q {                 // one query of a batch (64 B)
     uint64_t delta, s0, item0, nitems;
@@ -227,6 +230,7 @@ uint32_t pick_nw(uint32_t bits) {
 namespace {
 struct BlobLayout {
     size_t o_ku, o_kn, o_soff, o_supp, o_a0, o_a1, o_up, o_dn, o_diag, o_mrow, o_we, o_sq, o_sn, o_sp, o_sl, o_in, o_inacc, o_inaux, o_lt, o_gin;
+    size_t o_ltr, o_mr, o_mt;
     size_t g_oq, g_ox, g_opin, g_olam, g_on, g_tab, g_lt, g_fmid, g_rmid, g_wtab, g_pts, g_pcls, g_low;
     uint32_t NW, NWe;
 };
@@ -281,6 +285,8 @@ BlobLayout build_blob(const HostEngine& E, const Z8Image& Z, const SegImage& G, 
     O.o_in = B.put(Z.inner.data(), Z.inner.size() * 8); O.o_inacc = B.put(Z.inner_acc.data(), Z.inner_acc.size() * 8);
     O.o_inaux = B.put(Z.inner_aux.data(), Z.inner_aux.size() * 4);
     O.o_lt = B.put(Z.lt.data(), Z.lt.size() * 8); O.o_gin = B.put(Z.gin.data(), Z.gin.size() * 8);
+    O.o_ltr = B.put(Z.lt_rest.data(), Z.lt_rest.size() * 8); O.o_mr = B.put(Z.mid_r.data(), Z.mid_r.size() * 8);
+    O.o_mt = B.put(Z.mid_term.data(), Z.mid_term.size() * 4);
     // segment-table image
     O.g_oq = B.put(G.oq.data(), G.oq.size() * 8); O.g_ox = B.put(G.ox.data(), G.ox.size() * 4);
     O.g_opin = B.put(G.opin.data(), G.opin.size() * 4); O.g_olam = B.put(G.olam.data(), G.olam.size() * 4);
@@ -330,7 +336,8 @@ std::shared_ptr<DevEngine> dev_engine_upload(std::shared_ptr<const HostEngine> E
     z.slot_q = (const uint64_t*)at(O.o_sq); z.slot_n = (const uint64_t*)at(O.o_sn); z.slot_pin = (const uint32_t*)at(O.o_sp); z.slot_lam = (const uint32_t*)at(O.o_sl);
     z.inner = (const double*)at(O.o_in); z.inner_acc = (const double*)at(O.o_inacc); z.inaux = (const uint32_t*)at(O.o_inaux);
     z.lt = (const double*)at(O.o_lt); z.gin = (const double*)at(O.o_gin);
-    z.nslots = Z.nslots; z.nbytes = Z.nbytes; z.npin = Z.npin; z.imag0 = Z.imag0 ? 1u : 0u;
+    z.lt_rest = (const double*)at(O.o_ltr); z.mid_r = (const double*)at(O.o_mr); z.mid_term = (const uint32_t*)at(O.o_mt);
+    z.nslots = Z.nslots; z.nbytes = Z.nbytes; z.npin = Z.npin; z.imag0 = Z.imag0 ? 1u : 0u; z.nx = Z.nx; z.pad = 0;
     SegDev& g = D->sg;
     g.oq = (const uint64_t*)at(O.g_oq); g.ox = (const uint32_t*)at(O.g_ox); g.opin = (const uint32_t*)at(O.g_opin);
     g.olam = (const uint32_t*)at(O.g_olam); g.on = (const uint64_t*)at(O.g_on); g.table = (const float4*)at(O.g_tab);
@@ -647,6 +654,9 @@ __device__ __forceinline__ void wave_sum_dpp(float (&v)[8]) {
     dpp_merge<0x143, 0xC>(v);    // row_bcast:31
 }
 
+// x with its sign bit flipped where sw has bit 31 set (sw is 0 or 0x80000000)
+__device__ __forceinline__ double flip_sign(double m, uint32_t sw) { return __hiloint2double(__double2hiint(m) ^ (int)sw, __double2loint(m)); }
+
 // Rotate (xr, xi) by i^((K>>1)&3) and add it, in FP64, into the even or odd
 // accumulator (both are updated, the other with zero, so the accumulator
 // index stays static and the array stays in registers).
@@ -654,13 +664,15 @@ __device__ __forceinline__ void fold64(double xr, double xi, uint32_t K, double 
     const uint32_t r2 = (K >> 1) & 3u;
     const double a = (r2 & 1u) ? xi : xr;
     const double b = (r2 & 1u) ? xr : xi;
-    const double yr = (r2 == 1u || r2 == 2u) ? -a : a;
-    const double yi = (r2 >= 2u) ? -b : b;
-    const bool odd = K & 1u;
-    acc[0] += odd ? 0.0 : yr;
-    acc[1] += odd ? 0.0 : yi;
-    acc[2] += odd ? yr : 0.0;
-    acc[3] += odd ? yi : 0.0;
+    // signs of i^r2 (r2 = 1, 2 negate the real part, r2 = 2, 3 the imaginary)
+    const double yr = flip_sign(a, ((r2 + 1u) & 2u) << 30);
+    const double yi = flip_sign(b, (r2 & 2u) << 30);
+    // even / odd by FMAs with 1 and 0 (exact: x * 0 + acc = acc)
+    const double e = (K & 1u) ? 0.0 : 1.0, o = (K & 1u) ? 1.0 : 0.0;
+    acc[0] = fma(e, yr, acc[0]);
+    acc[1] = fma(e, yi, acc[1]);
+    acc[2] = fma(o, yr, acc[2]);
+    acc[3] = fma(o, yi, acc[3]);
 }
 
 // Lane state over the outer coordinates: integer phase K at c_in = 0, the
@@ -721,36 +733,82 @@ __device__ __forceinline__ double lane_factor(const double* s_lt, uint32_t nbyte
 // gin[pio ^ pin(c_in)] from LDS (the address follows the same Gray recurrence
 // as the sign) and flips that factor's sign word: per branch a sign XOR, an
 // address XOR, an LDS read and one FP64 FMA.
+// Signs of the three low inner slots applied to the 8 accumulators of one
+// outer point: (tr, ti) of its 256-branch sum.
+template <bool IMAG>
+__device__ __forceinline__ void combine_low(uint32_t hs, const double (&a)[8], double& tr, double& ti) {
+    const double s0 = (hs & 1u) ? -1.0 : 1.0, s1 = (hs & 2u) ? -1.0 : 1.0, s2 = (hs & 4u) ? -1.0 : 1.0;
+    const double ev = fma(s2, fma(s1, a[6], a[4]), fma(s1, a[2], a[0]));          // l = 0, 2, 4, 6
+    const double od = fma(s2, fma(s1, a[7], a[5]), fma(s1, a[3], a[1]));          // l = 1, 3, 5, 7
+    if constexpr (IMAG) { tr = ev; ti = s0 * od; }
+    else { tr = fma(s0, od, ev); ti = 0.0; }
+}
+
+// The 256-entry inner table held in VGPRs: lane r of each 16-lane row keeps
+// entries r, 16 + r, ..., 240 + r (16 register pairs), so entry e is entry
+// e & 15 of row-local register e >> 4 and reaches every lane of the row
+// through a DPP row broadcast (row_newbcast) folded into the FMA itself:
+// v_fmac_f64_dpp issues at the full FP64 rate, and the walk then reads no
+// memory at all.  Every lane of the wave must be active while the table is
+// broadcast (a broadcast from a disabled lane does not write).
+struct InnerRegs { double w[16]; };
+__device__ __forceinline__ void load_inner_regs(const Z8Dev& E, InnerRegs& T) {
+    const uint32_t r = threadIdx.x & 15u;
+#pragma unroll
+    for (int k = 0; k < 16; ++k) T.w[k] = E.inner_acc[16 * k + r];
+    // a DPP read of a VGPR needs two wait states after a VALU write of it
+    asm volatile("s_nop 1");
+}
+// acc += table entry (row lane SEL of w) * m
+template <int SEL>
+__device__ __forceinline__ void fmac_bcast(double& acc, double w, double m) {
+    asm("v_fmac_f64_dpp %0, %1, %2 row_newbcast:%3 row_mask:0xf bank_mask:0xf" : "+v"(acc) : "v"(w), "v"(m), "i"(SEL));
+}
+// table entry (row lane SEL of w)
+template <int SEL>
+__device__ __forceinline__ double bcast(double w) {
+    double r;
+    asm("v_mov_b64_dpp %0, %1 row_newbcast:%2 row_mask:0xf bank_mask:0xf" : "=v"(r) : "v"(w), "i"(SEL));
+    return r;
+}
+// Step Q of the inner Gray walk: flip the multiplier's sign for high slot
+// ctz(Q), then entries 8Q .. 8Q + 7 into the 8 low-slot accumulators.  Step 0
+// (multiplier +1) sets the accumulators to its entries.
+template <int Q>
+__device__ __forceinline__ void inner_steps(double (&a)[8], const InnerRegs& T, double& m, const uint32_t (&sm)[5]) {
+    if constexpr (Q < 32) {
+        constexpr int k = (8 * Q) >> 4, s = (8 * Q) & 15;
+        const double w = T.w[k];
+        if constexpr (Q == 0) {
+            a[0] = bcast<0>(w); a[1] = bcast<1>(w); a[2] = bcast<2>(w); a[3] = bcast<3>(w);
+            a[4] = bcast<4>(w); a[5] = bcast<5>(w); a[6] = bcast<6>(w); a[7] = bcast<7>(w);
+        } else {
+            m = flip_sign(m, sm[__builtin_ctz(Q)]);
+            asm volatile("" : "+v"(m));     // one running value: the flip is an XOR in place
+            fmac_bcast<s + 0>(a[0], w, m); fmac_bcast<s + 1>(a[1], w, m);
+            fmac_bcast<s + 2>(a[2], w, m); fmac_bcast<s + 3>(a[3], w, m);
+            fmac_bcast<s + 4>(a[4], w, m); fmac_bcast<s + 5>(a[5], w, m);
+            fmac_bcast<s + 6>(a[6], w, m); fmac_bcast<s + 7>(a[7], w, m);
+        }
+        inner_steps<Q + 1>(a, T, m, sm);
+    }
+}
+
 template <bool HASG, bool IMAG>
-__device__ __forceinline__ void inner_walk(const Z8Dev& E, const uint32_t (&ps)[8], const double* s_gin, const LaneState& S, double F,
-                                           double (&acc)[4]) {
+__device__ __forceinline__ void inner_walk(const Z8Dev& E, const uint32_t (&ps)[8], const double* s_gin, const InnerRegs& T,
+                                           const LaneState& S, double F, double (&acc)[4]) {
     double tr = 0.0, ti = 0.0;
     if constexpr (!HASG) {
-        const PQ_CONST double* Wv = cview(E.inner_acc);
         uint32_t sm[5];
 #pragma unroll
         for (int j = 0; j < 5; ++j) sm[j] = (S.hs << (28 - j)) & 0x80000000u;      // bit 3 + j of hs
+        // The multiplier is +-1.0: an empty asm hides its constant low word,
+        // so a sign flip is one XOR on the high word in place.
         double m = 1.0;
-        double a[8] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
-#pragma unroll
-        for (int q = 0; q < 32; ++q) {
-            if (q) m = __hiloint2double(__double2hiint(m) ^ (int)sm[__builtin_ctz(q)], __double2loint(m));
-            // An empty asm on the block pointer keeps each step's scalar load
-            // next to its use (the table is in the constant address space,
-            // so otherwise all 256 entries are hoisted and the SGPRs spill).
-            const PQ_CONST double* bw = Wv + 8 * q;
-            asm volatile("" : "+s"(bw) : : "memory");
-            double w[8];
-#pragma unroll
-            for (int l = 0; l < 8; ++l) w[l] = bw[l];
-#pragma unroll
-            for (int l = 0; l < 8; ++l) a[l] = fma(m, w[l], a[l]);
-        }
-        const double s0 = (S.hs & 1u) ? -1.0 : 1.0, s1 = (S.hs & 2u) ? -1.0 : 1.0, s2 = (S.hs & 4u) ? -1.0 : 1.0;
-        const double ev = fma(s2, fma(s1, a[6], a[4]), fma(s1, a[2], a[0]));          // l = 0, 2, 4, 6
-        const double od = fma(s2, fma(s1, a[7], a[5]), fma(s1, a[3], a[1]));          // l = 1, 3, 5, 7
-        if constexpr (IMAG) { tr = ev; ti = s0 * od; }
-        else tr = fma(s0, od, ev);
+        asm volatile("" : "+v"(m));
+        double a[8];
+        inner_steps<0>(a, T, m, sm);
+        combine_low<IMAG>(S.hs, a, tr, ti);
     } else {
         const PQ_CONST double* Wv = cview(E.inner);
         uint32_t sm[8];
@@ -791,11 +849,12 @@ __global__ __launch_bounds__(256) void k_z8_walk(const Z8Dev* __restrict__ engs,
                                                  double* __restrict__ partial) {
     __shared__ double s_lt[8 * 256];
     __shared__ double s_gin[HASG ? 4096 : 1];
+    __shared__ double s_fm[HASG ? 1 : 256];
     __shared__ unsigned long long s_item;
-    __shared__ int s_eng;
+    __shared__ int s_key, s_fq;
     __shared__ double s_red[4][4];
     const uint32_t tid = threadIdx.x;
-    if (tid == 0) s_eng = -1;
+    if (tid == 0) { s_key = -1; s_fq = -1; }
     for (;;) {
         __syncthreads();
         if (tid == 0) s_item = item_begin + atomicAdd(counter, 1ull);
@@ -806,27 +865,81 @@ __global__ __launch_bounds__(256) void k_z8_walk(const Z8Dev* __restrict__ engs,
         while (hi - lo > 1) { const uint32_t mid = (lo + hi) >> 1; if (cview(qs)[mid].item0 <= item) lo = mid; else hi = mid; }
         const Z8Q Q = cload(qs + lo);
         const Z8Dev E = cload(engs + Q.eng);
-        if ((int)Q.eng != s_eng) {
-            for (uint32_t k = tid; k < E.nbytes * 256u; k += 256) s_lt[k] = E.lt[k];
+        // Mid slots that are all pivot-free (one free term each, touched by no
+        // other slot): their magnitudes are the same for every lane of the
+        // query, one table entry per mid point, and a lane's factor is
+        // computed once instead of at every mid point.
+        const bool fast = !HASG && Q.M <= E.nx;
+        const int key = 2 * (int)Q.eng + (fast ? 1 : 0);
+        if (key != s_key) {
+            const double* lt = fast ? E.lt_rest : E.lt;
+            for (uint32_t k = tid; k < E.nbytes * 256u; k += 256) s_lt[k] = lt[k];
             if constexpr (HASG) for (uint32_t k = tid; k < (1u << E.npin); k += 256) s_gin[k] = E.gin[k];
             __syncthreads();
-            if (tid == 0) s_eng = (int)Q.eng;
+            if (tid == 0) s_key = key;
+        }
+        if constexpr (!HASG) {
+            if (fast && (int)lo != s_fq) {
+                for (uint32_t o = tid; o < (1u << Q.M); o += 256) {
+                    const uint32_t g = o ^ (o >> 1);
+                    double f = 1.0;
+                    for (uint32_t j = 0; j < Q.M; ++j) f *= E.mid_r[2 * j + (((uint32_t)(Q.s0 >> E.mid_term[j]) ^ (g >> j)) & 1u)];
+                    s_fm[o] = f;
+                }
+                __syncthreads();
+                if (tid == 0) s_fq = (int)lo;
+            }
         }
         double acc[4] = {0.0, 0.0, 0.0, 0.0};
         const uint64_t lane = (item - Q.item0) * 256 + tid;
-        if ((lane >> Q.H) == 0) {
+        const bool valid = (lane >> Q.H) == 0;
+        InnerRegs T;
+        if constexpr (!HASG) load_inner_regs(E, T);
+        // without pivots every lane walks (the table broadcast needs the whole
+        // wave); lanes past the query's 2^H repeat a valid lane's coset and
+        // are zeroed below
+        if (!HASG || valid) {
             LaneState S;
             seed_lane(E, Q, lane, S);
             uint32_t ps[8];
 #pragma unroll
             for (int j = 0; j < 8; ++j) ps[j] = HASG ? (cview(E.slot_pin)[j] << 3) : 0u;
             const uint32_t nout = 1u << Q.M;
-            for (uint32_t o = 0; o < nout; ++o) {
-                if (o) flip_mid(E, Q, 8u + (uint32_t)__builtin_ctz(o), S);
-                const double F = lane_factor(s_lt, E.nbytes, S.sout);
-                inner_walk<HASG, IMAG>(E, ps, s_gin, S, F, acc);
+            if (fast) {
+                // the lane's factor without the mid slots' terms, times the
+                // magnitudes of the pivot-free slots among its hi slots
+                double Fl = lane_factor(s_lt, E.nbytes, S.sout);
+                for (uint32_t j = Q.M; j < E.nx; ++j) {
+                    const double r0 = cview(E.mid_r)[2 * j], r1 = cview(E.mid_r)[2 * j + 1];
+                    Fl *= ((S.sout >> cview(E.mid_term)[j]) & 1) ? r1 : r0;
+                }
+                // each mid slot's cross term with the lane's hi slots
+                uint32_t P = 0;
+                for (uint32_t j = 0; j < Q.M; ++j) P |= (uint32_t)(__popcll(cview(E.slot_q)[8 + j] & S.cout) & 1) << j;
+                // along the mid walk: the mid-only part of each step is the same
+                // for every lane; the lane adds its cross term
+                uint32_t cmid = 0;
+                for (uint32_t o = 0; o < nout; ++o) {
+                    if (o) {
+                        const uint32_t j = (uint32_t)__builtin_ctz(o), p = 8u + j;
+                        const uint64_t qp = cview(E.slot_q)[p];
+                        const uint32_t ell = cview(E.slot_lam)[p] + (((uint32_t)(Q.delta >> p) & 1u) << 2);
+                        const uint32_t du = (((cmid >> j) & 1u) ? 8u - ell : ell) + ((uint32_t)(__popcll(qp & ((uint64_t)cmid << 8)) & 1) << 2);
+                        S.K += du + (((P >> j) & 1u) << 2);
+                        S.hs ^= (uint32_t)qp & 0xFFu;
+                        cmid ^= 1u << j;
+                    }
+                    inner_walk<HASG, IMAG>(E, ps, s_gin, T, S, Fl * s_fm[o], acc);
+                }
+            } else {
+                for (uint32_t o = 0; o < nout; ++o) {
+                    if (o) flip_mid(E, Q, 8u + (uint32_t)__builtin_ctz(o), S);
+                    const double F = lane_factor(s_lt, E.nbytes, S.sout);
+                    inner_walk<HASG, IMAG>(E, ps, s_gin, T, S, F, acc);
+                }
             }
         }
+        if (!valid) acc[0] = acc[1] = acc[2] = acc[3] = 0.0;
         // workgroup reduction in FP64, in a fixed order
 #pragma unroll
         for (int off = 32; off > 0; off >>= 1) {
@@ -1424,7 +1537,7 @@ __global__ __launch_bounds__(256) void k_z8_reduce(const uint64_t* __restrict__ 
 
 // Segment split: a lane walks 2^(8+M) branches and 2^H lanes cover a query
 // (H = d-8-M).  M depends on d alone (M = clamp(d - offset, 0, max)), so a
-// query is decomposed the same way in every batch and its FP32 partial sums,
+// query is decomposed the same way in every batch and its partial sums,
 // hence its result, are bitwise the same alone and in any mix.  The offset
 // and cap are set by the autotuner (polyq_amd.autotune); the default keeps
 // H = 19 (2048 items of 256 lanes per query) until M reaches 6.

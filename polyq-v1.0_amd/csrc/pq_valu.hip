@@ -1,6 +1,8 @@
 // VALU issue-rate probe for gfx950: long independent chains of one vector
 // instruction per kernel (int32 xor / and / add / popcount, FP32 FMA, FP64 FMA,
-// and the branch walk's own mix of one int32 xor per FP64 FMA).  Each lane runs
+// the branch walk's own mix of one int32 xor per FP64 FMA, and the FP64 FMA
+// with one operand from an SGPR pair or broadcast across each 16-lane row by
+// DPP, the two ways a uniform table entry can reach it).  Each lane runs
 // 8 independent chains, so the rate measured is issue throughput, not latency.
 // The kernel's clock is read in-kernel (s_memtime against the constant-rate
 // s_memrealtime) so that rates can be quoted in lanes per clock per CU.
@@ -18,12 +20,12 @@ namespace pq {
 extern thread_local std::string g_err;     // pq_last_error() (pq_api.cpp)
 
 namespace {
-enum ProbeOp { P_XOR = 0, P_AND, P_ADD, P_BCNT, P_FMA32, P_FMA64, P_XOR_FMA64, P_COUNT };
+enum ProbeOp { P_XOR = 0, P_AND, P_ADD, P_BCNT, P_FMA32, P_FMA64, P_XOR_FMA64, P_FMAC64_S, P_FMAC64_DPP, P_COUNT };
 const char* const kProbeNames[P_COUNT] = {"v_xor_b32", "v_and_b32", "v_add_u32", "v_bcnt_u32_b32", "v_fma_f32", "v_fma_f64",
-                                          "v_xor_b32+v_fma_f64"};
+                                          "v_xor_b32+v_fma_f64", "v_fmac_f64 (SGPR operand)", "v_fmac_f64_dpp row_newbcast"};
 
 template <int OP>
-__device__ __forceinline__ void probe_step(uint32_t (&a)[8], double (&d)[8], uint32_t b, float fb, double db) {
+__device__ __forceinline__ void probe_step(uint32_t (&a)[8], double (&d)[8], uint32_t b, float fb, double db, double sdb) {
 #pragma unroll
     for (int k = 0; k < 8; ++k) {
         if constexpr (OP == P_XOR) asm volatile("v_xor_b32 %0, %0, %1" : "+v"(a[k]) : "v"(b));
@@ -35,6 +37,9 @@ __device__ __forceinline__ void probe_step(uint32_t (&a)[8], double (&d)[8], uin
             asm volatile("v_fma_f32 %0, %0, %1, %2" : "+v"(f) : "v"(fb), "v"(0.0f));
             a[k] = __float_as_uint(f);
         } else if constexpr (OP == P_FMA64) asm volatile("v_fma_f64 %0, %0, %1, %2" : "+v"(d[k]) : "v"(db), "v"(0.0));
+        else if constexpr (OP == P_FMAC64_S) asm volatile("v_fmac_f64 %0, %1, %2" : "+v"(d[k]) : "s"(sdb), "v"(db));
+        else if constexpr (OP == P_FMAC64_DPP)
+            asm volatile("v_fmac_f64_dpp %0, %1, %2 row_newbcast:5 row_mask:0xf bank_mask:0xf" : "+v"(d[k]) : "v"(db), "v"(db));
         else {
             if (k & 1) asm volatile("v_fma_f64 %0, %0, %1, %2" : "+v"(d[k]) : "v"(db), "v"(0.0));
             else asm volatile("v_xor_b32 %0, %0, %1" : "+v"(a[k]) : "v"(b));
@@ -57,11 +62,12 @@ __global__ __launch_bounds__(256) void k_valu(uint32_t iters, uint32_t seed, uin
     const uint32_t b = seed * 3u + 1u;
     const float fb = 1.0f;
     const double db = 1.0;
+    const double sdb = __builtin_amdgcn_readfirstlane(seed) == 7u ? 1.0 : 0.5;   // uniform: an SGPR pair
     uint64_t c0 = 0, w0 = 0;
     if (blockIdx.x == 0 && threadIdx.x == 0) { c0 = clock64(); w0 = wall_clock64(); }
     for (uint32_t it = 0; it < iters; ++it) {
 #pragma unroll
-        for (int r = 0; r < 8; ++r) probe_step<OP>(a, d, b, fb, db);
+        for (int r = 0; r < 8; ++r) probe_step<OP>(a, d, b, fb, db, sdb);
     }
     if (blockIdx.x == 0 && threadIdx.x == 0) {
         clk[0] = clock64() - c0;
@@ -130,6 +136,8 @@ int valu_probe(double* out, size_t cap, uint32_t blocks_per_cu, uint32_t iters) 
             case P_BCNT: run_probe<P_BCNT>(grid, iters, sink, clk, e0, e1, st, ms, hc); break;
             case P_FMA32: run_probe<P_FMA32>(grid, iters, sink, clk, e0, e1, st, ms, hc); break;
             case P_FMA64: run_probe<P_FMA64>(grid, iters, sink, clk, e0, e1, st, ms, hc); break;
+            case P_FMAC64_S: run_probe<P_FMAC64_S>(grid, iters, sink, clk, e0, e1, st, ms, hc); break;
+            case P_FMAC64_DPP: run_probe<P_FMAC64_DPP>(grid, iters, sink, clk, e0, e1, st, ms, hc); break;
             default: run_probe<P_XOR_FMA64>(grid, iters, sink, clk, e0, e1, st, ms, hc); break;
         }
         const double lane_ops = (double)grid * 256.0 * (double)iters * 64.0;

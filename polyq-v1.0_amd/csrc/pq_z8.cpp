@@ -66,7 +66,13 @@ Z8Image build_z8_image(const HostEngine& E) {
         Z.imag0 = true;
     }
     while (N.size() < kInner) { N.push_back(0); U.emplace_back(W, 0ull); }
-    for (uint32_t j = 0; j < d; ++j) if (!used[j]) { N.push_back(E.kernel[j]); U.emplace_back((E.kernel_u.data() + (size_t)j * W), (E.kernel_u.data() + (size_t)j * W) + W); }
+    // outer slots: pivot-free kernel vectors first (each is one free term)
+    for (int pass = 0; pass < 2; ++pass)
+        for (uint32_t j = 0; j < d; ++j) {
+            if (used[j] || (pivots_of(E.kernel[j]) == 0) != (pass == 0)) continue;
+            N.push_back(E.kernel[j]); U.emplace_back((E.kernel_u.data() + (size_t)j * W), (E.kernel_u.data() + (size_t)j * W) + W);
+            if (pass == 0 && Z.nx < kMidFree) { Z.mid_term.push_back(E.free_col[j]); ++Z.nx; }
+        }
     Z.nslots = (uint32_t)N.size();
     if (Z.nslots > 64) throw std::invalid_argument("too many slots");
     const uint32_t S = Z.nslots;
@@ -138,18 +144,31 @@ Z8Image build_z8_image(const HostEngine& E) {
             i ^= i >> 1; i ^= i >> 2; i ^= i >> 4;
             Z.inner_acc[8 * q + l] = Z.inner[i];
         }
-    // Lane-factor byte tables
+    // Lane-factor byte tables, with every lane term and without the free
+    // terms of the pivot-free outer slots kInner .. kInner + nx - 1
+    uint64_t mid_terms = 0;
+    for (uint32_t j = 0; j < Z.nx; ++j) {
+        const uint32_t k = Z.mid_term[j];
+        if (Z.slot_n[kInner + j] != 1ull << k) throw std::logic_error("pivot-free outer slot is not its free term alone");
+        mid_terms |= 1ull << k;
+        Z.mid_r.push_back(E.r0[k]);
+        Z.mid_r.push_back(E.r1[k]);
+    }
     Z.nbytes = (t + 7) / 8;
     Z.lt.assign((size_t)Z.nbytes * 256, 1.0);
+    Z.lt_rest.assign((size_t)Z.nbytes * 256, 1.0);
     for (uint32_t b = 0; b < Z.nbytes; ++b)
         for (uint32_t v = 0; v < 256; ++v) {
-            double f = 1.0;
+            double f = 1.0, fr = 1.0;
             for (uint32_t q = 0; q < 8; ++q) {
                 const uint32_t k = 8 * b + q;
                 if (k >= t || !((Z.lane_terms >> k) & 1)) continue;
-                f *= ((v >> q) & 1) ? E.r1[k] : E.r0[k];
+                const double r = ((v >> q) & 1) ? E.r1[k] : E.r0[k];
+                f *= r;
+                if (!((mid_terms >> k) & 1)) fr *= r;
             }
             Z.lt[(size_t)b * 256 + v] = f;
+            Z.lt_rest[(size_t)b * 256 + v] = fr;
         }
     // inner pivot table
     Z.gin.assign((size_t)1 << Z.npin, 1.0);

@@ -759,7 +759,7 @@ def valu_probe(blocks_per_cu=8, iters=16384):
     """VALU issue rates of the device (pq_valu_probe): per instruction,
     lane-ops/s, lanes per clock per CU at the in-kernel clock, that clock (GHz)
     and the kernel time."""
-    n = 3 + 4 * 7
+    n = 3 + 4 * 16
     o = np.zeros(n, dtype=np.float64)
     k = _check(lib.pq_valu_probe(_ptr(o, _f64p), n, blocks_per_cu, iters))
     ops = {}

@@ -168,6 +168,49 @@ def test_config2_small_seed_against_oracle():
             assert rel.max() < 1e-9
 
 
+def test_branch_walk_point_pairs_and_single_points_agree():
+    # the per-branch walk takes its outer points in pairs when a lane walks
+    # mid slots (M >= 1) and one at a time otherwise; force each in turn on
+    # circuits of branch dimension 7..14 and on the config-2 generator at seed 1
+    # (d = 19), and check both against the FP64 form and the oracle
+    rng = W.Rng(17)
+    saved = pq.get_walk_tuning("branch_walk")
+    try:
+        cases = 0
+        for trial in range(12):
+            c = W.sandwich_layer(14 + rng.below(10), 14 + rng.below(20), 60, 20, rng)
+            sims = {f: pq.Simulator(c, strategy="cached", form=f) for f in ("branch_walk", "plain_fp64")}
+            d = sims["branch_walk"].stats()[5]
+            xs = [rng.below(1 << 10) for _ in range(6)]
+            ys = [rng.below(1 << 10) for _ in range(6)]
+            want = sims["plain_fp64"].amplitudes(xs, ys)
+            res = {}
+            for tuning in ((8, 6), (8, 1), (64, 0)):      # pairs over up to 2^6 and 2^1 points, single points
+                pq.set_walk_tuning(*tuning, form="branch_walk")
+                res[tuning] = sims["branch_walk"].amplitudes(xs, ys)
+            scale = max(np.max(np.abs(want)), 1e-300)
+            for tuning, got in res.items():
+                err = np.max(np.abs(got - want)) / scale
+                assert err < BW_REL, (trial, tuning, err)
+            if trial < 3:
+                o = O.OSim(O.OCircuit.from_circuit(c), "cached")
+                ref = np.array([o.amplitude(x, y) for x, y in zip(xs, ys)])
+                assert np.max(np.abs(res[(8, 6)] - ref)) <= BW_REL * max(np.max(np.abs(ref)), 1e-300)
+            cases += d >= 9
+        assert cases >= 6
+        c, xs, ys = W.config2(1)
+        want = pq.Simulator(c, strategy="cached", form="plain_fp64").amplitudes(xs, ys)
+        s = pq.Simulator(c, strategy="cached", form="branch_walk")
+        for tuning in ((8, 6), (12, 3), (64, 0)):
+            pq.set_walk_tuning(*tuning, form="branch_walk")
+            got = s.amplitudes(xs, ys)
+            err = np.max(np.abs(got - want)) / np.max(np.abs(want))
+            print("config2 seed 1, tuning", tuning, "max err / max|a| =", err)
+            assert err < BW_REL
+    finally:
+        pq.set_walk_tuning(saved[0], saved[1], form="branch_walk")
+
+
 def test_normalisation_sum_of_probabilities():
     # size-independent property: sum_y |<y|C|0>|^2 = 1
     rng = W.Rng(9)
