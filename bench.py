@@ -108,11 +108,11 @@ SEG_OPS_PER_POINT = 30.0
 # profiles/k_seg_walk_config2.md): the table once per XCD L2 and the item
 # partials.
 SEG_TRAFFIC = {("headline", 16, 2): 1.84e6, ("saturated", 16, 6): 5.37e6}
-# HBM bytes per branch-walk launch on the headline step, keyed by the lane
-# walk tuning (FETCH_SIZE x 2 + WRITE_SIZE, copy-calibrated;
+# HBM bytes per branch-walk launch on the headline step, keyed by the mid
+# coordinates M a lane walks (FETCH_SIZE x 2 + WRITE_SIZE, copy-calibrated;
 # profiles/k_z8_walk_config2.md): the item partials (4 doubles per 256-lane
-# item) and the tables once per XCD.
-BRANCH_TRAFFIC = {(26, 4): 1.31e6}
+# item, 2^(22 - M) items per amplitude) and the tables once per XCD.
+BRANCH_TRAFFIC = {6: 0.533e6, 4: 1.30e6}
 PEAK_CLOCK_HZ = 2.4e9       # MI355X peak engine clock
 
 
@@ -203,10 +203,15 @@ def headline(args, pq, W, np, probe):
     timing = pq.last_timing()
     assert np.array_equal(again, got)
     ops = info["ops_per_branch"]
-    # without inner pivots the inner step is one FP64 FMA per branch (the sign
-    # XOR comes once per 8 branches): priced at the measured v_fma_f64 rate;
+    mid_bits = pq.branch_walk_mid_bits(d, tuned[0], tuned[1])
+    # without inner pivots the inner step is one FP64 FMA per branch, with the
+    # table entry broadcast by DPP (the sign XOR comes once per 8 branches):
+    # priced at the faster of the measured v_fma_f64 and v_fmac_f64_dpp rates;
     # with pivots at the xor/FMA mix
-    peak, lanes, peak_op = _peak(probe, "v_fma_f64" if ops == 1 else "v_xor_b32+v_fma_f64", 64)
+    if ops == 1:
+        peak, lanes, peak_op = max(_peak(probe, "v_fma_f64", 64), _peak(probe, "v_fmac_f64_dpp row_newbcast", 64))
+    else:
+        peak, lanes, peak_op = _peak(probe, "v_xor_b32+v_fma_f64", 64)
     achieved = branches_per_step * ops / (kernel_avg_ms * 1e-3)
     s8d = _s8d_ops(st)
     kern = "k_z8_walk<%s, %s>" % ("true" if info["inner_pivots"] else "false", "true" if info["imag_slot0"] else "false")
@@ -215,7 +220,7 @@ def headline(args, pq, W, np, probe):
         "branches_per_step": branches_per_step, "elapsed": elapsed, "kernel_avg_ms": kernel_avg_ms,
         "timing": timing, "dev": dev, "tuning": list(tuned), "build_s": build_s, "upload_s": upload_s,
         "roofline": {"bound": "valu", "achieved": achieved / 1e12, "peak": peak / 1e12, "unit": "Tlane-op/s",
-                     "frac": achieved / peak, "traffic": BRANCH_TRAFFIC.get(tuple(tuned)), "kernel": kern,
+                     "frac": achieved / peak, "traffic": BRANCH_TRAFFIC.get(mid_bits), "kernel": kern, "mid_bits": mid_bits,
                      "kernel_avg_ms": kernel_avg_ms, "ops_per_branch": ops, "branches_per_launch": branches_per_step,
                      "peak_mix": peak_op, "peak_lanes_per_clk_per_cu": lanes,
                      "kernel_branches_per_s": branches_per_step / (kernel_avg_ms * 1e-3),

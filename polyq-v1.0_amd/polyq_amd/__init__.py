@@ -630,6 +630,15 @@ _TUNE_GRID = {"z8": ((8, 10, 11, 12, 13, 14, 16), (0, 2, 4, 6, 8)),
               "branch_walk": ((23, 24, 25, 26, 27, 28, 29), (4, 6, 8))}
 
 
+def branch_walk_mid_bits(d, offset, max_mid):
+    """Mid coordinates a lane walks in the per-branch walk for an engine of
+    branch dimension d at lane walk tuning (offset, max_mid): 2^(8 + M)
+    branches per lane (the library's own rule, z8_auto_mid_bits)."""
+    if d <= offset:
+        return 0
+    return min(d - offset, max_mid, d - 8 if d > 8 else 0)
+
+
 def autotune(sims, inputs, outputs, form="z8", offsets=None, max_mids=None, repeats=3, cache=True, objective="wall", rounds=3):
     """Pick the lane walk split (offset, max_mid) of a batched form that
     minimises the time of this batch: its wall time per call (objective
@@ -653,7 +662,12 @@ def autotune(sims, inputs, outputs, form="z8", offsets=None, max_mids=None, repe
     for off in offsets:
         for mx in max_mids:
             set_walk_tuning(off, mx, form)
-            key = tuple(s.seg_info()["mid_bits"] for s in distinct) if form == "z8" else (off, mx)
+            if form == "z8":
+                key = tuple(s.seg_info()["mid_bits"] for s in distinct)
+            elif all(s.stats()[6] == 0 for s in distinct):      # one branch engine per simulator
+                key = tuple(branch_walk_mid_bits(s.z8_info()["d"], off, mx) for s in distinct)
+            else:
+                key = (off, mx)
             configs.setdefault(key, []).append((off, mx))
     samples = {k: [] for k in configs}
     prev = get_timing()
