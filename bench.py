@@ -112,7 +112,7 @@ SEG_TRAFFIC = {("headline", 16, 2): 1.84e6, ("saturated", 16, 6): 5.37e6}
 # coordinates M a lane walks (FETCH_SIZE x 2 + WRITE_SIZE, copy-calibrated;
 # profiles/k_z8_walk_config2.md): the item partials (4 doubles per 256-lane
 # item, 2^(22 - M) items per amplitude) and the tables once per XCD.
-BRANCH_TRAFFIC = {6: 0.533e6, 4: 1.30e6}
+BRANCH_TRAFFIC = {1: 8.66e6, 2: 4.47e6, 3: 2.37e6, 4: 1.32e6, 5: 0.80e6, 6: 0.54e6, 7: 0.41e6}
 PEAK_CLOCK_HZ = 2.4e9       # MI355X peak engine clock
 
 

@@ -152,7 +152,9 @@ typedef struct pq_engine pq_engine;
  * engine built from its fields; the descriptor is checked for consistency
  * (RREF shape, kernel inside ker K, kernel_u = xor of mcols, Diag lines,
  * symmetric b_fixed, zero tails).  The device images are uploaded on first
- * evaluation and stay until pq_engine_destroy.  At most 64 terms. */
+ * evaluation and stay until pq_engine_destroy.  Any number of terms, of which
+ * at most 64 may vary over the branch subspace (the others are constant and
+ * folded into the prologue). */
 int pq_engine_create(const pq_engine_desc* desc, pq_engine** out);
 void pq_engine_destroy(pq_engine* engine);
 /* Engine::stats (src/engine.rs:415-425): free_vars, planes, lines, radical,
