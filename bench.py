@@ -261,11 +261,19 @@ def headline(args, pq, W, np, probe):
         tp0 = time.perf_counter()
         pa = psim.amplitudes(xs, ys)
         tp1 = time.perf_counter()
+        p_int, l_int, op_int = _peak(probe, "v_xor_b32", 64)
+        s8d = _s8d_ops(st)
         res["plain"] = {
             "amplitudes_per_s": nq / (tp1 - tp0),
             "branches_per_s": branches_per_step / (tp1 - tp0),
             "branch_walk_vs_fp64": _err_summary(got, pa),
             "z8_vs_fp64": _err_summary(zgot, pa),
+            # k_plain priced at the reference algorithm's per-branch integer
+            # work (popcount phase8, term update, combine) against the int32
+            # VALU peak; its FP64 complex accumulation per branch comes on top
+            "roofline_ref_ops": {"kernel": "k_plain", "ops_per_branch": s8d, "peak": p_int / 1e12, "unit": "Tlane-op/s",
+                                 "peak_op": op_int, "achieved": branches_per_step * s8d / (tp1 - tp0) / 1e12,
+                                 "frac": branches_per_step * s8d / (tp1 - tp0) / p_int},
         }
 
     if not args.no_cpu:

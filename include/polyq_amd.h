@@ -305,8 +305,10 @@ int pq_sim_seg_lane_ref(pq_sim* sim, uint64_t x, uint64_t y, uint32_t mid_bits, 
 int pq_sim_seg_selfcheck(pq_sim* sim, uint64_t x, uint64_t y, uint32_t trials, uint64_t seed, double* max_err);
 /* Branch-walk image summary: d, inner directions, slots, compact inner pivots,
  * rank K, whether inner entries alternate real/imaginary, VALU lane-ops per
- * branch of the inner step. */
-int pq_sim_z8_info(pq_sim* sim, uint32_t out7[7]);
+ * branch of the inner step, and the pivot-free outer slots that can serve as
+ * mid slots with per-query magnitudes (a lane walk of M <= that many mid
+ * coordinates takes the kernel's fast path). */
+int pq_sim_z8_info(pq_sim* sim, uint32_t out8[8]);
 
 /* VALU issue-rate probe (no counterpart in the reference): one kernel per
  * instruction (int32 xor, and, add, popcount; FP32 FMA; FP64 FMA; one int32

@@ -175,12 +175,18 @@ def test_branch_walk_point_pairs_and_single_points_agree():
     # (d = 19), and check both against the FP64 form and the oracle
     rng = W.Rng(17)
     saved = pq.get_walk_tuning("branch_walk")
+    paths = {"fast": 0, "generic": 0}
     try:
         cases = 0
         for trial in range(12):
             c = W.sandwich_layer(14 + rng.below(10), 14 + rng.below(20), 60, 20, rng)
             sims = {f: pq.Simulator(c, strategy="cached", form=f) for f in ("branch_walk", "plain_fp64")}
             d = sims["branch_walk"].stats()[5]
+            zi = sims["branch_walk"].z8_info()
+            for off, mx in ((8, 6), (8, 1)):
+                m = pq.branch_walk_mid_bits(d, off, mx)
+                if m:     # pivot-free mid slots take the kernel's fast path
+                    paths["fast" if zi["inner_pivots"] == 0 and m <= zi["pivot_free_mid"] else "generic"] += 1
             xs = [rng.below(1 << 10) for _ in range(6)]
             ys = [rng.below(1 << 10) for _ in range(6)]
             want = sims["plain_fp64"].amplitudes(xs, ys)
@@ -198,6 +204,8 @@ def test_branch_walk_point_pairs_and_single_points_agree():
                 assert np.max(np.abs(res[(8, 6)] - ref)) <= BW_REL * max(np.max(np.abs(ref)), 1e-300)
             cases += d >= 9
         assert cases >= 6
+        print("mid walks by kernel path:", paths)
+        assert paths["fast"] >= 2 and paths["generic"] >= 2
         c, xs, ys = W.config2(1)
         want = pq.Simulator(c, strategy="cached", form="plain_fp64").amplitudes(xs, ys)
         s = pq.Simulator(c, strategy="cached", form="branch_walk")
