@@ -274,7 +274,7 @@ struct Z8Image {
     std::vector<uint8_t> slot_real;    // slot holds a direction (not a phantom)
     std::vector<double> inner;         // 256 signed FP64 values W[i] (entry i covers c_in = gray(i))
     // The same 256 values in the order of the accumulator walk (no inner
-    // pivots): entry 8 q + l covers c_in = (gray5(q) << 3) | l.
+    // pivots): entry 16 q + l covers c_in = (gray4(q) << 4) | l.
     std::vector<double> inner_acc;
     std::vector<uint32_t> inner_aux;   // 256 x pin*8 (LDS byte offset of the inner pivots), then 256 x kin
     std::vector<double> lt;            // nbytes x 256 lane-factor byte tables (FP64)

@@ -772,26 +772,46 @@ __device__ __forceinline__ double bcast(double w) {
     return r;
 }
 // Step Q of the inner Gray walk: flip the multiplier's sign for high slot
-// ctz(Q), then entries 8Q .. 8Q + 7 into the 8 low-slot accumulators.  Step 0
-// (multiplier +1) sets the accumulators to its entries.
+// ctz(Q), then entries 16Q .. 16Q + 15 (register pair Q, all 16 row lanes)
+// into the 16 low-slot accumulators.  Step 0 (multiplier +1) sets the
+// accumulators to its entries.
+template <int L>
+__device__ __forceinline__ void bcast_all(double (&a)[16], double w) {
+    if constexpr (L < 16) { a[L] = bcast<L>(w); bcast_all<L + 1>(a, w); }
+}
+template <int L>
+__device__ __forceinline__ void fmac_all(double (&a)[16], double w, double m) {
+    if constexpr (L < 16) { fmac_bcast<L>(a[L], w, m); fmac_all<L + 1>(a, w, m); }
+}
 template <int Q>
-__device__ __forceinline__ void inner_steps(double (&a)[8], const InnerRegs& T, double& m, const uint32_t (&sm)[5]) {
-    if constexpr (Q < 32) {
-        constexpr int k = (8 * Q) >> 4, s = (8 * Q) & 15;
-        const double w = T.w[k];
+__device__ __forceinline__ void inner_steps(double (&a)[16], const InnerRegs& T, double& m, const uint32_t (&sm)[4]) {
+    if constexpr (Q < 16) {
         if constexpr (Q == 0) {
-            a[0] = bcast<0>(w); a[1] = bcast<1>(w); a[2] = bcast<2>(w); a[3] = bcast<3>(w);
-            a[4] = bcast<4>(w); a[5] = bcast<5>(w); a[6] = bcast<6>(w); a[7] = bcast<7>(w);
+            bcast_all<0>(a, T.w[0]);
         } else {
             m = flip_sign(m, sm[__builtin_ctz(Q)]);
             asm volatile("" : "+v"(m));     // one running value: the flip is an XOR in place
-            fmac_bcast<s + 0>(a[0], w, m); fmac_bcast<s + 1>(a[1], w, m);
-            fmac_bcast<s + 2>(a[2], w, m); fmac_bcast<s + 3>(a[3], w, m);
-            fmac_bcast<s + 4>(a[4], w, m); fmac_bcast<s + 5>(a[5], w, m);
-            fmac_bcast<s + 6>(a[6], w, m); fmac_bcast<s + 7>(a[7], w, m);
+            fmac_all<0>(a, T.w[Q], m);
         }
         inner_steps<Q + 1>(a, T, m, sm);
     }
+}
+
+// Signs of the four low inner slots applied to the 16 accumulators of one
+// outer point: (tr, ti) of its 256-branch sum (bit 0 of l is the class).
+template <bool IMAG>
+__device__ __forceinline__ void combine_low16(uint32_t hs, const double (&a)[16], double& tr, double& ti) {
+    const double s0 = (hs & 1u) ? -1.0 : 1.0, s1 = (hs & 2u) ? -1.0 : 1.0, s2 = (hs & 4u) ? -1.0 : 1.0,
+                 s3 = (hs & 8u) ? -1.0 : 1.0;
+    double v[2];
+#pragma unroll
+    for (int c = 0; c < 2; ++c) {
+        const double e0 = fma(s1, a[c + 2], a[c]), e1 = fma(s1, a[c + 6], a[c + 4]);
+        const double e2 = fma(s1, a[c + 10], a[c + 8]), e3 = fma(s1, a[c + 14], a[c + 12]);
+        v[c] = fma(s3, fma(s2, e3, e2), fma(s2, e1, e0));
+    }
+    if constexpr (IMAG) { tr = v[0]; ti = s0 * v[1]; }
+    else { tr = fma(s0, v[1], v[0]); ti = 0.0; }
 }
 
 template <bool HASG, bool IMAG>
@@ -799,16 +819,16 @@ __device__ __forceinline__ void inner_walk(const Z8Dev& E, const uint32_t (&ps)[
                                            const LaneState& S, double F, double (&acc)[4]) {
     double tr = 0.0, ti = 0.0;
     if constexpr (!HASG) {
-        uint32_t sm[5];
+        uint32_t sm[4];
 #pragma unroll
-        for (int j = 0; j < 5; ++j) sm[j] = (S.hs << (28 - j)) & 0x80000000u;      // bit 3 + j of hs
+        for (int j = 0; j < 4; ++j) sm[j] = (S.hs << (27 - j)) & 0x80000000u;      // bit 4 + j of hs
         // The multiplier is +-1.0: an empty asm hides its constant low word,
         // so a sign flip is one XOR on the high word in place.
         double m = 1.0;
         asm volatile("" : "+v"(m));
-        double a[8];
+        double a[16];
         inner_steps<0>(a, T, m, sm);
-        combine_low<IMAG>(S.hs, a, tr, ti);
+        combine_low16<IMAG>(S.hs, a, tr, ti);
     } else {
         const PQ_CONST double* Wv = cview(E.inner);
         uint32_t sm[8];
@@ -838,6 +858,46 @@ __device__ __forceinline__ void inner_walk(const Z8Dev& E, const uint32_t (&ps)[
         }
     }
     fold64(tr * F, ti * F, S.K, acc);
+}
+
+// Fast-path point: the same inner walk and low-slot combine as inner_walk,
+// with the sign mask hs = hs_lane ^ hsu and the fold done in the lane's own
+// frame.  The point's phase is K_seed + D + 4 par with D the same for every
+// lane, so its rotation i^((D >> 1) & 3) and its EV / OD routing are the same
+// for every lane too: they are FMAs with uniform 0 / +-1 coefficients (SGPR
+// operands), and the lane's part is the sign par (bit 31 of parw) on the
+// mid-slot magnitude fm.  The lane frame, omega^K_seed and the lane factor,
+// is applied once at the end of the lane's walk.
+template <bool IMAG>
+__device__ __forceinline__ void inner_walk_mid(const InnerRegs& T, uint32_t hs, uint32_t parw, double fm, uint32_t D,
+                                               double (&accl)[4]) {
+    uint32_t sm[4];
+#pragma unroll
+    for (int j = 0; j < 4; ++j) sm[j] = (hs << (27 - j)) & 0x80000000u;
+    double m = 1.0;
+    asm volatile("" : "+v"(m));
+    double a[16];
+    inner_steps<0>(a, T, m, sm);
+    double tr, ti;
+    combine_low16<IMAG>(hs, a, tr, ti);
+    const double fs = flip_sign(fm, parw);
+    tr *= fs;
+    // uniform coefficients: i^r (tr + i ti) into EV (D even) or OD (D odd)
+    const uint32_t r = (D >> 1) & 3u, odd = D & 1u;
+    const double c = (r == 0u) ? 1.0 : (r == 2u) ? -1.0 : 0.0;      // re i^r
+    const double sn = (r == 1u) ? 1.0 : (r == 3u) ? -1.0 : 0.0;     // im i^r
+    const double ce = odd ? 0.0 : c, se = odd ? 0.0 : sn, co = odd ? c : 0.0, so = odd ? sn : 0.0;
+    accl[0] = fma(ce, tr, accl[0]);
+    accl[1] = fma(se, tr, accl[1]);
+    accl[2] = fma(co, tr, accl[2]);
+    accl[3] = fma(so, tr, accl[3]);
+    if constexpr (IMAG) {
+        ti *= fs;
+        accl[0] = fma(-se, ti, accl[0]);
+        accl[1] = fma(ce, ti, accl[1]);
+        accl[2] = fma(-so, ti, accl[2]);
+        accl[3] = fma(co, ti, accl[3]);
+    }
 }
 
 // One instantiation per inner-step variant (pivot lookups or not, imaginary
@@ -916,21 +976,26 @@ __global__ __launch_bounds__(256) void k_z8_walk(const Z8Dev* __restrict__ engs,
                 // each mid slot's cross term with the lane's hi slots
                 uint32_t P = 0;
                 for (uint32_t j = 0; j < Q.M; ++j) P |= (uint32_t)(__popcll(cview(E.slot_q)[8 + j] & S.cout) & 1) << j;
-                // along the mid walk: the mid-only part of each step is the same
-                // for every lane; the lane adds its cross term
-                uint32_t cmid = 0;
+                // along the mid walk the phase step D of each flip is the same
+                // for every lane; the lane's cross term only flips the sign
+                // (parw), and the sign-mask step is a uniform byte
+                uint32_t cmid = 0, D = 0, parw = 0;
+                double accl[4] = {0.0, 0.0, 0.0, 0.0};
                 for (uint32_t o = 0; o < nout; ++o) {
                     if (o) {
                         const uint32_t j = (uint32_t)__builtin_ctz(o), p = 8u + j;
                         const uint64_t qp = cview(E.slot_q)[p];
                         const uint32_t ell = cview(E.slot_lam)[p] + (((uint32_t)(Q.delta >> p) & 1u) << 2);
-                        const uint32_t du = (((cmid >> j) & 1u) ? 8u - ell : ell) + ((uint32_t)(__popcll(qp & ((uint64_t)cmid << 8)) & 1) << 2);
-                        S.K += du + (((P >> j) & 1u) << 2);
+                        D += (((cmid >> j) & 1u) ? 8u - ell : ell) + ((uint32_t)(__popcll(qp & ((uint64_t)cmid << 8)) & 1) << 2);
+                        parw ^= (P << (31 - j)) & 0x80000000u;
                         S.hs ^= (uint32_t)qp & 0xFFu;
                         cmid ^= 1u << j;
                     }
-                    inner_walk<HASG, IMAG>(E, ps, s_gin, T, S, Fl * s_fm[o], acc);
+                    inner_walk_mid<IMAG>(T, S.hs, parw, s_fm[o], D, accl);
                 }
+                // undo the lane frame: omega^K_seed (EV + omega OD) times the lane factor
+                fold64(accl[0] * Fl, accl[1] * Fl, S.K, acc);
+                fold64(accl[2] * Fl, accl[3] * Fl, S.K + 1u, acc);
             } else {
                 for (uint32_t o = 0; o < nout; ++o) {
                     if (o) flip_mid(E, Q, 8u + (uint32_t)__builtin_ctz(o), S);

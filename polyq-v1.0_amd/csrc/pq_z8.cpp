@@ -135,14 +135,14 @@ Z8Image build_z8_image(const HostEngine& E) {
         Z.inner_aux[i] = pin << 3;
         Z.inner_aux[256 + i] = kin;
     }
-    // Accumulator-walk order: c_in = (gray5(q) << 3) | l at 8 q + l; entry i
+    // Accumulator-walk order: c_in = (gray4(q) << 4) | l at 16 q + l; entry i
     // of the Gray order covers gray(i), so c_in sits at i = gray^-1(c_in).
     Z.inner_acc.assign(256, 0.0);
-    for (uint32_t q = 0; q < 32; ++q)
-        for (uint32_t l = 0; l < 8; ++l) {
-            uint32_t c = ((q ^ (q >> 1)) << 3) | l, i = c;
+    for (uint32_t q = 0; q < 16; ++q)
+        for (uint32_t l = 0; l < 16; ++l) {
+            uint32_t c = ((q ^ (q >> 1)) << 4) | l, i = c;
             i ^= i >> 1; i ^= i >> 2; i ^= i >> 4;
-            Z.inner_acc[8 * q + l] = Z.inner[i];
+            Z.inner_acc[16 * q + l] = Z.inner[i];
         }
     // Lane-factor byte tables, with every lane term and without the free
     // terms of the pivot-free outer slots kInner .. kInner + nx - 1
