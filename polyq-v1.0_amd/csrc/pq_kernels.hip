@@ -799,19 +799,23 @@ __device__ __forceinline__ void inner_steps(double (&a)[16], const InnerRegs& T,
 
 // Signs of the four low inner slots applied to the 16 accumulators of one
 // outer point: (tr, ti) of its 256-branch sum (bit 0 of l is the class).
+// The same with the four low-slot signs s[k] = (-1)^(bit k of hs) given.
 template <bool IMAG>
-__device__ __forceinline__ void combine_low16(uint32_t hs, const double (&a)[16], double& tr, double& ti) {
-    const double s0 = (hs & 1u) ? -1.0 : 1.0, s1 = (hs & 2u) ? -1.0 : 1.0, s2 = (hs & 4u) ? -1.0 : 1.0,
-                 s3 = (hs & 8u) ? -1.0 : 1.0;
+__device__ __forceinline__ void combine_signs(const double (&s)[4], const double (&a)[16], double& tr, double& ti) {
     double v[2];
 #pragma unroll
     for (int c = 0; c < 2; ++c) {
-        const double e0 = fma(s1, a[c + 2], a[c]), e1 = fma(s1, a[c + 6], a[c + 4]);
-        const double e2 = fma(s1, a[c + 10], a[c + 8]), e3 = fma(s1, a[c + 14], a[c + 12]);
-        v[c] = fma(s3, fma(s2, e3, e2), fma(s2, e1, e0));
+        const double e0 = fma(s[1], a[c + 2], a[c]), e1 = fma(s[1], a[c + 6], a[c + 4]);
+        const double e2 = fma(s[1], a[c + 10], a[c + 8]), e3 = fma(s[1], a[c + 14], a[c + 12]);
+        v[c] = fma(s[3], fma(s[2], e3, e2), fma(s[2], e1, e0));
     }
-    if constexpr (IMAG) { tr = v[0]; ti = s0 * v[1]; }
-    else { tr = fma(s0, v[1], v[0]); ti = 0.0; }
+    if constexpr (IMAG) { tr = v[0]; ti = s[0] * v[1]; }
+    else { tr = fma(s[0], v[1], v[0]); ti = 0.0; }
+}
+template <bool IMAG>
+__device__ __forceinline__ void combine_low16(uint32_t hs, const double (&a)[16], double& tr, double& ti) {
+    const double s[4] = {(hs & 1u) ? -1.0 : 1.0, (hs & 2u) ? -1.0 : 1.0, (hs & 4u) ? -1.0 : 1.0, (hs & 8u) ? -1.0 : 1.0};
+    combine_signs<IMAG>(s, a, tr, ti);
 }
 
 template <bool HASG, bool IMAG>
@@ -861,25 +865,23 @@ __device__ __forceinline__ void inner_walk(const Z8Dev& E, const uint32_t (&ps)[
 }
 
 // Fast-path point: the same inner walk and low-slot combine as inner_walk,
-// with the sign mask hs = hs_lane ^ hsu and the fold done in the lane's own
-// frame.  The point's phase is K_seed + D + 4 par with D the same for every
+// with the point's sign words sm (hs bits 4..7 at bit 31) and low-slot signs
+// slo (hs bits 0..3 as +-1.0) carried from point to point, and the fold done
+// in the lane's own frame.  The point's phase is K_seed + D + 4 par with D the same for every
 // lane, so its rotation i^((D >> 1) & 3) and its EV / OD routing are the same
 // for every lane too: they are FMAs with uniform 0 / +-1 coefficients (SGPR
 // operands), and the lane's part is the sign par (bit 31 of parw) on the
 // mid-slot magnitude fm.  The lane frame, omega^K_seed and the lane factor,
 // is applied once at the end of the lane's walk.
 template <bool IMAG>
-__device__ __forceinline__ void inner_walk_mid(const InnerRegs& T, uint32_t hs, uint32_t parw, double fm, uint32_t D,
-                                               double (&accl)[4]) {
-    uint32_t sm[4];
-#pragma unroll
-    for (int j = 0; j < 4; ++j) sm[j] = (hs << (27 - j)) & 0x80000000u;
+__device__ __forceinline__ void inner_walk_mid(const InnerRegs& T, const uint32_t (&sm)[4], const double (&slo)[4], uint32_t parw,
+                                               double fm, uint32_t D, double (&accl)[4]) {
     double m = 1.0;
     asm volatile("" : "+v"(m));
     double a[16];
     inner_steps<0>(a, T, m, sm);
     double tr, ti;
-    combine_low16<IMAG>(hs, a, tr, ti);
+    combine_signs<IMAG>(slo, a, tr, ti);
     const double fs = flip_sign(fm, parw);
     tr *= fs;
     // uniform coefficients: i^r (tr + i ti) into EV (D even) or OD (D odd)
@@ -981,6 +983,16 @@ __global__ __launch_bounds__(256) void k_z8_walk(const Z8Dev* __restrict__ engs,
                 // (parw), and the sign-mask step is a uniform byte
                 uint32_t cmid = 0, D = 0, parw = 0;
                 double accl[4] = {0.0, 0.0, 0.0, 0.0};
+                // the point's sign words and low-slot signs, updated in place by
+                // the uniform sign-mask step of each flip
+                uint32_t sm[4];
+                double slo[4];
+#pragma unroll
+                for (int k = 0; k < 4; ++k) {
+                    sm[k] = (S.hs << (27 - k)) & 0x80000000u;
+                    slo[k] = (S.hs >> k) & 1u ? -1.0 : 1.0;
+                    asm volatile("" : "+v"(slo[k]));
+                }
                 for (uint32_t o = 0; o < nout; ++o) {
                     if (o) {
                         const uint32_t j = (uint32_t)__builtin_ctz(o), p = 8u + j;
@@ -988,10 +1000,16 @@ __global__ __launch_bounds__(256) void k_z8_walk(const Z8Dev* __restrict__ engs,
                         const uint32_t ell = cview(E.slot_lam)[p] + (((uint32_t)(Q.delta >> p) & 1u) << 2);
                         D += (((cmid >> j) & 1u) ? 8u - ell : ell) + ((uint32_t)(__popcll(qp & ((uint64_t)cmid << 8)) & 1) << 2);
                         parw ^= (P << (31 - j)) & 0x80000000u;
-                        S.hs ^= (uint32_t)qp & 0xFFu;
+                        const uint32_t x = (uint32_t)qp & 0xFFu;      // the sign-mask step, the same for every lane
+#pragma unroll
+                        for (int k = 0; k < 4; ++k) {
+                            sm[k] ^= (x << (27 - k)) & 0x80000000u;
+                            slo[k] = flip_sign(slo[k], (x << (31 - k)) & 0x80000000u);
+                            asm volatile("" : "+v"(slo[k]));
+                        }
                         cmid ^= 1u << j;
                     }
-                    inner_walk_mid<IMAG>(T, S.hs, parw, s_fm[o], D, accl);
+                    inner_walk_mid<IMAG>(T, sm, slo, parw, s_fm[o], D, accl);
                 }
                 // undo the lane frame: omega^K_seed (EV + omega OD) times the lane factor
                 fold64(accl[0] * Fl, accl[1] * Fl, S.K, acc);
