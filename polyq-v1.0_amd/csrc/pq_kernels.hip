@@ -722,27 +722,15 @@ __device__ __forceinline__ double lane_factor(const double* s_lt, uint32_t nbyte
 // inner basis so), without it every entry is real.  The sign of branch c_in is
 // (-1)^|c_in & hs|, hs being the lane's inner sign mask.
 //
-// Without inner pivots (HASG false) the walk keeps 8 accumulators, one per
-// pattern l of the three low inner slots, and walks the five high slots in
-// Gray order: per branch one FP64 FMA of the +-1 multiplier of the high slots
-// with the uniform table entry (16-dword scalar loads, 8 entries per step),
-// and one XOR on the multiplier's sign word per 8 branches.  The low slots'
-// signs are applied once, when the 8 accumulators are combined (7 FMAs).
+// Without inner pivots the table lives in VGPRs and the walk keeps 16
+// accumulators, one per pattern l of the four low inner slots, walking the
+// four high slots in Gray order (inner_steps, inner_walk_mid below).
 //
-// With inner pivots every branch also reads its pivot factor
-// gin[pio ^ pin(c_in)] from LDS (the address follows the same Gray recurrence
-// as the sign) and flips that factor's sign word: per branch a sign XOR, an
-// address XOR, an LDS read and one FP64 FMA.
-// Signs of the three low inner slots applied to the 8 accumulators of one
-// outer point: (tr, ti) of its 256-branch sum.
-template <bool IMAG>
-__device__ __forceinline__ void combine_low(uint32_t hs, const double (&a)[8], double& tr, double& ti) {
-    const double s0 = (hs & 1u) ? -1.0 : 1.0, s1 = (hs & 2u) ? -1.0 : 1.0, s2 = (hs & 4u) ? -1.0 : 1.0;
-    const double ev = fma(s2, fma(s1, a[6], a[4]), fma(s1, a[2], a[0]));          // l = 0, 2, 4, 6
-    const double od = fma(s2, fma(s1, a[7], a[5]), fma(s1, a[3], a[1]));          // l = 1, 3, 5, 7
-    if constexpr (IMAG) { tr = ev; ti = s0 * od; }
-    else { tr = fma(s0, od, ev); ti = 0.0; }
-}
+// With inner pivots (inner_walk_pivots) every branch also reads its pivot
+// factor gin[pio ^ pin(c_in)] from LDS (the address follows the same Gray
+// recurrence as the sign) and flips that factor's sign word: per branch a sign
+// XOR, an address XOR, an LDS read and one FP64 FMA with the table entry (a
+// 16-dword scalar load per 16 entries).
 
 // The 256-entry inner table held in VGPRs: lane r of each 16-lane row keeps
 // entries r, 16 + r, ..., 240 + r (16 register pairs), so entry e is entry
@@ -799,7 +787,9 @@ __device__ __forceinline__ void inner_steps(double (&a)[16], const InnerRegs& T,
 
 // Signs of the four low inner slots applied to the 16 accumulators of one
 // outer point: (tr, ti) of its 256-branch sum (bit 0 of l is the class).
-// The same with the four low-slot signs s[k] = (-1)^(bit k of hs) given.
+// Signs of the four low inner slots, s[k] = (-1)^(bit k of hs), applied to
+// the 16 accumulators of one outer point: (tr, ti) of its 256-branch sum (bit
+// 0 of l is the class).
 template <bool IMAG>
 __device__ __forceinline__ void combine_signs(const double (&s)[4], const double (&a)[16], double& tr, double& ti) {
     double v[2];
@@ -812,28 +802,13 @@ __device__ __forceinline__ void combine_signs(const double (&s)[4], const double
     if constexpr (IMAG) { tr = v[0]; ti = s[0] * v[1]; }
     else { tr = fma(s[0], v[1], v[0]); ti = 0.0; }
 }
-template <bool IMAG>
-__device__ __forceinline__ void combine_low16(uint32_t hs, const double (&a)[16], double& tr, double& ti) {
-    const double s[4] = {(hs & 1u) ? -1.0 : 1.0, (hs & 2u) ? -1.0 : 1.0, (hs & 4u) ? -1.0 : 1.0, (hs & 8u) ? -1.0 : 1.0};
-    combine_signs<IMAG>(s, a, tr, ti);
-}
 
-template <bool HASG, bool IMAG>
-__device__ __forceinline__ void inner_walk(const Z8Dev& E, const uint32_t (&ps)[8], const double* s_gin, const InnerRegs& T,
-                                           const LaneState& S, double F, double (&acc)[4]) {
+
+template <bool IMAG>
+__device__ __forceinline__ void inner_walk_pivots(const Z8Dev& E, const uint32_t (&ps)[8], const double* s_gin, const LaneState& S,
+                                                  double F, double (&acc)[4]) {
     double tr = 0.0, ti = 0.0;
-    if constexpr (!HASG) {
-        uint32_t sm[4];
-#pragma unroll
-        for (int j = 0; j < 4; ++j) sm[j] = (S.hs << (27 - j)) & 0x80000000u;      // bit 4 + j of hs
-        // The multiplier is +-1.0: an empty asm hides its constant low word,
-        // so a sign flip is one XOR on the high word in place.
-        double m = 1.0;
-        asm volatile("" : "+v"(m));
-        double a[16];
-        inner_steps<0>(a, T, m, sm);
-        combine_low16<IMAG>(S.hs, a, tr, ti);
-    } else {
+    {
         const PQ_CONST double* Wv = cview(E.inner);
         uint32_t sm[8];
 #pragma unroll
@@ -864,15 +839,15 @@ __device__ __forceinline__ void inner_walk(const Z8Dev& E, const uint32_t (&ps)[
     fold64(tr * F, ti * F, S.K, acc);
 }
 
-// Fast-path point: the same inner walk and low-slot combine as inner_walk,
-// with the point's sign words sm (hs bits 4..7 at bit 31) and low-slot signs
-// slo (hs bits 0..3 as +-1.0) carried from point to point, and the fold done
-// in the lane's own frame.  The point's phase is K_seed + D + 4 par with D the same for every
-// lane, so its rotation i^((D >> 1) & 3) and its EV / OD routing are the same
-// for every lane too: they are FMAs with uniform 0 / +-1 coefficients (SGPR
-// operands), and the lane's part is the sign par (bit 31 of parw) on the
-// mid-slot magnitude fm.  The lane frame, omega^K_seed and the lane factor,
-// is applied once at the end of the lane's walk.
+// One mid point without inner pivots, with the point's sign words sm (hs bits
+// 4..7 at bit 31) and low-slot signs slo (hs bits 0..3 as +-1.0) carried from
+// point to point, and the fold done in the lane's own frame.  The point's
+// phase is K_seed + D + 4 par with D the same for every lane, so its rotation
+// i^((D >> 1) & 3) and its EV / OD routing are the same for every lane too:
+// they are FMAs with uniform 0 / +-1 coefficients (SGPR operands), and the
+// lane's part is the sign par (bit 31 of parw) on the point's magnitude fm.
+// The lane frame, omega^K_seed and the part of the lane factor common to all
+// its points, is applied once at the end of the lane's walk.
 template <bool IMAG>
 __device__ __forceinline__ void inner_walk_mid(const InnerRegs& T, const uint32_t (&sm)[4], const double (&slo)[4], uint32_t parw,
                                                double fm, uint32_t D, double (&accl)[4]) {
@@ -967,13 +942,20 @@ __global__ __launch_bounds__(256) void k_z8_walk(const Z8Dev* __restrict__ engs,
 #pragma unroll
             for (int j = 0; j < 8; ++j) ps[j] = HASG ? (cview(E.slot_pin)[j] << 3) : 0u;
             const uint32_t nout = 1u << Q.M;
-            if (fast) {
-                // the lane's factor without the mid slots' terms, times the
-                // magnitudes of the pivot-free slots among its hi slots
-                double Fl = lane_factor(s_lt, E.nbytes, S.sout);
-                for (uint32_t j = Q.M; j < E.nx; ++j) {
-                    const double r0 = cview(E.mid_r)[2 * j], r1 = cview(E.mid_r)[2 * j + 1];
-                    Fl *= ((S.sout >> cview(E.mid_term)[j]) & 1) ? r1 : r0;
+            if (!HASG) {
+                // The fast path (pivot-free mid slots): the lane's factor without
+                // the mid slots' terms, times the magnitudes of the pivot-free
+                // slots among its hi slots; each mid point multiplies in its
+                // per-query magnitude.  Otherwise a mid slot can share a pivot
+                // with the hi slots: each mid point takes the whole lane factor
+                // of its branch vector from the byte tables.
+                double Fl = 1.0;
+                if (fast) {
+                    Fl = lane_factor(s_lt, E.nbytes, S.sout);
+                    for (uint32_t j = Q.M; j < E.nx; ++j) {
+                        const double r0 = cview(E.mid_r)[2 * j], r1 = cview(E.mid_r)[2 * j + 1];
+                        Fl *= ((S.sout >> cview(E.mid_term)[j]) & 1) ? r1 : r0;
+                    }
                 }
                 // each mid slot's cross term with the lane's hi slots
                 uint32_t P = 0;
@@ -1008,8 +990,10 @@ __global__ __launch_bounds__(256) void k_z8_walk(const Z8Dev* __restrict__ engs,
                             asm volatile("" : "+v"(slo[k]));
                         }
                         cmid ^= 1u << j;
+                        if (!fast) S.sout ^= cview(E.slot_n)[p];
                     }
-                    inner_walk_mid<IMAG>(T, sm, slo, parw, s_fm[o], D, accl);
+                    const double fm = fast ? s_fm[o] : lane_factor(s_lt, E.nbytes, S.sout);
+                    inner_walk_mid<IMAG>(T, sm, slo, parw, fm, D, accl);
                 }
                 // undo the lane frame: omega^K_seed (EV + omega OD) times the lane factor
                 fold64(accl[0] * Fl, accl[1] * Fl, S.K, acc);
@@ -1018,7 +1002,7 @@ __global__ __launch_bounds__(256) void k_z8_walk(const Z8Dev* __restrict__ engs,
                 for (uint32_t o = 0; o < nout; ++o) {
                     if (o) flip_mid(E, Q, 8u + (uint32_t)__builtin_ctz(o), S);
                     const double F = lane_factor(s_lt, E.nbytes, S.sout);
-                    inner_walk<HASG, IMAG>(E, ps, s_gin, T, S, F, acc);
+                    inner_walk_pivots<IMAG>(E, ps, s_gin, S, F, acc);
                 }
             }
         }
