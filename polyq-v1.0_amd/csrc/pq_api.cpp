@@ -950,6 +950,7 @@ int pq_sim_z8_info(pq_sim* h, uint32_t* o) {
         const Z8Image& Z = *h->s.eng->z8[0];
         o[0] = Z.d; o[1] = Z.nin; o[2] = Z.nslots; o[3] = Z.npin; o[4] = h->s.eng->host[0]->rk; o[5] = Z.imag0 ? 1 : 0; o[6] = Z.ops_per_branch();
         o[7] = Z.nx;
+        o[8] = Z.low_pivot_free ? 1u : 0u;
         return 0;
     })
 }

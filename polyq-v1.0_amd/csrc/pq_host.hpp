@@ -285,16 +285,20 @@ struct Z8Image {
     // of slot kInner + j and its magnitude r_b; lt_rest are the lane-factor
     // byte tables without those nx terms.
     uint32_t nx = 0;
+    // The four low inner slots touch no pivot, so with inner pivots the pivot
+    // factor of a branch depends on the four high inner slots alone.
+    bool low_pivot_free = false;
     std::vector<uint32_t> mid_term;
     std::vector<double> mid_r;
     std::vector<double> lt_rest;
     std::vector<double> gin;           // 2^npin inner pivot factor table (FP64)
     std::vector<uint32_t> pin_term;    // term index of each compact inner pivot bit
     uint64_t lane_terms = 0;           // terms whose ratio is carried by the lane factor
-    // VALU lane-ops per branch of the inner step: with inner pivots a sign
-    // XOR, an address XOR, an LDS read and one FP64 FMA; without, one FP64 FMA
-    // (the sign is applied once per 8 branches).
-    uint32_t ops_per_branch() const { return npin ? 4u : 1u; }
+    // VALU lane-ops per branch of the inner step: one FP64 FMA (the sign, and
+    // with inner pivots the pivot factor, is applied once per 16 branches),
+    // except with inner pivots touched by the low inner slots: then a sign
+    // XOR, an address XOR, an LDS read and one FP64 FMA.
+    uint32_t ops_per_branch() const { return npin && !low_pivot_free ? 4u : 1u; }
 };
 Z8Image build_z8_image(const HostEngine& E);
 // Query part of the Z8 walk: K0, delta bits over slots, s0, compact pivots.

@@ -145,7 +145,7 @@ struct Z8Dev {
     const double* lt_rest;   // nbytes x 256, without the terms of the pivot-free outer slots 8 .. 8 + nx - 1
     const double* mid_r;     // nx x {r0, r1}: magnitudes of those slots' terms
     const uint32_t* mid_term;  // nx: their term indices
-    uint32_t nslots, nbytes, npin, imag0, nx, pad;
+    uint32_t nslots, nbytes, npin, imag0, nx, lowfree;   // lowfree: the four low inner slots touch no pivot
 };
 struct Z8Q {                 // one query of a batch (64 B)
     uint64_t delta, s0, item0, nitems;
@@ -337,7 +337,8 @@ std::shared_ptr<DevEngine> dev_engine_upload(std::shared_ptr<const HostEngine> E
     z.inner = (const double*)at(O.o_in); z.inner_acc = (const double*)at(O.o_inacc); z.inaux = (const uint32_t*)at(O.o_inaux);
     z.lt = (const double*)at(O.o_lt); z.gin = (const double*)at(O.o_gin);
     z.lt_rest = (const double*)at(O.o_ltr); z.mid_r = (const double*)at(O.o_mr); z.mid_term = (const uint32_t*)at(O.o_mt);
-    z.nslots = Z.nslots; z.nbytes = Z.nbytes; z.npin = Z.npin; z.imag0 = Z.imag0 ? 1u : 0u; z.nx = Z.nx; z.pad = 0;
+    z.nslots = Z.nslots; z.nbytes = Z.nbytes; z.npin = Z.npin; z.imag0 = Z.imag0 ? 1u : 0u; z.nx = Z.nx;
+    z.lowfree = Z.low_pivot_free ? 1u : 0u;
     SegDev& g = D->sg;
     g.oq = (const uint64_t*)at(O.g_oq); g.ox = (const uint32_t*)at(O.g_ox); g.opin = (const uint32_t*)at(O.g_opin);
     g.olam = (const uint32_t*)at(O.g_olam); g.on = (const uint64_t*)at(O.g_on); g.table = (const float4*)at(O.g_tab);
@@ -787,6 +788,26 @@ __device__ __forceinline__ void inner_steps(double (&a)[16], const InnerRegs& T,
 
 // Signs of the four low inner slots applied to the 16 accumulators of one
 // outer point: (tr, ti) of its 256-branch sum (bit 0 of l is the class).
+// The same walk for an engine whose inner directions touch pivots but whose
+// four low inner slots do not: the pivot factor gin[pio ^ pin(c_in)] then
+// depends on the step q alone, so step q reads it once from LDS (the byte
+// offset follows the Gray recurrence over the high slots' pivots ps) and its
+// signed value is the multiplier of the step's 16 FMAs.  The accumulators
+// start at zero.
+template <int Q>
+__device__ __forceinline__ void inner_steps_piv(double (&a)[16], const InnerRegs& T, const double* s_gin, uint32_t addr, uint32_t sgn,
+                                                const uint32_t (&ps)[4], const uint32_t (&sm)[4]) {
+    if constexpr (Q < 16) {
+        if constexpr (Q > 0) {
+            addr ^= ps[__builtin_ctz(Q)];
+            sgn ^= sm[__builtin_ctz(Q)];
+        }
+        const double m = flip_sign(*(const double*)((const char*)s_gin + addr), sgn);
+        fmac_all<0>(a, T.w[Q], m);
+        inner_steps_piv<Q + 1>(a, T, s_gin, addr, sgn, ps, sm);
+    }
+}
+
 // Signs of the four low inner slots, s[k] = (-1)^(bit k of hs), applied to
 // the 16 accumulators of one outer point: (tr, ti) of its 256-branch sum (bit
 // 0 of l is the class).
@@ -847,14 +868,23 @@ __device__ __forceinline__ void inner_walk_pivots(const Z8Dev& E, const uint32_t
 // they are FMAs with uniform 0 / +-1 coefficients (SGPR operands), and the
 // lane's part is the sign par (bit 31 of parw) on the point's magnitude fm.
 // The lane frame, omega^K_seed and the part of the lane factor common to all
-// its points, is applied once at the end of the lane's walk.
-template <bool IMAG>
+// its points, is applied once at the end of the lane's walk.  With PIV the
+// inner steps carry the pivot factor (inner_steps_piv, pio the point's inner
+// pivots).
+template <bool IMAG, bool PIV>
 __device__ __forceinline__ void inner_walk_mid(const InnerRegs& T, const uint32_t (&sm)[4], const double (&slo)[4], uint32_t parw,
-                                               double fm, uint32_t D, double (&accl)[4]) {
-    double m = 1.0;
-    asm volatile("" : "+v"(m));
+                                               double fm, uint32_t D, double (&accl)[4], const double* s_gin, uint32_t pio,
+                                               const uint32_t (&ps)[4]) {
     double a[16];
-    inner_steps<0>(a, T, m, sm);
+    if constexpr (PIV) {
+#pragma unroll
+        for (int l = 0; l < 16; ++l) a[l] = 0.0;
+        inner_steps_piv<0>(a, T, s_gin, pio << 3, 0u, ps, sm);
+    } else {
+        double m = 1.0;
+        asm volatile("" : "+v"(m));
+        inner_steps<0>(a, T, m, sm);
+    }
     double tr, ti;
     combine_signs<IMAG>(slo, a, tr, ti);
     const double fs = flip_sign(fm, parw);
@@ -931,24 +961,29 @@ __global__ __launch_bounds__(256) void k_z8_walk(const Z8Dev* __restrict__ engs,
         const uint64_t lane = (item - Q.item0) * 256 + tid;
         const bool valid = (lane >> Q.H) == 0;
         InnerRegs T;
-        if constexpr (!HASG) load_inner_regs(E, T);
+        // the table in VGPRs: always without inner pivots, and with them when
+        // the four low inner slots are pivot-free (uniform per engine)
+        const bool lanewalk = !HASG || E.lowfree;
+        if (lanewalk) load_inner_regs(E, T);
         // without pivots every lane walks (the table broadcast needs the whole
         // wave); lanes past the query's 2^H repeat a valid lane's coset and
         // are zeroed below
-        if (!HASG || valid) {
+        if (lanewalk || valid) {
             LaneState S;
             seed_lane(E, Q, lane, S);
             uint32_t ps[8];
 #pragma unroll
             for (int j = 0; j < 8; ++j) ps[j] = HASG ? (cview(E.slot_pin)[j] << 3) : 0u;
             const uint32_t nout = 1u << Q.M;
-            if (!HASG) {
+            if (lanewalk) {
                 // The fast path (pivot-free mid slots): the lane's factor without
                 // the mid slots' terms, times the magnitudes of the pivot-free
                 // slots among its hi slots; each mid point multiplies in its
                 // per-query magnitude.  Otherwise a mid slot can share a pivot
                 // with the hi slots: each mid point takes the whole lane factor
-                // of its branch vector from the byte tables.
+                // of its branch vector from the byte tables.  With inner pivots
+                // (the four low inner slots pivot-free) the inner steps also
+                // carry the pivot factor of the point's inner pivots pio.
                 double Fl = 1.0;
                 if (fast) {
                     Fl = lane_factor(s_lt, E.nbytes, S.sout);
@@ -991,9 +1026,11 @@ __global__ __launch_bounds__(256) void k_z8_walk(const Z8Dev* __restrict__ engs,
                         }
                         cmid ^= 1u << j;
                         if (!fast) S.sout ^= cview(E.slot_n)[p];
+                        if (HASG) S.pio ^= cview(E.slot_pin)[p];
                     }
                     const double fm = fast ? s_fm[o] : lane_factor(s_lt, E.nbytes, S.sout);
-                    inner_walk_mid<IMAG>(T, sm, slo, parw, fm, D, accl);
+                    const uint32_t pshi[4] = {ps[4], ps[5], ps[6], ps[7]};
+                    inner_walk_mid<IMAG, HASG>(T, sm, slo, parw, fm, D, accl, s_gin, S.pio, pshi);
                 }
                 // undo the lane frame: omega^K_seed (EV + omega OD) times the lane factor
                 fold64(accl[0] * Fl, accl[1] * Fl, S.K, acc);

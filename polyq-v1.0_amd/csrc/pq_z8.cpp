@@ -99,6 +99,11 @@ Z8Image build_z8_image(const HostEngine& E) {
         for (uint32_t q = 0; q < S; ++q)
             if (q != p && Z.slot_real[q] && z8_cross(E, U[p].data(), N[p], U[q].data(), N[q])) Z.slot_q[p] |= 1ull << q;
     }
+    // Pivot-free inner directions come first and the imaginary-class carrier
+    // moved into slot 0 is the first of its class, so the low four slots are
+    // pivot-free whenever at least four inner directions are and the carrier
+    // (if any) is one of them.
+    Z.low_pivot_free = (Z.slot_pin[0] | Z.slot_pin[1] | Z.slot_pin[2] | Z.slot_pin[3]) == 0;
 
     // Terms carried by the lane factor: all but the free terms of the inner
     // directions and the inner-touched pivots.

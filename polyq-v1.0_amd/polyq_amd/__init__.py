@@ -421,10 +421,10 @@ class Simulator:
         return tuple(int(v) for v in o)
 
     def z8_info(self):
-        o = np.zeros(8, dtype=np.uint32)
+        o = np.zeros(9, dtype=np.uint32)
         _check(lib.pq_sim_z8_info(self._h, _ptr(o, _u32p)))
-        return dict(zip(("d", "inner", "slots", "inner_pivots", "rank_k", "imag_slot0", "ops_per_branch", "pivot_free_mid"),
-                        (int(v) for v in o)))
+        return dict(zip(("d", "inner", "slots", "inner_pivots", "rank_k", "imag_slot0", "ops_per_branch", "pivot_free_mid",
+                         "low_slots_pivot_free"), (int(v) for v in o)))
 
     def seg_info(self):
         o = np.zeros(8, dtype=np.uint64)

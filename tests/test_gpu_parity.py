@@ -168,14 +168,16 @@ def test_config2_small_seed_against_oracle():
             assert rel.max() < 1e-9
 
 
-def test_branch_walk_point_pairs_and_single_points_agree():
-    # the per-branch walk takes its outer points in pairs when a lane walks
-    # mid slots (M >= 1) and one at a time otherwise; force each in turn on
+def test_branch_walk_paths_and_walk_lengths_agree():
+    # the per-branch walk at several mid-walk lengths, on the kernel's paths:
+    # pivot-free mid slots (fast), mid slots sharing pivots (generic), inner
+    # pivots with the four low inner slots pivot-free (lane walk) or not (one
+    # point at a time); force each walk length in turn on
     # circuits of branch dimension 7..14 and on the config-2 generator at seed 1
     # (d = 19), and check both against the FP64 form and the oracle
     rng = W.Rng(17)
     saved = pq.get_walk_tuning("branch_walk")
-    paths = {"fast": 0, "generic": 0}
+    paths = {"fast": 0, "generic": 0, "pivots_lane_walk": 0, "pivots_per_point": 0}
     try:
         cases = 0
         for trial in range(12):
@@ -183,6 +185,8 @@ def test_branch_walk_point_pairs_and_single_points_agree():
             sims = {f: pq.Simulator(c, strategy="cached", form=f) for f in ("branch_walk", "plain_fp64")}
             d = sims["branch_walk"].stats()[5]
             zi = sims["branch_walk"].z8_info()
+            if zi["inner_pivots"]:
+                paths["pivots_lane_walk" if zi["low_slots_pivot_free"] else "pivots_per_point"] += 1
             for off, mx in ((8, 6), (8, 1)):
                 m = pq.branch_walk_mid_bits(d, off, mx)
                 if m:     # pivot-free mid slots take the kernel's fast path
@@ -206,6 +210,7 @@ def test_branch_walk_point_pairs_and_single_points_agree():
         assert cases >= 6
         print("mid walks by kernel path:", paths)
         assert paths["fast"] >= 2 and paths["generic"] >= 2
+        assert paths["pivots_lane_walk"] >= 2 and paths["pivots_per_point"] >= 2
         c, xs, ys = W.config2(1)
         want = pq.Simulator(c, strategy="cached", form="plain_fp64").amplitudes(xs, ys)
         s = pq.Simulator(c, strategy="cached", form="branch_walk")
