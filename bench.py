@@ -6,8 +6,9 @@ circuit "H on all qubits; 36 T + 300 CX/CZ + 100 S interleaved (SplitMix64
 seed 5); H on all qubits" (t = 30 terms after T merging, branch dimension
 d = 30: 16 x 2^30 branches), all 16 evaluated in one launch of the per-branch
 walk (`k_z8_walk`): every one of the 2^30 branches of each amplitude is
-visited on the GPU, one sign XOR and one FP64 FMA per branch, with integer Z8
-phases and FP64 accumulation.  Engine images are resident in HBM before
+visited on the GPU, one FP64 FMA per branch (the sign flipped once per 16
+branches), with integer Z8 phases and FP64 accumulation.  Engine images are
+resident in HBM before
 timing starts; the host build of the engine (compile, normal form, branch
 tables) is timed apart and reported in `config`.  Reported beside it:
   * the default served form (segment walk: per-engine FP64 Walsh-Hadamard
@@ -203,9 +204,9 @@ def headline(args, pq, W, np, probe):
     timing = pq.last_timing()
     assert np.array_equal(again, got)
     ops = info["ops_per_branch"]
-    mid_bits = pq.branch_walk_mid_bits(d, tuned[0], tuned[1])
+    mid_bits = pq.branch_walk_mid_bits(d, tuned[0], tuned[1], info["inner_slots"])
     # without inner pivots the inner step is one FP64 FMA per branch, with the
-    # table entry broadcast by DPP (the sign XOR comes once per 8 branches):
+    # table entry broadcast by DPP (the sign XOR comes once per 16 branches):
     # priced at the faster of the measured v_fma_f64 and v_fmac_f64_dpp rates;
     # with pivots at the xor/FMA mix
     if ops == 1:

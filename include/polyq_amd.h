@@ -307,9 +307,10 @@ int pq_sim_seg_selfcheck(pq_sim* sim, uint64_t x, uint64_t y, uint32_t trials, u
  * rank K, whether inner entries alternate real/imaginary, VALU lane-ops per
  * branch of the inner step, and the pivot-free outer slots that can serve as
  * mid slots with per-query magnitudes (a lane walk of M <= that many mid
- * coordinates takes the kernel's fast path), and whether the four low inner
- * slots touch no pivot. */
-int pq_sim_z8_info(pq_sim* sim, uint32_t out9[9]);
+ * coordinates takes the kernel's fast path), whether the four low inner
+ * slots touch no pivot, and the inner slots per outer point (the walk sums
+ * 2^that branches per outer point). */
+int pq_sim_z8_info(pq_sim* sim, uint32_t out10[10]);
 
 /* VALU issue-rate probe (no counterpart in the reference): one kernel per
  * instruction (int32 xor, and, add, popcount; FP32 FMA; FP64 FMA; one int32

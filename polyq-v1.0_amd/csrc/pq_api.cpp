@@ -736,17 +736,17 @@ int pq_sim_walk_dump(pq_sim* h, uint64_t x, uint64_t y, uint32_t mid_bits, size_
         Prepared P;
         if (!prepare_assignment(E, asg, P)) return 1;
         Z8QueryHost q = z8_query(E, Z, P);
-        const uint32_t nout = Z.nslots - 8, M = std::min(mid_bits, nout), H = nout - M;
-        const size_t total = (size_t)1 << (8 + M + H);
+        const uint32_t K = Z.kin, nout = Z.nslots - K, M = std::min(mid_bits, nout), H = nout - M;
+        const size_t total = (size_t)1 << (K + M + H);
         if (total > cap) throw std::invalid_argument("walk dump buffer too small");
         R.ensure_device();
         dev_dump_walk(*R.dev[0], q, M, k, mag);
         for (size_t idx = 0; idx < total; ++idx) {
-            const uint64_t i = idx & 255, o = (idx >> 8) & ((1ull << M) - 1), hi = idx >> (8 + M);
+            const uint64_t i = idx & ((1ull << K) - 1), o = (idx >> K) & ((1ull << M) - 1), hi = idx >> (K + M);
             const uint64_t gi = i ^ (i >> 1), go = o ^ (o >> 1);
             uint64_t sv = P.s0;
             for (uint32_t p = 0; p < Z.nslots; ++p) {
-                bool bit = p < 8 ? ((gi >> p) & 1) : (p < 8 + M ? ((go >> (p - 8)) & 1) : ((hi >> (p - 8 - M)) & 1));
+                bool bit = p < K ? ((gi >> p) & 1) : (p < K + M ? ((go >> (p - K)) & 1) : ((hi >> (p - K - M)) & 1));
                 if (bit && Z.slot_real[p]) sv ^= Z.slot_n[p];
             }
             s[idx] = sv;
@@ -951,6 +951,7 @@ int pq_sim_z8_info(pq_sim* h, uint32_t* o) {
         o[0] = Z.d; o[1] = Z.nin; o[2] = Z.nslots; o[3] = Z.npin; o[4] = h->s.eng->host[0]->rk; o[5] = Z.imag0 ? 1 : 0; o[6] = Z.ops_per_branch();
         o[7] = Z.nx;
         o[8] = Z.low_pivot_free ? 1u : 0u;
+        o[9] = Z.kin;
         return 0;
     })
 }

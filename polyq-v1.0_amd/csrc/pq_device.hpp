@@ -67,7 +67,7 @@ bool dev_z8_ready(Z8Slot& S);
 void dev_z8_wait(Z8Slot& S, std::vector<std::complex<double>>& out, Z8Timing* timing);
 // Mid-bit heuristic: enough items to fill every CU several times, long
 // enough walks to amortise the lane seed.
-uint32_t z8_auto_mid_bits(uint32_t d, uint64_t batch_branches);
+uint32_t z8_auto_mid_bits(uint32_t d, uint32_t kin);   // mid coordinates of a lane walk (kin inner slots)
 void z8_set_tuning(uint32_t offset, uint32_t max_mid);
 void z8_get_tuning(uint32_t* offset, uint32_t* max_mid);
 void seg_set_tuning(uint32_t offset, uint32_t max_mid);

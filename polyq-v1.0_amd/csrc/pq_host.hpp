@@ -256,14 +256,22 @@ bool survives_row(const HostEngine& E, const uint64_t* u);
 uint32_t total_phase(const HostEngine& E, const uint64_t* u, uint64_t s);
 
 // ------------------------------------------------------- Z8 walk image
-// Coordinates of the branch subspace are laid out in slots: 8 inner slots
-// (walked from a uniform 256-entry table; slots without a real coordinate are
-// phantoms whose table entries are zero), then the outer slots, whose first M
-// are walked in Gray order per lane and the rest index the lane's coset.
-constexpr uint32_t kInner = 8;
+// Coordinates of the branch subspace are laid out in slots: kin inner slots
+// (walked from a uniform 2^kin-entry table; slots without a real coordinate
+// are phantoms whose table entries are zero), then the outer slots, whose
+// first M are walked in Gray order per lane and the rest index the lane's
+// coset.  The walk keeps 16 accumulators for the four low inner slots and
+// steps over the kin - 4 high ones in Gray order (2^(kin - 4) steps of 16
+// branches per outer point).  The table lives in VGPRs (2^(kin - 3) of them),
+// so its size is set by the registers the kernel variant has left: 10 inner
+// slots when no inner direction touches a pivot, 9 when the walk also carries
+// pivot state.
+constexpr uint32_t kInnerWide = 10;
+constexpr uint32_t kInnerPiv = 9;
 constexpr uint32_t kMidFree = 8;     // pivot-free outer slots whose magnitudes a lane takes from a per-query table
 struct Z8Image {
     uint32_t d = 0, nslots = 0, nin = 0, t = 0, npin = 0, nbytes = 0;
+    uint32_t kin = 0;                  // inner slots: kInnerWide, or kInnerPiv (the walk variant with pivot lookups)
     bool imag0 = false;                // inner entries alternate real / imaginary with slot 0
     std::vector<uint64_t> slot_u;      // packed shift row of each slot's direction (host only)
     std::vector<uint64_t> slot_dual;   // query dual row of each slot (host only)
@@ -272,21 +280,21 @@ struct Z8Image {
     std::vector<uint32_t> slot_pin;    // inner-touched pivot bits of N, compacted
     std::vector<uint8_t> slot_lam;     // query-independent linear coefficient
     std::vector<uint8_t> slot_real;    // slot holds a direction (not a phantom)
-    std::vector<double> inner;         // 256 signed FP64 values W[i] (entry i covers c_in = gray(i))
-    // The same 256 values in the order of the accumulator walk (no inner
-    // pivots): entry 16 q + l covers c_in = (gray4(q) << 4) | l.
+    std::vector<double> inner;         // 2^kin signed FP64 values W[i] (entry i covers c_in = gray(i))
+    // The same values in the order of the accumulator walk (no inner
+    // pivots): entry 16 q + l covers c_in = (gray(q) << 4) | l.
     std::vector<double> inner_acc;
-    std::vector<uint32_t> inner_aux;   // 256 x pin*8 (LDS byte offset of the inner pivots), then 256 x kin
+    std::vector<uint32_t> inner_aux;   // 2^kin x pin*8 (LDS byte offset of the inner pivots), then 2^kin x phase
     std::vector<double> lt;            // nbytes x 256 lane-factor byte tables (FP64)
     // Pivot-free outer slots come first after the inner ones: slots
-    // kInner .. kInner + nx - 1 are single free terms that no other slot
-    // touches (nx <= kMidFree), so as mid slots their magnitudes are the same
-    // for every lane of a query.  mid_term[j] / mid_r[2 j + b] are the term
-    // of slot kInner + j and its magnitude r_b; lt_rest are the lane-factor
+    // kin .. kin + nx - 1 are single free terms that no other slot touches
+    // (nx <= kMidFree), so as mid slots their magnitudes are the same for
+    // every lane of a query.  mid_term[j] / mid_r[2 j + b] are the term of
+    // slot kin + j and its magnitude r_b; lt_rest are the lane-factor
     // byte tables without those nx terms.
     uint32_t nx = 0;
     // The four low inner slots touch no pivot, so with inner pivots the pivot
-    // factor of a branch depends on the four high inner slots alone.
+    // factor of a branch depends on the high inner slots alone.
     bool low_pivot_free = false;
     std::vector<uint32_t> mid_term;
     std::vector<double> mid_r;

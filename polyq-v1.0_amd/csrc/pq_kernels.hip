@@ -10,11 +10,13 @@
 //                 items of many queries and engines.  A lane owns one coset of
 //                 the outer coordinates, walks the mid coordinates in Gray
 //                 order by single-slot updates of its integer state, and sums
-//                 the 256 inner branches of each outer point from a uniform
-//                 FP64 table (scalar loads): per branch one sign flip and one
-//                 FP64 FMA (plus one LDS lookup when inner directions touch
-//                 pivots).  Every branch is visited; partials accumulate in
-//                 FP64, split by the parity of the lane phase.
+//                 the 2^kin inner branches of each outer point from a
+//                 uniform FP64 table held in VGPRs (DPP row broadcasts): per
+//                 branch one FP64 FMA, the sign flipped once per 16 branches
+//                 (an LDS pivot-factor lookup per branch when the low inner
+//                 directions touch pivots).  Every branch is visited;
+//                 partials accumulate in FP64, split by the parity of the
+//                 lane phase.
 //  k_z8_reduce    FP64 cross-workgroup reduction and global scaling.
 //  k_dump_s, k_dump_walk   per-branch outputs for parity tests.
 //
@@ -137,15 +139,16 @@ struct Z8Dev {
     const uint64_t* slot_n;
     const uint32_t* slot_pin;
     const uint32_t* slot_lam;
-    const double* inner;     // 256 signed FP64 values W[i]
+    const double* inner;     // 2^kin signed FP64 values W[i]
     const double* inner_acc; // the same in accumulator-walk order
-    const uint32_t* inaux;   // 256 x pin*8, then 256 x kin
+    const uint32_t* inaux;   // 2^kin x pin*8, then 2^kin x phase
     const double* lt;        // nbytes x 256
     const double* gin;       // 2^npin
-    const double* lt_rest;   // nbytes x 256, without the terms of the pivot-free outer slots 8 .. 8 + nx - 1
+    const double* lt_rest;   // nbytes x 256, without the terms of the pivot-free outer slots kin .. kin + nx - 1
     const double* mid_r;     // nx x {r0, r1}: magnitudes of those slots' terms
     const uint32_t* mid_term;  // nx: their term indices
     uint32_t nslots, nbytes, npin, imag0, nx, lowfree;   // lowfree: the four low inner slots touch no pivot
+    uint32_t kin, pad0;      // inner slots (kInnerWide, or kInnerPiv for the variants with pivot lookups)
 };
 struct Z8Q {                 // one query of a batch (64 B)
     uint64_t delta, s0, item0, nitems;
@@ -339,6 +342,7 @@ std::shared_ptr<DevEngine> dev_engine_upload(std::shared_ptr<const HostEngine> E
     z.lt_rest = (const double*)at(O.o_ltr); z.mid_r = (const double*)at(O.o_mr); z.mid_term = (const uint32_t*)at(O.o_mt);
     z.nslots = Z.nslots; z.nbytes = Z.nbytes; z.npin = Z.npin; z.imag0 = Z.imag0 ? 1u : 0u; z.nx = Z.nx;
     z.lowfree = Z.low_pivot_free ? 1u : 0u;
+    z.kin = Z.kin;
     SegDev& g = D->sg;
     g.oq = (const uint64_t*)at(O.g_oq); g.ox = (const uint32_t*)at(O.g_ox); g.opin = (const uint32_t*)at(O.g_opin);
     g.olam = (const uint32_t*)at(O.g_olam); g.on = (const uint64_t*)at(O.g_on); g.table = (const float4*)at(O.g_tab);
@@ -683,9 +687,16 @@ struct LaneState {
     uint64_t cout, sout;
     uint32_t K, hs, pio;
 };
+// Inner-table geometry of a walk with NIN inner slots: 2^NIN entries, 16
+// accumulators for the four low slots, 2^(NIN - 4) steps over the high ones.
+template <uint32_t NIN>
+struct InnerGeom {
+    static constexpr uint32_t N = 1u << NIN, mask = N - 1, high = NIN - 4, steps = 1u << (NIN - 4);
+};
+template <uint32_t NIN>
 __device__ __forceinline__ void seed_lane(const Z8Dev& E, const Z8Q& Q, uint64_t lane, LaneState& S) {
-    S.cout = 0; S.sout = Q.s0; S.K = Q.K0; S.hs = (uint32_t)Q.delta & 0xFFu; S.pio = Q.pio0;
-    const uint32_t base = 8u + Q.M;
+    S.cout = 0; S.sout = Q.s0; S.K = Q.K0; S.hs = (uint32_t)Q.delta & InnerGeom<NIN>::mask; S.pio = Q.pio0;
+    const uint32_t base = NIN + Q.M;
     for (uint32_t b = 0; b < Q.H; ++b) {
         const uint32_t p = base + b;
         const uint64_t qp = cview(E.slot_q)[p];
@@ -695,12 +706,13 @@ __device__ __forceinline__ void seed_lane(const Z8Dev& E, const Z8Q& Q, uint64_t
         if ((lane >> b) & 1) {
             S.K += ell + ((__popcll(qp & S.cout) & 1u) << 2);
             S.cout |= 1ull << p;
-            S.hs ^= (uint32_t)qp & 0xFFu;
+            S.hs ^= (uint32_t)qp & InnerGeom<NIN>::mask;
             S.pio ^= pin;
             S.sout ^= np;
         }
     }
 }
+template <uint32_t NIN>
 __device__ __forceinline__ void flip_mid(const Z8Dev& E, const Z8Q& Q, uint32_t p, LaneState& S) {
     const uint64_t qp = cview(E.slot_q)[p];
     const uint32_t ell = cview(E.slot_lam)[p] + (((uint32_t)(Q.delta >> p) & 1u) << 2);
@@ -708,7 +720,7 @@ __device__ __forceinline__ void flip_mid(const Z8Dev& E, const Z8Q& Q, uint32_t 
     const bool was = (S.cout >> p) & 1;
     S.K += (was ? (8u - ell) : ell) + par;
     S.cout ^= 1ull << p;
-    S.hs ^= (uint32_t)qp & 0xFFu;
+    S.hs ^= (uint32_t)qp & InnerGeom<NIN>::mask;
     S.pio ^= cview(E.slot_pin)[p];
     S.sout ^= cview(E.slot_n)[p];
 }
@@ -718,14 +730,14 @@ __device__ __forceinline__ double lane_factor(const double* s_lt, uint32_t nbyte
     return F;
 }
 
-// The 256 inner branches of one outer point.  Each table entry is real or
+// The 2^NIN inner branches of one outer point.  Each table entry is real or
 // imaginary; with IMAG the class is bit 0 of c_in (the builder arranges the
 // inner basis so), without it every entry is real.  The sign of branch c_in is
 // (-1)^|c_in & hs|, hs being the lane's inner sign mask.
 //
 // Without inner pivots the table lives in VGPRs and the walk keeps 16
 // accumulators, one per pattern l of the four low inner slots, walking the
-// four high slots in Gray order (inner_steps, inner_walk_mid below).
+// NIN - 4 high slots in Gray order (inner_steps, inner_walk_mid below).
 //
 // With inner pivots (inner_walk_pivots) every branch also reads its pivot
 // factor gin[pio ^ pin(c_in)] from LDS (the address follows the same Gray
@@ -733,18 +745,20 @@ __device__ __forceinline__ double lane_factor(const double* s_lt, uint32_t nbyte
 // XOR, an address XOR, an LDS read and one FP64 FMA with the table entry (a
 // 16-dword scalar load per 16 entries).
 
-// The 256-entry inner table held in VGPRs: lane r of each 16-lane row keeps
-// entries r, 16 + r, ..., 240 + r (16 register pairs), so entry e is entry
-// e & 15 of row-local register e >> 4 and reaches every lane of the row
+// The 2^NIN-entry inner table held in VGPRs: lane r of each 16-lane row
+// keeps entries r, 16 + r, 32 + r, ... (2^(NIN - 4) register pairs), so entry
+// e is entry e & 15 of row-local register e >> 4 and reaches every lane of the row
 // through a DPP row broadcast (row_newbcast) folded into the FMA itself:
 // v_fmac_f64_dpp issues at the full FP64 rate, and the walk then reads no
 // memory at all.  Every lane of the wave must be active while the table is
 // broadcast (a broadcast from a disabled lane does not write).
-struct InnerRegs { double w[16]; };
-__device__ __forceinline__ void load_inner_regs(const Z8Dev& E, InnerRegs& T) {
+template <uint32_t NIN>
+struct InnerRegs { double w[InnerGeom<NIN>::steps]; };
+template <uint32_t NIN>
+__device__ __forceinline__ void load_inner_regs(const Z8Dev& E, InnerRegs<NIN>& T) {
     const uint32_t r = threadIdx.x & 15u;
 #pragma unroll
-    for (int k = 0; k < 16; ++k) T.w[k] = E.inner_acc[16 * k + r];
+    for (int k = 0; k < (int)InnerGeom<NIN>::steps; ++k) T.w[k] = E.inner_acc[16 * k + r];
     // a DPP read of a VGPR needs two wait states after a VALU write of it
     asm volatile("s_nop 1");
 }
@@ -772,9 +786,9 @@ template <int L>
 __device__ __forceinline__ void fmac_all(double (&a)[16], double w, double m) {
     if constexpr (L < 16) { fmac_bcast<L>(a[L], w, m); fmac_all<L + 1>(a, w, m); }
 }
-template <int Q>
-__device__ __forceinline__ void inner_steps(double (&a)[16], const InnerRegs& T, double& m, const uint32_t (&sm)[4]) {
-    if constexpr (Q < 16) {
+template <uint32_t NIN, int Q>
+__device__ __forceinline__ void inner_steps(double (&a)[16], const InnerRegs<NIN>& T, double& m, const uint32_t (&sm)[InnerGeom<NIN>::high]) {
+    if constexpr (Q < (int)InnerGeom<NIN>::steps) {
         if constexpr (Q == 0) {
             bcast_all<0>(a, T.w[0]);
         } else {
@@ -782,7 +796,7 @@ __device__ __forceinline__ void inner_steps(double (&a)[16], const InnerRegs& T,
             asm volatile("" : "+v"(m));     // one running value: the flip is an XOR in place
             fmac_all<0>(a, T.w[Q], m);
         }
-        inner_steps<Q + 1>(a, T, m, sm);
+        inner_steps<NIN, Q + 1>(a, T, m, sm);
     }
 }
 
@@ -794,17 +808,17 @@ __device__ __forceinline__ void inner_steps(double (&a)[16], const InnerRegs& T,
 // offset follows the Gray recurrence over the high slots' pivots ps) and its
 // signed value is the multiplier of the step's 16 FMAs.  The accumulators
 // start at zero.
-template <int Q>
-__device__ __forceinline__ void inner_steps_piv(double (&a)[16], const InnerRegs& T, const double* s_gin, uint32_t addr, uint32_t sgn,
-                                                const uint32_t (&ps)[4], const uint32_t (&sm)[4]) {
-    if constexpr (Q < 16) {
+template <uint32_t NIN, int Q>
+__device__ __forceinline__ void inner_steps_piv(double (&a)[16], const InnerRegs<NIN>& T, const double* s_gin, uint32_t addr, uint32_t sgn,
+                                                const uint32_t (&ps)[InnerGeom<NIN>::high], const uint32_t (&sm)[InnerGeom<NIN>::high]) {
+    if constexpr (Q < (int)InnerGeom<NIN>::steps) {
         if constexpr (Q > 0) {
             addr ^= ps[__builtin_ctz(Q)];
             sgn ^= sm[__builtin_ctz(Q)];
         }
         const double m = flip_sign(*(const double*)((const char*)s_gin + addr), sgn);
         fmac_all<0>(a, T.w[Q], m);
-        inner_steps_piv<Q + 1>(a, T, s_gin, addr, sgn, ps, sm);
+        inner_steps_piv<NIN, Q + 1>(a, T, s_gin, addr, sgn, ps, sm);
     }
 }
 
@@ -825,19 +839,19 @@ __device__ __forceinline__ void combine_signs(const double (&s)[4], const double
 }
 
 
-template <bool IMAG>
-__device__ __forceinline__ void inner_walk_pivots(const Z8Dev& E, const uint32_t (&ps)[8], const double* s_gin, const LaneState& S,
+template <uint32_t NIN, bool IMAG>
+__device__ __forceinline__ void inner_walk_pivots(const Z8Dev& E, const uint32_t (&ps)[NIN], const double* s_gin, const LaneState& S,
                                                   double F, double (&acc)[4]) {
     double tr = 0.0, ti = 0.0;
     {
         const PQ_CONST double* Wv = cview(E.inner);
-        uint32_t sm[8];
+        uint32_t sm[NIN];
 #pragma unroll
-        for (int j = 0; j < 8; ++j) sm[j] = (S.hs << (31 - j)) & 0x80000000u;
+        for (int j = 0; j < (int)NIN; ++j) sm[j] = (S.hs << (31 - j)) & 0x80000000u;
         uint32_t sgn = 0;
         uint32_t addr = S.pio << 3;      // LDS byte offset of gin[pio ^ pin(gray(i))]
 #pragma unroll
-        for (int blk = 0; blk < 16; ++blk) {
+        for (int blk = 0; blk < (int)InnerGeom<NIN>::steps; ++blk) {
             double w[16];
             const PQ_CONST double* bw = Wv + 16 * blk;
             asm volatile("" : "+s"(bw) : : "memory");
@@ -861,7 +875,7 @@ __device__ __forceinline__ void inner_walk_pivots(const Z8Dev& E, const uint32_t
 }
 
 // One mid point without inner pivots, with the point's sign words sm (hs bits
-// 4..7 at bit 31) and low-slot signs slo (hs bits 0..3 as +-1.0) carried from
+// 4.. at bit 31) and low-slot signs slo (hs bits 0..3 as +-1.0) carried from
 // point to point, and the fold done in the lane's own frame.  The point's
 // phase is K_seed + D + 4 par with D the same for every lane, so its rotation
 // i^((D >> 1) & 3) and its EV / OD routing are the same for every lane too:
@@ -871,19 +885,19 @@ __device__ __forceinline__ void inner_walk_pivots(const Z8Dev& E, const uint32_t
 // its points, is applied once at the end of the lane's walk.  With PIV the
 // inner steps carry the pivot factor (inner_steps_piv, pio the point's inner
 // pivots).
-template <bool IMAG, bool PIV>
-__device__ __forceinline__ void inner_walk_mid(const InnerRegs& T, const uint32_t (&sm)[4], const double (&slo)[4], uint32_t parw,
-                                               double fm, uint32_t D, double (&accl)[4], const double* s_gin, uint32_t pio,
-                                               const uint32_t (&ps)[4]) {
+template <uint32_t NIN, bool IMAG, bool PIV>
+__device__ __forceinline__ void inner_walk_mid(const InnerRegs<NIN>& T, const uint32_t (&sm)[InnerGeom<NIN>::high], const double (&slo)[4],
+                                               uint32_t parw, double fm, uint32_t D, double (&accl)[4], const double* s_gin, uint32_t pio,
+                                               const uint32_t (&ps)[InnerGeom<NIN>::high]) {
     double a[16];
     if constexpr (PIV) {
 #pragma unroll
         for (int l = 0; l < 16; ++l) a[l] = 0.0;
-        inner_steps_piv<0>(a, T, s_gin, pio << 3, 0u, ps, sm);
+        inner_steps_piv<NIN, 0>(a, T, s_gin, pio << 3, 0u, ps, sm);
     } else {
         double m = 1.0;
         asm volatile("" : "+v"(m));
-        inner_steps<0>(a, T, m, sm);
+        inner_steps<NIN, 0>(a, T, m, sm);
     }
     double tr, ti;
     combine_signs<IMAG>(slo, a, tr, ti);
@@ -909,11 +923,15 @@ __device__ __forceinline__ void inner_walk_mid(const InnerRegs& T, const uint32_
 
 // One instantiation per inner-step variant (pivot lookups or not, imaginary
 // class pattern or not); a batch launches one persistent grid per variant
-// present, each over its own contiguous range of queries and items.
+// present, each over its own contiguous range of queries and items.  The
+// variant with pivot lookups walks kInnerPiv inner slots, the other
+// kInnerWide (the engine images are built to match).
 template <bool HASG, bool IMAG>
 __global__ __launch_bounds__(256) void k_z8_walk(const Z8Dev* __restrict__ engs, const Z8Q* __restrict__ qs, uint32_t q_lo, uint32_t q_hi,
                                                  uint64_t item_begin, uint64_t item_end, unsigned long long* __restrict__ counter,
                                                  double* __restrict__ partial) {
+    constexpr uint32_t NIN = HASG ? kInnerPiv : kInnerWide;
+    constexpr uint32_t NH = InnerGeom<NIN>::high;
     __shared__ double s_lt[8 * 256];
     __shared__ double s_gin[HASG ? 4096 : 1];
     __shared__ double s_fm[HASG ? 1 : 256];
@@ -960,20 +978,20 @@ __global__ __launch_bounds__(256) void k_z8_walk(const Z8Dev* __restrict__ engs,
         double acc[4] = {0.0, 0.0, 0.0, 0.0};
         const uint64_t lane = (item - Q.item0) * 256 + tid;
         const bool valid = (lane >> Q.H) == 0;
-        InnerRegs T;
+        InnerRegs<NIN> T;
         // the table in VGPRs: always without inner pivots, and with them when
         // the four low inner slots are pivot-free (uniform per engine)
         const bool lanewalk = !HASG || E.lowfree;
-        if (lanewalk) load_inner_regs(E, T);
+        if (lanewalk) load_inner_regs<NIN>(E, T);
         // without pivots every lane walks (the table broadcast needs the whole
         // wave); lanes past the query's 2^H repeat a valid lane's coset and
         // are zeroed below
         if (lanewalk || valid) {
             LaneState S;
-            seed_lane(E, Q, lane, S);
-            uint32_t ps[8];
+            seed_lane<NIN>(E, Q, lane, S);
+            uint32_t ps[NIN];
 #pragma unroll
-            for (int j = 0; j < 8; ++j) ps[j] = HASG ? (cview(E.slot_pin)[j] << 3) : 0u;
+            for (int j = 0; j < (int)NIN; ++j) ps[j] = HASG ? (cview(E.slot_pin)[j] << 3) : 0u;
             const uint32_t nout = 1u << Q.M;
             if (lanewalk) {
                 // The fast path (pivot-free mid slots): the lane's factor without
@@ -994,7 +1012,7 @@ __global__ __launch_bounds__(256) void k_z8_walk(const Z8Dev* __restrict__ engs,
                 }
                 // each mid slot's cross term with the lane's hi slots
                 uint32_t P = 0;
-                for (uint32_t j = 0; j < Q.M; ++j) P |= (uint32_t)(__popcll(cview(E.slot_q)[8 + j] & S.cout) & 1) << j;
+                for (uint32_t j = 0; j < Q.M; ++j) P |= (uint32_t)(__popcll(cview(E.slot_q)[NIN + j] & S.cout) & 1) << j;
                 // along the mid walk the phase step D of each flip is the same
                 // for every lane; the lane's cross term only flips the sign
                 // (parw), and the sign-mask step is a uniform byte
@@ -1002,25 +1020,27 @@ __global__ __launch_bounds__(256) void k_z8_walk(const Z8Dev* __restrict__ engs,
                 double accl[4] = {0.0, 0.0, 0.0, 0.0};
                 // the point's sign words and low-slot signs, updated in place by
                 // the uniform sign-mask step of each flip
-                uint32_t sm[4];
+                uint32_t sm[NH];
                 double slo[4];
 #pragma unroll
+                for (int k = 0; k < (int)NH; ++k) sm[k] = (S.hs << (27 - k)) & 0x80000000u;
+#pragma unroll
                 for (int k = 0; k < 4; ++k) {
-                    sm[k] = (S.hs << (27 - k)) & 0x80000000u;
                     slo[k] = (S.hs >> k) & 1u ? -1.0 : 1.0;
                     asm volatile("" : "+v"(slo[k]));
                 }
                 for (uint32_t o = 0; o < nout; ++o) {
                     if (o) {
-                        const uint32_t j = (uint32_t)__builtin_ctz(o), p = 8u + j;
+                        const uint32_t j = (uint32_t)__builtin_ctz(o), p = NIN + j;
                         const uint64_t qp = cview(E.slot_q)[p];
                         const uint32_t ell = cview(E.slot_lam)[p] + (((uint32_t)(Q.delta >> p) & 1u) << 2);
-                        D += (((cmid >> j) & 1u) ? 8u - ell : ell) + ((uint32_t)(__popcll(qp & ((uint64_t)cmid << 8)) & 1) << 2);
+                        D += (((cmid >> j) & 1u) ? 8u - ell : ell) + ((uint32_t)(__popcll(qp & ((uint64_t)cmid << NIN)) & 1) << 2);
                         parw ^= (P << (31 - j)) & 0x80000000u;
-                        const uint32_t x = (uint32_t)qp & 0xFFu;      // the sign-mask step, the same for every lane
+                        const uint32_t x = (uint32_t)qp & InnerGeom<NIN>::mask;      // the sign-mask step, the same for every lane
+#pragma unroll
+                        for (int k = 0; k < (int)NH; ++k) sm[k] ^= (x << (27 - k)) & 0x80000000u;
 #pragma unroll
                         for (int k = 0; k < 4; ++k) {
-                            sm[k] ^= (x << (27 - k)) & 0x80000000u;
                             slo[k] = flip_sign(slo[k], (x << (31 - k)) & 0x80000000u);
                             asm volatile("" : "+v"(slo[k]));
                         }
@@ -1029,17 +1049,19 @@ __global__ __launch_bounds__(256) void k_z8_walk(const Z8Dev* __restrict__ engs,
                         if (HASG) S.pio ^= cview(E.slot_pin)[p];
                     }
                     const double fm = fast ? s_fm[o] : lane_factor(s_lt, E.nbytes, S.sout);
-                    const uint32_t pshi[4] = {ps[4], ps[5], ps[6], ps[7]};
-                    inner_walk_mid<IMAG, HASG>(T, sm, slo, parw, fm, D, accl, s_gin, S.pio, pshi);
+                    uint32_t pshi[NH];
+#pragma unroll
+                    for (int k = 0; k < (int)NH; ++k) pshi[k] = ps[4 + k];
+                    inner_walk_mid<NIN, IMAG, HASG>(T, sm, slo, parw, fm, D, accl, s_gin, S.pio, pshi);
                 }
                 // undo the lane frame: omega^K_seed (EV + omega OD) times the lane factor
                 fold64(accl[0] * Fl, accl[1] * Fl, S.K, acc);
                 fold64(accl[2] * Fl, accl[3] * Fl, S.K + 1u, acc);
             } else {
                 for (uint32_t o = 0; o < nout; ++o) {
-                    if (o) flip_mid(E, Q, 8u + (uint32_t)__builtin_ctz(o), S);
+                    if (o) flip_mid<NIN>(E, Q, NIN + (uint32_t)__builtin_ctz(o), S);
                     const double F = lane_factor(s_lt, E.nbytes, S.sout);
-                    inner_walk_pivots<IMAG>(E, ps, s_gin, S, F, acc);
+                    inner_walk_pivots<NIN, IMAG>(E, ps, s_gin, S, F, acc);
                 }
             }
         }
@@ -1639,22 +1661,25 @@ __global__ __launch_bounds__(256) void k_z8_reduce(const uint64_t* __restrict__ 
 }
 
 
-// Segment split: a lane walks 2^(8+M) branches and 2^H lanes cover a query
-// (H = d-8-M).  M depends on d alone (M = clamp(d - offset, 0, max)), so a
+// Segment split: a lane walks 2^(kin+M) branches and 2^H lanes cover a query
+// (H = d-kin-M).  M depends on the engine alone (M = clamp(d - offset, 0,
+// max), at most d - kin), so a
 // query is decomposed the same way in every batch and its partial sums,
 // hence its result, are bitwise the same alone and in any mix.  The offset
 // and cap are set by the autotuner (polyq_amd.autotune); the default keeps
 // H = 19 (2048 items of 256 lanes per query) until M reaches 6.
 static std::atomic<uint32_t> g_mid_offset{27}, g_mid_max{6};
 void z8_set_tuning(uint32_t offset, uint32_t max_mid) {
-    if (offset < 8 || max_mid > 16) throw std::invalid_argument("walk tuning out of range");
+    // M = clamp(d - offset, 0, max_mid) is further capped at d - kin, so any
+    // offset up to the largest branch dimension is meaningful
+    if (offset > 64 || max_mid > 16) throw std::invalid_argument("walk tuning out of range");
     g_mid_offset = offset; g_mid_max = max_mid;
 }
 void z8_get_tuning(uint32_t* offset, uint32_t* max_mid) { *offset = g_mid_offset; *max_mid = g_mid_max; }
-uint32_t z8_auto_mid_bits(uint32_t d, uint64_t /*batch_branches*/) {
+uint32_t z8_auto_mid_bits(uint32_t d, uint32_t kin) {
     const uint32_t off = g_mid_offset, mx = g_mid_max;
     if (d <= off) return 0;
-    return std::min<uint32_t>(std::min<uint32_t>(d - off, mx), d > 8 ? d - 8 : 0);
+    return std::min<uint32_t>(std::min<uint32_t>(d - off, mx), d > kin ? d - kin : 0);
 }
 // Segment walk: a lane walks 2^M outer points (each a 2^I-branch table
 // entry) with M = clamp(nout - offset, 0, max), capped by the walkable slots
@@ -1667,12 +1692,12 @@ void seg_set_tuning(uint32_t offset, uint32_t max_mid) {
 void seg_get_tuning(uint32_t* offset, uint32_t* max_mid) { *offset = g_seg_offset; *max_mid = g_seg_max; }
 uint32_t dev_mid_bits(const DevEngine& D, int walk) {
     if (walk == kWalkSeg) return seg_mid_bits(*D.seg, g_seg_offset, g_seg_max);
-    return z8_auto_mid_bits(D.z8->d, 0);
+    return z8_auto_mid_bits(D.z8->d, D.z8->kin);
 }
 
 namespace {
 constexpr int kVariants = 4;   // branch walk: bit 1 inner pivot lookups, bit 0 imaginary class pattern
-int variant_of(const Z8Image& Z) { return (Z.npin ? 2 : 0) | (Z.imag0 ? 1 : 0); }
+int variant_of(const Z8Image& Z) { return (Z.kin == kInnerPiv ? 2 : 0) | (Z.imag0 ? 1 : 0); }
 // What one batch uploads: engine descriptors, 64-byte query records, scales
 // and the output order, plus the item ranges of the launches.
 struct BatchPlan {
@@ -1719,7 +1744,7 @@ BatchPlan plan_branch(const std::vector<Z8Job>& jobs) {
         else e = (uint32_t)(it - seen.begin());
         Z8Q q{};
         q.delta = J.q.delta; q.s0 = J.q.s0; q.eng = e; q.K0 = J.q.K0; q.pio0 = J.q.pio0;
-        const uint32_t nout = Z.nslots - 8;
+        const uint32_t nout = Z.nslots - Z.kin;
         q.M = std::min(J.mid_bits, nout);
         q.H = nout - q.M;
         if (q.H > 40) throw std::invalid_argument("branch dimension too large for one batch");
@@ -2377,6 +2402,7 @@ void dev_dump_s(const DevEngine& D, const Prepared& P, const uint64_t* s, size_t
 }
 
 // Every branch of the walk for one query, with the walk's own lane state.
+template <uint32_t NIN>
 __global__ __launch_bounds__(256) void k_dump_walk(Z8Dev E, Z8Q Q, uint8_t* __restrict__ kout, float* __restrict__ mout) {
     __shared__ double s_lt[8 * 256];
     __shared__ double s_gin[4096];
@@ -2387,16 +2413,16 @@ __global__ __launch_bounds__(256) void k_dump_walk(Z8Dev E, Z8Q Q, uint8_t* __re
     const uint64_t lane = (uint64_t)blockIdx.x * 256 + tid;
     if ((lane >> Q.H) != 0) return;
     LaneState S;
-    seed_lane(E, Q, lane, S);
+    seed_lane<NIN>(E, Q, lane, S);
     const uint32_t nout = 1u << Q.M;
     for (uint32_t o = 0; o < nout; ++o) {
-        if (o) flip_mid(E, Q, 8u + (uint32_t)__builtin_ctz(o), S);
+        if (o) flip_mid<NIN>(E, Q, NIN + (uint32_t)__builtin_ctz(o), S);
         const double F = lane_factor(s_lt, E.nbytes, S.sout);
-        for (uint32_t i = 0; i < 256; ++i) {
+        for (uint32_t i = 0; i < InnerGeom<NIN>::N; ++i) {
             const uint32_t g = i ^ (i >> 1);
             const double wv = E.inner[i];
-            const uint32_t pin8 = E.inaux[i], kin = E.inaux[256 + i];
-            const uint64_t idx = (((lane << Q.M) + o) << 8) + i;
+            const uint32_t pin8 = E.inaux[i], kin = E.inaux[InnerGeom<NIN>::N + i];
+            const uint64_t idx = (((lane << Q.M) + o) << NIN) + i;
             if (wv == 0.0) { kout[idx] = 255; mout[idx] = 0.f; continue; }
             const uint32_t par = (uint32_t)__popc(g & S.hs) & 1u;
             const uint32_t k = (S.K + kin + 4u * par) & 7u;
@@ -2465,15 +2491,17 @@ void dev_dump_walk(const DevEngine& D, const Z8QueryHost& qh, uint32_t mid_bits,
     const Z8Image& Z = *D.z8;
     Z8Q q{};
     q.delta = qh.delta; q.s0 = qh.s0; q.K0 = qh.K0; q.pio0 = qh.pio0;
-    const uint32_t nout = Z.nslots - 8;
+    const uint32_t nout = Z.nslots - Z.kin;
     q.M = std::min(mid_bits, nout); q.H = nout - q.M;
-    if (8 + q.M + q.H > 26) throw std::invalid_argument("dump_walk is limited to 2^26 branches");
-    const uint64_t n = 1ull << (8 + q.M + q.H);
+    if (Z.kin + q.M + q.H > 26) throw std::invalid_argument("dump_walk is limited to 2^26 branches");
+    const uint64_t n = 1ull << (Z.kin + q.M + q.H);
     std::lock_guard<std::mutex> lk(C.mu);
     uint8_t* dk = (uint8_t*)C.d_misc.get(n + 256);
     float* dm = (float*)C.d_misc2.get(n * 4 + 256);
     const uint64_t lanes = 1ull << q.H;
-    hipLaunchKernelGGL(k_dump_walk, dim3((uint32_t)((lanes + 255) / 256)), dim3(256), 0, C.stream, D.z, q, dk, dm);
+    if (Z.kin == kInnerPiv) hipLaunchKernelGGL(k_dump_walk<kInnerPiv>, dim3((uint32_t)((lanes + 255) / 256)), dim3(256), 0, C.stream, D.z, q, dk, dm);
+    else if (Z.kin == kInnerWide) hipLaunchKernelGGL(k_dump_walk<kInnerWide>, dim3((uint32_t)((lanes + 255) / 256)), dim3(256), 0, C.stream, D.z, q, dk, dm);
+    else throw std::logic_error("walk image with an unknown inner slot count");
     HIPCHECK(hipGetLastError());
     HIPCHECK(hipMemcpyAsync(k, dk, n, hipMemcpyDeviceToHost, C.stream));
     HIPCHECK(hipMemcpyAsync(mag, dm, n * 4, hipMemcpyDeviceToHost, C.stream));

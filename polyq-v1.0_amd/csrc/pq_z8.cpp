@@ -5,7 +5,7 @@
 //     E B omega^{k(c)} prod_k r_k[s(c)_k]
 // with k(c) the Z8 quadratic form built in pq_engine.cpp.  The product over
 // terms splits into three parts, each handled where it varies:
-//   inner free terms      -> folded into the uniform 256-entry table W[i]
+//   inner free terms      -> folded into the uniform 2^kin-entry table W[i]
 //   pivots touched by an  -> per-branch LDS lookup gin[pio ^ pin_i]
 //   inner direction
 //   every other term      -> lane factor, 8-bit byte tables, once per outer step
@@ -16,23 +16,25 @@
 
 namespace pq {
 
-Z8Image build_z8_image(const HostEngine& E) {
+// The image with K inner slots.
+static Z8Image build_z8_slots(const HostEngine& E, const uint32_t K) {
     Z8Image Z;
     const uint32_t d = E.d, t = E.tdev, W = E.L.words();   // device terms (see HostEngine::tdev)
     if (d > 56) throw std::invalid_argument("branch dimension above 56 is not supported by the Z8 walk");
-    Z.d = d; Z.t = t;
+    Z.d = d; Z.t = t; Z.kin = K;
+    const uint32_t NI = 1u << K;
     std::vector<int32_t> piv_index(t, -1);
     for (uint32_t i = 0; i < E.rkdev; ++i) piv_index[E.pivot_col[i]] = (int32_t)i;
     auto pivots_of = [&](uint64_t N) { uint64_t pm = 0; for (uint32_t k = 0; k < t; ++k) if (((N >> k) & 1) && piv_index[k] >= 0) pm |= 1ull << piv_index[k]; return pm; };
 
-    // Choose up to 8 inner directions: pivot-free kernel vectors first, then
+    // Choose up to K inner directions: pivot-free kernel vectors first, then
     // the ones adding the fewest new pivots while at most 12 pivots are
     // touched (the FP64 gin table stays within 32 KB of LDS).
     std::vector<uint8_t> used(d, 0);
     std::vector<uint32_t> inner;
     uint64_t pin_set = 0;
-    for (uint32_t j = 0; j < d && inner.size() < kInner; ++j) if (pivots_of(E.kernel[j]) == 0) { inner.push_back(j); used[j] = 1; }
-    while (inner.size() < kInner) {
+    for (uint32_t j = 0; j < d && inner.size() < K; ++j) if (pivots_of(E.kernel[j]) == 0) { inner.push_back(j); used[j] = 1; }
+    while (inner.size() < K) {
         int best = -1; uint32_t bestc = 99;
         for (uint32_t j = 0; j < d; ++j) {
             if (used[j]) continue;
@@ -65,7 +67,7 @@ Z8Image build_z8_image(const HostEngine& E) {
         std::swap(N[0], N[p1]); std::swap(U[0], U[p1]);
         Z.imag0 = true;
     }
-    while (N.size() < kInner) { N.push_back(0); U.emplace_back(W, 0ull); }
+    while (N.size() < K) { N.push_back(0); U.emplace_back(W, 0ull); }
     // outer slots: pivot-free kernel vectors first (each is one free term)
     for (int pass = 0; pass < 2; ++pass)
         for (uint32_t j = 0; j < d; ++j) {
@@ -77,7 +79,7 @@ Z8Image build_z8_image(const HostEngine& E) {
     if (Z.nslots > 64) throw std::invalid_argument("too many slots");
     const uint32_t S = Z.nslots;
     Z.slot_real.assign(S, 0);
-    for (uint32_t p = 0; p < S; ++p) Z.slot_real[p] = (p < Z.nin || p >= kInner) ? 1 : 0;
+    for (uint32_t p = 0; p < S; ++p) Z.slot_real[p] = (p < Z.nin || p >= K) ? 1 : 0;
 
     // compact numbering of the pivots touched by inner directions
     uint64_t touched = 0;
@@ -115,9 +117,9 @@ Z8Image build_z8_image(const HostEngine& E) {
     Z.lane_terms = all & ~excl;
 
     // Inner table: entry i covers c_in = gray(i); phantom bits give zero.
-    Z.inner.assign(256, 0.0);
-    Z.inner_aux.assign(512, 0u);
-    for (uint32_t i = 0; i < 256; ++i) {
+    Z.inner.assign(NI, 0.0);
+    Z.inner_aux.assign(2 * NI, 0u);
+    for (uint32_t i = 0; i < NI; ++i) {
         const uint32_t c = i ^ (i >> 1);
         uint32_t pin = 0, kin = 0;
         uint64_t sin = 0;
@@ -138,23 +140,23 @@ Z8Image build_z8_image(const HostEngine& E) {
             Z.inner[i] = (kin & 4) ? -fin : fin;
         }
         Z.inner_aux[i] = pin << 3;
-        Z.inner_aux[256 + i] = kin;
+        Z.inner_aux[NI + i] = kin;
     }
-    // Accumulator-walk order: c_in = (gray4(q) << 4) | l at 16 q + l; entry i
+    // Accumulator-walk order: c_in = (gray(q) << 4) | l at 16 q + l; entry i
     // of the Gray order covers gray(i), so c_in sits at i = gray^-1(c_in).
-    Z.inner_acc.assign(256, 0.0);
-    for (uint32_t q = 0; q < 16; ++q)
+    Z.inner_acc.assign(NI, 0.0);
+    for (uint32_t q = 0; q < (NI >> 4); ++q)
         for (uint32_t l = 0; l < 16; ++l) {
             uint32_t c = ((q ^ (q >> 1)) << 4) | l, i = c;
-            i ^= i >> 1; i ^= i >> 2; i ^= i >> 4;
+            for (uint32_t sh = 1; sh < 32; sh <<= 1) i ^= i >> sh;
             Z.inner_acc[16 * q + l] = Z.inner[i];
         }
     // Lane-factor byte tables, with every lane term and without the free
-    // terms of the pivot-free outer slots kInner .. kInner + nx - 1
+    // terms of the pivot-free outer slots K .. K + nx - 1
     uint64_t mid_terms = 0;
     for (uint32_t j = 0; j < Z.nx; ++j) {
         const uint32_t k = Z.mid_term[j];
-        if (Z.slot_n[kInner + j] != 1ull << k) throw std::logic_error("pivot-free outer slot is not its free term alone");
+        if (Z.slot_n[K + j] != 1ull << k) throw std::logic_error("pivot-free outer slot is not its free term alone");
         mid_terms |= 1ull << k;
         Z.mid_r.push_back(E.r0[k]);
         Z.mid_r.push_back(E.r1[k]);
@@ -183,6 +185,16 @@ Z8Image build_z8_image(const HostEngine& E) {
         Z.gin[pi] = f;
     }
     return Z;
+}
+
+// kInnerWide inner slots when no inner direction touches a pivot (the plain
+// walk variant); otherwise kInnerPiv, walked by the variant with pivot
+// lookups, whose pivot state leaves no registers for the larger table (the
+// kInnerPiv image may then turn out pivot-free: that variant walks it too).
+Z8Image build_z8_image(const HostEngine& E) {
+    Z8Image Z = build_z8_slots(E, kInnerWide);
+    if (Z.npin == 0) return Z;
+    return build_z8_slots(E, kInnerPiv);
 }
 
 Z8QueryHost z8_query(const HostEngine& E, const Z8Image& Z, const Prepared& P) {

@@ -421,10 +421,10 @@ class Simulator:
         return tuple(int(v) for v in o)
 
     def z8_info(self):
-        o = np.zeros(9, dtype=np.uint32)
+        o = np.zeros(10, dtype=np.uint32)
         _check(lib.pq_sim_z8_info(self._h, _ptr(o, _u32p)))
         return dict(zip(("d", "inner", "slots", "inner_pivots", "rank_k", "imag_slot0", "ops_per_branch", "pivot_free_mid",
-                         "low_slots_pivot_free"), (int(v) for v in o)))
+                         "low_slots_pivot_free", "inner_slots"), (int(v) for v in o)))
 
     def seg_info(self):
         o = np.zeros(8, dtype=np.uint64)
@@ -631,13 +631,15 @@ _TUNE_GRID = {"z8": ((8, 10, 11, 12, 13, 14, 16), (0, 2, 4, 6, 8)),
               "branch_walk": ((23, 24, 25, 26, 27, 28, 29), (4, 6, 8))}
 
 
-def branch_walk_mid_bits(d, offset, max_mid):
+def branch_walk_mid_bits(d, offset, max_mid, inner_slots):
     """Mid coordinates a lane walks in the per-branch walk for an engine of
-    branch dimension d at lane walk tuning (offset, max_mid): 2^(8 + M)
-    branches per lane (the library's own rule, z8_auto_mid_bits)."""
+    branch dimension d at lane walk tuning (offset, max_mid), with
+    inner_slots inner coordinates per outer point (z8_info()["inner_slots"]):
+    2^(inner_slots + M) branches per lane (the library's own rule,
+    z8_auto_mid_bits)."""
     if d <= offset:
         return 0
-    return min(d - offset, max_mid, d - 8 if d > 8 else 0)
+    return min(d - offset, max_mid, d - inner_slots if d > inner_slots else 0)
 
 
 def autotune(sims, inputs, outputs, form="z8", offsets=None, max_mids=None, repeats=3, cache=True, objective="wall", rounds=3):
@@ -666,7 +668,7 @@ def autotune(sims, inputs, outputs, form="z8", offsets=None, max_mids=None, repe
             if form == "z8":
                 key = tuple(s.seg_info()["mid_bits"] for s in distinct)
             elif all(s.stats()[6] == 0 for s in distinct):      # one branch engine per simulator
-                key = tuple(branch_walk_mid_bits(s.z8_info()["d"], off, mx) for s in distinct)
+                key = tuple(branch_walk_mid_bits(s.z8_info()["d"], off, mx, s.z8_info()["inner_slots"]) for s in distinct)
             else:
                 key = (off, mx)
             configs.setdefault(key, []).append((off, mx))

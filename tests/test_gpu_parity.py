@@ -188,7 +188,7 @@ def test_branch_walk_paths_and_walk_lengths_agree():
             if zi["inner_pivots"]:
                 paths["pivots_lane_walk" if zi["low_slots_pivot_free"] else "pivots_per_point"] += 1
             for off, mx in ((8, 6), (8, 1)):
-                m = pq.branch_walk_mid_bits(d, off, mx)
+                m = pq.branch_walk_mid_bits(d, off, mx, zi["inner_slots"])
                 if m:     # pivot-free mid slots take the kernel's fast path
                     paths["fast" if zi["inner_pivots"] == 0 and m <= zi["pivot_free_mid"] else "generic"] += 1
             xs = [rng.below(1 << 10) for _ in range(6)]
@@ -206,7 +206,7 @@ def test_branch_walk_paths_and_walk_lengths_agree():
                 o = O.OSim(O.OCircuit.from_circuit(c), "cached")
                 ref = np.array([o.amplitude(x, y) for x, y in zip(xs, ys)])
                 assert np.max(np.abs(res[(8, 6)] - ref)) <= BW_REL * max(np.max(np.abs(ref)), 1e-300)
-            cases += d >= 9
+            cases += d > zi["inner_slots"]
         assert cases >= 6
         print("mid walks by kernel path:", paths)
         assert paths["fast"] >= 2 and paths["generic"] >= 2

@@ -51,8 +51,8 @@ def test_noalloc_needs_resident_engines_and_bounds_the_batch():
     with pytest.raises(pq.PolyqError, match="not resident"):
         _call(lib.pq_sim_amplitudes_noalloc, sim, xs[:1], ys[:1], "z8")
     _call(lib.pq_sim_amplitudes, sim, xs[:1], ys[:1], "z8")
-    # 8192 queries of this engine (d = 24, no mid walk: 256 items of 256
-    # lanes each) need 64 MiB of item partials
+    # 8192 queries of this engine (d = 24, 10 inner slots, no mid walk: 64
+    # items of 256 lanes each) need 16 MiB of item partials
     rng = W.Rng(10)
     n = 8192
     bx = [rng.below(1 << 40) for _ in range(n)]
