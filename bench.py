@@ -112,8 +112,9 @@ SEG_TRAFFIC = {("headline", 16, 2): 1.84e6, ("saturated", 16, 6): 5.37e6}
 # HBM bytes per branch-walk launch on the headline step, keyed by the mid
 # coordinates M a lane walks (FETCH_SIZE x 2 + WRITE_SIZE, copy-calibrated;
 # profiles/k_z8_walk_config2.md): the item partials (4 doubles per 256-lane
-# item, 2^(22 - M) items per amplitude) and the tables once per XCD.
-BRANCH_TRAFFIC = {1: 8.66e6, 2: 4.47e6, 3: 2.37e6, 4: 1.32e6, 5: 0.80e6, 6: 0.54e6, 7: 0.41e6}
+# item, 2^(12 - M) items per amplitude with 10 inner slots) and the tables
+# once per XCD.
+BRANCH_TRAFFIC = {3: 0.93e6, 4: 0.67e6, 5: 0.54e6, 6: 0.47e6, 7: 0.44e6}
 PEAK_CLOCK_HZ = 2.4e9       # MI355X peak engine clock
 
 
