@@ -104,11 +104,11 @@ SEG_OPS_PER_POINT = 30.0
 # and the double-float arithmetic (walk magnitude times the complex table
 # entry: 2 x 4; compensated complex accumulation: 2 x 8).  The lane seed, the
 # lane factor and the reductions are overhead, not counted.
-# HBM bytes per walk launch, keyed by launch shape, table bits and lane walk
-# length (FETCH_SIZE x 2 + WRITE_SIZE, copy-calibrated;
+# HBM bytes per walk launch, keyed by launch shape, table bits, lane walk
+# length and waves per workgroup (FETCH_SIZE x 2 + WRITE_SIZE, copy-calibrated;
 # profiles/k_seg_walk_config2.md): the table once per XCD L2 and the item
 # partials.
-SEG_TRAFFIC = {("headline", 16, 2): 1.84e6, ("saturated", 16, 6): 5.37e6}
+SEG_TRAFFIC = {("headline", 16, 2, 4): 1.84e6, ("saturated", 16, 6, 4): 5.37e6}
 # HBM bytes per branch-walk launch on the headline step, keyed by the mid
 # coordinates M a lane walks (FETCH_SIZE x 2 + WRITE_SIZE, copy-calibrated;
 # profiles/k_z8_walk_config2.md): the item partials (4 doubles per 256-lane
@@ -157,6 +157,15 @@ def _s8d_ops(st):
     4 W_t + 14 with W = 32-bit words of planes, lines and terms."""
     w = lambda n: (n + 31) // 32
     return 4 * w(st[1]) + 4 * w(st[2]) + 4 * w(st[4]) + 14
+
+
+def _apply_tuning(pq, text):
+    """Apply "offset,max_mid[,waves]" to the segment walk; returns (offset, max_mid)."""
+    v = [int(t) for t in text.split(",")]
+    pq.set_walk_tuning(v[0], v[1])
+    if len(v) > 2:
+        pq.set_walk_shape(v[2])
+    return v[0], v[1]
 
 
 def _timed_kernel_ms(pq, np, fn, reps):
@@ -235,10 +244,10 @@ def headline(args, pq, W, np, probe):
     seg = sim.seg_info()
     points_per_step = nq * (1 << seg["outer"])
     if args.segment_tuning:
-        pq.set_walk_tuning(*(int(v) for v in args.segment_tuning.split(",")))
-        stune = tuple(int(v) for v in args.segment_tuning.split(","))
+        stune = _apply_tuning(pq, args.segment_tuning)
     else:
         stune = pq.autotune([sim] * nq, xs, ys, repeats=7, rounds=5, cache=False)[:2]
+    swaves = pq.get_walk_shape()
     for _ in range(args.warmup):
         sim.amplitudes(xs, ys, form="z8")
     ts = time.perf_counter()
@@ -248,13 +257,13 @@ def headline(args, pq, W, np, probe):
     z_ms, zagain = _timed_kernel_ms(pq, np, lambda: sim.amplitudes(xs, ys, form="z8"), min(args.steps, 50))
     assert np.array_equal(zagain, zgot)
     seg = sim.seg_info()            # with the tuned lane walk length
-    zkern = "k_seg_walk_args<%s>" % ("true" if seg["mid_bits"] <= 4 else "false")
+    zkern = "k_seg_walk_args<%s, %d>" % ("true" if seg["mid_bits"] <= 4 else "false", swaves)
     res["segment"] = {
         "form": "z8 (default: per-engine FP64 Walsh-Hadamard tables, outer Gray walk, double-float lanes)",
         "amplitudes_per_s": nq * args.steps / z_elapsed, "ms_per_step": 1e3 * z_elapsed / args.steps,
-        "walk_tuning": list(stune), "segment": seg, "got": zgot,
+        "walk_tuning": list(stune), "workgroup_waves": swaves, "segment": seg, "got": zgot,
         "roofline": _seg_roofline(probe, points_per_step, z_ms, kernel=zkern,
-                                  traffic=SEG_TRAFFIC.get(("headline", seg["inner"], seg["mid_bits"]))),
+                                  traffic=SEG_TRAFFIC.get(("headline", seg["inner"], seg["mid_bits"], swaves))),
     }
 
     if not args.no_plain:
@@ -332,13 +341,13 @@ def saturated(args, pq, W, np, probe, sim=None):
     rng = W.Rng(99)
     big_x = [rng.below(1 << 40) for _ in range(4096)]
     big_y = [0] * 4096
-    default = pq.get_walk_tuning()
+    default, default_waves = pq.get_walk_tuning(), pq.get_walk_shape()
     S = [sim] * 4096
     if args.saturated_tuning:
-        off, mx = (int(v) for v in args.saturated_tuning.split(","))
-        pq.set_walk_tuning(off, mx)
+        off, mx = _apply_tuning(pq, args.saturated_tuning)
     else:
         off, mx, table = pq.autotune(S, big_x, big_y, repeats=2, cache=False, objective="kernel")
+    waves = pq.get_walk_shape()
     bms, bwall = [], []
     for _ in range(args.saturated_reps):
         tb = time.perf_counter()
@@ -350,11 +359,13 @@ def saturated(args, pq, W, np, probe, sim=None):
             bms.append(pq.last_timing()["kernel_ms"])
     tuned = sim.seg_info()["mid_bits"]
     pq.set_walk_tuning(*default)
+    pq.set_walk_shape(default_waves)
     return _seg_roofline(probe, 4096 * (1 << seg["outer"]), float(np.median(bms)),
-                         {"queries": 4096, "walk_tuning": [off, mx], "mid_bits": tuned,
+                         {"queries": 4096, "walk_tuning": [off, mx], "workgroup_waves": waves, "mid_bits": tuned,
                           "amplitudes_per_s": 4096 / float(np.median(bwall)),
                           "branches_per_s": 4096 * (1 << d) / float(np.median(bwall))},
-                         traffic=SEG_TRAFFIC.get(("saturated", seg["inner"], tuned)), kernel="k_seg_walk<false>")
+                         traffic=SEG_TRAFFIC.get(("saturated", seg["inner"], tuned, waves)),
+                         kernel="k_seg_walk<%s, %d>" % ("true" if tuned <= 4 else "false", waves))
 
 
 def precision(args, pq, W, np):
@@ -420,7 +431,8 @@ def batch(args, pq, W, np):
         "workload": "%d circuits (n ~ U[20,56], T ~ U[8,32], 7.5n CX/CZ, 2.5n S, seed 1) x 16 (x, y)" % len(work),
         "queries": len(S), "branch_dim_min": min(ds), "branch_dim_max": max(ds), "branches": branches,
         "generate_s": tb - tg, "host_build_s": te - tb, "host_build_ms_per_circuit": 1e3 * (te - tb) / len(work),
-        "tuning_default": list(default), "tuning_chosen": [off, mx], "autotune_kernel_ms": table,
+        "tuning_default": list(default), "tuning_chosen": [off, mx], "workgroup_waves": pq.get_walk_shape(),
+        "autotune_kernel_ms": table,
         "eager_ms_median": float(np.median(eager_ms)), "eager_kernel_ms": tm["kernel_ms"], "items": tm["items"],
         "graph_ms_median": float(np.median(gms)), "graph_wall_ms_median": float(np.median(wall)),
         "graph_replays_equal_eager": same,
@@ -587,12 +599,13 @@ def main():
     ap.add_argument("--sweep-seconds", type=float, default=5.0)
     ap.add_argument("--batch-circuits", type=int, default=256)
     ap.add_argument("--saturated-reps", type=int, default=5)
-    ap.add_argument("--saturated-tuning", default=None, help="offset,max_mid for the saturating launch instead of autotuning")
+    ap.add_argument("--saturated-tuning", default=None,
+                    help="offset,max_mid[,waves] for the saturating launch instead of autotuning")
     ap.add_argument("--precision-out", default=None)
     ap.add_argument("--headline-tuning", default="",
                     help="offset,max_mid of the headline (branch walk) lane walk instead of autotuning it (profiling runs)")
     ap.add_argument("--segment-tuning", default="",
-                    help="offset,max_mid of the segment walk on the headline step instead of autotuning it")
+                    help="offset,max_mid[,waves] of the segment walk on the headline step instead of autotuning it")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run three 256 MiB device copies (reference traffic for FETCH_SIZE/WRITE_SIZE counters)")
     args = ap.parse_args()

@@ -202,10 +202,17 @@ int pq_best_pairs(const uint64_t st6[6], uint32_t avail, uint32_t tclass, int gp
 /* Lane walk length of a batched form (no counterpart in the reference).  A
  * lane walks 2^M outer Gray points, M = clamp(n - offset, 0, max_mid), with n
  * the outer coordinates of the query (PQ_FORM_Z8: d - I; PQ_FORM_BRANCH_WALK:
- * d - 8).  Set by the autotuner; a fixed setting makes every result
- * independent of the batch it runs in. */
+ * d minus the engine's inner slots, pq_sim_z8_info).  Set by the autotuner; a
+ * fixed setting makes every result independent of the batch it runs in. */
 int pq_set_walk_tuning(int form, uint32_t offset, uint32_t max_mid);
 int pq_get_walk_tuning(int form, uint32_t out2[2]);
+/* Workgroup shape of a batched form (no counterpart in the reference): waves
+ * of 64 lanes per workgroup, and so lanes per work item.  PQ_FORM_Z8 takes 1,
+ * 2 or 4 (default 4); PQ_FORM_BRANCH_WALK only 4 (its register-bound
+ * occupancy is 2 workgroups of 4 waves per CU).  Global like the walk
+ * length, and set by the autotuner with it. */
+int pq_set_walk_shape(int form, uint32_t waves);
+int pq_get_walk_shape(int form, uint32_t* waves);
 /* Cap on the table bits I + npin of the segment tables of engines built
  * afterwards (default 16).  An engine of branch dimension d gets
  * min(cap, max(14, d - 14)) bits: 2^bits entries x 16 B, so 256 KB at 14 bits

@@ -666,6 +666,19 @@ int pq_get_walk_tuning(int form, uint32_t* o) {
         return 0;
     })
 }
+int pq_set_walk_shape(int form, uint32_t waves) {
+    PQ_TRY({
+        if (walk_of_form(form) == kWalkSeg) seg_set_waves(waves);
+        else if (waves != 4) throw std::invalid_argument("the branch walk runs workgroups of 4 waves");
+        return 0;
+    })
+}
+int pq_get_walk_shape(int form, uint32_t* waves) {
+    PQ_TRY({
+        *waves = walk_of_form(form) == kWalkSeg ? seg_get_waves() : 4u;
+        return 0;
+    })
+}
 int pq_set_segment_bits(uint32_t bits) { PQ_TRY({ seg_set_bits(bits); return 0; }) }
 uint32_t pq_get_segment_bits(void) { return seg_get_bits(); }
 

@@ -72,6 +72,8 @@ void z8_set_tuning(uint32_t offset, uint32_t max_mid);
 void z8_get_tuning(uint32_t* offset, uint32_t* max_mid);
 void seg_set_tuning(uint32_t offset, uint32_t max_mid);
 void seg_get_tuning(uint32_t* offset, uint32_t* max_mid);
+void seg_set_waves(uint32_t waves);     // segment walk workgroup shape: 1, 2 or 4 waves
+uint32_t seg_get_waves();
 // Lane walk length of an engine's queries for the given form.
 uint32_t dev_mid_bits(const DevEngine& D, int walk);
 

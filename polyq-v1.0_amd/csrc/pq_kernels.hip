@@ -1249,7 +1249,7 @@ __device__ __forceinline__ void seg_fold4(const float (&A)[4][4], float (&dst)[4
     df_add(dst[2], dst[3], A[P + 2][0], A[P + 2][1]);
 }
 
-// Persistent work-queue kernel over (query, 256-lane block) items.  A lane
+// Persistent work-queue kernel over (query, block of 64 WAVES lanes) items.  A lane
 // seeds its state from its coset index (integer phase, sign mask, pivots,
 // lane factor), then covers the 2^M outer points of its segment.  The point
 // records come grouped by engine phase class Ue, so each group adds into its
@@ -1323,11 +1323,13 @@ __device__ __forceinline__ void dpp_add4(double (&v)[4]) {
 }
 // Workgroup reduction of the four sums in FP64: a lane's double-float pair
 // hi + lo is exact in FP64; each wave adds over DPP in the fixed order of
-// wave_sum_dpp (total in lane 63), then the four waves through LDS.  The
+// wave_sum_dpp (total in lane 63), then the WAVES waves through LDS (a
+// one-wave workgroup reads lane 63 directly: no LDS, no barrier).  The
 // item's four partial sums (EV re, im, OD re, im), one per lane of lanes
 // 0..3: a single 32-byte write, into pinned host memory for small batches
 // (the host finishes each query as its items arrive) or device memory before
 // k_z8_reduce.  Each word is one aligned 8-byte store.
+template <uint32_t WAVES>
 __device__ __forceinline__ void seg_publish(double (*s_red)[4], const float (&ev)[4], const float (&od)[4], uint64_t item,
                                             double* __restrict__ partial) {
     const uint32_t tid = threadIdx.x;
@@ -1338,15 +1340,27 @@ __device__ __forceinline__ void seg_publish(double (*s_red)[4], const float (&ev
     dpp_add4<0x140, 0xF>(wv);    // row_mirror
     dpp_add4<0x142, 0xA>(wv);    // row_bcast:15
     dpp_add4<0x143, 0xC>(wv);    // row_bcast:31
-    if ((tid & 63) == 63) {
-#pragma unroll
-        for (int c = 0; c < 4; ++c) s_red[tid >> 6][c] = wv[c];
-    }
-    __syncthreads();
-    if (tid < 4) {
+    if constexpr (WAVES == 1) {
         double v = 0.0;
-        for (int w = 0; w < 4; ++w) v += s_red[w][tid];
-        partial[4 * item + tid] = v;
+#pragma unroll
+        for (uint32_t c = 0; c < 4; ++c) {
+            const uint64_t u = (uint64_t)__double_as_longlong(wv[c]);
+            const uint32_t lo = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)u, 63);
+            const uint32_t hi = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(u >> 32), 63);
+            if (tid == c) v = __longlong_as_double((long long)(((uint64_t)hi << 32) | lo));
+        }
+        if (tid < 4) partial[4 * item + tid] = v;
+    } else {
+        if ((tid & 63) == 63) {
+#pragma unroll
+            for (int c = 0; c < 4; ++c) s_red[tid >> 6][c] = wv[c];
+        }
+        __syncthreads();
+        if (tid < 4) {
+            double v = 0.0;
+            for (uint32_t w = 0; w < WAVES; ++w) v += s_red[w][tid];
+            partial[4 * item + tid] = v;
+        }
     }
 }
 // XOR and sum of a value over the 64 lanes of the wave, broadcast to all
@@ -1387,16 +1401,17 @@ __device__ __forceinline__ uint32_t wave_add(uint32_t v) {
 //   hs, pio, s: the query's value XOR the lane-low entry XOR the c_hi slots.
 // The results equal the slot-by-slot seed bit for bit (integers mod 8 and
 // XORs), and the magnitudes multiply in the same slot order.
-template <uint32_t MW>
+template <uint32_t MW, uint32_t WAVES>
 __device__ __forceinline__ void seg_short(double (*s_red)[4], const SegDev& E, const SegQ& Q, uint64_t item, double* __restrict__ partial) {
     constexpr uint32_t NP = 1u << MW;
+    constexpr uint32_t WLOG = WAVES == 4 ? 2 : WAVES == 2 ? 1 : 0;
     const uint32_t tid = threadIdx.x;
     const uint32_t L6 = E.L6, nout = E.nout, I = E.I, nbytes = E.nbytes, Mcap = E.Mcap;
-    const uint64_t lane = (item - Q.item0) * 256 + tid;
+    const uint64_t lane = (item - Q.item0) * (64 * WAVES) + tid;
     const bool live = (lane >> Q.H) == 0;
     const uint32_t lowm = (1u << L6) - 1u;
     const uint32_t cl = tid & lowm;
-    const uint64_t chi = L6 == 6 ? (((item - Q.item0) << 2) | (uint64_t)__builtin_amdgcn_readfirstlane(tid >> 6)) : 0ull;
+    const uint64_t chi = L6 == 6 ? (((item - Q.item0) << WLOG) | (uint64_t)__builtin_amdgcn_readfirstlane(tid >> 6)) : 0ull;
     const uint64_t cplace = chi << (L6 + MW);
     // ---- loads, none behind a branch (indices clamped into the arrays)
     const uint32_t* lw = E.lowtab + 6 * cl;
@@ -1493,24 +1508,25 @@ __device__ __forceinline__ void seg_short(double (*s_red)[4], const SegDev& E, c
         }
         seg_lane_total(eu, ou, K, fh, fl, ev, od);
     }
-    seg_publish(s_red, ev, od, item, partial);
+    seg_publish<WAVES>(s_red, ev, od, item, partial);
 }
 
-// One item (256 lanes of one query): reload the engine tables into LDS if
+// One item (64 WAVES lanes of one query): reload the engine tables into LDS if
 // the workgroup's last item was of another engine (eng, uniform), seed every
 // lane, walk its segment, and write the item's four partial sums.
-template <bool SHORT>
+template <bool SHORT, uint32_t WAVES>
 __device__ __forceinline__ void seg_item(const SegLds& S, const SegDev& E, const SegQ& Q, uint64_t item, int& eng, double* __restrict__ partial) {
     if constexpr (SHORT) {
         switch (Q.M) {
-        case 0: seg_short<0>(S.red, E, Q, item, partial); break;
-        case 1: seg_short<1>(S.red, E, Q, item, partial); break;
-        case 2: seg_short<2>(S.red, E, Q, item, partial); break;
-        case 3: seg_short<3>(S.red, E, Q, item, partial); break;
-        default: seg_short<kSegShortWalk>(S.red, E, Q, item, partial); break;
+        case 0: seg_short<0, WAVES>(S.red, E, Q, item, partial); break;
+        case 1: seg_short<1, WAVES>(S.red, E, Q, item, partial); break;
+        case 2: seg_short<2, WAVES>(S.red, E, Q, item, partial); break;
+        case 3: seg_short<3, WAVES>(S.red, E, Q, item, partial); break;
+        default: seg_short<kSegShortWalk, WAVES>(S.red, E, Q, item, partial); break;
         }
         return;
     }
+    constexpr uint32_t NT = 64 * WAVES;
     const uint32_t tid = threadIdx.x;
     float2* s_lt = S.lt;
     uint64_t *s_oq = S.oq, *s_on = S.on;
@@ -1522,18 +1538,18 @@ __device__ __forceinline__ void seg_item(const SegLds& S, const SegDev& E, const
                                                  (LdsU32*)s_low};
     // once per engine per workgroup of a persistent launch
     if ((int)Q.eng != eng) {
-        for (uint32_t k = tid; k < E.nbytes * 256u; k += 256) s_lt[k] = E.lt[k];
-        if (tid < E.nout) {
-            s_oq[tid] = E.oq[tid]; s_on[tid] = E.on[tid];
-            s_ox[tid] = E.ox[tid]; s_opin[tid] = E.opin[tid]; s_olam[tid] = E.olam[tid];
+        for (uint32_t k = tid; k < E.nbytes * 256u; k += NT) s_lt[k] = E.lt[k];
+        for (uint32_t k = tid; k < E.nout; k += NT) {
+            s_oq[k] = E.oq[k]; s_on[k] = E.on[k];
+            s_ox[k] = E.ox[k]; s_opin[k] = E.opin[k]; s_olam[k] = E.olam[k];
         }
         if (tid < 4 * E.Mcap) s_rmid[tid] = E.rmid[tid];
-        for (uint32_t k = tid; k < (6u << E.L6); k += 256) s_low[k] = E.lowtab[k];
+        for (uint32_t k = tid; k < (6u << E.L6); k += NT) s_low[k] = E.lowtab[k];
         __syncthreads();
         eng = (int)Q.eng;
     }
     float ev[4] = {0.f, 0.f, 0.f, 0.f}, od[4] = {0.f, 0.f, 0.f, 0.f};
-    const uint64_t lane = (item - Q.item0) * 256 + tid;
+    const uint64_t lane = (item - Q.item0) * NT + tid;
     if ((lane >> Q.H) == 0) {
         SegLaneDev L;
         seg_seed(E, T, Q, lane, L);
@@ -1591,7 +1607,7 @@ __device__ __forceinline__ void seg_item(const SegLds& S, const SegDev& E, const
         // lane total = omega^K_seed F (EVu + omega ODu)
         seg_lane_total(eu, ou, L.K, fh, fl, ev, od);
     }
-    seg_publish(s_red, ev, od, item, partial);
+    seg_publish<WAVES>(s_red, ev, od, item, partial);
 }
 
 // SHORT: every query of the launch walks at most 2^kSegShortWalk points
@@ -1601,8 +1617,10 @@ __device__ __forceinline__ void seg_item(const SegLds& S, const SegDev& E, const
 // a batch of at most one item per workgroup takes no queue atomics at all.
 // The long-walk variant is held to 7 waves per SIMD (70 VGPRs, no spills):
 // its table reads are latency-bound and want every resident wave.
-template <bool SHORT>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(SHORT ? 4 : 7, 8))) void k_seg_walk(const SegDev* __restrict__ engs, const SegQ* __restrict__ qs, const uint32_t* __restrict__ item_q,
+// WAVES (1, 2 or 4) is the workgroup shape: an item is the 64 WAVES lanes of
+// one workgroup.
+template <bool SHORT, uint32_t WAVES>
+__global__ __launch_bounds__(64 * WAVES) __attribute__((amdgpu_waves_per_eu(SHORT ? 4 : 7, 8))) void k_seg_walk(const SegDev* __restrict__ engs, const SegQ* __restrict__ qs, const uint32_t* __restrict__ item_q,
                                                   uint64_t item_lo, uint64_t item_end, unsigned long long* __restrict__ counter, double* __restrict__ partial) {
     PQ_SEG_LDS(lds);
     __shared__ unsigned long long s_item;
@@ -1612,7 +1630,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(SHORT ? 4 :
         const uint32_t qi = cview(item_q)[item];
         const SegQ Q = cload(qs + qi);
         const SegDev E = cload(engs + Q.eng);
-        seg_item<SHORT>(lds, E, Q, item, eng, partial);
+        seg_item<SHORT, WAVES>(lds, E, Q, item, eng, partial);
         if (item_end - item_lo <= gridDim.x) break;    // every item was a first item
         __syncthreads();                               // LDS free for the next item
         if (threadIdx.x == 0) s_item = item_lo + gridDim.x + atomicAdd(counter, 1ull);
@@ -1637,8 +1655,8 @@ struct SegArgs {
 // idle).  The query record and its engine descriptor sit together at an
 // offset given by blockIdx.y alone: the kernel's only batch of argument
 // loads is issued as the wave starts.
-template <bool SHORT>
-__global__ __launch_bounds__(256) void k_seg_walk_args(const SegArgs A, double* __restrict__ partial) {
+template <bool SHORT, uint32_t WAVES>
+__global__ __launch_bounds__(64 * WAVES) void k_seg_walk_args(const SegArgs A, double* __restrict__ partial) {
     PQ_SEG_LDS(lds);
     const SegQ Q = A.a[blockIdx.y].q;
     const SegDev E = A.a[blockIdx.y].e;
@@ -1647,7 +1665,7 @@ __global__ __launch_bounds__(256) void k_seg_walk_args(const SegArgs A, double* 
     asm volatile("" ::"s"(E.pts), "s"(E.pcls), "s"(E.lowtab), "s"(E.I), "s"(E.nout), "s"(E.L6), "s"(E.Mcap), "s"(E.nbytes));
     if (blockIdx.x >= Q.nitems) return;          // uniform: the whole workgroup
     int eng = -1;
-    seg_item<SHORT>(lds, E, Q, Q.item0 + blockIdx.x, eng, partial);
+    seg_item<SHORT, WAVES>(lds, E, Q, Q.item0 + blockIdx.x, eng, partial);
 }
 
 // FP64 cross-workgroup reduction (fixed order) and global scaling per query.
@@ -1690,6 +1708,15 @@ void seg_set_tuning(uint32_t offset, uint32_t max_mid) {
     g_seg_offset = offset; g_seg_max = max_mid;
 }
 void seg_get_tuning(uint32_t* offset, uint32_t* max_mid) { *offset = g_seg_offset; *max_mid = g_seg_max; }
+// Workgroup shape of the segment walk: waves per workgroup (1, 2 or 4), so an
+// item is 64, 128 or 256 lanes.  Like the walk length it is global, so a
+// query's items and partial sums do not depend on the batch it is in.
+static std::atomic<uint32_t> g_seg_waves{4};
+void seg_set_waves(uint32_t waves) {
+    if (waves != 1 && waves != 2 && waves != 4) throw std::invalid_argument("segment walk workgroups have 1, 2 or 4 waves");
+    g_seg_waves = waves;
+}
+uint32_t seg_get_waves() { return g_seg_waves; }
 uint32_t dev_mid_bits(const DevEngine& D, int walk) {
     if (walk == kWalkSeg) return seg_mid_bits(*D.seg, g_seg_offset, g_seg_max);
     return z8_auto_mid_bits(D.z8->d, D.z8->kin);
@@ -1707,6 +1734,7 @@ struct BatchPlan {
     std::vector<double2> scale;
     std::vector<uint32_t> order;
     uint64_t total_items = 0, branches = 0;
+    uint32_t waves = 4;                 // workgroup waves (segment walk; the branch walk's are 4)
     uint32_t q_lo[kVariants] = {0, 0, 0, 0}, q_hi[kVariants] = {0, 0, 0, 0};
     uint64_t it_lo[kVariants] = {0, 0, 0, 0}, it_hi[kVariants] = {0, 0, 0, 0};
     uint32_t nq() const { return (uint32_t)(qs.size() / 64); }
@@ -1762,6 +1790,8 @@ BatchPlan plan_branch(const std::vector<Z8Job>& jobs) {
 BatchPlan plan_seg(const std::vector<Z8Job>& jobs) {
     BatchPlan B;
     B.walk = kWalkSeg;
+    B.waves = g_seg_waves;
+    const uint64_t lanes = 64ull * B.waves;
     std::vector<const DevEngine*> seen;
     std::vector<uint32_t> idx(jobs.size());
     std::iota(idx.begin(), idx.end(), 0u);
@@ -1796,7 +1826,7 @@ BatchPlan plan_seg(const std::vector<Z8Job>& jobs) {
         const int v = q.M <= kSegShortWalk ? 1 : 0;
         if (v != cur) { B.q_lo[v] = B.nq(); B.it_lo[v] = B.total_items; cur = v; }
         q.item0 = B.total_items;
-        q.nitems = ((1ull << q.H) + 255) / 256;
+        q.nitems = ((1ull << q.H) + lanes - 1) / lanes;
         B.total_items += q.nitems;
         B.item_q.insert(B.item_q.end(), q.nitems, B.nq());
         B.branches += 1ull << G.d;
@@ -1812,22 +1842,22 @@ BatchPlan plan_batch(const std::vector<Z8Job>& jobs, int walk) {
     if (walk == kWalkBranch) return plan_branch(jobs);
     throw std::invalid_argument("unknown walk form");
 }
-// Resident 256-thread workgroups per CU of a kernel, asked of the runtime
-// once per kernel.
+// Resident workgroups of `block` threads per CU of a kernel, asked of the
+// runtime once per kernel.
 template <class K>
-int blocks_per_cu(K kernel) {
+int blocks_per_cu(K kernel, int block) {
     static std::mutex mu;
     static std::vector<std::pair<const void*, int>> known;
     std::lock_guard<std::mutex> lk(mu);
     for (const auto& e : known) if (e.first == (const void*)kernel) return e.second;
     int per_cu = 0;
-    HIPCHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, 256, 0));
+    HIPCHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, block, 0));
     known.emplace_back((const void*)kernel, per_cu);
     return per_cu;
 }
 template <class K>
-uint32_t grid_for(Ctx& C, K kernel, uint64_t items) {
-    const int per_cu = blocks_per_cu(kernel);
+uint32_t grid_for(Ctx& C, K kernel, uint64_t items, int block = 256) {
+    const int per_cu = blocks_per_cu(kernel, block);
     const uint64_t g = (uint64_t)std::max(per_cu, 1) * (uint64_t)C.cus;
     return (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(g, items));
 }
@@ -1866,6 +1896,18 @@ void stage_copy(const BatchPlan& B, const StageLayout& L, uint8_t* h) {
     std::memcpy(h + L.oo, B.order.data(), B.order.size() * 4);
     std::memcpy(h + L.oi, B.item_q.data(), B.item_q.size() * 4);
 }
+// Variant v of a segment batch with WAVES-wave workgroups.
+template <uint32_t WAVES>
+uint32_t launch_seg(Ctx& C, hipStream_t st, const BatchPlan& B, int v, const void* de, const void* dq, const void* item_q,
+                    unsigned long long* ctr, double* part, bool one_pass) {
+    const uint64_t items = B.it_hi[v] - B.it_lo[v];
+    auto kern = v ? k_seg_walk<true, WAVES> : k_seg_walk<false, WAVES>;
+    const uint32_t grid = one_pass ? (uint32_t)items : grid_for(C, kern, items, 64 * WAVES);
+    hipLaunchKernelGGL(kern, dim3(grid), dim3(64 * WAVES), 0, st, (const SegDev*)de, (const SegQ*)dq, (const uint32_t*)item_q,
+                       B.it_lo[v], B.it_hi[v], ctr + v, part);
+    HIPCHECK(hipGetLastError());
+    return grid;
+}
 // Launches the walk kernels of a batch; item partials go to part (device
 // memory, or pinned host memory seen through its device address).  With
 // one_pass every item gets a workgroup of its own (the hardware dispatcher
@@ -1880,12 +1922,11 @@ uint32_t launch_walks(Ctx& C, hipStream_t st, const BatchPlan& B, uint8_t* dbase
         for (int v = 0; v < 2; ++v) {
             const uint64_t items = B.it_hi[v] - B.it_lo[v];
             if (items == 0) continue;
-            auto kern = v ? k_seg_walk<true> : k_seg_walk<false>;
-            const uint32_t grid = one_pass ? (uint32_t)items : grid_for(C, kern, items);
-            hipLaunchKernelGGL(kern, dim3(grid), dim3(256), 0, st, (const SegDev*)de, (const SegQ*)dq, (const uint32_t*)(dbase + L.oi),
-                               B.it_lo[v], B.it_hi[v], ctr + v, part);
-            HIPCHECK(hipGetLastError());
-            g += grid;
+            switch (B.waves) {
+            case 1: g += launch_seg<1>(C, st, B, v, de, dq, dbase + L.oi, ctr, part, one_pass); break;
+            case 2: g += launch_seg<2>(C, st, B, v, de, dq, dbase + L.oi, ctr, part, one_pass); break;
+            default: g += launch_seg<4>(C, st, B, v, de, dq, dbase + L.oi, ctr, part, one_pass); break;
+            }
         }
         return g;
     }
@@ -1918,6 +1959,15 @@ void make_args(const BatchPlan& B, int v, SegArgs& A, uint32_t& gx, uint32_t& gy
         gx = std::max<uint32_t>(gx, (uint32_t)r.q.nitems);
     }
 }
+template <uint32_t WAVES>
+void launch_args_v(hipStream_t st, int v, const SegArgs& A, uint32_t gx, uint32_t gy, double* part, hipEvent_t e0, hipEvent_t e1) {
+    auto kern = v ? k_seg_walk_args<true, WAVES> : k_seg_walk_args<false, WAVES>;
+    if (e0 || e1)
+        hipExtLaunchKernelGGL(kern, dim3(gx, gy), dim3(64 * WAVES), 0, st, e0, e1, 0, A, part);
+    else
+        hipLaunchKernelGGL(kern, dim3(gx, gy), dim3(64 * WAVES), 0, st, A, part);
+    HIPCHECK(hipGetLastError());
+}
 // With t0/t1 set, the two events take their timestamps from the walk
 // kernels' own dispatches (start of the first, end of the last), so the
 // interval holds no marker packets and reads what a kernel trace reads.
@@ -1934,12 +1984,12 @@ uint32_t launch_args(hipStream_t st, const BatchPlan& B, double* part, hipEvent_
     uint32_t g = 0;
     for (int v = 0; v < 2; ++v) {
         if (gy[v] == 0) continue;
-        auto kern = v ? k_seg_walk_args<true> : k_seg_walk_args<false>;
-        if (t0 || t1)
-            hipExtLaunchKernelGGL(kern, dim3(gx[v], gy[v]), dim3(256), 0, st, v == first ? t0 : nullptr, v == last ? t1 : nullptr, 0, A[v], part);
-        else
-            hipLaunchKernelGGL(kern, dim3(gx[v], gy[v]), dim3(256), 0, st, A[v], part);
-        HIPCHECK(hipGetLastError());
+        hipEvent_t e0 = v == first ? t0 : nullptr, e1 = v == last ? t1 : nullptr;
+        switch (B.waves) {
+        case 1: launch_args_v<1>(st, v, A[v], gx[v], gy[v], part, e0, e1); break;
+        case 2: launch_args_v<2>(st, v, A[v], gx[v], gy[v], part, e0, e1); break;
+        default: launch_args_v<4>(st, v, A[v], gx[v], gy[v], part, e0, e1); break;
+        }
         g += gx[v] * gy[v];
     }
     return g;

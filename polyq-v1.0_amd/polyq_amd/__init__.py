@@ -61,6 +61,8 @@ lib.pq_set_timing.argtypes = [ctypes.c_int]
 lib.pq_get_timing.argtypes = []
 lib.pq_set_walk_tuning.argtypes = [ctypes.c_int, ctypes.c_uint32, ctypes.c_uint32]
 lib.pq_get_walk_tuning.argtypes = [ctypes.c_int, _u32p]
+lib.pq_set_walk_shape.argtypes = [ctypes.c_int, ctypes.c_uint32]
+lib.pq_get_walk_shape.argtypes = [ctypes.c_int, _u32p]
 lib.pq_set_segment_bits.argtypes = [ctypes.c_uint32]
 lib.pq_get_segment_bits.restype = ctypes.c_uint32
 lib.pq_sim_seg_info.argtypes = [_sim_p, _u64p]
@@ -616,6 +618,18 @@ def get_walk_tuning(form="z8"):
     return int(o[0]), int(o[1])
 
 
+def set_walk_shape(waves, form="z8"):
+    """Workgroup shape of a batched form: waves of 64 lanes per workgroup
+    (the segment walk takes 1, 2 or 4; the branch walk only 4)."""
+    _check(lib.pq_set_walk_shape(FORMS[form], int(waves)))
+
+
+def get_walk_shape(form="z8"):
+    o = np.zeros(1, dtype=np.uint32)
+    _check(lib.pq_get_walk_shape(FORMS[form], _ptr(o, _u32p)))
+    return int(o[0])
+
+
 def set_segment_bits(bits):
     """Cap on the table bits I + npin of the segment tables of simulators
     created afterwards (default 16); an engine of branch dimension d gets
@@ -627,6 +641,7 @@ def get_segment_bits():
     return int(lib.pq_get_segment_bits())
 
 
+_TUNE_SHAPES = {"z8": (1, 2, 4), "branch_walk": (4,)}
 _TUNE_GRID = {"z8": ((8, 10, 11, 12, 13, 14, 16), (0, 2, 4, 6, 8)),
               "branch_walk": ((23, 24, 25, 26, 27, 28, 29), (4, 6, 8))}
 
@@ -642,22 +657,25 @@ def branch_walk_mid_bits(d, offset, max_mid, inner_slots):
     return min(d - offset, max_mid, d - inner_slots if d > inner_slots else 0)
 
 
-def autotune(sims, inputs, outputs, form="z8", offsets=None, max_mids=None, repeats=3, cache=True, objective="wall", rounds=3):
-    """Pick the lane walk split (offset, max_mid) of a batched form that
-    minimises the time of this batch: its wall time per call (objective
-    "wall", the default: host prologue, launch and wait included) or its walk
-    kernel time from HIP events ("kernel").  Splits that give every engine
-    of the batch the same walk length are one configuration and are timed
-    once; the configurations are timed in `rounds` interleaved rounds of
-    `repeats` calls (after one untimed call each), so slow drifts of the host
-    do not favour one of them, and each is scored by its median.  Applies
-    the best and, if cache, stores it per device and form in
-    ~/.cache/polyq_amd/walk_tuning.json.
-    Returns (offset, max_mid, {"offset/max_mid": ms})."""
+def autotune(sims, inputs, outputs, form="z8", offsets=None, max_mids=None, shapes=None, repeats=3, cache=True, objective="wall",
+             rounds=3):
+    """Pick the lane walk split (offset, max_mid) and the workgroup shape
+    (waves per workgroup, `shapes`) of a batched form that minimise the time
+    of this batch: its wall time per call (objective "wall", the default:
+    host prologue, launch and wait included) or its walk kernel time from HIP
+    events ("kernel").  Splits that give every engine of the batch the same
+    walk length are one configuration, timed once per shape; the (walk,
+    shape) trials are timed in `rounds` interleaved rounds of `repeats` calls
+    (after one untimed call each), so slow drifts of the host do not favour
+    one of them, and each is scored by its median.  Applies the best and, if
+    cache, stores it per device and form in ~/.cache/polyq_amd/walk_tuning.json.
+    Returns (offset, max_mid, {"offset/max_mid/waves": ms}); the chosen shape
+    is get_walk_shape(form)."""
     import json
     import time
     offsets = offsets or _TUNE_GRID[form][0]
     max_mids = max_mids or _TUNE_GRID[form][1]
+    shapes = shapes or _TUNE_SHAPES[form]
     xs = _u64_array(inputs)
     ys = _u64_array(outputs)
     distinct = list({id(s): s for s in sims}.values())
@@ -672,30 +690,33 @@ def autotune(sims, inputs, outputs, form="z8", offsets=None, max_mids=None, repe
             else:
                 key = (off, mx)
             configs.setdefault(key, []).append((off, mx))
-    samples = {k: [] for k in configs}
+    trials = [(key, w) for key in configs for w in shapes]
+    samples = {t: [] for t in trials}
     prev = get_timing()
     set_timing(objective == "kernel")
     try:
         for _ in range(rounds):
-            for key, splits in configs.items():
-                set_walk_tuning(splits[0][0], splits[0][1], form)
+            for key, w in trials:
+                set_walk_tuning(configs[key][0][0], configs[key][0][1], form)
+                set_walk_shape(w, form)
                 batch_amplitudes(sims, xs, ys, form=form)     # untimed
                 for _ in range(repeats):
                     t0 = time.perf_counter()
                     batch_amplitudes(sims, xs, ys, form=form)
                     wall = 1e3 * (time.perf_counter() - t0)
-                    samples[key].append(last_timing()["kernel_ms"] if objective == "kernel" else wall)
+                    samples[(key, w)].append(last_timing()["kernel_ms"] if objective == "kernel" else wall)
     finally:
         set_timing(prev)
     table = {}
     best = None
-    for key, splits in configs.items():
-        t = float(np.median(samples[key]))
-        for off, mx in splits:
-            table["%d/%d" % (off, mx)] = t
+    for key, w in trials:
+        t = float(np.median(samples[(key, w)]))
+        for off, mx in configs[key]:
+            table["%d/%d/%d" % (off, mx, w)] = t
         if best is None or t < best[0]:
-            best = (t, splits[0][0], splits[0][1])
+            best = (t, configs[key][0][0], configs[key][0][1], w)
     set_walk_tuning(best[1], best[2], form)
+    set_walk_shape(best[3], form)
     if cache:
         try:
             path = os.path.join(os.path.expanduser("~"), ".cache", "polyq_amd", "walk_tuning.json")
@@ -704,7 +725,7 @@ def autotune(sims, inputs, outputs, form="z8", offsets=None, max_mids=None, repe
             if os.path.exists(path):
                 with open(path) as f:
                     data = json.load(f)
-            data["%s|%s" % (device_info(), form)] = {"offset": best[1], "max_mid": best[2], "table_ms": table}
+            data["%s|%s" % (device_info(), form)] = {"offset": best[1], "max_mid": best[2], "waves": best[3], "table_ms": table}
             with open(path, "w") as f:
                 json.dump(data, f)
         except OSError:
@@ -723,6 +744,7 @@ def load_tuning(form="z8"):
         return None
     if rec:
         set_walk_tuning(rec["offset"], rec["max_mid"], form)
+        set_walk_shape(rec.get("waves", 4), form)
         return rec["offset"], rec["max_mid"]
     return None
 
