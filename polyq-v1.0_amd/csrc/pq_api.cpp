@@ -10,6 +10,7 @@
 #include <map>
 #include <cstring>
 #include <algorithm>
+#include <exception>
 
 namespace pq {
 
@@ -18,23 +19,37 @@ thread_local double g_timing[5] = {0, 0, 0, 0, 0};
 
 std::shared_ptr<RankEng> build_rank(const PathSum& ps, const std::vector<uint32_t>& fixed, int strategy, uint32_t pairs) {
     auto R = std::make_shared<RankEng>();
-    auto add = [&](std::shared_ptr<HostEngine> he) {
-        auto z = std::make_shared<Z8Image>(build_z8_image(*he));
-        auto g = std::make_shared<SegImage>(build_seg_image(*he));
+    auto push = [&](std::shared_ptr<HostEngine> he, std::shared_ptr<Z8Image> z, std::shared_ptr<SegImage> g) {
         R->image_bytes += dev_engine_image_bytes(*he, *z, *g);
         R->host.push_back(he);
         R->z8.push_back(z);
         R->seg.push_back(g);
     };
+    auto add = [&](std::shared_ptr<HostEngine> he) {
+        push(he, std::make_shared<Z8Image>(build_z8_image(*he)), std::make_shared<SegImage>(build_seg_image(*he)));
+    };
     uint32_t nt = 0;
     auto cands = pair_candidates(ps, fixed, nt);
     uint32_t p = 0;
     if (strategy == PQ_AUTO || strategy == PQ_GPU_AUTO) {
-        // rank.rs:115-123: base engine, cost model, rebuild with p pairs if p > 0
-        auto base = build_engine(ps, fixed);
-        p = strategy == PQ_AUTO ? best_pairs(base->stats(), (uint32_t)cands.size(), nt)
-                                : best_pairs_gpu(base->stats(), (uint32_t)cands.size(), nt, kGpuBranchSeconds);
-        if (p == 0) { add(base); R->pairs = 0; }
+        // rank.rs:115-123: base engine, cost model, rebuild with p pairs if p > 0.
+        // A base engine beyond the device's term limit still has its stats,
+        // and the pair engines may be within the limit.
+        std::shared_ptr<HostEngine> base;
+        EngineStats st;
+        try {
+            base = build_engine(ps, fixed);
+            st = base->stats();
+        } catch (const TermLimit& e) {
+            st = e.st;
+        }
+        p = strategy == PQ_AUTO ? best_pairs(st, (uint32_t)cands.size(), nt)
+                                : best_pairs_gpu(st, (uint32_t)cands.size(), nt, kGpuBranchSeconds);
+        if (p == 0) {
+            if (!base) throw TermLimit(st);
+            add(base);
+            R->pairs = 0;
+        }
     } else if (strategy == PQ_CACHED) {
         p = 0;
         add(build_engine(ps, fixed));
@@ -45,12 +60,31 @@ std::shared_ptr<RankEng> build_rank(const PathSum& ps, const std::vector<uint32_
         throw std::invalid_argument("strategy must be PQ_AUTO, PQ_CACHED, PQ_PAIRS or PQ_GPU_AUTO");
     }
     if (R->host.empty()) {
-        // rank.rs:88-103: one engine per substitution pattern of the chosen pairs
-        for (uint32_t pattern = 0; pattern < (1u << p); ++pattern) {
-            PathSum b = ps;
-            for (uint32_t i = 0; i < p; ++i) b.substitute(cands[i].first, cands[i].second, (pattern >> i) & 1);
-            add(build_engine(b, fixed));
-        }
+        // rank.rs:88-103: one engine per substitution pattern of the chosen
+        // pairs.  The patterns' engines and walk images are independent: they
+        // are built on the host pool and kept in pattern order.
+        const size_t np = size_t(1) << p;
+        std::vector<std::shared_ptr<HostEngine>> hs(np);
+        std::vector<std::shared_ptr<Z8Image>> zs(np);
+        std::vector<std::shared_ptr<SegImage>> gs(np);
+        std::mutex emu;
+        std::exception_ptr err;
+        ChunkPool::get().run(np, 1, 64, [&](size_t lo, size_t hi, unsigned) {
+            try {
+                for (size_t pattern = lo; pattern < hi; ++pattern) {
+                    PathSum b = ps;
+                    for (uint32_t i = 0; i < p; ++i) b.substitute(cands[i].first, cands[i].second, (pattern >> i) & 1);
+                    hs[pattern] = build_engine(b, fixed);
+                    zs[pattern] = std::make_shared<Z8Image>(build_z8_image(*hs[pattern]));
+                    gs[pattern] = std::make_shared<SegImage>(build_seg_image(*hs[pattern]));
+                }
+            } catch (...) {
+                std::lock_guard<std::mutex> lk(emu);
+                if (!err) err = std::current_exception();
+            }
+        });
+        if (err) std::rethrow_exception(err);
+        for (size_t pattern = 0; pattern < np; ++pattern) push(hs[pattern], zs[pattern], gs[pattern]);
         R->pairs = p;
     }
     return R;

@@ -362,7 +362,7 @@ void place_terms(HostEngine& E, const std::vector<Bits>& kernel_full, const std:
         for (const Bits& n : kernel_full) n.each([&](uint32_t k) { active[k] = 1; });
         for (uint32_t k = 0; k < t; ++k) if (active[k]) perm.push_back(k);
         E.tdev = (uint32_t)perm.size();
-        if (E.tdev > 64) throw std::invalid_argument("more than 64 non-Clifford terms vary over the branch subspace");
+        if (E.tdev > 64) throw TermLimit(EngineStats{E.m, E.L.g, E.L.nodd, E.L.neven, t, (uint64_t)kernel_full.size()});
         for (uint32_t k = 0; k < t; ++k) if (!active[k]) perm.push_back(k);
     }
     std::vector<uint32_t> pos(t);

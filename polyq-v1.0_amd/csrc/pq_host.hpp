@@ -189,7 +189,16 @@ std::shared_ptr<HostEngine> build_engine(const PathSum& ps, const std::vector<ui
 // Place the terms (device terms first, see HostEngine::tdev) given the RREF
 // kernel vectors over all t terms and their non-pivot columns: permutes mcols,
 // theta and the pivots, and sets kernel, free_col, kernel_u, tdev and rkdev.
+// Throws TermLimit when more than 64 terms vary over the branch subspace.
 void place_terms(HostEngine& E, const std::vector<Bits>& kernel_full, const std::vector<uint32_t>& free_full);
+// An engine whose branch vectors vary in more than 64 terms: the device walks
+// keep them in one 64-bit word.  It carries the engine's stats, which are
+// complete by then, so Strategy::Auto can still run its cost model on it.
+struct TermLimit : std::invalid_argument {
+    EngineStats st;
+    explicit TermLimit(const EngineStats& s)
+        : std::invalid_argument("more than 64 non-Clifford terms vary over the branch subspace"), st(s) {}
+};
 // Derive the evaluator's data from an engine's core fields (fixed values,
 // fixed edges b_fixed, layout L, ucols, mcols, diag, o3, sigma, the RREF
 // pivot_col / pivot_row / consistency, kernel, free_col, kernel_u, theta, nh,
