@@ -289,6 +289,8 @@ def headline(args, pq, W, np, probe):
 
     if not args.no_cpu:
         O = _oracle()
+        # a one-GPU job on the GPU host gets a share of 16 CPU threads, whatever
+        # os.cpu_count() reports there (the count is recorded as host_nproc)
         threads = max(1, min(16, os.cpu_count() or 1))
         oc = O.OCircuit.from_circuit(c)
         osim = O.OSim(oc, "cached", threads=threads)
