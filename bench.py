@@ -511,6 +511,7 @@ def _serve_run(pq, np, sched, fresh):
     t_end = time.perf_counter() - t0
     st = srv.stats()
     hs = srv.host_stats()
+    phases = srv.phase_stats()
     srv.close()
     lat = np.array(lat)
     pct = lambda a, q: float(np.percentile(a, q)) if len(a) else None
@@ -524,6 +525,8 @@ def _serve_run(pq, np, sched, fresh):
         "batches": st["batches"], "overlapped_batches": st["overlapped_batches"], "max_batch": st["max_batch"],
         "builds": st["builds"], "evictions": st["evictions"], "resident_bytes": st["resident_bytes"],
         "host_bytes": hs["host_bytes"], "host_drops": hs["host_drops"], "walk_kernel_ms_total": st["kernel_ms"],
+        # mean ms per request phase, over all requests and over the slowest 1%
+        "phases_ms": phases,
     }
     return out, vals
 

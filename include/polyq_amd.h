@@ -267,6 +267,14 @@ int pq_server_stats(pq_server* srv, double out10[10]);
 int pq_server_set_host_budget(pq_server* srv, uint64_t bytes);
 /* host image bytes held, host budget, circuits dropped by the host LRU. */
 int pq_server_host_stats(pq_server* srv, double out3[3]);
+/* Where requests spend their time (no counterpart in the reference), over the
+ * last 65536 completed requests: out[0] = their count; out[1..6] = mean ms
+ * per phase over all of them; out[7..12] = the same over the slowest 1%.
+ * Phases: queue (submit to a host worker), host build (compile, normal form,
+ * walk images; marginal sandwich engines), upload (admission and copy to
+ * HBM), prologue (boundary, prepare, query records), wait for launch, device
+ * and finish (launch to result). */
+int pq_server_phase_stats(pq_server* srv, double out13[13]);
 
 /* ---- parity and diagnostics (no counterpart in the reference) ---- */
 /* Per-branch Gauss-sum outputs on the first branch engine of the cached

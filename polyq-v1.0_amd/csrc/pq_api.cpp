@@ -229,20 +229,34 @@ bool Sim::assignment(uint64_t x, uint64_t y, std::vector<std::pair<uint32_t, uin
 }
 
 std::shared_ptr<const RankEng> Sim::marginal_engine(const std::vector<uint32_t>& qubits) {
-    std::lock_guard<std::mutex> lk(mu);
-    if (!sw) sw = std::make_unique<Sandwich>(compile_sandwich(circ));
     std::vector<uint32_t> key(qubits);
     std::sort(key.begin(), key.end());
     key.erase(std::unique(key.begin(), key.end()), key.end());
+    std::vector<uint32_t> f;
+    const Sandwich* sp;
+    {
+        std::lock_guard<std::mutex> lk(mu);
+        if (!sw) sw = std::make_unique<Sandwich>(compile_sandwich(circ));
+        auto it = marg.find(key);
+        if (it != marg.end()) {
+            it->second.last_use = ++marg_clock;
+            return it->second.r;
+        }
+        // Sandwich::fixed_for (sim.rs:95-103)
+        f = sw->ps.io_vars();
+        for (uint32_t q : qubits) if (std::find(f.begin(), f.end(), sw->junction[q]) == f.end()) f.push_back(sw->junction[q]);
+        sp = sw.get();          // made once, never replaced
+    }
+    // built outside the lock: marginals of other qubit sets (and amplitude
+    // requests) on this simulator go on meanwhile; a concurrent build of the
+    // same set keeps the first one cached
+    auto R = build_rank(sp->ps, f, strategy, pairs_opt);
+    std::lock_guard<std::mutex> lk(mu);
     auto it = marg.find(key);
     if (it != marg.end()) {
         it->second.last_use = ++marg_clock;
         return it->second.r;
     }
-    // Sandwich::fixed_for (sim.rs:95-103)
-    std::vector<uint32_t> f = sw->ps.io_vars();
-    for (uint32_t q : qubits) if (std::find(f.begin(), f.end(), sw->junction[q]) == f.end()) f.push_back(sw->junction[q]);
-    auto R = build_rank(sw->ps, f, strategy, pairs_opt);
     if (marg.size() >= kMargCap) {
         auto lru = marg.begin();
         for (auto i = marg.begin(); i != marg.end(); ++i) if (i->second.last_use < lru->second.last_use) lru = i;

@@ -17,6 +17,7 @@
 //    and reduction order depend only on its own branch dimension.
 #include "pq_api_internal.hpp"
 #include "../../include/polyq_amd.h"
+#include <array>
 #include <atomic>
 #include <chrono>
 #include <condition_variable>
@@ -61,6 +62,10 @@ struct Req {
     Circuit circ;
     int strategy = PQ_CACHED;
     double t_submit = 0, t_done = 0;
+    // phase stamps: worker start, host build done, engines resident, jobs
+    // ready, batch launched (the submit fast path sets the first three to
+    // t_submit)
+    double t_start = 0, t_built = 0, t_res = 0, t_ready = 0, t_launch = 0;
     std::vector<Z8Job> jobs;       // prepared device work
     std::shared_ptr<const RankEng> keep;   // the engine set the jobs point into (marginals)
     std::complex<double> val{0, 0};
@@ -110,6 +115,11 @@ struct Server {
     uint64_t next_ticket = 1, use_clock = 0;
     // statistics
     uint64_t completed = 0, failed = 0, batches = 0, evictions = 0, builds = 0, max_batch = 0, overlapped = 0;
+    // phase durations (s) of the last kPhaseLog completed requests: queue,
+    // build, upload, prologue, wait for launch, device and finish
+    static constexpr size_t kPhaseLog = 1 << 16;
+    std::vector<std::array<float, 6>> phase_log;
+    size_t phase_next = 0;
     double timed_kernel_ms = 0;      // walk-kernel time of the timed batches
     uint64_t timed_batches = 0, launched = 0;
 
@@ -281,6 +291,7 @@ struct Server {
                 r->entry->pinned++;
                 r->entry->last_use = ++use_clock;
             }
+            r->t_start = now_s();
             try {
                 const bool built_now = build_entry(r->entry, *r);
                 if (!r->entry->err.empty()) throw std::runtime_error(r->entry->err);
@@ -290,15 +301,20 @@ struct Server {
                 if (r->kind == 2) {
                     r->keep = S.marginal_engine(r->mq);
                     account_host(r->entry);
+                    r->t_built = now_s();
                     make_resident(r->entry, *r->keep);
+                    r->t_res = now_s();
                     if (S.marginal_assignment(r->x, r->mq.data(), r->mo.data(), r->mq.size(), asg)) add_jobs(*r->keep, asg, walk, r->jobs, owner, 0);
                 } else {
                     if (built_now) account_host(r->entry);
+                    r->t_built = now_s();
                     make_resident(r->entry, *S.eng);
                     S.eng->ensure_device();
+                    r->t_res = now_s();
                     add_query_jobs_xy(*S.eng, r->x, r->y, walk, r->jobs, owner, 0);
                 }
                 finish_jobs(*r);
+                r->t_ready = now_s();
                 std::lock_guard<std::mutex> lk(mu);
                 ready_q.push_back(r);
                 nready++;
@@ -317,6 +333,15 @@ struct Server {
         }
     }
 
+    void log_phases(const Req& r) {     // with mu held
+        const double t_launch = r.t_launch > 0 ? r.t_launch : r.t_ready;
+        const std::array<float, 6> ph{(float)(r.t_start - r.t_submit), (float)(r.t_built - r.t_start), (float)(r.t_res - r.t_built),
+                                      (float)(r.t_ready - r.t_res), (float)(t_launch - r.t_ready), (float)(r.t_done - t_launch)};
+        if (phase_log.size() < kPhaseLog) phase_log.push_back(ph);
+        else phase_log[phase_next] = ph;
+        phase_next = (phase_next + 1) % kPhaseLog;
+    }
+
     void complete(std::vector<std::shared_ptr<Req>>& batch, const std::vector<std::complex<double>>& res) {
         const double t = now_s();
         std::lock_guard<std::mutex> lk(mu);
@@ -328,6 +353,7 @@ struct Server {
             else if (r->kind == 1) r->val = {std::norm(a), 0};
             else r->val = {std::max(a.real(), 0.0), 0};        // sim.rs:230
             r->status = 1; r->t_done = t;
+            log_phases(*r);
             r->jobs.clear();
             unpin(*r);
             inflight--;
@@ -369,6 +395,8 @@ struct Server {
                 const int s = order[0] < 0 ? 0 : (order[0] == 0 ? 1 : 0);   // a free slot (at most one is in flight here)
                 std::vector<Z8Job> jobs;
                 for (auto& r : batch) for (auto& J : r->jobs) jobs.push_back(J);
+                const double tl = now_s();
+                for (auto& r : batch) r->t_launch = tl;
                 if (jobs.empty()) { complete(batch, {}); continue; }
                 // timing events on every 8th batch only: each event is a
                 // queue barrier that lengthens a short batch
@@ -460,10 +488,12 @@ struct Server {
             cv_work.notify_one();
             return r->ticket;
         }
+        r->t_start = r->t_built = r->t_res = r->t_submit;
         try {
             std::vector<size_t> owner;
             add_query_jobs_xy(*r->entry->sim->eng, r->x, r->y, walk, r->jobs, owner, 0);
             finish_jobs(*r);
+            r->t_ready = now_s();
         } catch (std::exception& e) {
             std::lock_guard<std::mutex> lk(mu);
             r->status = -1; r->err = e.what(); r->t_done = now_s();
@@ -546,6 +576,31 @@ int pq_server_stats(pq_server* h, double* o) {
     o[4] = (double)S.resident.load(); o[5] = (double)S.builds; o[6] = (double)S.max_batch; o[7] = (double)S.overlapped;
     o[8] = S.timed_batches ? S.timed_kernel_ms * (double)S.batches / (double)S.timed_batches : 0.0;
     o[9] = (double)S.entries.size();
+    return 0;
+}
+
+int pq_server_phase_stats(pq_server* h, double* o) {
+    Server& S = *h->s;
+    std::vector<std::array<float, 6>> log;
+    {
+        std::lock_guard<std::mutex> lk(S.mu);
+        log = S.phase_log;
+    }
+    for (int k = 0; k < 13; ++k) o[k] = 0.0;
+    o[0] = (double)log.size();
+    if (log.empty()) return 0;
+    std::vector<std::pair<double, size_t>> tot(log.size());
+    for (size_t i = 0; i < log.size(); ++i) {
+        double t = 0;
+        for (int k = 0; k < 6; ++k) t += log[i][k];
+        tot[i] = {t, i};
+    }
+    std::sort(tot.begin(), tot.end());
+    const size_t ntail = std::max<size_t>(1, log.size() / 100);
+    for (size_t i = 0; i < log.size(); ++i)
+        for (int k = 0; k < 6; ++k) o[1 + k] += 1e3 * log[i][k] / (double)log.size();
+    for (size_t j = log.size() - ntail; j < log.size(); ++j)
+        for (int k = 0; k < 6; ++k) o[7 + k] += 1e3 * log[tot[j].second][k] / (double)ntail;
     return 0;
 }
 

@@ -136,7 +136,10 @@ static Z8Image build_z8_slots(const HostEngine& E, const uint32_t K) {
             const uint32_t cls = (kin >> 1) & 1;
             if (cls != (Z.imag0 ? (c & 1u) : 0u)) throw std::logic_error("inner real/imaginary classes do not follow slot 0");
             double fin = 1.0;
-            for (uint32_t k = 0; k < t; ++k) if ((inner_free >> k) & 1) fin *= ((sin >> k) & 1) ? E.r1[k] : E.r0[k];
+            for (uint64_t rest = inner_free; rest; rest &= rest - 1) {
+                const uint32_t k = (uint32_t)__builtin_ctzll(rest);
+                fin *= ((sin >> k) & 1) ? E.r1[k] : E.r0[k];
+            }
             Z.inner[i] = (kin & 4) ? -fin : fin;
         }
         Z.inner_aux[i] = pin << 3;
@@ -187,14 +190,22 @@ static Z8Image build_z8_slots(const HostEngine& E, const uint32_t K) {
     return Z;
 }
 
-// kInnerWide inner slots when no inner direction touches a pivot (the plain
-// walk variant); otherwise kInnerPiv, walked by the variant with pivot
-// lookups, whose pivot state leaves no registers for the larger table (the
-// kInnerPiv image may then turn out pivot-free: that variant walks it too).
+// kInnerWide inner slots when that many pivot-free kernel vectors exist (or
+// every vector is pivot-free): the inner directions, chosen pivot-free first,
+// then touch no pivot and the plain walk variant runs the image.  Otherwise
+// kInnerPiv, walked by the variant with pivot lookups, whose pivot state
+// leaves no registers for the larger table (that variant also walks a
+// kInnerPiv image that turns out pivot-free).
 Z8Image build_z8_image(const HostEngine& E) {
-    Z8Image Z = build_z8_slots(E, kInnerWide);
-    if (Z.npin == 0) return Z;
-    return build_z8_slots(E, kInnerPiv);
+    std::vector<int32_t> piv_index(E.tdev, -1);
+    for (uint32_t i = 0; i < E.rkdev; ++i) piv_index[E.pivot_col[i]] = (int32_t)i;
+    uint32_t free_vecs = 0;
+    for (uint32_t j = 0; j < E.d; ++j) {
+        bool pf = true;
+        for (uint64_t b = E.kernel[j]; b && pf; b &= b - 1) pf = piv_index[__builtin_ctzll(b)] < 0;
+        free_vecs += pf;
+    }
+    return build_z8_slots(E, free_vecs >= std::min(E.d, kInnerWide) ? kInnerWide : kInnerPiv);
 }
 
 Z8QueryHost z8_query(const HostEngine& E, const Z8Image& Z, const Prepared& P) {

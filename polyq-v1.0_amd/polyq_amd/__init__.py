@@ -824,6 +824,7 @@ lib.pq_server_wait.argtypes = [ctypes.c_void_p, ctypes.c_uint64, ctypes.c_double
 lib.pq_server_stats.argtypes = [ctypes.c_void_p, _f64p]
 lib.pq_server_set_host_budget.argtypes = [ctypes.c_void_p, ctypes.c_uint64]
 lib.pq_server_host_stats.argtypes = [ctypes.c_void_p, _f64p]
+lib.pq_server_phase_stats.argtypes = [ctypes.c_void_p, _f64p]
 
 KINDS = {"amplitude": 0, "probability": 1, "marginal": 2}
 
@@ -847,6 +848,16 @@ class Server:
         o = np.zeros(3, dtype=np.float64)
         _check(lib.pq_server_host_stats(self._h, _ptr(o, _f64p)))
         return dict(zip(("host_bytes", "host_budget", "host_drops"), o.tolist()))
+
+    PHASES = ("queue", "build", "upload", "prologue", "launch_wait", "device")
+
+    def phase_stats(self):
+        """Mean ms per request phase over the last 65536 completed requests
+        ("all") and over the slowest 1% of them ("slowest_1pct")."""
+        o = np.zeros(13, dtype=np.float64)
+        _check(lib.pq_server_phase_stats(self._h, _ptr(o, _f64p)))
+        return {"requests": int(o[0]), "all": dict(zip(self.PHASES, o[1:7].tolist())),
+                "slowest_1pct": dict(zip(self.PHASES, o[7:13].tolist()))}
 
     def submit(self, circuit, kind, x=0, y=0, measured=()):
         ops, q0, q1, ph = circuit.arrays()
