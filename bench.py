@@ -200,7 +200,7 @@ def headline(args, pq, W, np, probe):
         tuned = tuple(int(v) for v in args.headline_tuning.split(","))
         pq.set_walk_tuning(*tuned, form="branch_walk")
     else:
-        tuned = pq.autotune([sim] * nq, xs, ys, form="branch_walk", repeats=3, rounds=2, cache=False)[:2]
+        tuned = pq.autotune([sim] * nq, xs, ys, form="branch_walk", repeats=8, rounds=4, cache=False)[:2]
     for _ in range(args.warmup):
         sim.amplitudes(xs, ys)
     t0 = time.perf_counter()
