@@ -236,3 +236,29 @@ def test_server_host_lru_drops_idle_builds_and_rebuilds():
         want = complex(*s.amplitude(x, y)) if kind == "amplitude" else complex(s.marginal(x, meas), 0.0)
         assert v == want
     srv.close()
+
+
+def test_server_phase_timings_add_up_to_latency():
+    # the six phases of a request run from its submission to its result, so
+    # their means add up to the mean latency the waits report; the slowest 1%
+    # add up to at least that mean
+    work = _mixed_circuits(19, 12, nmax=16)
+    srv = pq.Server(hbm_budget_bytes=64 << 30, workers=4)
+    rnd = random.Random(8)
+    tickets = []
+    for rep in range(4):
+        for c, pairs in work:
+            kind = rnd.choice(["amplitude", "probability", "marginal"])
+            x, y = pairs[rnd.randrange(4)]
+            meas = [(q, rnd.random() < 0.5) for q in rnd.sample(range(c.num_qubits), 2)] if kind == "marginal" else ()
+            tickets.append(srv.submit(c, kind, x, y, meas))
+    lat = [srv.wait(t, timeout=120.0)[1] for t in tickets]
+    ph = srv.phase_stats()
+    srv.close()
+    assert ph["requests"] == len(tickets)
+    assert set(ph["all"]) == set(pq.Server.PHASES)
+    assert all(v >= 0 for v in ph["all"].values()) and all(v >= 0 for v in ph["slowest_1pct"].values())
+    total = sum(ph["all"].values())
+    assert abs(total - float(np.mean(lat))) <= 1e-3 * float(np.mean(lat)) + 1e-4, (total, np.mean(lat))
+    assert sum(ph["slowest_1pct"].values()) >= total
+    assert ph["all"]["build"] > 0          # the first request of each circuit builds it
