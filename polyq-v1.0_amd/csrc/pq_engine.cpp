@@ -31,8 +31,19 @@ namespace pq {
 // ties by row index.  Bucket bitsets per (class, weight) replace the two
 // ordered sets: the lowest row of the lightest non-empty bucket is the
 // smallest (weight, index) entry, so the pivot sequence is the same.
-NormalForm normal_form(const std::vector<Bits>& b, const Bits& parity) {
+// With z4 given, z[r] = q(P_r) mod 4 follows each basis row: q is a Z4
+// quadratic refinement of the Gram form beta (q(x) = sum_i v_i x_i + 2 sum_{i<j}
+// b_ij x_i x_j, beta = B + diag(v mod 2)), so q(x ^ y) = q(x) + q(y) + 2
+// beta(x, y) mod 4, and beta(P_r, P_s) is the current Gram entry (r, s).
+NormalForm normal_form(const std::vector<Bits>& b, const Bits& parity, const std::vector<uint8_t>* z4) {
     const uint32_t m = (uint32_t)b.size();
+    const bool track = z4 != nullptr;
+    std::vector<uint8_t> z;
+    if (track) {
+        if (z4->size() != m) throw std::logic_error("normal form: Z4 values do not match the Gram rows");
+        z = *z4;
+        for (uint32_t i = 0; i < m; ++i) if ((z[i] & 1) != (uint8_t)parity.get(i)) throw std::logic_error("normal form: Z4 value and parity disagree");
+    }
     const size_t W = (m + 63) / 64;
     // Gram and basis rows in two flat arrays (row r at r * W), scratch rows
     // allocated once: a pivot step allocates nothing.
@@ -116,10 +127,12 @@ NormalForm normal_form(const std::vector<Bits>& b, const Bits& parity) {
             each(s, [&](uint32_t r) {
                 uint64_t* g = G(r); uint64_t* p = P(r);
                 for (size_t k = 0; k < W; ++k) { g[k] ^= s[k]; p[k] ^= pa[k]; }
+                if (track) z[r] = (uint8_t)((z[r] + z[ua] + 2) & 3);     // beta(P_r, P_a) = 1
             });
             unclass(ua);
             clr(alive.data(), ua);
             nf.odd.push_back(to_bits(pa));
+            if (track) nf.zodd.push_back(z[ua]);
             each(s, [&](uint32_t r) { reclass(r); });
             continue;
         }
@@ -135,36 +148,28 @@ NormalForm normal_form(const std::vector<Bits>& b, const Bits& parity) {
         for (size_t k = 0; k < W; ++k) touched[k] = sa[k] | sb[k];
         each(touched, [&](uint32_t r) {
             uint64_t* g = G(r); uint64_t* p = P(r);
-            if (bit(sa, r)) for (size_t k = 0; k < W; ++k) { g[k] ^= sb[k]; p[k] ^= pb[k]; }
-            if (bit(sb, r)) for (size_t k = 0; k < W; ++k) { g[k] ^= sa[k]; p[k] ^= pa[k]; }
+            const bool ra = bit(sa, r), rb = bit(sb, r);
+            if (ra) for (size_t k = 0; k < W; ++k) { g[k] ^= sb[k]; p[k] ^= pb[k]; }
+            if (rb) for (size_t k = 0; k < W; ++k) { g[k] ^= sa[k]; p[k] ^= pa[k]; }
+            if (track) {
+                // P_r ^= P_b with beta(P_r, P_b) = [r in sb]; then P_r ^= P_a
+                // with beta = beta(P_r, P_a) (+ beta(P_b, P_a) = 1 if r in sa),
+                // which is 0 either way
+                if (ra) z[r] = (uint8_t)((z[r] + z[bb] + 2 * (rb ? 1 : 0)) & 3);
+                if (rb) z[r] = (uint8_t)((z[r] + z[ua]) & 3);
+            }
         });
         unclass(ua); unclass(bb);
         clr(alive.data(), ua); clr(alive.data(), bb);
         nf.pa.push_back(to_bits(pa)); nf.pb.push_back(to_bits(pb));
+        if (track) { nf.zpa.push_back(z[ua]); nf.zpb.push_back(z[bb]); }
         each(touched, [&](uint32_t r) { reclass(r); });
     }
-    each(alive.data(), [&](uint32_t r) { nf.even.push_back(to_bits(P(r))); });
+    each(alive.data(), [&](uint32_t r) {
+        nf.even.push_back(to_bits(P(r)));
+        if (track) nf.zeven.push_back(z[r]);
+    });
     return nf;
-}
-
-// d(p) = q(p) mod 4 for a basis vector (normal_form.rs:339-348)
-static uint32_t z4_of(const std::vector<Bits>& b, const Bits& v0, const Bits& v1, const Bits& p) {
-    // linear part plus 2 per edge of b inside p; b is symmetric with a zero
-    // diagonal, so the edges are counted once each as the pairs (i < j):
-    // row j against the bits of p below j
-    uint32_t lin = p.and_popc(v0) + 2 * p.and_popc(v1);
-    uint32_t edges = 0;
-    const uint64_t* pw = p.w.data();
-    const size_t W = p.w.size();
-    for (size_t k = 0; k < W; ++k) {
-        for (uint64_t x = pw[k]; x; x &= x - 1) {
-            const uint32_t j = (uint32_t)(k * 64 + __builtin_ctzll(x));
-            const uint64_t* r = b[j].w.data();
-            for (size_t q = 0; q < k; ++q) edges += (uint32_t)__builtin_popcountll(r[q] & pw[q]);
-            edges += (uint32_t)__builtin_popcountll(r[k] & pw[k] & ((1ull << (j & 63)) - 1ull));
-        }
-    }
-    return (lin + 2 * (edges & 1)) & 3;
 }
 
 static inline bool row_parity(const uint64_t* x, const uint64_t* y, uint32_t nw) {
@@ -256,7 +261,9 @@ std::shared_ptr<HostEngine> build_engine(const PathSum& ps, const std::vector<ui
     }
     E.b_fixed.assign(E.nf, Bits(E.nf));
     for (uint32_t i = 0; i < E.nf; ++i) ps.adj[fixed[i]].each([&](uint32_t j) { if (E.fixed_pos[j] >= 0) E.b_fixed[i].set((uint32_t)E.fixed_pos[j]); });
-    NormalForm nf = normal_form(bff, parity);
+    std::vector<uint8_t> z4(m);
+    for (uint32_t i = 0; i < m; ++i) z4[i] = (uint8_t)(ps.v[freev[i]] & 3);
+    NormalForm nf = normal_form(bff, parity, &z4);
     ULayout& L = E.L;
     L.g = (uint32_t)nf.pa.size(); L.nodd = (uint32_t)nf.odd.size(); L.neven = (uint32_t)nf.even.size();
     L.Wg = (L.g + 63) / 64; L.Wo = (L.nodd + 63) / 64; L.We = (L.neven + 63) / 64;
@@ -287,22 +294,21 @@ std::shared_ptr<HostEngine> build_engine(const PathSum& ps, const std::vector<ui
     E.mcols.assign((size_t)E.t * W, 0);
     for (uint32_t k = 0; k < E.t; ++k) std::copy_n(&cols[(size_t)crow[term_vars[k]] * W], W, (E.mcols.data() + (size_t)k * W));
 
-    // Diag (normal_form.rs:354-391)
-    Bits v0(m), v1(m);
-    for (uint32_t i = 0; i < m; ++i) { v0.put(i, ps.v[freev[i]] & 1); v1.put(i, ps.v[freev[i]] & 2); }
+    // Diag (normal_form.rs:354-391) from the basis vectors' Z4 values d(p) =
+    // q(p) mod 4 (normal_form.rs:339-348), carried through the elimination
     E.diag.assign(W, 0);
     E.o3.assign(L.Wo, 0);
     for (uint32_t j = 0; j < L.g; ++j) {
-        if (z4_of(bff, v0, v1, nf.pa[j]) == 2) E.diag[L.off_a() + (j >> 6)] |= 1ull << (j & 63);
-        if (z4_of(bff, v0, v1, nf.pb[j]) == 2) E.diag[L.off_b() + (j >> 6)] |= 1ull << (j & 63);
+        if (nf.zpa[j] == 2) E.diag[L.off_a() + (j >> 6)] |= 1ull << (j & 63);
+        if (nf.zpb[j] == 2) E.diag[L.off_b() + (j >> 6)] |= 1ull << (j & 63);
     }
     E.sigma = 0;
     for (uint32_t l = 0; l < L.nodd; ++l) {
-        if (z4_of(bff, v0, v1, nf.odd[l]) == 1) { E.diag[L.off_o() + (l >> 6)] |= 1ull << (l & 63); E.sigma += 1; }
+        if (nf.zodd[l] == 1) { E.diag[L.off_o() + (l >> 6)] |= 1ull << (l & 63); E.sigma += 1; }
         else { E.o3[l >> 6] |= 1ull << (l & 63); E.sigma -= 1; }
     }
     for (uint32_t l = 0; l < L.neven; ++l)
-        if (z4_of(bff, v0, v1, nf.even[l]) == 2) E.diag[L.off_e() + (l >> 6)] |= 1ull << (l & 63);
+        if (nf.zeven[l] == 2) E.diag[L.off_e() + (l >> 6)] |= 1ull << (l & 63);
 
     // RREF of K = M restricted to the radical coordinates (engine.rs:119-184),
     // rows over all t terms

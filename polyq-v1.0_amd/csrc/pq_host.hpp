@@ -103,8 +103,14 @@ Sandwich compile_sandwich(const Circuit& c);                                // p
 struct NormalForm {
     uint32_t m = 0;
     std::vector<Bits> pa, pb, odd, even;
+    // with the Z4 linear values given: q(p) mod 4 of each basis vector, the
+    // values normal_form.rs:339-391 reads the Diag from
+    std::vector<uint8_t> zpa, zpb, zodd, zeven;
 };
-NormalForm normal_form(const std::vector<Bits>& b, const Bits& parity);    // normal_form.rs:240-336 (MinWeight)
+// normal_form.rs:240-336 (MinWeight).  z4 (optional): the linear coefficient
+// mod 4 of each variable (its parity is `parity`); the Z4 value of every basis
+// vector is then carried through the row operations.
+NormalForm normal_form(const std::vector<Bits>& b, const Bits& parity, const std::vector<uint8_t>* z4 = nullptr);
 
 // Shift coordinates packed in one row: [a: Wg words | b: Wg | o: Wo | e: We]
 // (u64 words).  Along kernel directions the e part is zero.
