@@ -43,8 +43,8 @@ struct Entry {
     bool in_table = true;      // false once the host LRU dropped it
     size_t host_bytes = 0;     // host images accounted against the host budget
     std::mutex build_mu;
-    bool built = false;
-    std::string err;
+    bool built = false;        // sim exists (its amplitude engines may not yet)
+    std::string err;           // the amplitude engine build failed
     std::unique_ptr<Sim> sim;
     size_t bytes = 0;          // device bytes accounted while resident
     bool resident = false;
@@ -211,28 +211,42 @@ struct Server {
         trim_host_locked(E.get());
     }
 
-    // Host build once per circuit (src/sim.rs:144-148, src/rank.rs:88-123).
-    // True when this call built it.
+    // Host build once per circuit (src/sim.rs:144-148).  The amplitude engine
+    // set (src/rank.rs:88-123) is built by the first request that needs it:
+    // a marginal request needs only its sandwich engines, so a new circuit
+    // whose first request is a marginal waits for those alone.  A failed
+    // amplitude build is kept in err and fails the amplitude and probability
+    // requests of the circuit; marginals go on.  True when this call built
+    // something.
     bool build_entry(const std::shared_ptr<Entry>& E, const Req& r) {
         std::lock_guard<std::mutex> bl(E->build_mu);
-        if (E->built || !E->err.empty()) return false;
-        try {
+        bool did = false;
+        if (!E->built) {
             auto s = std::make_unique<Sim>();
             s->circ = r.circ;
             s->strategy = r.strategy;
-            s->ps = compile(s->circ);
-            s->eng = build_rank(s->ps, s->ps.io_vars(), r.strategy, 0);
-            s->eng->bprog = boundary_prog(s->ps, *s->eng->host[0]);
-            s->eng->has_bprog = true;
             E->sim = std::move(s);
             std::lock_guard<std::mutex> lk(mu);
             E->built = true;
             builds++;
-            return true;
-        } catch (std::exception& e) {
-            E->err = e.what();
+            did = true;
         }
-        return false;
+        if (r.kind != 2 && !E->sim->eng && E->err.empty()) {
+            try {
+                Sim& S = *E->sim;
+                PathSum ps = compile(S.circ);
+                auto eng = build_rank(ps, ps.io_vars(), S.strategy, 0);
+                eng->bprog = boundary_prog(ps, *eng->host[0]);
+                eng->has_bprog = true;
+                std::lock_guard<std::mutex> lk(S.mu);
+                S.ps = std::move(ps);
+                S.eng = eng;
+                did = true;
+            } catch (std::exception& e) {
+                E->err = e.what();
+            }
+        }
+        return did;
     }
 
     static size_t estimate_bytes(const RankEng& R) { return R.image_bytes; }
@@ -294,7 +308,7 @@ struct Server {
             r->t_start = now_s();
             try {
                 const bool built_now = build_entry(r->entry, *r);
-                if (!r->entry->err.empty()) throw std::runtime_error(r->entry->err);
+                if (r->kind != 2 && !r->entry->err.empty()) throw std::runtime_error(r->entry->err);
                 Sim& S = *r->entry->sim;
                 std::vector<std::pair<uint32_t, uint8_t>> asg;
                 std::vector<size_t> owner;
@@ -477,7 +491,7 @@ struct Server {
             // entry is never evicted, so the images stay after it is released.
             if (kind != 2 && E.build_mu.try_lock()) {
                 if (E.built && E.err.empty()) {
-                    here = E.sim->eng->on_device();
+                    here = E.sim->eng && E.sim->eng->on_device();
                 }
                 E.build_mu.unlock();
                 if (here) { E.pinned++; E.last_use = ++use_clock; }
