@@ -29,6 +29,7 @@
 #include <mutex>
 #include <chrono>
 #include <thread>
+#include <functional>
 #include <atomic>
 #include <cstring>
 #include <algorithm>
@@ -304,6 +305,23 @@ BlobLayout build_blob(const HostEngine& E, const Z8Image& Z, const SegImage& G, 
 }
 }  // namespace
 
+// Streams for engine uploads (default priority), each used by one thread at
+// a time.  Made once, on the first upload (the context exists by then); a
+// failed creation is retried by the next upload.
+struct UploadStreams {
+    static constexpr size_t N = 8;
+    hipStream_t s[N] = {};
+    std::mutex mu[N];
+};
+UploadStreams& upload_streams() {
+    static UploadStreams* made = [] {
+        std::unique_ptr<UploadStreams> u(new UploadStreams());
+        for (auto& x : u->s) HIPCHECK(hipStreamCreateWithFlags(&x, hipStreamNonBlocking));
+        return u.release();
+    }();
+    return *made;
+}
+
 size_t dev_engine_image_bytes(const HostEngine& E, const Z8Image& Z, const SegImage& G) {
     BlobBuilder B;
     build_blob(E, Z, G, B);
@@ -312,7 +330,7 @@ size_t dev_engine_image_bytes(const HostEngine& E, const Z8Image& Z, const SegIm
 
 std::shared_ptr<DevEngine> dev_engine_upload(std::shared_ptr<const HostEngine> Ep, std::shared_ptr<const Z8Image> Zp,
                                              std::shared_ptr<const SegImage> Gp) {
-    Ctx& C = ctx();
+    (void)ctx();                 // the device context exists before the first upload
     const HostEngine& E = *Ep;
     const Z8Image& Z = *Zp;
     const SegImage& G = *Gp;
@@ -323,11 +341,18 @@ std::shared_ptr<DevEngine> dev_engine_upload(std::shared_ptr<const HostEngine> E
     const BlobLayout O = build_blob(E, Z, G, B);
     const uint32_t NW = O.NW, NWe = O.NWe;
 
-    std::lock_guard<std::mutex> lk(C.mu);
+    // the copy goes on an upload stream of its own: host workers uploading
+    // different engines neither wait for each other nor for the context's
+    // stream
     HIPCHECK(hipMalloc(&D->blob, B.bytes.size()));
     D->bytes = B.bytes.size();
-    HIPCHECK(hipMemcpyAsync(D->blob, B.bytes.data(), B.bytes.size(), hipMemcpyHostToDevice, C.stream));
-    HIPCHECK(hipStreamSynchronize(C.stream));
+    {
+        UploadStreams& U = upload_streams();
+        const size_t k = std::hash<std::thread::id>{}(std::this_thread::get_id()) % UploadStreams::N;
+        std::lock_guard<std::mutex> lk(U.mu[k]);
+        HIPCHECK(hipMemcpyAsync(D->blob, B.bytes.data(), B.bytes.size(), hipMemcpyHostToDevice, U.s[k]));
+        HIPCHECK(hipStreamSynchronize(U.s[k]));
+    }
     auto at = [&](size_t off) { return (const uint8_t*)D->blob + off; };
     PlainDev& P = D->plain;
     P.ku = (const uint32_t*)at(O.o_ku); P.kn = (const uint64_t*)at(O.o_kn); P.soff = (const uint32_t*)at(O.o_soff); P.supp = at(O.o_supp);
