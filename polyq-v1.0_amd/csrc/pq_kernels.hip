@@ -1643,9 +1643,10 @@ __device__ __forceinline__ void seg_item(const SegLds& S, const SegDev& E, const
 // The long-walk variant is held to 7 waves per SIMD (70 VGPRs, no spills):
 // its table reads are latency-bound and want every resident wave.
 // WAVES (1, 2 or 4) is the workgroup shape: an item is the 64 WAVES lanes of
-// one workgroup.
+// one workgroup.  With fewer waves per workgroup the long walk's LDS tables
+// (20 KB per workgroup) cap the occupancy at 2 WAVES waves per SIMD.
 template <bool SHORT, uint32_t WAVES>
-__global__ __launch_bounds__(64 * WAVES) __attribute__((amdgpu_waves_per_eu(SHORT ? 4 : 7, 8))) void k_seg_walk(const SegDev* __restrict__ engs, const SegQ* __restrict__ qs, const uint32_t* __restrict__ item_q,
+__global__ __launch_bounds__(64 * WAVES) __attribute__((amdgpu_waves_per_eu(SHORT ? 4 : (WAVES == 4 ? 7 : 2 * WAVES), 8))) void k_seg_walk(const SegDev* __restrict__ engs, const SegQ* __restrict__ qs, const uint32_t* __restrict__ item_q,
                                                   uint64_t item_lo, uint64_t item_end, unsigned long long* __restrict__ counter, double* __restrict__ partial) {
     PQ_SEG_LDS(lds);
     __shared__ unsigned long long s_item;

@@ -4,6 +4,9 @@
 // with one operand from an SGPR pair or broadcast across each 16-lane row by
 // DPP, the two ways a uniform table entry can reach it).  Each lane runs
 // 8 independent chains, so the rate measured is issue throughput, not latency.
+// Two more kernels measure the FP64 matrix core for comparison: 16x16x4 FP64
+// MFMAs alone, and MFMAs with VALU FP64 FMAs beside them (rates in lane-FMAs:
+// one such MFMA is 16 FMAs per lane).
 // The kernel's clock is read in-kernel (s_memtime against the constant-rate
 // s_memrealtime) so that rates can be quoted in lanes per clock per CU.
 //
@@ -11,6 +14,7 @@
 // nor vectorise the chains; none of them touches memory.
 #include <hip/hip_runtime.h>
 #include <cstdint>
+#include <algorithm>
 #include <cstring>
 #include <string>
 #include <vector>
@@ -20,9 +24,14 @@ namespace pq {
 extern thread_local std::string g_err;     // pq_last_error() (pq_api.cpp)
 
 namespace {
-enum ProbeOp { P_XOR = 0, P_AND, P_ADD, P_BCNT, P_FMA32, P_FMA64, P_XOR_FMA64, P_FMAC64_S, P_FMAC64_DPP, P_COUNT };
+enum ProbeOp { P_XOR = 0, P_AND, P_ADD, P_BCNT, P_FMA32, P_FMA64, P_XOR_FMA64, P_FMAC64_S, P_FMAC64_DPP, P_MFMA64, P_MFMA64_FMA64, P_COUNT };
 const char* const kProbeNames[P_COUNT] = {"v_xor_b32", "v_and_b32", "v_add_u32", "v_bcnt_u32_b32", "v_fma_f32", "v_fma_f64",
-                                          "v_xor_b32+v_fma_f64", "v_fmac_f64 (SGPR operand)", "v_fmac_f64_dpp row_newbcast"};
+                                          "v_xor_b32+v_fma_f64", "v_fmac_f64 (SGPR operand)", "v_fmac_f64_dpp row_newbcast",
+                                          "v_mfma_f64_16x16x4f64", "v_mfma_f64_16x16x4f64+2 v_fma_f64"};
+// FMA lanes per lane and iteration of the MFMA kernels: 8 MFMAs of 16 lane-FMAs,
+// plus 16 VALU FMAs in the mix.
+constexpr double kMfmaLaneOps[2] = {8.0 * 16.0, 8.0 * 16.0 + 16.0};
+typedef double f64x4 __attribute__((ext_vector_type(4)));
 
 template <int OP>
 __device__ __forceinline__ void probe_step(uint32_t (&a)[8], double (&d)[8], uint32_t b, float fb, double db, double sdb) {
@@ -79,6 +88,42 @@ __global__ __launch_bounds__(256) void k_valu(uint32_t iters, uint32_t seed, uin
     sink[tid] = x;
 }
 
+// iters x (8 FP64 MFMAs on independent accumulators, and with MIX 2 VALU
+// FP64 FMAs after each).
+template <bool MIX>
+__global__ __launch_bounds__(256) void k_mfma(uint32_t iters, uint32_t seed, uint32_t* __restrict__ sink, uint64_t* __restrict__ clk) {
+    const uint32_t tid = blockIdx.x * 256 + threadIdx.x;
+    f64x4 acc[8];
+    double d[16];
+#pragma unroll
+    for (int k = 0; k < 8; ++k) acc[k] = f64x4{1.0 + k, 0.0, 0.0, 0.0};
+#pragma unroll
+    for (int k = 0; k < 16; ++k) d[k] = 1.0 + k;
+    const double a = seed == 7u ? 1.0 : 0.5, b = 1.0;
+    uint64_t c0 = 0, w0 = 0;
+    if (blockIdx.x == 0 && threadIdx.x == 0) { c0 = clock64(); w0 = wall_clock64(); }
+    for (uint32_t it = 0; it < iters; ++it) {
+#pragma unroll
+        for (int k = 0; k < 8; ++k) {
+            acc[k] = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, acc[k], 0, 0, 0);
+            if constexpr (MIX) {
+                asm volatile("v_fma_f64 %0, %0, %1, %2" : "+v"(d[2 * k]) : "v"(b), "v"(0.0));
+                asm volatile("v_fma_f64 %0, %0, %1, %2" : "+v"(d[2 * k + 1]) : "v"(b), "v"(0.0));
+            }
+        }
+    }
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        clk[0] = clock64() - c0;
+        clk[1] = wall_clock64() - w0;
+    }
+    double x = 0.0;
+#pragma unroll
+    for (int k = 0; k < 8; ++k) x += acc[k][0] + acc[k][1] + acc[k][2] + acc[k][3];
+#pragma unroll
+    for (int k = 0; k < 16; ++k) x += d[k];
+    sink[tid] = (uint32_t)__double2loint(x);
+}
+
 #define VCHECK(x)                                                                                   \
     do {                                                                                            \
         hipError_t e_ = (x);                                                                        \
@@ -89,10 +134,15 @@ template <int OP>
 void run_probe(uint32_t grid, uint32_t iters, uint32_t* sink, uint64_t* clk, hipEvent_t e0, hipEvent_t e1, hipStream_t st,
                double& ms, uint64_t (&hclk)[2]) {
     // one untimed launch to warm the clock, then the timed one
-    hipLaunchKernelGGL(k_valu<OP>, dim3(grid), dim3(256), 0, st, iters / 4 + 1, 7u, sink, clk);
+    auto kern = [] {
+        if constexpr (OP == P_MFMA64) return k_mfma<false>;
+        else if constexpr (OP == P_MFMA64_FMA64) return k_mfma<true>;
+        else return k_valu<OP>;
+    }();
+    hipLaunchKernelGGL(kern, dim3(grid), dim3(256), 0, st, iters / 4 + 1, 7u, sink, clk);
     VCHECK(hipGetLastError());
     VCHECK(hipEventRecord(e0, st));
-    hipLaunchKernelGGL(k_valu<OP>, dim3(grid), dim3(256), 0, st, iters, 7u, sink, clk);
+    hipLaunchKernelGGL(kern, dim3(grid), dim3(256), 0, st, iters, 7u, sink, clk);
     VCHECK(hipGetLastError());
     VCHECK(hipEventRecord(e1, st));
     VCHECK(hipEventSynchronize(e1));
@@ -126,6 +176,8 @@ int valu_probe(double* out, size_t cap, uint32_t blocks_per_cu, uint32_t iters) 
     VCHECK(hipMalloc(&sink, (size_t)grid * 256 * 4));
     VCHECK(hipMalloc(&clk, 16));
     out[0] = cus; out[1] = wall_khz / 1e3; out[2] = blocks_per_cu;
+    // an FP64 MFMA issues for about as long as 16 VALU FP64 FMAs
+    const uint32_t miters = std::max<uint32_t>(1, iters / 16);
     for (int op = 0; op < P_COUNT; ++op) {
         double ms = 0;
         uint64_t hc[2] = {0, 0};
@@ -138,9 +190,12 @@ int valu_probe(double* out, size_t cap, uint32_t blocks_per_cu, uint32_t iters) 
             case P_FMA64: run_probe<P_FMA64>(grid, iters, sink, clk, e0, e1, st, ms, hc); break;
             case P_FMAC64_S: run_probe<P_FMAC64_S>(grid, iters, sink, clk, e0, e1, st, ms, hc); break;
             case P_FMAC64_DPP: run_probe<P_FMAC64_DPP>(grid, iters, sink, clk, e0, e1, st, ms, hc); break;
+            case P_MFMA64: run_probe<P_MFMA64>(grid, miters, sink, clk, e0, e1, st, ms, hc); break;
+            case P_MFMA64_FMA64: run_probe<P_MFMA64_FMA64>(grid, miters, sink, clk, e0, e1, st, ms, hc); break;
             default: run_probe<P_XOR_FMA64>(grid, iters, sink, clk, e0, e1, st, ms, hc); break;
         }
-        const double lane_ops = (double)grid * 256.0 * (double)iters * 64.0;
+        const bool mfma = op == P_MFMA64 || op == P_MFMA64_FMA64;
+        const double lane_ops = (double)grid * 256.0 * (mfma ? (double)miters * kMfmaLaneOps[op - P_MFMA64] : (double)iters * 64.0);
         const double rate = lane_ops / (ms * 1e-3);
         const double ghz = hc[1] ? (double)hc[0] / ((double)hc[1] / (wall_khz * 1e3)) / 1e9 : 0.0;
         double* o = out + 3 + 4 * op;
