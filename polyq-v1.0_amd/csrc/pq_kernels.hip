@@ -1249,6 +1249,9 @@ __device__ __forceinline__ uint32_t seg_lane_parities(const SegDev& E, P64 oq, u
 typedef float f32x4 __attribute__((ext_vector_type(4)));
 // Walks of at most 2^kSegShortWalk points issue all their table reads at once.
 constexpr uint32_t kSegShortWalk = 4;
+// Long walks issue the table reads of kSegGroup points of one class at a time
+// (4, 6 and 8 give the same kernel time on the saturated 4096-amplitude launch).
+constexpr uint32_t kSegGroup = 4;
 // A += f * T (double-float real times double-float complex).
 __device__ __forceinline__ void seg_point(float (&A)[4], float fh, float fl, const f32x4& T) {
     float ph, pl, qh, ql;
@@ -1599,13 +1602,13 @@ __device__ __forceinline__ void seg_item(const SegLds& S, const SegDev& E, const
             for (int c = 0; c < 4; ++c) {
                 const uint32_t p1 = cl[c + 1];
                 uint32_t p = cl[c];
-                // four points per step, all table loads first
-                for (; p + 4 <= p1; p += 4) {
-                    f32x4 Tq[4];
-                    uint32_t sg[4];
-                    float fh4[4], fl4[4];
+                // kSegGroup points per step, all table loads first
+                for (; p + kSegGroup <= p1; p += kSegGroup) {
+                    f32x4 Tq[kSegGroup];
+                    uint32_t sg[kSegGroup];
+                    float fh4[kSegGroup], fl4[kSegGroup];
 #pragma unroll
-                    for (int k = 0; k < 4; ++k) {
+                    for (int k = 0; k < (int)kSegGroup; ++k) {
                         const PQ_CONST uint32_t* e = pt + 4 * (p + k);
                         const uint32_t X = e[0], g = e[1];
                         Tq[k] = tab[base ^ X];
@@ -1615,7 +1618,7 @@ __device__ __forceinline__ void seg_item(const SegLds& S, const SegDev& E, const
                         fh4[k] = __uint_as_float(e[2]); fl4[k] = __uint_as_float(e[3]);
                     }
 #pragma unroll
-                    for (int k = 0; k < 4; ++k)
+                    for (int k = 0; k < (int)kSegGroup; ++k)
                         seg_point(A[c], __uint_as_float(__float_as_uint(fh4[k]) ^ sg[k]), __uint_as_float(__float_as_uint(fl4[k]) ^ sg[k]), Tq[k]);
                 }
                 for (; p < p1; ++p) {
