@@ -652,38 +652,6 @@ template <class T> __device__ __forceinline__ T cload(const T* p) {
     return out;
 }
 
-__device__ __forceinline__ void merge2(float& s, float& c, float s2, float c2) {
-    const float t = s + s2;
-    const float bp = t - s;
-    const float e = (s - (t - bp)) + (s2 - bp);
-    s = t;
-    c = (c + c2) + e;
-}
-
-// One step of a wave reduction of four double-float pairs over DPP: every
-// lane takes the pairs of the lane CTRL selects (0 in rows ROWS leaves out)
-// and merges them.  Register-to-register, no LDS round trip.
-template <int CTRL, int ROWS>
-__device__ __forceinline__ void dpp_merge(float (&v)[8]) {
-    float p[8];
-#pragma unroll
-    for (int i = 0; i < 8; ++i) p[i] = __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v[i]), CTRL, ROWS, 0xF, false));
-#pragma unroll
-    for (int i = 0; i < 8; i += 2) merge2(v[i], v[i + 1], p[i], p[i + 1]);
-}
-// Sum of the four pairs over the 64 lanes, in a fixed order, left in lane 63:
-// pairs and quads (quad_perm), 8 and 16 lanes (half-row and row mirror),
-// then rows 0+1 and 2+3 (row_bcast:15 into rows 1, 3) and all four
-// (row_bcast:31 into rows 2, 3).  All 64 lanes must be active.
-__device__ __forceinline__ void wave_sum_dpp(float (&v)[8]) {
-    dpp_merge<0xB1, 0xF>(v);     // quad_perm [1,0,3,2]
-    dpp_merge<0x4E, 0xF>(v);     // quad_perm [2,3,0,1]
-    dpp_merge<0x141, 0xF>(v);    // row_half_mirror
-    dpp_merge<0x140, 0xF>(v);    // row_mirror
-    dpp_merge<0x142, 0xA>(v);    // row_bcast:15
-    dpp_merge<0x143, 0xC>(v);    // row_bcast:31
-}
-
 // x with its sign bit flipped where sw has bit 31 set (sw is 0 or 0x80000000)
 __device__ __forceinline__ double flip_sign(double m, uint32_t sw) { return __hiloint2double(__double2hiint(m) ^ (int)sw, __double2loint(m)); }
 
@@ -1350,8 +1318,10 @@ __device__ __forceinline__ void dpp_add4(double (&v)[4]) {
     for (int i = 0; i < 4; ++i) v[i] += dpp_f64<CTRL, ROWS>(v[i]);
 }
 // Workgroup reduction of the four sums in FP64: a lane's double-float pair
-// hi + lo is exact in FP64; each wave adds over DPP in the fixed order of
-// wave_sum_dpp (total in lane 63), then the WAVES waves through LDS (a
+// hi + lo is exact in FP64; each wave adds over DPP in a fixed order (pairs
+// and quads by quad_perm, 8 and 16 lanes by the half-row and row mirrors,
+// then rows 0+1 and 2+3 by row_bcast:15 and all four by row_bcast:31; the
+// total lands in lane 63), then the WAVES waves through LDS (a
 // one-wave workgroup reads lane 63 directly: no LDS, no barrier).  The
 // item's four partial sums (EV re, im, OD re, im), one per lane of lanes
 // 0..3: a single 32-byte write, into pinned host memory for small batches
@@ -1392,7 +1362,7 @@ __device__ __forceinline__ void seg_publish(double (*s_red)[4], const float (&ev
     }
 }
 // XOR and sum of a value over the 64 lanes of the wave, broadcast to all
-// (the DPP order of wave_sum_dpp; all 64 lanes must be active).
+// (the DPP order of seg_publish; all 64 lanes must be active).
 template <int CTRL, int ROWS>
 __device__ __forceinline__ uint32_t dpp_src(uint32_t v) { return (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, CTRL, ROWS, 0xF, false); }
 __device__ __forceinline__ uint32_t wave_xor(uint32_t v) {
