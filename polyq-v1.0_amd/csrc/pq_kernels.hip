@@ -797,21 +797,33 @@ __device__ __forceinline__ void inner_steps(double (&a)[16], const InnerRegs<NIN
 // outer point: (tr, ti) of its 256-branch sum (bit 0 of l is the class).
 // The same walk for an engine whose inner directions touch pivots but whose
 // four low inner slots do not: the pivot factor gin[pio ^ pin(c_in)] then
-// depends on the step q alone, so step q reads it once from LDS (the byte
-// offset follows the Gray recurrence over the high slots' pivots ps) and its
-// signed value is the multiplier of the step's 16 FMAs.  The accumulators
-// start at zero.
+// depends on the step q alone, and the signed factor is the multiplier m of
+// the step's 16 FMAs.  A step whose flipped high slot touches no pivot (ps
+// zero, uniform: a scalar branch) only flips m's sign, as the walk without
+// pivots does; one that touches pivots reads the new factor from LDS (the
+// byte offset follows the Gray recurrence over the high slots' pivots).  The
+// builder puts the pivot-touching inner directions in the highest slots,
+// whose flips are the rarest of the Gray walk.  The accumulators start at
+// zero.
 template <uint32_t NIN, int Q>
 __device__ __forceinline__ void inner_steps_piv(double (&a)[16], const InnerRegs<NIN>& T, const double* s_gin, uint32_t addr, uint32_t sgn,
-                                                const uint32_t (&ps)[InnerGeom<NIN>::high], const uint32_t (&sm)[InnerGeom<NIN>::high]) {
+                                                double m, const uint32_t (&ps)[InnerGeom<NIN>::high],
+                                                const uint32_t (&sm)[InnerGeom<NIN>::high]) {
     if constexpr (Q < (int)InnerGeom<NIN>::steps) {
-        if constexpr (Q > 0) {
-            addr ^= ps[__builtin_ctz(Q)];
-            sgn ^= sm[__builtin_ctz(Q)];
+        if constexpr (Q == 0) {
+            m = flip_sign(*(const double*)((const char*)s_gin + addr), sgn);
+        } else {
+            constexpr int j = __builtin_ctz(Q);
+            sgn ^= sm[j];
+            if (ps[j]) {
+                addr ^= ps[j];
+                m = flip_sign(*(const double*)((const char*)s_gin + addr), sgn);
+            } else {
+                m = flip_sign(m, sm[j]);
+            }
         }
-        const double m = flip_sign(*(const double*)((const char*)s_gin + addr), sgn);
         fmac_all<0>(a, T.w[Q], m);
-        inner_steps_piv<NIN, Q + 1>(a, T, s_gin, addr, sgn, ps, sm);
+        inner_steps_piv<NIN, Q + 1>(a, T, s_gin, addr, sgn, m, ps, sm);
     }
 }
 
@@ -886,7 +898,7 @@ __device__ __forceinline__ void inner_walk_mid(const InnerRegs<NIN>& T, const ui
     if constexpr (PIV) {
 #pragma unroll
         for (int l = 0; l < 16; ++l) a[l] = 0.0;
-        inner_steps_piv<NIN, 0>(a, T, s_gin, pio << 3, 0u, ps, sm);
+        inner_steps_piv<NIN, 0>(a, T, s_gin, pio << 3, 0u, 0.0, ps, sm);
     } else {
         double m = 1.0;
         asm volatile("" : "+v"(m));
