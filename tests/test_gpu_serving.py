@@ -42,15 +42,16 @@ def test_batch_results_equal_alone_results_bitwise():
             assert abs(o.amplitude(x, y) - a) < 1e-10
 
 
-def test_graph_replays_match_eager():
+@pytest.mark.parametrize("form", ["z8", "branch_walk"])
+def test_graph_replays_match_eager(form):
     work = _mixed_circuits(12, 12)
     sims = [pq.Simulator(c, strategy="cached") for c, _ in work]
     S, X, Y = [], [], []
     for s, (c, pairs) in zip(sims, work):
         for x, y in pairs:
             S.append(s); X.append(x); Y.append(y)
-    eager = pq.batch_amplitudes(S, X, Y)
-    g = pq.Graph(S, X, Y)
+    eager = pq.batch_amplitudes(S, X, Y, form=form)
+    g = pq.Graph(S, X, Y, form=form)
     for rep in range(4):
         got, ms = g.run()
         np.testing.assert_array_equal(got, eager)
