@@ -1271,10 +1271,6 @@ typedef const __attribute__((address_space(1))) f32x4 GFloat4;
 typedef __attribute__((address_space(3))) const uint64_t LdsU64;
 typedef __attribute__((address_space(3))) const uint32_t LdsU32;
 typedef __attribute__((address_space(3))) const float LdsF32;
-// After its item partials are written, the last workgroup of a query (the
-// one whose increment of the query's done counter completes it) reduces the
-// query in FP64 and writes the scaled amplitude straight into the caller's
-// pinned host buffer: no separate reduction launch and no result copy.
 // The LDS a workgroup keeps across the items it walks: the engine's byte
 // tables and outer-slot arrays (so the lane seed reads LDS, not a chain of
 // scalar-cache round trips) and the wave sums.
@@ -1883,8 +1879,8 @@ uint32_t launch_v(Ctx& C, hipStream_t st, const BatchPlan& B, int v, const Z8Dev
 // Launch the walk of a staged batch (one persistent grid per branch-walk
 // variant present, or one segment-walk grid); returns the summed grid.
 // The work-queue counters come first in the upload and are zero in the host
-// copy, so the descriptor copy also clears them (no fill kernel and no memset
-// node in a captured graph).
+// copy, so an eager batch's descriptor copy also clears them (a graph uploads
+// its descriptors once and clears the counters with k_clear_counters).
 struct StageLayout {
     size_t oe, oq, os, oo, oi, tot;
 };
