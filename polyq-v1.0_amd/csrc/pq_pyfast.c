@@ -1,9 +1,13 @@
-/* CPython fast path for Simulator.amplitudes: basis-index arrays and the
- * result array are passed through the buffer protocol straight to the C ABI
- * (pq_sim_amplitudes), with the GIL released during the call.  ctypes spends
- * about 2.5 us per call converting the same arguments; this path spends a
- * fraction of that.  The Python wrapper takes it for C-contiguous uint64
- * arrays and keeps the ctypes call for everything else. */
+/* CPython fast paths of the Python bindings.
+ * - amplitudes: basis-index arrays and the result array are passed through
+ *   the buffer protocol straight to the C ABI (pq_sim_amplitudes), with the
+ *   GIL released during the call.  ctypes spends about 2.5 us per call
+ *   converting the same arguments; this path spends a fraction of that.  The
+ *   Python wrapper takes it for C-contiguous uint64 arrays and keeps the
+ *   ctypes call for everything else.
+ * - u64_fill: a list or tuple of basis indices into a uint64 array with the
+ *   checks of the Python path (integers only, none negative or above 64
+ *   bits), about 10 ns per index instead of about 140 ns. */
 #define PY_SSIZE_T_CLEAN
 #include <Python.h>
 #include <stdint.h>
@@ -35,8 +39,57 @@ static PyObject* fast_amplitudes(PyObject* self, PyObject* args) {
     return PyLong_FromLong(rc);
 }
 
+/* u64_fill(seq, out, reject) -> None.  seq: a sequence of basis indices
+ * (Python ints or numpy integer scalars); out: a writable C-contiguous buffer
+ * of len(seq) uint64; reject: a type or tuple of types refused outright (the
+ * bool types, which are integers to Python but not indices).  A non-integer
+ * raises TypeError; a negative index or one of more than 64 bits raises
+ * OverflowError, where a plain cast would wrap or truncate it. */
+static PyObject* fast_u64_fill(PyObject* self, PyObject* args) {
+    PyObject *seq, *oo, *reject;
+    (void)self;
+    if (!PyArg_ParseTuple(args, "OOO", &seq, &oo, &reject)) return NULL;
+    PyObject* fast = PySequence_Fast(seq, "basis indices must be a sequence");
+    if (!fast) return NULL;
+    Py_buffer bo;
+    if (PyObject_GetBuffer(oo, &bo, PyBUF_C_CONTIGUOUS | PyBUF_WRITABLE) != 0) { Py_DECREF(fast); return NULL; }
+    const Py_ssize_t n = PySequence_Fast_GET_SIZE(fast);
+    PyObject** items = PySequence_Fast_ITEMS(fast);
+    uint64_t* out = (uint64_t*)bo.buf;
+    int ok = bo.len == 8 * n;
+    if (!ok) PyErr_SetString(PyExc_ValueError, "u64_fill: out needs len(seq) uint64 values");
+    for (Py_ssize_t i = 0; ok && i < n; ++i) {
+        PyObject* it = items[i];
+        unsigned long long v;
+        if (PyLong_CheckExact(it)) {
+            v = PyLong_AsUnsignedLongLong(it);
+        } else {
+            /* other integers: numpy integer scalars, int subclasses; not
+             * the rejected types, floats or anything without __index__ */
+            const int r = PyObject_IsInstance(it, reject);
+            if (r < 0) { ok = 0; break; }
+            if (r || PyFloat_Check(it) || !PyIndex_Check(it)) {
+                PyErr_Format(PyExc_TypeError, "basis index must be an integer, not %.100s", Py_TYPE(it)->tp_name);
+                ok = 0;
+                break;
+            }
+            PyObject* li = PyNumber_Index(it);
+            if (!li) { ok = 0; break; }
+            v = PyLong_AsUnsignedLongLong(li);
+            Py_DECREF(li);
+        }
+        if (v == (unsigned long long)-1 && PyErr_Occurred()) { ok = 0; break; }
+        out[i] = (uint64_t)v;
+    }
+    PyBuffer_Release(&bo);
+    Py_DECREF(fast);
+    if (!ok) return NULL;
+    Py_RETURN_NONE;
+}
+
 static PyMethodDef methods[] = {
     {"amplitudes", fast_amplitudes, METH_VARARGS, "amplitudes(handle, xs, ys, out, form) -> status"},
+    {"u64_fill", fast_u64_fill, METH_VARARGS, "u64_fill(seq, out, reject) -> None"},
     {NULL, NULL, 0, NULL},
 };
 static struct PyModuleDef module = {PyModuleDef_HEAD_INIT, "_pqfast", NULL, -1, methods, NULL, NULL, NULL, NULL};

@@ -222,9 +222,16 @@ def test_direct_prologue_matches_assignment_prologue():
     assert checked >= 100
 
 
-def test_index_marshalling_to_uint64():
+@pytest.mark.parametrize("fast", [True, False])
+def test_index_marshalling_to_uint64(fast, monkeypatch):
     # basis indices reach the C-ABI as one uint64 array, from lists, tuples
-    # or arrays; a negative index is an error, not a wrapped value
+    # or arrays; a negative index is an error, not a wrapped value.  Lists go
+    # through the CPython extension's checked conversion, or the Python one
+    # where the extension is absent: both give the same values and errors.
+    if fast:
+        assert pq._pqfast is not None
+    else:
+        monkeypatch.setattr(pq, "_pqfast", None)
     for v in ([1, 2, 2**64 - 1], (0, 7), np.array([3, 4], dtype=np.int64), np.arange(5, dtype=np.uint64)[::2]):
         a = pq._u64_arg(v)
         assert list(a) == [int(x) for x in v]
@@ -249,6 +256,15 @@ def test_index_marshalling_to_uint64():
         pq._u64_arg(np.array([True]))
     assert pq._index(np.uint64(2**64 - 1)) == 2**64 - 1
     assert list(pq._u64_arg(np.array([7, 0], np.int32))) == [7, 0]
+    # lists mixing Python ints and numpy integer scalars; bools, floats and
+    # other objects inside a list are refused
+    assert list(pq._u64_array([np.uint64(2**64 - 1), np.int32(7), 2**63, 0])) == [2**64 - 1, 7, 2**63, 0]
+    assert list(pq._u64_array(range(3))) == [0, 1, 2]
+    with pytest.raises(OverflowError):
+        pq._u64_array([1, 1 << 64])
+    for bad in ([True, 2], [np.bool_(True)], [1.0], [np.float64(2)], ["a"], [None]):
+        with pytest.raises(TypeError):
+            pq._u64_array(bad)
 
 
 def test_circuit_extend_matches_builders_and_rejects_bad_ops():
