@@ -34,6 +34,7 @@
 #include <cstring>
 #include <algorithm>
 #include <numeric>
+#include <unordered_map>
 #include <cmath>
 
 namespace pq {
@@ -1751,21 +1752,53 @@ template <class T> void push_rec(std::vector<uint8_t>& v, const T& r) {
     v.resize(o + sizeof(T));
     std::memcpy(v.data() + o, &r, sizeof(T));
 }
+// The launch order of a batch's jobs: by key (launch variant, then heavier
+// items first), then the jobs of one engine together, then job index.  The
+// keys are computed once per job, and the order is total, so the plan does
+// not depend on the sort algorithm.
+struct PlanKey {
+    uint64_t k;
+    const DevEngine* eng;
+    uint32_t ji;
+};
+void sort_plan(std::vector<PlanKey>& v) {
+    std::sort(v.begin(), v.end(), [](const PlanKey& a, const PlanKey& b) {
+        if (a.k != b.k) return a.k < b.k;
+        if (a.eng != b.eng) return std::less<const DevEngine*>()(a.eng, b.eng);
+        return a.ji < b.ji;
+    });
+}
+// Index of engine `e` among the batch's descriptors, appending its
+// descriptor on first sight (jobs of one engine are mostly adjacent: the
+// previous job's engine is tried first).
+struct EngineIds {
+    std::unordered_map<const DevEngine*, uint32_t> ids;
+    const DevEngine* last = nullptr;
+    uint32_t last_id = 0;
+    template <class Rec>
+    uint32_t get(const DevEngine* e, const Rec& rec, std::vector<uint8_t>& engs) {
+        if (e == last) return last_id;
+        auto it = ids.find(e);
+        uint32_t id;
+        if (it == ids.end()) { id = (uint32_t)ids.size(); ids.emplace(e, id); push_rec(engs, rec); }
+        else id = it->second;
+        last = e; last_id = id;
+        return id;
+    }
+};
 BatchPlan plan_branch(const std::vector<Z8Job>& jobs) {
     BatchPlan B;
     B.walk = kWalkBranch;
-    std::vector<const DevEngine*> seen;
-    std::vector<uint32_t> idx(jobs.size());
-    std::iota(idx.begin(), idx.end(), 0u);
+    EngineIds seen;
     // grouped by kernel variant; heavy items first; jobs of one engine together
-    std::stable_sort(idx.begin(), idx.end(), [&](uint32_t a, uint32_t b) {
-        const int va = variant_of(*jobs[a].eng->z8), vb = variant_of(*jobs[b].eng->z8);
-        if (va != vb) return va < vb;
-        if (jobs[a].mid_bits != jobs[b].mid_bits) return jobs[a].mid_bits > jobs[b].mid_bits;
-        return jobs[a].eng < jobs[b].eng;
-    });
+    std::vector<PlanKey> order(jobs.size());
+    for (uint32_t ji = 0; ji < (uint32_t)jobs.size(); ++ji)
+        order[ji] = {((uint64_t)variant_of(*jobs[ji].eng->z8) << 32) | (uint64_t)(0xFFFFFFFFu - jobs[ji].mid_bits), jobs[ji].eng, ji};
+    sort_plan(order);
+    B.qs.reserve(64 * jobs.size());
     int cur = -1;
-    for (uint32_t ji : idx) {
+    for (const PlanKey& pk : order) {
+        const uint32_t ji = pk.ji;
         const Z8Job& J = jobs[ji];
         const Z8Image& Z = *J.eng->z8;
         const int v = variant_of(Z);
@@ -1773,10 +1806,7 @@ BatchPlan plan_branch(const std::vector<Z8Job>& jobs) {
             B.q_lo[v] = B.nq(); B.it_lo[v] = B.total_items;
             cur = v;
         }
-        auto it = std::find(seen.begin(), seen.end(), J.eng);
-        uint32_t e;
-        if (it == seen.end()) { e = (uint32_t)seen.size(); seen.push_back(J.eng); push_rec(B.engs, J.eng->z); }
-        else e = (uint32_t)(it - seen.begin());
+        const uint32_t e = seen.get(J.eng, J.eng->z, B.engs);
         Z8Q q{};
         q.delta = J.q.delta; q.s0 = J.q.s0; q.eng = e; q.K0 = J.q.K0; q.pio0 = J.q.pio0;
         const uint32_t nout = Z.nslots - Z.kin;
@@ -1799,9 +1829,7 @@ BatchPlan plan_seg(const std::vector<Z8Job>& jobs) {
     B.walk = kWalkSeg;
     B.waves = g_seg_waves;
     const uint64_t lanes = 64ull * B.waves;
-    std::vector<const DevEngine*> seen;
-    std::vector<uint32_t> idx(jobs.size());
-    std::iota(idx.begin(), idx.end(), 0u);
+    EngineIds seen;
     auto walk_bits = [&](const Z8Job& J) {
         const SegImage& G = *J.eng->seg;
         return std::min(J.mid_bits, std::min(G.Mcap, G.nout - G.L6));
@@ -1810,21 +1838,19 @@ BatchPlan plan_seg(const std::vector<Z8Job>& jobs) {
     // after (variant 1); within each, heavy items first and jobs of one
     // engine together (the LDS tables are reloaded only when a workgroup
     // moves to another engine).
-    std::stable_sort(idx.begin(), idx.end(), [&](uint32_t a, uint32_t b) {
-        const bool sa = walk_bits(jobs[a]) <= kSegShortWalk, sb = walk_bits(jobs[b]) <= kSegShortWalk;
-        if (sa != sb) return sb;
-        const uint32_t ha = jobs[a].eng->seg->nout - walk_bits(jobs[a]), hb = jobs[b].eng->seg->nout - walk_bits(jobs[b]);
-        if (ha != hb) return ha > hb;
-        return jobs[a].eng < jobs[b].eng;
-    });
+    std::vector<PlanKey> order(jobs.size());
+    for (uint32_t ji = 0; ji < (uint32_t)jobs.size(); ++ji) {
+        const uint32_t mb = walk_bits(jobs[ji]);
+        order[ji] = {((uint64_t)(mb <= kSegShortWalk ? 1 : 0) << 32) | (uint64_t)(0xFFFFFFFFu - (jobs[ji].eng->seg->nout - mb)), jobs[ji].eng, ji};
+    }
+    sort_plan(order);
+    B.qs.reserve(64 * jobs.size());
     int cur = -1;
-    for (uint32_t ji : idx) {
+    for (const PlanKey& pk : order) {
+        const uint32_t ji = pk.ji;
         const Z8Job& J = jobs[ji];
         const SegImage& G = *J.eng->seg;
-        auto it = std::find(seen.begin(), seen.end(), J.eng);
-        uint32_t e;
-        if (it == seen.end()) { e = (uint32_t)seen.size(); seen.push_back(J.eng); push_rec(B.engs, J.eng->sg); }
-        else e = (uint32_t)(it - seen.begin());
+        const uint32_t e = seen.get(J.eng, J.eng->sg, B.engs);
         SegQ q{};
         q.delta = J.sq.delta; q.s0 = J.sq.s0; q.eng = e; q.K0 = J.sq.K0; q.hs0 = J.sq.hs0; q.pio0 = J.sq.pio0;
         q.M = walk_bits(J);
