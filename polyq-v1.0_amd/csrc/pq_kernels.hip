@@ -2105,7 +2105,7 @@ void host_finish(const QueryTail& T, const double* part, hipEvent_t done, hipStr
         const uint64_t lo = 4 * T.item0[q], hi = 4 * (T.item0[q] + T.nitems[q]);
         for (uint64_t i = lo; i < hi;) {
             if (w[i] != kPending) { ++i; continue; }
-            if (finished) throw std::runtime_error("segment walk ended without writing every item partial");
+            if (finished) throw std::runtime_error("walk ended without writing every item partial");
             // Yield now and then: the runtime's own threads (completion
             // handling) may share this core, and a spin that never lets them
             // run gets the calling thread descheduled for long stretches
@@ -2128,14 +2128,14 @@ void host_finish(const QueryTail& T, const double* part, hipEvent_t done, hipStr
 }
 }  // namespace
 
-// Small segment-walk batches are finished on the host: the walk writes its
+// Small batches of either walk are finished on the host: the walk writes its
 // item partials straight into pinned host memory and the host reduces each
 // query as soon as its items have landed (no cross-workgroup atomics on the
 // device, no reduction launch, no result copy).  Larger batches reduce on the
 // device (k_z8_reduce) and copy the results.  The reduction order, hence
 // every result bit, is the same either way.
 constexpr uint64_t kHostFinishItems = 4096;
-bool host_finishes(const BatchPlan& B) { return B.walk == kWalkSeg && B.total_items <= kHostFinishItems; }
+bool host_finishes(const BatchPlan& B) { return B.total_items <= kHostFinishItems; }
 
 // One in-flight batch: its own stream, events and buffers, so a second batch
 // can be queued while this one still runs (continuous batching).  Timing
