@@ -168,6 +168,19 @@ int pq_engine_eval(pq_engine* engine, const uint8_t* values, size_t nf, int form
  * is the sum of its 2^p branch engines' values for the same fixed values
  * (src/rank.rs:184-190).  out has 2n doubles. */
 int pq_engine_eval_batch(size_t n, pq_engine* const* engines, const uint8_t* const* values, int form, double* out);
+/* Engine::sum_subspace (src/engine.rs:669-693): the sum of
+ * weight(s) ROOT8[phase8(u(s))] over the affine branch subspace
+ * s_start + span(kernel), walked from the start shift u_start (one shift row,
+ * packed as the descriptor's rows) and branch vector s_start (ceil(terms/64)
+ * words, the reference's term order): what prepare leaves in sc.u and sc.s0
+ * (src/engine.rs:596-606).  c_start[2] is prepare's c_start, the weight at
+ * s_start; the device derives every branch's weight from the term angles, as
+ * the reference does for every chunk but the first, and a c_start that is not
+ * that weight (relative 1e-9) is an error.  eval_with multiplies the result
+ * by the boundary phase and 2^(log2_magnitude - h/2) (src/engine.rs:557-565).
+ * An engine whose path sum vanishes identically gives 0. */
+int pq_engine_sum_subspace(pq_engine* engine, const uint64_t* u_start, const uint64_t* s_start, const double c_start[2], int form,
+                           double out[2]);
 /* Per-branch Gauss-sum outputs of the engine for explicit t-bit branch vectors
  * s, as pq_sim_branch_dump (normal_form.rs:472-493). */
 int pq_engine_branch_dump(pq_engine* engine, const uint8_t* values, size_t n, const uint64_t* s, uint8_t* zero, uint8_t* k8,

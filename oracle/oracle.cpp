@@ -1481,6 +1481,58 @@ int orc_engine_eval(const orc_sim* h, size_t branch, const uint8_t* vals, size_t
     } catch (std::exception& e) { g_err = e.what(); return -1; }
 }
 
+// The start of a sum_subspace call (engine.rs:557-561) and the call itself,
+// with the shift row and branch vector packed as in the descriptor layout.
+static void get_bits(BitVec& b, const uint64_t*& p) { for (size_t k = 0; k < b.w.size(); ++k) b.w[k] = p[k]; p += b.w.size(); }
+static UVec get_uvec(const Structure& st, const uint64_t* p) {
+    UVec u = st.zero_u();
+    get_bits(u.a, p); get_bits(u.b, p); get_bits(u.o, p); get_bits(u.e, p);
+    return u;
+}
+// Engine::prepare (engine.rs:569-610) of branch engine `branch`: sc.u, sc.s0
+// (ceil(t/64) words) and c_start; 1 if it returns None.
+int orc_engine_prepare(const orc_sim* h, size_t branch, const uint8_t* vals, size_t nf, uint64_t* u_out, uint64_t* s_out, double* c_out) {
+    try {
+        if (branch >= h->s.engine.branches.size()) { g_err = "no such branch engine"; return -1; }
+        const Engine& E = h->s.engine.branches[branch];
+        if (nf != E.fixed.size()) { g_err = "values do not match the fixed variables"; return -1; }
+        std::vector<bool> v(nf);
+        for (size_t i = 0; i < nf; ++i) v[i] = vals[i] != 0;
+        Phase ph; C64 cs; UVec u; BitVec s0;
+        if (!E.prepare(v, ph, cs, u, s0)) return 1;
+        put_uvec(u, E.st, u_out);
+        put_bits(s0, s_out, words_of(E.terms.size()));
+        c_out[0] = cs.re; c_out[1] = cs.im;
+        return 0;
+    } catch (std::exception& e) { g_err = e.what(); return -1; }
+}
+// Engine::sum_subspace (engine.rs:669-693) from explicit start arguments.
+int orc_engine_sum_subspace(const orc_sim* h, size_t branch, const uint64_t* u, const uint64_t* s, const double* c, double* out) {
+    try {
+        if (branch >= h->s.engine.branches.size()) { g_err = "no such branch engine"; return -1; }
+        const Engine& E = h->s.engine.branches[branch];
+        BitVec sv(E.terms.size());
+        const uint64_t* p = s;
+        get_bits(sv, p);
+        const C64 r = E.sum_subspace(get_uvec(E.st, u), sv, C64(c[0], c[1]));
+        out[0] = r.re; out[1] = r.im;
+        return 0;
+    } catch (std::exception& e) { g_err = e.what(); return -1; }
+}
+// Engine::weight (engine.rs:661-667) of a packed branch vector.
+int orc_engine_weight(const orc_sim* h, size_t branch, const uint64_t* s, double* out) {
+    try {
+        if (branch >= h->s.engine.branches.size()) { g_err = "no such branch engine"; return -1; }
+        const Engine& E = h->s.engine.branches[branch];
+        BitVec sv(E.terms.size());
+        const uint64_t* p = s;
+        get_bits(sv, p);
+        const C64 r = E.weight(sv);
+        out[0] = r.re; out[1] = r.im;
+        return 0;
+    } catch (std::exception& e) { g_err = e.what(); return -1; }
+}
+
 // Brute-force Gauss sum of one branch, independent of the normal form: the
 // sum over all 2^m free assignments z of i^{q_free(z)} (-1)^{shift(s).z} with
 // shift(s) = B_{free,F} x_F + sum_{k in s} e_{term var k} (engine.rs:8-31,

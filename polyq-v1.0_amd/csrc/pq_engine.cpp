@@ -373,6 +373,7 @@ void place_terms(HostEngine& E, const std::vector<Bits>& kernel_full, const std:
     }
     std::vector<uint32_t> pos(t);
     for (uint32_t i = 0; i < t; ++i) pos[perm[i]] = i;
+    E.term_pos = pos;
     std::vector<uint64_t> mc((size_t)t * W);
     std::vector<Phase> th(t);
     for (uint32_t i = 0; i < t; ++i) {
@@ -643,6 +644,70 @@ static bool prepare_ones(const HostEngine& E, const uint64_t* __restrict__ on, P
     }
     P.nonzero = true;
     return true;
+}
+
+bool prepare_start(const HostEngine& E, const uint64_t* u, const uint64_t* s_ref, Prepared& P) {
+    P.nonzero = false;
+    if (E.zero) return false;
+    const uint32_t W = E.L.words();
+    P.phase = 0;
+    P.u0.assign(u, u + W);
+    P.u_start.assign(u, u + W);
+    uint64_t s0 = 0;
+    P.s0_rest.clear();
+    for (uint32_t k = 0; k < E.t; ++k) {
+        if (!((s_ref[k >> 6] >> (k & 63)) & 1)) continue;
+        const uint32_t i = E.term_pos[k];
+        if (i < E.tdev) s0 |= 1ull << i;
+        else P.s0_rest.push_back(i);
+    }
+    // The walk's tables take the start on the coset's point whose non-pivot
+    // columns are all 0, as prepare's particular solution is: kernel vector
+    // j is the only one with a 1 in its column free_col[j], so XOR-ing in
+    // the kernel vectors whose column is set moves the start there without
+    // leaving the coset (the sum over the coset is the same).
+    uint64_t* us = P.u_start.data();
+    for (uint32_t j = 0; j < E.d; ++j) {
+        if (!((s0 >> E.free_col[j]) & 1)) continue;
+        s0 ^= E.kernel[j];
+        const uint64_t* ku = (E.kernel_u.data() + (size_t)j * W);
+        for (uint32_t w = 0; w < W; ++w) us[w] ^= ku[w];
+    }
+    u = us;
+    P.s0 = s0;
+    // ell_j = lam_j + 4 (parity(u . dual_j) ^ parity(s0 . kernel_j)), the
+    // parities prepare accumulates column by column
+    uint64_t dq = 0;
+    for (uint32_t j = 0; j < E.d; ++j)
+        if (row_parity(u, (E.dual.data() + (size_t)j * W), W) ^ ((__builtin_popcountll(s0 & E.kernel[j]) & 1) != 0)) dq |= 1ull << j;
+    P.dq = dq;
+    P.K0 = total_phase(E, u, s0);
+    P.scale = E.Ephase * E.Bmag;
+    if (E.tdev < E.t) {
+        // the constant terms, as in prepare
+        std::vector<uint8_t> in_s(E.t - E.tdev, 0);
+        for (uint32_t k : P.s0_rest) in_s[k - E.tdev] = 1;
+        uint32_t kr = 0;
+        double mr = 1.0;
+        for (uint32_t k = E.tdev; k < E.t; ++k) {
+            const bool sk = in_s[k - E.tdev];
+            kr += sk ? 6u : (E.neg_rest[k - E.tdev] ? 4u : 0u);
+            mr *= sk ? E.r1[k] : E.r0[k];
+        }
+        P.K0 = (P.K0 + kr) & 7u;
+        P.scale *= mr;
+    }
+    P.nonzero = true;
+    return true;
+}
+
+std::complex<double> weight_ref(const HostEngine& E, const uint64_t* s_ref) {
+    std::complex<double> c(1, 0);
+    for (uint32_t k = 0; k < E.t; ++k) {
+        const uint32_t i = E.term_pos[k];
+        c *= ((s_ref[k >> 6] >> (k & 63)) & 1) ? E.a1[i] : E.a0[i];
+    }
+    return c;
 }
 
 void prepared_ell(const HostEngine& E, Prepared& P) {

@@ -239,6 +239,48 @@ int pq_engine_eval_batch(size_t n, pq_engine* const* engines, const uint8_t* con
     })
 }
 
+int pq_engine_sum_subspace(pq_engine* e, const uint64_t* u_start, const uint64_t* s_start, const double* c_start, int form, double* out) {
+    PQ_TRY({
+        if (!e || !u_start || !s_start || !c_start || !out) throw std::invalid_argument("null argument");
+        const RankEng& R = *e->r;
+        const HostEngine& E = *R.host[0];
+        const ULayout& L = E.L;
+        if (!tail_clear(u_start + L.off_a(), L.g) || !tail_clear(u_start + L.off_b(), L.g) || !tail_clear(u_start + L.off_o(), L.nodd) ||
+            !tail_clear(u_start + L.off_e(), L.neven))
+            throw std::invalid_argument("u_start has bits beyond its part lengths");
+        if (!tail_clear(s_start, E.t)) throw std::invalid_argument("s_start has bits beyond the number of terms");
+        // c_start is prepare's weight at s_start (engine.rs:600-606); the
+        // walk derives every branch's weight from the term angles, as the
+        // reference does for every chunk but the first (engine.rs:677-684)
+        const std::complex<double> w = weight_ref(E, s_start), c(c_start[0], c_start[1]);
+        if (std::abs(c - w) > 1e-9 * std::max(std::abs(w), 1e-300))
+            throw std::invalid_argument("c_start is not the weight of s_start");
+        out[0] = out[1] = 0.0;
+        Prepared P;
+        if (!prepare_start(E, u_start, s_start, P)) return 0;
+        R.ensure_device();
+        std::complex<double> v;
+        if (form == PQ_FORM_PLAIN_FP64) {
+            v = dev_plain_sum(*R.dev[0], P, nullptr);
+        } else {
+            const int walk = walk_of_form(form);
+            Z8Job J;
+            J.eng = R.dev[0].get();
+            if (walk == kWalkSeg) J.sq = seg_query(E, *R.seg[0], P);
+            else J.q = z8_query(E, *R.z8[0], P);
+            J.scale = P.scale;
+            J.mid_bits = 0;
+            std::vector<Z8Job> jobs{J};
+            std::vector<size_t> owner{0};
+            std::vector<std::complex<double>> res;
+            run_jobs(jobs, owner, 1, walk, res);
+            v = res[0];
+        }
+        out[0] = v.real(); out[1] = v.imag();
+        return 0;
+    })
+}
+
 int pq_engine_branch_dump(pq_engine* e, const uint8_t* values, size_t n, const uint64_t* s, uint8_t* zero, uint8_t* k8, uint8_t* ktot,
                           uint32_t* r) {
     PQ_TRY({

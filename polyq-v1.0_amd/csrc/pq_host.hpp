@@ -163,6 +163,7 @@ struct HostEngine {
     std::vector<uint32_t> pivot_col;         // rk
     std::vector<Bits> pivot_row;             // rk rows over neven
     std::vector<Bits> consistency;           // rows over neven
+    std::vector<uint32_t> term_pos;          // t: position of the reference's term k in the order above
     std::vector<uint64_t> kernel;            // d kernel vectors over the device terms
     std::vector<uint32_t> free_col;          // d: the non-pivot column of kernel j
     std::vector<uint64_t> kernel_u;          // d packed shift rows
@@ -226,6 +227,17 @@ struct Prepared {
     std::complex<double> scale;      // e^{i phase} 2^{log2mag - h/2} E B
 };
 bool prepare(const HostEngine& E, const uint8_t* values, Prepared& P);
+// The prologue of Engine::sum_subspace's own arguments (engine.rs:669): the
+// start shift u (a packed shift row) and branch vector s_ref (t bits in the
+// reference's term order) of the affine branch subspace, which prepare leaves
+// in sc.u and sc.s0 (engine.rs:596-606), instead of fixed values.  P.phase is
+// 0 and P.scale is E B (the sum is that of weight(s) ROOT8[phase8(u(s))]
+// without eval_with's boundary phase and magnitude, engine.rs:557-565).
+// False (the sum is 0) for an engine whose path sum vanishes identically.
+bool prepare_start(const HostEngine& E, const uint64_t* u, const uint64_t* s_ref, Prepared& P);
+// weight(s) (engine.rs:661-667) of a branch vector in the reference's term
+// order, multiplied in that order.
+std::complex<double> weight_ref(const HostEngine& E, const uint64_t* s_ref);
 // Reference-style FP64 weight at s0 (engine.rs:229-238: the product of the
 // term multipliers a1[k] for k in s0, a0[k] otherwise, in term order) over the
 // device terms, and the constant product over the other terms; only the plain

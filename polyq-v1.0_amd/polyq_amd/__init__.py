@@ -86,6 +86,7 @@ lib.pq_engine_destroy.restype = None
 lib.pq_engine_stats.argtypes = [ctypes.c_void_p, _u64p]
 lib.pq_engine_eval.argtypes = [ctypes.c_void_p, _u8p, ctypes.c_size_t, ctypes.c_int, _f64p]
 lib.pq_engine_eval_batch.argtypes = [ctypes.c_size_t, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p]
+lib.pq_engine_sum_subspace.argtypes = [ctypes.c_void_p, _u64p, _u64p, _f64p, ctypes.c_int, _f64p]
 lib.pq_engine_branch_dump.argtypes = [ctypes.c_void_p, _u8p, ctypes.c_size_t, _u64p, _u8p, _u8p, _u8p, _u32p]
 lib.pq_best_pairs.argtypes = [_u64p, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_int]
 lib.pq_valu_probe.argtypes = [_f64p, ctypes.c_size_t, ctypes.c_uint32, ctypes.c_uint32]
@@ -536,6 +537,9 @@ class Engine:
         _check(lib.pq_engine_create(ctypes.byref(d), ctypes.byref(h)))
         self._h = h
         self.nf = int(fields["nf"])
+        w = lambda n: (int(n) + 63) // 64
+        self._uwords = 2 * w(fields["planes"]) + w(fields["lines"]) + w(fields["radical"])
+        self._swords = w(fields["terms"])
         self._keep = None
 
     def __del__(self):
@@ -554,6 +558,21 @@ class Engine:
         v = np.ascontiguousarray(values, dtype=np.uint8)
         out = (ctypes.c_double * 2)()
         _check(lib.pq_engine_eval(self._h, _ptr(v, _u8p), len(v), FORMS[form], out))
+        return complex(out[0], out[1])
+
+    def sum_subspace(self, u_start, s_start, c_start, form="z8"):
+        """Engine::sum_subspace (src/engine.rs:669-693): the sum of
+        weight(s) ROOT8[phase8(u(s))] over s_start + span(kernel), from the
+        start shift u_start (one packed shift row) and branch vector s_start
+        (packed terms bits, the reference's term order) with c_start its
+        weight."""
+        u = np.ascontiguousarray(u_start, dtype=np.uint64)
+        st = np.ascontiguousarray(s_start, dtype=np.uint64)
+        if u.size < self._uwords or st.size < self._swords:
+            raise ValueError("u_start needs %d words and s_start %d" % (self._uwords, self._swords))
+        c = np.array([complex(c_start).real, complex(c_start).imag], np.float64)
+        out = (ctypes.c_double * 2)()
+        _check(lib.pq_engine_sum_subspace(self._h, _ptr(u, _u64p), _ptr(st, _u64p), _ptr(c, _f64p), FORMS[form], out))
         return complex(out[0], out[1])
 
     def branch_dump(self, values, s_list):

@@ -63,6 +63,9 @@ def lib():
                                         _u64p, _u64p, _u64p, _u64p]
         L.orc_engine_values.argtypes = [ctypes.c_void_p, _u8p, _u8p, _u8p]
         L.orc_engine_eval.argtypes = [ctypes.c_void_p, ctypes.c_size_t, _u8p, ctypes.c_size_t, _f64p]
+        L.orc_engine_prepare.argtypes = [ctypes.c_void_p, ctypes.c_size_t, _u8p, ctypes.c_size_t, _u64p, _u64p, _f64p]
+        L.orc_engine_sum_subspace.argtypes = [ctypes.c_void_p, ctypes.c_size_t, _u64p, _u64p, _f64p, _f64p]
+        L.orc_engine_weight.argtypes = [ctypes.c_void_p, ctypes.c_size_t, _u64p, _f64p]
         _lib = L
     return _lib
 
@@ -289,4 +292,38 @@ def engine_eval(osim, branch, vals):
     out = np.zeros(2, np.float64)
     v = np.ascontiguousarray(vals, np.uint8)
     _check(lib().orc_engine_eval(osim.h, branch, _p(v, _u8p), len(v), _p(out, _f64p)))
+    return complex(out[0], out[1])
+
+
+def _start_words(osim, branch):
+    sz = np.zeros(13, np.uint64)
+    _check(lib().orc_engine_sizes(osim.h, branch, _p(sz, _u64p)))
+    g, nodd, ne, t = (int(v) for v in sz[1:5])
+    return 2 * _w(g) + _w(nodd) + _w(ne), _w(t)
+
+
+def engine_prepare(osim, branch, vals):
+    """Engine::prepare (engine.rs:569-610): (u_start, s_start, c_start), the
+    shift row and branch vector packed as in the descriptor layout, or None."""
+    uw, sw = _start_words(osim, branch)
+    u = np.zeros(max(1, uw), np.uint64); s = np.zeros(max(1, sw), np.uint64); c = np.zeros(2, np.float64)
+    v = np.ascontiguousarray(vals, np.uint8)
+    rc = _check(lib().orc_engine_prepare(osim.h, branch, _p(v, _u8p), len(v), _p(u, _u64p), _p(s, _u64p), _p(c, _f64p)))
+    return None if rc == 1 else (u, s, complex(c[0], c[1]))
+
+
+def engine_sum_subspace(osim, branch, u, s, c):
+    """Engine::sum_subspace (engine.rs:669-693) from explicit start arguments."""
+    out = np.zeros(2, np.float64)
+    cc = np.array([complex(c).real, complex(c).imag], np.float64)
+    u = np.ascontiguousarray(u, np.uint64); s = np.ascontiguousarray(s, np.uint64)
+    _check(lib().orc_engine_sum_subspace(osim.h, branch, _p(u, _u64p), _p(s, _u64p), _p(cc, _f64p), _p(out, _f64p)))
+    return complex(out[0], out[1])
+
+
+def engine_weight(osim, branch, s):
+    """Engine::weight (engine.rs:661-667) of a packed branch vector."""
+    out = np.zeros(2, np.float64)
+    s = np.ascontiguousarray(s, np.uint64)
+    _check(lib().orc_engine_weight(osim.h, branch, _p(s, _u64p), _p(out, _f64p)))
     return complex(out[0], out[1])

@@ -94,3 +94,41 @@ def test_small_engines_against_brute_force_all_boundaries():
                 assert abs(got - want) < 1e-10, (trial, x, y, got, want)
         checked += 1
     assert checked >= 20
+
+
+def _kernel_rows(f):
+    t, d = f["terms"], f["branch_dim"]
+    sw = (t + 63) // 64
+    uw = 2 * ((f["planes"] + 63) // 64) + (f["lines"] + 63) // 64 + (f["radical"] + 63) // 64
+    return f["kernel"][:d * sw].reshape(d, sw), f["kernel_u"][:d * uw].reshape(d, uw)
+
+
+@pytest.mark.parametrize("config", [1, 2])
+def test_sum_subspace_from_prepare_start(config):
+    # pq_engine_sum_subspace on the start prepare leaves (sc.u, sc.s0,
+    # c_start, engine.rs:596-606) against the oracle's Engine::sum_subspace
+    # in every form, and from another point of the same coset (the sum does
+    # not depend on where the walk starts)
+    c, xs, ys = W.config1(1) if config == 1 else W.config2(1)
+    o = _osim(c)
+    f = O.engine_fields(o)
+    e = pq.Engine(f)
+    K, KU = _kernel_rows(f)
+    checked = 0
+    for x, y in list(zip(xs, ys))[:6]:
+        v = O.engine_values(o, x, y)
+        st = O.engine_prepare(o, 0, v)
+        if st is None:
+            continue
+        u, s, cs = st
+        want = O.engine_sum_subspace(o, 0, u, s, cs)
+        tol = 1e-11 * max(abs(want), 1e-3 * abs(O.engine_weight(o, 0, s)) * 2.0 ** (f["branch_dim"] / 2))
+        for form in ("z8", "branch_walk", "plain_fp64"):
+            got = e.sum_subspace(u, s, cs, form=form)
+            assert abs(got - want) <= tol, (config, form, got, want)
+        j = len(K) // 2
+        s2, u2 = s ^ K[j], u ^ KU[j]
+        moved = e.sum_subspace(u2, s2, O.engine_weight(o, 0, s2))
+        assert abs(moved - want) <= tol
+        checked += 1
+    assert checked >= 4

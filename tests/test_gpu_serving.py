@@ -133,6 +133,25 @@ def test_large_batch_reduction_equals_fused_small_batches_bitwise():
     np.testing.assert_array_equal(big, small)
 
 
+def test_branch_walk_host_finish_equals_device_reduction_bitwise():
+    # The per-branch walk of the n=40 workload circuit at a lane walk of 3
+    # mid slots: 512 items per query.  16 queries (8192 items) reduce on the
+    # device; one query at a time (512 items) is finished on the host.  Same
+    # reduction order: same bits.
+    c, xs, ys = W.config2(5)
+    s = pq.Simulator(c, strategy="cached", form="branch_walk")
+    prev = pq.get_walk_tuning("branch_walk")
+    try:
+        pq.set_walk_tuning(27, 6, "branch_walk")
+        big = s.amplitudes(xs, ys)
+        assert pq.last_timing()["items"] == 16 * 512
+        small = [s.amplitudes(xs[i:i + 1], ys[i:i + 1]) for i in range(16)]
+        assert pq.last_timing()["items"] == 512
+        np.testing.assert_array_equal(big, np.concatenate(small))
+    finally:
+        pq.set_walk_tuning(*prev, "branch_walk")
+
+
 def test_host_finish_equals_device_reduction_many_items_per_query():
     # 1024 items per query (2^18 lanes of one point each): more items than
     # reduction threads, finished on the host in a batch of one query and on
