@@ -623,10 +623,11 @@ class Graph:
         self.n = n
 
     def run(self):
-        out = np.zeros(2 * self.n, dtype=np.float64)
+        """Replay: (amplitudes as complex128, device milliseconds of the replay)."""
+        out = np.empty(self.n, dtype=np.complex128)       # re, im interleaved: the 2n doubles the call writes
         ms = ctypes.c_double()
-        _check(lib.pq_graph_run(self._g, _ptr(out, _f64p), ctypes.byref(ms)))
-        return out[0::2] + 1j * out[1::2], ms.value
+        _check(lib.pq_graph_run(self._g, ctypes.cast(out.ctypes.data, _f64p), ctypes.byref(ms)))
+        return out, ms.value
 
     def __del__(self):
         g = getattr(self, "_g", None)
