@@ -162,10 +162,13 @@ void build_jobs(size_t n, int walk, const std::function<const RankEng*(size_t)>&
         return;
     }
     ChunkPool& pool = ChunkPool::get();
-    std::vector<std::vector<Z8Job>> cj(pool.threads());
-    std::vector<std::vector<size_t>> co(pool.threads());
-    std::vector<std::string> err(pool.threads());
-    const unsigned nch = pool.run(n, kChunk, n < 256 ? ChunkPool::kHot + 1 : pool.threads(), [&](size_t lo, size_t hi, unsigned c) {
+    // large batches in four chunks per thread: workers that wake late (a
+    // blocked worker takes tens of microseconds) still find work
+    const unsigned maxch = n < 256 ? ChunkPool::kHot + 1 : 4 * pool.threads();
+    std::vector<std::vector<Z8Job>> cj(maxch);
+    std::vector<std::vector<size_t>> co(maxch);
+    std::vector<std::string> err(maxch);
+    const unsigned nch = pool.run(n, kChunk, maxch, [&](size_t lo, size_t hi, unsigned c) {
         try {
             for (size_t i = lo; i < hi; ++i) add_query_jobs_xy(*rank_of(i), xs[i], ys[i], walk, cj[c], co[c], i);
         } catch (std::exception& e) {

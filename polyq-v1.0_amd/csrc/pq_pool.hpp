@@ -33,9 +33,10 @@ public:
     // Workers that poll for new regions (the rest block between regions).
     static constexpr unsigned kHot = 3;
 
-    // At most max_chunks chunks of at least min_chunk items each.
+    // At most max_chunks chunks of at least min_chunk items each.  More
+    // chunks than threads let workers that wake late still take a share.
     unsigned run(size_t n, size_t min_chunk, unsigned max_chunks, const std::function<void(size_t, size_t, unsigned)>& fn) {
-        const size_t cap = std::max<size_t>(1, std::min<size_t>(threads(), max_chunks));
+        const size_t cap = std::max<size_t>(1, max_chunks);
         const unsigned want = (unsigned)std::max<size_t>(1, std::min<size_t>(cap, n / std::max<size_t>(min_chunk, 1)));
         if (want <= 1) { fn(0, n, 0); return 1; }
         std::unique_lock<std::mutex> region(region_mu_, std::try_to_lock);
@@ -51,7 +52,7 @@ public:
         // counted here checks the generation again before it blocks), so
         // only the chunks they cannot cover wake a blocked worker: in steady
         // back-to-back use no wake-up system call is made at all.
-        const unsigned helpers = want - 1, polling = polling_.load(std::memory_order_acquire);
+        const unsigned helpers = std::min<unsigned>(want - 1, nworkers_), polling = polling_.load(std::memory_order_acquire);
         for (unsigned i = polling; i < helpers; ++i) cv_.notify_one();
         work(*task);
         while (task->done.load(std::memory_order_acquire) != task->chunks) std::this_thread::yield();
