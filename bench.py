@@ -579,8 +579,9 @@ def serve_sweep(args, pq, W, np):
         sched, circuits, fresh = _serve_schedule(W, rate, secs, rseed=rate, cseed=rate + 1)
         out, _ = _serve_run(pq, np, sched, fresh)
         row = {"rate_req_per_s": rate, "seconds": secs}
-        row.update({k: out[k] for k in ("requests", "failed", "p50_ms", "p99_ms", "throughput_req_per_s", "max_batch",
+        row.update({k: out[k] for k in ("requests", "failed", "p50_ms", "p99_ms", "max_ms", "throughput_req_per_s", "max_batch",
                                         "overlapped_batches", "batches", "builds", "host_drops")})
+        row["slowest_1pct_phases_ms"] = out["phases_ms"]["slowest_1pct"]
         rows.append(row)
         print("bench: serve_sweep rate %d p50 %.3f p99 %.3f thr %.0f" % (rate, out["p50_ms"], out["p99_ms"], out["throughput_req_per_s"]),
               file=sys.stderr, flush=True)
