@@ -377,36 +377,54 @@ struct Server {
         cv_mem.notify_all();
     }
 
+    // Batches in flight at once, each on a slot of its own (stream, events,
+    // buffers): requests that arrive while batches run are launched beside
+    // them as long as a slot is free.
+    static constexpr int kSlots = 4;
+
     void dispatch() {
-        std::shared_ptr<Z8Slot> slots[2];
-        std::vector<std::shared_ptr<Req>> held[2];
-        int order[2] = {-1, -1};     // launch order of the in-flight slots (oldest first)
+        std::shared_ptr<Z8Slot> slots[kSlots];
+        std::vector<std::shared_ptr<Req>> held[kSlots];
+        std::deque<int> order;       // in-flight slots, oldest first
         try {
-            slots[0] = dev_slot_create(); slots[1] = dev_slot_create();
+            for (auto& sl : slots) sl = dev_slot_create();
         } catch (std::exception&) {
             // no device: requests fail when they reach the dispatcher
+            for (auto& sl : slots) sl.reset();
         }
+        auto retire = [&](int s) {
+            std::vector<std::complex<double>> res;
+            Z8Timing T;
+            dev_z8_wait(*slots[s], res, &T);
+            if (std::isfinite(T.kernel_ms)) {
+                std::lock_guard<std::mutex> lk(mu);
+                timed_kernel_ms += T.kernel_ms;
+                timed_batches++;
+            }
+            complete(held[s], res);
+            held[s].clear();
+        };
         for (;;) {
             std::vector<std::shared_ptr<Req>> batch;
-            int inflight = (order[0] >= 0) + (order[1] >= 0);
+            const int inflight = (int)order.size();
             {
                 std::unique_lock<std::mutex> lk(mu);
                 if (inflight == 0) cv_ready.wait(lk, [&] { return stop || !ready_q.empty(); });
                 if (stop && ready_q.empty() && inflight == 0) return;
-                if (inflight < 2) {
+                if (inflight < kSlots) {
                     while (!ready_q.empty()) { batch.push_back(ready_q.front()); ready_q.pop_front(); }
                     nready -= batch.size();
                 }
             }
             if (!batch.empty()) {
                 if (!slots[0]) {
-                    std::vector<std::complex<double>> none;
                     std::lock_guard<std::mutex> lk(mu);
-                    for (auto& r : batch) { r->status = -1; r->err = "no HIP device available"; r->t_done = now_s(); r->jobs.clear(); unpin(*r); inflight--; failed++; }
+                    for (auto& r : batch) { r->status = -1; r->err = "no HIP device available"; r->t_done = now_s(); r->jobs.clear(); unpin(*r); this->inflight--; failed++; }
                     cv_done.notify_all();
                     continue;
                 }
-                const int s = order[0] < 0 ? 0 : (order[0] == 0 ? 1 : 0);   // a free slot (at most one is in flight here)
+                int s = 0;                   // a free slot (fewer than kSlots are in flight here)
+                while (std::find(order.begin(), order.end(), s) != order.end()) ++s;
                 std::vector<Z8Job> jobs;
                 for (auto& r : batch) for (auto& J : r->jobs) jobs.push_back(J);
                 const double tl = now_s();
@@ -416,44 +434,44 @@ struct Server {
                 // queue barrier that lengthens a short batch
                 dev_z8_launch(*slots[s], jobs, walk, (launched++ & 7) == 0);
                 held[s] = std::move(batch);
-                if (order[0] < 0) order[0] = s; else { order[1] = s; overlapped++; }
+                if (!order.empty()) overlapped++;
+                order.push_back(s);
                 std::lock_guard<std::mutex> lk(mu);
                 batches++;
                 max_batch = std::max<uint64_t>(max_batch, held[s].size());
             }
-            // retire the oldest batch: block when both slots are busy, else
-            // poll it while watching for new requests to launch beside it
-            if (order[0] >= 0) {
-                const int s = order[0];
-                bool done = order[1] >= 0 || dev_z8_ready(*slots[s]);
-                if (!done) {
-                    // poll the batch without sleeping for up to 2 ms (a
-                    // batch of a few requests takes tens of microseconds),
-                    // going back to launch as soon as a request is ready;
-                    // after that, wait in 200 us steps
-                    const double t0 = now_s();
-                    while (!(done = dev_z8_ready(*slots[s]))) {
-                        if (nready.load(std::memory_order_acquire) > 0 || now_s() - t0 > 2e-3) break;
-                        std::this_thread::yield();
-                    }
-                }
-                if (!done) {
-                    std::unique_lock<std::mutex> lk(mu);
-                    if (ready_q.empty() && !stop) cv_ready.wait_for(lk, std::chrono::microseconds(200));
-                    continue;
-                }
-                std::vector<std::complex<double>> res;
-                Z8Timing T;
-                dev_z8_wait(*slots[s], res, &T);
-                if (std::isfinite(T.kernel_ms)) {
-                    std::lock_guard<std::mutex> lk(mu);
-                    timed_kernel_ms += T.kernel_ms;
-                    timed_batches++;
-                }
-                complete(held[s], res);
-                held[s].clear();
-                order[0] = order[1]; order[1] = -1;
+            if (order.empty()) continue;
+            // retire every finished batch, oldest first
+            bool retired = false;
+            for (size_t i = 0; i < order.size();) {
+                if (dev_z8_ready(*slots[order[i]])) { retire(order[i]); order.erase(order.begin() + (long)i); retired = true; }
+                else ++i;
             }
+            if (retired) continue;
+            if ((int)order.size() == kSlots) {
+                // every slot busy: wait for the oldest
+                retire(order.front());
+                order.pop_front();
+                continue;
+            }
+            // poll the oldest batch without sleeping for up to 2 ms (a batch
+            // of a few requests takes tens of microseconds), going back to
+            // launch as soon as a request is ready; after that, wait in
+            // 200 us steps
+            const int s = order.front();
+            bool done = false;
+            const double t0 = now_s();
+            while (!(done = dev_z8_ready(*slots[s]))) {
+                if (nready.load(std::memory_order_acquire) > 0 || now_s() - t0 > 2e-3) break;
+                std::this_thread::yield();
+            }
+            if (done) {
+                retire(s);
+                order.pop_front();
+                continue;
+            }
+            std::unique_lock<std::mutex> lk(mu);
+            if (ready_q.empty() && !stop) cv_ready.wait_for(lk, std::chrono::microseconds(200));
         }
     }
 
