@@ -243,8 +243,9 @@ int pq_last_timing(double out5[5]);
 int pq_set_timing(int on);
 int pq_get_timing(void);
 
-/* Replayable batch captured in a hipGraph (descriptor upload, counter clear,
- * persistent walk, FP64 reduction, result download). */
+/* Replayable batch captured in a hipGraph: the descriptors are uploaded once
+ * when the graph is made; a replay clears the queue counters, runs the walk
+ * and, for large batches, the FP64 reduction and result download. */
 typedef struct pq_graph pq_graph;
 int pq_graph_create(size_t n, pq_sim* const* sims, const uint64_t* inputs, const uint64_t* outputs, int form, pq_graph** out);
 int pq_graph_run(pq_graph* g, double* out, double* total_ms);
@@ -256,8 +257,8 @@ void pq_graph_destroy(pq_graph* g);
  * within hbm_budget_bytes (LRU eviction of idle entries, back-pressure while
  * requests in flight hold the rest; a request that cannot fit fails with an
  * "HBM budget" error instead of waiting).
- * Requests that arrive while a batch runs are launched as the next batch on a
- * second stream.  kind: 0 amplitude <y|C|x>, 1 probability |<y|C|x>|^2,
+ * Requests that arrive while batches run are launched as the next batch on
+ * another stream (four batch slots).  kind: 0 amplitude <y|C|x>, 1 probability |<y|C|x>|^2,
  * 2 marginal probability of the k listed (qubit, outcome) pairs for input x
  * (src/sim.rs:171-231). */
 typedef struct pq_server pq_server;
