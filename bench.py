@@ -264,6 +264,7 @@ def headline(args, pq, W, np, probe):
     else:
         stune = pq.autotune([sim] * nq, xs, ys, repeats=7, rounds=5, cache=False)[:2]
     swaves = pq.get_walk_shape()
+    _settle(lambda: sim.amplitudes(xs, ys, form="z8"))
     for _ in range(args.warmup):
         sim.amplitudes(xs, ys, form="z8")
     ts = time.perf_counter()
