@@ -168,22 +168,42 @@ SegImage build_seg_image(const HostEngine& E) {
 
     // Inner directions within a table of 2^(I + npin) entries: pivot-free ones
     // first, then those adding the fewest new pivots.
-    const uint32_t B = seg_bits_for(d);
-    std::vector<uint8_t> used(d, 0);
+    std::vector<uint8_t> used;
     uint64_t pin_set = 0;
-    for (uint32_t j = 0; j < d && S.inner_j.size() < B; ++j)
-        if (pivots_of(E.kernel[j]) == 0) { S.inner_j.push_back(j); used[j] = 1; }
-    for (;;) {
-        int best = -1; uint32_t bestc = 99;
-        for (uint32_t j = 0; j < d; ++j) {
-            if (used[j]) continue;
-            const uint64_t pj = pivots_of(E.kernel[j]);
-            const uint32_t tot = (uint32_t)S.inner_j.size() + 1 + (uint32_t)__builtin_popcountll(pin_set | pj);
-            const uint32_t c = (uint32_t)__builtin_popcountll(pj & ~pin_set);
-            if (tot <= B && c < bestc) { bestc = c; best = (int)j; }
+    auto choose_inner = [&](uint32_t B) {
+        S.inner_j.clear();
+        used.assign(d, 0);
+        pin_set = 0;
+        for (uint32_t j = 0; j < d && S.inner_j.size() < B; ++j)
+            if (pivots_of(E.kernel[j]) == 0) { S.inner_j.push_back(j); used[j] = 1; }
+        for (;;) {
+            int best = -1; uint32_t bestc = 99;
+            for (uint32_t j = 0; j < d; ++j) {
+                if (used[j]) continue;
+                const uint64_t pj = pivots_of(E.kernel[j]);
+                const uint32_t tot = (uint32_t)S.inner_j.size() + 1 + (uint32_t)__builtin_popcountll(pin_set | pj);
+                const uint32_t c = (uint32_t)__builtin_popcountll(pj & ~pin_set);
+                if (tot <= B && c < bestc) { bestc = c; best = (int)j; }
+            }
+            if (best < 0) break;
+            S.inner_j.push_back((uint32_t)best); used[best] = 1; pin_set |= pivots_of(E.kernel[best]);
         }
-        if (best < 0) break;
-        S.inner_j.push_back((uint32_t)best); used[best] = 1; pin_set |= pivots_of(E.kernel[best]);
+    };
+    choose_inner(seg_bits_for(d));
+    // When the inner directions touch many pivots, the table's bits go to
+    // pivot patterns and few are left for inner directions: the walk of one
+    // query can then pass 2^24 outer points (a 4-qubit marginal's sandwich
+    // engine of d = 39 kept 4 inner directions and walked 2^35 points, 25 ms).
+    // Such an engine gets up to 2^kSegBigBits table entries: a host build of
+    // a few milliseconds against a walk several times longer for every query
+    // (that engine: 5 ms to build, 6 ms to walk).
+    constexpr uint32_t kSegBigBits = 20, kSegBigOuter = 24;
+    if (d - (uint32_t)S.inner_j.size() > kSegBigOuter && seg_bits_for(d) < kSegBigBits) {
+        const std::vector<uint32_t> keep = S.inner_j;
+        const std::vector<uint8_t> keep_used = used;
+        const uint64_t keep_pin = pin_set;
+        choose_inner(kSegBigBits);
+        if (S.inner_j.size() <= keep.size()) { S.inner_j = keep; used = keep_used; pin_set = keep_pin; }
     }
     S.I = (uint32_t)S.inner_j.size();
     std::vector<int32_t> cpos(E.rkdev, -1);
