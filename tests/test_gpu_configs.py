@@ -8,6 +8,8 @@ sample (src/engine.rs:745-774 and src/sim.rs:329-342 patterns at scale).
       64 spread over d against the oracle
   Poisson serving (config-4 circuits, 70/20/10 mix, 64 GB): every result
       bitwise equal to its alone result, 64 against the oracle
+  the two heaviest marginals of the 3,200 req/s serving schedule (d = 37,
+      39; large segment tables) against the dense statevector
 """
 import os
 import random
@@ -191,3 +193,53 @@ def test_config5_serving_results_equal_alone_and_oracle():
         if checked == 64:
             break
     assert checked == 64
+
+
+def _serve_circuits(rate, secs, rseed, cseed):
+    """The request stream of the serving benchmark's Poisson schedule (the
+    same draws in the same order): (kind, circuit, x, measured) per request."""
+    rnd = random.Random(rseed)
+    rng = W.Rng(cseed)
+    t, out, circuits = 0.0, [], []
+    while True:
+        t += rnd.expovariate(rate)
+        if t >= secs:
+            return out
+        if not circuits or rnd.random() < 0.5:
+            c = W.config4_circuit(rng)
+            circuits.append(c)
+        else:
+            c = circuits[rnd.randrange(len(circuits))]
+        u = rnd.random()
+        n = c.num_qubits
+        x, y = rnd.getrandbits(n), rnd.getrandbits(n)
+        if u < 0.9:
+            out.append(("amplitude" if u < 0.7 else "probability", c, x, ()))
+        else:
+            out.append(("marginal", c, x, tuple((q, rnd.random() < 0.5) for q in rnd.sample(range(n), 4))))
+
+
+def test_pivot_heavy_marginal_against_dense():
+    # The heaviest 4-qubit marginals of the 3,200 req/s serving schedule:
+    # sandwich engines of d = 37 and 39 whose inner directions touch many
+    # pivots (they get segment tables of up to 2^20 entries), on 23 and 21
+    # qubits, against the marginal of the dense statevector
+    reqs = _serve_circuits(3200.0, 5.0, 3200, 3201)
+    heavy = []
+    for kind, c, x, meas in reqs:
+        if kind == "marginal" and c.num_qubits <= 23:
+            s = pq.Simulator(c, strategy="cached")
+            d = s.marginal_stats([q for q, _ in meas])[5]
+            if d >= 37:
+                heavy.append((s, c, x, meas, d))
+    assert sorted(h[4] for h in heavy) == [37, 39]
+    for s, c, x, meas, d in heavy:
+        got = s.marginal(x, list(meas))
+        sv = O.dense(O.OCircuit.from_circuit(c), x)
+        idx = np.arange(1 << c.num_qubits)
+        keep = np.ones(idx.shape, bool)
+        for q, b in meas:
+            keep &= ((idx >> q) & 1) == (1 if b else 0)
+        want = float(np.sum(np.abs(sv[keep]) ** 2))
+        print("d=%d n=%d marginal %.12g dense %.12g" % (d, c.num_qubits, got, want), flush=True)
+        assert abs(got - want) <= 1e-10, (d, got, want)
