@@ -946,9 +946,16 @@ __global__ __launch_bounds__(256) void k_z8_walk(const Z8Dev* __restrict__ engs,
     __shared__ double s_red[4][4];
     const uint32_t tid = threadIdx.x;
     if (tid == 0) { s_key = -1; s_fq = -1; }
+    // the inner table stays in VGPRs from item to item of one engine
+    InnerRegs<NIN> T;
+    uint32_t t_eng = 0xFFFFFFFFu;
+    // a workgroup's first item is its own index; later ones come from the
+    // queue (counter starts at 0)
+    bool first = true;
     for (;;) {
         __syncthreads();
-        if (tid == 0) s_item = item_begin + atomicAdd(counter, 1ull);
+        if (tid == 0) s_item = first ? item_begin + blockIdx.x : item_begin + gridDim.x + atomicAdd(counter, 1ull);
+        first = false;
         __syncthreads();
         const uint64_t item = s_item;
         if (item >= item_end) break;
@@ -984,11 +991,10 @@ __global__ __launch_bounds__(256) void k_z8_walk(const Z8Dev* __restrict__ engs,
         double acc[4] = {0.0, 0.0, 0.0, 0.0};
         const uint64_t lane = (item - Q.item0) * 256 + tid;
         const bool valid = (lane >> Q.H) == 0;
-        InnerRegs<NIN> T;
         // the table in VGPRs: always without inner pivots, and with them when
         // the four low inner slots are pivot-free (uniform per engine)
         const bool lanewalk = !HASG || E.lowfree;
-        if (lanewalk) load_inner_regs<NIN>(E, T);
+        if (lanewalk && Q.eng != t_eng) { load_inner_regs<NIN>(E, T); t_eng = Q.eng; }
         // without pivots every lane walks (the table broadcast needs the whole
         // wave); lanes past the query's 2^H repeat a valid lane's coset and
         // are zeroed below
